@@ -1,0 +1,922 @@
+// Host side of libeppx: the C ABI declared in include/eppx.h.
+//
+// A context owns one HIP stream on one device.  A particle system owns its
+// structure-of-arrays state twice (the cell-list build moves every particle
+// from one copy to the other in cell order), the positions snapshot of the
+// last search and the cell arrays of that search.  Everything is enqueued on
+// the context's stream; waiting calls synchronise that stream only.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "eppx_internal.h"
+
+using namespace eppx;
+
+struct PendingTiming {
+    const char *name;
+    hipEvent_t start, stop;
+};
+
+struct eppx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string error;
+    int32_t math_mode = EPPX_MATH_EXACT;
+    bool timing = false;
+    std::vector<PendingTiming> pending;
+    std::vector<hipEvent_t> event_pool;
+    std::map<std::string, std::pair<double, int64_t>> timed;  // name -> (sum us, launches)
+    uint32_t *d_flags = nullptr;
+};
+
+struct eppx_ns {
+    eppx_ctx *ctx = nullptr;
+    eppx_domain2d domain{};
+    Grid grid{};
+    uint32_t adj[kMaxTag + 1] = {};
+    int32_t max_tag = 0;
+    int32_t M = 0;
+};
+
+struct eppx_ps {
+    eppx_ctx *ctx = nullptr;
+    int32_t cap = 0, n = 0;
+    eppx_layout layout{};
+    Fields buf[2]{};
+    int cur = 0;
+    float2 *pos_nb = nullptr;
+    // cell arrays of the last search
+    int32_t n_cells = -1;
+    int32_t *counts = nullptr, *start = nullptr, *tile_sums = nullptr;
+    int32_t *key = nullptr, *tmpslot = nullptr, *members = nullptr, *ncount = nullptr, *n_alive = nullptr;
+    unsigned long long *pairs = nullptr;  // [0] last search, [1] running total
+    float *h_range = nullptr;
+    float rmin2 = -1.0f, rmax2 = INFINITY;
+    // search state
+    eppx_ns *last_ns = nullptr;
+    int32_t last_crit = 0, last_action = 0, last_kernel = 0, last_fluid = 0, last_wall = 0;
+    bool searched = false;
+    // columns the kernels do not own, kept on the host between upload and download
+    std::vector<int32_t> pass_int_cols, pass_float_cols;
+    std::vector<int32_t> pass_int;
+    std::vector<float> pass_float;
+    Fields &f() { return buf[cur]; }
+};
+
+namespace {
+
+int32_t set_error(eppx_ctx *ctx, int32_t code, const std::string &msg) {
+    if (ctx) ctx->error = msg;
+    return code;
+}
+
+int32_t hip_fail(eppx_ctx *ctx, hipError_t e, const char *what) {
+    char buf[512];
+    snprintf(buf, sizeof(buf), "%s: %s (%d)", what, hipGetErrorString(e), (int)e);
+    return set_error(ctx, e == hipErrorOutOfMemory ? EPPX_E_NOMEM : EPPX_E_HIP, buf);
+}
+
+#define HIP_OK(ctx, expr)                                         \
+    do {                                                          \
+        hipError_t e_ = (expr);                                   \
+        if (e_ != hipSuccess) return hip_fail((ctx), e_, #expr);  \
+    } while (0)
+
+template <class T>
+hipError_t dalloc(T **p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    return hipMalloc((void **)p, count * sizeof(T));
+}
+
+template <class T>
+void dfree(T *&p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+hipError_t alloc_fields(Fields &f, int32_t cap) {
+    hipError_t e = hipSuccess;
+    auto a = [&](auto **p) {
+        if (e == hipSuccess) e = dalloc(p, (size_t)cap);
+    };
+    a(&f.pos); a(&f.vel); a(&f.dvel);
+    a(&f.mass); a(&f.vol); a(&f.rho); a(&f.drho); a(&f.p); a(&f.h); a(&f.wv); a(&f.wvrho);
+    a(&f.tag); a(&f.mov); a(&f.id); a(&f.cell); a(&f.alive);
+    return e;
+}
+
+void free_fields(Fields &f) {
+    dfree(f.pos); dfree(f.vel); dfree(f.dvel);
+    dfree(f.mass); dfree(f.vol); dfree(f.rho); dfree(f.drho); dfree(f.p); dfree(f.h); dfree(f.wv); dfree(f.wvrho);
+    dfree(f.tag); dfree(f.mov); dfree(f.id); dfree(f.cell); dfree(f.alive);
+}
+
+bool owned_float(const eppx_layout &L, int c) {
+    auto in = [&](int start, int width) { return start >= 0 && c >= start && c < start + width; };
+    return in(L.PositionVec, 2) || in(L.VelocityVec, 2) || in(L.dVelocityVec, 2) || in(L.Mass, 1) ||
+           in(L.Volume, 1) || in(L.Density, 1) || in(L.dDensity, 1) || in(L.Pressure, 1) || in(L.H, 1) ||
+           in(L.SumWeight, 1) || in(L.SumWeightedDensity, 1) || in(L.nRVec, 2 * L.max_neighbour_number) ||
+           in(L.nR, L.max_neighbour_number) || in(L.nW, L.max_neighbour_number) ||
+           in(L.nDW, L.max_neighbour_number) || in(L.nHInv, L.max_neighbour_number);
+}
+
+bool owned_int(const eppx_layout &L, int c) {
+    auto in = [&](int start, int width) { return start >= 0 && c >= start && c < start + width; };
+    return in(L.Tag, 1) || in(L.IsMovable, 1) || in(L.nCount, 1) || in(L.nIndex, L.max_neighbour_number);
+}
+
+Columns columns_of(const eppx_ps *ps) {
+    const eppx_layout &L = ps->layout;
+    return Columns{ps->cap, L.n_int, L.n_float, L.Tag, L.IsMovable, L.PositionVec, L.VelocityVec, L.dVelocityVec,
+                   L.Mass, L.Volume, L.Density, L.dDensity, L.Pressure, L.H, L.SumWeight, L.SumWeightedDensity};
+}
+
+// Event-bracketed launch when the context times kernels.
+template <class F>
+hipError_t timed(eppx_ctx *ctx, const char *name, F &&launch) {
+    if (!ctx->timing) return launch();
+    auto take = [&]() {
+        hipEvent_t e = nullptr;
+        if (!ctx->event_pool.empty()) {
+            e = ctx->event_pool.back();
+            ctx->event_pool.pop_back();
+        } else {
+            (void)hipEventCreate(&e);
+        }
+        return e;
+    };
+    PendingTiming t{name, take(), take()};
+    (void)hipEventRecord(t.start, ctx->stream);
+    hipError_t e = launch();
+    (void)hipEventRecord(t.stop, ctx->stream);
+    ctx->pending.push_back(t);
+    return e;
+}
+
+void resolve_timing(eppx_ctx *ctx) {
+    for (auto &t : ctx->pending) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, t.start, t.stop) == hipSuccess) {
+            auto &slot = ctx->timed[t.name];
+            slot.first += 1000.0 * ms;
+            slot.second += 1;
+        }
+        ctx->event_pool.push_back(t.start);
+        ctx->event_pool.push_back(t.stop);
+    }
+    ctx->pending.clear();
+}
+
+int32_t read_flags(eppx_ctx *ctx) {
+    uint32_t flags = 0;
+    HIP_OK(ctx, hipMemcpyAsync(&flags, ctx->d_flags, sizeof(flags), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->timing) resolve_timing(ctx);
+    if (!flags) return EPPX_OK;
+    HIP_OK(ctx, hipMemsetAsync(ctx->d_flags, 0, sizeof(uint32_t), ctx->stream));
+    HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
+    std::string msg = "capacity exceeded:";
+    if (flags & kFlagCellOverflow) msg += " a cell holds more than max_neighbour_number particles;";
+    if (flags & kFlagNeighbourOverflow) msg += " a particle has more than max_neighbour_number neighbours;";
+    if (flags & kFlagRankFallback) msg += " a cell holds more than 1024 particles (in-cell order unspecified);";
+    return set_error(ctx, EPPX_E_CAPACITY, msg);
+}
+
+int32_t ensure_cells(eppx_ps *ps, int32_t n_cells) {
+    if (ps->n_cells == n_cells) return EPPX_OK;
+    eppx_ctx *ctx = ps->ctx;
+    HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
+    dfree(ps->counts);
+    dfree(ps->start);
+    dfree(ps->tile_sums);
+    const int64_t nb = (int64_t)n_cells + 1;  // cells + dead bucket
+    HIP_OK(ctx, dalloc(&ps->counts, (size_t)nb));
+    HIP_OK(ctx, dalloc(&ps->start, (size_t)nb + 1));
+    HIP_OK(ctx, dalloc(&ps->tile_sums, (size_t)((nb + kScanTile - 1) / kScanTile)));
+    ps->n_cells = n_cells;
+    return EPPX_OK;
+}
+
+Walk make_walk(const eppx_ps *ps, const eppx_ns *ns, int32_t criterion) {
+    Walk w{};
+    const Fields &f = ps->buf[ps->cur];
+    w.g = ns->grid;
+    w.start = ps->start;
+    w.pos_nb = ps->pos_nb;
+    w.tag = f.tag;
+    w.cell = f.cell;
+    w.h = f.h;
+    w.n_alive = ps->n_alive;
+    memcpy(w.adj, ns->adj, sizeof(w.adj));
+    w.criterion = criterion;
+    w.rmin2 = ps->rmin2;
+    w.rmax2 = ps->rmax2;
+    return w;
+}
+
+bool has(int32_t col) { return col >= 0; }
+
+// Columns an action reads or writes must exist in the layout (the reference
+// fails to compile an action that names a missing PM field).
+int32_t check_columns(eppx_ps *ps, std::initializer_list<std::pair<int32_t, const char *>> cols, const char *what) {
+    for (auto &c : cols)
+        if (!has(c.first))
+            return set_error(ps->ctx, EPPX_E_ARG, std::string(what) + " needs column " + c.second);
+    return EPPX_OK;
+}
+
+int32_t physics_arg(eppx_ctx *ctx, const void *params, size_t size, Physics *out) {
+    if (!params || size != sizeof(eppx_physics))
+        return set_error(ctx, EPPX_E_ARG, "parameter block must be an eppx_physics");
+    const eppx_physics *p = static_cast<const eppx_physics *>(params);
+    if (p->kernel != EPPX_KERNEL_CUBIC && p->kernel != EPPX_KERNEL_WENDLAND_C2)
+        return set_error(ctx, EPPX_E_ARG, "unknown kernel");
+    if (p->eos != EPPX_EOS_LINEAR && p->eos != EPPX_EOS_TAIT) return set_error(ctx, EPPX_E_ARG, "unknown eos");
+    if (p->visc != EPPX_VISC_CLASSIC && p->visc != EPPX_VISC_ARTIFICIAL)
+        return set_error(ctx, EPPX_E_ARG, "unknown viscosity");
+    *out = to_physics(*p);
+    return EPPX_OK;
+}
+
+int32_t field_arg(eppx_ctx *ctx, const eppx_ps *ps, const void *params, size_t size, eppx_field_op *out) {
+    if (!params || size != sizeof(eppx_field_op))
+        return set_error(ctx, EPPX_E_ARG, "parameter block must be an eppx_field_op");
+    *out = *static_cast<const eppx_field_op *>(params);
+    const eppx_layout &L = ps->layout;
+    int32_t col;
+    switch (out->field) {
+        case EPPX_FIELD_POSITION: col = L.PositionVec; break;
+        case EPPX_FIELD_VELOCITY: col = L.VelocityVec; break;
+        case EPPX_FIELD_DVELOCITY: col = L.dVelocityVec; break;
+        case EPPX_FIELD_MASS: col = L.Mass; break;
+        case EPPX_FIELD_VOLUME: col = L.Volume; break;
+        case EPPX_FIELD_DENSITY: col = L.Density; break;
+        case EPPX_FIELD_DDENSITY: col = L.dDensity; break;
+        case EPPX_FIELD_PRESSURE: col = L.Pressure; break;
+        default: return set_error(ctx, EPPX_E_ARG, "unknown field");
+    }
+    const int32_t width = out->field <= EPPX_FIELD_DVELOCITY ? 2 : 1;
+    if (col < 0 || out->component < 0 || out->component >= width)
+        return set_error(ctx, EPPX_E_ARG, "field not in the layout or component out of range");
+    return EPPX_OK;
+}
+
+// clean! + insert + cell-order move + find + fused action.
+int32_t do_search(eppx_ps *ps, eppx_ns *ns, int32_t criterion, int32_t action, const Physics &ph) {
+    eppx_ctx *ctx = ps->ctx;
+    int32_t rc = ensure_cells(ps, ns->grid.n_cells);
+    if (rc) return rc;
+    const int32_t nc = ns->grid.n_cells;
+    hipStream_t s = ctx->stream;
+    Fields cur = ps->buf[ps->cur], nxt = ps->buf[ps->cur ^ 1];
+    HIP_OK(ctx, timed(ctx, "cell_reset", [&] { return launch_reset_search(ps->counts, nc + 1, ps->pairs, s); }));
+    HIP_OK(ctx, timed(ctx, "cell_key", [&] {
+               return launch_key_count(cur, ps->n, ns->grid, ps->counts, ps->key, ps->tmpslot, ctx->d_flags, ns->M,
+                                       s);
+           }));
+    HIP_OK(ctx, timed(ctx, "cell_scan", [&] { return launch_scan(ps->counts, nc + 1, ps->start, ps->tile_sums, s); }));
+    HIP_OK(ctx, timed(ctx, "cell_members", [&] {
+               return launch_fill_members(cur, ps->n, nc, ps->key, ps->tmpslot, ps->start, ps->members, s);
+           }));
+    HIP_OK(ctx, timed(ctx, "cell_permute", [&] {
+               return launch_permute(cur, nxt, ps->pos_nb, ps->n, ps->cap, nc, ps->key, ps->tmpslot, ps->start,
+                                     ps->members, ps->n_alive, ctx->d_flags, s);
+           }));
+    ps->cur ^= 1;
+    Walk w = make_walk(ps, ns, criterion);
+    const int fast = ctx->math_mode == EPPX_MATH_FAST;
+    HIP_OK(ctx, timed(ctx, "search", [&] {
+               return launch_search(w, ps->f(), ps->n, action, ph, fast, ns->M, ps->ncount, &ps->pairs[0],
+                                    &ps->pairs[1], ctx->d_flags, s);
+           }));
+    ps->last_ns = ns;
+    ps->last_crit = criterion;
+    ps->last_action = action;
+    ps->last_kernel = ph.kernel;
+    ps->last_fluid = ph.fluid_tag;
+    ps->last_wall = ph.wall_tag;
+    ps->searched = true;
+    return EPPX_OK;
+}
+
+int32_t do_self(eppx_ps *ps, int32_t action, const Physics &ph, const eppx_field_op &op) {
+    eppx_ctx *ctx = ps->ctx;
+    const char *name = action == EPPX_SELF_CONTINUITY_EOS ? "continuity_eos"
+                       : action == EPPX_SELF_GRAVITY_MOVE ? "gravity_move"
+                       : action == EPPX_SELF_FILTER       ? "filter_self"
+                                                          : "scale_field";
+    HIP_OK(ctx, timed(ctx, name, [&] { return launch_self(ps->f(), ps->n, action, ph, op, ctx->stream); }));
+    return EPPX_OK;
+}
+
+int32_t do_interaction(eppx_ps *ps, eppx_ns *ns, int32_t action, Physics ph, const eppx_field_op &op) {
+    eppx_ctx *ctx = ps->ctx;
+    ph.kernel = ps->last_kernel;  // the stored nW / nDW of the reference come from the search
+    Walk w = make_walk(ps, ns, ps->last_crit);
+    const int fast = ctx->math_mode == EPPX_MATH_FAST;
+    const char *name = action == EPPX_INTER_MOMENTUM ? "momentum"
+                       : action == EPPX_INTER_FILTER ? "filter_inter"
+                                                     : "add_per_neighbour";
+    HIP_OK(ctx, timed(ctx, name, [&] { return launch_interaction(w, ps->f(), ps->n, action, ph, op, fast,
+                                                                 ctx->stream); }));
+    return EPPX_OK;
+}
+
+}  // namespace
+
+// ===================================================================== ABI
+extern "C" {
+
+int32_t eppx_abi_version(void) { return EPPX_ABI_VERSION; }
+
+int32_t eppx_ctx_create(int32_t device, eppx_ctx **out) {
+    if (!out) return EPPX_E_ARG;
+    *out = nullptr;
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || device < 0 || device >= count) return EPPX_E_HIP;
+    eppx_ctx *ctx = new eppx_ctx();
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+        dalloc(&ctx->d_flags, 1) != hipSuccess || hipMemset(ctx->d_flags, 0, sizeof(uint32_t)) != hipSuccess) {
+        if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+        dfree(ctx->d_flags);
+        delete ctx;
+        return EPPX_E_HIP;
+    }
+    *out = ctx;
+    return EPPX_OK;
+}
+
+int32_t eppx_ctx_destroy(eppx_ctx *ctx) {
+    if (!ctx) return EPPX_E_ARG;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto &t : ctx->pending) {
+        (void)hipEventDestroy(t.start);
+        (void)hipEventDestroy(t.stop);
+    }
+    for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
+    dfree(ctx->d_flags);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return EPPX_OK;
+}
+
+const char *eppx_last_error(const eppx_ctx *ctx) { return ctx ? ctx->error.c_str() : "null context"; }
+
+int32_t eppx_sync(eppx_ctx *ctx) {
+    if (!ctx) return EPPX_E_ARG;
+    HIP_OK(ctx, hipSetDevice(ctx->device));
+    return read_flags(ctx);
+}
+
+int32_t eppx_set_math_mode(eppx_ctx *ctx, int32_t mode) {
+    if (!ctx || (mode != EPPX_MATH_EXACT && mode != EPPX_MATH_FAST)) return EPPX_E_ARG;
+    ctx->math_mode = mode;
+    return EPPX_OK;
+}
+
+int32_t eppx_set_timing(eppx_ctx *ctx, int32_t enable) {
+    if (!ctx) return EPPX_E_ARG;
+    ctx->timing = enable != 0;
+    return EPPX_OK;
+}
+
+int32_t eppx_kernel_time(eppx_ctx *ctx, const char *kernel, double *avg_us, int64_t *launches, int32_t reset) {
+    if (!ctx || !kernel) return EPPX_E_ARG;
+    HIP_OK(ctx, hipSetDevice(ctx->device));
+    HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
+    resolve_timing(ctx);
+    auto it = ctx->timed.find(kernel);
+    const double sum = it == ctx->timed.end() ? 0.0 : it->second.first;
+    const int64_t cnt = it == ctx->timed.end() ? 0 : it->second.second;
+    if (avg_us) *avg_us = cnt ? sum / (double)cnt : 0.0;
+    if (launches) *launches = cnt;
+    if (reset) ctx->timed.clear();
+    return EPPX_OK;
+}
+
+// Domain2D{Int32, Float32}(gap, first_x, first_y, last_x, last_y), Domain2D.jl:28-67.
+int32_t eppx_domain2d_init(double gap, double first_x, double first_y, double last_x, double last_y,
+                           eppx_domain2d *d) {
+    if (!d) return EPPX_E_ARG;
+    const float g = (float)gap;
+    d->gap = g;
+    d->gap_square = g * g;
+    d->first_x = (float)first_x;
+    d->first_y = (float)first_y;
+    d->last_x = (float)last_x;
+    d->last_y = (float)last_y;
+    d->span_x = d->last_x - d->first_x;
+    d->span_y = d->last_y - d->first_y;
+    const float nx = floorf(d->span_x / g), ny = floorf(d->span_y / g);
+    if (!(nx >= 1.0f) || !(ny >= 1.0f) || nx > 2.0e9f || ny > 2.0e9f || (double)nx * (double)ny > 2.0e9) {
+        return EPPX_E_ARG;
+    }
+    d->n_x = (int32_t)nx;
+    d->n_y = (int32_t)ny;
+    d->n = d->n_x * d->n_y;
+    d->gap_x = d->span_x / (float)d->n_x;
+    d->gap_y = d->span_y / (float)d->n_y;
+    d->gap_x_inv = 1.0f / d->gap_x;
+    d->gap_y_inv = 1.0f / d->gap_y;
+    return EPPX_OK;
+}
+
+// inside / indexCartesianFromPosition / indexLinearFromPosition, Domain2D.jl:110-137.
+int32_t eppx_domain2d_locate(const eppx_domain2d *d, float x, float y, int32_t *inside, int32_t *pi, int32_t *pj,
+                             int32_t *linear) {
+    if (!d) return EPPX_E_ARG;
+    const int in = d->first_x <= x && x <= d->last_x && d->first_y <= y && y <= d->last_y;
+    int32_t i = (int32_t)floorf((x - d->first_x) * d->gap_x_inv) + 1;
+    i = std::max(1, std::min(d->n_x, i));
+    int32_t j = (int32_t)floorf((y - d->first_y) * d->gap_y_inv) + 1;
+    j = std::max(1, std::min(d->n_y, j));
+    if (inside) *inside = in;
+    if (pi) *pi = i;
+    if (pj) *pj = j;
+    if (linear) *linear = i + d->n_x * (j - 1);
+    return EPPX_OK;
+}
+
+int32_t eppx_ps_create(eppx_ctx *ctx, int32_t n_capacity, const eppx_layout *layout, eppx_ps **out) {
+    if (!ctx || !layout || !out || n_capacity < 0) return set_error(ctx, EPPX_E_ARG, "eppx_ps_create: bad argument");
+    *out = nullptr;
+    const eppx_layout &L = *layout;
+    if (L.n_int < 0 || L.n_float < 0 || L.max_neighbour_number < 0)
+        return set_error(ctx, EPPX_E_ARG, "eppx_ps_create: negative table width");
+    auto fits = [](int32_t col, int32_t width, int32_t n) { return col < 0 || (col + width <= n); };
+    const int32_t M = L.max_neighbour_number;
+    if (!fits(L.Tag, 1, L.n_int) || !fits(L.IsMovable, 1, L.n_int) || !fits(L.nCount, 1, L.n_int) ||
+        !fits(L.nIndex, M, L.n_int) || !fits(L.PositionVec, 2, L.n_float) || !fits(L.VelocityVec, 2, L.n_float) ||
+        !fits(L.dVelocityVec, 2, L.n_float) || !fits(L.Mass, 1, L.n_float) || !fits(L.Volume, 1, L.n_float) ||
+        !fits(L.Density, 1, L.n_float) || !fits(L.dDensity, 1, L.n_float) || !fits(L.Pressure, 1, L.n_float) ||
+        !fits(L.H, 1, L.n_float) || !fits(L.Gap, 1, L.n_float) || !fits(L.SumWeight, 1, L.n_float) ||
+        !fits(L.SumWeightedDensity, 1, L.n_float) || !fits(L.nRVec, 2 * M, L.n_float) ||
+        !fits(L.nR, M, L.n_float) || !fits(L.nW, M, L.n_float) || !fits(L.nDW, M, L.n_float) ||
+        !fits(L.nHInv, M, L.n_float))
+        return set_error(ctx, EPPX_E_ARG, "eppx_ps_create: a column lies outside its table");
+    if (!has(L.PositionVec) || !has(L.Tag))
+        return set_error(ctx, EPPX_E_ARG, "eppx_ps_create: PositionVec and Tag are required");
+    HIP_OK(ctx, hipSetDevice(ctx->device));
+    eppx_ps *ps = new eppx_ps();
+    ps->ctx = ctx;
+    ps->cap = n_capacity;
+    ps->layout = L;
+    hipError_t e = alloc_fields(ps->buf[0], n_capacity);
+    if (e == hipSuccess) e = alloc_fields(ps->buf[1], n_capacity);
+    if (e == hipSuccess) e = dalloc(&ps->pos_nb, (size_t)n_capacity);
+    if (e == hipSuccess) e = dalloc(&ps->key, (size_t)n_capacity);
+    if (e == hipSuccess) e = dalloc(&ps->tmpslot, (size_t)n_capacity);
+    if (e == hipSuccess) e = dalloc(&ps->members, (size_t)n_capacity);
+    if (e == hipSuccess) e = dalloc(&ps->ncount, (size_t)n_capacity);
+    if (e == hipSuccess) e = dalloc(&ps->n_alive, 1);
+    if (e == hipSuccess) e = dalloc(&ps->pairs, 2);
+    if (e == hipSuccess) e = dalloc(&ps->h_range, 3);
+    if (e == hipSuccess) e = hipMemsetAsync(ps->pairs, 0, 2 * sizeof(unsigned long long), ctx->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(ps->n_alive, 0, sizeof(int32_t), ctx->stream);
+    if (e != hipSuccess) {
+        eppx_ps_destroy(ps);
+        return hip_fail(ctx, e, "eppx_ps_create");
+    }
+    for (int c = 0; c < L.n_int; ++c)
+        if (!owned_int(L, c)) ps->pass_int_cols.push_back(c);
+    for (int c = 0; c < L.n_float; ++c)
+        if (!owned_float(L, c)) ps->pass_float_cols.push_back(c);
+    *out = ps;
+    return EPPX_OK;
+}
+
+int32_t eppx_ps_destroy(eppx_ps *ps) {
+    if (!ps) return EPPX_E_ARG;
+    (void)hipSetDevice(ps->ctx->device);
+    (void)hipStreamSynchronize(ps->ctx->stream);
+    free_fields(ps->buf[0]);
+    free_fields(ps->buf[1]);
+    dfree(ps->pos_nb);
+    dfree(ps->counts); dfree(ps->start); dfree(ps->tile_sums);
+    dfree(ps->key); dfree(ps->tmpslot); dfree(ps->members); dfree(ps->ncount); dfree(ps->n_alive);
+    dfree(ps->pairs); dfree(ps->h_range);
+    delete ps;
+    return EPPX_OK;
+}
+
+int32_t eppx_ps_upload(eppx_ps *ps, int32_t n_particles, const int32_t *is_alive, const int32_t *cell_index,
+                       const int32_t *int_props, const float *float_props) {
+    if (!ps) return EPPX_E_ARG;
+    eppx_ctx *ctx = ps->ctx;
+    const eppx_layout &L = ps->layout;
+    if (n_particles < 0 || n_particles > ps->cap || !is_alive || !cell_index || (L.n_int && !int_props) ||
+        (L.n_float && !float_props))
+        return set_error(ctx, EPPX_E_ARG, "eppx_ps_upload: bad argument (n_particles must be <= capacity)");
+    HIP_OK(ctx, hipSetDevice(ctx->device));
+    const size_t cap = (size_t)ps->cap;
+    int32_t *d_ip = nullptr, *d_alive = nullptr, *d_cell = nullptr;
+    float *d_fp = nullptr;
+    HIP_OK(ctx, dalloc(&d_ip, cap * (size_t)L.n_int));
+    HIP_OK(ctx, dalloc(&d_fp, cap * (size_t)L.n_float));
+    HIP_OK(ctx, dalloc(&d_alive, cap));
+    HIP_OK(ctx, dalloc(&d_cell, cap));
+    hipStream_t s = ctx->stream;
+    hipError_t e = hipSuccess;
+    if (cap) {
+        if (L.n_int) e = hipMemcpyAsync(d_ip, int_props, cap * L.n_int * sizeof(int32_t), hipMemcpyHostToDevice, s);
+        if (e == hipSuccess && L.n_float)
+            e = hipMemcpyAsync(d_fp, float_props, cap * L.n_float * sizeof(float), hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(d_alive, is_alive, cap * sizeof(int32_t), hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(d_cell, cell_index, cap * sizeof(int32_t), hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = launch_from_ref(ps->buf[0], ps->cap, columns_of(ps), d_ip, d_fp, d_alive, d_cell, s);
+    }
+    ps->cur = 0;
+    ps->n = n_particles;
+    // H range over alive particles for the criterion's exact-safe bounds.
+    const float init[3] = {INFINITY, 0.0f, 0.0f};
+    float range[3] = {0.0f, 0.0f, 0.0f};
+    if (e == hipSuccess) e = hipMemcpyAsync(ps->h_range, init, sizeof(init), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && has(L.H) && n_particles) e = launch_h_range(ps->buf[0], n_particles, ps->h_range, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(range, ps->h_range, sizeof(range), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    dfree(d_ip);
+    dfree(d_fp);
+    dfree(d_alive);
+    dfree(d_cell);
+    if (e != hipSuccess) return hip_fail(ctx, e, "eppx_ps_upload");
+    uint32_t bad;
+    memcpy(&bad, &range[2], sizeof(bad));
+    const double hmin = range[0], hmax = range[1];
+    if (bad || !(hmin >= 1e-30) || !(hmax <= 1e30) || !(hmin <= hmax)) {
+        ps->rmin2 = -1.0f;
+        ps->rmax2 = INFINITY;
+    } else {
+        const double eps = 1.0 / 1048576.0;  // 2^-20, well above the criterion's rounding
+        ps->rmin2 = nextafterf((float)(4.0 * hmin * hmin * (1.0 - eps)), 0.0f);
+        ps->rmax2 = nextafterf((float)(4.0 * hmax * hmax * (1.0 + eps)), INFINITY);
+    }
+    // keep the columns the kernels do not own
+    ps->pass_int.assign(ps->pass_int_cols.size() * cap, 0);
+    ps->pass_float.assign(ps->pass_float_cols.size() * cap, 0.0f);
+    for (size_t k = 0; k < ps->pass_int_cols.size(); ++k)
+        memcpy(&ps->pass_int[k * cap], int_props + (size_t)ps->pass_int_cols[k] * cap, cap * sizeof(int32_t));
+    for (size_t k = 0; k < ps->pass_float_cols.size(); ++k)
+        memcpy(&ps->pass_float[k * cap], float_props + (size_t)ps->pass_float_cols[k] * cap, cap * sizeof(float));
+    ps->searched = false;
+    ps->last_ns = nullptr;
+    return EPPX_OK;
+}
+
+int32_t eppx_ps_download(eppx_ps *ps, eppx_ns *ns, int32_t *n_particles, int32_t *is_alive, int32_t *cell_index,
+                         int32_t *int_props, float *float_props) {
+    if (!ps) return EPPX_E_ARG;
+    eppx_ctx *ctx = ps->ctx;
+    const eppx_layout &L = ps->layout;
+    if (!is_alive || !cell_index || (L.n_int && !int_props) || (L.n_float && !float_props))
+        return set_error(ctx, EPPX_E_ARG, "eppx_ps_download: bad argument");
+    HIP_OK(ctx, hipSetDevice(ctx->device));
+    const size_t cap = (size_t)ps->cap;
+    if (n_particles) *n_particles = ps->n;
+    if (!cap) return read_flags(ctx);
+    int32_t *d_ip = nullptr, *d_alive = nullptr, *d_cell = nullptr;
+    float *d_fp = nullptr;
+    HIP_OK(ctx, dalloc(&d_ip, cap * (size_t)L.n_int));
+    HIP_OK(ctx, dalloc(&d_fp, cap * (size_t)L.n_float));
+    HIP_OK(ctx, dalloc(&d_alive, cap));
+    HIP_OK(ctx, dalloc(&d_cell, cap));
+    hipStream_t s = ctx->stream;
+    hipError_t e = hipMemsetAsync(d_ip, 0, std::max<size_t>(1, cap * L.n_int) * sizeof(int32_t), s);
+    if (e == hipSuccess) e = hipMemsetAsync(d_fp, 0, std::max<size_t>(1, cap * L.n_float) * sizeof(float), s);
+    if (e == hipSuccess) e = launch_to_ref(ps->f(), ps->cap, columns_of(ps), d_ip, d_fp, d_alive, d_cell, s);
+    const int32_t M = L.max_neighbour_number;
+    if (e == hipSuccess && ns && ps->searched && ps->last_ns == ns &&
+        (has(L.nCount) || has(L.nIndex) || has(L.nRVec) || has(L.nR) || has(L.nW) || has(L.nDW) || has(L.nHInv))) {
+        NeighbourExport ex{};
+        ex.n_capacity = ps->cap;
+        ex.M = M;
+        ex.nCount = has(L.nCount) ? d_ip + cap * L.nCount : nullptr;
+        ex.nIndex = has(L.nIndex) && M ? d_ip + cap * L.nIndex : nullptr;
+        ex.nRVec = has(L.nRVec) && M ? d_fp + cap * L.nRVec : nullptr;
+        ex.nR = has(L.nR) && M ? d_fp + cap * L.nR : nullptr;
+        ex.nW = has(L.nW) && M ? d_fp + cap * L.nW : nullptr;
+        ex.nDW = has(L.nDW) && M ? d_fp + cap * L.nDW : nullptr;
+        ex.nHInv = has(L.nHInv) && M ? d_fp + cap * L.nHInv : nullptr;
+        ex.kernel = ps->last_kernel;
+        ex.with_kernel = ps->last_action == EPPX_SEARCH_KERNEL_CONTINUITY;
+        ex.fluid_tag = ps->last_fluid;
+        ex.wall_tag = ps->last_wall;
+        e = launch_export(make_walk(ps, ns, ps->last_crit), ps->f(), ps->n, nullptr, nullptr, ex, s);
+    }
+    if (e == hipSuccess && L.n_int)
+        e = hipMemcpyAsync(int_props, d_ip, cap * L.n_int * sizeof(int32_t), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && L.n_float)
+        e = hipMemcpyAsync(float_props, d_fp, cap * L.n_float * sizeof(float), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(is_alive, d_alive, cap * sizeof(int32_t), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(cell_index, d_cell, cap * sizeof(int32_t), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    dfree(d_ip);
+    dfree(d_fp);
+    dfree(d_alive);
+    dfree(d_cell);
+    if (e != hipSuccess) return hip_fail(ctx, e, "eppx_ps_download");
+    if (ps->pass_int.size() == ps->pass_int_cols.size() * cap)
+        for (size_t k = 0; k < ps->pass_int_cols.size(); ++k)
+            memcpy(int_props + (size_t)ps->pass_int_cols[k] * cap, &ps->pass_int[k * cap], cap * sizeof(int32_t));
+    if (ps->pass_float.size() == ps->pass_float_cols.size() * cap)
+        for (size_t k = 0; k < ps->pass_float_cols.size(); ++k)
+            memcpy(float_props + (size_t)ps->pass_float_cols[k] * cap, &ps->pass_float[k * cap], cap * sizeof(float));
+    return read_flags(ctx);
+}
+
+int32_t eppx_ps_alive_count(eppx_ps *ps, int32_t *out) {
+    if (!ps || !out) return EPPX_E_ARG;
+    eppx_ctx *ctx = ps->ctx;
+    HIP_OK(ctx, hipSetDevice(ctx->device));
+    std::vector<int32_t> alive((size_t)ps->n);
+    if (ps->n)
+        HIP_OK(ctx, hipMemcpyAsync(alive.data(), ps->f().alive, alive.size() * sizeof(int32_t), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+    HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
+    int32_t c = 0;
+    for (int32_t a : alive) c += a == 1;
+    *out = c;
+    return EPPX_OK;
+}
+
+int32_t eppx_ns_create(eppx_ctx *ctx, const eppx_domain2d *domain, const int32_t *adjacency, int32_t max_tag,
+                       int32_t max_neighbour_number, eppx_ns **out) {
+    if (!ctx || !domain || !adjacency || !out || max_tag < 1 || max_tag > kMaxTag || max_neighbour_number < 0)
+        return set_error(ctx, EPPX_E_ARG, "eppx_ns_create: bad argument (1 <= max_tag <= 32)");
+    if (domain->n_x < 1 || domain->n_y < 1 || (int64_t)domain->n_x * domain->n_y != domain->n)
+        return set_error(ctx, EPPX_E_ARG, "eppx_ns_create: inconsistent domain");
+    eppx_ns *ns = new eppx_ns();
+    ns->ctx = ctx;
+    ns->domain = *domain;
+    const eppx_domain2d &d = *domain;
+    ns->grid = Grid{d.first_x, d.first_y, d.last_x, d.last_y, d.gap_x_inv, d.gap_y_inv, d.gap_square,
+                    d.n_x, d.n_y, d.n};
+    ns->max_tag = max_tag;
+    ns->M = max_neighbour_number;
+    for (int t = 1; t <= max_tag; ++t)
+        for (int u = 1; u <= max_tag; ++u)
+            if (adjacency[(t - 1) + (size_t)max_tag * (u - 1)] == 1) ns->adj[t] |= 1u << (u - 1);
+    *out = ns;
+    return EPPX_OK;
+}
+
+int32_t eppx_ns_destroy(eppx_ns *ns) {
+    if (!ns) return EPPX_E_ARG;
+    delete ns;
+    return EPPX_OK;
+}
+
+int32_t eppx_ns_cell_stencil(eppx_ns *ns, int32_t *count, int32_t *list) {
+    if (!ns || !count || !list) return EPPX_E_ARG;
+    eppx_ctx *ctx = ns->ctx;
+    HIP_OK(ctx, hipSetDevice(ctx->device));
+    const size_t nc = (size_t)ns->grid.n_cells;
+    int32_t *d_count = nullptr, *d_list = nullptr;
+    HIP_OK(ctx, dalloc(&d_count, nc));
+    HIP_OK(ctx, dalloc(&d_list, nc * 9));
+    hipStream_t s = ctx->stream;
+    hipError_t e = hipMemsetAsync(d_list, 0, nc * 9 * sizeof(int32_t), s);
+    if (e == hipSuccess) e = launch_stencil(ns->grid, d_count, d_list, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(count, d_count, nc * sizeof(int32_t), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(list, d_list, nc * 9 * sizeof(int32_t), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    dfree(d_count);
+    dfree(d_list);
+    if (e != hipSuccess) return hip_fail(ctx, e, "eppx_ns_cell_stencil");
+    return EPPX_OK;
+}
+
+int32_t eppx_search(eppx_ps *ps, eppx_ns *ns, int32_t criterion, int32_t action, const void *params,
+                    size_t params_size) {
+    if (!ps || !ns) return EPPX_E_ARG;
+    eppx_ctx *ctx = ps->ctx;
+    if (ns->ctx != ctx) return set_error(ctx, EPPX_E_ARG, "eppx_search: systems belong to different contexts");
+    if (criterion != EPPX_CRIT_DEFAULT && criterion != EPPX_CRIT_SYMMETRY)
+        return set_error(ctx, EPPX_E_ARG, "eppx_search: unknown criterion");
+    const eppx_layout &L = ps->layout;
+    if (criterion == EPPX_CRIT_SYMMETRY && !has(L.H))
+        return set_error(ctx, EPPX_E_ARG, "eppx_search: the symmetry criterion needs column H");
+    Physics ph{};
+    if (action == EPPX_SEARCH_KERNEL_CONTINUITY) {
+        int32_t rc = physics_arg(ctx, params, params_size, &ph);
+        if (rc) return rc;
+        rc = check_columns(ps, {{L.H, "H"}, {L.VelocityVec, "VelocityVec"}, {L.Mass, "Mass"}, {L.Volume, "Volume"},
+                                {L.Density, "Density"}, {L.dDensity, "dDensity"}},
+                           "sComputeKernelAndiContinuity");
+        if (rc) return rc;
+    } else if (action != EPPX_SEARCH_NONE) {
+        return set_error(ctx, EPPX_E_ARG, "eppx_search: unknown action");
+    }
+    HIP_OK(ctx, hipSetDevice(ctx->device));
+    return do_search(ps, ns, criterion, action, ph);
+}
+
+int32_t eppx_selfaction(eppx_ps *ps, int32_t action, const void *params, size_t params_size) {
+    if (!ps) return EPPX_E_ARG;
+    eppx_ctx *ctx = ps->ctx;
+    const eppx_layout &L = ps->layout;
+    Physics ph{};
+    eppx_field_op op{};
+    int32_t rc;
+    switch (action) {
+        case EPPX_SELF_CONTINUITY_EOS:
+            rc = physics_arg(ctx, params, params_size, &ph);
+            if (!rc)
+                rc = check_columns(ps, {{L.Density, "Density"}, {L.dDensity, "dDensity"}, {L.Volume, "Volume"},
+                                        {L.Mass, "Mass"}, {L.Pressure, "Pressure"}},
+                                   "sContinuity");
+            break;
+        case EPPX_SELF_GRAVITY_MOVE:
+            rc = physics_arg(ctx, params, params_size, &ph);
+            if (!rc)
+                rc = check_columns(ps, {{L.VelocityVec, "VelocityVec"}, {L.dVelocityVec, "dVelocityVec"}},
+                                   "sMomentum");
+            break;
+        case EPPX_SELF_FILTER:
+            rc = physics_arg(ctx, params, params_size, &ph);
+            if (!rc)
+                rc = check_columns(ps, {{L.SumWeight, "SumWeight"}, {L.SumWeightedDensity, "SumWeightedDensity"},
+                                        {L.H, "H"}, {L.Volume, "Volume"}, {L.Mass, "Mass"}, {L.Density, "Density"},
+                                        {L.Pressure, "Pressure"}},
+                                   "sFilter");
+            break;
+        case EPPX_SELF_SCALE_FIELD:
+            rc = field_arg(ctx, ps, params, params_size, &op);
+            break;
+        default:
+            rc = set_error(ctx, EPPX_E_ARG, "eppx_selfaction: unknown action");
+    }
+    if (rc) return rc;
+    HIP_OK(ctx, hipSetDevice(ctx->device));
+    return do_self(ps, action, ph, op);
+}
+
+int32_t eppx_interaction(eppx_ps *ps, eppx_ns *ns, int32_t action, const void *params, size_t params_size) {
+    if (!ps || !ns) return EPPX_E_ARG;
+    eppx_ctx *ctx = ps->ctx;
+    if (!ps->searched || ps->last_ns != ns)
+        return set_error(ctx, EPPX_E_STATE, "eppx_interaction: no search on this neighbour system yet");
+    const eppx_layout &L = ps->layout;
+    Physics ph{};
+    eppx_field_op op{};
+    int32_t rc;
+    switch (action) {
+        case EPPX_INTER_MOMENTUM:
+        case EPPX_INTER_FILTER:
+            if (ps->last_action != EPPX_SEARCH_KERNEL_CONTINUITY)
+                return set_error(ctx, EPPX_E_STATE,
+                                 "eppx_interaction: kernel values come from a search with "
+                                 "EPPX_SEARCH_KERNEL_CONTINUITY");
+            rc = physics_arg(ctx, params, params_size, &ph);
+            if (!rc && action == EPPX_INTER_MOMENTUM)
+                rc = check_columns(ps, {{L.Density, "Density"}, {L.Pressure, "Pressure"}, {L.Mass, "Mass"},
+                                        {L.VelocityVec, "VelocityVec"}, {L.dVelocityVec, "dVelocityVec"}},
+                                   "iMomentum");
+            if (!rc && action == EPPX_INTER_FILTER)
+                rc = check_columns(ps, {{L.SumWeight, "SumWeight"}, {L.SumWeightedDensity, "SumWeightedDensity"},
+                                        {L.Volume, "Volume"}, {L.Mass, "Mass"}},
+                                   "iFilter");
+            break;
+        case EPPX_INTER_ADD_PER_NEIGHBOUR:
+            rc = field_arg(ctx, ps, params, params_size, &op);
+            break;
+        default:
+            rc = set_error(ctx, EPPX_E_ARG, "eppx_interaction: unknown action");
+    }
+    if (rc) return rc;
+    HIP_OK(ctx, hipSetDevice(ctx->device));
+    return do_interaction(ps, ns, action, ph, op);
+}
+
+int32_t eppx_step(eppx_ps *ps, eppx_ns *ns, const eppx_physics *phys, int32_t n_steps, int32_t filter_interval,
+                  int64_t step_offset) {
+    if (!ps || !ns || !phys || n_steps < 0) return EPPX_E_ARG;
+    eppx_ctx *ctx = ps->ctx;
+    if (!ps->searched || ps->last_ns != ns || ps->last_action != EPPX_SEARCH_KERNEL_CONTINUITY ||
+        ps->last_crit != EPPX_CRIT_SYMMETRY)
+        return set_error(ctx, EPPX_E_STATE,
+                         "eppx_step: needs a prior eppx_search(symmetry, kernel+continuity) on this system");
+    Physics ph{};
+    int32_t rc = physics_arg(ctx, phys, sizeof(*phys), &ph);
+    if (rc) return rc;
+    const eppx_layout &L = ps->layout;
+    rc = check_columns(ps, {{L.Density, "Density"}, {L.dDensity, "dDensity"}, {L.Volume, "Volume"}, {L.Mass, "Mass"},
+                            {L.Pressure, "Pressure"}, {L.VelocityVec, "VelocityVec"},
+                            {L.dVelocityVec, "dVelocityVec"}, {L.H, "H"}},
+                       "eppx_step");
+    if (rc) return rc;
+    if (filter_interval > 0)
+        rc = check_columns(ps, {{L.SumWeight, "SumWeight"}, {L.SumWeightedDensity, "SumWeightedDensity"}},
+                           "eppx_step filter");
+    if (rc) return rc;
+    HIP_OK(ctx, hipSetDevice(ctx->device));
+    const eppx_field_op op{};
+    for (int32_t k = 1; k <= n_steps; ++k) {
+        if ((rc = do_self(ps, EPPX_SELF_CONTINUITY_EOS, ph, op))) return rc;
+        if ((rc = do_interaction(ps, ns, EPPX_INTER_MOMENTUM, ph, op))) return rc;
+        if ((rc = do_self(ps, EPPX_SELF_GRAVITY_MOVE, ph, op))) return rc;
+        if ((rc = do_search(ps, ns, EPPX_CRIT_SYMMETRY, EPPX_SEARCH_KERNEL_CONTINUITY, ph))) return rc;
+        if (filter_interval > 0 && (step_offset + k) % filter_interval == 0) {
+            if ((rc = do_interaction(ps, ns, EPPX_INTER_FILTER, ph, op))) return rc;
+            if ((rc = do_self(ps, EPPX_SELF_FILTER, ph, op))) return rc;
+        }
+    }
+    return EPPX_OK;
+}
+
+int32_t eppx_pair_count(eppx_ps *ps, eppx_ns *ns, int64_t *out) {
+    if (!ps || !ns || !out) return EPPX_E_ARG;
+    eppx_ctx *ctx = ps->ctx;
+    if (!ps->searched || ps->last_ns != ns) return set_error(ctx, EPPX_E_STATE, "eppx_pair_count: no search yet");
+    HIP_OK(ctx, hipSetDevice(ctx->device));
+    unsigned long long v = 0;
+    HIP_OK(ctx, hipMemcpyAsync(&v, &ps->pairs[0], sizeof(v), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
+    *out = (int64_t)v;
+    return read_flags(ctx);
+}
+
+int32_t eppx_pair_total(eppx_ps *ps, int64_t *out, int32_t reset) {
+    if (!ps || !out) return EPPX_E_ARG;
+    eppx_ctx *ctx = ps->ctx;
+    HIP_OK(ctx, hipSetDevice(ctx->device));
+    unsigned long long v = 0;
+    HIP_OK(ctx, hipMemcpyAsync(&v, &ps->pairs[1], sizeof(v), hipMemcpyDeviceToHost, ctx->stream));
+    if (reset) HIP_OK(ctx, hipMemsetAsync(&ps->pairs[1], 0, sizeof(unsigned long long), ctx->stream));
+    HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
+    *out = (int64_t)v;
+    return EPPX_OK;
+}
+
+int32_t eppx_export_pairs(eppx_ps *ps, eppx_ns *ns, int64_t *offsets, int32_t *js, int64_t capacity) {
+    if (!ps || !ns || !offsets || (capacity > 0 && !js)) return EPPX_E_ARG;
+    eppx_ctx *ctx = ps->ctx;
+    if (!ps->searched || ps->last_ns != ns) return set_error(ctx, EPPX_E_STATE, "eppx_export_pairs: no search yet");
+    HIP_OK(ctx, hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    int32_t na = 0;
+    HIP_OK(ctx, hipMemcpyAsync(&na, ps->n_alive, sizeof(na), hipMemcpyDeviceToHost, s));
+    HIP_OK(ctx, hipStreamSynchronize(s));
+    std::vector<int32_t> cnt((size_t)na), ids((size_t)na);
+    if (na) {
+        HIP_OK(ctx, hipMemcpyAsync(cnt.data(), ps->ncount, na * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        HIP_OK(ctx, hipMemcpyAsync(ids.data(), ps->f().id, na * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        HIP_OK(ctx, hipStreamSynchronize(s));
+    }
+    const int32_t n = ps->n;
+    std::vector<int64_t> by_row((size_t)n + 1, 0);
+    for (int32_t k = 0; k < na; ++k) by_row[(size_t)ids[k] + 1] = cnt[k];
+    for (int32_t r = 0; r < n; ++r) by_row[(size_t)r + 1] += by_row[(size_t)r];
+    memcpy(offsets, by_row.data(), by_row.size() * sizeof(int64_t));
+    const int64_t total = by_row[(size_t)n];
+    if (total > capacity) {
+        char buf[160];
+        snprintf(buf, sizeof(buf), "eppx_export_pairs: %lld pairs exceed capacity %lld", (long long)total,
+                 (long long)capacity);
+        return set_error(ctx, EPPX_E_CAPACITY, buf);
+    }
+    if (!total) return EPPX_OK;
+    std::vector<int64_t> row_start((size_t)na);
+    for (int32_t k = 0; k < na; ++k) row_start[k] = by_row[(size_t)ids[k]];
+    int64_t *d_rs = nullptr;
+    int32_t *d_js = nullptr;
+    HIP_OK(ctx, dalloc(&d_rs, (size_t)na));
+    HIP_OK(ctx, dalloc(&d_js, (size_t)total));
+    NeighbourExport ex{};
+    ex.n_capacity = ps->cap;
+    hipError_t e = hipMemcpyAsync(d_rs, row_start.data(), na * sizeof(int64_t), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = launch_export(make_walk(ps, ns, ps->last_crit), ps->f(), ps->n, d_rs, d_js, ex, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(js, d_js, (size_t)total * sizeof(int32_t), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    dfree(d_rs);
+    dfree(d_js);
+    if (e != hipSuccess) return hip_fail(ctx, e, "eppx_export_pairs");
+    return read_flags(ctx);
+}
+
+int32_t eppx_check_finite(eppx_ps *ps) {
+    if (!ps) return EPPX_E_ARG;
+    eppx_ctx *ctx = ps->ctx;
+    HIP_OK(ctx, hipSetDevice(ctx->device));
+    uint32_t *d_bad = nullptr, bad = 0;
+    HIP_OK(ctx, dalloc(&d_bad, 1));
+    hipError_t e = hipMemsetAsync(d_bad, 0, sizeof(uint32_t), ctx->stream);
+    if (e == hipSuccess && ps->n) e = launch_finite(ps->f(), ps->n, d_bad, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(&bad, d_bad, sizeof(bad), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    dfree(d_bad);
+    if (e != hipSuccess) return hip_fail(ctx, e, "eppx_check_finite");
+    if (bad) return set_error(ctx, EPPX_E_NONFINITE, "non-finite density, pressure, position or velocity");
+    return EPPX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,109 @@
+"""Algorithm: search! / selfaction! / interaction! over libeppx (src/Algorithm).
+
+Same call surface as the reference (NeighbourSearch.jl:323-413,
+Action.jl:53-206).  Like the reference, each call waits for the device and
+then reports (synchronize after every launch); pass sync=False to leave the
+work queued.  static_* and dynamic_* exist for source compatibility: the HIP
+kernels are compiled once, so the two differ in nothing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+from . import SPH, _lib
+from .Class import Domain2D, NeighbourSystem, ParticleSystem
+
+kDefaultThreadNumber = 256
+kDefaultMaxNeighbourNumber = 50
+
+
+class _Criterion:
+    def __init__(self, cid, name):
+        self.id, self.name = cid, name
+
+    def __repr__(self):
+        return self.name
+
+
+defaultSearchCriterion = _Criterion(_lib.CRIT_DEFAULT, "defaultSearchCriterion")      # NeighbourSearch.jl:141-151
+symmetrySearchCriterion = _Criterion(_lib.CRIT_SYMMETRY, "symmetrySearchCriterion")   # NeighbourSearch.jl:153-164
+
+
+def defaultSelfaction():
+    return SPH.defaultSelfaction()
+
+
+def _criterion(criterion) -> int:
+    if isinstance(criterion, _Criterion):
+        return criterion.id
+    raise NotImplementedError("only defaultSearchCriterion and symmetrySearchCriterion cross the C ABI")
+
+
+def _action(action, kind: str) -> SPH.Action:
+    if action is None:
+        action = SPH.defaultSelfaction()
+    if not isinstance(action, SPH.Action):
+        raise NotImplementedError("Julia-closure actions cannot cross the C ABI; use the eppx.SPH action menu")
+    if action.kind != kind:
+        raise ValueError(f"{action.name} is a {action.kind} action, not a {kind} action")
+    return action
+
+
+def search(particle_system: ParticleSystem, domain: Domain2D, neighbour_system: NeighbourSystem, *,
+           criterion=defaultSearchCriterion, action=None, n_threads: int = kDefaultThreadNumber, sync: bool = True):
+    """search!(ps, domain, ns; criterion, action!, n_threads)."""
+    act = _action(action, "search")
+    blk, size = act.params(particle_system)
+    ctx = particle_system.parallel.ctx
+    _lib.check(_lib.lib().eppx_search(particle_system.handle, neighbour_system.handle, _criterion(criterion), act.id,
+                                      C.byref(blk), size), ctx, "search")
+    particle_system.last_ns = neighbour_system
+    if sync:
+        particle_system.parallel.synchronize()
+
+
+static_search = search
+dynamic_search = search
+
+
+def selfaction(particle_system: ParticleSystem, action, *, n_threads: int = kDefaultThreadNumber, sync: bool = True):
+    """selfaction!(ps, action!; n_threads)."""
+    act = _action(action, "self")
+    blk, size = act.params(particle_system)
+    ctx = particle_system.parallel.ctx
+    _lib.check(_lib.lib().eppx_selfaction(particle_system.handle, act.id, C.byref(blk), size), ctx, "selfaction")
+    if sync:
+        particle_system.parallel.synchronize()
+
+
+static_selfaction = selfaction
+dynamic_selfaction = selfaction
+
+
+def interaction(particle_system: ParticleSystem, action, *, n_threads: int = kDefaultThreadNumber,
+                sync: bool = True):
+    """interaction!(ps, action!; n_threads) over the neighbour lists of the last search."""
+    act = _action(action, "inter")
+    if particle_system.last_ns is None:
+        raise _lib.EppxError(_lib.EPPX_E_STATE, "interaction before any search")
+    blk, size = act.params(particle_system)
+    ctx = particle_system.parallel.ctx
+    _lib.check(_lib.lib().eppx_interaction(particle_system.handle, particle_system.last_ns.handle, act.id,
+                                           C.byref(blk), size), ctx, "interaction")
+    if sync:
+        particle_system.parallel.synchronize()
+
+
+static_interaction = interaction
+dynamic_interaction = interaction
+
+
+def step(particle_system: ParticleSystem, neighbour_system: NeighbourSystem, physics: SPH.Physics, n_steps: int = 1,
+         filter_interval: int = 0, step_offset: int = 0, sync: bool = True):
+    """n_steps of the dam-break loop body (main.jl:166-183) in one call."""
+    p = physics.c()
+    ctx = particle_system.parallel.ctx
+    _lib.check(_lib.lib().eppx_step(particle_system.handle, neighbour_system.handle, C.byref(p), int(n_steps),
+                                    int(filter_interval), int(step_offset)), ctx, "step")
+    if sync:
+        particle_system.parallel.synchronize()

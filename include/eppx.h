@@ -1,0 +1,227 @@
+/*
+ * eppx.h -- C ABI of libeppx, the MI355X (gfx950) HIP implementation of
+ * EtherParallelParticles.jl's SPH time step: cell-list neighbour search,
+ * the cubic-spline / Wendland C2 kernels, continuity, momentum (pressure +
+ * classic or artificial viscosity), Tait / linear EOS and the integrator.
+ *
+ * Plain C: opaque handles, POD structs, pointers and sizes; every function
+ * returns an int32_t status (EPPX_OK == 0).  Each entry point names the
+ * reference function it replaces (paths relative to the reference package).
+ *
+ * Data crosses the boundary only in the reference's own host layout:
+ * column-major (n_capacity, n_columns) Int32 / Float32 property tables plus
+ * is_alive and cell_index vectors (src/Class/ParticleSystem/ParticleSystemBase.jl:12-18).
+ * The library re-lays them out as cell-sorted structure-of-arrays buffers in
+ * HBM and never keeps a caller's pointer after a call returns.
+ *
+ * Calls on one context are issued in order on the context's HIP stream and
+ * return without waiting, except *_download, eppx_sync, eppx_pair_count and
+ * eppx_export_pairs, which wait.  Device-side conditions the reference leaves
+ * undefined (per-cell or per-particle neighbour capacity exceeded) raise a
+ * sticky flag that the next waiting call reports as EPPX_E_CAPACITY.
+ * One host thread at a time per context.
+ */
+#ifndef EPPX_H
+#define EPPX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EPPX_ABI_VERSION 1
+
+/* ---------------------------------------------------------------- status */
+enum {
+    EPPX_OK = 0,
+    EPPX_E_ARG = 1,       /* invalid argument / layout / parameter block      */
+    EPPX_E_HIP = 2,       /* HIP runtime error (message in eppx_last_error)    */
+    EPPX_E_CAPACITY = 3,  /* cell or neighbour count above max_neighbour_number */
+    EPPX_E_NONFINITE = 4, /* non-finite density found by eppx_check_finite     */
+    EPPX_E_STATE = 5,     /* call order violated (e.g. interaction before search) */
+    EPPX_E_NOMEM = 6      /* device or host allocation failed                  */
+};
+
+typedef struct eppx_ctx eppx_ctx; /* device + stream + last error               */
+typedef struct eppx_ps eppx_ps;   /* particle system (Class.ParticleSystem)     */
+typedef struct eppx_ns eppx_ns;   /* neighbour system (Class.NeighbourSystem)   */
+
+/* Field for field order the reference's struct Domain2D
+ * (src/Class/Domain/Domain2D.jl:10-26). */
+typedef struct {
+    float gap, gap_square;
+    int32_t n_x, n_y, n;
+    float first_x, last_x, first_y, last_y;
+    float span_x, span_y, gap_x, gap_y, gap_x_inv, gap_y_inv;
+} eppx_domain2d;
+
+/* Column map of a particle system: the 0-based start column of each named
+ * field in the int / float tables, -1 when absent.  It is the reference's
+ * NamedIndex offsets minus one (src/Class/ParticleSystem/NamedIndex.jl:50-91)
+ * after basic_index_map_dict renaming (BasicIndex.jl:20-44). */
+typedef struct {
+    int32_t n_int, n_float, max_neighbour_number;
+    int32_t Tag, IsMovable, nCount, nIndex;                   /* int columns   */
+    int32_t PositionVec, VelocityVec, dVelocityVec, Mass, Volume, Density, dDensity, Pressure, H, Gap;
+    int32_t SumWeight, SumWeightedDensity;                    /* float columns */
+    int32_t nRVec, nR, nW, nDW, nHInv;                        /* neighbour cols */
+} eppx_layout;
+
+/* Search criteria (src/Algorithm/NeighbourSearch.jl:141-164). */
+enum { EPPX_CRIT_DEFAULT = 0, EPPX_CRIT_SYMMETRY = 1 };
+
+/* Smoothing kernels (src/SPH/Kernel/CubicSpline.jl, WendlandC2.jl), 2D. */
+enum { EPPX_KERNEL_CUBIC = 0, EPPX_KERNEL_WENDLAND_C2 = 1 };
+enum { EPPX_EOS_LINEAR = 0, EPPX_EOS_TAIT = 1 };
+enum { EPPX_VISC_CLASSIC = 0, EPPX_VISC_ARTIFICIAL = 1 };
+
+/* The reference's actions are Julia closures inlined into its kernels; they
+ * cannot cross a C ABI, so the dam-break driver's composite actions
+ * (example/collapse_dry/same/main.jl:74-144) and two generic field actions
+ * are exposed as IDs with a parameter block. */
+enum { /* search actions (fused after each particle's neighbour list) */
+    EPPX_SEARCH_NONE = 0,               /* defaultSelfaction!, Action.jl:27-35            */
+    EPPX_SEARCH_KERNEL_CONTINUITY = 1   /* sComputeKernelAndiContinuity!, main.jl:74-90    */
+};
+enum { /* self actions */
+    EPPX_SELF_CONTINUITY_EOS = 1,       /* sContinuity!, main.jl:92-105                     */
+    EPPX_SELF_GRAVITY_MOVE = 2,         /* sMomentum!, main.jl:120-126                      */
+    EPPX_SELF_FILTER = 3,               /* sFilter, main.jl:136-144                         */
+    EPPX_SELF_SCALE_FIELD = 10          /* field[I] *= value                                */
+};
+enum { /* interactions */
+    EPPX_INTER_MOMENTUM = 1,            /* iMomentum!, main.jl:107-118                      */
+    EPPX_INTER_FILTER = 2,              /* iFilter, main.jl:128-134                         */
+    EPPX_INTER_ADD_PER_NEIGHBOUR = 10   /* field[I] += value once per neighbour             */
+};
+
+/* Parameter block of every physics action (fields an action does not use
+ * are ignored).  Tags are the driver's FLUID_TAG / WALL_TAG. */
+typedef struct {
+    int32_t fluid_tag, wall_tag, kernel, eos, visc;
+    float dt;           /* time step                                   */
+    float rho0, p0;     /* reference density, background pressure      */
+    float c0;           /* linear EOS sound speed: p = c0^2 (rho - rho0) + p0 */
+    float tait_b;       /* Tait: p = tait_b ((rho/rho0)^7 - 1) + p0     */
+    float gx, gy;       /* gravity                                     */
+    float mu;           /* classic viscosity                           */
+    float alpha, beta;  /* artificial viscosity                        */
+    float av_c;         /* artificial-viscosity sound speed            */
+    float coefficient;  /* pressure correction coefficient (0 in the driver) */
+} eppx_physics;
+
+/* Fields addressable by the generic field actions. */
+enum {
+    EPPX_FIELD_POSITION = 0, EPPX_FIELD_VELOCITY = 1, EPPX_FIELD_DVELOCITY = 2, EPPX_FIELD_MASS = 3,
+    EPPX_FIELD_VOLUME = 4, EPPX_FIELD_DENSITY = 5, EPPX_FIELD_DDENSITY = 6, EPPX_FIELD_PRESSURE = 7
+};
+typedef struct {
+    int32_t field, component; /* component 0/1 for vector fields, 0 for scalars */
+    float value;
+} eppx_field_op;
+
+/* Arithmetic mode of the pair kernels.  EXACT evaluates every pair term
+ * with the reference's operation order, correctly rounded divide / sqrt and
+ * no FMA; FAST uses FMA and hardware reciprocal / sqrt in the physics terms
+ * (the accept / reject decision of the criterion stays exact in both). */
+enum { EPPX_MATH_EXACT = 0, EPPX_MATH_FAST = 1 };
+
+/* ------------------------------------------------------------ context */
+int32_t eppx_abi_version(void);
+/* Replaces KernelAbstractions.device!(Backend, Device) (src/DataIO/Config/Input.jl:17). */
+int32_t eppx_ctx_create(int32_t device, eppx_ctx **out);
+int32_t eppx_ctx_destroy(eppx_ctx *ctx);
+const char *eppx_last_error(const eppx_ctx *ctx);
+/* Environment.synchronize / KernelAbstractions.synchronize
+ * (src/Environment/Parallel.jl:104-108): waits, then reports sticky flags. */
+int32_t eppx_sync(eppx_ctx *ctx);
+/* Selects EPPX_MATH_EXACT (default) or EPPX_MATH_FAST for later launches. */
+int32_t eppx_set_math_mode(eppx_ctx *ctx, int32_t mode);
+
+/* ------------------------------------------------------------- domain */
+/* Domain2D{Int32,Float32}(gap, first_x, first_y, last_x, last_y),
+ * src/Class/Domain/Domain2D.jl:28-67 (host arithmetic, no device work). */
+int32_t eppx_domain2d_init(double gap, double first_x, double first_y, double last_x, double last_y,
+                           eppx_domain2d *out);
+/* inside / indexCartesianFromPosition / indexLinearFromPosition,
+ * Domain2D.jl:110-137 (host; 1-based indices). */
+int32_t eppx_domain2d_locate(const eppx_domain2d *d, float x, float y, int32_t *inside, int32_t *i, int32_t *j,
+                             int32_t *linear);
+
+/* ---------------------------------------------------- particle system */
+/* ParticleSystemBase(parallel, n_capacity, n_int, n_float),
+ * src/Class/ParticleSystem/ParticleSystemBase.jl:52-67. */
+int32_t eppx_ps_create(eppx_ctx *ctx, int32_t n_capacity, const eppx_layout *layout, eppx_ps **out);
+int32_t eppx_ps_destroy(eppx_ps *ps);
+/* Class.asyncto!/syncto!(device, host), ParticleSystemBase.jl:20-50 and
+ * ParticleSystem.jl:275-307: host tables -> device. */
+int32_t eppx_ps_upload(eppx_ps *ps, int32_t n_particles, const int32_t *is_alive, const int32_t *cell_index,
+                       const int32_t *int_props, const float *float_props);
+/* Class.asyncto!/syncto!(host, device): device -> host tables.  When ns is
+ * given and the layout declares them, nCount / nIndex / nRVec / nR / nW /
+ * nDW / nHInv are reconstructed from the last search (nW / nDW / nHInv only
+ * where the last search action was EPPX_SEARCH_KERNEL_CONTINUITY). */
+int32_t eppx_ps_download(eppx_ps *ps, eppx_ns *ns, int32_t *n_particles, int32_t *is_alive, int32_t *cell_index,
+                         int32_t *int_props, float *float_props);
+/* Class.get_alive_n_particles, ParticleSystem.jl:154-158 (waits). */
+int32_t eppx_ps_alive_count(eppx_ps *ps, int32_t *out);
+
+/* --------------------------------------------------- neighbour system */
+/* Class.NeighbourSystem(parallel, domain, active_pair, NonePeriodicBoundaryPolicy;
+ * max_neighbour_number), src/Class/NeighbourSystem/NeighbourSystem.jl:68-86
+ * with ActivePair.jl:17-33.  adjacency is the (max_tag, max_tag) column-major
+ * Int32 matrix, max_tag <= 32. */
+int32_t eppx_ns_create(eppx_ctx *ctx, const eppx_domain2d *domain, const int32_t *adjacency, int32_t max_tag,
+                       int32_t max_neighbour_number, eppx_ns **out);
+int32_t eppx_ns_destroy(eppx_ns *ns);
+/* neighbour_cell_index_count_ / _list_ built by
+ * device_initializeNeighbourSystem!, NeighbourSystemBase.jl:28-48:
+ * count (n_cells) and list (n_cells, 9) column-major, 1-based (waits). */
+int32_t eppx_ns_cell_stencil(eppx_ns *ns, int32_t *count, int32_t *list);
+
+/* --------------------------------------------------------- algorithm */
+/* Algorithm.search! / static_search! / dynamic_search!,
+ * src/Algorithm/NeighbourSearch.jl:323-413 (clean!, insert :12-47,
+ * find :166-227, fused action).  params: eppx_physics for
+ * EPPX_SEARCH_KERNEL_CONTINUITY, may be NULL for EPPX_SEARCH_NONE. */
+int32_t eppx_search(eppx_ps *ps, eppx_ns *ns, int32_t criterion, int32_t action, const void *params,
+                    size_t params_size);
+/* Algorithm.selfaction! / static_ / dynamic_, src/Algorithm/Action.jl:39-117.
+ * params: eppx_physics, or eppx_field_op for EPPX_SELF_SCALE_FIELD. */
+int32_t eppx_selfaction(eppx_ps *ps, int32_t action, const void *params, size_t params_size);
+/* Algorithm.interaction! / static_ / dynamic_, src/Algorithm/Action.jl:121-206,
+ * over the neighbour lists of the last search on ns.  params: eppx_physics,
+ * or eppx_field_op for EPPX_INTER_ADD_PER_NEIGHBOUR. */
+int32_t eppx_interaction(eppx_ps *ps, eppx_ns *ns, int32_t action, const void *params, size_t params_size);
+/* n_steps iterations of the dam-break loop body (main.jl:166-183):
+ * sContinuity!, iMomentum!, sMomentum!, search!(symmetry criterion,
+ * sComputeKernelAndiContinuity!) and, when filter_interval > 0, the Shepard
+ * filter every filter_interval steps counted from step_offset + 1.  Replays a
+ * captured hipGraph; requires one prior eppx_search on (ps, ns). */
+int32_t eppx_step(eppx_ps *ps, eppx_ns *ns, const eppx_physics *phys, int32_t n_steps, int32_t filter_interval,
+                  int64_t step_offset);
+
+/* Σ nCount over alive particles after the last search on ns (waits). */
+int32_t eppx_pair_count(eppx_ps *ps, eppx_ns *ns, int64_t *out);
+/* Running Σ nCount over every search since the last reset (waits). */
+int32_t eppx_pair_total(eppx_ps *ps, int64_t *out, int32_t reset);
+/* Neighbour lists of the last search in the reference's list order (stencil
+ * order, then ascending particle index within a cell), as CSR over particle
+ * index 0..n_particles-1: offsets (n_particles + 1) and 0-based js
+ * (capacity entries; fails with EPPX_E_CAPACITY when too small) (waits). */
+int32_t eppx_export_pairs(eppx_ps *ps, eppx_ns *ns, int64_t *offsets, int32_t *js, int64_t capacity);
+/* Returns EPPX_E_NONFINITE when a live particle's density, pressure,
+ * position or velocity is not finite (waits). */
+int32_t eppx_check_finite(eppx_ps *ps);
+/* Average device duration in microseconds of the named kernel family over
+ * the launches timed since the last reset (HIP events on the context's
+ * stream; timing is enabled by eppx_set_timing). */
+int32_t eppx_set_timing(eppx_ctx *ctx, int32_t enable);
+int32_t eppx_kernel_time(eppx_ctx *ctx, const char *kernel, double *avg_us, int64_t *launches, int32_t reset);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,181 @@
+"""libeppx (HIP, through the C ABI) against the restatement on the dam-break
+cases.  EXACT math mode must agree bit for bit: same pair lists in the same
+order, same floats.  FAST mode must agree within the tolerances stated below.
+"""
+import numpy as np
+import pytest
+
+import parity_util as U
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def small():
+    from eppx import cases
+    return cases.small(M=40)
+
+
+def test_small_search_exact(parallel, small):
+    """Pair lists (ids in list order) and every neighbour column of the first search."""
+    ps, ns, ref = U.search_both(parallel, small)
+    host = U.download(ps)
+    assert int((host.is_alive_ == 0).sum()) == 153
+    assert np.array_equal(host.cell_index_, ref.cell_index)
+    U.assert_neighbour_columns_equal(host, ref)
+    U.assert_state_equal(host, ref)
+    off, js = U.export_pairs(ps, ns)
+    roff, rjs = U.reference_pairs_in_order(ref)
+    assert np.array_equal(off, roff) and np.array_equal(js, rjs)
+
+
+@pytest.mark.parametrize("route", ["step", "api"])
+def test_small_steps_exact(parallel, small, route):
+    """25 dam-break steps, via eppx_step and via the separate selfaction /
+    interaction / search calls of main.jl:169-179."""
+    from eppx import Algorithm, SPH
+    ps, ns, ref = U.search_both(parallel, small)
+    n = 25
+    if route == "step":
+        Algorithm.step(ps, ns, small.physics, n_steps=n)
+    else:
+        ph = small.physics
+        for _ in range(n):
+            Algorithm.selfaction(ps, SPH.sContinuity(ph), sync=False)
+            Algorithm.interaction(ps, SPH.iMomentum(ph), sync=False)
+            Algorithm.selfaction(ps, SPH.sMomentum(ph), sync=False)
+            Algorithm.search(ps, small.domain, ns, criterion=Algorithm.symmetrySearchCriterion,
+                             action=SPH.sComputeKernelAndiContinuity(ph), sync=False)
+        parallel.synchronize()
+    err, pairs = ref.run_steps(n, U.oracle_physics(small.physics))
+    assert err == 0
+    host = U.download(ps)
+    U.assert_state_equal(host, ref)
+    U.assert_neighbour_columns_equal(host, ref)
+
+
+def test_small_1000_steps(parallel, small, oracle):
+    """1000 steps (0.048 s of flow): bit-identical state; so the surge front
+    (max x of fluid) is identical, well inside 0.5 dx; mass is exact."""
+    from eppx import Algorithm
+    ps, ns, ref = U.search_both(parallel, small)
+    Algorithm.step(ps, ns, small.physics, n_steps=1000)
+    err, _ = ref.run_steps(1000, U.oracle_physics(small.physics), threads=0)
+    host = U.download(ps)
+    U.assert_state_equal(host, ref, names=("PositionVec", "VelocityVec", "Density", "Pressure"))
+    fluid = (ref.IP[: ref.n, ref.L.Tag] == 1) & (ref.is_alive[: ref.n] == 1)
+    front_gpu = host.float_properties_[: ref.n, ref.L.PositionVec][fluid].max()
+    front_ref = ref.FP[: ref.n, ref.L.PositionVec][fluid].max()
+    assert abs(front_gpu - front_ref) <= 0.5 * small.dx
+    m = ref.L.Mass
+    assert host.float_properties_[: ref.n, m][fluid].astype(np.float64).sum() == \
+        small.host.float_properties_[: ref.n, m][fluid].astype(np.float64).sum()
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_jittered_pairs_exact(parallel, seed):
+    """Random sub-dx jitter of the fluid (ties broken generically): pair lists exact."""
+    from eppx import cases
+    case = cases.dam_break(0.01, 2.0, M=64)
+    rng = np.random.default_rng(seed)
+    h = case.host
+    fluid = h.int_properties_[:, 0] == 1
+    pos = h.float_properties_[:, 0:2]
+    pos[fluid] += (rng.uniform(-0.45, 0.45, (fluid.sum(), 2)) * case.dx).astype(np.float32)
+    ps, ns, ref = U.search_both(parallel, case)
+    off, js = U.export_pairs(ps, ns)
+    roff, rjs = U.reference_pairs_in_order(ref)
+    assert np.array_equal(off, roff) and np.array_equal(js, rjs)
+    U.assert_state_equal(U.download(ps), ref, names=("dDensity",))
+
+
+def test_fast_mode_within_tolerance(parallel, small):
+    """FAST pair arithmetic: one step against the exact restatement.
+    density: |d rho| <= 1e-5 |rho| per particle;
+    acceleration: ||du_fast - du_ref||_inf <= 1e-5 ||du_ref||_inf (normwise,
+    because du sums terms that cancel to ~g) after the momentum pass."""
+    from eppx import Algorithm, Class, SPH
+    parallel.set_math("fast")
+    try:
+        ps, ns, ref = U.search_both(parallel, small)
+        ph = small.physics
+        Algorithm.selfaction(ps, SPH.sContinuity(ph))
+        Algorithm.interaction(ps, SPH.iMomentum(ph))
+        host = U.download(ps)
+    finally:
+        parallel.set_math("exact")
+    oph = U.oracle_physics(ph)
+    ref.selfaction(1, oph)
+    ref.interaction(1, oph)
+    alive = ref.is_alive[: ref.n] == 1
+    L = ref.L
+    rho_g, rho_r = host.float_properties_[alive, L.Density], ref.FP[: ref.n][alive, L.Density]
+    assert np.max(np.abs(rho_g - rho_r) / np.abs(rho_r)) <= 1e-5
+    du_g = host.float_properties_[alive, L.dVelocityVec: L.dVelocityVec + 2].astype(np.float64)
+    du_r = ref.FP[: ref.n][alive, L.dVelocityVec: L.dVelocityVec + 2].astype(np.float64)
+    assert np.max(np.abs(du_g - du_r)) <= 1e-5 * np.max(np.abs(du_r)) + 1e-30
+
+
+def test_large_first_search_exact(parallel):
+    """2,030,018 particles (dx = 0.001): every pair list and drho identical."""
+    from eppx import cases
+    case = cases.large(M=64)
+    ps, ns, ref = U.search_both(parallel, case)
+    off, js = U.export_pairs(ps, ns)
+    roff, rjs = U.reference_pairs_in_order(ref)
+    assert off[-1] == roff[-1]
+    assert np.array_equal(off, roff) and np.array_equal(js, rjs)
+    U.assert_state_equal(U.download(ps), ref, names=("dDensity",))
+
+
+# ------------------------------------------------------------- error paths
+
+def test_capacity_overflow_reported(parallel, small):
+    """max_neighbour_number below the real neighbour count: the reference writes
+    out of bounds; libeppx reports EPPX_E_CAPACITY."""
+    from eppx import Algorithm, EppxError, SPH
+    ps, ns = U.device(parallel, small, M=8)
+    with pytest.raises(EppxError) as e:
+        Algorithm.search(ps, small.domain, ns, criterion=Algorithm.symmetrySearchCriterion,
+                         action=SPH.sComputeKernelAndiContinuity(small.physics))
+    assert e.value.status == 3
+    parallel.synchronize()  # the flag is cleared once reported
+
+
+def test_outside_particle_dies_and_stays_dead(parallel, small):
+    from eppx import Algorithm, SPH
+    case = small
+    h = case.host
+    ps, ns = U.device(parallel, case)
+    host = U.download(ps)
+    host.float_properties_[7, 0] = 100.0  # far outside the domain
+    from eppx import Class
+    Class.syncto(ps, host)
+    Algorithm.search(ps, case.domain, ns, criterion=Algorithm.symmetrySearchCriterion,
+                     action=SPH.sComputeKernelAndiContinuity(case.physics))
+    Algorithm.step(ps, ns, case.physics, n_steps=3)
+    out = U.download(ps)
+    assert out.is_alive_[7] == 0 and out.cell_index_[7] == 0
+    assert int((out.is_alive_ == 0).sum()) == 154
+    assert ps.get_alive_n_particles() == h.n_particles_ - 154
+
+
+def test_interaction_before_search_is_rejected(parallel, small):
+    from eppx import Algorithm, EppxError, SPH
+    ps, ns = U.device(parallel, small)
+    ps.last_ns = ns
+    with pytest.raises(EppxError) as e:
+        Algorithm.interaction(ps, SPH.iMomentum(small.physics))
+    assert e.value.status == 5
+
+
+def test_zero_particles(parallel, small):
+    from eppx import Algorithm, Class, SPH
+    ps = Class.ParticleSystem(parallel, small.domain, 0, small.int_nt, small.float_nt, small.basic_parameters,
+                              n_capacity=16)
+    ns = Class.NeighbourSystem(parallel, small.domain, [(1, 1), (1, 2), (2, 1)], max_neighbour_number=40)
+    Algorithm.search(ps, small.domain, ns, criterion=Algorithm.symmetrySearchCriterion,
+                     action=SPH.sComputeKernelAndiContinuity(small.physics))
+    Algorithm.step(ps, ns, small.physics, n_steps=2)
+    off, js = U.export_pairs(ps, ns)
+    assert off.tolist() == [0] and js.size == 0
