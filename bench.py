@@ -1,0 +1,195 @@
+"""Benchmark: pair-interactions/s of the 2D dam break on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload large|small] [--math exact|fast]
+
+Workload "large" (default): 1.0 x 2.0 m water column in a 4 x 3 m tank,
+dx = 0.001 (2,000,000 fluid + 30,018 wall particles), Wendland C2, h = 2dx,
+Tait EOS, artificial viscosity, Float32; one step = the dam-break loop body
+(continuity + EOS, momentum, gravity + move, cell-list build + neighbour
+search + kernel + continuity) on state resident in HBM.
+
+Prints one JSON line.  value = sum over the timed steps of the accepted
+directed pairs of each step's search (the reference's sum of nCount) / wall
+time of those steps.  roofline: the kernel with the largest share of device
+time, its algorithmic bytes per launch / its average HIP-event duration.
+cpu_baseline: the C restatement of the reference algorithm (reference layout,
+OpenMP over the host's cores) on a bounded sample of the same workload.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "etherparallelparticles.jl_amd"))
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per particle of each kernel family: every per-particle
+# array the kernel needs, read once and written once (neighbour re-reads and
+# the cell-order move are implementation cost and are not counted), plus the
+# cell-start array read once per cell.
+ALG_BYTES = {
+    # pos_nb 8, vel 8, mass 4, vol 4, rho 4, drho 4 (rmw), tag 4, h 4, cell 4 in; drho 4, count 4 out
+    "search": (52, 4),
+    # pos_nb 8, vel 8, rho 4, p 4, mass 4, tag 4, h 4, cell 4, dvel 8 in; dvel 8 out
+    "momentum": (56, 4),
+    # alive 4, tag 4, rho 4, drho 4, mass 4 in; rho 4, drho 4, vol 4, p 4 out
+    "continuity_eos": (36, 0),
+    # alive 4, tag 4, dvel 8, pos 8, vel 8 in; dvel 8, pos 8, vel 8 out
+    "gravity_move": (56, 0),
+}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def kernel_times(par, names):
+    from eppx import _lib
+    L = _lib.lib()
+    out = {}
+    for name in names:
+        avg, cnt = C.c_double(), C.c_int64()
+        _lib.check(L.eppx_kernel_time(par.ctx, name.encode(), C.byref(avg), C.byref(cnt), 0), par.ctx, "time")
+        if cnt.value:
+            out[name] = (avg.value, cnt.value)
+    return out
+
+
+def cpu_baseline(case, steps, threads):
+    """Time the reference-algorithm restatement (oracle/) on `steps` steps after one warm-up step."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle as O
+    from eppx import cases
+
+    full = cases.dam_break(case.dx, case.h_factor, case.kernel, M=64)  # reference layout with M = 64 lists
+    h = full.host
+    ref = O.System(h.int_properties_, h.float_properties_, h.is_alive_, h.cell_index_, h.n_particles_,
+                   O.make_layout(full.int_nt, full.float_nt, h.n_capacity), O.domain2d(*full.domain.args),
+                   np.array([[1, 1], [1, 0]]), 64)
+    ph = O.physics(**full.physics.as_dict())
+    ref.search(O.CRIT_SYMMETRY, 1, ph, serial=False, threads=threads)
+    ref.run_steps(1, ph, serial=False, threads=threads)
+    t0 = time.perf_counter()
+    err, pairs = ref.run_steps(steps, ph, serial=False, threads=threads)
+    dt = time.perf_counter() - t0
+    return float(pairs.sum()) / dt, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", choices=["large", "small"], default="large")
+    ap.add_argument("--math", choices=["exact", "fast"], default="fast")
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "libeppx runs one GPU per process; multi-GPU runs are not part of this "
+                                   "build (see DESIGN.md: how the path would shard)", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import eppx
+    from eppx import Algorithm, Class, SPH, _lib, cases
+
+    if not os.path.exists(eppx.library_path()):
+        eppx.build()
+    t_setup = time.perf_counter()
+    if args.workload == "large":
+        case = cases.dam_break(0.001, 2.0, SPH.WendlandC2, M=1)   # slim host tables; device never stores lists
+        M = 64
+    else:
+        case = cases.dam_break(0.02, 1.5, SPH.CubicSpline, M=1)
+        M = 40
+    log(f"case: {case.n_particles} particles, {case.domain.get_n()} cells ({time.perf_counter() - t_setup:.1f}s)")
+    par = eppx.Environment.Parallel(device=0, math=args.math)
+    ps = Class.ParticleSystem(par, case.domain, case.n_particles, case.int_nt, case.float_nt, case.basic_parameters)
+    Class.syncto(ps, case.host)
+    ns = Class.NeighbourSystem(par, case.domain, cases.ACTIVE_PAIR, max_neighbour_number=M)
+    Algorithm.search(ps, case.domain, ns, criterion=Algorithm.symmetrySearchCriterion,
+                     action=SPH.sComputeKernelAndiContinuity(case.physics))
+    L = _lib.lib()
+    Algorithm.step(ps, ns, case.physics, n_steps=args.warmup)
+    log(f"warmup {args.warmup} steps done")
+    total = C.c_int64()
+    _lib.check(L.eppx_pair_total(ps.handle, C.byref(total), 1), par.ctx, "pair_total")
+    _lib.check(L.eppx_set_timing(par.ctx, 1), par.ctx, "timing")
+    L.eppx_kernel_time(par.ctx, b"search", None, None, 1)  # clear
+
+    par.synchronize()
+    t0 = time.perf_counter()
+    Algorithm.step(ps, ns, case.physics, n_steps=args.steps, step_offset=args.warmup, sync=False)
+    par.synchronize()
+    elapsed = time.perf_counter() - t0
+
+    _lib.check(L.eppx_pair_total(ps.handle, C.byref(total), 1), par.ctx, "pair_total")
+    _lib.check(L.eppx_check_finite(ps.handle), par.ctx, "finite")
+    pairs = total.value
+    names = ["search", "momentum", "continuity_eos", "gravity_move", "cell_reset", "cell_key", "cell_scan",
+             "cell_members", "cell_permute"]
+    kt = kernel_times(par, names)
+    log("kernel averages (us): " + ", ".join(f"{k}={v[0]:.1f}x{v[1]}" for k, v in kt.items()))
+    dominant = max((k for k in kt if k in ALG_BYTES), key=lambda k: kt[k][0] * kt[k][1])
+    per_p, per_cell = ALG_BYTES[dominant]
+    alg_bytes = per_p * case.n_particles + per_cell * case.domain.get_n()
+    avg_us = kt[dominant][0]
+    achieved = alg_bytes / (avg_us * 1e-6) / 1e9
+    device_us_per_step = sum(v[0] * v[1] for v in kt.values()) / args.steps
+    step_bytes = sum(ALG_BYTES[k][0] * case.n_particles + ALG_BYTES[k][1] * case.domain.get_n() for k in ALG_BYTES)
+
+    traffic = None
+    tpath = os.path.join(REPO, "profiles", f"traffic_{dominant}_{args.workload}.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu:
+        threads = len(os.sched_getaffinity(0))
+        threads = min(threads, int(os.environ.get("OMP_NUM_THREADS", threads)))
+        rate, secs = cpu_baseline(case, args.cpu_steps, threads)
+        cpu = {"value": rate, "unit": "pair-interactions/s", "cores": threads, "kind": "port",
+               "sample": f"{args.cpu_steps} steps of the same {case.n_particles}-particle dam break after one "
+                         f"warm-up step, C/OpenMP restatement of the reference algorithm on the reference layout "
+                         f"(per-cell atomic insert, materialised M=64 neighbour lists), {secs:.1f} s"}
+
+    ms = 1000.0 * elapsed / args.steps
+    out = {
+        "metric": "pair-interactions/s at 2M-particle 2D dam break",
+        "value": pairs / elapsed,
+        "unit": "pair-interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "steps_per_s": 1000.0 / ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"2D dam break {args.workload}: {case.n_particles} particles (dx={case.dx}, "
+                               f"{'Wendland C2' if args.workload == 'large' else 'cubic spline'}, "
+                               f"h={case.h_factor}dx, Tait EOS, artificial viscosity)",
+                   "particles": case.n_particles, "cells": case.domain.get_n(), "math": args.math,
+                   "pairs_per_step": pairs / args.steps},
+        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "alg_bytes_per_launch": alg_bytes, "avg_launch_us": avg_us},
+        "step_roofline": {"alg_bytes_per_step": step_bytes, "device_us_per_step": device_us_per_step,
+                          "hbm_frac": step_bytes / (device_us_per_step * 1e-6) / 1e9 / HBM_PEAK_GBS},
+        "kernels_us": {k: round(v[0], 2) for k, v in kt.items()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    par.close()
+
+
+if __name__ == "__main__":
+    main()

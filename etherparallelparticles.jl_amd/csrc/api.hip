@@ -33,6 +33,8 @@ struct eppx_ctx {
     std::vector<hipEvent_t> event_pool;
     std::map<std::string, std::pair<double, int64_t>> timed;  // name -> (sum us, launches)
     uint32_t *d_flags = nullptr;
+    int32_t children = 0;   // live particle / neighbour systems
+    bool closing = false;   // destroy requested while children were alive
 };
 
 struct eppx_ns {
@@ -74,6 +76,21 @@ namespace {
 int32_t set_error(eppx_ctx *ctx, int32_t code, const std::string &msg) {
     if (ctx) ctx->error = msg;
     return code;
+}
+
+// Frees a context once it was asked to close and no system refers to it.
+void release_ctx(eppx_ctx *ctx) {
+    if (!ctx->closing || ctx->children > 0) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto &t : ctx->pending) {
+        (void)hipEventDestroy(t.start);
+        (void)hipEventDestroy(t.stop);
+    }
+    for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
+    if (ctx->d_flags) (void)hipFree(ctx->d_flags);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
 }
 
 int32_t hip_fail(eppx_ctx *ctx, hipError_t e, const char *what) {
@@ -355,18 +372,12 @@ int32_t eppx_ctx_create(int32_t device, eppx_ctx **out) {
     return EPPX_OK;
 }
 
+// A context with live particle / neighbour systems is freed when the last
+// of them is destroyed, so handles may be released in any order.
 int32_t eppx_ctx_destroy(eppx_ctx *ctx) {
-    if (!ctx) return EPPX_E_ARG;
-    (void)hipSetDevice(ctx->device);
-    (void)hipStreamSynchronize(ctx->stream);
-    for (auto &t : ctx->pending) {
-        (void)hipEventDestroy(t.start);
-        (void)hipEventDestroy(t.stop);
-    }
-    for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
-    dfree(ctx->d_flags);
-    (void)hipStreamDestroy(ctx->stream);
-    delete ctx;
+    if (!ctx || ctx->closing) return EPPX_E_ARG;
+    ctx->closing = true;
+    release_ctx(ctx);
     return EPPX_OK;
 }
 
@@ -467,8 +478,10 @@ int32_t eppx_ps_create(eppx_ctx *ctx, int32_t n_capacity, const eppx_layout *lay
     if (!has(L.PositionVec) || !has(L.Tag))
         return set_error(ctx, EPPX_E_ARG, "eppx_ps_create: PositionVec and Tag are required");
     HIP_OK(ctx, hipSetDevice(ctx->device));
+    if (ctx->closing) return set_error(ctx, EPPX_E_STATE, "eppx_ps_create: context is closed");
     eppx_ps *ps = new eppx_ps();
     ps->ctx = ctx;
+    ctx->children += 1;
     ps->cap = n_capacity;
     ps->layout = L;
     hipError_t e = alloc_fields(ps->buf[0], n_capacity);
@@ -505,7 +518,10 @@ int32_t eppx_ps_destroy(eppx_ps *ps) {
     dfree(ps->counts); dfree(ps->start); dfree(ps->tile_sums);
     dfree(ps->key); dfree(ps->tmpslot); dfree(ps->members); dfree(ps->ncount); dfree(ps->n_alive);
     dfree(ps->pairs); dfree(ps->h_range);
+    eppx_ctx *ctx = ps->ctx;
     delete ps;
+    ctx->children -= 1;
+    release_ctx(ctx);
     return EPPX_OK;
 }
 
@@ -610,7 +626,7 @@ int32_t eppx_ps_download(eppx_ps *ps, eppx_ns *ns, int32_t *n_particles, int32_t
         ex.with_kernel = ps->last_action == EPPX_SEARCH_KERNEL_CONTINUITY;
         ex.fluid_tag = ps->last_fluid;
         ex.wall_tag = ps->last_wall;
-        e = launch_export(make_walk(ps, ns, ps->last_crit), ps->f(), ps->n, nullptr, nullptr, ex, s);
+        e = launch_export(make_walk(ps, ns, ps->last_crit), ps->f(), ps->n, nullptr, nullptr, ps->ncount, ex, s);
     }
     if (e == hipSuccess && L.n_int)
         e = hipMemcpyAsync(int_props, d_ip, cap * L.n_int * sizeof(int32_t), hipMemcpyDeviceToHost, s);
@@ -654,8 +670,10 @@ int32_t eppx_ns_create(eppx_ctx *ctx, const eppx_domain2d *domain, const int32_t
         return set_error(ctx, EPPX_E_ARG, "eppx_ns_create: bad argument (1 <= max_tag <= 32)");
     if (domain->n_x < 1 || domain->n_y < 1 || (int64_t)domain->n_x * domain->n_y != domain->n)
         return set_error(ctx, EPPX_E_ARG, "eppx_ns_create: inconsistent domain");
+    if (ctx->closing) return set_error(ctx, EPPX_E_STATE, "eppx_ns_create: context is closed");
     eppx_ns *ns = new eppx_ns();
     ns->ctx = ctx;
+    ctx->children += 1;
     ns->domain = *domain;
     const eppx_domain2d &d = *domain;
     ns->grid = Grid{d.first_x, d.first_y, d.last_x, d.last_y, d.gap_x_inv, d.gap_y_inv, d.gap_square,
@@ -671,7 +689,10 @@ int32_t eppx_ns_create(eppx_ctx *ctx, const eppx_domain2d *domain, const int32_t
 
 int32_t eppx_ns_destroy(eppx_ns *ns) {
     if (!ns) return EPPX_E_ARG;
+    eppx_ctx *ctx = ns->ctx;
     delete ns;
+    ctx->children -= 1;
+    release_ctx(ctx);
     return EPPX_OK;
 }
 
@@ -894,7 +915,8 @@ int32_t eppx_export_pairs(eppx_ps *ps, eppx_ns *ns, int64_t *offsets, int32_t *j
     NeighbourExport ex{};
     ex.n_capacity = ps->cap;
     hipError_t e = hipMemcpyAsync(d_rs, row_start.data(), na * sizeof(int64_t), hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) e = launch_export(make_walk(ps, ns, ps->last_crit), ps->f(), ps->n, d_rs, d_js, ex, s);
+    if (e == hipSuccess)
+        e = launch_export(make_walk(ps, ns, ps->last_crit), ps->f(), ps->n, d_rs, d_js, ps->ncount, ex, s);
     if (e == hipSuccess) e = hipMemcpyAsync(js, d_js, (size_t)total * sizeof(int32_t), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     dfree(d_rs);

@@ -98,7 +98,7 @@ hipError_t launch_self(Fields f, int32_t n, int32_t action, const Physics &ph, c
 hipError_t launch_interaction(const Walk &w, Fields f, int32_t n, int32_t action, const Physics &ph,
                               const eppx_field_op &op, int32_t fast, hipStream_t s);
 hipError_t launch_export(const Walk &w, Fields f, int32_t n, const int64_t *row_start, int32_t *js,
-                         NeighbourExport ex, hipStream_t s);
+                         const int32_t *ncount, NeighbourExport ex, hipStream_t s);
 hipError_t launch_from_ref(Fields f, int32_t cap, Columns c, const int32_t *ip, const float *fp,
                            const int32_t *is_alive, const int32_t *cell_index, hipStream_t s);
 hipError_t launch_to_ref(Fields f, int32_t cap, Columns c, int32_t *ip, float *fp, int32_t *is_alive,

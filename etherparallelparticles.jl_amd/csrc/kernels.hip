@@ -180,7 +180,9 @@ __global__ __launch_bounds__(kBlock) void k_key_count(Fields f, int32_t n, Grid 
         k = g.n_cells;
         if (f.alive[s] == 1) {
             int32_t c = f.cell[s];
-            if (!(f.mov[s] == 0 && c != 0)) {
+            // a cached cell outside this grid (caller data, or another domain) is recomputed
+            if (c < 0 || c > g.n_cells) c = -1;
+            if (!(f.mov[s] == 0 && c > 0)) {
                 const float2 x = f.pos[s];
                 if (g.first_x <= x.x && x.x <= g.last_x && g.first_y <= x.y && x.y <= g.last_y) {
                     int32_t i = (int32_t)floorf((x.x - g.first_x) * g.gap_x_inv) + 1;
@@ -602,7 +604,7 @@ __global__ __launch_bounds__(kBlock) void k_self(Fields f, int32_t n, Physics ph
 // (nCount, nIndex, nRVec, nR, nW, nDW, nHInv; NeighbourSearch.jl:215-220,
 // Library/Kernel.jl:123-126).
 __global__ __launch_bounds__(kBlock) void k_export(Walk w, Fields f, const int64_t *row_start, int32_t *js,
-                                                    NeighbourExport ex) {
+                                                    const int32_t *ncount, NeighbourExport ex) {
     const int32_t fluid_tag = ex.fluid_tag, wall_tag = ex.wall_tag;
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= *w.n_alive) return;
@@ -610,10 +612,11 @@ __global__ __launch_bounds__(kBlock) void k_export(Walk w, Fields f, const int64
     const int ti = w.tag[i];
     const float hi = w.h[i];
     const size_t cap = (size_t)ex.n_capacity;
-    int64_t o = row_start ? row_start[i] : 0;
+    const int64_t o = row_start ? row_start[i] : 0;
+    const int limit = js ? ncount[i] : 0;  // the row length the search counted
     int k = 0;
     walk(w, i, w.pos_nb[i], ti, hi, [&](int j, int tj, float dx, float dy, float dr2) {
-        if (js) js[o + k] = f.id[j];
+        if (js && k < limit) js[o + k] = f.id[j];
         if (k < ex.M) {
             if (ex.nIndex) ex.nIndex[row + cap * k] = f.id[j] + 1;
             if (ex.nRVec) {
@@ -844,8 +847,8 @@ hipError_t launch_interaction(const Walk &w, Fields f, int32_t n, int32_t action
 }
 
 hipError_t launch_export(const Walk &w, Fields f, int32_t n, const int64_t *row_start, int32_t *js,
-                         NeighbourExport ex, hipStream_t s) {
-    hipLaunchKernelGGL(k_export, grid_for(n), dim3(kBlock), 0, s, w, f, row_start, js, ex);
+                         const int32_t *ncount, NeighbourExport ex, hipStream_t s) {
+    hipLaunchKernelGGL(k_export, grid_for(n), dim3(kBlock), 0, s, w, f, row_start, js, ncount, ex);
     return hipGetLastError();
 }
 

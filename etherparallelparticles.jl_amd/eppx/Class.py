@@ -267,16 +267,20 @@ class ParticleSystem(_ParticleSystemBase):
         _lib.check(_lib.lib().eppx_ps_create(parallel.ctx, self.n_capacity, C.byref(self.layout), C.byref(h)),
                    parallel.ctx, "ParticleSystem")
         self.handle = h
+        _lib.track(self)
         # ParticleSystem.jl:74-77: the first n_particles rows start alive
         host = mirror(self)
         host.is_alive_[: self.n_particles_] = 1
         syncto(self, host)
 
+    def close(self):
+        if getattr(self, "handle", None) is not None and not _lib.shutting_down():
+            _lib.lib().eppx_ps_destroy(self.handle)
+        self.handle = None
+
     def __del__(self):
         try:
-            if getattr(self, "handle", None) is not None:
-                _lib.lib().eppx_ps_destroy(self.handle)
-                self.handle = None
+            self.close()
         except Exception:
             pass
 
@@ -377,12 +381,16 @@ class NeighbourSystem:
                                              adj.shape[0], self.max_neighbour_number, C.byref(h)),
                    parallel.ctx, "NeighbourSystem")
         self.handle = h
+        _lib.track(self)
+
+    def close(self):
+        if getattr(self, "handle", None) is not None and not _lib.shutting_down():
+            _lib.lib().eppx_ns_destroy(self.handle)
+        self.handle = None
 
     def __del__(self):
         try:
-            if getattr(self, "handle", None) is not None:
-                _lib.lib().eppx_ns_destroy(self.handle)
-                self.handle = None
+            self.close()
         except Exception:
             pass
 

@@ -16,6 +16,8 @@ from . import _lib
 class Parallel:
     """Parallel{Int32, Float32, HIP, device}: type conversion and the device context."""
 
+    is_context = True
+
     def __init__(self, IT=np.int32, FT=np.float32, device: int = 0, math: str = "exact"):
         if np.dtype(IT) != np.int32 or np.dtype(FT) != np.float32:
             raise NotImplementedError("libeppx computes in Int32 / Float32 (the reference's IT / FT)")
@@ -47,6 +49,7 @@ class Parallel:
             if st != _lib.EPPX_OK:
                 raise _lib.EppxError(st, f"cannot open HIP device {self.device}")
             self._ctx = h
+            _lib.track(self)
             self.set_math(self._math)
         return self._ctx
 
@@ -63,9 +66,10 @@ class Parallel:
             _lib.check(_lib.lib().eppx_sync(self._ctx), self._ctx, "synchronize")
 
     def close(self) -> None:
-        if self._ctx is not None:
+        """Release the context (freed once its particle / neighbour systems are gone)."""
+        if self._ctx is not None and not _lib.shutting_down():
             _lib.lib().eppx_ctx_destroy(self._ctx)
-            self._ctx = None
+        self._ctx = None
 
     def __del__(self):
         try:

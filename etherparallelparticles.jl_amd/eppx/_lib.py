@@ -5,8 +5,10 @@ the C ABI.  Loading fails loudly when libeppx.so has not been built.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
+import weakref
 
 from . import _build
 
@@ -112,6 +114,32 @@ def lib() -> C.CDLL:
             fn.restype = res
         _lib = L
     return _lib
+
+
+# Live handles, closed at interpreter exit in dependency order (systems
+# before their contexts) while the HIP runtime is still up.
+_live = weakref.WeakSet()
+_shut_down = False
+
+
+def track(obj) -> None:
+    _live.add(obj)
+
+
+def shutting_down() -> bool:
+    return _shut_down
+
+
+@atexit.register
+def _close_all() -> None:
+    global _shut_down
+    objs = list(_live)
+    for o in objs:
+        if getattr(o, "is_context", False) is False:
+            o.close()
+    for o in objs:
+        o.close()
+    _shut_down = True
 
 
 def check(status: int, ctx=None, what: str = "") -> None:
