@@ -119,6 +119,8 @@ def main():
     log(f"warmup {args.warmup} steps done")
     total = C.c_int64()
     _lib.check(L.eppx_pair_total(ps.handle, C.byref(total), 1), par.ctx, "pair_total")
+    fb = C.c_int64()
+    _lib.check(L.eppx_pair_fallback_blocks(ps.handle, C.byref(fb), 1), par.ctx, "fallback")
     _lib.check(L.eppx_set_timing(par.ctx, 1), par.ctx, "timing")
     L.eppx_kernel_time(par.ctx, b"search", None, None, 1)  # clear
 
@@ -131,6 +133,9 @@ def main():
     _lib.check(L.eppx_pair_total(ps.handle, C.byref(total), 1), par.ctx, "pair_total")
     _lib.check(L.eppx_check_finite(ps.handle), par.ctx, "finite")
     pairs = total.value
+    _lib.check(L.eppx_pair_fallback_blocks(ps.handle, C.byref(fb), 1), par.ctx, "fallback")
+    blocks_per_launch = -(-case.n_particles // 128)
+    log(f"pair blocks reading HBM instead of LDS: {fb.value / (2 * args.steps) / blocks_per_launch:.4%}")
     names = ["search", "momentum", "continuity_eos", "gravity_move", "cell_reset", "cell_key", "cell_scan",
              "cell_members", "cell_permute"]
     kt = kernel_times(par, names)
@@ -178,7 +183,8 @@ def main():
                                f"{'Wendland C2' if args.workload == 'large' else 'cubic spline'}, "
                                f"h={case.h_factor}dx, Tait EOS, artificial viscosity)",
                    "particles": case.n_particles, "cells": case.domain.get_n(), "math": args.math,
-                   "pairs_per_step": pairs / args.steps},
+                   "pairs_per_step": pairs / args.steps,
+                   "untiled_block_fraction": fb.value / (2 * args.steps) / blocks_per_launch},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "alg_bytes_per_launch": alg_bytes, "avg_launch_us": avg_us},

@@ -52,12 +52,13 @@ struct eppx_ps {
     eppx_layout layout{};
     Fields buf[2]{};
     int cur = 0;
-    float2 *pos_nb = nullptr;
+    float4 *geo = nullptr;  // (x, y, h, tag) snapshot of the last search, sorted order
     // cell arrays of the last search
     int32_t n_cells = -1;
     int32_t *counts = nullptr, *start = nullptr, *tile_sums = nullptr;
     int32_t *key = nullptr, *tmpslot = nullptr, *members = nullptr, *ncount = nullptr, *n_alive = nullptr;
     unsigned long long *pairs = nullptr;  // [0] last search, [1] running total
+    uint32_t *fallback = nullptr;         // pair blocks that walked global memory
     float *h_range = nullptr;
     float rmin2 = -1.0f, rmax2 = INFINITY;
     // search state
@@ -123,15 +124,15 @@ hipError_t alloc_fields(Fields &f, int32_t cap) {
     auto a = [&](auto **p) {
         if (e == hipSuccess) e = dalloc(p, (size_t)cap);
     };
-    a(&f.pos); a(&f.vel); a(&f.dvel);
-    a(&f.mass); a(&f.vol); a(&f.rho); a(&f.drho); a(&f.p); a(&f.h); a(&f.wv); a(&f.wvrho);
+    a(&f.pos); a(&f.vel); a(&f.dvel); a(&f.rp);
+    a(&f.mass); a(&f.vol); a(&f.drho); a(&f.h); a(&f.wv); a(&f.wvrho);
     a(&f.tag); a(&f.mov); a(&f.id); a(&f.cell); a(&f.alive);
     return e;
 }
 
 void free_fields(Fields &f) {
-    dfree(f.pos); dfree(f.vel); dfree(f.dvel);
-    dfree(f.mass); dfree(f.vol); dfree(f.rho); dfree(f.drho); dfree(f.p); dfree(f.h); dfree(f.wv); dfree(f.wvrho);
+    dfree(f.pos); dfree(f.vel); dfree(f.dvel); dfree(f.rp);
+    dfree(f.mass); dfree(f.vol); dfree(f.drho); dfree(f.h); dfree(f.wv); dfree(f.wvrho);
     dfree(f.tag); dfree(f.mov); dfree(f.id); dfree(f.cell); dfree(f.alive);
 }
 
@@ -226,10 +227,9 @@ Walk make_walk(const eppx_ps *ps, const eppx_ns *ns, int32_t criterion) {
     const Fields &f = ps->buf[ps->cur];
     w.g = ns->grid;
     w.start = ps->start;
-    w.pos_nb = ps->pos_nb;
-    w.tag = f.tag;
+    w.geo = ps->geo;
     w.cell = f.cell;
-    w.h = f.h;
+    w.fallback = ps->fallback;
     w.n_alive = ps->n_alive;
     memcpy(w.adj, ns->adj, sizeof(w.adj));
     w.criterion = criterion;
@@ -303,7 +303,7 @@ int32_t do_search(eppx_ps *ps, eppx_ns *ns, int32_t criterion, int32_t action, c
                return launch_fill_members(cur, ps->n, nc, ps->key, ps->tmpslot, ps->start, ps->members, s);
            }));
     HIP_OK(ctx, timed(ctx, "cell_permute", [&] {
-               return launch_permute(cur, nxt, ps->pos_nb, ps->n, ps->cap, nc, ps->key, ps->tmpslot, ps->start,
+               return launch_permute(cur, nxt, ps->geo, ps->n, ps->cap, nc, ps->key, ps->tmpslot, ps->start,
                                      ps->members, ps->n_alive, ctx->d_flags, s);
            }));
     ps->cur ^= 1;
@@ -486,7 +486,9 @@ int32_t eppx_ps_create(eppx_ctx *ctx, int32_t n_capacity, const eppx_layout *lay
     ps->layout = L;
     hipError_t e = alloc_fields(ps->buf[0], n_capacity);
     if (e == hipSuccess) e = alloc_fields(ps->buf[1], n_capacity);
-    if (e == hipSuccess) e = dalloc(&ps->pos_nb, (size_t)n_capacity);
+    if (e == hipSuccess) e = dalloc(&ps->geo, (size_t)n_capacity);
+    if (e == hipSuccess) e = dalloc(&ps->fallback, 1);
+    if (e == hipSuccess) e = hipMemsetAsync(ps->fallback, 0, sizeof(uint32_t), ctx->stream);
     if (e == hipSuccess) e = dalloc(&ps->key, (size_t)n_capacity);
     if (e == hipSuccess) e = dalloc(&ps->tmpslot, (size_t)n_capacity);
     if (e == hipSuccess) e = dalloc(&ps->members, (size_t)n_capacity);
@@ -514,7 +516,8 @@ int32_t eppx_ps_destroy(eppx_ps *ps) {
     (void)hipStreamSynchronize(ps->ctx->stream);
     free_fields(ps->buf[0]);
     free_fields(ps->buf[1]);
-    dfree(ps->pos_nb);
+    dfree(ps->geo);
+    dfree(ps->fallback);
     dfree(ps->counts); dfree(ps->start); dfree(ps->tile_sums);
     dfree(ps->key); dfree(ps->tmpslot); dfree(ps->members); dfree(ps->ncount); dfree(ps->n_alive);
     dfree(ps->pairs); dfree(ps->h_range);
@@ -923,6 +926,18 @@ int32_t eppx_export_pairs(eppx_ps *ps, eppx_ns *ns, int64_t *offsets, int32_t *j
     dfree(d_js);
     if (e != hipSuccess) return hip_fail(ctx, e, "eppx_export_pairs");
     return read_flags(ctx);
+}
+
+int32_t eppx_pair_fallback_blocks(eppx_ps *ps, int64_t *out, int32_t reset) {
+    if (!ps || !out) return EPPX_E_ARG;
+    eppx_ctx *ctx = ps->ctx;
+    HIP_OK(ctx, hipSetDevice(ctx->device));
+    uint32_t v = 0;
+    HIP_OK(ctx, hipMemcpyAsync(&v, ps->fallback, sizeof(v), hipMemcpyDeviceToHost, ctx->stream));
+    if (reset) HIP_OK(ctx, hipMemsetAsync(ps->fallback, 0, sizeof(uint32_t), ctx->stream));
+    HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
+    *out = v;
+    return EPPX_OK;
 }
 
 int32_t eppx_check_finite(eppx_ps *ps) {

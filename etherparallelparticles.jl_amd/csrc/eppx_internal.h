@@ -32,7 +32,8 @@ struct Grid {
 // (cell, id); dead particles follow.
 struct Fields {
     float2 *pos, *vel, *dvel;
-    float *mass, *vol, *rho, *drho, *p, *h, *wv, *wvrho;
+    float2 *rp;  // (density, pressure)
+    float *mass, *vol, *drho, *h, *wv, *wvrho;
     int32_t *tag, *mov, *id, *cell, *alive;
 };
 
@@ -40,13 +41,13 @@ struct Fields {
 struct Walk {
     Grid g;
     const int32_t *start;     // (n_cells + 2) exclusive scan of cell counts; slot ranges
-    const float2 *pos_nb;     // positions at the last search (sorted order)
-    const int32_t *tag, *cell;
-    const float *h;
+    const float4 *geo;        // (x, y, h, tag bits) at the last search, sorted order
+    const int32_t *cell;
     const int32_t *n_alive;   // device scalar
     uint32_t adj[kMaxTag + 1];  // adj[t]: bit (u-1) set when tag t is adjacent to tag u
     int32_t criterion;
     float rmin2, rmax2;       // symmetry-criterion accept / reject bounds (exact-safe)
+    uint32_t *fallback;       // blocks that walked global memory instead of LDS (counter)
 };
 
 struct Physics {
@@ -87,7 +88,7 @@ hipError_t launch_key_count(Fields f, int32_t n, Grid g, int32_t *counts, int32_
 hipError_t launch_scan(const int32_t *counts, int32_t n, int32_t *start, int32_t *block_sums, hipStream_t s);
 hipError_t launch_fill_members(const Fields f, int32_t n, int32_t n_cells, const int32_t *key,
                                const int32_t *tmpslot, const int32_t *start, int32_t *members, hipStream_t s);
-hipError_t launch_permute(Fields cur, Fields nxt, float2 *pos_nb, int32_t n, int32_t cap, int32_t n_cells,
+hipError_t launch_permute(Fields cur, Fields nxt, float4 *geo, int32_t n, int32_t cap, int32_t n_cells,
                           const int32_t *key, const int32_t *tmpslot, const int32_t *start,
                           const int32_t *members, int32_t *n_alive, uint32_t *flags, hipStream_t s);
 hipError_t launch_search(const Walk &w, Fields f, int32_t n, int32_t action, const Physics &ph, int32_t fast,

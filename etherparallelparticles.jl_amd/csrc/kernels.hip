@@ -6,6 +6,14 @@
 // each expression in the reference's written order (left-to-right products,
 // no FMA).  The FAST variants contract explicitly with fmaf and use the
 // hardware reciprocal / reciprocal-sqrt; they never touch the criterion.
+//
+// Pair kernels (search + continuity, momentum) keep one thread per particle
+// and split each neighbour cell into two passes: first every candidate of
+// the cell is tested (one 16-byte load of x, y, h, tag each) into a 64-bit
+// accept mask, then the pair physics runs over the set bits in ascending
+// order.  The physics therefore executes once per accepted pair instead of
+// once per candidate under a divergent branch, and the summation order stays
+// the reference's list order.
 #include <math.h>
 
 #include "eppx_internal.h"
@@ -14,6 +22,7 @@ namespace eppx {
 namespace {
 
 constexpr int kRankLimit = 1024;  // largest cell ordered by particle index
+constexpr int kPairBlock = 256;   // threads per pair-kernel block
 
 // sigma of the 2D kernels: FT(10/7/pi) (CubicSpline.jl:14), FT(7/4/pi) (WendlandC2.jl:13)
 constexpr float kSigmaCubic = (float)(10.0 / 7.0 / 3.14159265358979323846);
@@ -22,16 +31,18 @@ constexpr float kSigmaWendland = (float)(7.0 / 4.0 / 3.14159265358979323846);
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ int tag_of(float4 g) { return __float_as_int(g.w); }
 
 // Mean.harmonic / invharmonic (src/Math/Mean.jl:28-34)
 __device__ __forceinline__ float harmonic(float a, float b) { return 2.0f * a * b / (a + b); }
 __device__ __forceinline__ float invharmonic(float a, float b) { return (a + b) / (2.0f * a * b); }
 
 // _gradient of CubicSpline.jl:41-55 / WendlandC2.jl:37-47 (2D), written order.
-__device__ __forceinline__ float grad_exact(int kernel, float r, float hinv) {
+template <int K>
+__device__ __forceinline__ float grad_exact(float r, float hinv) {
     const float q = r * hinv;
     const float h3 = hinv * hinv * hinv;
-    if (kernel == EPPX_KERNEL_CUBIC) {
+    if (K == EPPX_KERNEL_CUBIC) {
         if (q < 1.0f) return kSigmaCubic * h3 * q * (3.0f * q - 4.0f) * 0.75f;
         if (q < 2.0f) {
             const float t = 2.0f - q;
@@ -44,6 +55,21 @@ __device__ __forceinline__ float grad_exact(int kernel, float r, float hinv) {
         return -kSigmaWendland * h3 * 0.625f * q * (t * t * t);
     }
     return 0.0f;
+}
+
+// Same function contracted: dw = sigma h^3 poly(q).
+template <int K>
+__device__ __forceinline__ float grad_fast(float r, float hinv) {
+    const float q = r * hinv;
+    const float h3 = hinv * hinv * hinv;
+    if (K == EPPX_KERNEL_CUBIC) {
+        const float t = 2.0f - q;
+        const float inner = q * fmaf(3.0f, q, -4.0f);
+        const float outer = -(t * t);
+        return (kSigmaCubic * 0.75f) * h3 * (q < 1.0f ? inner : (q < 2.0f ? outer : 0.0f));
+    }
+    const float t = fmaxf(2.0f - q, 0.0f);
+    return (-kSigmaWendland * 0.625f) * h3 * q * (t * t * t);
 }
 
 // _value of CubicSpline.jl:17-31 / WendlandC2.jl:16-26 (2D), written order.
@@ -65,23 +91,9 @@ __device__ __forceinline__ float value_exact(int kernel, float r, float hinv) {
     return 0.0f;
 }
 
-// Same functions, contracted: dw = sigma h^3 * poly(q).
-__device__ __forceinline__ float grad_fast(int kernel, float r, float hinv) {
-    const float q = r * hinv;
-    const float h3 = hinv * hinv * hinv;
-    if (kernel == EPPX_KERNEL_CUBIC) {
-        if (q < 1.0f) return (kSigmaCubic * 0.75f) * h3 * q * fmaf(3.0f, q, -4.0f);
-        if (q < 2.0f) {
-            const float t = 2.0f - q;
-            return (-kSigmaCubic * 0.75f) * h3 * (t * t);
-        }
-        return 0.0f;
-    }
-    if (q < 2.0f) {
-        const float t = 2.0f - q;
-        return (-kSigmaWendland * 0.625f) * h3 * q * (t * t * t);
-    }
-    return 0.0f;
+__device__ __forceinline__ float grad_exact_rt(int kernel, float r, float hinv) {
+    return kernel == EPPX_KERNEL_CUBIC ? grad_exact<EPPX_KERNEL_CUBIC>(r, hinv)
+                                       : grad_exact<EPPX_KERNEL_WENDLAND_C2>(r, hinv);
 }
 
 // Equation of state of the driver's sContinuity!: linear c0^2 (rho - rho0) + p0
@@ -97,18 +109,37 @@ __device__ __forceinline__ float eos(const Physics &ph, float rho) {
     return ph.tait_b * (t * t * x - 1.0f) + ph.p0;
 }
 
+// The accept / reject decision of device_findNeighbourParticlesFromCells!
+// (NeighbourSearch.jl:199-214) for candidate geometry gj seen from (xi, hi):
+// dx = x_I - x_J, dr2 = dx*dx + dy*dy and the criterion.  Exact in every mode.
+__device__ __forceinline__ bool accept_pair(const Walk &w, float2 xi, float hi, float4 gj, float &dx, float &dy,
+                                            float &dr2) {
+    dx = xi.x - gj.x;
+    dy = xi.y - gj.y;
+    dr2 = dx * dx + dy * dy;
+    if (w.criterion == EPPX_CRIT_DEFAULT) return dr2 <= w.g.gap_square;  // defaultSearchCriterion
+    if (dr2 > w.rmax2) return false;  // beyond (2 max h)^2 with margin: rejected by the exact test too
+    if (dr2 < w.rmin2) return true;   // inside (2 min h)^2 with margin: accepted by the exact test too
+    const float h2 = harmonic(hi, gj.z) * 2.0f;  // symmetrySearchCriterion
+    return dr2 <= h2 * h2;
+}
+
+__device__ __forceinline__ bool adjacent(uint32_t adj_i, int tj) {
+    return tj >= 1 && tj <= kMaxTag && ((adj_i >> (tj - 1)) & 1u);
+}
+
 // ------------------------------------------------------------------ walk
-// Visits the accepted neighbours of sorted slot i in the reference's list
-// order: neighbour cells in device_initializeNeighbourSystem! order (x offset
-// outer, y offset inner; NeighbourSystemBase.jl:37-47), then ascending
-// particle index inside a cell (the order of a serial insert).  The test is
-// device_findNeighbourParticlesFromCells! (NeighbourSearch.jl:189-222):
-// I != J, adjacency, dx = x_I - x_J, dr2 = dx*dx + dy*dy, criterion.
+// Visits the accepted neighbours of sorted slot i, reading global memory, in
+// the reference's list order: neighbour cells in device_initializeNeighbourSystem!
+// order (x offset outer, y offset inner; NeighbourSystemBase.jl:37-47), then
+// ascending particle index inside a cell (the order of a serial insert).
 template <class Visit>
-__device__ __forceinline__ void walk(const Walk &w, int i, float2 xi, int ti, float hi, Visit &&visit) {
+__device__ __forceinline__ void walk(const Walk &w, int i, float4 gi, Visit &&visit) {
     const int c = w.cell[i] - 1;
     const int cx = c % w.g.n_x, cy = c / w.g.n_x;
+    const int ti = tag_of(gi);
     const uint32_t adj = (ti >= 1 && ti <= kMaxTag) ? w.adj[ti] : 0u;
+    const float2 xi = make_float2(gi.x, gi.y);
     for (int di = -1; di <= 1; ++di) {
         const int ii = cx + di;
         if (ii < 0 || ii >= w.g.n_x) continue;
@@ -119,24 +150,53 @@ __device__ __forceinline__ void walk(const Walk &w, int i, float2 xi, int ti, fl
             const int b = w.start[k], e = w.start[k + 1];
             for (int j = b; j < e; ++j) {
                 if (j == i) continue;
-                const int tj = w.tag[j];
-                if (tj < 1 || tj > kMaxTag || !((adj >> (tj - 1)) & 1u)) continue;
-                const float2 xj = w.pos_nb[j];
-                const float dx = xi.x - xj.x;
-                const float dy = xi.y - xj.y;
-                const float dr2 = dx * dx + dy * dy;
-                bool ok;
-                if (w.criterion == EPPX_CRIT_DEFAULT) {
-                    ok = dr2 <= w.g.gap_square;  // defaultSearchCriterion
-                } else if (dr2 > w.rmax2) {
-                    ok = false;  // beyond (2 max h)^2 with margin: rejected by the exact test too
-                } else if (dr2 < w.rmin2) {
-                    ok = true;   // inside (2 min h)^2 with margin: accepted by the exact test too
-                } else {
-                    const float h2 = harmonic(hi, w.h[j]) * 2.0f;  // symmetrySearchCriterion
-                    ok = dr2 <= h2 * h2;
+                const float4 gj = w.geo[j];
+                const int tj = tag_of(gj);
+                if (!adjacent(adj, tj)) continue;
+                float dx, dy, dr2;
+                if (accept_pair(w, xi, gi.z, gj, dx, dy, dr2)) visit(j, tj, dx, dy, dr2);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------- masked walk
+// Same visit order as walk(), in two passes per neighbour cell (chunks of 64
+// candidates): `count(tj)` sees every accepted neighbour, and `pair(j, tj)`
+// then runs for the accepted ones that `keep(tj)` in ascending slot order.
+template <class Keep, class Count, class Pair>
+__device__ __forceinline__ void walk_masked(const Walk &w, int i, float4 gi, Keep &&keep, Count &&count,
+                                            Pair &&pair) {
+    const int c = w.cell[i] - 1;
+    const int cx = c % w.g.n_x, cy = c / w.g.n_x;
+    const int ti = tag_of(gi);
+    const uint32_t adj = (ti >= 1 && ti <= kMaxTag) ? w.adj[ti] : 0u;
+    const float2 xi = make_float2(gi.x, gi.y);
+    for (int s9 = 0; s9 < 9; ++s9) {
+        const int ii = cx + s9 / 3 - 1, jj = cy + s9 % 3 - 1;
+        if (ii < 0 || ii >= w.g.n_x || jj < 0 || jj >= w.g.n_y) continue;
+        const int k = ii + w.g.n_x * jj;
+        const int b = w.start[k], e = w.start[k + 1];
+        for (int cb = b; cb < e; cb += 64) {
+            const int ce = min(e, cb + 64);
+            unsigned long long m = 0ull;
+#pragma unroll 4
+            for (int j = cb; j < ce; ++j) {
+                const float4 gj = w.geo[j];
+                const int tj = tag_of(gj);
+                float dx, dy, dr2;
+                const bool ok = j != i && adjacent(adj, tj) && accept_pair(w, xi, gi.z, gj, dx, dy, dr2);
+                if (ok) {
+                    count(tj);
+                    if (keep(tj)) m |= 1ull << (j - cb);
                 }
-                if (ok) visit(j, tj, dx, dy, dr2);
+            }
+            while (m) {
+                const int j = cb + __builtin_ctzll(m);
+                m &= m - 1ull;
+                const float4 gj = w.geo[j];
+                const float dx = xi.x - gj.x, dy = xi.y - gj.y;
+                pair(j, tag_of(gj), gj.z, dx, dy, dx * dx + dy * dy);
             }
         }
     }
@@ -305,8 +365,9 @@ __global__ __launch_bounds__(kBlock) void k_fill_members(Fields f, int32_t n, in
 
 // Final position of each slot: cell start + number of cell members with a
 // smaller particle index (the order a serial insert gives); dead particles
-// keep their provisional slot after the alive ones.  Moves every field.
-__global__ __launch_bounds__(kBlock) void k_permute(Fields cur, Fields nxt, float2 *pos_nb, int32_t n, int32_t cap,
+// keep their provisional slot after the alive ones.  Moves every field and
+// writes the search snapshot geo = (x, y, h, tag).
+__global__ __launch_bounds__(kBlock) void k_permute(Fields cur, Fields nxt, float4 *geo, int32_t n, int32_t cap,
                                                      int32_t n_cells, const int32_t *key, const int32_t *tmpslot,
                                                      const int32_t *start, const int32_t *members,
                                                      int32_t *n_alive, uint32_t *flags) {
@@ -336,76 +397,84 @@ __global__ __launch_bounds__(kBlock) void k_permute(Fields cur, Fields nxt, floa
         }
     }
     const float2 x = cur.pos[s];
+    const float h = cur.h[s];
+    const int32_t tag = cur.tag[s];
     nxt.pos[dest] = x;
-    pos_nb[dest] = x;
+    geo[dest] = make_float4(x.x, x.y, h, __int_as_float(tag));
     nxt.vel[dest] = cur.vel[s];
     nxt.dvel[dest] = cur.dvel[s];
+    nxt.rp[dest] = cur.rp[s];
     nxt.mass[dest] = cur.mass[s];
     nxt.vol[dest] = cur.vol[s];
-    nxt.rho[dest] = cur.rho[s];
     nxt.drho[dest] = cur.drho[s];
-    nxt.p[dest] = cur.p[s];
-    nxt.h[dest] = cur.h[s];
+    nxt.h[dest] = h;
     nxt.wv[dest] = cur.wv[s];
     nxt.wvrho[dest] = cur.wvrho[s];
-    nxt.tag[dest] = cur.tag[s];
+    nxt.tag[dest] = tag;
     nxt.mov[dest] = cur.mov[s];
     nxt.id[dest] = cur.id[s];
     nxt.cell[dest] = cur.cell[s];
     nxt.alive[dest] = alive;
 }
 
-// ------------------------------------------------------------ pair kernels
+// ------------------------------------------------------------ pair physics
+// iValueGradient! + iClassicContinuity! / iBalancedContinuity!
+// (Library/Kernel.jl:114-128, Continuity.jl:23-47): one term of drho_i;
+// mj_or_rhoVj is m_j (fluid-fluid) or rho_i * V_j (fluid-wall, wall-fluid).
+template <bool FAST, int K>
+__device__ __forceinline__ float continuity_term(float hi, float hj, float2 ui, float2 uj, float mj_or_rhoVj,
+                                                 float dx, float dy, float dr2) {
+    if (!FAST) {
+        const float r = sqrtf(dr2);
+        const float hinv = invharmonic(hi, hj);
+        const float dw = grad_exact<K>(r, hinv);
+        float vdotx = 0.0f;
+        vdotx += dx * (ui.x - uj.x);
+        vdotx += dy * (ui.y - uj.y);
+        return mj_or_rhoVj * vdotx * dw / r;
+    }
+    const float rinv = frsq(dr2);
+    const float r = dr2 * rinv;
+    const float hinv = (hi + hj) * frcp(2.0f * hi * hj);
+    const float dw = grad_fast<K>(r, hinv);
+    const float vdotx = fmaf(dy, ui.y - uj.y, dx * (ui.x - uj.x));
+    return (mj_or_rhoVj * vdotx) * (dw * rinv);
+}
+
 // Search pass: find the neighbours of every alive particle and run the fused
 // action (EPPX_SEARCH_KERNEL_CONTINUITY: sComputeKernelAndiContinuity!,
-// main.jl:74-90 -> iValueGradient! Library/Kernel.jl:114-128,
-// iClassicContinuity! / iBalancedContinuity! Continuity.jl:23-47).
-template <bool FAST, int ACTION>
-__global__ __launch_bounds__(kBlock) void k_search(Walk w, Fields f, Physics ph, int32_t M, int32_t *ncount,
-                                                    unsigned long long *pairs_last,
-                                                    unsigned long long *pairs_total, uint32_t *flags) {
-    __shared__ int lds[kBlock / 64];
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    const int na = *w.n_alive;
+// main.jl:74-90).
+template <bool FAST, int ACTION, int K>
+__global__ __launch_bounds__(kPairBlock) void k_search(Walk w, Fields f, Physics ph, int32_t M, int32_t *ncount,
+                                                        unsigned long long *pairs_last,
+                                                        unsigned long long *pairs_total, uint32_t *flags) {
+    __shared__ int lds[kPairBlock / 64];
+    const int i = blockIdx.x * kPairBlock + threadIdx.x;
+    const bool active = i < *w.n_alive;
     int cnt = 0;
-    if (i < na) {
-        const float2 xi = w.pos_nb[i];
-        const int ti = w.tag[i];
-        const float hi = w.h[i];
+    if (active) {
+        const float4 gi = w.geo[i];
+        const int ti = tag_of(gi);
+        const bool i_fluid = ti == ph.fluid_tag, i_wall = ti == ph.wall_tag;
         if (ACTION == EPPX_SEARCH_KERNEL_CONTINUITY) {
             const float2 ui = f.vel[i];
-            const float rho_i = f.rho[i];
+            const float rho_i = f.rp[i].x;
             float acc = f.drho[i];
-            const bool i_fluid = ti == ph.fluid_tag, i_wall = ti == ph.wall_tag;
-            walk(w, i, xi, ti, hi, [&](int j, int tj, float dx, float dy, float dr2) {
-                ++cnt;
-                const bool ff = i_fluid && tj == ph.fluid_tag;
-                const bool mixed = (i_fluid && tj == ph.wall_tag) || (i_wall && tj == ph.fluid_tag);
-                if (!(ff || mixed)) return;
-                const float hj = w.h[j];
-                const float2 uj = f.vel[j];
-                if (!FAST) {
-                    const float r = sqrtf(dr2);
-                    const float hinv = invharmonic(hi, hj);
-                    const float dw = grad_exact(ph.kernel, r, hinv);
-                    float vdotx = 0.0f;
-                    vdotx += dx * (ui.x - uj.x);
-                    vdotx += dy * (ui.y - uj.y);
-                    const float term = ff ? f.mass[j] * vdotx * dw / r : rho_i * f.vol[j] * vdotx * dw / r;
-                    acc += term;
-                } else {
-                    const float rinv = frsq(dr2);
-                    const float r = dr2 * rinv;
-                    const float hinv = (hi + hj) * frcp(2.0f * hi * hj);
-                    const float dw = grad_fast(ph.kernel, r, hinv);
-                    const float vdotx = fmaf(dy, ui.y - uj.y, dx * (ui.x - uj.x));
-                    const float mj = ff ? f.mass[j] : rho_i * f.vol[j];
-                    acc = fmaf(mj * vdotx, dw * rinv, acc);
-                }
-            });
+            // pair classes of the continuity action: fluid-fluid (classic), fluid-wall / wall-fluid (balanced)
+            walk_masked(
+                w, i, gi,
+                [&](int tj) {
+                    return (i_fluid && (tj == ph.fluid_tag || tj == ph.wall_tag)) || (i_wall && tj == ph.fluid_tag);
+                },
+                [&](int) { ++cnt; },
+                [&](int j, int tj, float hj, float dx, float dy, float dr2) {
+                    const bool ff = i_fluid && tj == ph.fluid_tag;
+                    const float m = ff ? f.mass[j] : rho_i * f.vol[j];
+                    acc += continuity_term<FAST, K>(gi.z, hj, ui, f.vel[j], m, dx, dy, dr2);
+                });
             f.drho[i] = acc;
         } else {
-            walk(w, i, xi, ti, hi, [&](int, int, float, float, float) { ++cnt; });
+            walk(w, i, gi, [&](int, int, float, float, float) { ++cnt; });
         }
         ncount[i] = cnt;
         if (cnt > M) atomicOr(flags, kFlagNeighbourOverflow);
@@ -417,121 +486,133 @@ __global__ __launch_bounds__(kBlock) void k_search(Walk w, Fields f, Physics ph,
     }
 }
 
-// Interaction pass over the last search's neighbour lists
-// (device_interaction!, Action.jl:121-138).
-//  EPPX_INTER_MOMENTUM: iMomentum! main.jl:107-118 -> iClassicPressure! /
-//    iBalancedPressure! (Pressure.jl:24-60), iClassicViscosity! /
-//    sArtificialViscosity! (Viscosity.jl:10-52).
+// iMomentum! (main.jl:107-118): iClassicPressure! (fluid-fluid) or
+// iBalancedPressure! (fluid-wall) (Pressure.jl:24-60), then
+// iClassicViscosity! or sArtificialViscosity! (Viscosity.jl:10-52), added to
+// the acceleration of fluid particle i.
+template <bool FAST, int K, int VISC>
+__device__ __forceinline__ void momentum_pair(const Physics &ph, float2 &acc, bool classic, float hi, float hj,
+                                              float2 ui, float rho_i, float p_i, float pr_i, float2 uj, float rho_j,
+                                              float p_j, float m_j, float dx, float dy, float dr2) {
+    const float dux = ui.x - uj.x, duy = ui.y - uj.y;
+    if (!FAST) {
+        const float r = sqrtf(dr2);
+        const float hinv = invharmonic(hi, hj);
+        const float dw = grad_exact<K>(r, hinv);
+        float prho2 = classic ? p_i / (rho_i * rho_i) + p_j / (rho_j * rho_j) : (p_i + p_j) / (rho_i * rho_j);
+        prho2 += fabsf(prho2) * ph.coefficient;
+        prho2 *= -m_j * dw / r;
+        acc.x += prho2 * dx;
+        acc.y += prho2 * dy;
+        if (VISC == EPPX_VISC_CLASSIC) {
+            const float h = 1.0f / hinv;
+            const float az = r * r + 0.01f * h * h;
+            const float force = 2.0f * ph.mu * m_j * r * dw / (rho_i * rho_j * az);
+            acc.x += force * dux;
+            acc.y += force * duy;
+        } else {
+            float vdotx = 0.0f;
+            vdotx += dx * dux;
+            vdotx += dy * duy;
+            if (vdotx > 0.0f) return;
+            const float rho = (rho_i + rho_j) * 0.5f;
+            const float h = 1.0f / hinv;
+            const float az = r * r + 0.01f * h * h;
+            const float phi = vdotx / (hinv * az);
+            float force = (-ph.alpha * ph.av_c + ph.beta * phi) * phi / rho;
+            force *= -m_j / (rho_i * rho_j * r) * dw;
+            acc.x += force * dx;
+            acc.y += force * dy;
+        }
+        return;
+    }
+    const float rinv = frsq(dr2);
+    const float r = dr2 * rinv;
+    const float hh = 2.0f * hi * hj;
+    const float hinv = (hi + hj) * frcp(hh);
+    const float dw = grad_fast<K>(r, hinv);
+    const float rirj = rho_i * rho_j;
+    float prho2 = classic ? fmaf(p_j, frcp(rho_j * rho_j), pr_i) : (p_i + p_j) * frcp(rirj);
+    prho2 = fmaf(fabsf(prho2), ph.coefficient, prho2);
+    const float mdw = -m_j * dw;
+    prho2 *= mdw * rinv;
+    acc.x = fmaf(prho2, dx, acc.x);
+    acc.y = fmaf(prho2, dy, acc.y);
+    if (VISC == EPPX_VISC_CLASSIC) {
+        const float h = hh * frcp(hi + hj);
+        const float az = fmaf(r, r, 0.01f * h * h);
+        const float force = (2.0f * ph.mu) * m_j * r * dw * frcp(rirj * az);
+        acc.x = fmaf(force, dux, acc.x);
+        acc.y = fmaf(force, duy, acc.y);
+    } else {
+        const float vdotx = fmaf(dy, duy, dx * dux);
+        if (vdotx > 0.0f) return;
+        const float h = hh * frcp(hi + hj);
+        const float az = fmaf(r, r, 0.01f * h * h);
+        const float phi = vdotx * frcp(hinv * az);
+        const float force =
+            fmaf(ph.beta, phi, -ph.alpha * ph.av_c) * phi * frcp(0.5f * (rho_i + rho_j)) * (mdw * frcp(rirj * r));
+        acc.x = fmaf(force, dx, acc.x);
+        acc.y = fmaf(force, dy, acc.y);
+    }
+}
+
+template <bool FAST, int K, int VISC>
+__global__ __launch_bounds__(kPairBlock) void k_momentum(Walk w, Fields f, Physics ph) {
+    const int i = blockIdx.x * kPairBlock + threadIdx.x;
+    if (i >= *w.n_alive) return;
+    const float4 gi = w.geo[i];
+    if (tag_of(gi) != ph.fluid_tag) return;  // iMomentum! acts on fluid particles only
+    const float2 ui = f.vel[i], rp_i = f.rp[i];
+    float2 acc = f.dvel[i];
+    const float pr_i = FAST ? rp_i.y * frcp(rp_i.x * rp_i.x) : 0.0f;
+    walk_masked(
+        w, i, gi, [&](int tj) { return tj == ph.fluid_tag || tj == ph.wall_tag; }, [&](int) {},
+        [&](int j, int tj, float hj, float dx, float dy, float dr2) {
+            const float2 rp_j = f.rp[j];
+            momentum_pair<FAST, K, VISC>(ph, acc, tj == ph.fluid_tag, gi.z, hj, ui, rp_i.x, rp_i.y, pr_i, f.vel[j],
+                                         rp_j.x, rp_j.y, f.mass[j], dx, dy, dr2);
+        });
+    f.dvel[i] = acc;
+}
+
+// Other interactions over the last search's lists (device_interaction!,
+// Action.jl:121-138), walking global memory:
 //  EPPX_INTER_FILTER: iFilter main.jl:128-134 -> iKernelFilter! (Filter.jl:10-22).
 //  EPPX_INTER_ADD_PER_NEIGHBOUR: field[I] += value per neighbour.
-template <bool FAST, int ACTION>
+template <int ACTION>
 __global__ __launch_bounds__(kBlock) void k_interaction(Walk w, Fields f, Physics ph, eppx_field_op op) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= *w.n_alive) return;
-    const float2 xi = w.pos_nb[i];
-    const int ti = w.tag[i];
-    const float hi = w.h[i];
-    if (ACTION == EPPX_INTER_MOMENTUM) {
-        if (ti != ph.fluid_tag) return;
-        const float2 ui = f.vel[i];
-        const float rho_i = f.rho[i], p_i = f.p[i];
-        float2 acc = f.dvel[i];
-        const float pr_i = FAST ? p_i * frcp(rho_i * rho_i) : 0.0f;
-        walk(w, i, xi, ti, hi, [&](int j, int tj, float dx, float dy, float dr2) {
-            const bool classic = tj == ph.fluid_tag;
-            if (!classic && tj != ph.wall_tag) return;
-            const float hj = w.h[j];
-            const float rho_j = f.rho[j], p_j = f.p[j], m_j = f.mass[j];
-            const float2 uj = f.vel[j];
-            const float dux = ui.x - uj.x, duy = ui.y - uj.y;
-            if (!FAST) {
-                const float r = sqrtf(dr2);
-                const float hinv = invharmonic(hi, hj);
-                const float dw = grad_exact(ph.kernel, r, hinv);
-                float prho2 = classic ? p_i / (rho_i * rho_i) + p_j / (rho_j * rho_j) : (p_i + p_j) / (rho_i * rho_j);
-                prho2 += fabsf(prho2) * ph.coefficient;
-                prho2 *= -m_j * dw / r;
-                acc.x += prho2 * dx;
-                acc.y += prho2 * dy;
-                if (ph.visc == EPPX_VISC_CLASSIC) {
-                    const float h = 1.0f / hinv;
-                    const float az = r * r + 0.01f * h * h;
-                    const float force = 2.0f * ph.mu * m_j * r * dw / (rho_i * rho_j * az);
-                    acc.x += force * dux;
-                    acc.y += force * duy;
-                } else {
-                    float vdotx = 0.0f;
-                    vdotx += dx * dux;
-                    vdotx += dy * duy;
-                    if (vdotx > 0.0f) return;
-                    const float rho = (rho_i + rho_j) * 0.5f;
-                    const float h = 1.0f / hinv;
-                    const float az = r * r + 0.01f * h * h;
-                    const float phi = vdotx / (hinv * az);
-                    float force = (-ph.alpha * ph.av_c + ph.beta * phi) * phi / rho;
-                    force *= -m_j / (rho_i * rho_j * r) * dw;
-                    acc.x += force * dx;
-                    acc.y += force * dy;
-                }
-            } else {
-                const float rinv = frsq(dr2);
-                const float r = dr2 * rinv;
-                const float hinv = (hi + hj) * frcp(2.0f * hi * hj);
-                const float dw = grad_fast(ph.kernel, r, hinv);
-                const float rirj = rho_i * rho_j;
-                float prho2 = classic ? fmaf(p_j, frcp(rho_j * rho_j), pr_i) : (p_i + p_j) * frcp(rirj);
-                prho2 = fmaf(fabsf(prho2), ph.coefficient, prho2);
-                const float mdw = -m_j * dw;
-                prho2 *= mdw * rinv;
-                acc.x = fmaf(prho2, dx, acc.x);
-                acc.y = fmaf(prho2, dy, acc.y);
-                if (ph.visc == EPPX_VISC_CLASSIC) {
-                    const float h = frcp(hinv);
-                    const float az = fmaf(r, r, 0.01f * h * h);
-                    const float force = (2.0f * ph.mu) * m_j * r * dw * frcp(rirj * az);
-                    acc.x = fmaf(force, dux, acc.x);
-                    acc.y = fmaf(force, duy, acc.y);
-                } else {
-                    const float vdotx = fmaf(dy, duy, dx * dux);
-                    if (vdotx > 0.0f) return;
-                    const float h = frcp(hinv);
-                    const float az = fmaf(r, r, 0.01f * h * h);
-                    const float phi = vdotx * frcp(hinv * az);
-                    const float force = fmaf(ph.beta, phi, -ph.alpha * ph.av_c) * phi * frcp(0.5f * (rho_i + rho_j)) *
-                                        (mdw * frcp(rirj * r));
-                    acc.x = fmaf(force, dx, acc.x);
-                    acc.y = fmaf(force, dy, acc.y);
-                }
-            }
-        });
-        f.dvel[i] = acc;
-    } else if (ACTION == EPPX_INTER_FILTER) {
+    const float4 gi = w.geo[i];
+    const int ti = tag_of(gi);
+    if (ACTION == EPPX_INTER_FILTER) {
         if (ti != ph.fluid_tag) return;
         float wv = f.wv[i], wvrho = f.wvrho[i];
-        walk(w, i, xi, ti, hi, [&](int j, int tj, float, float, float dr2) {
+        walk(w, i, gi, [&](int j, int tj, float, float, float dr2) {
             if (tj != ph.fluid_tag) return;
             const float r = sqrtf(dr2);
-            const float wgt = value_exact(ph.kernel, r, invharmonic(hi, w.h[j]));
+            const float wgt = value_exact(ph.kernel, r, invharmonic(gi.z, w.geo[j].z));
             wv += wgt * f.vol[j];
             wvrho += wgt * f.mass[j];
         });
         f.wv[i] = wv;
         f.wvrho[i] = wvrho;
     } else {
-        float *col = nullptr;
+        float *col;
+        int stride = 1;
         switch (op.field) {
-            case EPPX_FIELD_POSITION: col = reinterpret_cast<float *>(f.pos) + op.component; break;
-            case EPPX_FIELD_VELOCITY: col = reinterpret_cast<float *>(f.vel) + op.component; break;
-            case EPPX_FIELD_DVELOCITY: col = reinterpret_cast<float *>(f.dvel) + op.component; break;
+            case EPPX_FIELD_POSITION: col = reinterpret_cast<float *>(f.pos) + op.component; stride = 2; break;
+            case EPPX_FIELD_VELOCITY: col = reinterpret_cast<float *>(f.vel) + op.component; stride = 2; break;
+            case EPPX_FIELD_DVELOCITY: col = reinterpret_cast<float *>(f.dvel) + op.component; stride = 2; break;
             case EPPX_FIELD_MASS: col = f.mass; break;
             case EPPX_FIELD_VOLUME: col = f.vol; break;
-            case EPPX_FIELD_DENSITY: col = f.rho; break;
+            case EPPX_FIELD_DENSITY: col = reinterpret_cast<float *>(f.rp); stride = 2; break;
             case EPPX_FIELD_DDENSITY: col = f.drho; break;
-            default: col = f.p; break;
+            default: col = reinterpret_cast<float *>(f.rp) + 1; stride = 2; break;
         }
-        const int stride = op.field <= EPPX_FIELD_DVELOCITY ? 2 : 1;
         float v = col[(int64_t)i * stride];
-        walk(w, i, xi, ti, hi, [&](int, int, float, float, float) { v += op.value; });
+        walk(w, i, gi, [&](int, int, float, float, float) { v += op.value; });
         col[(int64_t)i * stride] = v;
     }
 }
@@ -551,11 +632,10 @@ __global__ __launch_bounds__(kBlock) void k_self(Fields f, int32_t n, Physics ph
     const int tag = f.tag[s];
     if (ACTION == EPPX_SELF_CONTINUITY_EOS) {
         if (tag != ph.fluid_tag && tag != ph.wall_tag) return;
-        const float rho = f.rho[s] + f.drho[s] * ph.dt;
-        f.rho[s] = rho;
+        const float rho = f.rp[s].x + f.drho[s] * ph.dt;
         f.drho[s] = 0.0f;
         f.vol[s] = f.mass[s] / rho;
-        f.p[s] = eos(ph, rho);
+        f.rp[s] = make_float2(rho, eos(ph, rho));
     } else if (ACTION == EPPX_SELF_GRAVITY_MOVE) {
         if (tag != ph.fluid_tag) return;
         float2 du = f.dvel[s];
@@ -576,11 +656,10 @@ __global__ __launch_bounds__(kBlock) void k_self(Fields f, int32_t n, Physics ph
         const float wv = f.wv[s] + w0 * f.vol[s];
         const float wvrho = f.wvrho[s] + w0 * f.mass[s];
         const float rho = wvrho / wv;
-        f.rho[s] = rho;
         f.wv[s] = 0.0f;
         f.wvrho[s] = 0.0f;
         f.vol[s] = f.mass[s] / rho;
-        f.p[s] = eos(ph, rho);
+        f.rp[s] = make_float2(rho, eos(ph, rho));
     } else {
         float *col;
         int stride = 1;
@@ -590,9 +669,9 @@ __global__ __launch_bounds__(kBlock) void k_self(Fields f, int32_t n, Physics ph
             case EPPX_FIELD_DVELOCITY: col = reinterpret_cast<float *>(f.dvel) + op.component; stride = 2; break;
             case EPPX_FIELD_MASS: col = f.mass; break;
             case EPPX_FIELD_VOLUME: col = f.vol; break;
-            case EPPX_FIELD_DENSITY: col = f.rho; break;
+            case EPPX_FIELD_DENSITY: col = reinterpret_cast<float *>(f.rp); stride = 2; break;
             case EPPX_FIELD_DDENSITY: col = f.drho; break;
-            default: col = f.p; break;
+            default: col = reinterpret_cast<float *>(f.rp) + 1; stride = 2; break;
         }
         col[(int64_t)s * stride] *= op.value;
     }
@@ -609,13 +688,14 @@ __global__ __launch_bounds__(kBlock) void k_export(Walk w, Fields f, const int64
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= *w.n_alive) return;
     const int row = f.id[i];
-    const int ti = w.tag[i];
-    const float hi = w.h[i];
+    const float4 gi = w.geo[i];
+    const int ti = tag_of(gi);
+    const float hi = gi.z;
     const size_t cap = (size_t)ex.n_capacity;
     const int64_t o = row_start ? row_start[i] : 0;
     const int limit = js ? ncount[i] : 0;  // the row length the search counted
     int k = 0;
-    walk(w, i, w.pos_nb[i], ti, hi, [&](int j, int tj, float dx, float dy, float dr2) {
+    walk(w, i, gi, [&](int j, int tj, float dx, float dy, float dr2) {
         if (js && k < limit) js[o + k] = f.id[j];
         if (k < ex.M) {
             if (ex.nIndex) ex.nIndex[row + cap * k] = f.id[j] + 1;
@@ -628,10 +708,10 @@ __global__ __launch_bounds__(kBlock) void k_export(Walk w, Fields f, const int64
             const bool kernel_pair = (ti == fluid_tag && (tj == fluid_tag || tj == wall_tag)) ||
                                      (ti == wall_tag && tj == fluid_tag);
             if (ex.with_kernel && kernel_pair) {
-                const float hinv = invharmonic(hi, w.h[j]);
+                const float hinv = invharmonic(hi, w.geo[j].z);
                 if (ex.nHInv) ex.nHInv[row + cap * k] = hinv;
                 if (ex.nW) ex.nW[row + cap * k] = value_exact(ex.kernel, r, hinv);
-                if (ex.nDW) ex.nDW[row + cap * k] = grad_exact(ex.kernel, r, hinv);
+                if (ex.nDW) ex.nDW[row + cap * k] = grad_exact_rt(ex.kernel, r, hinv);
             }
         }
         ++k;
@@ -650,11 +730,10 @@ __global__ __launch_bounds__(kBlock) void k_from_ref(Fields f, int32_t cap, Colu
     f.pos[s] = make_float2(fcol(c.PositionVec), c.PositionVec >= 0 ? fcol(c.PositionVec + 1) : 0.0f);
     f.vel[s] = make_float2(fcol(c.VelocityVec), c.VelocityVec >= 0 ? fcol(c.VelocityVec + 1) : 0.0f);
     f.dvel[s] = make_float2(fcol(c.dVelocityVec), c.dVelocityVec >= 0 ? fcol(c.dVelocityVec + 1) : 0.0f);
+    f.rp[s] = make_float2(fcol(c.Density), fcol(c.Pressure));
     f.mass[s] = fcol(c.Mass);
     f.vol[s] = fcol(c.Volume);
-    f.rho[s] = fcol(c.Density);
     f.drho[s] = fcol(c.dDensity);
-    f.p[s] = fcol(c.Pressure);
     f.h[s] = fcol(c.H);
     f.wv[s] = fcol(c.SumWeight);
     f.wvrho[s] = fcol(c.SumWeightedDensity);
@@ -683,11 +762,12 @@ __global__ __launch_bounds__(kBlock) void k_to_ref(Fields f, int32_t cap, Column
     put2(c.PositionVec, f.pos[s]);
     put2(c.VelocityVec, f.vel[s]);
     put2(c.dVelocityVec, f.dvel[s]);
+    const float2 rp = f.rp[s];
+    put(c.Density, rp.x);
+    put(c.Pressure, rp.y);
     put(c.Mass, f.mass[s]);
     put(c.Volume, f.vol[s]);
-    put(c.Density, f.rho[s]);
     put(c.dDensity, f.drho[s]);
-    put(c.Pressure, f.p[s]);
     put(c.H, f.h[s]);
     put(c.SumWeight, f.wv[s]);
     put(c.SumWeightedDensity, f.wvrho[s]);
@@ -717,26 +797,37 @@ __global__ __launch_bounds__(kBlock) void k_stencil(Grid g, int32_t *count, int3
 __global__ __launch_bounds__(kBlock) void k_finite(Fields f, int32_t n, uint32_t *bad) {
     const int s = blockIdx.x * kBlock + threadIdx.x;
     if (s >= n || f.alive[s] != 1) return;
-    const float2 x = f.pos[s], u = f.vel[s];
-    if (!isfinite(f.rho[s]) || !isfinite(f.p[s]) || !isfinite(x.x) || !isfinite(x.y) || !isfinite(u.x) ||
-        !isfinite(u.y))
+    const float2 x = f.pos[s], u = f.vel[s], rp = f.rp[s];
+    if (!isfinite(rp.x) || !isfinite(rp.y) || !isfinite(x.x) || !isfinite(x.y) || !isfinite(u.x) || !isfinite(u.y))
         atomicOr(bad, 1u);
 }
 
 // min / max of H over alive particles; h is positive so the float bit
-// patterns order like unsigned integers (non-positive or NaN h marks the
-// range invalid via the sentinel written by the host).
+// patterns order like unsigned integers.  range[2] is set when some alive
+// h is not a positive finite number.  One atomic per wave.
 __global__ __launch_bounds__(kBlock) void k_h_range(Fields f, int32_t n, float *range) {
     const int s = blockIdx.x * kBlock + threadIdx.x;
-    if (s >= n || f.alive[s] != 1) return;
-    const float h = f.h[s];
-    unsigned int *r = reinterpret_cast<unsigned int *>(range);
-    if (!(h > 0.0f) || !isfinite(h)) {
-        atomicMax(&r[2], 1u);
-        return;
+    unsigned int lo = 0xffffffffu, hi = 0u, bad = 0u;
+    if (s < n && f.alive[s] == 1) {
+        const float h = f.h[s];
+        if (!(h > 0.0f) || !isfinite(h)) {
+            bad = 1u;
+        } else {
+            lo = __float_as_uint(h);
+            hi = lo;
+        }
     }
-    atomicMin(&r[0], __float_as_uint(h));
-    atomicMax(&r[1], __float_as_uint(h));
+    for (int off = 32; off > 0; off >>= 1) {
+        lo = min(lo, (unsigned int)__shfl_down((int)lo, off));
+        hi = max(hi, (unsigned int)__shfl_down((int)hi, off));
+        bad |= (unsigned int)__shfl_down((int)bad, off);
+    }
+    if (lane_id() == 0) {
+        unsigned int *r = reinterpret_cast<unsigned int *>(range);
+        if (lo != 0xffffffffu) atomicMin(&r[0], lo);
+        if (hi) atomicMax(&r[1], hi);
+        if (bad) atomicMax(&r[2], 1u);
+    }
 }
 
 inline dim3 grid_for(int64_t n, int block = kBlock) {
@@ -779,28 +870,36 @@ hipError_t launch_fill_members(const Fields f, int32_t n, int32_t n_cells, const
     return hipGetLastError();
 }
 
-hipError_t launch_permute(Fields cur, Fields nxt, float2 *pos_nb, int32_t n, int32_t cap, int32_t n_cells,
+hipError_t launch_permute(Fields cur, Fields nxt, float4 *geo, int32_t n, int32_t cap, int32_t n_cells,
                           const int32_t *key, const int32_t *tmpslot, const int32_t *start,
                           const int32_t *members, int32_t *n_alive, uint32_t *flags, hipStream_t s) {
-    hipLaunchKernelGGL(k_permute, grid_for(cap), dim3(kBlock), 0, s, cur, nxt, pos_nb, n, cap, n_cells, key,
-                       tmpslot, start, members, n_alive, flags);
+    hipLaunchKernelGGL(k_permute, grid_for(cap), dim3(kBlock), 0, s, cur, nxt, geo, n, cap, n_cells, key, tmpslot,
+                       start, members, n_alive, flags);
     return hipGetLastError();
 }
 
 hipError_t launch_search(const Walk &w, Fields f, int32_t n, int32_t action, const Physics &ph, int32_t fast,
                          int32_t M, int32_t *ncount, unsigned long long *pairs_last,
                          unsigned long long *pairs_total, uint32_t *flags, hipStream_t s) {
-    const dim3 g = grid_for(n);
-    if (action == EPPX_SEARCH_KERNEL_CONTINUITY) {
-        if (fast)
-            hipLaunchKernelGGL((k_search<true, EPPX_SEARCH_KERNEL_CONTINUITY>), g, dim3(kBlock), 0, s, w, f, ph, M,
-                               ncount, pairs_last, pairs_total, flags);
+    const dim3 g = grid_for(n, kPairBlock), b(kPairBlock);
+    constexpr int KC = EPPX_KERNEL_CUBIC, KW = EPPX_KERNEL_WENDLAND_C2, KCONT = EPPX_SEARCH_KERNEL_CONTINUITY;
+    if (action == KCONT) {
+        const bool cubic = ph.kernel == KC;
+        if (fast && cubic)
+            hipLaunchKernelGGL((k_search<true, KCONT, KC>), g, b, 0, s, w, f, ph, M, ncount, pairs_last, pairs_total,
+                               flags);
+        else if (fast)
+            hipLaunchKernelGGL((k_search<true, KCONT, KW>), g, b, 0, s, w, f, ph, M, ncount, pairs_last, pairs_total,
+                               flags);
+        else if (cubic)
+            hipLaunchKernelGGL((k_search<false, KCONT, KC>), g, b, 0, s, w, f, ph, M, ncount, pairs_last,
+                               pairs_total, flags);
         else
-            hipLaunchKernelGGL((k_search<false, EPPX_SEARCH_KERNEL_CONTINUITY>), g, dim3(kBlock), 0, s, w, f, ph, M,
-                               ncount, pairs_last, pairs_total, flags);
+            hipLaunchKernelGGL((k_search<false, KCONT, KW>), g, b, 0, s, w, f, ph, M, ncount, pairs_last,
+                               pairs_total, flags);
     } else {
-        hipLaunchKernelGGL((k_search<false, EPPX_SEARCH_NONE>), g, dim3(kBlock), 0, s, w, f, ph, M, ncount,
-                           pairs_last, pairs_total, flags);
+        hipLaunchKernelGGL((k_search<false, EPPX_SEARCH_NONE, KW>), g, b, 0, s, w, f, ph, M, ncount, pairs_last,
+                           pairs_total, flags);
     }
     return hipGetLastError();
 }
@@ -825,22 +924,32 @@ hipError_t launch_self(Fields f, int32_t n, int32_t action, const Physics &ph, c
     return hipGetLastError();
 }
 
+template <bool FAST, int K>
+static void launch_momentum_v(const Walk &w, Fields f, const Physics &ph, dim3 g, hipStream_t s) {
+    if (ph.visc == EPPX_VISC_CLASSIC)
+        hipLaunchKernelGGL((k_momentum<FAST, K, EPPX_VISC_CLASSIC>), g, dim3(kPairBlock), 0, s, w, f, ph);
+    else
+        hipLaunchKernelGGL((k_momentum<FAST, K, EPPX_VISC_ARTIFICIAL>), g, dim3(kPairBlock), 0, s, w, f, ph);
+}
+
 hipError_t launch_interaction(const Walk &w, Fields f, int32_t n, int32_t action, const Physics &ph,
                               const eppx_field_op &op, int32_t fast, hipStream_t s) {
-    const dim3 g = grid_for(n);
     switch (action) {
-        case EPPX_INTER_MOMENTUM:
-            if (fast)
-                hipLaunchKernelGGL((k_interaction<true, EPPX_INTER_MOMENTUM>), g, dim3(kBlock), 0, s, w, f, ph, op);
-            else
-                hipLaunchKernelGGL((k_interaction<false, EPPX_INTER_MOMENTUM>), g, dim3(kBlock), 0, s, w, f, ph, op);
+        case EPPX_INTER_MOMENTUM: {
+            const dim3 g = grid_for(n, kPairBlock);
+            const bool cubic = ph.kernel == EPPX_KERNEL_CUBIC;
+            if (fast && cubic) launch_momentum_v<true, EPPX_KERNEL_CUBIC>(w, f, ph, g, s);
+            else if (fast) launch_momentum_v<true, EPPX_KERNEL_WENDLAND_C2>(w, f, ph, g, s);
+            else if (cubic) launch_momentum_v<false, EPPX_KERNEL_CUBIC>(w, f, ph, g, s);
+            else launch_momentum_v<false, EPPX_KERNEL_WENDLAND_C2>(w, f, ph, g, s);
             break;
+        }
         case EPPX_INTER_FILTER:
-            hipLaunchKernelGGL((k_interaction<false, EPPX_INTER_FILTER>), g, dim3(kBlock), 0, s, w, f, ph, op);
+            hipLaunchKernelGGL(k_interaction<EPPX_INTER_FILTER>, grid_for(n), dim3(kBlock), 0, s, w, f, ph, op);
             break;
         default:
-            hipLaunchKernelGGL((k_interaction<false, EPPX_INTER_ADD_PER_NEIGHBOUR>), g, dim3(kBlock), 0, s, w, f,
-                               ph, op);
+            hipLaunchKernelGGL(k_interaction<EPPX_INTER_ADD_PER_NEIGHBOUR>, grid_for(n), dim3(kBlock), 0, s, w, f, ph,
+                               op);
             break;
     }
     return hipGetLastError();

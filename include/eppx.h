@@ -212,6 +212,10 @@ int32_t eppx_pair_total(eppx_ps *ps, int64_t *out, int32_t reset);
  * index 0..n_particles-1: offsets (n_particles + 1) and 0-based js
  * (capacity entries; fails with EPPX_E_CAPACITY when too small) (waits). */
 int32_t eppx_export_pairs(eppx_ps *ps, eppx_ns *ns, int64_t *offsets, int32_t *js, int64_t capacity);
+/* Number of pair-kernel blocks (search and momentum) that could not stage
+ * their neighbourhood in on-chip memory and read it from HBM instead, since
+ * the last reset (a performance diagnostic; results do not depend on it). */
+int32_t eppx_pair_fallback_blocks(eppx_ps *ps, int64_t *out, int32_t reset);
 /* Returns EPPX_E_NONFINITE when a live particle's density, pressure,
  * position or velocity is not finite (waits). */
 int32_t eppx_check_finite(eppx_ps *ps);
