@@ -80,6 +80,24 @@ def cpu_baseline(case, steps, threads):
     return float(pairs.sum()) / dt, dt
 
 
+def copy_calibration(n_bytes: int, repeats: int = 3) -> None:
+    """Device-to-device copies of n_bytes (HIP's own copy kernel): the known-size
+    reference that scales rocprofv3's FETCH_SIZE / WRITE_SIZE readings."""
+    hip = C.CDLL("libamdhip64.so")
+    src, dst = C.c_void_p(), C.c_void_p()
+    for p in (src, dst):
+        if hip.hipMalloc(C.byref(p), C.c_size_t(n_bytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(src, 1, C.c_size_t(n_bytes))
+    for _ in range(repeats):
+        if hip.hipMemcpy(dst, src, C.c_size_t(n_bytes), 3) != 0:  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(json.dumps({"copy_calibration_bytes": n_bytes, "repeats": repeats}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,7 +107,12 @@ def main():
     ap.add_argument("--math", choices=["exact", "fast"], default="fast")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run device copies of BYTES (profiler calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "libeppx runs one GPU per process; multi-GPU runs are not part of this "
                                    "build (see DESIGN.md: how the path would shard)", "n_gpus": args.gpus}))

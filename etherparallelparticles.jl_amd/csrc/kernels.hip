@@ -125,7 +125,8 @@ __device__ __forceinline__ bool accept_pair(const Walk &w, float2 xi, float hi, 
 }
 
 __device__ __forceinline__ bool adjacent(uint32_t adj_i, int tj) {
-    return tj >= 1 && tj <= kMaxTag && ((adj_i >> (tj - 1)) & 1u);
+    const uint32_t t1 = (uint32_t)tj - 1u;
+    return (t1 < (uint32_t)kMaxTag) & ((adj_i >> (t1 & 31u)) & 1u);
 }
 
 // ------------------------------------------------------------------ walk
@@ -160,46 +161,129 @@ __device__ __forceinline__ void walk(const Walk &w, int i, float4 gi, Visit &&vi
     }
 }
 
-// ---------------------------------------------------------- masked walk
-// Same visit order as walk(), in two passes per neighbour cell (chunks of 64
-// candidates): `count(tj)` sees every accepted neighbour, and `pair(j, tj)`
-// then runs for the accepted ones that `keep(tj)` in ascending slot order.
-template <class Keep, class Count, class Pair>
-__device__ __forceinline__ void walk_masked(const Walk &w, int i, float4 gi, Keep &&keep, Count &&count,
-                                            Pair &&pair) {
+// ------------------------------------------------------------ row walk
+// Two passes over the 3x3 cell neighbourhood of slot i.
+//  Pass 1: the three cells of each neighbour row are contiguous in the
+//    cell-sorted order, so each row is one slot range [b_r, e_r) whose
+//    candidates are tested (one 16-byte load each) into a 64-bit accept mask.
+//  Pass 2: `pair` runs for every accepted neighbour that `keep(tj)`.  ORDERED
+//    visits them in the reference's list order (x offset outer, y offset
+//    inner, ascending slot inside a cell), so sums are bit-identical to the
+//    reference; otherwise row by row, in one loop that keeps every lane busy.
+// A lane whose row range exceeds 64 slots walks global memory one candidate
+// at a time instead.  Returns the number of accepted neighbours (nCount).
+__device__ __forceinline__ unsigned long long sel3(int r, unsigned long long a, unsigned long long b,
+                                                   unsigned long long c) {
+    return r == 0 ? a : (r == 1 ? b : c);
+}
+__device__ __forceinline__ int sel3(int r, int a, int b, int c) { return r == 0 ? a : (r == 1 ? b : c); }
+
+// Accept bit of one candidate: adjacency and criterion without branches on
+// the common path; only candidates inside the symmetry criterion's tie band
+// take the exact harmonic-mean test (a rarely taken branch).
+__device__ __forceinline__ uint32_t accept_bit(const Walk &w, float2 xi, float hi, uint32_t adj, float4 gj) {
+    const uint32_t t1 = (uint32_t)tag_of(gj) - 1u;
+    const uint32_t adj_ok = (t1 < (uint32_t)kMaxTag) & (adj >> (t1 & 31u));
+    const float dx = xi.x - gj.x, dy = xi.y - gj.y;
+    const float dr2 = dx * dx + dy * dy;
+    uint32_t ok;
+    if (w.criterion == EPPX_CRIT_DEFAULT) {
+        ok = dr2 <= w.g.gap_square;
+    } else {
+        ok = dr2 < w.rmin2;
+        if (!ok && dr2 <= w.rmax2) {
+            const float h2 = harmonic(hi, gj.z) * 2.0f;
+            ok = dr2 <= h2 * h2;
+        }
+    }
+    return ok & adj_ok & 1u;
+}
+
+__device__ __forceinline__ unsigned long long accept_row(const Walk &w, int b, int e, float2 xi, float hi,
+                                                         uint32_t adj) {
+    unsigned long long m = 0ull;
+    int j = b;
+    for (; j + 4 <= e; j += 4) {  // four independent loads in flight per iteration
+        const float4 g0 = w.geo[j], g1 = w.geo[j + 1], g2 = w.geo[j + 2], g3 = w.geo[j + 3];
+        const uint32_t bits = accept_bit(w, xi, hi, adj, g0) | (accept_bit(w, xi, hi, adj, g1) << 1) |
+                              (accept_bit(w, xi, hi, adj, g2) << 2) | (accept_bit(w, xi, hi, adj, g3) << 3);
+        m |= (unsigned long long)bits << (j - b);
+    }
+    for (; j < e; ++j) m |= (unsigned long long)accept_bit(w, xi, hi, adj, w.geo[j]) << (j - b);
+    return m;
+}
+
+template <bool ORDERED, class Keep, class Pair>
+__device__ __forceinline__ int walk_rows(const Walk &w, int i, float4 gi, Keep &&keep, Pair &&pair) {
     const int c = w.cell[i] - 1;
-    const int cx = c % w.g.n_x, cy = c / w.g.n_x;
+    const int nx = w.g.n_x;
+    const int cx = c % nx, cy = c / nx;
+    const int x0 = max(cx - 1, 0), x1 = min(cx + 1, nx - 1);
+    int b[3], e[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        const int jj = cy + r - 1;
+        b[r] = e[r] = 0;
+        if (jj >= 0 && jj < w.g.n_y) {
+            b[r] = w.start[x0 + nx * jj];
+            e[r] = w.start[x1 + 1 + nx * jj];
+        }
+    }
+    if (e[0] - b[0] > 64 || e[1] - b[1] > 64 || e[2] - b[2] > 64) {
+        int cnt = 0;
+        walk(w, i, gi, [&](int j, int tj, float dx, float dy, float dr2) {
+            ++cnt;
+            if (keep(tj)) pair(j, tj, w.geo[j].z, dx, dy, dr2);
+        });
+        return cnt;
+    }
     const int ti = tag_of(gi);
     const uint32_t adj = (ti >= 1 && ti <= kMaxTag) ? w.adj[ti] : 0u;
     const float2 xi = make_float2(gi.x, gi.y);
-    for (int s9 = 0; s9 < 9; ++s9) {
-        const int ii = cx + s9 / 3 - 1, jj = cy + s9 % 3 - 1;
-        if (ii < 0 || ii >= w.g.n_x || jj < 0 || jj >= w.g.n_y) continue;
-        const int k = ii + w.g.n_x * jj;
-        const int b = w.start[k], e = w.start[k + 1];
-        for (int cb = b; cb < e; cb += 64) {
-            const int ce = min(e, cb + 64);
-            unsigned long long m = 0ull;
-#pragma unroll 4
-            for (int j = cb; j < ce; ++j) {
-                const float4 gj = w.geo[j];
-                const int tj = tag_of(gj);
-                float dx, dy, dr2;
-                const bool ok = j != i && adjacent(adj, tj) && accept_pair(w, xi, gi.z, gj, dx, dy, dr2);
-                if (ok) {
-                    count(tj);
-                    if (keep(tj)) m |= 1ull << (j - cb);
-                }
-            }
+    const unsigned long long m0 = accept_row(w, b[0], e[0], xi, gi.z, adj);
+    unsigned long long m1 = accept_row(w, b[1], e[1], xi, gi.z, adj);
+    const unsigned long long m2 = accept_row(w, b[2], e[2], xi, gi.z, adj);
+    m1 &= ~(1ull << (i - b[1]));  // I != J
+    const int cnt = __popcll(m0) + __popcll(m1) + __popcll(m2);
+    auto visit = [&](int j) {
+        const float4 gj = w.geo[j];
+        const int tj = tag_of(gj);
+        if (!keep(tj)) return;
+        const float dx = xi.x - gj.x, dy = xi.y - gj.y;
+        pair(j, tj, gj.z, dx, dy, dx * dx + dy * dy);
+    };
+    if (ORDERED) {
+        for (int s9 = 0; s9 < 9; ++s9) {
+            const int ii = cx + s9 / 3 - 1, r = s9 % 3, jj = cy + r - 1;
+            if (ii < 0 || ii >= nx || jj < 0 || jj >= w.g.n_y) continue;
+            const int k = ii + nx * jj;
+            const int cb = w.start[k], len = w.start[k + 1] - cb;
+            if (len == 0) continue;
+            const int lo = cb - sel3(r, b[0], b[1], b[2]);  // < 64: the cell lies inside its row range
+            unsigned long long m = sel3(r, m0, m1, m2) >> lo;
+            if (len < 64) m &= (1ull << len) - 1ull;
             while (m) {
                 const int j = cb + __builtin_ctzll(m);
                 m &= m - 1ull;
-                const float4 gj = w.geo[j];
-                const float dx = xi.x - gj.x, dy = xi.y - gj.y;
-                pair(j, tag_of(gj), gj.z, dx, dy, dx * dx + dy * dy);
+                visit(j);
             }
         }
+    } else {
+        int r = 0, base = b[0];
+        unsigned long long m = m0;
+        while (true) {
+            while (!m && r < 2) {
+                ++r;
+                m = sel3(r, m0, m1, m2);
+                base = sel3(r, b[0], b[1], b[2]);
+            }
+            if (!m) break;
+            const int j = base + __builtin_ctzll(m);
+            m &= m - 1ull;
+            visit(j);
+        }
     }
+    return cnt;
 }
 
 __device__ __forceinline__ int block_sum_i32(int v, int *lds) {
@@ -461,17 +545,15 @@ __global__ __launch_bounds__(kPairBlock) void k_search(Walk w, Fields f, Physics
             const float rho_i = f.rp[i].x;
             float acc = f.drho[i];
             // pair classes of the continuity action: fluid-fluid (classic), fluid-wall / wall-fluid (balanced)
-            walk_masked(
-                w, i, gi,
-                [&](int tj) {
-                    return (i_fluid && (tj == ph.fluid_tag || tj == ph.wall_tag)) || (i_wall && tj == ph.fluid_tag);
-                },
-                [&](int) { ++cnt; },
-                [&](int j, int tj, float hj, float dx, float dy, float dr2) {
-                    const bool ff = i_fluid && tj == ph.fluid_tag;
-                    const float m = ff ? f.mass[j] : rho_i * f.vol[j];
-                    acc += continuity_term<FAST, K>(gi.z, hj, ui, f.vel[j], m, dx, dy, dr2);
-                });
+            auto keep = [&](int tj) {
+                return (i_fluid && (tj == ph.fluid_tag || tj == ph.wall_tag)) || (i_wall && tj == ph.fluid_tag);
+            };
+            auto term = [&](int j, int tj, float hj, float dx, float dy, float dr2) {
+                const bool ff = i_fluid && tj == ph.fluid_tag;
+                const float m = ff ? f.mass[j] : rho_i * f.vol[j];
+                acc += continuity_term<FAST, K>(gi.z, hj, ui, f.vel[j], m, dx, dy, dr2);
+            };
+            cnt = walk_rows<!FAST>(w, i, gi, keep, term);
             f.drho[i] = acc;
         } else {
             walk(w, i, gi, [&](int, int, float, float, float) { ++cnt; });
@@ -566,8 +648,8 @@ __global__ __launch_bounds__(kPairBlock) void k_momentum(Walk w, Fields f, Physi
     const float2 ui = f.vel[i], rp_i = f.rp[i];
     float2 acc = f.dvel[i];
     const float pr_i = FAST ? rp_i.y * frcp(rp_i.x * rp_i.x) : 0.0f;
-    walk_masked(
-        w, i, gi, [&](int tj) { return tj == ph.fluid_tag || tj == ph.wall_tag; }, [&](int) {},
+    walk_rows<!FAST>(
+        w, i, gi, [&](int tj) { return tj == ph.fluid_tag || tj == ph.wall_tag; },
         [&](int j, int tj, float hj, float dx, float dy, float dr2) {
             const float2 rp_j = f.rp[j];
             momentum_pair<FAST, K, VISC>(ph, acc, tj == ph.fluid_tag, gi.z, hj, ui, rp_i.x, rp_i.y, pr_i, f.vel[j],
