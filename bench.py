@@ -157,7 +157,7 @@ def main():
     _lib.check(L.eppx_check_finite(ps.handle), par.ctx, "finite")
     pairs = total.value
     _lib.check(L.eppx_pair_fallback_blocks(ps.handle, C.byref(fb), 1), par.ctx, "fallback")
-    blocks_per_launch = -(-case.n_particles // 128)
+    blocks_per_launch = -(-case.n_particles // 256)  # 256 particles per pair block
     log(f"pair blocks reading HBM instead of LDS: {fb.value / (2 * args.steps) / blocks_per_launch:.4%}")
     names = ["search", "momentum", "continuity_eos", "gravity_move", "cell_reset", "cell_key", "cell_scan",
              "cell_members", "cell_permute"]

@@ -124,14 +124,14 @@ hipError_t alloc_fields(Fields &f, int32_t cap) {
     auto a = [&](auto **p) {
         if (e == hipSuccess) e = dalloc(p, (size_t)cap);
     };
-    a(&f.pos); a(&f.vel); a(&f.dvel); a(&f.rp);
+    a(&f.pos); a(&f.dvel); a(&f.dyn);
     a(&f.mass); a(&f.vol); a(&f.drho); a(&f.h); a(&f.wv); a(&f.wvrho);
     a(&f.tag); a(&f.mov); a(&f.id); a(&f.cell); a(&f.alive);
     return e;
 }
 
 void free_fields(Fields &f) {
-    dfree(f.pos); dfree(f.vel); dfree(f.dvel); dfree(f.rp);
+    dfree(f.pos); dfree(f.dvel); dfree(f.dyn);
     dfree(f.mass); dfree(f.vol); dfree(f.drho); dfree(f.h); dfree(f.wv); dfree(f.wvrho);
     dfree(f.tag); dfree(f.mov); dfree(f.id); dfree(f.cell); dfree(f.alive);
 }

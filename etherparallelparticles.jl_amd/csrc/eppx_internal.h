@@ -31,8 +31,8 @@ struct Grid {
 // row id[s].  After a search, slots [0, n_alive) are alive and sorted by
 // (cell, id); dead particles follow.
 struct Fields {
-    float2 *pos, *vel, *dvel;
-    float2 *rp;  // (density, pressure)
+    float2 *pos, *dvel;
+    float4 *dyn;  // (u, v, density, pressure): what a neighbour's pair term reads, one 16-byte record
     float *mass, *vol, *drho, *h, *wv, *wvrho;
     int32_t *tag, *mov, *id, *cell, *alive;
 };

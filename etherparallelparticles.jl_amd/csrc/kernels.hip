@@ -23,6 +23,7 @@ namespace {
 
 constexpr int kRankLimit = 1024;  // largest cell ordered by particle index
 constexpr int kPairBlock = 256;   // threads per pair-kernel block
+constexpr int kTileRow = 384;     // neighbour slots per row a pair block stages in LDS
 
 // sigma of the 2D kernels: FT(10/7/pi) (CubicSpline.jl:14), FT(7/4/pi) (WendlandC2.jl:13)
 constexpr float kSigmaCubic = (float)(10.0 / 7.0 / 3.14159265358979323846);
@@ -166,12 +167,15 @@ __device__ __forceinline__ void walk(const Walk &w, int i, float4 gi, Visit &&vi
 //  Pass 1: the three cells of each neighbour row are contiguous in the
 //    cell-sorted order, so each row is one slot range [b_r, e_r) whose
 //    candidates are tested (one 16-byte load each) into a 64-bit accept mask.
-//  Pass 2: `pair` runs for every accepted neighbour that `keep(tj)`.  ORDERED
-//    visits them in the reference's list order (x offset outer, y offset
-//    inner, ascending slot inside a cell), so sums are bit-identical to the
-//    reference; otherwise row by row, in one loop that keeps every lane busy.
+//  Pass 2: `pair` runs for every accepted neighbour whose tag is in
+//    keep_tags (bit t-1 for tag t; the tag filter is folded into the pass-1
+//    masks, so pass 2 issues its loads at once).  ORDERED visits them in the
+//    reference's list order (x offset outer, y offset inner, ascending slot
+//    inside a cell), so sums are bit-identical to the reference; otherwise
+//    row by row, in one loop that keeps every lane busy.
 // A lane whose row range exceeds 64 slots walks global memory one candidate
-// at a time instead.  Returns the number of accepted neighbours (nCount).
+// at a time instead.  With COUNT, returns the number of accepted neighbours
+// of every tag (nCount); otherwise 0.
 __device__ __forceinline__ unsigned long long sel3(int r, unsigned long long a, unsigned long long b,
                                                    unsigned long long c) {
     return r == 0 ? a : (r == 1 ? b : c);
@@ -181,6 +185,11 @@ __device__ __forceinline__ int sel3(int r, int a, int b, int c) { return r == 0 
 // Accept bit of one candidate: adjacency and criterion without branches on
 // the common path; only candidates inside the symmetry criterion's tie band
 // take the exact harmonic-mean test (a rarely taken branch).
+__device__ __forceinline__ uint32_t tag_bit(int t) {
+    const uint32_t t1 = (uint32_t)t - 1u;
+    return t1 < (uint32_t)kMaxTag ? 1u << t1 : 0u;
+}
+
 __device__ __forceinline__ uint32_t accept_bit(const Walk &w, float2 xi, float hi, uint32_t adj, float4 gj) {
     const uint32_t t1 = (uint32_t)tag_of(gj) - 1u;
     const uint32_t adj_ok = (t1 < (uint32_t)kMaxTag) & (adj >> (t1 & 31u));
@@ -199,22 +208,40 @@ __device__ __forceinline__ uint32_t accept_bit(const Walk &w, float2 xi, float h
     return ok & adj_ok & 1u;
 }
 
-__device__ __forceinline__ unsigned long long accept_row(const Walk &w, int b, int e, float2 xi, float hi,
-                                                         uint32_t adj) {
+// Candidate geometry of the row walk: slot j of neighbour row r, read from
+// global memory or from the block's LDS copy of its three row ranges.
+struct GlobalRows {
+    const float4 *geo;
+    __device__ __forceinline__ int off(int) const { return 0; }
+    __device__ __forceinline__ float4 get(int k) const { return geo[k]; }
+    __device__ __forceinline__ float4 at(int r, int j) const { return get(j + off(r)); }
+};
+struct TileRows {
+    const float4 *tile;  // LDS: row r, slot j at tile[j + off(r)]
+    int off0, off1, off2;
+    __device__ __forceinline__ int off(int r) const { return sel3(r, off0, off1, off2); }
+    __device__ __forceinline__ float4 get(int k) const { return tile[k]; }
+    __device__ __forceinline__ float4 at(int r, int j) const { return get(j + off(r)); }
+};
+
+template <class Src>
+__device__ __forceinline__ unsigned long long accept_row(const Walk &w, const Src &src, int r, int b, int e,
+                                                         float2 xi, float hi, uint32_t adj) {
     unsigned long long m = 0ull;
     int j = b;
     for (; j + 4 <= e; j += 4) {  // four independent loads in flight per iteration
-        const float4 g0 = w.geo[j], g1 = w.geo[j + 1], g2 = w.geo[j + 2], g3 = w.geo[j + 3];
+        const float4 g0 = src.at(r, j), g1 = src.at(r, j + 1), g2 = src.at(r, j + 2), g3 = src.at(r, j + 3);
         const uint32_t bits = accept_bit(w, xi, hi, adj, g0) | (accept_bit(w, xi, hi, adj, g1) << 1) |
                               (accept_bit(w, xi, hi, adj, g2) << 2) | (accept_bit(w, xi, hi, adj, g3) << 3);
         m |= (unsigned long long)bits << (j - b);
     }
-    for (; j < e; ++j) m |= (unsigned long long)accept_bit(w, xi, hi, adj, w.geo[j]) << (j - b);
+    for (; j < e; ++j) m |= (unsigned long long)accept_bit(w, xi, hi, adj, src.at(r, j)) << (j - b);
     return m;
 }
 
-template <bool ORDERED, class Keep, class Pair>
-__device__ __forceinline__ int walk_rows(const Walk &w, int i, float4 gi, Keep &&keep, Pair &&pair) {
+template <bool ORDERED, bool COUNT, class Src, class Pair>
+__device__ __forceinline__ int walk_rows(const Walk &w, const Src &src, int i, float4 gi, uint32_t keep_tags,
+                                         Pair &&pair) {
     const int c = w.cell[i] - 1;
     const int nx = w.g.n_x;
     const int cx = c % nx, cy = c / nx;
@@ -233,24 +260,33 @@ __device__ __forceinline__ int walk_rows(const Walk &w, int i, float4 gi, Keep &
         int cnt = 0;
         walk(w, i, gi, [&](int j, int tj, float dx, float dy, float dr2) {
             ++cnt;
-            if (keep(tj)) pair(j, tj, w.geo[j].z, dx, dy, dr2);
+            if (tag_bit(tj) & keep_tags) pair(j, tj, w.geo[j].z, dx, dy, dr2);
         });
-        return cnt;
+        return COUNT ? cnt : 0;
     }
     const int ti = tag_of(gi);
     const uint32_t adj = (ti >= 1 && ti <= kMaxTag) ? w.adj[ti] : 0u;
+    const uint32_t kadj = adj & keep_tags;
     const float2 xi = make_float2(gi.x, gi.y);
-    const unsigned long long m0 = accept_row(w, b[0], e[0], xi, gi.z, adj);
-    unsigned long long m1 = accept_row(w, b[1], e[1], xi, gi.z, adj);
-    const unsigned long long m2 = accept_row(w, b[2], e[2], xi, gi.z, adj);
-    m1 &= ~(1ull << (i - b[1]));  // I != J
-    const int cnt = __popcll(m0) + __popcll(m1) + __popcll(m2);
-    auto visit = [&](int j) {
-        const float4 gj = w.geo[j];
-        const int tj = tag_of(gj);
-        if (!keep(tj)) return;
+    const unsigned long long m0 = accept_row(w, src, 0, b[0], e[0], xi, gi.z, kadj);
+    unsigned long long m1 = accept_row(w, src, 1, b[1], e[1], xi, gi.z, kadj);
+    const unsigned long long m2 = accept_row(w, src, 2, b[2], e[2], xi, gi.z, kadj);
+    const unsigned long long self = 1ull << (i - b[1]);
+    m1 &= ~self;  // I != J
+    int cnt = 0;
+    if (COUNT) {
+        if (adj & ~keep_tags) {  // some adjacent tags are counted but not visited
+            cnt = __popcll(accept_row(w, src, 0, b[0], e[0], xi, gi.z, adj)) +
+                  __popcll(accept_row(w, src, 1, b[1], e[1], xi, gi.z, adj) & ~self) +
+                  __popcll(accept_row(w, src, 2, b[2], e[2], xi, gi.z, adj));
+        } else {
+            cnt = __popcll(m0) + __popcll(m1) + __popcll(m2);
+        }
+    }
+    auto visit = [&](int j, int roff) {
+        const float4 gj = src.get(j + roff);
         const float dx = xi.x - gj.x, dy = xi.y - gj.y;
-        pair(j, tj, gj.z, dx, dy, dx * dx + dy * dy);
+        pair(j, tag_of(gj), gj.z, dx, dy, dx * dx + dy * dy);
     };
     if (ORDERED) {
         for (int s9 = 0; s9 < 9; ++s9) {
@@ -262,28 +298,73 @@ __device__ __forceinline__ int walk_rows(const Walk &w, int i, float4 gi, Keep &
             const int lo = cb - sel3(r, b[0], b[1], b[2]);  // < 64: the cell lies inside its row range
             unsigned long long m = sel3(r, m0, m1, m2) >> lo;
             if (len < 64) m &= (1ull << len) - 1ull;
+            const int roff = src.off(r);
             while (m) {
                 const int j = cb + __builtin_ctzll(m);
                 m &= m - 1ull;
-                visit(j);
+                visit(j, roff);
             }
         }
     } else {
-        int r = 0, base = b[0];
+        int r = 0, base = b[0], roff = src.off(0);
         unsigned long long m = m0;
         while (true) {
             while (!m && r < 2) {
                 ++r;
                 m = sel3(r, m0, m1, m2);
                 base = sel3(r, b[0], b[1], b[2]);
+                roff = src.off(r);
             }
             if (!m) break;
             const int j = base + __builtin_ctzll(m);
             m &= m - 1ull;
-            visit(j);
+            visit(j, roff);
         }
     }
     return cnt;
+}
+
+// Stages the geometry of a pair block's neighbourhood in LDS.  The block's
+// sorted slots [i0, i0 + kPairBlock) lie in cells c0 <= c <= c1; when these
+// share one cell row, the three neighbour rows of every particle of the
+// block are sub-ranges of the rows' slot ranges over cells c0 - 1 .. c1 + 1,
+// which are copied to tile (kTileRow slots per row).  Returns false, with
+// nothing staged, when the block spans two cell rows or a row range does not
+// fit; the block then reads global memory.  Every thread of the block calls
+// it (two barriers).
+__device__ __forceinline__ bool stage_rows(const Walk &w, int i0, int n, float4 *tile, int *info, TileRows &rows) {
+    const int t = threadIdx.x;
+    if (t < 3) {
+        const int nx = w.g.n_x;
+        const int c0 = w.cell[i0] - 1, c1 = w.cell[min(i0 + kPairBlock, n) - 1] - 1;
+        const int cy = c0 / nx, jj = cy + t - 1;
+        int b = 0, len = 0, ok = c1 / nx == cy;
+        if (ok && jj >= 0 && jj < w.g.n_y) {
+            b = w.start[max(c0 % nx - 1, 0) + nx * jj];
+            len = w.start[min(c1 % nx + 1, nx - 1) + 1 + nx * jj] - b;
+            ok = len <= kTileRow;
+        }
+        info[t] = b;
+        info[3 + t] = len;
+        info[6 + t] = ok;
+    }
+    __syncthreads();
+    const bool ok = info[6] && info[7] && info[8];
+    if (ok) {
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            const int b = info[r], len = info[3 + r];
+            for (int k = t; k < len; k += kPairBlock) tile[r * kTileRow + k] = w.geo[b + k];
+        }
+        rows.tile = tile;
+        rows.off0 = -info[0];
+        rows.off1 = kTileRow - info[1];
+        rows.off2 = 2 * kTileRow - info[2];
+    } else if (t == 0) {
+        atomicAdd(w.fallback, 1u);
+    }
+    __syncthreads();
+    return ok;
 }
 
 __device__ __forceinline__ int block_sum_i32(int v, int *lds) {
@@ -485,9 +566,8 @@ __global__ __launch_bounds__(kBlock) void k_permute(Fields cur, Fields nxt, floa
     const int32_t tag = cur.tag[s];
     nxt.pos[dest] = x;
     geo[dest] = make_float4(x.x, x.y, h, __int_as_float(tag));
-    nxt.vel[dest] = cur.vel[s];
+    nxt.dyn[dest] = cur.dyn[s];
     nxt.dvel[dest] = cur.dvel[s];
-    nxt.rp[dest] = cur.rp[s];
     nxt.mass[dest] = cur.mass[s];
     nxt.vol[dest] = cur.vol[s];
     nxt.drho[dest] = cur.drho[s];
@@ -533,27 +613,36 @@ __global__ __launch_bounds__(kPairBlock) void k_search(Walk w, Fields f, Physics
                                                         unsigned long long *pairs_last,
                                                         unsigned long long *pairs_total, uint32_t *flags) {
     __shared__ int lds[kPairBlock / 64];
-    const int i = blockIdx.x * kPairBlock + threadIdx.x;
-    const bool active = i < *w.n_alive;
+    __shared__ float4 tile[3 * kTileRow];
+    __shared__ int info[9];
+    const int n = *w.n_alive;
+    const int i0 = blockIdx.x * kPairBlock;
+    if (i0 >= n) return;  // the whole block
+    const int i = i0 + threadIdx.x;
+    const bool active = i < n;
+    TileRows rows;
+    const bool tiled = ACTION == EPPX_SEARCH_KERNEL_CONTINUITY && stage_rows(w, i0, n, tile, info, rows);
     int cnt = 0;
     if (active) {
         const float4 gi = w.geo[i];
         const int ti = tag_of(gi);
         const bool i_fluid = ti == ph.fluid_tag, i_wall = ti == ph.wall_tag;
         if (ACTION == EPPX_SEARCH_KERNEL_CONTINUITY) {
-            const float2 ui = f.vel[i];
-            const float rho_i = f.rp[i].x;
+            const float4 di = f.dyn[i];
+            const float2 ui = make_float2(di.x, di.y);
+            const float rho_i = di.z;
             float acc = f.drho[i];
             // pair classes of the continuity action: fluid-fluid (classic), fluid-wall / wall-fluid (balanced)
-            auto keep = [&](int tj) {
-                return (i_fluid && (tj == ph.fluid_tag || tj == ph.wall_tag)) || (i_wall && tj == ph.fluid_tag);
-            };
+            const uint32_t keep = i_fluid ? tag_bit(ph.fluid_tag) | tag_bit(ph.wall_tag)
+                                          : (i_wall ? tag_bit(ph.fluid_tag) : 0u);
+            const float2 *uv = reinterpret_cast<const float2 *>(f.dyn);  // (u, v) of slot j at uv[2 j]
             auto term = [&](int j, int tj, float hj, float dx, float dy, float dr2) {
                 const bool ff = i_fluid && tj == ph.fluid_tag;
                 const float m = ff ? f.mass[j] : rho_i * f.vol[j];
-                acc += continuity_term<FAST, K>(gi.z, hj, ui, f.vel[j], m, dx, dy, dr2);
+                acc += continuity_term<FAST, K>(gi.z, hj, ui, uv[2 * (size_t)j], m, dx, dy, dr2);
             };
-            cnt = walk_rows<!FAST>(w, i, gi, keep, term);
+            if (tiled) cnt = walk_rows<!FAST, true>(w, rows, i, gi, keep, term);
+            else cnt = walk_rows<!FAST, true>(w, GlobalRows{w.geo}, i, gi, keep, term);
             f.drho[i] = acc;
         } else {
             walk(w, i, gi, [&](int, int, float, float, float) { ++cnt; });
@@ -641,20 +730,29 @@ __device__ __forceinline__ void momentum_pair(const Physics &ph, float2 &acc, bo
 
 template <bool FAST, int K, int VISC>
 __global__ __launch_bounds__(kPairBlock) void k_momentum(Walk w, Fields f, Physics ph) {
-    const int i = blockIdx.x * kPairBlock + threadIdx.x;
-    if (i >= *w.n_alive) return;
+    __shared__ float4 tile[3 * kTileRow];
+    __shared__ int info[9];
+    const int n = *w.n_alive;
+    const int i0 = blockIdx.x * kPairBlock;
+    if (i0 >= n) return;  // the whole block
+    TileRows rows;
+    const bool tiled = stage_rows(w, i0, n, tile, info, rows);
+    const int i = i0 + threadIdx.x;
+    if (i >= n) return;
     const float4 gi = w.geo[i];
     if (tag_of(gi) != ph.fluid_tag) return;  // iMomentum! acts on fluid particles only
-    const float2 ui = f.vel[i], rp_i = f.rp[i];
+    const float4 di = f.dyn[i];
+    const float2 ui = make_float2(di.x, di.y);
     float2 acc = f.dvel[i];
-    const float pr_i = FAST ? rp_i.y * frcp(rp_i.x * rp_i.x) : 0.0f;
-    walk_rows<!FAST>(
-        w, i, gi, [&](int tj) { return tj == ph.fluid_tag || tj == ph.wall_tag; },
-        [&](int j, int tj, float hj, float dx, float dy, float dr2) {
-            const float2 rp_j = f.rp[j];
-            momentum_pair<FAST, K, VISC>(ph, acc, tj == ph.fluid_tag, gi.z, hj, ui, rp_i.x, rp_i.y, pr_i, f.vel[j],
-                                         rp_j.x, rp_j.y, f.mass[j], dx, dy, dr2);
-        });
+    const float pr_i = FAST ? di.w * frcp(di.z * di.z) : 0.0f;
+    auto term = [&](int j, int tj, float hj, float dx, float dy, float dr2) {
+        const float4 dj = f.dyn[j];
+        momentum_pair<FAST, K, VISC>(ph, acc, tj == ph.fluid_tag, gi.z, hj, ui, di.z, di.w, pr_i,
+                                     make_float2(dj.x, dj.y), dj.z, dj.w, f.mass[j], dx, dy, dr2);
+    };
+    const uint32_t keep = tag_bit(ph.fluid_tag) | tag_bit(ph.wall_tag);
+    if (tiled) walk_rows<!FAST, false>(w, rows, i, gi, keep, term);
+    else walk_rows<!FAST, false>(w, GlobalRows{w.geo}, i, gi, keep, term);
     f.dvel[i] = acc;
 }
 
@@ -662,6 +760,20 @@ __global__ __launch_bounds__(kPairBlock) void k_momentum(Walk w, Fields f, Physi
 // Action.jl:121-138), walking global memory:
 //  EPPX_INTER_FILTER: iFilter main.jl:128-134 -> iKernelFilter! (Filter.jl:10-22).
 //  EPPX_INTER_ADD_PER_NEIGHBOUR: field[I] += value per neighbour.
+// Component op.component of a named field as a strided float column.
+__device__ __forceinline__ float *field_column(const Fields &f, const eppx_field_op &op, int &stride) {
+    switch (op.field) {
+        case EPPX_FIELD_POSITION: stride = 2; return reinterpret_cast<float *>(f.pos) + op.component;
+        case EPPX_FIELD_VELOCITY: stride = 4; return reinterpret_cast<float *>(f.dyn) + op.component;
+        case EPPX_FIELD_DVELOCITY: stride = 2; return reinterpret_cast<float *>(f.dvel) + op.component;
+        case EPPX_FIELD_MASS: stride = 1; return f.mass;
+        case EPPX_FIELD_VOLUME: stride = 1; return f.vol;
+        case EPPX_FIELD_DENSITY: stride = 4; return reinterpret_cast<float *>(f.dyn) + 2;
+        case EPPX_FIELD_DDENSITY: stride = 1; return f.drho;
+        default: stride = 4; return reinterpret_cast<float *>(f.dyn) + 3;  // EPPX_FIELD_PRESSURE
+    }
+}
+
 template <int ACTION>
 __global__ __launch_bounds__(kBlock) void k_interaction(Walk w, Fields f, Physics ph, eppx_field_op op) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
@@ -681,18 +793,8 @@ __global__ __launch_bounds__(kBlock) void k_interaction(Walk w, Fields f, Physic
         f.wv[i] = wv;
         f.wvrho[i] = wvrho;
     } else {
-        float *col;
-        int stride = 1;
-        switch (op.field) {
-            case EPPX_FIELD_POSITION: col = reinterpret_cast<float *>(f.pos) + op.component; stride = 2; break;
-            case EPPX_FIELD_VELOCITY: col = reinterpret_cast<float *>(f.vel) + op.component; stride = 2; break;
-            case EPPX_FIELD_DVELOCITY: col = reinterpret_cast<float *>(f.dvel) + op.component; stride = 2; break;
-            case EPPX_FIELD_MASS: col = f.mass; break;
-            case EPPX_FIELD_VOLUME: col = f.vol; break;
-            case EPPX_FIELD_DENSITY: col = reinterpret_cast<float *>(f.rp); stride = 2; break;
-            case EPPX_FIELD_DDENSITY: col = f.drho; break;
-            default: col = reinterpret_cast<float *>(f.rp) + 1; stride = 2; break;
-        }
+        int stride;
+        float *col = field_column(f, op, stride);
         float v = col[(int64_t)i * stride];
         walk(w, i, gi, [&](int, int, float, float, float) { v += op.value; });
         col[(int64_t)i * stride] = v;
@@ -714,22 +816,26 @@ __global__ __launch_bounds__(kBlock) void k_self(Fields f, int32_t n, Physics ph
     const int tag = f.tag[s];
     if (ACTION == EPPX_SELF_CONTINUITY_EOS) {
         if (tag != ph.fluid_tag && tag != ph.wall_tag) return;
-        const float rho = f.rp[s].x + f.drho[s] * ph.dt;
+        float4 d = f.dyn[s];
+        const float rho = d.z + f.drho[s] * ph.dt;
         f.drho[s] = 0.0f;
         f.vol[s] = f.mass[s] / rho;
-        f.rp[s] = make_float2(rho, eos(ph, rho));
+        d.z = rho;
+        d.w = eos(ph, rho);
+        f.dyn[s] = d;
     } else if (ACTION == EPPX_SELF_GRAVITY_MOVE) {
         if (tag != ph.fluid_tag) return;
         float2 du = f.dvel[s];
         du.x += ph.gx;
         du.y += ph.gy;
-        float2 x = f.pos[s], u = f.vel[s];
-        x.x += (u.x + du.x * ph.dt * 0.5f) * ph.dt;
-        u.x += du.x * ph.dt;
-        x.y += (u.y + du.y * ph.dt * 0.5f) * ph.dt;
-        u.y += du.y * ph.dt;
+        float2 x = f.pos[s];
+        float4 d = f.dyn[s];
+        x.x += (d.x + du.x * ph.dt * 0.5f) * ph.dt;
+        d.x += du.x * ph.dt;
+        x.y += (d.y + du.y * ph.dt * 0.5f) * ph.dt;
+        d.y += du.y * ph.dt;
         f.pos[s] = x;
-        f.vel[s] = u;
+        f.dyn[s] = d;
         f.dvel[s] = make_float2(0.0f, 0.0f);
     } else if (ACTION == EPPX_SELF_FILTER) {
         if (tag != ph.fluid_tag) return;
@@ -741,20 +847,13 @@ __global__ __launch_bounds__(kBlock) void k_self(Fields f, int32_t n, Physics ph
         f.wv[s] = 0.0f;
         f.wvrho[s] = 0.0f;
         f.vol[s] = f.mass[s] / rho;
-        f.rp[s] = make_float2(rho, eos(ph, rho));
+        float4 d = f.dyn[s];
+        d.z = rho;
+        d.w = eos(ph, rho);
+        f.dyn[s] = d;
     } else {
-        float *col;
-        int stride = 1;
-        switch (op.field) {
-            case EPPX_FIELD_POSITION: col = reinterpret_cast<float *>(f.pos) + op.component; stride = 2; break;
-            case EPPX_FIELD_VELOCITY: col = reinterpret_cast<float *>(f.vel) + op.component; stride = 2; break;
-            case EPPX_FIELD_DVELOCITY: col = reinterpret_cast<float *>(f.dvel) + op.component; stride = 2; break;
-            case EPPX_FIELD_MASS: col = f.mass; break;
-            case EPPX_FIELD_VOLUME: col = f.vol; break;
-            case EPPX_FIELD_DENSITY: col = reinterpret_cast<float *>(f.rp); stride = 2; break;
-            case EPPX_FIELD_DDENSITY: col = f.drho; break;
-            default: col = reinterpret_cast<float *>(f.rp) + 1; stride = 2; break;
-        }
+        int stride;
+        float *col = field_column(f, op, stride);
         col[(int64_t)s * stride] *= op.value;
     }
 }
@@ -810,9 +909,9 @@ __global__ __launch_bounds__(kBlock) void k_from_ref(Fields f, int32_t cap, Colu
     auto fcol = [&](int col) { return col >= 0 ? fp[s + C * col] : 0.0f; };
     auto icol = [&](int col) { return col >= 0 ? ip[s + C * col] : 0; };
     f.pos[s] = make_float2(fcol(c.PositionVec), c.PositionVec >= 0 ? fcol(c.PositionVec + 1) : 0.0f);
-    f.vel[s] = make_float2(fcol(c.VelocityVec), c.VelocityVec >= 0 ? fcol(c.VelocityVec + 1) : 0.0f);
+    f.dyn[s] = make_float4(fcol(c.VelocityVec), c.VelocityVec >= 0 ? fcol(c.VelocityVec + 1) : 0.0f,
+                           fcol(c.Density), fcol(c.Pressure));
     f.dvel[s] = make_float2(fcol(c.dVelocityVec), c.dVelocityVec >= 0 ? fcol(c.dVelocityVec + 1) : 0.0f);
-    f.rp[s] = make_float2(fcol(c.Density), fcol(c.Pressure));
     f.mass[s] = fcol(c.Mass);
     f.vol[s] = fcol(c.Volume);
     f.drho[s] = fcol(c.dDensity);
@@ -842,11 +941,11 @@ __global__ __launch_bounds__(kBlock) void k_to_ref(Fields f, int32_t cap, Column
         }
     };
     put2(c.PositionVec, f.pos[s]);
-    put2(c.VelocityVec, f.vel[s]);
+    const float4 d = f.dyn[s];
+    put2(c.VelocityVec, make_float2(d.x, d.y));
     put2(c.dVelocityVec, f.dvel[s]);
-    const float2 rp = f.rp[s];
-    put(c.Density, rp.x);
-    put(c.Pressure, rp.y);
+    put(c.Density, d.z);
+    put(c.Pressure, d.w);
     put(c.Mass, f.mass[s]);
     put(c.Volume, f.vol[s]);
     put(c.dDensity, f.drho[s]);
@@ -879,8 +978,9 @@ __global__ __launch_bounds__(kBlock) void k_stencil(Grid g, int32_t *count, int3
 __global__ __launch_bounds__(kBlock) void k_finite(Fields f, int32_t n, uint32_t *bad) {
     const int s = blockIdx.x * kBlock + threadIdx.x;
     if (s >= n || f.alive[s] != 1) return;
-    const float2 x = f.pos[s], u = f.vel[s], rp = f.rp[s];
-    if (!isfinite(rp.x) || !isfinite(rp.y) || !isfinite(x.x) || !isfinite(x.y) || !isfinite(u.x) || !isfinite(u.y))
+    const float2 x = f.pos[s];
+    const float4 d = f.dyn[s];
+    if (!isfinite(d.z) || !isfinite(d.w) || !isfinite(x.x) || !isfinite(x.y) || !isfinite(d.x) || !isfinite(d.y))
         atomicOr(bad, 1u);
 }
 
