@@ -9,6 +9,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <map>
 #include <string>
@@ -60,7 +61,9 @@ struct eppx_ps {
     unsigned long long *pairs = nullptr;  // [0] last search, [1] running total
     uint32_t *fallback = nullptr;         // pair blocks that walked global memory
     float *h_range = nullptr;
-    float rmin2 = -1.0f, rmax2 = INFINITY;
+    float rmin2 = -1.0f, rmax2 = INFINITY;  // symmetry-criterion bounds from the H range
+    uint32_t tags_present = 0xffffffffu;    // tag bits of the uploaded particles
+    bool odd_tags = true;                   // some tag outside 1..kMaxTag
     // search state
     eppx_ns *last_ns = nullptr;
     int32_t last_crit = 0, last_action = 0, last_kernel = 0, last_fluid = 0, last_wall = 0;
@@ -233,8 +236,16 @@ Walk make_walk(const eppx_ps *ps, const eppx_ns *ns, int32_t criterion) {
     w.n_alive = ps->n_alive;
     memcpy(w.adj, ns->adj, sizeof(w.adj));
     w.criterion = criterion;
-    w.rmin2 = ps->rmin2;
-    w.rmax2 = ps->rmax2;
+    if (criterion == EPPX_CRIT_DEFAULT) {
+        const float gs = ns->grid.gap_square;  // accept dr2 <= gap_square, i.e. dr2 < next float up
+        w.rmin2 = std::isfinite(gs) ? nextafterf(gs, INFINITY) : -1.0f;
+        w.rmax2 = std::isfinite(gs) ? -INFINITY : INFINITY;
+    } else {
+        w.rmin2 = ps->rmin2;
+        w.rmax2 = ps->rmax2;
+    }
+    w.tags_present = ps->tags_present;
+    w.check_all = (w.rmin2 <= w.rmax2) || ps->odd_tags;
     return w;
 }
 
@@ -258,6 +269,8 @@ int32_t physics_arg(eppx_ctx *ctx, const void *params, size_t size, Physics *out
     if (p->eos != EPPX_EOS_LINEAR && p->eos != EPPX_EOS_TAIT) return set_error(ctx, EPPX_E_ARG, "unknown eos");
     if (p->visc != EPPX_VISC_CLASSIC && p->visc != EPPX_VISC_ARTIFICIAL)
         return set_error(ctx, EPPX_E_ARG, "unknown viscosity");
+    if (p->fluid_tag < 1 || p->fluid_tag > kMaxTag || p->wall_tag < 1 || p->wall_tag > kMaxTag)
+        return set_error(ctx, EPPX_E_ARG, "fluid_tag and wall_tag must lie in 1..32");
     *out = to_physics(*p);
     return EPPX_OK;
 }
@@ -495,7 +508,7 @@ int32_t eppx_ps_create(eppx_ctx *ctx, int32_t n_capacity, const eppx_layout *lay
     if (e == hipSuccess) e = dalloc(&ps->ncount, (size_t)n_capacity);
     if (e == hipSuccess) e = dalloc(&ps->n_alive, 1);
     if (e == hipSuccess) e = dalloc(&ps->pairs, 2);
-    if (e == hipSuccess) e = dalloc(&ps->h_range, 3);
+    if (e == hipSuccess) e = dalloc(&ps->h_range, 5);
     if (e == hipSuccess) e = hipMemsetAsync(ps->pairs, 0, 2 * sizeof(unsigned long long), ctx->stream);
     if (e == hipSuccess) e = hipMemsetAsync(ps->n_alive, 0, sizeof(int32_t), ctx->stream);
     if (e != hipSuccess) {
@@ -556,11 +569,12 @@ int32_t eppx_ps_upload(eppx_ps *ps, int32_t n_particles, const int32_t *is_alive
     }
     ps->cur = 0;
     ps->n = n_particles;
-    // H range over alive particles for the criterion's exact-safe bounds.
-    const float init[3] = {INFINITY, 0.0f, 0.0f};
-    float range[3] = {0.0f, 0.0f, 0.0f};
+    // H range over alive particles for the criterion's exact-safe bounds, and
+    // the tags present.
+    const float init[5] = {INFINITY, 0.0f, 0.0f, 0.0f, 0.0f};
+    float range[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     if (e == hipSuccess) e = hipMemcpyAsync(ps->h_range, init, sizeof(init), hipMemcpyHostToDevice, s);
-    if (e == hipSuccess && has(L.H) && n_particles) e = launch_h_range(ps->buf[0], n_particles, ps->h_range, s);
+    if (e == hipSuccess && n_particles) e = launch_h_range(ps->buf[0], n_particles, ps->h_range, s);
     if (e == hipSuccess) e = hipMemcpyAsync(range, ps->h_range, sizeof(range), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     dfree(d_ip);
@@ -568,12 +582,24 @@ int32_t eppx_ps_upload(eppx_ps *ps, int32_t n_particles, const int32_t *is_alive
     dfree(d_alive);
     dfree(d_cell);
     if (e != hipSuccess) return hip_fail(ctx, e, "eppx_ps_upload");
-    uint32_t bad;
+    uint32_t bad, odd;
     memcpy(&bad, &range[2], sizeof(bad));
+    memcpy(&ps->tags_present, &range[3], sizeof(uint32_t));
+    memcpy(&odd, &range[4], sizeof(odd));
+    ps->odd_tags = odd != 0;
     const double hmin = range[0], hmax = range[1];
-    if (bad || !(hmin >= 1e-30) || !(hmax <= 1e30) || !(hmin <= hmax)) {
+    if (bad || !has(L.H) || !(hmin >= 1e-30) || !(hmax <= 1e30) || !(hmin <= hmax)) {
         ps->rmin2 = -1.0f;
         ps->rmax2 = INFINITY;
+    } else if (hmin == hmax) {
+        // One H for all: the criterion compares dr2 with one threshold,
+        // evaluated here with the device's operations (Mean.harmonic, then
+        // *2 and squared; IEEE f32, no contraction).
+        const float h = range[0];
+        const float h2 = 2.0f * h * h / (h + h) * 2.0f;
+        const float t = h2 * h2;
+        ps->rmin2 = std::isfinite(t) ? nextafterf(t, INFINITY) : -1.0f;
+        ps->rmax2 = std::isfinite(t) ? -INFINITY : INFINITY;
     } else {
         const double eps = 1.0 / 1048576.0;  // 2^-20, well above the criterion's rounding
         ps->rmin2 = nextafterf((float)(4.0 * hmin * hmin * (1.0 - eps)), 0.0f);

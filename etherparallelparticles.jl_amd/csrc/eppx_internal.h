@@ -46,7 +46,9 @@ struct Walk {
     const int32_t *n_alive;   // device scalar
     uint32_t adj[kMaxTag + 1];  // adj[t]: bit (u-1) set when tag t is adjacent to tag u
     int32_t criterion;
-    float rmin2, rmax2;       // symmetry-criterion accept / reject bounds (exact-safe)
+    float rmin2, rmax2;       // criterion accepts every dr2 < rmin2, rejects every dr2 > rmax2
+    uint32_t tags_present;    // tag bits of the particles (a superset)
+    int32_t check_all;        // the band is not empty, or some tag lies outside 1..kMaxTag
     uint32_t *fallback;       // blocks that walked global memory instead of LDS (counter)
 };
 

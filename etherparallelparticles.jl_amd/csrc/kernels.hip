@@ -32,7 +32,6 @@ constexpr float kSigmaWendland = (float)(7.0 / 4.0 / 3.14159265358979323846);
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
-__device__ __forceinline__ int tag_of(float4 g) { return __float_as_int(g.w); }
 
 // Mean.harmonic / invharmonic (src/Math/Mean.jl:28-34)
 __device__ __forceinline__ float harmonic(float a, float b) { return 2.0f * a * b / (a + b); }
@@ -110,24 +109,33 @@ __device__ __forceinline__ float eos(const Physics &ph, float rho) {
     return ph.tait_b * (t * t * x - 1.0f) + ph.p0;
 }
 
-// The accept / reject decision of device_findNeighbourParticlesFromCells!
-// (NeighbourSearch.jl:199-214) for candidate geometry gj seen from (xi, hi):
-// dx = x_I - x_J, dr2 = dx*dx + dy*dy and the criterion.  Exact in every mode.
-__device__ __forceinline__ bool accept_pair(const Walk &w, float2 xi, float hi, float4 gj, float &dx, float &dy,
-                                            float &dr2) {
-    dx = xi.x - gj.x;
-    dy = xi.y - gj.y;
-    dr2 = dx * dx + dy * dy;
-    if (w.criterion == EPPX_CRIT_DEFAULT) return dr2 <= w.g.gap_square;  // defaultSearchCriterion
-    if (dr2 > w.rmax2) return false;  // beyond (2 max h)^2 with margin: rejected by the exact test too
-    if (dr2 < w.rmin2) return true;   // inside (2 min h)^2 with margin: accepted by the exact test too
-    const float h2 = harmonic(hi, gj.z) * 2.0f;  // symmetrySearchCriterion
-    return dr2 <= h2 * h2;
+// The search snapshot geo = (x, y, h, tag bit) stores tag t as the bit
+// 1 << (t - 1); a tag outside 1..kMaxTag gets 0, which no tag is adjacent to.
+__device__ __forceinline__ uint32_t tag_bit(int t) {
+    const uint32_t t1 = (uint32_t)t - 1u;
+    return t1 < (uint32_t)kMaxTag ? 1u << t1 : 0u;
+}
+__device__ __forceinline__ uint32_t tbit_of(float4 g) { return __float_as_uint(g.w); }
+// ActivePair row (adjacency bits) of the tag whose bit is tb.
+__device__ __forceinline__ uint32_t adj_of(const Walk &w, uint32_t tb) {
+    return tb ? w.adj[__builtin_ctz(tb) + 1] : 0u;
 }
 
-__device__ __forceinline__ bool adjacent(uint32_t adj_i, int tj) {
-    const uint32_t t1 = (uint32_t)tj - 1u;
-    return (t1 < (uint32_t)kMaxTag) & ((adj_i >> (t1 & 31u)) & 1u);
+// The accept / reject decision of device_findNeighbourParticlesFromCells!
+// (NeighbourSearch.jl:199-214) for a candidate at dr2 = dx*dx + dy*dy
+// (dx = x_I - x_J; two roundings, no FMA).  Exact in every mode: the host
+// sets rmin2 / rmax2 so that the criterion accepts every dr2 < rmin2 and
+// rejects every dr2 > rmax2 (rmax2 < rmin2 when that decides all of them);
+// only dr2 in the band between them evaluates the criterion itself.
+__device__ __forceinline__ bool criterion_exact(const Walk &w, float hi, float hj, float dr2) {
+    if (w.criterion == EPPX_CRIT_DEFAULT) return dr2 <= w.g.gap_square;  // defaultSearchCriterion
+    const float h2 = harmonic(hi, hj) * 2.0f;                            // symmetrySearchCriterion
+    return dr2 <= h2 * h2;
+}
+__device__ __forceinline__ bool accept_dr2(const Walk &w, float hi, float hj, float dr2) {
+    if (dr2 < w.rmin2) return true;
+    if (!(dr2 <= w.rmax2)) return false;
+    return criterion_exact(w, hi, hj, dr2);
 }
 
 // ------------------------------------------------------------------ walk
@@ -135,12 +143,12 @@ __device__ __forceinline__ bool adjacent(uint32_t adj_i, int tj) {
 // the reference's list order: neighbour cells in device_initializeNeighbourSystem!
 // order (x offset outer, y offset inner; NeighbourSystemBase.jl:37-47), then
 // ascending particle index inside a cell (the order of a serial insert).
+// visit(j, tag bit of j, dx, dy, dr2).
 template <class Visit>
 __device__ __forceinline__ void walk(const Walk &w, int i, float4 gi, Visit &&visit) {
     const int c = w.cell[i] - 1;
     const int cx = c % w.g.n_x, cy = c / w.g.n_x;
-    const int ti = tag_of(gi);
-    const uint32_t adj = (ti >= 1 && ti <= kMaxTag) ? w.adj[ti] : 0u;
+    const uint32_t adj = adj_of(w, tbit_of(gi));
     const float2 xi = make_float2(gi.x, gi.y);
     for (int di = -1; di <= 1; ++di) {
         const int ii = cx + di;
@@ -153,10 +161,11 @@ __device__ __forceinline__ void walk(const Walk &w, int i, float4 gi, Visit &&vi
             for (int j = b; j < e; ++j) {
                 if (j == i) continue;
                 const float4 gj = w.geo[j];
-                const int tj = tag_of(gj);
-                if (!adjacent(adj, tj)) continue;
-                float dx, dy, dr2;
-                if (accept_pair(w, xi, gi.z, gj, dx, dy, dr2)) visit(j, tj, dx, dy, dr2);
+                const uint32_t tbj = tbit_of(gj);
+                if (!(adj & tbj)) continue;
+                const float dx = xi.x - gj.x, dy = xi.y - gj.y;
+                const float dr2 = dx * dx + dy * dy;
+                if (accept_dr2(w, gi.z, gj.z, dr2)) visit(j, tbj, dx, dy, dr2);
             }
         }
     }
@@ -166,13 +175,13 @@ __device__ __forceinline__ void walk(const Walk &w, int i, float4 gi, Visit &&vi
 // Two passes over the 3x3 cell neighbourhood of slot i.
 //  Pass 1: the three cells of each neighbour row are contiguous in the
 //    cell-sorted order, so each row is one slot range [b_r, e_r) whose
-//    candidates are tested (one 16-byte load each) into a 64-bit accept mask.
+//    candidates are tested, four at a time, into a 64-bit accept mask.
 //  Pass 2: `pair` runs for every accepted neighbour whose tag is in
-//    keep_tags (bit t-1 for tag t; the tag filter is folded into the pass-1
-//    masks, so pass 2 issues its loads at once).  ORDERED visits them in the
-//    reference's list order (x offset outer, y offset inner, ascending slot
-//    inside a cell), so sums are bit-identical to the reference; otherwise
-//    row by row, in one loop that keeps every lane busy.
+//    keep_tags (the tag filter is folded into the pass-1 masks, so pass 2
+//    issues its loads at once).  ORDERED visits them in the reference's list
+//    order (x offset outer, y offset inner, ascending slot inside a cell), so
+//    sums are bit-identical to the reference; otherwise row by row, in one
+//    loop that keeps every lane busy.
 // A lane whose row range exceeds 64 slots walks global memory one candidate
 // at a time instead.  With COUNT, returns the number of accepted neighbours
 // of every tag (nCount); otherwise 0.
@@ -182,30 +191,33 @@ __device__ __forceinline__ unsigned long long sel3(int r, unsigned long long a, 
 }
 __device__ __forceinline__ int sel3(int r, int a, int b, int c) { return r == 0 ? a : (r == 1 ? b : c); }
 
-// Accept bit of one candidate: adjacency and criterion without branches on
-// the common path; only candidates inside the symmetry criterion's tie band
-// take the exact harmonic-mean test (a rarely taken branch).
-__device__ __forceinline__ uint32_t tag_bit(int t) {
-    const uint32_t t1 = (uint32_t)t - 1u;
-    return t1 < (uint32_t)kMaxTag ? 1u << t1 : 0u;
-}
-
-__device__ __forceinline__ uint32_t accept_bit(const Walk &w, float2 xi, float hi, uint32_t adj, float4 gj) {
-    const uint32_t t1 = (uint32_t)tag_of(gj) - 1u;
-    const uint32_t adj_ok = (t1 < (uint32_t)kMaxTag) & (adj >> (t1 & 31u));
-    const float dx = xi.x - gj.x, dy = xi.y - gj.y;
-    const float dr2 = dx * dx + dy * dy;
-    uint32_t ok;
-    if (w.criterion == EPPX_CRIT_DEFAULT) {
-        ok = dr2 <= w.g.gap_square;
-    } else {
-        ok = dr2 < w.rmin2;
-        if (!ok && dr2 <= w.rmax2) {
-            const float h2 = harmonic(hi, gj.z) * 2.0f;
-            ok = dr2 <= h2 * h2;
-        }
+// Accept bits of N candidates g[0..N) (bit k for candidate k).  Without
+// CHECK only dr2 < rmin2 is tested: the caller has established that the
+// band is empty and every candidate tag is adjacent.  With CHECK, candidates
+// in the band take the exact criterion (one branch per group, rarely taken)
+// and non-adjacent tags are cleared.
+template <bool CHECK, int N>
+__device__ __forceinline__ uint32_t accept_bits(const Walk &w, float2 xi, float hi, uint32_t adj,
+                                                const float4 (&g)[N]) {
+    float d[N];
+    uint32_t ok = 0u, band = 0u;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        const float dx = xi.x - g[k].x, dy = xi.y - g[k].y;
+        d[k] = dx * dx + dy * dy;
+        ok |= (uint32_t)(d[k] < w.rmin2) << k;
+        if (CHECK) band |= (uint32_t)(d[k] >= w.rmin2 && d[k] <= w.rmax2) << k;
     }
-    return ok & adj_ok & 1u;
+    if (CHECK) {
+        if (band) {
+#pragma unroll
+            for (int k = 0; k < N; ++k)
+                if ((band >> k) & 1u) ok |= (uint32_t)criterion_exact(w, hi, g[k].z, d[k]) << k;
+        }
+#pragma unroll
+        for (int k = 0; k < N; ++k) ok &= ~((uint32_t)((adj & tbit_of(g[k])) == 0u) << k);
+    }
+    return ok;
 }
 
 // Candidate geometry of the row walk: slot j of neighbour row r, read from
@@ -224,18 +236,19 @@ struct TileRows {
     __device__ __forceinline__ float4 at(int r, int j) const { return get(j + off(r)); }
 };
 
-template <class Src>
+template <bool CHECK, class Src>
 __device__ __forceinline__ unsigned long long accept_row(const Walk &w, const Src &src, int r, int b, int e,
                                                          float2 xi, float hi, uint32_t adj) {
     unsigned long long m = 0ull;
     int j = b;
     for (; j + 4 <= e; j += 4) {  // four independent loads in flight per iteration
-        const float4 g0 = src.at(r, j), g1 = src.at(r, j + 1), g2 = src.at(r, j + 2), g3 = src.at(r, j + 3);
-        const uint32_t bits = accept_bit(w, xi, hi, adj, g0) | (accept_bit(w, xi, hi, adj, g1) << 1) |
-                              (accept_bit(w, xi, hi, adj, g2) << 2) | (accept_bit(w, xi, hi, adj, g3) << 3);
-        m |= (unsigned long long)bits << (j - b);
+        const float4 g[4] = {src.at(r, j), src.at(r, j + 1), src.at(r, j + 2), src.at(r, j + 3)};
+        m |= (unsigned long long)accept_bits<CHECK, 4>(w, xi, hi, adj, g) << (j - b);
     }
-    for (; j < e; ++j) m |= (unsigned long long)accept_bit(w, xi, hi, adj, src.at(r, j)) << (j - b);
+    for (; j < e; ++j) {
+        const float4 g[1] = {src.at(r, j)};
+        m |= (unsigned long long)accept_bits<CHECK, 1>(w, xi, hi, adj, g) << (j - b);
+    }
     return m;
 }
 
@@ -258,27 +271,36 @@ __device__ __forceinline__ int walk_rows(const Walk &w, const Src &src, int i, f
     }
     if (e[0] - b[0] > 64 || e[1] - b[1] > 64 || e[2] - b[2] > 64) {
         int cnt = 0;
-        walk(w, i, gi, [&](int j, int tj, float dx, float dy, float dr2) {
+        walk(w, i, gi, [&](int j, uint32_t tbj, float dx, float dy, float dr2) {
             ++cnt;
-            if (tag_bit(tj) & keep_tags) pair(j, tj, w.geo[j].z, dx, dy, dr2);
+            if (tbj & keep_tags) pair(j, tbj, w.geo[j].z, dx, dy, dr2);
         });
         return COUNT ? cnt : 0;
     }
-    const int ti = tag_of(gi);
-    const uint32_t adj = (ti >= 1 && ti <= kMaxTag) ? w.adj[ti] : 0u;
+    const uint32_t adj = adj_of(w, tbit_of(gi));
     const uint32_t kadj = adj & keep_tags;
     const float2 xi = make_float2(gi.x, gi.y);
-    const unsigned long long m0 = accept_row(w, src, 0, b[0], e[0], xi, gi.z, kadj);
-    unsigned long long m1 = accept_row(w, src, 1, b[1], e[1], xi, gi.z, kadj);
-    const unsigned long long m2 = accept_row(w, src, 2, b[2], e[2], xi, gi.z, kadj);
+    // The band and the tag test are skipped for the whole wave when no lane
+    // needs them: the band is empty and every tag present is in kadj.
+    const bool check = __ballot(w.check_all || (w.tags_present & ~kadj) != 0u) != 0ull;
+    unsigned long long m0, m1, m2;
+    if (check) {
+        m0 = accept_row<true>(w, src, 0, b[0], e[0], xi, gi.z, kadj);
+        m1 = accept_row<true>(w, src, 1, b[1], e[1], xi, gi.z, kadj);
+        m2 = accept_row<true>(w, src, 2, b[2], e[2], xi, gi.z, kadj);
+    } else {
+        m0 = accept_row<false>(w, src, 0, b[0], e[0], xi, gi.z, kadj);
+        m1 = accept_row<false>(w, src, 1, b[1], e[1], xi, gi.z, kadj);
+        m2 = accept_row<false>(w, src, 2, b[2], e[2], xi, gi.z, kadj);
+    }
     const unsigned long long self = 1ull << (i - b[1]);
     m1 &= ~self;  // I != J
     int cnt = 0;
     if (COUNT) {
         if (adj & ~keep_tags) {  // some adjacent tags are counted but not visited
-            cnt = __popcll(accept_row(w, src, 0, b[0], e[0], xi, gi.z, adj)) +
-                  __popcll(accept_row(w, src, 1, b[1], e[1], xi, gi.z, adj) & ~self) +
-                  __popcll(accept_row(w, src, 2, b[2], e[2], xi, gi.z, adj));
+            cnt = __popcll(accept_row<true>(w, src, 0, b[0], e[0], xi, gi.z, adj)) +
+                  __popcll(accept_row<true>(w, src, 1, b[1], e[1], xi, gi.z, adj) & ~self) +
+                  __popcll(accept_row<true>(w, src, 2, b[2], e[2], xi, gi.z, adj));
         } else {
             cnt = __popcll(m0) + __popcll(m1) + __popcll(m2);
         }
@@ -286,7 +308,7 @@ __device__ __forceinline__ int walk_rows(const Walk &w, const Src &src, int i, f
     auto visit = [&](int j, int roff) {
         const float4 gj = src.get(j + roff);
         const float dx = xi.x - gj.x, dy = xi.y - gj.y;
-        pair(j, tag_of(gj), gj.z, dx, dy, dx * dx + dy * dy);
+        pair(j, tbit_of(gj), gj.z, dx, dy, dx * dx + dy * dy);
     };
     if (ORDERED) {
         for (int s9 = 0; s9 < 9; ++s9) {
@@ -306,19 +328,18 @@ __device__ __forceinline__ int walk_rows(const Walk &w, const Src &src, int i, f
             }
         }
     } else {
-        int r = 0, base = b[0], roff = src.off(0);
-        unsigned long long m = m0;
-        while (true) {
-            while (!m && r < 2) {
-                ++r;
-                m = sel3(r, m0, m1, m2);
-                base = sel3(r, b[0], b[1], b[2]);
-                roff = src.off(r);
-            }
-            if (!m) break;
-            const int j = base + __builtin_ctzll(m);
-            m &= m - 1ull;
-            visit(j, roff);
+        // Rows without accepted neighbours are dropped first, so the loop
+        // moves to the next row at most once per visited neighbour.
+        unsigned long long n0 = m0, n1 = m1, n2 = m2;
+        int b0 = b[0], b1 = b[1], b2 = b[2];
+        int o0 = src.off(0), o1 = src.off(1), o2 = src.off(2);
+        if (!n1) { n1 = n2; b1 = b2; o1 = o2; n2 = 0ull; }
+        if (!n0) { n0 = n1; b0 = b1; o0 = o1; n1 = n2; b1 = b2; o1 = o2; n2 = 0ull; }
+        while (n0) {
+            const int j = b0 + __builtin_ctzll(n0);
+            n0 &= n0 - 1ull;
+            visit(j, o0);
+            if (!n0) { n0 = n1; b0 = b1; o0 = o1; n1 = n2; b1 = b2; o1 = o2; n2 = 0ull; }
         }
     }
     return cnt;
@@ -565,7 +586,7 @@ __global__ __launch_bounds__(kBlock) void k_permute(Fields cur, Fields nxt, floa
     const float h = cur.h[s];
     const int32_t tag = cur.tag[s];
     nxt.pos[dest] = x;
-    geo[dest] = make_float4(x.x, x.y, h, __int_as_float(tag));
+    geo[dest] = make_float4(x.x, x.y, h, __uint_as_float(tag_bit(tag)));
     nxt.dyn[dest] = cur.dyn[s];
     nxt.dvel[dest] = cur.dvel[s];
     nxt.mass[dest] = cur.mass[s];
@@ -625,7 +646,7 @@ __global__ __launch_bounds__(kPairBlock) void k_search(Walk w, Fields f, Physics
     int cnt = 0;
     if (active) {
         const float4 gi = w.geo[i];
-        const int ti = tag_of(gi);
+        const int ti = f.tag[i];
         const bool i_fluid = ti == ph.fluid_tag, i_wall = ti == ph.wall_tag;
         if (ACTION == EPPX_SEARCH_KERNEL_CONTINUITY) {
             const float4 di = f.dyn[i];
@@ -636,16 +657,18 @@ __global__ __launch_bounds__(kPairBlock) void k_search(Walk w, Fields f, Physics
             const uint32_t keep = i_fluid ? tag_bit(ph.fluid_tag) | tag_bit(ph.wall_tag)
                                           : (i_wall ? tag_bit(ph.fluid_tag) : 0u);
             const float2 *uv = reinterpret_cast<const float2 *>(f.dyn);  // (u, v) of slot j at uv[2 j]
-            auto term = [&](int j, int tj, float hj, float dx, float dy, float dr2) {
-                const bool ff = i_fluid && tj == ph.fluid_tag;
-                const float m = ff ? f.mass[j] : rho_i * f.vol[j];
+            const uint32_t fluid_bit = tag_bit(ph.fluid_tag);
+            auto term = [&](int j, uint32_t tbj, float hj, float dx, float dy, float dr2) {
+                const bool ff = i_fluid && tbj == fluid_bit;
+                const float v = (ff ? f.mass : f.vol)[j];  // one load: m_j, or V_j scaled by rho_i
+                const float m = ff ? v : rho_i * v;
                 acc += continuity_term<FAST, K>(gi.z, hj, ui, uv[2 * (size_t)j], m, dx, dy, dr2);
             };
             if (tiled) cnt = walk_rows<!FAST, true>(w, rows, i, gi, keep, term);
             else cnt = walk_rows<!FAST, true>(w, GlobalRows{w.geo}, i, gi, keep, term);
             f.drho[i] = acc;
         } else {
-            walk(w, i, gi, [&](int, int, float, float, float) { ++cnt; });
+            walk(w, i, gi, [&](int, uint32_t, float, float, float) { ++cnt; });
         }
         ncount[i] = cnt;
         if (cnt > M) atomicOr(flags, kFlagNeighbourOverflow);
@@ -699,8 +722,7 @@ __device__ __forceinline__ void momentum_pair(const Physics &ph, float2 &acc, bo
     }
     const float rinv = frsq(dr2);
     const float r = dr2 * rinv;
-    const float hh = 2.0f * hi * hj;
-    const float hinv = (hi + hj) * frcp(hh);
+    const float hinv = (hi + hj) * frcp(2.0f * hi * hj);
     const float dw = grad_fast<K>(r, hinv);
     const float rirj = rho_i * rho_j;
     float prho2 = classic ? fmaf(p_j, frcp(rho_j * rho_j), pr_i) : (p_i + p_j) * frcp(rirj);
@@ -709,20 +731,20 @@ __device__ __forceinline__ void momentum_pair(const Physics &ph, float2 &acc, bo
     prho2 *= mdw * rinv;
     acc.x = fmaf(prho2, dx, acc.x);
     acc.y = fmaf(prho2, dy, acc.y);
+    // With h = 1 / hinv and q = r hinv: r^2 + 0.01 h^2 = (q^2 + 0.01) / hinv^2.
+    const float q = r * hinv;
+    const float qa = fmaf(q, q, 0.01f);
     if (VISC == EPPX_VISC_CLASSIC) {
-        const float h = hh * frcp(hi + hj);
-        const float az = fmaf(r, r, 0.01f * h * h);
-        const float force = (2.0f * ph.mu) * m_j * r * dw * frcp(rirj * az);
+        const float force = (2.0f * ph.mu) * m_j * r * dw * (hinv * hinv) * frcp(rirj * qa);
         acc.x = fmaf(force, dux, acc.x);
         acc.y = fmaf(force, duy, acc.y);
     } else {
+        // phi = vdotx / (hinv (r^2 + 0.01 h^2)) = vdotx hinv / (q^2 + 0.01); a separating
+        // pair (vdotx > 0) contributes nothing, which phi = 0 gives without a branch.
         const float vdotx = fmaf(dy, duy, dx * dux);
-        if (vdotx > 0.0f) return;
-        const float h = hh * frcp(hi + hj);
-        const float az = fmaf(r, r, 0.01f * h * h);
-        const float phi = vdotx * frcp(hinv * az);
+        const float phi = fminf(vdotx, 0.0f) * hinv * frcp(qa);
         const float force =
-            fmaf(ph.beta, phi, -ph.alpha * ph.av_c) * phi * frcp(0.5f * (rho_i + rho_j)) * (mdw * frcp(rirj * r));
+            fmaf(ph.beta, phi, -ph.alpha * ph.av_c) * phi * (mdw * frcp((0.5f * (rho_i + rho_j)) * rirj * r));
         acc.x = fmaf(force, dx, acc.x);
         acc.y = fmaf(force, dy, acc.y);
     }
@@ -739,15 +761,16 @@ __global__ __launch_bounds__(kPairBlock) void k_momentum(Walk w, Fields f, Physi
     const bool tiled = stage_rows(w, i0, n, tile, info, rows);
     const int i = i0 + threadIdx.x;
     if (i >= n) return;
+    if (f.tag[i] != ph.fluid_tag) return;  // iMomentum! acts on fluid particles only
     const float4 gi = w.geo[i];
-    if (tag_of(gi) != ph.fluid_tag) return;  // iMomentum! acts on fluid particles only
     const float4 di = f.dyn[i];
     const float2 ui = make_float2(di.x, di.y);
     float2 acc = f.dvel[i];
     const float pr_i = FAST ? di.w * frcp(di.z * di.z) : 0.0f;
-    auto term = [&](int j, int tj, float hj, float dx, float dy, float dr2) {
+    const uint32_t fluid_bit = tag_bit(ph.fluid_tag);
+    auto term = [&](int j, uint32_t tbj, float hj, float dx, float dy, float dr2) {
         const float4 dj = f.dyn[j];
-        momentum_pair<FAST, K, VISC>(ph, acc, tj == ph.fluid_tag, gi.z, hj, ui, di.z, di.w, pr_i,
+        momentum_pair<FAST, K, VISC>(ph, acc, tbj == fluid_bit, gi.z, hj, ui, di.z, di.w, pr_i,
                                      make_float2(dj.x, dj.y), dj.z, dj.w, f.mass[j], dx, dy, dr2);
     };
     const uint32_t keep = tag_bit(ph.fluid_tag) | tag_bit(ph.wall_tag);
@@ -779,12 +802,13 @@ __global__ __launch_bounds__(kBlock) void k_interaction(Walk w, Fields f, Physic
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= *w.n_alive) return;
     const float4 gi = w.geo[i];
-    const int ti = tag_of(gi);
+    const int ti = f.tag[i];
     if (ACTION == EPPX_INTER_FILTER) {
         if (ti != ph.fluid_tag) return;
         float wv = f.wv[i], wvrho = f.wvrho[i];
-        walk(w, i, gi, [&](int j, int tj, float, float, float dr2) {
-            if (tj != ph.fluid_tag) return;
+        const uint32_t fluid_bit = tag_bit(ph.fluid_tag);
+        walk(w, i, gi, [&](int j, uint32_t tbj, float, float, float dr2) {
+            if (tbj != fluid_bit) return;
             const float r = sqrtf(dr2);
             const float wgt = value_exact(ph.kernel, r, invharmonic(gi.z, w.geo[j].z));
             wv += wgt * f.vol[j];
@@ -796,7 +820,7 @@ __global__ __launch_bounds__(kBlock) void k_interaction(Walk w, Fields f, Physic
         int stride;
         float *col = field_column(f, op, stride);
         float v = col[(int64_t)i * stride];
-        walk(w, i, gi, [&](int, int, float, float, float) { v += op.value; });
+        walk(w, i, gi, [&](int, uint32_t, float, float, float) { v += op.value; });
         col[(int64_t)i * stride] = v;
     }
 }
@@ -870,13 +894,14 @@ __global__ __launch_bounds__(kBlock) void k_export(Walk w, Fields f, const int64
     if (i >= *w.n_alive) return;
     const int row = f.id[i];
     const float4 gi = w.geo[i];
-    const int ti = tag_of(gi);
+    const int ti = f.tag[i];
     const float hi = gi.z;
+    const uint32_t fluid_bit = tag_bit(fluid_tag), wall_bit = tag_bit(wall_tag);
     const size_t cap = (size_t)ex.n_capacity;
     const int64_t o = row_start ? row_start[i] : 0;
     const int limit = js ? ncount[i] : 0;  // the row length the search counted
     int k = 0;
-    walk(w, i, gi, [&](int j, int tj, float dx, float dy, float dr2) {
+    walk(w, i, gi, [&](int j, uint32_t tbj, float dx, float dy, float dr2) {
         if (js && k < limit) js[o + k] = f.id[j];
         if (k < ex.M) {
             if (ex.nIndex) ex.nIndex[row + cap * k] = f.id[j] + 1;
@@ -886,8 +911,8 @@ __global__ __launch_bounds__(kBlock) void k_export(Walk w, Fields f, const int64
             }
             const float r = sqrtf(dr2);
             if (ex.nR) ex.nR[row + cap * k] = r;
-            const bool kernel_pair = (ti == fluid_tag && (tj == fluid_tag || tj == wall_tag)) ||
-                                     (ti == wall_tag && tj == fluid_tag);
+            const bool kernel_pair = (ti == fluid_tag && (tbj == fluid_bit || tbj == wall_bit)) ||
+                                     (ti == wall_tag && tbj == fluid_bit);
             if (ex.with_kernel && kernel_pair) {
                 const float hinv = invharmonic(hi, w.geo[j].z);
                 if (ex.nHInv) ex.nHInv[row + cap * k] = hinv;
@@ -987,9 +1012,16 @@ __global__ __launch_bounds__(kBlock) void k_finite(Fields f, int32_t n, uint32_t
 // min / max of H over alive particles; h is positive so the float bit
 // patterns order like unsigned integers.  range[2] is set when some alive
 // h is not a positive finite number.  One atomic per wave.
+// range[3] collects the tag bits of particles 0..n-1 (alive or not) and
+// range[4] is set when one of them has a tag outside 1..kMaxTag.
 __global__ __launch_bounds__(kBlock) void k_h_range(Fields f, int32_t n, float *range) {
     const int s = blockIdx.x * kBlock + threadIdx.x;
-    unsigned int lo = 0xffffffffu, hi = 0u, bad = 0u;
+    unsigned int lo = 0xffffffffu, hi = 0u, bad = 0u, tags = 0u, odd = 0u;
+    if (s < n) {
+        const uint32_t tb = tag_bit(f.tag[s]);
+        tags = tb;
+        odd = tb == 0u;
+    }
     if (s < n && f.alive[s] == 1) {
         const float h = f.h[s];
         if (!(h > 0.0f) || !isfinite(h)) {
@@ -1003,12 +1035,16 @@ __global__ __launch_bounds__(kBlock) void k_h_range(Fields f, int32_t n, float *
         lo = min(lo, (unsigned int)__shfl_down((int)lo, off));
         hi = max(hi, (unsigned int)__shfl_down((int)hi, off));
         bad |= (unsigned int)__shfl_down((int)bad, off);
+        tags |= (unsigned int)__shfl_down((int)tags, off);
+        odd |= (unsigned int)__shfl_down((int)odd, off);
     }
     if (lane_id() == 0) {
         unsigned int *r = reinterpret_cast<unsigned int *>(range);
         if (lo != 0xffffffffu) atomicMin(&r[0], lo);
         if (hi) atomicMax(&r[1], hi);
         if (bad) atomicMax(&r[2], 1u);
+        if (tags) atomicOr(&r[3], tags);
+        if (odd) atomicMax(&r[4], 1u);
     }
 }
 
