@@ -60,8 +60,9 @@ def kernel_times(par, names):
     return out
 
 
-def cpu_baseline(case, steps, threads):
-    """Time the reference-algorithm restatement (oracle/) on `steps` steps after one warm-up step."""
+def cpu_baseline(case, seconds, threads, max_steps=200):
+    """Time the reference-algorithm restatement (oracle/): whole steps after one
+    warm-up step, until `seconds` of CPU time are spent (at least one step)."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle as O
     from eppx import cases
@@ -74,10 +75,14 @@ def cpu_baseline(case, steps, threads):
     ph = O.physics(**full.physics.as_dict())
     ref.search(O.CRIT_SYMMETRY, 1, ph, serial=False, threads=threads)
     ref.run_steps(1, ph, serial=False, threads=threads)
-    t0 = time.perf_counter()
-    err, pairs = ref.run_steps(steps, ph, serial=False, threads=threads)
-    dt = time.perf_counter() - t0
-    return float(pairs.sum()) / dt, dt
+    total, steps, dt = 0.0, 0, 0.0
+    while steps < max_steps and (steps == 0 or dt < seconds):
+        t0 = time.perf_counter()
+        err, pairs = ref.run_steps(1, ph, serial=False, threads=threads)
+        dt += time.perf_counter() - t0
+        total += float(pairs.sum())
+        steps += 1
+    return total / dt, dt, steps
 
 
 def copy_calibration(n_bytes: int, repeats: int = 3) -> None:
@@ -105,7 +110,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", choices=["large", "small"], default="large")
     ap.add_argument("--math", choices=["exact", "fast"], default="fast")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sample length")
+    ap.add_argument("--profile-steps", type=int, default=50, help="eager steps timed kernel by kernel")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run device copies of BYTES (profiler calibration)")
@@ -144,9 +150,8 @@ def main():
     _lib.check(L.eppx_pair_total(ps.handle, C.byref(total), 1), par.ctx, "pair_total")
     fb = C.c_int64()
     _lib.check(L.eppx_pair_fallback_blocks(ps.handle, C.byref(fb), 1), par.ctx, "fallback")
-    _lib.check(L.eppx_set_timing(par.ctx, 1), par.ctx, "timing")
-    L.eppx_kernel_time(par.ctx, b"search", None, None, 1)  # clear
 
+    # timed region: whole steps replayed as a recorded hipGraph, state resident in HBM
     par.synchronize()
     t0 = time.perf_counter()
     Algorithm.step(ps, ns, case.physics, n_steps=args.steps, step_offset=args.warmup, sync=False)
@@ -159,6 +164,12 @@ def main():
     _lib.check(L.eppx_pair_fallback_blocks(ps.handle, C.byref(fb), 1), par.ctx, "fallback")
     blocks_per_launch = -(-case.n_particles // 256)  # 256 particles per pair block
     log(f"pair blocks reading HBM instead of LDS: {fb.value / (2 * args.steps) / blocks_per_launch:.4%}")
+
+    # per-kernel durations: the same steps launched kernel by kernel between HIP events on the library's stream
+    _lib.check(L.eppx_set_timing(par.ctx, 1), par.ctx, "timing")
+    L.eppx_kernel_time(par.ctx, b"search", None, None, 1)  # clear
+    Algorithm.step(ps, ns, case.physics, n_steps=args.profile_steps, step_offset=args.warmup + args.steps)
+    _lib.check(L.eppx_set_timing(par.ctx, 0), par.ctx, "timing")
     names = ["search", "momentum", "continuity_eos", "gravity_move", "cell_reset", "cell_key", "cell_scan",
              "cell_members", "cell_permute"]
     kt = kernel_times(par, names)
@@ -168,22 +179,27 @@ def main():
     alg_bytes = per_p * case.n_particles + per_cell * case.domain.get_n()
     avg_us = kt[dominant][0]
     achieved = alg_bytes / (avg_us * 1e-6) / 1e9
-    device_us_per_step = sum(v[0] * v[1] for v in kt.values()) / args.steps
-    step_bytes = sum(ALG_BYTES[k][0] * case.n_particles + ALG_BYTES[k][1] * case.domain.get_n() for k in ALG_BYTES)
+    device_us_per_step = sum(v[0] * v[1] for v in kt.values()) / args.profile_steps
+    # the whole step's compulsory traffic: 56 B/particle for the pass after the move (x, u, m, tag, h,
+    # rho, V in; rho, p, V out; cell key written and read) and 52 B/particle for the momentum pass
+    # (x, u, rho, p, m, tag, h in; x, u out), plus the 12 B/cell of the cell table
+    step_bytes = 108 * case.n_particles + 12 * case.domain.get_n()
 
-    traffic = None
-    tpath = os.path.join(REPO, "profiles", f"traffic_{dominant}_{args.workload}.json")
+    traffic = valu = None
+    tpath = os.path.join(REPO, "profiles", f"traffic_{dominant}_{args.workload}_{args.math}.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            prof = json.load(f)
+        traffic = prof.get("hbm_bytes_per_launch")
+        valu = prof.get("valu_fraction")
 
     cpu = None
     if not args.no_cpu:
         threads = len(os.sched_getaffinity(0))
         threads = min(threads, int(os.environ.get("OMP_NUM_THREADS", threads)))
-        rate, secs = cpu_baseline(case, args.cpu_steps, threads)
+        rate, secs, nsteps = cpu_baseline(case, args.cpu_seconds, threads)
         cpu = {"value": rate, "unit": "pair-interactions/s", "cores": threads, "kind": "port",
-               "sample": f"{args.cpu_steps} steps of the same {case.n_particles}-particle dam break after one "
+               "sample": f"{nsteps} steps of the same {case.n_particles}-particle dam break after one "
                          f"warm-up step, C/OpenMP restatement of the reference algorithm on the reference layout "
                          f"(per-cell atomic insert, materialised M=64 neighbour lists), {secs:.1f} s"}
 
@@ -210,7 +226,7 @@ def main():
                    "untiled_block_fraction": fb.value / (2 * args.steps) / blocks_per_launch},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "alg_bytes_per_launch": alg_bytes, "avg_launch_us": avg_us},
+                     "alg_bytes_per_launch": alg_bytes, "avg_launch_us": avg_us, "valu_fraction": valu},
         "step_roofline": {"alg_bytes_per_step": step_bytes, "device_us_per_step": device_us_per_step,
                           "hbm_frac": step_bytes / (device_us_per_step * 1e-6) / 1e9 / HBM_PEAK_GBS},
         "kernels_us": {k: round(v[0], 2) for k, v in kt.items()},

@@ -9,6 +9,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <map>
@@ -45,6 +46,19 @@ struct eppx_ns {
     uint32_t adj[kMaxTag + 1] = {};
     int32_t max_tag = 0;
     int32_t M = 0;
+    uint64_t serial = 0;  // unique per created system (graphs recorded against it check it)
+};
+
+// One time step recorded as a hipGraph (eppx_step).  The step moves the
+// state from buffer `cur` to the other one, so each starting parity has its
+// own graph; kernel arguments are recorded by value, so a graph is replayed
+// only for the same neighbour system, physics and math mode, and dropped when
+// the particle data are uploaded again.
+struct StepGraph {
+    hipGraphExec_t exec = nullptr;
+    uint64_t ns_serial = 0;
+    Physics ph{};
+    int32_t math = -1;
 };
 
 struct eppx_ps {
@@ -53,7 +67,7 @@ struct eppx_ps {
     eppx_layout layout{};
     Fields buf[2]{};
     int cur = 0;
-    float4 *geo = nullptr;  // (x, y, h, tag) snapshot of the last search, sorted order
+    float4 *geo = nullptr;  // (x, y, h, m) snapshot of the last search, sorted order
     // cell arrays of the last search
     int32_t n_cells = -1;
     int32_t *counts = nullptr, *start = nullptr, *tile_sums = nullptr;
@@ -68,6 +82,7 @@ struct eppx_ps {
     eppx_ns *last_ns = nullptr;
     int32_t last_crit = 0, last_action = 0, last_kernel = 0, last_fluid = 0, last_wall = 0;
     bool searched = false;
+    StepGraph graphs[2];
     // columns the kernels do not own, kept on the host between upload and download
     std::vector<int32_t> pass_int_cols, pass_float_cols;
     std::vector<int32_t> pass_int;
@@ -232,6 +247,7 @@ Walk make_walk(const eppx_ps *ps, const eppx_ns *ns, int32_t criterion) {
     w.start = ps->start;
     w.geo = ps->geo;
     w.cell = f.cell;
+    w.tag = f.tag;
     w.fallback = ps->fallback;
     w.n_alive = ps->n_alive;
     memcpy(w.adj, ns->adj, sizeof(w.adj));
@@ -336,6 +352,22 @@ int32_t do_search(eppx_ps *ps, eppx_ns *ns, int32_t criterion, int32_t action, c
     return EPPX_OK;
 }
 
+void drop_graphs(eppx_ps *ps) {
+    if (ps->graphs[0].exec || ps->graphs[1].exec) (void)hipStreamSynchronize(ps->ctx->stream);
+    for (auto &g : ps->graphs) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        g = StepGraph{};
+    }
+}
+
+// The pair kernels read m_j from the search snapshot; an operation that
+// changes Mass after the search refreshes it there too.
+int32_t refresh_snapshot_mass(eppx_ps *ps, int32_t field) {
+    if (field != EPPX_FIELD_MASS || !ps->searched) return EPPX_OK;
+    HIP_OK(ps->ctx, launch_snapshot_mass(ps->geo, ps->f(), ps->n, ps->ctx->stream));
+    return EPPX_OK;
+}
+
 int32_t do_self(eppx_ps *ps, int32_t action, const Physics &ph, const eppx_field_op &op) {
     eppx_ctx *ctx = ps->ctx;
     const char *name = action == EPPX_SELF_CONTINUITY_EOS ? "continuity_eos"
@@ -343,7 +375,7 @@ int32_t do_self(eppx_ps *ps, int32_t action, const Physics &ph, const eppx_field
                        : action == EPPX_SELF_FILTER       ? "filter_self"
                                                           : "scale_field";
     HIP_OK(ctx, timed(ctx, name, [&] { return launch_self(ps->f(), ps->n, action, ph, op, ctx->stream); }));
-    return EPPX_OK;
+    return refresh_snapshot_mass(ps, action == EPPX_SELF_SCALE_FIELD ? op.field : -1);
 }
 
 int32_t do_interaction(eppx_ps *ps, eppx_ns *ns, int32_t action, Physics ph, const eppx_field_op &op) {
@@ -356,7 +388,7 @@ int32_t do_interaction(eppx_ps *ps, eppx_ns *ns, int32_t action, Physics ph, con
                                                      : "add_per_neighbour";
     HIP_OK(ctx, timed(ctx, name, [&] { return launch_interaction(w, ps->f(), ps->n, action, ph, op, fast,
                                                                  ctx->stream); }));
-    return EPPX_OK;
+    return refresh_snapshot_mass(ps, action == EPPX_INTER_ADD_PER_NEIGHBOUR ? op.field : -1);
 }
 
 }  // namespace
@@ -527,6 +559,7 @@ int32_t eppx_ps_destroy(eppx_ps *ps) {
     if (!ps) return EPPX_E_ARG;
     (void)hipSetDevice(ps->ctx->device);
     (void)hipStreamSynchronize(ps->ctx->stream);
+    drop_graphs(ps);
     free_fields(ps->buf[0]);
     free_fields(ps->buf[1]);
     dfree(ps->geo);
@@ -614,6 +647,7 @@ int32_t eppx_ps_upload(eppx_ps *ps, int32_t n_particles, const int32_t *is_alive
         memcpy(&ps->pass_float[k * cap], float_props + (size_t)ps->pass_float_cols[k] * cap, cap * sizeof(float));
     ps->searched = false;
     ps->last_ns = nullptr;
+    drop_graphs(ps);  // recorded kernel arguments (particle count, H bounds) may have changed
     return EPPX_OK;
 }
 
@@ -709,6 +743,8 @@ int32_t eppx_ns_create(eppx_ctx *ctx, const eppx_domain2d *domain, const int32_t
                     d.n_x, d.n_y, d.n};
     ns->max_tag = max_tag;
     ns->M = max_neighbour_number;
+    static std::atomic<uint64_t> next_serial{1};
+    ns->serial = next_serial++;
     for (int t = 1; t <= max_tag; ++t)
         for (int u = 1; u <= max_tag; ++u)
             if (adjacency[(t - 1) + (size_t)max_tag * (u - 1)] == 1) ns->adj[t] |= 1u << (u - 1);
@@ -869,16 +905,52 @@ int32_t eppx_step(eppx_ps *ps, eppx_ns *ns, const eppx_physics *phys, int32_t n_
                            "eppx_step filter");
     if (rc) return rc;
     HIP_OK(ctx, hipSetDevice(ctx->device));
+    if ((rc = ensure_cells(ps, ns->grid.n_cells))) return rc;  // no allocation while a step is recorded
     const eppx_field_op op{};
-    for (int32_t k = 1; k <= n_steps; ++k) {
-        if ((rc = do_self(ps, EPPX_SELF_CONTINUITY_EOS, ph, op))) return rc;
-        if ((rc = do_interaction(ps, ns, EPPX_INTER_MOMENTUM, ph, op))) return rc;
-        if ((rc = do_self(ps, EPPX_SELF_GRAVITY_MOVE, ph, op))) return rc;
-        if ((rc = do_search(ps, ns, EPPX_CRIT_SYMMETRY, EPPX_SEARCH_KERNEL_CONTINUITY, ph))) return rc;
-        if (filter_interval > 0 && (step_offset + k) % filter_interval == 0) {
-            if ((rc = do_interaction(ps, ns, EPPX_INTER_FILTER, ph, op))) return rc;
-            if ((rc = do_self(ps, EPPX_SELF_FILTER, ph, op))) return rc;
+    auto enqueue_step = [&](bool filter_now) -> int32_t {
+        int32_t r;
+        if ((r = do_self(ps, EPPX_SELF_CONTINUITY_EOS, ph, op))) return r;
+        if ((r = do_interaction(ps, ns, EPPX_INTER_MOMENTUM, ph, op))) return r;
+        if ((r = do_self(ps, EPPX_SELF_GRAVITY_MOVE, ph, op))) return r;
+        if ((r = do_search(ps, ns, EPPX_CRIT_SYMMETRY, EPPX_SEARCH_KERNEL_CONTINUITY, ph))) return r;
+        if (filter_now) {
+            if ((r = do_interaction(ps, ns, EPPX_INTER_FILTER, ph, op))) return r;
+            if ((r = do_self(ps, EPPX_SELF_FILTER, ph, op))) return r;
         }
+        return EPPX_OK;
+    };
+    for (int32_t k = 1; k <= n_steps; ++k) {
+        const bool filter_now = filter_interval > 0 && (step_offset + k) % filter_interval == 0;
+        if (filter_now || ctx->timing) {  // filter steps and per-kernel timing run eagerly
+            if ((rc = enqueue_step(filter_now))) return rc;
+            continue;
+        }
+        StepGraph &g = ps->graphs[ps->cur];
+        if (!g.exec || g.ns_serial != ns->serial || g.math != ctx->math_mode || memcmp(&g.ph, &ph, sizeof(ph)) != 0) {
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+            g = StepGraph{};
+            const int cur0 = ps->cur;
+            HIP_OK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+            rc = enqueue_step(false);
+            hipGraph_t graph = nullptr;
+            const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+            ps->cur = cur0;  // recorded, not run yet
+            if (rc == EPPX_OK && e != hipSuccess) rc = hip_fail(ctx, e, "eppx_step: hipStreamEndCapture");
+            if (rc == EPPX_OK) {
+                const hipError_t ei = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+                if (ei != hipSuccess) rc = hip_fail(ctx, ei, "eppx_step: hipGraphInstantiate");
+            }
+            if (graph) (void)hipGraphDestroy(graph);
+            if (rc) {
+                g = StepGraph{};
+                return rc;
+            }
+            g.ns_serial = ns->serial;
+            g.ph = ph;
+            g.math = ctx->math_mode;
+        }
+        HIP_OK(ctx, hipGraphLaunch(g.exec, ctx->stream));
+        ps->cur ^= 1;
     }
     return EPPX_OK;
 }

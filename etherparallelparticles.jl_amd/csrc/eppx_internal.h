@@ -41,7 +41,8 @@ struct Fields {
 struct Walk {
     Grid g;
     const int32_t *start;     // (n_cells + 2) exclusive scan of cell counts; slot ranges
-    const float4 *geo;        // (x, y, h, tag bits) at the last search, sorted order
+    const float4 *geo;        // (x, y, h, m) at the last search, sorted order
+    const int32_t *tag;       // tags, sorted order
     const int32_t *cell;
     const int32_t *n_alive;   // device scalar
     uint32_t adj[kMaxTag + 1];  // adj[t]: bit (u-1) set when tag t is adjacent to tag u
@@ -96,6 +97,7 @@ hipError_t launch_permute(Fields cur, Fields nxt, float4 *geo, int32_t n, int32_
 hipError_t launch_search(const Walk &w, Fields f, int32_t n, int32_t action, const Physics &ph, int32_t fast,
                          int32_t M, int32_t *ncount, unsigned long long *pairs_last,
                          unsigned long long *pairs_total, uint32_t *flags, hipStream_t s);
+hipError_t launch_snapshot_mass(float4 *geo, Fields f, int32_t n, hipStream_t s);
 hipError_t launch_self(Fields f, int32_t n, int32_t action, const Physics &ph, const eppx_field_op &op,
                        hipStream_t s);
 hipError_t launch_interaction(const Walk &w, Fields f, int32_t n, int32_t action, const Physics &ph,

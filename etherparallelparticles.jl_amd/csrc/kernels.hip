@@ -16,6 +16,8 @@
 // the reference's list order.
 #include <math.h>
 
+#include <algorithm>
+
 #include "eppx_internal.h"
 
 namespace eppx {
@@ -30,6 +32,16 @@ constexpr float kSigmaCubic = (float)(10.0 / 7.0 / 3.14159265358979323846);
 constexpr float kSigmaWendland = (float)(7.0 / 4.0 / 3.14159265358979323846);
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+// The dispatcher hands workgroup b to XCD b % 8 (8 XCDs, each with its own
+// L2).  Pair blocks are renumbered so that each XCD works through one
+// contiguous run of cell-sorted slots: the neighbour rows a block stages are
+// then mostly those its XCD-mates staged, and stay in that XCD's L2.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block(int b, int g) {
+    const int x = b % kXcds, k = b / kXcds, per = g / kXcds, rem = g % kXcds;
+    return x * per + min(x, rem) + k;
+}
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
 
@@ -109,13 +121,12 @@ __device__ __forceinline__ float eos(const Physics &ph, float rho) {
     return ph.tait_b * (t * t * x - 1.0f) + ph.p0;
 }
 
-// The search snapshot geo = (x, y, h, tag bit) stores tag t as the bit
-// 1 << (t - 1); a tag outside 1..kMaxTag gets 0, which no tag is adjacent to.
+// Tags enter the neighbour walk as the bit 1 << (t - 1); a tag outside
+// 1..kMaxTag gets 0, which no tag is adjacent to.
 __device__ __forceinline__ uint32_t tag_bit(int t) {
     const uint32_t t1 = (uint32_t)t - 1u;
     return t1 < (uint32_t)kMaxTag ? 1u << t1 : 0u;
 }
-__device__ __forceinline__ uint32_t tbit_of(float4 g) { return __float_as_uint(g.w); }
 // ActivePair row (adjacency bits) of the tag whose bit is tb.
 __device__ __forceinline__ uint32_t adj_of(const Walk &w, uint32_t tb) {
     return tb ? w.adj[__builtin_ctz(tb) + 1] : 0u;
@@ -148,7 +159,7 @@ template <class Visit>
 __device__ __forceinline__ void walk(const Walk &w, int i, float4 gi, Visit &&visit) {
     const int c = w.cell[i] - 1;
     const int cx = c % w.g.n_x, cy = c / w.g.n_x;
-    const uint32_t adj = adj_of(w, tbit_of(gi));
+    const uint32_t adj = adj_of(w, tag_bit(w.tag[i]));
     const float2 xi = make_float2(gi.x, gi.y);
     for (int di = -1; di <= 1; ++di) {
         const int ii = cx + di;
@@ -160,9 +171,9 @@ __device__ __forceinline__ void walk(const Walk &w, int i, float4 gi, Visit &&vi
             const int b = w.start[k], e = w.start[k + 1];
             for (int j = b; j < e; ++j) {
                 if (j == i) continue;
-                const float4 gj = w.geo[j];
-                const uint32_t tbj = tbit_of(gj);
+                const uint32_t tbj = tag_bit(w.tag[j]);
                 if (!(adj & tbj)) continue;
+                const float4 gj = w.geo[j];
                 const float dx = xi.x - gj.x, dy = xi.y - gj.y;
                 const float dr2 = dx * dx + dy * dy;
                 if (accept_dr2(w, gi.z, gj.z, dr2)) visit(j, tbj, dx, dy, dr2);
@@ -176,12 +187,13 @@ __device__ __forceinline__ void walk(const Walk &w, int i, float4 gi, Visit &&vi
 //  Pass 1: the three cells of each neighbour row are contiguous in the
 //    cell-sorted order, so each row is one slot range [b_r, e_r) whose
 //    candidates are tested, four at a time, into a 64-bit accept mask.
-//  Pass 2: `pair` runs for every accepted neighbour whose tag is in
-//    keep_tags (the tag filter is folded into the pass-1 masks, so pass 2
-//    issues its loads at once).  ORDERED visits them in the reference's list
-//    order (x offset outer, y offset inner, ascending slot inside a cell), so
-//    sums are bit-identical to the reference; otherwise row by row, in one
-//    loop that keeps every lane busy.
+//  Pass 2: for every accepted neighbour j whose tag is in keep_tags (the
+//    tag filter is folded into the pass-1 masks), p = fetch(j, tag bit)
+//    gathers what the pair term needs from global memory and
+//    apply(j, tag bit, h_j, m_j, dx, dy, dr2, p) adds the term.  ORDERED
+//    visits them in the reference's list order (x offset outer, y offset
+//    inner, ascending slot inside a cell), so sums are bit-identical to the
+//    reference; otherwise row by row, in one loop that keeps every lane busy.
 // A lane whose row range exceeds 64 slots walks global memory one candidate
 // at a time instead.  With COUNT, returns the number of accepted neighbours
 // of every tag (nCount); otherwise 0.
@@ -198,7 +210,7 @@ __device__ __forceinline__ int sel3(int r, int a, int b, int c) { return r == 0 
 // and non-adjacent tags are cleared.
 template <bool CHECK, int N>
 __device__ __forceinline__ uint32_t accept_bits(const Walk &w, float2 xi, float hi, uint32_t adj,
-                                                const float4 (&g)[N]) {
+                                                const float4 (&g)[N], const uint32_t (&tb)[N]) {
     float d[N];
     uint32_t ok = 0u, band = 0u;
 #pragma unroll
@@ -215,24 +227,30 @@ __device__ __forceinline__ uint32_t accept_bits(const Walk &w, float2 xi, float 
                 if ((band >> k) & 1u) ok |= (uint32_t)criterion_exact(w, hi, g[k].z, d[k]) << k;
         }
 #pragma unroll
-        for (int k = 0; k < N; ++k) ok &= ~((uint32_t)((adj & tbit_of(g[k])) == 0u) << k);
+        for (int k = 0; k < N; ++k) ok &= ~((uint32_t)((adj & tb[k]) == 0u) << k);
     }
     return ok;
 }
 
 // Candidate geometry of the row walk: slot j of neighbour row r, read from
 // global memory or from the block's LDS copy of its three row ranges.
+// get(k): the snapshot (x, y, h, m) of slot k; tbit(k): its tag bit.
 struct GlobalRows {
     const float4 *geo;
+    const int32_t *tag;
     __device__ __forceinline__ int off(int) const { return 0; }
     __device__ __forceinline__ float4 get(int k) const { return geo[k]; }
+    __device__ __forceinline__ uint32_t tbit(int k) const { return tag_bit(tag[k]); }
     __device__ __forceinline__ float4 at(int r, int j) const { return get(j + off(r)); }
 };
 struct TileRows {
-    const float4 *tile;  // LDS: row r, slot j at tile[j + off(r)]
+    const float4 *tile;   // LDS: row r, slot j at tile[j + off(r)]
+    const uint8_t *tix;   // LDS: tag - 1 of the same slot (63 outside 1..kMaxTag)
     int off0, off1, off2;
     __device__ __forceinline__ int off(int r) const { return sel3(r, off0, off1, off2); }
     __device__ __forceinline__ float4 get(int k) const { return tile[k]; }
+    // a 64-bit shift: slot index 63 lands outside the low 32 bits, giving 0
+    __device__ __forceinline__ uint32_t tbit(int k) const { return (uint32_t)(1ull << tix[k]); }
     __device__ __forceinline__ float4 at(int r, int j) const { return get(j + off(r)); }
 };
 
@@ -241,20 +259,27 @@ __device__ __forceinline__ unsigned long long accept_row(const Walk &w, const Sr
                                                          float2 xi, float hi, uint32_t adj) {
     unsigned long long m = 0ull;
     int j = b;
+    const int o = src.off(r);
     for (; j + 4 <= e; j += 4) {  // four independent loads in flight per iteration
-        const float4 g[4] = {src.at(r, j), src.at(r, j + 1), src.at(r, j + 2), src.at(r, j + 3)};
-        m |= (unsigned long long)accept_bits<CHECK, 4>(w, xi, hi, adj, g) << (j - b);
+        const float4 g[4] = {src.get(j + o), src.get(j + 1 + o), src.get(j + 2 + o), src.get(j + 3 + o)};
+        uint32_t tb[4] = {0u, 0u, 0u, 0u};
+        if (CHECK) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) tb[k] = src.tbit(j + k + o);
+        }
+        m |= (unsigned long long)accept_bits<CHECK, 4>(w, xi, hi, adj, g, tb) << (j - b);
     }
     for (; j < e; ++j) {
-        const float4 g[1] = {src.at(r, j)};
-        m |= (unsigned long long)accept_bits<CHECK, 1>(w, xi, hi, adj, g) << (j - b);
+        const float4 g[1] = {src.get(j + o)};
+        const uint32_t tb[1] = {CHECK ? src.tbit(j + o) : 0u};
+        m |= (unsigned long long)accept_bits<CHECK, 1>(w, xi, hi, adj, g, tb) << (j - b);
     }
     return m;
 }
 
-template <bool ORDERED, bool COUNT, class Src, class Pair>
+template <bool ORDERED, bool COUNT, class Src, class Fetch, class Apply>
 __device__ __forceinline__ int walk_rows(const Walk &w, const Src &src, int i, float4 gi, uint32_t keep_tags,
-                                         Pair &&pair) {
+                                         Fetch &&fetch, Apply &&apply) {
     const int c = w.cell[i] - 1;
     const int nx = w.g.n_x;
     const int cx = c % nx, cy = c / nx;
@@ -273,11 +298,14 @@ __device__ __forceinline__ int walk_rows(const Walk &w, const Src &src, int i, f
         int cnt = 0;
         walk(w, i, gi, [&](int j, uint32_t tbj, float dx, float dy, float dr2) {
             ++cnt;
-            if (tbj & keep_tags) pair(j, tbj, w.geo[j].z, dx, dy, dr2);
+            if (tbj & keep_tags) {
+                const float4 gj = w.geo[j];
+                apply(j, tbj, gj.z, gj.w, dx, dy, dr2, fetch(j, tbj));
+            }
         });
         return COUNT ? cnt : 0;
     }
-    const uint32_t adj = adj_of(w, tbit_of(gi));
+    const uint32_t adj = adj_of(w, tag_bit(w.tag[i]));
     const uint32_t kadj = adj & keep_tags;
     const float2 xi = make_float2(gi.x, gi.y);
     // The band and the tag test are skipped for the whole wave when no lane
@@ -305,10 +333,10 @@ __device__ __forceinline__ int walk_rows(const Walk &w, const Src &src, int i, f
             cnt = __popcll(m0) + __popcll(m1) + __popcll(m2);
         }
     }
-    auto visit = [&](int j, int roff) {
+    auto visit = [&](int j, int roff, uint32_t tbj, const auto &p) {
         const float4 gj = src.get(j + roff);
         const float dx = xi.x - gj.x, dy = xi.y - gj.y;
-        pair(j, tbit_of(gj), gj.z, dx, dy, dx * dx + dy * dy);
+        apply(j, tbj, gj.z, gj.w, dx, dy, dx * dx + dy * dy, p);
     };
     if (ORDERED) {
         for (int s9 = 0; s9 < 9; ++s9) {
@@ -324,7 +352,8 @@ __device__ __forceinline__ int walk_rows(const Walk &w, const Src &src, int i, f
             while (m) {
                 const int j = cb + __builtin_ctzll(m);
                 m &= m - 1ull;
-                visit(j, roff);
+                const uint32_t tbj = src.tbit(j + roff);
+                visit(j, roff, tbj, fetch(j, tbj));
             }
         }
     } else {
@@ -336,10 +365,11 @@ __device__ __forceinline__ int walk_rows(const Walk &w, const Src &src, int i, f
         if (!n1) { n1 = n2; b1 = b2; o1 = o2; n2 = 0ull; }
         if (!n0) { n0 = n1; b0 = b1; o0 = o1; n1 = n2; b1 = b2; o1 = o2; n2 = 0ull; }
         while (n0) {
-            const int j = b0 + __builtin_ctzll(n0);
+            const int j = b0 + __builtin_ctzll(n0), jo = o0;
             n0 &= n0 - 1ull;
-            visit(j, o0);
             if (!n0) { n0 = n1; b0 = b1; o0 = o1; n1 = n2; b1 = b2; o1 = o2; n2 = 0ull; }
+            const uint32_t tbj = src.tbit(j + jo);
+            visit(j, jo, tbj, fetch(j, tbj));
         }
     }
     return cnt;
@@ -353,7 +383,8 @@ __device__ __forceinline__ int walk_rows(const Walk &w, const Src &src, int i, f
 // nothing staged, when the block spans two cell rows or a row range does not
 // fit; the block then reads global memory.  Every thread of the block calls
 // it (two barriers).
-__device__ __forceinline__ bool stage_rows(const Walk &w, int i0, int n, float4 *tile, int *info, TileRows &rows) {
+__device__ __forceinline__ bool stage_rows(const Walk &w, int i0, int n, float4 *tile, uint8_t *tix, int *info,
+                                           TileRows &rows) {
     const int t = threadIdx.x;
     if (t < 3) {
         const int nx = w.g.n_x;
@@ -375,9 +406,14 @@ __device__ __forceinline__ bool stage_rows(const Walk &w, int i0, int n, float4 
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
             const int b = info[r], len = info[3 + r];
-            for (int k = t; k < len; k += kPairBlock) tile[r * kTileRow + k] = w.geo[b + k];
+            for (int k = t; k < len; k += kPairBlock) {
+                tile[r * kTileRow + k] = w.geo[b + k];
+                const uint32_t t1 = (uint32_t)w.tag[b + k] - 1u;
+                tix[r * kTileRow + k] = (uint8_t)(t1 < (uint32_t)kMaxTag ? t1 : 63u);
+            }
         }
         rows.tile = tile;
+        rows.tix = tix;
         rows.off0 = -info[0];
         rows.off1 = kTileRow - info[1];
         rows.off2 = 2 * kTileRow - info[2];
@@ -586,7 +622,7 @@ __global__ __launch_bounds__(kBlock) void k_permute(Fields cur, Fields nxt, floa
     const float h = cur.h[s];
     const int32_t tag = cur.tag[s];
     nxt.pos[dest] = x;
-    geo[dest] = make_float4(x.x, x.y, h, __uint_as_float(tag_bit(tag)));
+    geo[dest] = make_float4(x.x, x.y, h, cur.mass[s]);
     nxt.dyn[dest] = cur.dyn[s];
     nxt.dvel[dest] = cur.dvel[s];
     nxt.mass[dest] = cur.mass[s];
@@ -626,6 +662,11 @@ __device__ __forceinline__ float continuity_term(float hi, float hj, float2 ui, 
     return (mj_or_rhoVj * vdotx) * (dw * rinv);
 }
 
+struct ContinuityGather {
+    float2 uv;
+    float vol;
+};
+
 // Search pass: find the neighbours of every alive particle and run the fused
 // action (EPPX_SEARCH_KERNEL_CONTINUITY: sComputeKernelAndiContinuity!,
 // main.jl:74-90).
@@ -635,14 +676,15 @@ __global__ __launch_bounds__(kPairBlock) void k_search(Walk w, Fields f, Physics
                                                         unsigned long long *pairs_total, uint32_t *flags) {
     __shared__ int lds[kPairBlock / 64];
     __shared__ float4 tile[3 * kTileRow];
+    __shared__ uint8_t tix[3 * kTileRow];
     __shared__ int info[9];
     const int n = *w.n_alive;
-    const int i0 = blockIdx.x * kPairBlock;
+    const int i0 = xcd_block(blockIdx.x, gridDim.x) * kPairBlock;
     if (i0 >= n) return;  // the whole block
     const int i = i0 + threadIdx.x;
     const bool active = i < n;
     TileRows rows;
-    const bool tiled = ACTION == EPPX_SEARCH_KERNEL_CONTINUITY && stage_rows(w, i0, n, tile, info, rows);
+    const bool tiled = ACTION == EPPX_SEARCH_KERNEL_CONTINUITY && stage_rows(w, i0, n, tile, tix, info, rows);
     int cnt = 0;
     if (active) {
         const float4 gi = w.geo[i];
@@ -658,14 +700,20 @@ __global__ __launch_bounds__(kPairBlock) void k_search(Walk w, Fields f, Physics
                                           : (i_wall ? tag_bit(ph.fluid_tag) : 0u);
             const float2 *uv = reinterpret_cast<const float2 *>(f.dyn);  // (u, v) of slot j at uv[2 j]
             const uint32_t fluid_bit = tag_bit(ph.fluid_tag);
-            auto term = [&](int j, uint32_t tbj, float hj, float dx, float dy, float dr2) {
-                const bool ff = i_fluid && tbj == fluid_bit;
-                const float v = (ff ? f.mass : f.vol)[j];  // one load: m_j, or V_j scaled by rho_i
-                const float m = ff ? v : rho_i * v;
-                acc += continuity_term<FAST, K>(gi.z, hj, ui, uv[2 * (size_t)j], m, dx, dy, dr2);
+            // u_j, and V_j for mixed pairs only (fluid-fluid terms use m_j from the snapshot)
+            auto fetch = [&](int j, uint32_t tbj) {
+                ContinuityGather p;
+                p.uv = uv[2 * (size_t)j];
+                p.vol = (i_fluid && tbj == fluid_bit) ? 0.0f : f.vol[j];
+                return p;
             };
-            if (tiled) cnt = walk_rows<!FAST, true>(w, rows, i, gi, keep, term);
-            else cnt = walk_rows<!FAST, true>(w, GlobalRows{w.geo}, i, gi, keep, term);
+            auto term = [&](int, uint32_t tbj, float hj, float mj, float dx, float dy, float dr2,
+                            const ContinuityGather &p) {
+                const float m = (i_fluid && tbj == fluid_bit) ? mj : rho_i * p.vol;
+                acc += continuity_term<FAST, K>(gi.z, hj, ui, p.uv, m, dx, dy, dr2);
+            };
+            if (tiled) cnt = walk_rows<!FAST, true>(w, rows, i, gi, keep, fetch, term);
+            else cnt = walk_rows<!FAST, true>(w, GlobalRows{w.geo, w.tag}, i, gi, keep, fetch, term);
             f.drho[i] = acc;
         } else {
             walk(w, i, gi, [&](int, uint32_t, float, float, float) { ++cnt; });
@@ -753,12 +801,13 @@ __device__ __forceinline__ void momentum_pair(const Physics &ph, float2 &acc, bo
 template <bool FAST, int K, int VISC>
 __global__ __launch_bounds__(kPairBlock) void k_momentum(Walk w, Fields f, Physics ph) {
     __shared__ float4 tile[3 * kTileRow];
+    __shared__ uint8_t tix[3 * kTileRow];
     __shared__ int info[9];
     const int n = *w.n_alive;
-    const int i0 = blockIdx.x * kPairBlock;
+    const int i0 = xcd_block(blockIdx.x, gridDim.x) * kPairBlock;
     if (i0 >= n) return;  // the whole block
     TileRows rows;
-    const bool tiled = stage_rows(w, i0, n, tile, info, rows);
+    const bool tiled = stage_rows(w, i0, n, tile, tix, info, rows);
     const int i = i0 + threadIdx.x;
     if (i >= n) return;
     if (f.tag[i] != ph.fluid_tag) return;  // iMomentum! acts on fluid particles only
@@ -768,14 +817,14 @@ __global__ __launch_bounds__(kPairBlock) void k_momentum(Walk w, Fields f, Physi
     float2 acc = f.dvel[i];
     const float pr_i = FAST ? di.w * frcp(di.z * di.z) : 0.0f;
     const uint32_t fluid_bit = tag_bit(ph.fluid_tag);
-    auto term = [&](int j, uint32_t tbj, float hj, float dx, float dy, float dr2) {
-        const float4 dj = f.dyn[j];
+    auto fetch = [&](int j, uint32_t) { return f.dyn[j]; };
+    auto term = [&](int, uint32_t tbj, float hj, float mj, float dx, float dy, float dr2, const float4 &dj) {
         momentum_pair<FAST, K, VISC>(ph, acc, tbj == fluid_bit, gi.z, hj, ui, di.z, di.w, pr_i,
-                                     make_float2(dj.x, dj.y), dj.z, dj.w, f.mass[j], dx, dy, dr2);
+                                     make_float2(dj.x, dj.y), dj.z, dj.w, mj, dx, dy, dr2);
     };
     const uint32_t keep = tag_bit(ph.fluid_tag) | tag_bit(ph.wall_tag);
-    if (tiled) walk_rows<!FAST, false>(w, rows, i, gi, keep, term);
-    else walk_rows<!FAST, false>(w, GlobalRows{w.geo}, i, gi, keep, term);
+    if (tiled) walk_rows<!FAST, false>(w, rows, i, gi, keep, fetch, term);
+    else walk_rows<!FAST, false>(w, GlobalRows{w.geo, w.tag}, i, gi, keep, fetch, term);
     f.dvel[i] = acc;
 }
 
@@ -880,6 +929,12 @@ __global__ __launch_bounds__(kBlock) void k_self(Fields f, int32_t n, Physics ph
         float *col = field_column(f, op, stride);
         col[(int64_t)s * stride] *= op.value;
     }
+}
+
+// Mass column of the search snapshot after Mass changed (slots in sorted order).
+__global__ __launch_bounds__(kBlock) void k_snapshot_mass(float4 *geo, Fields f, int32_t n) {
+    const int s = blockIdx.x * kBlock + threadIdx.x;
+    if (s < n) geo[s].w = f.mass[s];
 }
 
 // ------------------------------------------------------- layout conversion
@@ -1011,24 +1066,24 @@ __global__ __launch_bounds__(kBlock) void k_finite(Fields f, int32_t n, uint32_t
 
 // min / max of H over alive particles; h is positive so the float bit
 // patterns order like unsigned integers.  range[2] is set when some alive
-// h is not a positive finite number.  One atomic per wave.
-// range[3] collects the tag bits of particles 0..n-1 (alive or not) and
+// h is not a positive finite number.  Grid-stride over the particles, one
+// set of atomics per block.  range[3] collects the tag bits of particles 0..n-1 (alive or not) and
 // range[4] is set when one of them has a tag outside 1..kMaxTag.
 __global__ __launch_bounds__(kBlock) void k_h_range(Fields f, int32_t n, float *range) {
-    const int s = blockIdx.x * kBlock + threadIdx.x;
+    __shared__ unsigned int part[5][kBlock / 64];
     unsigned int lo = 0xffffffffu, hi = 0u, bad = 0u, tags = 0u, odd = 0u;
-    if (s < n) {
+    for (int s = blockIdx.x * kBlock + threadIdx.x; s < n; s += gridDim.x * kBlock) {
         const uint32_t tb = tag_bit(f.tag[s]);
-        tags = tb;
-        odd = tb == 0u;
-    }
-    if (s < n && f.alive[s] == 1) {
-        const float h = f.h[s];
-        if (!(h > 0.0f) || !isfinite(h)) {
-            bad = 1u;
-        } else {
-            lo = __float_as_uint(h);
-            hi = lo;
+        tags |= tb;
+        odd |= tb == 0u;
+        if (f.alive[s] == 1) {
+            const float h = f.h[s];
+            if (!(h > 0.0f) || !isfinite(h)) {
+                bad = 1u;
+            } else {
+                lo = min(lo, __float_as_uint(h));
+                hi = max(hi, __float_as_uint(h));
+            }
         }
     }
     for (int off = 32; off > 0; off >>= 1) {
@@ -1038,7 +1093,23 @@ __global__ __launch_bounds__(kBlock) void k_h_range(Fields f, int32_t n, float *
         tags |= (unsigned int)__shfl_down((int)tags, off);
         odd |= (unsigned int)__shfl_down((int)odd, off);
     }
+    const int wave = threadIdx.x >> 6;
     if (lane_id() == 0) {
+        part[0][wave] = lo;
+        part[1][wave] = hi;
+        part[2][wave] = bad;
+        part[3][wave] = tags;
+        part[4][wave] = odd;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < kBlock / 64; ++k) {
+            lo = min(lo, part[0][k]);
+            hi = max(hi, part[1][k]);
+            bad |= part[2][k];
+            tags |= part[3][k];
+            odd |= part[4][k];
+        }
         unsigned int *r = reinterpret_cast<unsigned int *>(range);
         if (lo != 0xffffffffu) atomicMin(&r[0], lo);
         if (hi) atomicMax(&r[1], hi);
@@ -1201,8 +1272,14 @@ hipError_t launch_finite(Fields f, int32_t n, uint32_t *bad, hipStream_t s) {
     return hipGetLastError();
 }
 
+hipError_t launch_snapshot_mass(float4 *geo, Fields f, int32_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_snapshot_mass, grid_for(n), dim3(kBlock), 0, s, geo, f, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_h_range(Fields f, int32_t n, float *range, hipStream_t s) {
-    hipLaunchKernelGGL(k_h_range, grid_for(n), dim3(kBlock), 0, s, f, n, range);
+    const dim3 g((unsigned)std::min<int64_t>(grid_for(n).x, 1024));  // grid-stride: one atomic set per block
+    hipLaunchKernelGGL(k_h_range, g, dim3(kBlock), 0, s, f, n, range);
     return hipGetLastError();
 }
 

@@ -54,6 +54,44 @@ def test_small_steps_exact(parallel, small, route):
     U.assert_neighbour_columns_equal(host, ref)
 
 
+def test_step_graph_replays_interleaved(parallel, small):
+    """eppx_step replays a recorded hipGraph (one per buffer parity).  Replays
+    interleaved with eager calls, host synchronisation and odd step counts
+    must each leave the state the restatement has after as many steps."""
+    from eppx import Algorithm, SPH
+    ps, ns, ref = U.search_both(parallel, small)
+    ph = small.physics
+    oph = U.oracle_physics(ph)
+    for seg, route in ((3, "step"), (2, "api"), (4, "step"), (1, "api"), (5, "step"), (2, "step")):
+        if route == "step":
+            Algorithm.step(ps, ns, ph, n_steps=seg)
+        else:
+            for _ in range(seg):
+                Algorithm.selfaction(ps, SPH.sContinuity(ph), sync=False)
+                Algorithm.interaction(ps, SPH.iMomentum(ph), sync=False)
+                Algorithm.selfaction(ps, SPH.sMomentum(ph), sync=False)
+                Algorithm.search(ps, small.domain, ns, criterion=Algorithm.symmetrySearchCriterion,
+                                 action=SPH.sComputeKernelAndiContinuity(ph), sync=False)
+            parallel.synchronize()
+        err, _ = ref.run_steps(seg, oph)
+        assert err == 0
+        U.assert_state_equal(U.download(ps), ref,
+                             names=("PositionVec", "VelocityVec", "Density", "Pressure", "dDensity", "Volume"))
+
+
+def test_mass_changed_after_search(parallel, small):
+    """Mass scaled between the search and the momentum pass: iMomentum! reads
+    m_j at interaction time, so the pair terms must see the new masses."""
+    from eppx import Algorithm, SPH
+    ps, ns, ref = U.search_both(parallel, small)
+    ph = small.physics
+    Algorithm.selfaction(ps, SPH.ScaleField("Mass", 0, 1.5))
+    Algorithm.interaction(ps, SPH.iMomentum(ph))
+    ref.selfaction(10, column=ref.L.Mass, value=1.5)
+    ref.interaction(1, U.oracle_physics(ph))
+    U.assert_state_equal(U.download(ps), ref, names=("Mass", "dVelocityVec"))
+
+
 def test_small_1000_steps(parallel, small, oracle):
     """1000 steps (0.048 s of flow): bit-identical state; so the surge front
     (max x of fluid) is identical, well inside 0.5 dx; mass is exact."""
