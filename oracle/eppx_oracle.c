@@ -366,7 +366,7 @@ static void self_one(const or_layout *L, int32_t I, int32_t *IP, float *FP, int3
         /* sFilter main.jl:136-144 -> sKernelFilter! Filter.jl:24-38, sVolume!, eos */
         if (tag != P->fluid_tag) return;
         float h = FPC(L, FP, I, L->H);
-        float w0 = or_kernel_value0(P->kernel, h);
+        float w0 = OR_MUTATION == 9 ? 0.0f : or_kernel_value0(P->kernel, h);
         FPC(L, FP, I, L->SumWeight) += w0 * FPC(L, FP, I, L->Volume);
         FPC(L, FP, I, L->SumWeightedDensity) += w0 * FPC(L, FP, I, L->Mass);
         FPC(L, FP, I, L->Density) = FPC(L, FP, I, L->SumWeightedDensity) / FPC(L, FP, I, L->SumWeight);
@@ -451,7 +451,8 @@ static void momentum_pair(const or_layout *L, int32_t I, int32_t NI, float *FP, 
 static void filter_pair(const or_layout *L, int32_t I, int32_t NI, float *FP, const int32_t *IP,
                         const or_physics *P) {
     int32_t J = IPC(L, IP, I, L->nIndex + NI) - 1;
-    if (IPC(L, IP, I, L->Tag) != P->fluid_tag || IPC(L, IP, J, L->Tag) != P->fluid_tag) return;
+    if (IPC(L, IP, I, L->Tag) != P->fluid_tag) return;
+    if (IPC(L, IP, J, L->Tag) != P->fluid_tag && OR_MUTATION != 8) return;
     float w = FPC(L, FP, I, L->nW + NI);
     FPC(L, FP, I, L->SumWeight) += w * FPC(L, FP, J, L->Volume);
     FPC(L, FP, I, L->SumWeightedDensity) += w * FPC(L, FP, J, L->Mass);

@@ -79,6 +79,25 @@ def test_step_graph_replays_interleaved(parallel, small):
                              names=("PositionVec", "VelocityVec", "Density", "Pressure", "dDensity", "Volume"))
 
 
+def test_shepard_filter_exact(parallel, small):
+    """iFilter + sFilter (main.jl:128-144) after a search, and eppx_step with
+    the filter every 5 steps (main.jl:180-183): bit-identical to the restatement."""
+    from eppx import Algorithm, SPH
+    ph = small.physics
+    oph = U.oracle_physics(ph)
+    ps, ns, ref = U.search_both(parallel, small)
+    Algorithm.interaction(ps, SPH.iFilter(ph))
+    Algorithm.selfaction(ps, SPH.sFilter(ph))
+    ref.interaction(2, oph)
+    ref.selfaction(3, oph)
+    names = ("Density", "Volume", "Pressure", "SumWeight", "SumWeightedDensity")
+    U.assert_state_equal(U.download(ps), ref, names=names)
+    Algorithm.step(ps, ns, ph, n_steps=12, filter_interval=5)
+    err, _ = ref.run_steps(12, oph, filter_interval=5)
+    assert err == 0
+    U.assert_state_equal(U.download(ps), ref, names=names + ("PositionVec", "VelocityVec"))
+
+
 def test_mass_changed_after_search(parallel, small):
     """Mass scaled between the search and the momentum pass: iMomentum! reads
     m_j at interaction time, so the pair terms must see the new masses."""

@@ -330,3 +330,74 @@ def test_small_dam_break_case_and_conservation(oracle):
     assert y.min() > -0.02 and np.all(np.isfinite(s.FP[: s.n, s.L.Density]))
     walls = s.IP[: s.n, s.L.Tag] == 2
     assert np.array_equal(s.FP[: s.n, s.L.PositionVec][walls], c.FP[: c.n_particles, c.layout.PositionVec][walls])
+
+
+# ------------------------------------------------------------ Shepard filter
+# iKernelFilter! / sKernelFilter! (Filter.jl:10-38): rho_i = (w0 m_i + sum_j W_ij m_j) /
+# (w0 V_i + sum_j W_ij V_j) over fluid neighbours j, then V = m / rho and p = eos(rho).
+
+def shepard(O, s, ph):
+    assert s.search(O.CRIT_SYMMETRY, 1, ph) == 0
+    s.interaction(2, ph)
+    s.selfaction(3, ph)
+    return s.FP[: s.n, s.L.Density].astype(np.float64)
+
+
+def wall_row_lattice(O, variant=None):
+    """Fluid lattice whose bottom three rows are walls five times denser."""
+    nx, ny, dx = 20, 20, 0.01
+    tags = np.ones(nx * ny, np.int32)
+    tags[: 3 * nx] = 2
+    s, x, y = lattice(O, nx, ny, dx, tags=tags, variant=variant)
+    wall = tags == 2
+    s.FP[: s.n, s.L.Density][wall] *= 5.0
+    s.FP[: s.n, s.L.Mass][wall] *= 5.0
+    return s, ~wall
+
+
+def test_shepard_filter_uniform_and_walls_excluded(oracle):
+    """A uniform fluid stays at rho0 (up to rounding), also next to denser walls."""
+    s, fluid = wall_row_lattice(oracle)
+    rho = shepard(oracle, s, phys(oracle))
+    assert np.max(np.abs(rho[fluid] / 1000.0 - 1.0)) < 1e-6
+    assert s.FP[: s.n, s.L.SumWeight].max() == 0.0  # sums are reset
+
+
+def test_shepard_filter_catches_wall_neighbours(oracle):
+    s, fluid = wall_row_lattice(oracle, variant=oracle.mutant(8))
+    rho = shepard(oracle, s, phys(oracle))
+    assert np.max(np.abs(rho[fluid] / 1000.0 - 1.0)) > 1e-2
+
+
+def two_particle_shepard(O, variant=None):
+    """Two fluid particles at r = 0.7 h with masses rho0 V and 3 rho0 V: particle 1 becomes
+    rho0 (1 + 3 k) / (1 + k), k = W(q) / W(0) = (2 - q)^4 (1 + 2 q) / 16 (Wendland C2, 2D)."""
+    s, x, y = lattice(O, 2, 1, 0.01, variant=variant)
+    h = float(s.FP[0, s.L.H])
+    s.FP[1, s.L.PositionVec] = s.FP[0, s.L.PositionVec] + np.float32(0.7 * h)
+    s.FP[1, s.L.Mass] *= 3.0
+    s.FP[1, s.L.Density] *= 3.0
+    rho = shepard(O, s, phys(O))
+    q = float(s.FP[1, s.L.PositionVec] - s.FP[0, s.L.PositionVec]) / h
+    k = (2.0 - q) ** 4 * (1.0 + 2.0 * q) / 16.0
+    return rho[0], 1000.0 * (1.0 + 3.0 * k) / (1.0 + k)
+
+
+def test_shepard_filter_two_particles(oracle):
+    got, want = two_particle_shepard(oracle)
+    assert abs(got / want - 1.0) < 1e-6
+
+
+def test_shepard_filter_catches_dropped_self_term(oracle):
+    got, want = two_particle_shepard(oracle, variant=oracle.mutant(9))
+    assert not abs(got / want - 1.0) < 1e-3
+
+
+def test_shepard_filter_scales_with_mass(oracle):
+    """Scaling every mass (and density) by 3 scales the filtered density by 3."""
+    a, _ = wall_row_lattice(oracle)
+    b, _ = wall_row_lattice(oracle)
+    b.FP[: b.n, b.L.Mass] *= 3.0
+    b.FP[: b.n, b.L.Density] *= 3.0
+    ra, rb = shepard(oracle, a, phys(oracle)), shepard(oracle, b, phys(oracle))
+    assert np.max(np.abs(rb / (3.0 * ra) - 1.0)) < 1e-6
