@@ -144,14 +144,14 @@ hipError_t alloc_fields(Fields &f, int32_t cap) {
     };
     a(&f.pos); a(&f.dvel); a(&f.dyn);
     a(&f.mass); a(&f.vol); a(&f.drho); a(&f.h); a(&f.wv); a(&f.wvrho);
-    a(&f.tag); a(&f.mov); a(&f.id); a(&f.cell); a(&f.alive);
+    a(&f.tag); a(&f.mov); a(&f.id); a(&f.cell); a(&f.alive); a(&f.cs);
     return e;
 }
 
 void free_fields(Fields &f) {
     dfree(f.pos); dfree(f.dvel); dfree(f.dyn);
     dfree(f.mass); dfree(f.vol); dfree(f.drho); dfree(f.h); dfree(f.wv); dfree(f.wvrho);
-    dfree(f.tag); dfree(f.mov); dfree(f.id); dfree(f.cell); dfree(f.alive);
+    dfree(f.tag); dfree(f.mov); dfree(f.id); dfree(f.cell); dfree(f.alive); dfree(f.cs);
 }
 
 bool owned_float(const eppx_layout &L, int c) {
@@ -232,7 +232,7 @@ int32_t ensure_cells(eppx_ps *ps, int32_t n_cells) {
     dfree(ps->counts);
     dfree(ps->start);
     dfree(ps->tile_sums);
-    const int64_t nb = (int64_t)n_cells + 1;  // cells + dead bucket
+    const int64_t nb = (int64_t)n_cells + 2;  // cells + dead bucket + released bucket
     HIP_OK(ctx, dalloc(&ps->counts, (size_t)nb));
     HIP_OK(ctx, dalloc(&ps->start, (size_t)nb + 1));
     HIP_OK(ctx, dalloc(&ps->tile_sums, (size_t)((nb + kScanTile - 1) / kScanTile)));
@@ -240,28 +240,70 @@ int32_t ensure_cells(eppx_ps *ps, int32_t n_cells) {
     return EPPX_OK;
 }
 
+// Bounds of symmetrySearchCriterion for alive particles whose H lies in
+// [hmin, hmax] (valid: every such H is a positive finite number).
+void symmetry_bounds(bool valid, float hmin_f, float hmax_f, float *rmin2, float *rmax2) {
+    const double hmin = hmin_f, hmax = hmax_f;
+    if (!valid || !(hmin >= 1e-30) || !(hmax <= 1e30) || !(hmin <= hmax)) {
+        *rmin2 = -1.0f;
+        *rmax2 = INFINITY;
+    } else if (hmin == hmax) {
+        // One H for all: the criterion compares dr2 with one threshold,
+        // evaluated here with the device's operations (Mean.harmonic, then
+        // *2 and squared; IEEE f32, no contraction).
+        const float h = hmin_f;
+        const float h2 = 2.0f * h * h / (h + h) * 2.0f;
+        const float t = h2 * h2;
+        *rmin2 = std::isfinite(t) ? nextafterf(t, INFINITY) : -1.0f;
+        *rmax2 = std::isfinite(t) ? -INFINITY : INFINITY;
+    } else {
+        const double eps = 1.0 / 1048576.0;  // 2^-20, well above the criterion's rounding
+        *rmin2 = nextafterf((float)(4.0 * hmin * hmin * (1.0 - eps)), 0.0f);
+        *rmax2 = nextafterf((float)(4.0 * hmax * hmax * (1.0 + eps)), INFINITY);
+    }
+}
+
+// Criterion bounds: the criterion accepts every dr2 < rmin2 and rejects
+// every dr2 > rmax2.  defaultSearchCriterion: dr2 <= gap_square, i.e.
+// dr2 < the next float up, with an empty band; symmetry: the bounds the H
+// range gives.
+void criterion_bounds(int32_t criterion, const Grid &g, float sym_min2, float sym_max2, float *rmin2, float *rmax2) {
+    if (criterion == EPPX_CRIT_DEFAULT) {
+        const float gs = g.gap_square;
+        *rmin2 = std::isfinite(gs) ? nextafterf(gs, INFINITY) : -1.0f;
+        *rmax2 = std::isfinite(gs) ? -INFINITY : INFINITY;
+    } else {
+        *rmin2 = sym_min2;
+        *rmax2 = sym_max2;
+    }
+}
+
+// The particle system's one case: its grid, criterion bounds and tags; it
+// always advances.
+CaseDesc case_of(const eppx_ps *ps, const eppx_ns *ns, int32_t criterion) {
+    CaseDesc c{};
+    c.g = ns->grid;
+    criterion_bounds(criterion, ns->grid, ps->rmin2, ps->rmax2, &c.rmin2, &c.rmax2);
+    c.tags_present = ps->tags_present;
+    c.check_all = (c.rmin2 <= c.rmax2) || ps->odd_tags;
+    c.steps_left = 1;
+    return c;
+}
+
 Walk make_walk(const eppx_ps *ps, const eppx_ns *ns, int32_t criterion) {
     Walk w{};
     const Fields &f = ps->buf[ps->cur];
-    w.g = ns->grid;
+    w.one = case_of(ps, ns, criterion);
+    w.cases = nullptr;
     w.start = ps->start;
     w.geo = ps->geo;
     w.cell = f.cell;
     w.tag = f.tag;
+    w.cs = f.cs;
     w.fallback = ps->fallback;
     w.n_alive = ps->n_alive;
     memcpy(w.adj, ns->adj, sizeof(w.adj));
     w.criterion = criterion;
-    if (criterion == EPPX_CRIT_DEFAULT) {
-        const float gs = ns->grid.gap_square;  // accept dr2 <= gap_square, i.e. dr2 < next float up
-        w.rmin2 = std::isfinite(gs) ? nextafterf(gs, INFINITY) : -1.0f;
-        w.rmax2 = std::isfinite(gs) ? -INFINITY : INFINITY;
-    } else {
-        w.rmin2 = ps->rmin2;
-        w.rmax2 = ps->rmax2;
-    }
-    w.tags_present = ps->tags_present;
-    w.check_all = (w.rmin2 <= w.rmax2) || ps->odd_tags;
     return w;
 }
 
@@ -322,17 +364,17 @@ int32_t do_search(eppx_ps *ps, eppx_ns *ns, int32_t criterion, int32_t action, c
     const int32_t nc = ns->grid.n_cells;
     hipStream_t s = ctx->stream;
     Fields cur = ps->buf[ps->cur], nxt = ps->buf[ps->cur ^ 1];
-    HIP_OK(ctx, timed(ctx, "cell_reset", [&] { return launch_reset_search(ps->counts, nc + 1, ps->pairs, s); }));
+    HIP_OK(ctx, timed(ctx, "cell_reset", [&] { return launch_reset_search(ps->counts, nc + 2, ps->pairs, s); }));
     HIP_OK(ctx, timed(ctx, "cell_key", [&] {
-               return launch_key_count(cur, ps->n, ns->grid, ps->counts, ps->key, ps->tmpslot, ctx->d_flags, ns->M,
-                                       s);
+               return launch_key_count(cur, ps->n, case_of(ps, ns, criterion), nullptr, nc, ps->counts, ps->key,
+                                       ps->tmpslot, ctx->d_flags, ns->M, s);
            }));
-    HIP_OK(ctx, timed(ctx, "cell_scan", [&] { return launch_scan(ps->counts, nc + 1, ps->start, ps->tile_sums, s); }));
+    HIP_OK(ctx, timed(ctx, "cell_scan", [&] { return launch_scan(ps->counts, nc + 2, ps->start, ps->tile_sums, s); }));
     HIP_OK(ctx, timed(ctx, "cell_members", [&] {
                return launch_fill_members(cur, ps->n, nc, ps->key, ps->tmpslot, ps->start, ps->members, s);
            }));
     HIP_OK(ctx, timed(ctx, "cell_permute", [&] {
-               return launch_permute(cur, nxt, ps->geo, ps->n, ps->cap, nc, ps->key, ps->tmpslot, ps->start,
+               return launch_permute(cur, nxt, ps->geo, ps->n, 0, nc, ps->key, ps->tmpslot, ps->start,
                                      ps->members, ps->n_alive, ctx->d_flags, s);
            }));
     ps->cur ^= 1;
@@ -340,7 +382,7 @@ int32_t do_search(eppx_ps *ps, eppx_ns *ns, int32_t criterion, int32_t action, c
     const int fast = ctx->math_mode == EPPX_MATH_FAST;
     HIP_OK(ctx, timed(ctx, "search", [&] {
                return launch_search(w, ps->f(), ps->n, action, ph, fast, ns->M, ps->ncount, &ps->pairs[0],
-                                    &ps->pairs[1], ctx->d_flags, s);
+                                    &ps->pairs[1], ctx->d_flags, 0, s);
            }));
     ps->last_ns = ns;
     ps->last_crit = criterion;
@@ -374,7 +416,7 @@ int32_t do_self(eppx_ps *ps, int32_t action, const Physics &ph, const eppx_field
                        : action == EPPX_SELF_GRAVITY_MOVE ? "gravity_move"
                        : action == EPPX_SELF_FILTER       ? "filter_self"
                                                           : "scale_field";
-    HIP_OK(ctx, timed(ctx, name, [&] { return launch_self(ps->f(), ps->n, action, ph, op, ctx->stream); }));
+    HIP_OK(ctx, timed(ctx, name, [&] { return launch_self(ps->f(), ps->n, action, ph, op, nullptr, ctx->stream); }));
     return refresh_snapshot_mass(ps, action == EPPX_SELF_SCALE_FIELD ? op.field : -1);
 }
 
@@ -538,11 +580,11 @@ int32_t eppx_ps_create(eppx_ctx *ctx, int32_t n_capacity, const eppx_layout *lay
     if (e == hipSuccess) e = dalloc(&ps->tmpslot, (size_t)n_capacity);
     if (e == hipSuccess) e = dalloc(&ps->members, (size_t)n_capacity);
     if (e == hipSuccess) e = dalloc(&ps->ncount, (size_t)n_capacity);
-    if (e == hipSuccess) e = dalloc(&ps->n_alive, 1);
+    if (e == hipSuccess) e = dalloc(&ps->n_alive, 2);
     if (e == hipSuccess) e = dalloc(&ps->pairs, 2);
     if (e == hipSuccess) e = dalloc(&ps->h_range, 5);
     if (e == hipSuccess) e = hipMemsetAsync(ps->pairs, 0, 2 * sizeof(unsigned long long), ctx->stream);
-    if (e == hipSuccess) e = hipMemsetAsync(ps->n_alive, 0, sizeof(int32_t), ctx->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(ps->n_alive, 0, 2 * sizeof(int32_t), ctx->stream);
     if (e != hipSuccess) {
         eppx_ps_destroy(ps);
         return hip_fail(ctx, e, "eppx_ps_create");
@@ -598,7 +640,9 @@ int32_t eppx_ps_upload(eppx_ps *ps, int32_t n_particles, const int32_t *is_alive
             e = hipMemcpyAsync(d_fp, float_props, cap * L.n_float * sizeof(float), hipMemcpyHostToDevice, s);
         if (e == hipSuccess) e = hipMemcpyAsync(d_alive, is_alive, cap * sizeof(int32_t), hipMemcpyHostToDevice, s);
         if (e == hipSuccess) e = hipMemcpyAsync(d_cell, cell_index, cap * sizeof(int32_t), hipMemcpyHostToDevice, s);
-        if (e == hipSuccess) e = launch_from_ref(ps->buf[0], ps->cap, columns_of(ps), d_ip, d_fp, d_alive, d_cell, s);
+        // both buffers: the cell-order move rewrites only the slots in use
+        for (int b = 0; b < 2 && e == hipSuccess; ++b)
+            e = launch_from_ref(ps->buf[b], 0, columns_of(ps), d_ip, d_fp, d_alive, d_cell, 0, s);
     }
     ps->cur = 0;
     ps->n = n_particles;
@@ -620,24 +664,7 @@ int32_t eppx_ps_upload(eppx_ps *ps, int32_t n_particles, const int32_t *is_alive
     memcpy(&ps->tags_present, &range[3], sizeof(uint32_t));
     memcpy(&odd, &range[4], sizeof(odd));
     ps->odd_tags = odd != 0;
-    const double hmin = range[0], hmax = range[1];
-    if (bad || !has(L.H) || !(hmin >= 1e-30) || !(hmax <= 1e30) || !(hmin <= hmax)) {
-        ps->rmin2 = -1.0f;
-        ps->rmax2 = INFINITY;
-    } else if (hmin == hmax) {
-        // One H for all: the criterion compares dr2 with one threshold,
-        // evaluated here with the device's operations (Mean.harmonic, then
-        // *2 and squared; IEEE f32, no contraction).
-        const float h = range[0];
-        const float h2 = 2.0f * h * h / (h + h) * 2.0f;
-        const float t = h2 * h2;
-        ps->rmin2 = std::isfinite(t) ? nextafterf(t, INFINITY) : -1.0f;
-        ps->rmax2 = std::isfinite(t) ? -INFINITY : INFINITY;
-    } else {
-        const double eps = 1.0 / 1048576.0;  // 2^-20, well above the criterion's rounding
-        ps->rmin2 = nextafterf((float)(4.0 * hmin * hmin * (1.0 - eps)), 0.0f);
-        ps->rmax2 = nextafterf((float)(4.0 * hmax * hmax * (1.0 + eps)), INFINITY);
-    }
+    symmetry_bounds(!bad && has(L.H), range[0], range[1], &ps->rmin2, &ps->rmax2);
     // keep the columns the kernels do not own
     ps->pass_int.assign(ps->pass_int_cols.size() * cap, 0);
     ps->pass_float.assign(ps->pass_float_cols.size() * cap, 0.0f);
@@ -671,7 +698,7 @@ int32_t eppx_ps_download(eppx_ps *ps, eppx_ns *ns, int32_t *n_particles, int32_t
     hipStream_t s = ctx->stream;
     hipError_t e = hipMemsetAsync(d_ip, 0, std::max<size_t>(1, cap * L.n_int) * sizeof(int32_t), s);
     if (e == hipSuccess) e = hipMemsetAsync(d_fp, 0, std::max<size_t>(1, cap * L.n_float) * sizeof(float), s);
-    if (e == hipSuccess) e = launch_to_ref(ps->f(), ps->cap, columns_of(ps), d_ip, d_fp, d_alive, d_cell, s);
+    if (e == hipSuccess) e = launch_to_ref(ps->f(), ps->cap, columns_of(ps), d_ip, d_fp, d_alive, d_cell, -1, s);
     const int32_t M = L.max_neighbour_number;
     if (e == hipSuccess && ns && ps->searched && ps->last_ns == ns &&
         (has(L.nCount) || has(L.nIndex) || has(L.nRVec) || has(L.nR) || has(L.nW) || has(L.nDW) || has(L.nHInv))) {
@@ -1052,6 +1079,476 @@ int32_t eppx_check_finite(eppx_ps *ps) {
     if (e != hipSuccess) return hip_fail(ctx, e, "eppx_check_finite");
     if (bad) return set_error(ctx, EPPX_E_NONFINITE, "non-finite density, pressure, position or velocity");
     return EPPX_OK;
+}
+
+}  // extern "C"
+
+// ===================================================================== batch
+// Many independent cases in one set of device buffers (an eppx_ps that owns
+// the slots) and one segmented cell list: case c owns the global cells
+// [cell_base, cell_base + n_cells) and its particles sort after those of
+// every case with a smaller cell_base.  One launch per kernel per step
+// advances every case that has steps left; each case stops after exactly its
+// own step budget.  The physics is shared except for dt.
+
+namespace {
+
+struct BatchCase {
+    bool used = false;
+    int32_t n = 0;                   // rows of the case's reference tables
+    std::vector<int32_t> pass_int;   // columns the kernels do not own, n rows each
+    std::vector<float> pass_float;
+};
+
+}  // namespace
+
+struct eppx_batch {
+    eppx_ctx *ctx = nullptr;
+    eppx_ps *ps = nullptr;   // the slots of every case
+    eppx_ns ns{};            // adjacency and max_neighbour_number (the grids are per case)
+    int32_t max_cases = 0;
+    Physics ph{};
+    bool ph_set = false;
+    CaseDesc *d_cases = nullptr;
+    std::vector<CaseDesc> cases;  // what d_cases holds
+    std::vector<BatchCase> meta;
+    int32_t total_cells = 0;
+    uint64_t generation = 1;  // membership changes; recorded steps are only replayed for the same one
+    StepGraph graphs[2];
+};
+
+namespace {
+
+bool same_physics_class(const Physics &a, const Physics &b) {
+    Physics x = a, y = b;
+    x.dt = y.dt = 0.0f;
+    return memcmp(&x, &y, sizeof(x)) == 0;
+}
+
+void drop_batch_graphs(eppx_batch *b) {
+    if (b->graphs[0].exec || b->graphs[1].exec) (void)hipStreamSynchronize(b->ctx->stream);
+    for (auto &g : b->graphs) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        g = StepGraph{};
+    }
+}
+
+int32_t upload_cases(eppx_batch *b) {
+    eppx_ctx *ctx = b->ctx;
+    HIP_OK(ctx, hipMemcpyAsync(b->d_cases, b->cases.data(), b->cases.size() * sizeof(CaseDesc),
+                               hipMemcpyHostToDevice, ctx->stream));
+    HIP_OK(ctx, hipStreamSynchronize(ctx->stream));  // the host vector may change right after
+    return EPPX_OK;
+}
+
+// The device decrements steps_left; the host copy takes those counts before
+// it changes the table and writes it back.
+int32_t pull_steps(eppx_batch *b) {
+    eppx_ctx *ctx = b->ctx;
+    std::vector<CaseDesc> now(b->max_cases);
+    HIP_OK(ctx, hipMemcpyAsync(now.data(), b->d_cases, now.size() * sizeof(CaseDesc), hipMemcpyDeviceToHost,
+                               ctx->stream));
+    HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
+    for (int c = 0; c < b->max_cases; ++c) b->cases[c].steps_left = now[c].steps_left;
+    return EPPX_OK;
+}
+
+// Cell bases in slot order of the cases in use, the cell arrays sized for
+// them, and the table on the device.
+int32_t relayout(eppx_batch *b) {
+    int64_t base = 0;
+    for (int c = 0; c < b->max_cases; ++c) {
+        if (!b->meta[c].used) continue;
+        b->cases[c].cell_base = (int32_t)base;
+        base += b->cases[c].g.n_cells;
+    }
+    if (base > INT32_MAX - 3) return set_error(b->ctx, EPPX_E_CAPACITY, "batch: more than 2^31 cells");
+    b->total_cells = (int32_t)base;
+    int32_t rc = ensure_cells(b->ps, b->total_cells);
+    if (rc) return rc;
+    b->generation += 1;
+    drop_batch_graphs(b);
+    return upload_cases(b);
+}
+
+Walk batch_walk(const eppx_batch *b) {
+    const eppx_ps *ps = b->ps;
+    Walk w{};
+    w.cases = b->d_cases;
+    w.start = ps->start;
+    w.geo = ps->geo;
+    const Fields &f = ps->buf[ps->cur];
+    w.cell = f.cell;
+    w.tag = f.tag;
+    w.cs = f.cs;
+    w.fallback = ps->fallback;
+    w.n_alive = ps->n_alive;
+    memcpy(w.adj, b->ns.adj, sizeof(w.adj));
+    w.criterion = EPPX_CRIT_SYMMETRY;
+    return w;
+}
+
+// The segmented cell build and (search) the fused kernel + continuity action
+// for the cases `which` selects (0: with steps left, 1: waiting for a first
+// search).
+int32_t batch_cell_build(eppx_batch *b, bool search, int32_t which) {
+    eppx_ps *ps = b->ps;
+    eppx_ctx *ctx = b->ctx;
+    hipStream_t s = ctx->stream;
+    const int32_t nc = b->total_cells;
+    Fields cur = ps->buf[ps->cur], nxt = ps->buf[ps->cur ^ 1];
+    HIP_OK(ctx, timed(ctx, "cell_reset", [&] { return launch_reset_search(ps->counts, nc + 2, ps->pairs, s); }));
+    HIP_OK(ctx, timed(ctx, "cell_key", [&] {
+               return launch_key_count(cur, ps->n, CaseDesc{}, b->d_cases, nc, ps->counts, ps->key, ps->tmpslot,
+                                       ctx->d_flags, b->ns.M, s);
+           }));
+    HIP_OK(ctx, timed(ctx, "cell_scan", [&] { return launch_scan(ps->counts, nc + 2, ps->start, ps->tile_sums, s); }));
+    HIP_OK(ctx, timed(ctx, "cell_members", [&] {
+               return launch_fill_members(cur, ps->n, nc, ps->key, ps->tmpslot, ps->start, ps->members, s);
+           }));
+    HIP_OK(ctx, timed(ctx, "cell_permute", [&] {
+               return launch_permute(cur, nxt, ps->geo, ps->n, 1, nc, ps->key, ps->tmpslot, ps->start, ps->members,
+                                     ps->n_alive, ctx->d_flags, s);
+           }));
+    ps->cur ^= 1;
+    if (!search) return EPPX_OK;
+    const Walk w = batch_walk(b);
+    const int fast = ctx->math_mode == EPPX_MATH_FAST;
+    HIP_OK(ctx, timed(ctx, "search", [&] {
+               return launch_search(w, ps->f(), ps->n, EPPX_SEARCH_KERNEL_CONTINUITY, b->ph, fast, b->ns.M,
+                                    ps->ncount, &ps->pairs[0], &ps->pairs[1], ctx->d_flags, which, s);
+           }));
+    return EPPX_OK;
+}
+
+// One step of every case with steps left (main.jl:166-183 per case).
+int32_t batch_enqueue_step(eppx_batch *b) {
+    eppx_ps *ps = b->ps;
+    eppx_ctx *ctx = b->ctx;
+    hipStream_t s = ctx->stream;
+    const eppx_field_op op{};
+    const int fast = ctx->math_mode == EPPX_MATH_FAST;
+    HIP_OK(ctx, timed(ctx, "continuity_eos", [&] {
+               return launch_self(ps->f(), ps->n, EPPX_SELF_CONTINUITY_EOS, b->ph, op, b->d_cases, s);
+           }));
+    const Walk w = batch_walk(b);
+    HIP_OK(ctx, timed(ctx, "momentum", [&] {
+               return launch_interaction(w, ps->f(), ps->n, EPPX_INTER_MOMENTUM, b->ph, op, fast, s);
+           }));
+    HIP_OK(ctx, timed(ctx, "gravity_move", [&] {
+               return launch_self(ps->f(), ps->n, EPPX_SELF_GRAVITY_MOVE, b->ph, op, b->d_cases, s);
+           }));
+    int32_t rc = batch_cell_build(b, true, 0);
+    if (rc) return rc;
+    HIP_OK(ctx, launch_tick(b->d_cases, b->max_cases, s));
+    return EPPX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t eppx_batch_bytes_per_particle(int32_t *out) {
+    if (!out) return EPPX_E_ARG;
+    // two buffers of 80 B (the 76 B of particle state and the case id), the
+    // 16 B search snapshot, key, provisional slot, cell member and count
+    *out = 2 * 80 + 16 + 4 * 4;
+    return EPPX_OK;
+}
+
+int32_t eppx_batch_create(eppx_ctx *ctx, const eppx_layout *layout, int32_t particle_capacity, int32_t case_capacity,
+                          const int32_t *adjacency, int32_t max_tag, int32_t max_neighbour_number, eppx_batch **out) {
+    if (!ctx || !layout || !out || particle_capacity < 0 || case_capacity < 1 || !adjacency || max_tag < 1 ||
+        max_tag > kMaxTag || max_neighbour_number < 0)
+        return set_error(ctx, EPPX_E_ARG, "eppx_batch_create: bad argument");
+    *out = nullptr;
+    eppx_ps *ps = nullptr;
+    int32_t rc = eppx_ps_create(ctx, particle_capacity, layout, &ps);
+    if (rc) return rc;
+    eppx_batch *b = new eppx_batch();
+    b->ctx = ctx;
+    b->ps = ps;
+    ps->n = 0;
+    b->max_cases = case_capacity;
+    b->ns.ctx = ctx;
+    b->ns.max_tag = max_tag;
+    b->ns.M = max_neighbour_number;
+    for (int t = 1; t <= max_tag; ++t)
+        for (int u = 1; u <= max_tag; ++u)
+            if (adjacency[(t - 1) + (size_t)max_tag * (u - 1)] == 1) b->ns.adj[t] |= 1u << (u - 1);
+    b->cases.assign(case_capacity, CaseDesc{});
+    b->meta.assign(case_capacity, BatchCase{});
+    hipError_t e = dalloc(&b->d_cases, (size_t)case_capacity);
+    if (e != hipSuccess) {
+        eppx_batch_destroy(b);
+        return hip_fail(ctx, e, "eppx_batch_create");
+    }
+    if ((rc = relayout(b))) {
+        eppx_batch_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return EPPX_OK;
+}
+
+int32_t eppx_batch_destroy(eppx_batch *b) {
+    if (!b) return EPPX_E_ARG;
+    (void)hipSetDevice(b->ctx->device);
+    drop_batch_graphs(b);
+    dfree(b->d_cases);
+    eppx_ps *ps = b->ps;
+    delete b;
+    return ps ? eppx_ps_destroy(ps) : EPPX_OK;
+}
+
+int32_t eppx_batch_add(eppx_batch *b, const eppx_domain2d *domain, const eppx_physics *phys, int32_t n_particles,
+                       const int32_t *is_alive, const int32_t *cell_index, const int32_t *int_props,
+                       const float *float_props, int32_t steps, int32_t *case_id) {
+    if (!b || !domain || !phys || !case_id || n_particles < 0 || steps < 0 || !is_alive || !cell_index)
+        return EPPX_E_ARG;
+    eppx_ctx *ctx = b->ctx;
+    eppx_ps *ps = b->ps;
+    const eppx_layout &L = ps->layout;
+    if ((L.n_int && !int_props) || (L.n_float && !float_props))
+        return set_error(ctx, EPPX_E_ARG, "eppx_batch_add: missing property table");
+    Physics ph{};
+    int32_t rc = physics_arg(ctx, phys, sizeof(*phys), &ph);
+    if (rc) return rc;
+    if (b->ph_set && !same_physics_class(ph, b->ph))
+        return set_error(ctx, EPPX_E_ARG, "eppx_batch_add: physics differs from the batch's (only dt may differ)");
+    rc = check_columns(ps, {{L.Density, "Density"}, {L.dDensity, "dDensity"}, {L.Volume, "Volume"},
+                            {L.Mass, "Mass"}, {L.Pressure, "Pressure"}, {L.VelocityVec, "VelocityVec"},
+                            {L.dVelocityVec, "dVelocityVec"}, {L.H, "H"}},
+                       "eppx_batch_add");
+    if (rc) return rc;
+    if (domain->n_x < 1 || domain->n_y < 1 || domain->n != domain->n_x * domain->n_y)
+        return set_error(ctx, EPPX_E_ARG, "eppx_batch_add: bad domain");
+    int c = 0;
+    while (c < b->max_cases && b->meta[c].used) ++c;
+    if (c == b->max_cases) return set_error(ctx, EPPX_E_CAPACITY, "eppx_batch_add: every case slot is in use");
+    if ((int64_t)ps->n + n_particles > ps->cap) {
+        char buf[160];
+        snprintf(buf, sizeof(buf), "eppx_batch_add: %d particles do not fit (%d of %d slots in use)", n_particles,
+                 ps->n, ps->cap);
+        return set_error(ctx, EPPX_E_CAPACITY, buf);
+    }
+    HIP_OK(ctx, hipSetDevice(ctx->device));
+    if ((rc = pull_steps(b))) return rc;
+    // H range over alive particles and the tags present, on the host
+    const size_t rows = (size_t)n_particles;
+    float hmin = INFINITY, hmax = 0.0f;
+    bool bad = false, odd = false;
+    uint32_t tags = 0;
+    for (size_t r = 0; r < rows; ++r) {
+        const int32_t t = int_props[r + rows * (size_t)L.Tag];
+        const uint32_t t1 = (uint32_t)t - 1u;
+        if (t1 < (uint32_t)kMaxTag) tags |= 1u << t1;
+        else odd = true;
+        if (is_alive[r] != 1) continue;
+        const float h = float_props[r + rows * (size_t)L.H];
+        if (!(h > 0.0f) || !std::isfinite(h)) bad = true;
+        else {
+            hmin = std::min(hmin, h);
+            hmax = std::max(hmax, h);
+        }
+    }
+    CaseDesc cd{};
+    const eppx_domain2d &d = *domain;
+    cd.g = Grid{d.first_x, d.first_y, d.last_x, d.last_y, d.gap_x_inv, d.gap_y_inv, d.gap_square, d.n_x, d.n_y, d.n};
+    symmetry_bounds(!bad, hmin, hmax, &cd.rmin2, &cd.rmax2);
+    cd.tags_present = tags;
+    cd.check_all = (cd.rmin2 <= cd.rmax2) || odd;
+    cd.dt = ph.dt;
+    cd.steps_left = steps;
+    cd.fresh = 1;
+    // the case's tables into slots [n, n + n_particles) of the current buffer
+    if (n_particles) {
+        int32_t *d_ip = nullptr, *d_alive = nullptr, *d_cell = nullptr;
+        float *d_fp = nullptr;
+        HIP_OK(ctx, dalloc(&d_ip, rows * (size_t)L.n_int));
+        HIP_OK(ctx, dalloc(&d_fp, rows * (size_t)L.n_float));
+        HIP_OK(ctx, dalloc(&d_alive, rows));
+        HIP_OK(ctx, dalloc(&d_cell, rows));
+        hipStream_t s = ctx->stream;
+        hipError_t e = hipSuccess;
+        if (L.n_int) e = hipMemcpyAsync(d_ip, int_props, rows * L.n_int * sizeof(int32_t), hipMemcpyHostToDevice, s);
+        if (e == hipSuccess && L.n_float)
+            e = hipMemcpyAsync(d_fp, float_props, rows * L.n_float * sizeof(float), hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(d_alive, is_alive, rows * sizeof(int32_t), hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(d_cell, cell_index, rows * sizeof(int32_t), hipMemcpyHostToDevice, s);
+        Columns cols = columns_of(ps);
+        cols.n_capacity = n_particles;
+        if (e == hipSuccess) e = launch_from_ref(ps->f(), ps->n, cols, d_ip, d_fp, d_alive, d_cell, c, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        dfree(d_ip);
+        dfree(d_fp);
+        dfree(d_alive);
+        dfree(d_cell);
+        if (e != hipSuccess) return hip_fail(ctx, e, "eppx_batch_add");
+    }
+    BatchCase &m = b->meta[c];
+    m.used = true;
+    m.n = n_particles;
+    m.pass_int.assign(ps->pass_int_cols.size() * rows, 0);
+    m.pass_float.assign(ps->pass_float_cols.size() * rows, 0.0f);
+    for (size_t k = 0; k < ps->pass_int_cols.size(); ++k)
+        memcpy(&m.pass_int[k * rows], int_props + (size_t)ps->pass_int_cols[k] * rows, rows * sizeof(int32_t));
+    for (size_t k = 0; k < ps->pass_float_cols.size(); ++k)
+        memcpy(&m.pass_float[k * rows], float_props + (size_t)ps->pass_float_cols[k] * rows, rows * sizeof(float));
+    ps->n += n_particles;
+    b->cases[c] = cd;
+    if (!b->ph_set) {
+        b->ph = ph;
+        b->ph_set = true;
+    }
+    if ((rc = relayout(b))) return rc;
+    // the case's first search (main.jl:160), before it takes part in steps
+    if ((rc = batch_cell_build(b, true, 1))) return rc;
+    b->cases[c].fresh = 0;
+    if ((rc = upload_cases(b))) return rc;
+    *case_id = c;
+    return read_flags(ctx);
+}
+
+int32_t eppx_batch_step(eppx_batch *b, int32_t n_steps) {
+    if (!b || n_steps < 0) return EPPX_E_ARG;
+    eppx_ctx *ctx = b->ctx;
+    eppx_ps *ps = b->ps;
+    HIP_OK(ctx, hipSetDevice(ctx->device));
+    if (!b->ph_set || ps->n == 0) return EPPX_OK;
+    for (int32_t k = 0; k < n_steps; ++k) {
+        int32_t rc;
+        if (ctx->timing) {
+            if ((rc = batch_enqueue_step(b))) return rc;
+            continue;
+        }
+        StepGraph &g = b->graphs[ps->cur];
+        if (!g.exec || g.ns_serial != b->generation || g.math != ctx->math_mode) {
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+            g = StepGraph{};
+            const int cur0 = ps->cur;
+            HIP_OK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+            rc = batch_enqueue_step(b);
+            hipGraph_t graph = nullptr;
+            const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+            ps->cur = cur0;  // recorded, not run yet
+            if (rc == EPPX_OK && e != hipSuccess) rc = hip_fail(ctx, e, "eppx_batch_step: hipStreamEndCapture");
+            if (rc == EPPX_OK) {
+                const hipError_t ei = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+                if (ei != hipSuccess) rc = hip_fail(ctx, ei, "eppx_batch_step: hipGraphInstantiate");
+            }
+            if (graph) (void)hipGraphDestroy(graph);
+            if (rc) {
+                g = StepGraph{};
+                return rc;
+            }
+            g.ns_serial = b->generation;
+            g.math = ctx->math_mode;
+        }
+        HIP_OK(ctx, hipGraphLaunch(g.exec, ctx->stream));
+        ps->cur ^= 1;
+    }
+    return EPPX_OK;
+}
+
+int32_t eppx_batch_steps_left(eppx_batch *b, int32_t *steps_left) {
+    if (!b || !steps_left) return EPPX_E_ARG;
+    eppx_ctx *ctx = b->ctx;
+    HIP_OK(ctx, hipSetDevice(ctx->device));
+    std::vector<CaseDesc> now(b->max_cases);
+    HIP_OK(ctx, hipMemcpyAsync(now.data(), b->d_cases, now.size() * sizeof(CaseDesc), hipMemcpyDeviceToHost,
+                               ctx->stream));
+    HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
+    for (int c = 0; c < b->max_cases; ++c) {
+        steps_left[c] = b->meta[c].used ? now[c].steps_left : -1;
+        b->cases[c].steps_left = now[c].steps_left;
+    }
+    return read_flags(ctx);
+}
+
+int32_t eppx_batch_download(eppx_batch *b, int32_t case_id, int32_t *n_particles, int32_t *is_alive,
+                            int32_t *cell_index, int32_t *int_props, float *float_props) {
+    if (!b || case_id < 0 || case_id >= b->max_cases || !n_particles || !is_alive || !cell_index) return EPPX_E_ARG;
+    eppx_ctx *ctx = b->ctx;
+    eppx_ps *ps = b->ps;
+    const BatchCase &m = b->meta[case_id];
+    if (!m.used) return set_error(ctx, EPPX_E_STATE, "eppx_batch_download: no such case");
+    const eppx_layout &L = ps->layout;
+    if ((L.n_int && !int_props) || (L.n_float && !float_props)) return EPPX_E_ARG;
+    HIP_OK(ctx, hipSetDevice(ctx->device));
+    const size_t rows = (size_t)m.n;
+    *n_particles = m.n;
+    if (!rows) return EPPX_OK;
+    int32_t *d_ip = nullptr, *d_alive = nullptr, *d_cell = nullptr;
+    float *d_fp = nullptr;
+    HIP_OK(ctx, dalloc(&d_ip, std::max<size_t>(1, rows * (size_t)L.n_int)));
+    HIP_OK(ctx, dalloc(&d_fp, std::max<size_t>(1, rows * (size_t)L.n_float)));
+    HIP_OK(ctx, dalloc(&d_alive, rows));
+    HIP_OK(ctx, dalloc(&d_cell, rows));
+    hipStream_t s = ctx->stream;
+    hipError_t e = hipMemsetAsync(d_ip, 0, std::max<size_t>(1, rows * L.n_int) * sizeof(int32_t), s);
+    if (e == hipSuccess) e = hipMemsetAsync(d_fp, 0, std::max<size_t>(1, rows * L.n_float) * sizeof(float), s);
+    Columns cols = columns_of(ps);
+    cols.n_capacity = m.n;
+    if (e == hipSuccess) e = launch_to_ref(ps->f(), ps->n, cols, d_ip, d_fp, d_alive, d_cell, case_id, s);
+    if (e == hipSuccess && L.n_int)
+        e = hipMemcpyAsync(int_props, d_ip, rows * L.n_int * sizeof(int32_t), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && L.n_float)
+        e = hipMemcpyAsync(float_props, d_fp, rows * L.n_float * sizeof(float), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(is_alive, d_alive, rows * sizeof(int32_t), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(cell_index, d_cell, rows * sizeof(int32_t), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    dfree(d_ip);
+    dfree(d_fp);
+    dfree(d_alive);
+    dfree(d_cell);
+    if (e != hipSuccess) return hip_fail(ctx, e, "eppx_batch_download");
+    for (size_t k = 0; k < ps->pass_int_cols.size(); ++k)
+        memcpy(int_props + (size_t)ps->pass_int_cols[k] * rows, &m.pass_int[k * rows], rows * sizeof(int32_t));
+    for (size_t k = 0; k < ps->pass_float_cols.size(); ++k)
+        memcpy(float_props + (size_t)ps->pass_float_cols[k] * rows, &m.pass_float[k * rows], rows * sizeof(float));
+    return read_flags(ctx);
+}
+
+int32_t eppx_batch_remove(eppx_batch *b, int32_t case_id) {
+    if (!b || case_id < 0 || case_id >= b->max_cases) return EPPX_E_ARG;
+    eppx_ctx *ctx = b->ctx;
+    eppx_ps *ps = b->ps;
+    if (!b->meta[case_id].used) return set_error(ctx, EPPX_E_STATE, "eppx_batch_remove: no such case");
+    HIP_OK(ctx, hipSetDevice(ctx->device));
+    int32_t rc = pull_steps(b);
+    if (rc) return rc;
+    // the case's slots sort last at a cell build, then the slot count drops to the kept ones
+    b->cases[case_id].removed = 1;
+    b->cases[case_id].steps_left = 0;
+    if ((rc = upload_cases(b))) return rc;
+    if (ps->n) {
+        if ((rc = batch_cell_build(b, false, 0))) return rc;
+        int32_t kept = 0;
+        HIP_OK(ctx, hipMemcpyAsync(&kept, ps->n_alive + 1, sizeof(kept), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
+        ps->n = kept;
+    }
+    b->meta[case_id] = BatchCase{};
+    b->cases[case_id] = CaseDesc{};
+    if ((rc = relayout(b))) return rc;
+    return read_flags(ctx);
+}
+
+int32_t eppx_batch_info(eppx_batch *b, int32_t *slots_in_use, int32_t *cases_in_use) {
+    if (!b) return EPPX_E_ARG;
+    if (slots_in_use) *slots_in_use = b->ps->n;
+    if (cases_in_use) {
+        int32_t k = 0;
+        for (const auto &m : b->meta) k += m.used;
+        *cases_in_use = k;
+    }
+    return EPPX_OK;
+}
+
+int32_t eppx_batch_pair_total(eppx_batch *b, int64_t *out, int32_t reset) {
+    if (!b) return EPPX_E_ARG;
+    return eppx_pair_total(b->ps, out, reset);
 }
 
 }  // extern "C"

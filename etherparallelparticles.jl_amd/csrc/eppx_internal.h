@@ -27,29 +27,47 @@ struct Grid {
     int32_t n_x, n_y, n_cells;
 };
 
-// One particle system's structure-of-arrays state; slot s holds reference
-// row id[s].  After a search, slots [0, n_alive) are alive and sorted by
-// (cell, id); dead particles follow.
+// One case (an independent particle system) of a set of device buffers.  A
+// particle system holds one case; a batch holds many, side by side in one
+// segmented cell list: case c owns the global cells
+// [cell_base, cell_base + g.n_cells), and its particles sort before those of
+// every later case.
+struct CaseDesc {
+    Grid g;
+    int32_t cell_base;
+    float rmin2, rmax2;     // criterion accepts every dr2 < rmin2, rejects every dr2 > rmax2
+    uint32_t tags_present;  // tag bits of the case's particles (a superset)
+    int32_t check_all;      // the band is not empty, or some tag lies outside 1..kMaxTag
+    float dt;               // batch: the case's time step (a particle system uses the call's physics)
+    int32_t steps_left;     // > 0: the case advances; a batch step decrements it
+    int32_t fresh;          // batch: the case waits for its first search
+    int32_t removed;        // batch: the case's slots are released at the next cell build
+};
+
+// Structure-of-arrays particle state; slot s holds row id[s] of case cs[s].
+// After a search, slots [0, n_alive) are alive and sorted by (global cell,
+// id); dead particles follow, then (batch) released ones.
 struct Fields {
     float2 *pos, *dvel;
     float4 *dyn;  // (u, v, density, pressure): what a neighbour's pair term reads, one 16-byte record
     float *mass, *vol, *drho, *h, *wv, *wvrho;
-    int32_t *tag, *mov, *id, *cell, *alive;
+    int32_t *tag, *mov, *id, *cell, *alive, *cs;
 };
 
-// Neighbour-walk inputs shared by the pair kernels.
+// Neighbour-walk inputs shared by the pair kernels.  A particle system
+// passes its one case by value (one, cases == nullptr); a batch passes its
+// device case table and each slot's case.
 struct Walk {
-    Grid g;
-    const int32_t *start;     // (n_cells + 2) exclusive scan of cell counts; slot ranges
+    CaseDesc one;
+    const CaseDesc *cases;
+    const int32_t *start;     // exclusive scan of the global cell counts; slot ranges
     const float4 *geo;        // (x, y, h, m) at the last search, sorted order
     const int32_t *tag;       // tags, sorted order
-    const int32_t *cell;
-    const int32_t *n_alive;   // device scalar
+    const int32_t *cell;      // cell within the particle's case (1-based), sorted order
+    const int32_t *cs;        // case of each slot, sorted order
+    const int32_t *n_alive;   // device scalars: [0] alive slots, [1] kept slots
     uint32_t adj[kMaxTag + 1];  // adj[t]: bit (u-1) set when tag t is adjacent to tag u
     int32_t criterion;
-    float rmin2, rmax2;       // criterion accepts every dr2 < rmin2, rejects every dr2 > rmax2
-    uint32_t tags_present;    // tag bits of the particles (a superset)
-    int32_t check_all;        // the band is not empty, or some tag lies outside 1..kMaxTag
     uint32_t *fallback;       // blocks that walked global memory instead of LDS (counter)
 };
 
@@ -86,28 +104,40 @@ struct NeighbourExport {
 // All launchers enqueue on `stream` and return the hipError_t of the launch.
 hipError_t launch_fill_i32(int32_t *p, int32_t v, int64_t n, hipStream_t s);
 hipError_t launch_reset_search(int32_t *counts, int32_t n_counts, unsigned long long *pairs_last, hipStream_t s);
-hipError_t launch_key_count(Fields f, int32_t n, Grid g, int32_t *counts, int32_t *key, int32_t *tmpslot,
-                            uint32_t *flags, int32_t max_per_cell, hipStream_t s);
+// Cell keys: cell_base + cell - 1 of the particle's case (`one`, or
+// cases[cs[s]] when cases is given) for alive particles, n_cells (all cases)
+// for dead ones, n_cells + 1 for released ones.
+hipError_t launch_key_count(Fields f, int32_t n, const CaseDesc &one, const CaseDesc *cases, int32_t n_cells,
+                            int32_t *counts, int32_t *key, int32_t *tmpslot, uint32_t *flags, int32_t max_per_cell,
+                            hipStream_t s);
 hipError_t launch_scan(const int32_t *counts, int32_t n, int32_t *start, int32_t *block_sums, hipStream_t s);
 hipError_t launch_fill_members(const Fields f, int32_t n, int32_t n_cells, const int32_t *key,
                                const int32_t *tmpslot, const int32_t *start, int32_t *members, hipStream_t s);
-hipError_t launch_permute(Fields cur, Fields nxt, float4 *geo, int32_t n, int32_t cap, int32_t n_cells,
+hipError_t launch_permute(Fields cur, Fields nxt, float4 *geo, int32_t n, int32_t batch, int32_t n_cells,
                           const int32_t *key, const int32_t *tmpslot, const int32_t *start,
                           const int32_t *members, int32_t *n_alive, uint32_t *flags, hipStream_t s);
+// which = 0: the action runs for particles of cases with steps left; 1: of fresh cases.
 hipError_t launch_search(const Walk &w, Fields f, int32_t n, int32_t action, const Physics &ph, int32_t fast,
                          int32_t M, int32_t *ncount, unsigned long long *pairs_last,
-                         unsigned long long *pairs_total, uint32_t *flags, hipStream_t s);
+                         unsigned long long *pairs_total, uint32_t *flags, int32_t which, hipStream_t s);
 hipError_t launch_snapshot_mass(float4 *geo, Fields f, int32_t n, hipStream_t s);
+// cases == nullptr: every alive particle, dt from ph; otherwise particles of
+// cases with steps left, dt from their case.
 hipError_t launch_self(Fields f, int32_t n, int32_t action, const Physics &ph, const eppx_field_op &op,
-                       hipStream_t s);
+                       const CaseDesc *cases, hipStream_t s);
+hipError_t launch_tick(CaseDesc *cases, int32_t n_cases, hipStream_t s);
+hipError_t launch_set_case(CaseDesc *d, const CaseDesc &v, hipStream_t s);
 hipError_t launch_interaction(const Walk &w, Fields f, int32_t n, int32_t action, const Physics &ph,
                               const eppx_field_op &op, int32_t fast, hipStream_t s);
 hipError_t launch_export(const Walk &w, Fields f, int32_t n, const int64_t *row_start, int32_t *js,
                          const int32_t *ncount, NeighbourExport ex, hipStream_t s);
-hipError_t launch_from_ref(Fields f, int32_t cap, Columns c, const int32_t *ip, const float *fp,
-                           const int32_t *is_alive, const int32_t *cell_index, hipStream_t s);
-hipError_t launch_to_ref(Fields f, int32_t cap, Columns c, int32_t *ip, float *fp, int32_t *is_alive,
-                         int32_t *cell_index, hipStream_t s);
+// Rows [0, c.n_capacity) of one case's reference tables into slots
+// [offset, offset + c.n_capacity), as case `cs`.
+hipError_t launch_from_ref(Fields f, int32_t offset, Columns c, const int32_t *ip, const float *fp,
+                           const int32_t *is_alive, const int32_t *cell_index, int32_t cs, hipStream_t s);
+// Slots [0, n) of case `cs` (every slot when cs < 0) back into reference rows id[s].
+hipError_t launch_to_ref(Fields f, int32_t n, Columns c, int32_t *ip, float *fp, int32_t *is_alive,
+                         int32_t *cell_index, int32_t cs, hipStream_t s);
 hipError_t launch_stencil(Grid g, int32_t *count, int32_t *list, hipStream_t s);
 hipError_t launch_finite(Fields f, int32_t n, uint32_t *bad, hipStream_t s);
 hipError_t launch_h_range(Fields f, int32_t n, float *hmin_hmax, hipStream_t s);

@@ -138,15 +138,15 @@ __device__ __forceinline__ uint32_t adj_of(const Walk &w, uint32_t tb) {
 // sets rmin2 / rmax2 so that the criterion accepts every dr2 < rmin2 and
 // rejects every dr2 > rmax2 (rmax2 < rmin2 when that decides all of them);
 // only dr2 in the band between them evaluates the criterion itself.
-__device__ __forceinline__ bool criterion_exact(const Walk &w, float hi, float hj, float dr2) {
-    if (w.criterion == EPPX_CRIT_DEFAULT) return dr2 <= w.g.gap_square;  // defaultSearchCriterion
+__device__ __forceinline__ bool criterion_exact(const Walk &w, const CaseDesc &cd, float hi, float hj, float dr2) {
+    if (w.criterion == EPPX_CRIT_DEFAULT) return dr2 <= cd.g.gap_square;  // defaultSearchCriterion
     const float h2 = harmonic(hi, hj) * 2.0f;                            // symmetrySearchCriterion
     return dr2 <= h2 * h2;
 }
-__device__ __forceinline__ bool accept_dr2(const Walk &w, float hi, float hj, float dr2) {
-    if (dr2 < w.rmin2) return true;
-    if (!(dr2 <= w.rmax2)) return false;
-    return criterion_exact(w, hi, hj, dr2);
+__device__ __forceinline__ bool accept_dr2(const Walk &w, const CaseDesc &cd, float hi, float hj, float dr2) {
+    if (dr2 < cd.rmin2) return true;
+    if (!(dr2 <= cd.rmax2)) return false;
+    return criterion_exact(w, cd, hi, hj, dr2);
 }
 
 // ------------------------------------------------------------------ walk
@@ -156,18 +156,18 @@ __device__ __forceinline__ bool accept_dr2(const Walk &w, float hi, float hj, fl
 // ascending particle index inside a cell (the order of a serial insert).
 // visit(j, tag bit of j, dx, dy, dr2).
 template <class Visit>
-__device__ __forceinline__ void walk(const Walk &w, int i, float4 gi, Visit &&visit) {
+__device__ __forceinline__ void walk(const Walk &w, const CaseDesc &cd, int i, float4 gi, Visit &&visit) {
     const int c = w.cell[i] - 1;
-    const int cx = c % w.g.n_x, cy = c / w.g.n_x;
+    const int cx = c % cd.g.n_x, cy = c / cd.g.n_x;
     const uint32_t adj = adj_of(w, tag_bit(w.tag[i]));
     const float2 xi = make_float2(gi.x, gi.y);
     for (int di = -1; di <= 1; ++di) {
         const int ii = cx + di;
-        if (ii < 0 || ii >= w.g.n_x) continue;
+        if (ii < 0 || ii >= cd.g.n_x) continue;
         for (int dj = -1; dj <= 1; ++dj) {
             const int jj = cy + dj;
-            if (jj < 0 || jj >= w.g.n_y) continue;
-            const int k = ii + w.g.n_x * jj;
+            if (jj < 0 || jj >= cd.g.n_y) continue;
+            const int k = cd.cell_base + ii + cd.g.n_x * jj;
             const int b = w.start[k], e = w.start[k + 1];
             for (int j = b; j < e; ++j) {
                 if (j == i) continue;
@@ -176,7 +176,7 @@ __device__ __forceinline__ void walk(const Walk &w, int i, float4 gi, Visit &&vi
                 const float4 gj = w.geo[j];
                 const float dx = xi.x - gj.x, dy = xi.y - gj.y;
                 const float dr2 = dx * dx + dy * dy;
-                if (accept_dr2(w, gi.z, gj.z, dr2)) visit(j, tbj, dx, dy, dr2);
+                if (accept_dr2(w, cd, gi.z, gj.z, dr2)) visit(j, tbj, dx, dy, dr2);
             }
         }
     }
@@ -209,7 +209,7 @@ __device__ __forceinline__ int sel3(int r, int a, int b, int c) { return r == 0 
 // in the band take the exact criterion (one branch per group, rarely taken)
 // and non-adjacent tags are cleared.
 template <bool CHECK, int N>
-__device__ __forceinline__ uint32_t accept_bits(const Walk &w, float2 xi, float hi, uint32_t adj,
+__device__ __forceinline__ uint32_t accept_bits(const Walk &w, const CaseDesc &cd, float2 xi, float hi, uint32_t adj,
                                                 const float4 (&g)[N], const uint32_t (&tb)[N]) {
     float d[N];
     uint32_t ok = 0u, band = 0u;
@@ -217,14 +217,14 @@ __device__ __forceinline__ uint32_t accept_bits(const Walk &w, float2 xi, float 
     for (int k = 0; k < N; ++k) {
         const float dx = xi.x - g[k].x, dy = xi.y - g[k].y;
         d[k] = dx * dx + dy * dy;
-        ok |= (uint32_t)(d[k] < w.rmin2) << k;
-        if (CHECK) band |= (uint32_t)(d[k] >= w.rmin2 && d[k] <= w.rmax2) << k;
+        ok |= (uint32_t)(d[k] < cd.rmin2) << k;
+        if (CHECK) band |= (uint32_t)(d[k] >= cd.rmin2 && d[k] <= cd.rmax2) << k;
     }
     if (CHECK) {
         if (band) {
 #pragma unroll
             for (int k = 0; k < N; ++k)
-                if ((band >> k) & 1u) ok |= (uint32_t)criterion_exact(w, hi, g[k].z, d[k]) << k;
+                if ((band >> k) & 1u) ok |= (uint32_t)criterion_exact(w, cd, hi, g[k].z, d[k]) << k;
         }
 #pragma unroll
         for (int k = 0; k < N; ++k) ok &= ~((uint32_t)((adj & tb[k]) == 0u) << k);
@@ -255,8 +255,8 @@ struct TileRows {
 };
 
 template <bool CHECK, class Src>
-__device__ __forceinline__ unsigned long long accept_row(const Walk &w, const Src &src, int r, int b, int e,
-                                                         float2 xi, float hi, uint32_t adj) {
+__device__ __forceinline__ unsigned long long accept_row(const Walk &w, const CaseDesc &cd, const Src &src, int r,
+                                                         int b, int e, float2 xi, float hi, uint32_t adj) {
     unsigned long long m = 0ull;
     int j = b;
     const int o = src.off(r);
@@ -267,21 +267,21 @@ __device__ __forceinline__ unsigned long long accept_row(const Walk &w, const Sr
 #pragma unroll
             for (int k = 0; k < 4; ++k) tb[k] = src.tbit(j + k + o);
         }
-        m |= (unsigned long long)accept_bits<CHECK, 4>(w, xi, hi, adj, g, tb) << (j - b);
+        m |= (unsigned long long)accept_bits<CHECK, 4>(w, cd, xi, hi, adj, g, tb) << (j - b);
     }
     for (; j < e; ++j) {
         const float4 g[1] = {src.get(j + o)};
         const uint32_t tb[1] = {CHECK ? src.tbit(j + o) : 0u};
-        m |= (unsigned long long)accept_bits<CHECK, 1>(w, xi, hi, adj, g, tb) << (j - b);
+        m |= (unsigned long long)accept_bits<CHECK, 1>(w, cd, xi, hi, adj, g, tb) << (j - b);
     }
     return m;
 }
 
 template <bool ORDERED, bool COUNT, class Src, class Fetch, class Apply>
-__device__ __forceinline__ int walk_rows(const Walk &w, const Src &src, int i, float4 gi, uint32_t keep_tags,
-                                         Fetch &&fetch, Apply &&apply) {
+__device__ __forceinline__ int walk_rows(const Walk &w, const CaseDesc &cd, const Src &src, int i, float4 gi,
+                                         uint32_t keep_tags, Fetch &&fetch, Apply &&apply) {
     const int c = w.cell[i] - 1;
-    const int nx = w.g.n_x;
+    const int nx = cd.g.n_x, base = cd.cell_base;
     const int cx = c % nx, cy = c / nx;
     const int x0 = max(cx - 1, 0), x1 = min(cx + 1, nx - 1);
     int b[3], e[3];
@@ -289,14 +289,14 @@ __device__ __forceinline__ int walk_rows(const Walk &w, const Src &src, int i, f
     for (int r = 0; r < 3; ++r) {
         const int jj = cy + r - 1;
         b[r] = e[r] = 0;
-        if (jj >= 0 && jj < w.g.n_y) {
-            b[r] = w.start[x0 + nx * jj];
-            e[r] = w.start[x1 + 1 + nx * jj];
+        if (jj >= 0 && jj < cd.g.n_y) {
+            b[r] = w.start[base + x0 + nx * jj];
+            e[r] = w.start[base + x1 + 1 + nx * jj];
         }
     }
     if (e[0] - b[0] > 64 || e[1] - b[1] > 64 || e[2] - b[2] > 64) {
         int cnt = 0;
-        walk(w, i, gi, [&](int j, uint32_t tbj, float dx, float dy, float dr2) {
+        walk(w, cd, i, gi, [&](int j, uint32_t tbj, float dx, float dy, float dr2) {
             ++cnt;
             if (tbj & keep_tags) {
                 const float4 gj = w.geo[j];
@@ -310,25 +310,25 @@ __device__ __forceinline__ int walk_rows(const Walk &w, const Src &src, int i, f
     const float2 xi = make_float2(gi.x, gi.y);
     // The band and the tag test are skipped for the whole wave when no lane
     // needs them: the band is empty and every tag present is in kadj.
-    const bool check = __ballot(w.check_all || (w.tags_present & ~kadj) != 0u) != 0ull;
+    const bool check = __ballot(cd.check_all || (cd.tags_present & ~kadj) != 0u) != 0ull;
     unsigned long long m0, m1, m2;
     if (check) {
-        m0 = accept_row<true>(w, src, 0, b[0], e[0], xi, gi.z, kadj);
-        m1 = accept_row<true>(w, src, 1, b[1], e[1], xi, gi.z, kadj);
-        m2 = accept_row<true>(w, src, 2, b[2], e[2], xi, gi.z, kadj);
+        m0 = accept_row<true>(w, cd, src, 0, b[0], e[0], xi, gi.z, kadj);
+        m1 = accept_row<true>(w, cd, src, 1, b[1], e[1], xi, gi.z, kadj);
+        m2 = accept_row<true>(w, cd, src, 2, b[2], e[2], xi, gi.z, kadj);
     } else {
-        m0 = accept_row<false>(w, src, 0, b[0], e[0], xi, gi.z, kadj);
-        m1 = accept_row<false>(w, src, 1, b[1], e[1], xi, gi.z, kadj);
-        m2 = accept_row<false>(w, src, 2, b[2], e[2], xi, gi.z, kadj);
+        m0 = accept_row<false>(w, cd, src, 0, b[0], e[0], xi, gi.z, kadj);
+        m1 = accept_row<false>(w, cd, src, 1, b[1], e[1], xi, gi.z, kadj);
+        m2 = accept_row<false>(w, cd, src, 2, b[2], e[2], xi, gi.z, kadj);
     }
     const unsigned long long self = 1ull << (i - b[1]);
     m1 &= ~self;  // I != J
     int cnt = 0;
     if (COUNT) {
         if (adj & ~keep_tags) {  // some adjacent tags are counted but not visited
-            cnt = __popcll(accept_row<true>(w, src, 0, b[0], e[0], xi, gi.z, adj)) +
-                  __popcll(accept_row<true>(w, src, 1, b[1], e[1], xi, gi.z, adj) & ~self) +
-                  __popcll(accept_row<true>(w, src, 2, b[2], e[2], xi, gi.z, adj));
+            cnt = __popcll(accept_row<true>(w, cd, src, 0, b[0], e[0], xi, gi.z, adj)) +
+                  __popcll(accept_row<true>(w, cd, src, 1, b[1], e[1], xi, gi.z, adj) & ~self) +
+                  __popcll(accept_row<true>(w, cd, src, 2, b[2], e[2], xi, gi.z, adj));
         } else {
             cnt = __popcll(m0) + __popcll(m1) + __popcll(m2);
         }
@@ -341,8 +341,8 @@ __device__ __forceinline__ int walk_rows(const Walk &w, const Src &src, int i, f
     if (ORDERED) {
         for (int s9 = 0; s9 < 9; ++s9) {
             const int ii = cx + s9 / 3 - 1, r = s9 % 3, jj = cy + r - 1;
-            if (ii < 0 || ii >= nx || jj < 0 || jj >= w.g.n_y) continue;
-            const int k = ii + nx * jj;
+            if (ii < 0 || ii >= nx || jj < 0 || jj >= cd.g.n_y) continue;
+            const int k = base + ii + nx * jj;
             const int cb = w.start[k], len = w.start[k + 1] - cb;
             if (len == 0) continue;
             const int lo = cb - sel3(r, b[0], b[1], b[2]);  // < 64: the cell lies inside its row range
@@ -377,23 +377,25 @@ __device__ __forceinline__ int walk_rows(const Walk &w, const Src &src, int i, f
 
 // Stages the geometry of a pair block's neighbourhood in LDS.  The block's
 // sorted slots [i0, i0 + kPairBlock) lie in cells c0 <= c <= c1; when these
-// share one cell row, the three neighbour rows of every particle of the
-// block are sub-ranges of the rows' slot ranges over cells c0 - 1 .. c1 + 1,
-// which are copied to tile (kTileRow slots per row).  Returns false, with
-// nothing staged, when the block spans two cell rows or a row range does not
-// fit; the block then reads global memory.  Every thread of the block calls
-// it (two barriers).
+// belong to one case and share one cell row, the three neighbour rows of
+// every particle of the block are sub-ranges of the rows' slot ranges over
+// cells c0 - 1 .. c1 + 1, which are copied to tile (kTileRow slots per row).
+// Returns false, with nothing staged, when the block spans two cases or two
+// cell rows or a row range does not fit; the block then reads global memory.
+// Every thread of the block calls it (two barriers).
 __device__ __forceinline__ bool stage_rows(const Walk &w, int i0, int n, float4 *tile, uint8_t *tix, int *info,
                                            TileRows &rows) {
     const int t = threadIdx.x;
     if (t < 3) {
-        const int nx = w.g.n_x;
-        const int c0 = w.cell[i0] - 1, c1 = w.cell[min(i0 + kPairBlock, n) - 1] - 1;
+        const int last = min(i0 + kPairBlock, n) - 1;
+        const CaseDesc cd = w.cases ? w.cases[w.cs[i0]] : w.one;
+        const int nx = cd.g.n_x;
+        const int c0 = w.cell[i0] - 1, c1 = w.cell[last] - 1;
         const int cy = c0 / nx, jj = cy + t - 1;
-        int b = 0, len = 0, ok = c1 / nx == cy;
-        if (ok && jj >= 0 && jj < w.g.n_y) {
-            b = w.start[max(c0 % nx - 1, 0) + nx * jj];
-            len = w.start[min(c1 % nx + 1, nx - 1) + 1 + nx * jj] - b;
+        int b = 0, len = 0, ok = c1 / nx == cy && (!w.cases || w.cs[last] == w.cs[i0]);
+        if (ok && jj >= 0 && jj < cd.g.n_y) {
+            b = w.start[cd.cell_base + max(c0 % nx - 1, 0) + nx * jj];
+            len = w.start[cd.cell_base + min(c1 % nx + 1, nx - 1) + 1 + nx * jj] - b;
             ok = len <= kTileRow;
         }
         info[t] = b;
@@ -453,14 +455,17 @@ __global__ void k_reset_search(int32_t *counts, int32_t n, unsigned long long *p
 // key is the 0-based cell, n_cells for dead particles.  Counts are taken with
 // one atomic per distinct key in the wave (the slots of a sorted wave share
 // few cells), which also yields each particle's provisional slot.
-__global__ __launch_bounds__(kBlock) void k_key_count(Fields f, int32_t n, Grid g, int32_t *counts, int32_t *key,
+__global__ __launch_bounds__(kBlock) void k_key_count(Fields f, int32_t n, CaseDesc one, const CaseDesc *cases,
+                                                       int32_t n_cells, int32_t *counts, int32_t *key,
                                                        int32_t *tmpslot, uint32_t *flags, int32_t max_per_cell) {
     const int32_t s = blockIdx.x * kBlock + threadIdx.x;
     const bool valid = s < n;
     int32_t k = -1;
     if (valid) {
-        k = g.n_cells;
-        if (f.alive[s] == 1) {
+        const CaseDesc cd = cases ? cases[f.cs[s]] : one;
+        const Grid &g = cd.g;
+        k = cd.removed ? n_cells + 1 : n_cells;
+        if (!cd.removed && f.alive[s] == 1) {
             int32_t c = f.cell[s];
             // a cached cell outside this grid (caller data, or another domain) is recomputed
             if (c < 0 || c > g.n_cells) c = -1;
@@ -479,7 +484,7 @@ __global__ __launch_bounds__(kBlock) void k_key_count(Fields f, int32_t n, Grid 
                     c = 0;
                 }
             }
-            if (c > 0) k = c - 1;
+            if (c > 0) k = cd.cell_base + c - 1;
         }
         key[s] = k;
     }
@@ -493,7 +498,7 @@ __global__ __launch_bounds__(kBlock) void k_key_count(Fields f, int32_t n, Grid 
         if (lane_id() == leader) {
             const int32_t cnt = __popcll(same);
             base = atomicAdd(&counts[lk], cnt);
-            if (lk < g.n_cells && base + cnt > max_per_cell) atomicOr(flags, kFlagCellOverflow);
+            if (lk < n_cells && base + cnt > max_per_cell) atomicOr(flags, kFlagCellOverflow);
         }
         base = __shfl(base, leader);
         if (valid && k == lk) slot = base + __popcll(same & ((1ull << lane_id()) - 1ull));
@@ -585,20 +590,23 @@ __global__ __launch_bounds__(kBlock) void k_fill_members(Fields f, int32_t n, in
     if (k < n_cells) members[start[k] + tmpslot[s]] = f.id[s];
 }
 
-// Final position of each slot: cell start + number of cell members with a
-// smaller particle index (the order a serial insert gives); dead particles
-// keep their provisional slot after the alive ones.  Moves every field and
-// writes the search snapshot geo = (x, y, h, tag).
-__global__ __launch_bounds__(kBlock) void k_permute(Fields cur, Fields nxt, float4 *geo, int32_t n, int32_t cap,
+// Final position of each of the n slots in use: cell start + number of cell
+// members with a smaller particle index (the order a serial insert gives);
+// dead, then released particles keep their provisional slot after the alive
+// ones.  Moves every field and writes the search snapshot geo = (x, y, h, m).
+__global__ __launch_bounds__(kBlock) void k_permute(Fields cur, Fields nxt, float4 *geo, int32_t n, int32_t batch,
                                                      int32_t n_cells, const int32_t *key, const int32_t *tmpslot,
                                                      const int32_t *start, const int32_t *members,
                                                      int32_t *n_alive, uint32_t *flags) {
     const int32_t s = blockIdx.x * kBlock + threadIdx.x;
-    if (s >= cap) return;
-    if (s == 0) *n_alive = start[n_cells];
+    if (s == 0) {
+        n_alive[0] = start[n_cells];      // alive slots
+        n_alive[1] = start[n_cells + 1];  // alive and dead slots (released ones follow)
+    }
+    if (s >= n) return;
     int32_t dest = s;
     int32_t alive = cur.alive[s];
-    if (s < n) {
+    {
         const int32_t k = key[s];
         const int32_t b = start[k];
         if (k < n_cells) {
@@ -636,6 +644,7 @@ __global__ __launch_bounds__(kBlock) void k_permute(Fields cur, Fields nxt, floa
     nxt.id[dest] = cur.id[s];
     nxt.cell[dest] = cur.cell[s];
     nxt.alive[dest] = alive;
+    if (batch) nxt.cs[dest] = cur.cs[s];  // a particle system's slots all hold case 0
 }
 
 // ------------------------------------------------------------ pair physics
@@ -670,10 +679,11 @@ struct ContinuityGather {
 // Search pass: find the neighbours of every alive particle and run the fused
 // action (EPPX_SEARCH_KERNEL_CONTINUITY: sComputeKernelAndiContinuity!,
 // main.jl:74-90).
-template <bool FAST, int ACTION, int K>
+template <bool FAST, int ACTION, int K, bool BATCH>
 __global__ __launch_bounds__(kPairBlock) void k_search(Walk w, Fields f, Physics ph, int32_t M, int32_t *ncount,
                                                         unsigned long long *pairs_last,
-                                                        unsigned long long *pairs_total, uint32_t *flags) {
+                                                        unsigned long long *pairs_total, uint32_t *flags,
+                                                        int32_t which) {
     __shared__ int lds[kPairBlock / 64];
     __shared__ float4 tile[3 * kTileRow];
     __shared__ uint8_t tix[3 * kTileRow];
@@ -682,11 +692,12 @@ __global__ __launch_bounds__(kPairBlock) void k_search(Walk w, Fields f, Physics
     const int i0 = xcd_block(blockIdx.x, gridDim.x) * kPairBlock;
     if (i0 >= n) return;  // the whole block
     const int i = i0 + threadIdx.x;
-    const bool active = i < n;
     TileRows rows;
     const bool tiled = ACTION == EPPX_SEARCH_KERNEL_CONTINUITY && stage_rows(w, i0, n, tile, tix, info, rows);
     int cnt = 0;
-    if (active) {
+    const CaseDesc cd = BATCH ? w.cases[w.cs[min(i, n - 1)]] : w.one;
+    // a batch runs the action for the cases that advance (which = 0) or wait for a first search (1)
+    if (i < n && (which == 0 ? cd.steps_left > 0 : cd.fresh != 0)) {
         const float4 gi = w.geo[i];
         const int ti = f.tag[i];
         const bool i_fluid = ti == ph.fluid_tag, i_wall = ti == ph.wall_tag;
@@ -712,11 +723,11 @@ __global__ __launch_bounds__(kPairBlock) void k_search(Walk w, Fields f, Physics
                 const float m = (i_fluid && tbj == fluid_bit) ? mj : rho_i * p.vol;
                 acc += continuity_term<FAST, K>(gi.z, hj, ui, p.uv, m, dx, dy, dr2);
             };
-            if (tiled) cnt = walk_rows<!FAST, true>(w, rows, i, gi, keep, fetch, term);
-            else cnt = walk_rows<!FAST, true>(w, GlobalRows{w.geo, w.tag}, i, gi, keep, fetch, term);
+            if (tiled) cnt = walk_rows<!FAST, true>(w, cd, rows, i, gi, keep, fetch, term);
+            else cnt = walk_rows<!FAST, true>(w, cd, GlobalRows{w.geo, w.tag}, i, gi, keep, fetch, term);
             f.drho[i] = acc;
         } else {
-            walk(w, i, gi, [&](int, uint32_t, float, float, float) { ++cnt; });
+            walk(w, cd, i, gi, [&](int, uint32_t, float, float, float) { ++cnt; });
         }
         ncount[i] = cnt;
         if (cnt > M) atomicOr(flags, kFlagNeighbourOverflow);
@@ -798,7 +809,7 @@ __device__ __forceinline__ void momentum_pair(const Physics &ph, float2 &acc, bo
     }
 }
 
-template <bool FAST, int K, int VISC>
+template <bool FAST, int K, int VISC, bool BATCH>
 __global__ __launch_bounds__(kPairBlock) void k_momentum(Walk w, Fields f, Physics ph) {
     __shared__ float4 tile[3 * kTileRow];
     __shared__ uint8_t tix[3 * kTileRow];
@@ -811,6 +822,8 @@ __global__ __launch_bounds__(kPairBlock) void k_momentum(Walk w, Fields f, Physi
     const int i = i0 + threadIdx.x;
     if (i >= n) return;
     if (f.tag[i] != ph.fluid_tag) return;  // iMomentum! acts on fluid particles only
+    const CaseDesc cd = BATCH ? w.cases[w.cs[i]] : w.one;
+    if (cd.steps_left <= 0) return;  // a batch case that has finished
     const float4 gi = w.geo[i];
     const float4 di = f.dyn[i];
     const float2 ui = make_float2(di.x, di.y);
@@ -823,8 +836,8 @@ __global__ __launch_bounds__(kPairBlock) void k_momentum(Walk w, Fields f, Physi
                                      make_float2(dj.x, dj.y), dj.z, dj.w, mj, dx, dy, dr2);
     };
     const uint32_t keep = tag_bit(ph.fluid_tag) | tag_bit(ph.wall_tag);
-    if (tiled) walk_rows<!FAST, false>(w, rows, i, gi, keep, fetch, term);
-    else walk_rows<!FAST, false>(w, GlobalRows{w.geo, w.tag}, i, gi, keep, fetch, term);
+    if (tiled) walk_rows<!FAST, false>(w, cd, rows, i, gi, keep, fetch, term);
+    else walk_rows<!FAST, false>(w, cd, GlobalRows{w.geo, w.tag}, i, gi, keep, fetch, term);
     f.dvel[i] = acc;
 }
 
@@ -852,11 +865,12 @@ __global__ __launch_bounds__(kBlock) void k_interaction(Walk w, Fields f, Physic
     if (i >= *w.n_alive) return;
     const float4 gi = w.geo[i];
     const int ti = f.tag[i];
+    const CaseDesc cd = w.cases ? w.cases[w.cs[i]] : w.one;
     if (ACTION == EPPX_INTER_FILTER) {
         if (ti != ph.fluid_tag) return;
         float wv = f.wv[i], wvrho = f.wvrho[i];
         const uint32_t fluid_bit = tag_bit(ph.fluid_tag);
-        walk(w, i, gi, [&](int j, uint32_t tbj, float, float, float dr2) {
+        walk(w, cd, i, gi, [&](int j, uint32_t tbj, float, float, float dr2) {
             if (tbj != fluid_bit) return;
             const float r = sqrtf(dr2);
             const float wgt = value_exact(ph.kernel, r, invharmonic(gi.z, w.geo[j].z));
@@ -869,7 +883,7 @@ __global__ __launch_bounds__(kBlock) void k_interaction(Walk w, Fields f, Physic
         int stride;
         float *col = field_column(f, op, stride);
         float v = col[(int64_t)i * stride];
-        walk(w, i, gi, [&](int, uint32_t, float, float, float) { v += op.value; });
+        walk(w, cd, i, gi, [&](int, uint32_t, float, float, float) { v += op.value; });
         col[(int64_t)i * stride] = v;
     }
 }
@@ -883,9 +897,15 @@ __global__ __launch_bounds__(kBlock) void k_interaction(Walk w, Fields f, Physic
 //  EPPX_SELF_FILTER: sFilter main.jl:136-144 -> sKernelFilter! Filter.jl:24-38;
 //  EPPX_SELF_SCALE_FIELD: field[I] *= value.
 template <int ACTION>
-__global__ __launch_bounds__(kBlock) void k_self(Fields f, int32_t n, Physics ph, eppx_field_op op) {
+__global__ __launch_bounds__(kBlock) void k_self(Fields f, int32_t n, Physics ph, eppx_field_op op,
+                                                 const CaseDesc *cases) {
     const int s = blockIdx.x * kBlock + threadIdx.x;
     if (s >= n || f.alive[s] != 1) return;
+    if (cases) {  // a batch: cases that have finished stay as they are; dt is the case's
+        const CaseDesc &cd = cases[f.cs[s]];
+        if (cd.steps_left <= 0) return;
+        ph.dt = cd.dt;
+    }
     const int tag = f.tag[s];
     if (ACTION == EPPX_SELF_CONTINUITY_EOS) {
         if (tag != ph.fluid_tag && tag != ph.wall_tag) return;
@@ -937,6 +957,15 @@ __global__ __launch_bounds__(kBlock) void k_snapshot_mass(float4 *geo, Fields f,
     if (s < n) geo[s].w = f.mass[s];
 }
 
+// Writes one case descriptor (a kernel, so that a recorded step may carry it).
+__global__ void k_set_case(CaseDesc *d, CaseDesc v) { *d = v; }
+
+// One batch step done: every case with steps left has one fewer.
+__global__ __launch_bounds__(kBlock) void k_tick(CaseDesc *cases, int32_t n_cases) {
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c < n_cases && cases[c].steps_left > 0) cases[c].steps_left -= 1;
+}
+
 // ------------------------------------------------------- layout conversion
 // Neighbour lists of the last search, written as a CSR by particle index
 // and / or into the reference's per-particle neighbour columns
@@ -956,7 +985,7 @@ __global__ __launch_bounds__(kBlock) void k_export(Walk w, Fields f, const int64
     const int64_t o = row_start ? row_start[i] : 0;
     const int limit = js ? ncount[i] : 0;  // the row length the search counted
     int k = 0;
-    walk(w, i, gi, [&](int j, uint32_t tbj, float dx, float dy, float dr2) {
+    walk(w, w.cases ? w.cases[w.cs[i]] : w.one, i, gi, [&](int j, uint32_t tbj, float dx, float dy, float dr2) {
         if (js && k < limit) js[o + k] = f.id[j];
         if (k < ex.M) {
             if (ex.nIndex) ex.nIndex[row + cap * k] = f.id[j] + 1;
@@ -980,14 +1009,15 @@ __global__ __launch_bounds__(kBlock) void k_export(Walk w, Fields f, const int64
     if (ex.nCount) ex.nCount[row] = k;
 }
 
-__global__ __launch_bounds__(kBlock) void k_from_ref(Fields f, int32_t cap, Columns c, const int32_t *ip,
+__global__ __launch_bounds__(kBlock) void k_from_ref(Fields f, int32_t offset, Columns c, const int32_t *ip,
                                                       const float *fp, const int32_t *is_alive,
-                                                      const int32_t *cell_index) {
-    const int s = blockIdx.x * kBlock + threadIdx.x;
-    if (s >= cap) return;
-    const size_t C = (size_t)cap;
-    auto fcol = [&](int col) { return col >= 0 ? fp[s + C * col] : 0.0f; };
-    auto icol = [&](int col) { return col >= 0 ? ip[s + C * col] : 0; };
+                                                      const int32_t *cell_index, int32_t cs) {
+    const int row = blockIdx.x * kBlock + threadIdx.x;
+    if (row >= c.n_capacity) return;
+    const size_t C = (size_t)c.n_capacity;
+    auto fcol = [&](int col) { return col >= 0 ? fp[row + C * col] : 0.0f; };
+    auto icol = [&](int col) { return col >= 0 ? ip[row + C * col] : 0; };
+    const int s = offset + row;
     f.pos[s] = make_float2(fcol(c.PositionVec), c.PositionVec >= 0 ? fcol(c.PositionVec + 1) : 0.0f);
     f.dyn[s] = make_float4(fcol(c.VelocityVec), c.VelocityVec >= 0 ? fcol(c.VelocityVec + 1) : 0.0f,
                            fcol(c.Density), fcol(c.Pressure));
@@ -1000,16 +1030,17 @@ __global__ __launch_bounds__(kBlock) void k_from_ref(Fields f, int32_t cap, Colu
     f.wvrho[s] = fcol(c.SumWeightedDensity);
     f.tag[s] = icol(c.Tag);
     f.mov[s] = icol(c.IsMovable);
-    f.id[s] = s;
-    f.alive[s] = is_alive[s];
-    f.cell[s] = cell_index[s];
+    f.id[s] = row;
+    f.alive[s] = is_alive[row];
+    f.cell[s] = cell_index[row];
+    f.cs[s] = cs;
 }
 
-__global__ __launch_bounds__(kBlock) void k_to_ref(Fields f, int32_t cap, Columns c, int32_t *ip, float *fp,
-                                                    int32_t *is_alive, int32_t *cell_index) {
+__global__ __launch_bounds__(kBlock) void k_to_ref(Fields f, int32_t n, Columns c, int32_t *ip, float *fp,
+                                                    int32_t *is_alive, int32_t *cell_index, int32_t cs) {
     const int s = blockIdx.x * kBlock + threadIdx.x;
-    if (s >= cap) return;
-    const size_t C = (size_t)cap;
+    if (s >= n || (cs >= 0 && f.cs[s] != cs)) return;
+    const size_t C = (size_t)c.n_capacity;
     const int row = f.id[s];
     auto put = [&](int col, float v) {
         if (col >= 0) fp[row + C * col] = v;
@@ -1137,10 +1168,11 @@ hipError_t launch_reset_search(int32_t *counts, int32_t n, unsigned long long *p
     return hipGetLastError();
 }
 
-hipError_t launch_key_count(Fields f, int32_t n, Grid g, int32_t *counts, int32_t *key, int32_t *tmpslot,
-                            uint32_t *flags, int32_t max_per_cell, hipStream_t s) {
-    hipLaunchKernelGGL(k_key_count, grid_for(n), dim3(kBlock), 0, s, f, n, g, counts, key, tmpslot, flags,
-                       max_per_cell);
+hipError_t launch_key_count(Fields f, int32_t n, const CaseDesc &one, const CaseDesc *cases, int32_t n_cells,
+                            int32_t *counts, int32_t *key, int32_t *tmpslot, uint32_t *flags, int32_t max_per_cell,
+                            hipStream_t s) {
+    hipLaunchKernelGGL(k_key_count, grid_for(n), dim3(kBlock), 0, s, f, n, one, cases, n_cells, counts, key,
+                       tmpslot, flags, max_per_cell);
     return hipGetLastError();
 }
 
@@ -1159,66 +1191,81 @@ hipError_t launch_fill_members(const Fields f, int32_t n, int32_t n_cells, const
     return hipGetLastError();
 }
 
-hipError_t launch_permute(Fields cur, Fields nxt, float4 *geo, int32_t n, int32_t cap, int32_t n_cells,
+hipError_t launch_permute(Fields cur, Fields nxt, float4 *geo, int32_t n, int32_t batch, int32_t n_cells,
                           const int32_t *key, const int32_t *tmpslot, const int32_t *start,
                           const int32_t *members, int32_t *n_alive, uint32_t *flags, hipStream_t s) {
-    hipLaunchKernelGGL(k_permute, grid_for(cap), dim3(kBlock), 0, s, cur, nxt, geo, n, cap, n_cells, key, tmpslot,
+    hipLaunchKernelGGL(k_permute, grid_for(n), dim3(kBlock), 0, s, cur, nxt, geo, n, batch, n_cells, key, tmpslot,
                        start, members, n_alive, flags);
     return hipGetLastError();
 }
 
+template <bool FAST, int K>
+static void launch_search_k(const Walk &w, Fields f, const Physics &ph, int32_t M, int32_t *ncount,
+                            unsigned long long *pairs_last, unsigned long long *pairs_total, uint32_t *flags,
+                            int32_t which, dim3 g, hipStream_t s) {
+    constexpr int KCONT = EPPX_SEARCH_KERNEL_CONTINUITY;
+    if (w.cases)
+        hipLaunchKernelGGL((k_search<FAST, KCONT, K, true>), g, dim3(kPairBlock), 0, s, w, f, ph, M, ncount,
+                           pairs_last, pairs_total, flags, which);
+    else
+        hipLaunchKernelGGL((k_search<FAST, KCONT, K, false>), g, dim3(kPairBlock), 0, s, w, f, ph, M, ncount,
+                           pairs_last, pairs_total, flags, which);
+}
+
 hipError_t launch_search(const Walk &w, Fields f, int32_t n, int32_t action, const Physics &ph, int32_t fast,
                          int32_t M, int32_t *ncount, unsigned long long *pairs_last,
-                         unsigned long long *pairs_total, uint32_t *flags, hipStream_t s) {
-    const dim3 g = grid_for(n, kPairBlock), b(kPairBlock);
-    constexpr int KC = EPPX_KERNEL_CUBIC, KW = EPPX_KERNEL_WENDLAND_C2, KCONT = EPPX_SEARCH_KERNEL_CONTINUITY;
-    if (action == KCONT) {
+                         unsigned long long *pairs_total, uint32_t *flags, int32_t which, hipStream_t s) {
+    const dim3 g = grid_for(n, kPairBlock);
+    constexpr int KC = EPPX_KERNEL_CUBIC, KW = EPPX_KERNEL_WENDLAND_C2;
+    if (action == EPPX_SEARCH_KERNEL_CONTINUITY) {
         const bool cubic = ph.kernel == KC;
-        if (fast && cubic)
-            hipLaunchKernelGGL((k_search<true, KCONT, KC>), g, b, 0, s, w, f, ph, M, ncount, pairs_last, pairs_total,
-                               flags);
-        else if (fast)
-            hipLaunchKernelGGL((k_search<true, KCONT, KW>), g, b, 0, s, w, f, ph, M, ncount, pairs_last, pairs_total,
-                               flags);
-        else if (cubic)
-            hipLaunchKernelGGL((k_search<false, KCONT, KC>), g, b, 0, s, w, f, ph, M, ncount, pairs_last,
-                               pairs_total, flags);
-        else
-            hipLaunchKernelGGL((k_search<false, KCONT, KW>), g, b, 0, s, w, f, ph, M, ncount, pairs_last,
-                               pairs_total, flags);
+        if (fast && cubic) launch_search_k<true, KC>(w, f, ph, M, ncount, pairs_last, pairs_total, flags, which, g, s);
+        else if (fast) launch_search_k<true, KW>(w, f, ph, M, ncount, pairs_last, pairs_total, flags, which, g, s);
+        else if (cubic) launch_search_k<false, KC>(w, f, ph, M, ncount, pairs_last, pairs_total, flags, which, g, s);
+        else launch_search_k<false, KW>(w, f, ph, M, ncount, pairs_last, pairs_total, flags, which, g, s);
     } else {
-        hipLaunchKernelGGL((k_search<false, EPPX_SEARCH_NONE, KW>), g, b, 0, s, w, f, ph, M, ncount, pairs_last,
-                           pairs_total, flags);
+        if (w.cases)
+            hipLaunchKernelGGL((k_search<false, EPPX_SEARCH_NONE, KW, true>), g, dim3(kPairBlock), 0, s, w, f, ph, M,
+                               ncount, pairs_last, pairs_total, flags, which);
+        else
+            hipLaunchKernelGGL((k_search<false, EPPX_SEARCH_NONE, KW, false>), g, dim3(kPairBlock), 0, s, w, f, ph,
+                               M, ncount, pairs_last, pairs_total, flags, which);
     }
     return hipGetLastError();
 }
 
 hipError_t launch_self(Fields f, int32_t n, int32_t action, const Physics &ph, const eppx_field_op &op,
-                       hipStream_t s) {
+                       const CaseDesc *cases, hipStream_t s) {
     const dim3 g = grid_for(n);
     switch (action) {
         case EPPX_SELF_CONTINUITY_EOS:
-            hipLaunchKernelGGL(k_self<EPPX_SELF_CONTINUITY_EOS>, g, dim3(kBlock), 0, s, f, n, ph, op);
+            hipLaunchKernelGGL(k_self<EPPX_SELF_CONTINUITY_EOS>, g, dim3(kBlock), 0, s, f, n, ph, op, cases);
             break;
         case EPPX_SELF_GRAVITY_MOVE:
-            hipLaunchKernelGGL(k_self<EPPX_SELF_GRAVITY_MOVE>, g, dim3(kBlock), 0, s, f, n, ph, op);
+            hipLaunchKernelGGL(k_self<EPPX_SELF_GRAVITY_MOVE>, g, dim3(kBlock), 0, s, f, n, ph, op, cases);
             break;
         case EPPX_SELF_FILTER:
-            hipLaunchKernelGGL(k_self<EPPX_SELF_FILTER>, g, dim3(kBlock), 0, s, f, n, ph, op);
+            hipLaunchKernelGGL(k_self<EPPX_SELF_FILTER>, g, dim3(kBlock), 0, s, f, n, ph, op, cases);
             break;
         default:
-            hipLaunchKernelGGL(k_self<EPPX_SELF_SCALE_FIELD>, g, dim3(kBlock), 0, s, f, n, ph, op);
+            hipLaunchKernelGGL(k_self<EPPX_SELF_SCALE_FIELD>, g, dim3(kBlock), 0, s, f, n, ph, op, cases);
             break;
     }
     return hipGetLastError();
 }
 
+template <bool FAST, int K, bool BATCH>
+static void launch_momentum_b(const Walk &w, Fields f, const Physics &ph, dim3 g, hipStream_t s) {
+    if (ph.visc == EPPX_VISC_CLASSIC)
+        hipLaunchKernelGGL((k_momentum<FAST, K, EPPX_VISC_CLASSIC, BATCH>), g, dim3(kPairBlock), 0, s, w, f, ph);
+    else
+        hipLaunchKernelGGL((k_momentum<FAST, K, EPPX_VISC_ARTIFICIAL, BATCH>), g, dim3(kPairBlock), 0, s, w, f, ph);
+}
+
 template <bool FAST, int K>
 static void launch_momentum_v(const Walk &w, Fields f, const Physics &ph, dim3 g, hipStream_t s) {
-    if (ph.visc == EPPX_VISC_CLASSIC)
-        hipLaunchKernelGGL((k_momentum<FAST, K, EPPX_VISC_CLASSIC>), g, dim3(kPairBlock), 0, s, w, f, ph);
-    else
-        hipLaunchKernelGGL((k_momentum<FAST, K, EPPX_VISC_ARTIFICIAL>), g, dim3(kPairBlock), 0, s, w, f, ph);
+    if (w.cases) launch_momentum_b<FAST, K, true>(w, f, ph, g, s);
+    else launch_momentum_b<FAST, K, false>(w, f, ph, g, s);
 }
 
 hipError_t launch_interaction(const Walk &w, Fields f, int32_t n, int32_t action, const Physics &ph,
@@ -1250,15 +1297,26 @@ hipError_t launch_export(const Walk &w, Fields f, int32_t n, const int64_t *row_
     return hipGetLastError();
 }
 
-hipError_t launch_from_ref(Fields f, int32_t cap, Columns c, const int32_t *ip, const float *fp,
-                           const int32_t *is_alive, const int32_t *cell_index, hipStream_t s) {
-    hipLaunchKernelGGL(k_from_ref, grid_for(cap), dim3(kBlock), 0, s, f, cap, c, ip, fp, is_alive, cell_index);
+hipError_t launch_from_ref(Fields f, int32_t offset, Columns c, const int32_t *ip, const float *fp,
+                           const int32_t *is_alive, const int32_t *cell_index, int32_t cs, hipStream_t s) {
+    hipLaunchKernelGGL(k_from_ref, grid_for(c.n_capacity), dim3(kBlock), 0, s, f, offset, c, ip, fp, is_alive,
+                       cell_index, cs);
     return hipGetLastError();
 }
 
-hipError_t launch_to_ref(Fields f, int32_t cap, Columns c, int32_t *ip, float *fp, int32_t *is_alive,
-                         int32_t *cell_index, hipStream_t s) {
-    hipLaunchKernelGGL(k_to_ref, grid_for(cap), dim3(kBlock), 0, s, f, cap, c, ip, fp, is_alive, cell_index);
+hipError_t launch_to_ref(Fields f, int32_t n, Columns c, int32_t *ip, float *fp, int32_t *is_alive,
+                         int32_t *cell_index, int32_t cs, hipStream_t s) {
+    hipLaunchKernelGGL(k_to_ref, grid_for(n), dim3(kBlock), 0, s, f, n, c, ip, fp, is_alive, cell_index, cs);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_case(CaseDesc *d, const CaseDesc &v, hipStream_t s) {
+    hipLaunchKernelGGL(k_set_case, dim3(1), dim3(1), 0, s, d, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_tick(CaseDesc *cases, int32_t n_cases, hipStream_t s) {
+    hipLaunchKernelGGL(k_tick, grid_for(n_cases), dim3(kBlock), 0, s, cases, n_cases);
     return hipGetLastError();
 }
 

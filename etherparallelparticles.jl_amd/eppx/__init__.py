@@ -11,9 +11,9 @@ Host-side mirror of the reference's module layout:
 
 Every compute call goes through include/eppx.h; there is no CPU fallback.
 """
-from . import Algorithm, Class, Environment, Geometry, SPH, cases  # noqa: F401
+from . import Algorithm, Batch, Class, Environment, Geometry, SPH, cases  # noqa: F401
 from ._build import build  # noqa: F401
 from ._lib import EppxError, library_path, lib  # noqa: F401
 
-__all__ = ["Algorithm", "Class", "Environment", "Geometry", "SPH", "cases", "build", "EppxError", "lib",
+__all__ = ["Algorithm", "Batch", "Class", "Environment", "Geometry", "SPH", "cases", "build", "EppxError", "lib",
            "library_path"]

@@ -108,6 +108,17 @@ def lib() -> C.CDLL:
             "eppx_pair_fallback_blocks": ([vp, P(i64), i32], i32),
             "eppx_set_timing": ([vp, i32], i32),
             "eppx_kernel_time": ([vp, C.c_char_p, P(C.c_double), P(i64), i32], i32),
+            "eppx_batch_bytes_per_particle": ([P(i32)], i32),
+            "eppx_batch_create": ([vp, P(Layout), i32, i32, P(i32), i32, i32, P(vp)], i32),
+            "eppx_batch_destroy": ([vp], i32),
+            "eppx_batch_add": ([vp, P(Domain2D), P(Physics), i32, P(i32), P(i32), P(i32), P(C.c_float), i32,
+                                P(i32)], i32),
+            "eppx_batch_step": ([vp, i32], i32),
+            "eppx_batch_steps_left": ([vp, P(i32)], i32),
+            "eppx_batch_download": ([vp, i32, P(i32), P(i32), P(i32), P(i32), P(C.c_float)], i32),
+            "eppx_batch_remove": ([vp, i32], i32),
+            "eppx_batch_info": ([vp, P(i32), P(i32)], i32),
+            "eppx_batch_pair_total": ([vp, P(i64), i32], i32),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)

@@ -225,6 +225,44 @@ int32_t eppx_check_finite(eppx_ps *ps);
 int32_t eppx_set_timing(eppx_ctx *ctx, int32_t enable);
 int32_t eppx_kernel_time(eppx_ctx *ctx, const char *kernel, double *avg_us, int64_t *launches, int32_t reset);
 
+/* ---------------------------------------------------------------- batch
+ * No counterpart in the reference, whose driver runs one case per process.
+ * A batch keeps many independent cases (each with its own Domain2D, H and
+ * dt; the rest of eppx_physics shared) resident in one set of device
+ * buffers with a segmented cell list, and advances every case that has steps
+ * left with one launch per kernel per step (a recorded hipGraph).  Each case
+ * stops after exactly its own step budget; cases can be added and removed
+ * between steps (continuous batching).  All cases use the layout, adjacency
+ * and max_neighbour_number given at creation, symmetrySearchCriterion and the
+ * dam-break step (sContinuity!, iMomentum!, sMomentum!, search! with
+ * sComputeKernelAndiContinuity!). */
+typedef struct eppx_batch eppx_batch;
+/* Device bytes per particle slot, for admission by HBM footprint. */
+int32_t eppx_batch_bytes_per_particle(int32_t *out);
+int32_t eppx_batch_create(eppx_ctx *ctx, const eppx_layout *layout, int32_t particle_capacity, int32_t case_capacity,
+                          const int32_t *adjacency, int32_t max_tag, int32_t max_neighbour_number, eppx_batch **out);
+int32_t eppx_batch_destroy(eppx_batch *b);
+/* Adds one case from its reference tables (n_particles rows, column-major,
+ * as eppx_ps_upload) and runs its first search; it then advances for `steps`
+ * batch steps.  Fails with EPPX_E_CAPACITY when slots or case slots run out,
+ * EPPX_E_ARG when the physics differs from the batch's other than in dt. */
+int32_t eppx_batch_add(eppx_batch *b, const eppx_domain2d *domain, const eppx_physics *phys, int32_t n_particles,
+                       const int32_t *is_alive, const int32_t *cell_index, const int32_t *int_props,
+                       const float *float_props, int32_t steps, int32_t *case_id);
+/* n_steps batch steps: every case with steps left advances one step each. */
+int32_t eppx_batch_step(eppx_batch *b, int32_t n_steps);
+/* Steps left of every case slot (case_capacity entries; -1 when unused) (waits). */
+int32_t eppx_batch_steps_left(eppx_batch *b, int32_t *steps_left);
+/* A case's reference tables (as eppx_ps_download, without the neighbour
+ * columns) (waits). */
+int32_t eppx_batch_download(eppx_batch *b, int32_t case_id, int32_t *n_particles, int32_t *is_alive,
+                            int32_t *cell_index, int32_t *int_props, float *float_props);
+/* Releases a case's slots and case slot. */
+int32_t eppx_batch_remove(eppx_batch *b, int32_t case_id);
+int32_t eppx_batch_info(eppx_batch *b, int32_t *slots_in_use, int32_t *cases_in_use);
+/* Running Σ nCount over the batch's searches since the last reset (waits). */
+int32_t eppx_batch_pair_total(eppx_batch *b, int64_t *out, int32_t reset);
+
 #ifdef __cplusplus
 }
 #endif
