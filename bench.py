@@ -103,12 +103,202 @@ def copy_calibration(n_bytes: int, repeats: int = 3) -> None:
     print(json.dumps({"copy_calibration_bytes": n_bytes, "repeats": repeats}), flush=True)
 
 
+def _warm_up(par):
+    """Loads the kernels and the graph machinery once (a tiny batch and a tiny case)."""
+    from eppx import Algorithm, Batch, Class, SPH, cases
+    c = cases.dam_break(0.04, 2.0, SPH.WendlandC2, M=1, fluid_width=0.4, fluid_height=0.4)
+    b = Batch.Batch(par, c.host, 2 * c.n_particles, 2, cases.ACTIVE_PAIR, max_neighbour_number=64)
+    b.add(c.domain, c.physics, c.host, 3)
+    b.step(4)
+    b.close()
+    ps = Class.ParticleSystem(par, c.domain, c.n_particles, c.int_nt, c.float_nt, c.basic_parameters)
+    Class.syncto(ps, c.host)
+    ns = Class.NeighbourSystem(par, c.domain, cases.ACTIVE_PAIR, max_neighbour_number=64)
+    Algorithm.search(ps, c.domain, ns, criterion=Algorithm.symmetrySearchCriterion,
+                     action=SPH.sComputeKernelAndiContinuity(c.physics))
+    Algorithm.step(ps, ns, c.physics, n_steps=3)
+    ps.close()
+    ns.close()
+
+
+def _percentile(xs, q):
+    xs = sorted(xs)
+    if not xs:
+        return None
+    k = (len(xs) - 1) * q
+    lo = int(k)
+    hi = min(lo + 1, len(xs) - 1)
+    return xs[lo] + (xs[hi] - xs[lo]) * (k - lo)
+
+
+def run_mix(args):
+    """64 uneven dam-break cases (seed 0), 500 steps each: one batch (every case in one segmented cell list,
+    one launch per kernel per step, recorded graph) against the same cases one by one on the particle-system
+    path.  Both timings cover upload, first search, the steps and download."""
+    import eppx
+    from eppx import Algorithm, Batch, Class, SPH, cases
+    cs = cases.mix_cases(0, steps=args.mix_steps, M=1)
+    total_n = sum(c.n_particles for c in cs)
+    log(f"mix: {len(cs)} cases, {total_n} particles ({min(c.n_particles for c in cs)}..."
+        f"{max(c.n_particles for c in cs)}), {args.mix_steps} steps each")
+    par = eppx.Environment.Parallel(device=0, math=args.math)
+    _warm_up(par)
+
+    par.synchronize()
+    t0 = time.perf_counter()
+    b = Batch.Batch(par, cs[0].host, total_n, len(cs), cases.ACTIVE_PAIR, max_neighbour_number=64)
+    ids = [b.add(c.domain, c.physics, c.host, c.steps) for c in cs]
+    b.pair_total(reset=True)
+    par.synchronize()
+    t1 = time.perf_counter()
+    b.step(args.mix_steps)
+    t2 = time.perf_counter()
+    pairs_batch = b.pair_total(reset=True)
+    if (b.steps_left()[ids] != 0).any():
+        raise RuntimeError("a case of the batch did not finish")
+    for i in ids:
+        b.download(i)
+    t3 = time.perf_counter()
+    b.close()
+    log(f"batch: {t3 - t0:.3f} s end to end, steps {t2 - t1:.3f} s, {pairs_batch} pairs")
+
+    L = eppx._lib.lib()
+    done, pairs_single, steps_single = [], 0, 0.0
+    t0s = time.perf_counter()
+    for c in cs:
+        ps = Class.ParticleSystem(par, c.domain, c.n_particles, c.int_nt, c.float_nt, c.basic_parameters)
+        Class.syncto(ps, c.host)
+        ns = Class.NeighbourSystem(par, c.domain, cases.ACTIVE_PAIR, max_neighbour_number=64)
+        Algorithm.search(ps, c.domain, ns, criterion=Algorithm.symmetrySearchCriterion,
+                         action=SPH.sComputeKernelAndiContinuity(c.physics))
+        total = C.c_int64()
+        eppx._lib.check(L.eppx_pair_total(ps.handle, C.byref(total), 1), par.ctx, "pair_total")
+        par.synchronize()
+        ts = time.perf_counter()
+        Algorithm.step(ps, ns, c.physics, n_steps=c.steps)
+        steps_single += time.perf_counter() - ts
+        eppx._lib.check(L.eppx_pair_total(ps.handle, C.byref(total), 1), par.ctx, "pair_total")
+        pairs_single += total.value
+        host = Class.mirror(ps)
+        Class.syncto(host, ps)
+        ps.close()
+        ns.close()
+        done.append(time.perf_counter() - t0s)
+    t_single = time.perf_counter() - t0s
+    log(f"one by one: {t_single:.3f} s end to end, steps {steps_single:.3f} s, {pairs_single} pairs")
+    if pairs_single != pairs_batch:
+        raise RuntimeError(f"pair counts differ: batch {pairs_batch}, one by one {pairs_single}")
+    t_batch = t3 - t0
+    out = {
+        "metric": "aggregate pair-interactions/s, 64-case uneven 2D dam-break mix, batched",
+        "value": pairs_batch / t_batch, "unit": "pair-interactions/s", "n_gpus": 1, "steps": args.mix_steps,
+        "warmup": 0, "ms_per_step": 1000.0 * (t2 - t1) / args.mix_steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"64 dam-break cases from seed 0 (dx in {{0.02, 0.01, 0.005, 0.0025}}, width "
+                               f"U[0.5, 1.5], aspect U[0.5, 2], Wendland C2, h = 2dx), {args.mix_steps} steps each",
+                   "particles": total_n, "cases": len(cs), "math": args.math,
+                   "case_particles_min": min(c.n_particles for c in cs),
+                   "case_particles_max": max(c.n_particles for c in cs)},
+        "one_by_one": {"value": pairs_single / t_single, "seconds": t_single, "steps_seconds": steps_single},
+        "batched": {"seconds": t_batch, "steps_seconds": t2 - t1, "add_seconds": t1 - t0,
+                    "download_seconds": t3 - t2},
+        "speedup_end_to_end": t_single / t_batch,
+        "speedup_steps": steps_single / (t2 - t1),
+        "latency_s": {"batched_p50": t_batch, "batched_p99": t_batch,
+                      "one_by_one_p50": _percentile(done, 0.5), "one_by_one_p99": _percentile(done, 0.99)},
+        "pairs": pairs_batch,
+    }
+    print(json.dumps(out), flush=True)
+    par.close()
+
+
+def run_serve(args):
+    """Serving stream (seed 1): Poisson arrivals of dam-break requests (N log-uniform in [1e4, 1e6], dx from a
+    1 x 2 m column, steps U{200..2000}).  Continuous batching: requests are admitted as slots and case slots free
+    up (admission by device bytes per particle against a fixed HBM budget), run together in one batch, and leave
+    when their budget is done.  Reports p50 / p99 time to completion and aggregate pair-interactions/s."""
+    import eppx
+    from eppx import Batch, SPH, cases
+    reqs = cases.serving_stream(1, n_requests=args.serve_requests, rate_hz=args.serve_rate)
+    hosts = [cases.dam_break(dx, 2.0, SPH.WendlandC2, M=1, fluid_width=1.0, fluid_height=2.0) for _, _, dx, _ in reqs]
+    bpp = Batch.bytes_per_particle()
+    slots = int(args.serve_budget_gb * 1e9) // bpp
+    biggest = max(h.n_particles for h in hosts)
+    if slots < biggest:
+        raise SystemExit(f"HBM budget too small for the largest request ({biggest} particles)")
+    log(f"serve: {len(reqs)} requests over {reqs[-1][0]:.1f} s, {sum(h.n_particles for h in hosts)} particles, "
+        f"budget {args.serve_budget_gb} GB = {slots} slots of {bpp} B")
+    par = eppx.Environment.Parallel(device=0, math=args.math)
+    _warm_up(par)
+    b = Batch.Batch(par, hosts[0].host, slots, args.serve_slots, cases.ACTIVE_PAIR, max_neighbour_number=64)
+    pending = list(range(len(reqs)))
+    active = {}  # case id -> request
+    used = 0
+    latency, pairs, chunks = [None] * len(reqs), 0, 0
+    par.synchronize()
+    t0 = time.perf_counter()
+    while pending or active:
+        now = time.perf_counter() - t0
+        while pending and reqs[pending[0]][0] <= now:  # admit in arrival order while it fits
+            r = pending[0]
+            n = hosts[r].n_particles
+            if used + n > slots or len(active) >= args.serve_slots:
+                break
+            cid = b.add(hosts[r].domain, hosts[r].physics, hosts[r].host, reqs[r][3])
+            active[cid] = r
+            used += n
+            pending.pop(0)
+        if not active:
+            time.sleep(max(0.0, reqs[pending[0]][0] - (time.perf_counter() - t0)))
+            continue
+        left = b.steps_left()
+        k = int(min(left[c] for c in active))
+        if k > 0:
+            b.pair_total(reset=True)
+            b.step(min(k, args.serve_chunk), sync=False)
+            pairs += b.pair_total(reset=True)
+            chunks += 1
+            left = b.steps_left()
+        for cid in [c for c in active if left[c] == 0]:
+            b.download(cid)
+            r = active.pop(cid)
+            latency[r] = time.perf_counter() - t0 - reqs[r][0]
+            used -= hosts[r].n_particles
+            b.remove(cid)
+    makespan = time.perf_counter() - t0
+    b.close()
+    lat = [x for x in latency if x is not None]
+    out = {
+        "metric": "serving time-to-completion p50/p99, Poisson 2D dam-break stream (continuous batching)",
+        "value": pairs / makespan, "unit": "pair-interactions/s", "n_gpus": 1, "steps": None, "warmup": 0,
+        "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{len(reqs)} requests, seed 1, Poisson at {args.serve_rate}/s, N log-uniform in "
+                               f"[1e4, 1e6] (dx = sqrt(2/N)), steps U{{200..2000}}, Wendland C2, h = 2dx",
+                   "math": args.math, "particles": sum(h.n_particles for h in hosts),
+                   "steps_total": sum(r[3] for r in reqs)},
+        "latency_s": {"p50": _percentile(lat, 0.5), "p99": _percentile(lat, 0.99), "mean": sum(lat) / len(lat),
+                      "max": max(lat)},
+        "admission": {"bytes_per_particle": bpp, "budget_bytes": slots * bpp, "slots": slots,
+                      "case_slots": args.serve_slots},
+        "makespan_s": makespan, "chunks": chunks, "pairs": pairs,
+    }
+    print(json.dumps(out), flush=True)
+    par.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", choices=["large", "small"], default="large")
+    ap.add_argument("--workload", choices=["large", "small", "mix", "serve"], default="large")
+    ap.add_argument("--mix-steps", type=int, default=500)
+    ap.add_argument("--serve-requests", type=int, default=32)
+    ap.add_argument("--serve-rate", type=float, default=4.0, help="Poisson arrival rate (1/s)")
+    ap.add_argument("--serve-budget-gb", type=float, default=2.0, help="HBM admitted to resident cases")
+    ap.add_argument("--serve-slots", type=int, default=32, help="cases resident at once")
+    ap.add_argument("--serve-chunk", type=int, default=100, help="largest number of steps between polls")
     ap.add_argument("--math", choices=["exact", "fast"], default="fast")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sample length")
     ap.add_argument("--profile-steps", type=int, default=50, help="eager steps timed kernel by kernel")
@@ -129,6 +319,10 @@ def main():
 
     if not os.path.exists(eppx.library_path()):
         eppx.build()
+    if args.workload == "mix":
+        return run_mix(args)
+    if args.workload == "serve":
+        return run_serve(args)
     t_setup = time.perf_counter()
     if args.workload == "large":
         case = cases.dam_break(0.001, 2.0, SPH.WendlandC2, M=1)   # slim host tables; device never stores lists
