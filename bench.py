@@ -148,6 +148,9 @@ def run_mix(args):
     t0 = time.perf_counter()
     b = Batch.Batch(par, cs[0].host, total_n, len(cs), cases.ACTIVE_PAIR, max_neighbour_number=64)
     ids = [b.add(c.domain, c.physics, c.host, c.steps) for c in cs]
+    ta = time.perf_counter()
+    tune = b.autotune(args.tune_trials) if args.tune_trials else None
+    tb = time.perf_counter()
     b.pair_total(reset=True)
     par.synchronize()
     t1 = time.perf_counter()
@@ -188,7 +191,7 @@ def run_mix(args):
     log(f"one by one: {t_single:.3f} s end to end, steps {steps_single:.3f} s, {pairs_single} pairs")
     if pairs_single != pairs_batch:
         raise RuntimeError(f"pair counts differ: batch {pairs_batch}, one by one {pairs_single}")
-    t_batch = t3 - t0
+    t_batch = (t3 - t0) - (tb - ta)  # the autotuner's trials are reported apart
     out = {
         "metric": "aggregate pair-interactions/s, 64-case uneven 2D dam-break mix, batched",
         "value": pairs_batch / t_batch, "unit": "pair-interactions/s", "n_gpus": 1, "steps": args.mix_steps,
@@ -200,8 +203,10 @@ def run_mix(args):
                    "case_particles_min": min(c.n_particles for c in cs),
                    "case_particles_max": max(c.n_particles for c in cs)},
         "one_by_one": {"value": pairs_single / t_single, "seconds": t_single, "steps_seconds": steps_single},
-        "batched": {"seconds": t_batch, "steps_seconds": t2 - t1, "add_seconds": t1 - t0,
-                    "download_seconds": t3 - t2},
+        "batched": {"seconds": t_batch, "steps_seconds": t2 - t1, "add_seconds": (ta - t0),
+                    "download_seconds": t3 - t2, "autotune_seconds": tb - ta,
+                    "pair_blocks": {"search": tune["search_block"], "momentum": tune["momentum_block"]}
+                    if tune else None},
         "speedup_end_to_end": t_single / t_batch,
         "speedup_steps": steps_single / (t2 - t1),
         "latency_s": {"batched_p50": t_batch, "batched_p99": t_batch,
@@ -302,6 +307,7 @@ def main():
     ap.add_argument("--math", choices=["exact", "fast"], default="fast")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sample length")
     ap.add_argument("--profile-steps", type=int, default=50, help="eager steps timed kernel by kernel")
+    ap.add_argument("--tune-trials", type=int, default=3, help="autotuner trials per block size (0: no tuning)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run device copies of BYTES (profiler calibration)")
@@ -340,6 +346,10 @@ def main():
     L = _lib.lib()
     Algorithm.step(ps, ns, case.physics, n_steps=args.warmup)
     log(f"warmup {args.warmup} steps done")
+    tune = Algorithm.autotune(ps, ns, case.physics, trials=args.tune_trials) if args.tune_trials else None
+    if tune:
+        log(f"autotune: search block {tune['search_block']}, momentum block {tune['momentum_block']} "
+            f"({tune['search_us']}, {tune['momentum_us']})")
     total = C.c_int64()
     _lib.check(L.eppx_pair_total(ps.handle, C.byref(total), 1), par.ctx, "pair_total")
     fb = C.c_int64()
@@ -417,6 +427,8 @@ def main():
                                f"h={case.h_factor}dx, Tait EOS, artificial viscosity)",
                    "particles": case.n_particles, "cells": case.domain.get_n(), "math": args.math,
                    "pairs_per_step": pairs / args.steps,
+                   "pair_blocks": {"search": tune["search_block"], "momentum": tune["momentum_block"]}
+                   if tune else {"search": 256, "momentum": 256},
                    "untiled_block_fraction": fb.value / (2 * args.steps) / blocks_per_launch},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,

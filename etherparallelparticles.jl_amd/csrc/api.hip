@@ -83,6 +83,7 @@ struct eppx_ps {
     int32_t last_crit = 0, last_action = 0, last_kernel = 0, last_fluid = 0, last_wall = 0;
     bool searched = false;
     StepGraph graphs[2];
+    int32_t search_block = 256, momentum_block = 256;  // pair-block sizes (eppx_autotune picks them)
     // columns the kernels do not own, kept on the host between upload and download
     std::vector<int32_t> pass_int_cols, pass_float_cols;
     std::vector<int32_t> pass_int;
@@ -382,7 +383,7 @@ int32_t do_search(eppx_ps *ps, eppx_ns *ns, int32_t criterion, int32_t action, c
     const int fast = ctx->math_mode == EPPX_MATH_FAST;
     HIP_OK(ctx, timed(ctx, "search", [&] {
                return launch_search(w, ps->f(), ps->n, action, ph, fast, ns->M, ps->ncount, &ps->pairs[0],
-                                    &ps->pairs[1], ctx->d_flags, 0, s);
+                                    &ps->pairs[1], ctx->d_flags, 0, ps->search_block, s);
            }));
     ps->last_ns = ns;
     ps->last_crit = criterion;
@@ -428,9 +429,84 @@ int32_t do_interaction(eppx_ps *ps, eppx_ns *ns, int32_t action, Physics ph, con
     const char *name = action == EPPX_INTER_MOMENTUM ? "momentum"
                        : action == EPPX_INTER_FILTER ? "filter_inter"
                                                      : "add_per_neighbour";
-    HIP_OK(ctx, timed(ctx, name, [&] { return launch_interaction(w, ps->f(), ps->n, action, ph, op, fast,
-                                                                 ctx->stream); }));
+    HIP_OK(ctx, timed(ctx, name, [&] {
+               return launch_interaction(w, ps->f(), ps->n, action, ph, op, fast, ps->momentum_block, ctx->stream);
+           }));
     return refresh_snapshot_mass(ps, action == EPPX_INTER_ADD_PER_NEIGHBOUR ? op.field : -1);
+}
+
+// Autotuner of the pair-block size: times the FAST search action and the
+// momentum pass on the current state for each candidate block size (the
+// first trial of each warms up), keeps the fastest of each kernel, and
+// restores what the trials wrote (dDensity, dVelocity, the pair counters),
+// so the state does not change.  us[k] (k = 0..2 search, 3..5 momentum, for
+// 128 / 256 / 512 threads) receives the average microseconds.  EXACT math
+// keeps 256.
+int32_t tune_pair_blocks(eppx_ps *ps, const Walk &w, const Physics &ph, int32_t M, int32_t which, int32_t trials,
+                         double *us) {
+    static const int32_t kBlocks[3] = {128, 256, 512};
+    eppx_ctx *ctx = ps->ctx;
+    for (int k = 0; k < 6; ++k) us[k] = 0.0;
+    if (ctx->math_mode != EPPX_MATH_FAST || ps->n == 0) {
+        ps->search_block = ps->momentum_block = 256;
+        return EPPX_OK;
+    }
+    hipStream_t s = ctx->stream;
+    const size_t n = (size_t)ps->n;
+    const Fields f = ps->f();
+    float *drho = nullptr;
+    float2 *dvel = nullptr;
+    unsigned long long *pairs = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = dalloc(&drho, n);
+    if (e == hipSuccess) e = dalloc(&dvel, n);
+    if (e == hipSuccess) e = dalloc(&pairs, 2);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipMemcpyAsync(drho, f.drho, n * sizeof(float), hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(dvel, f.dvel, n * sizeof(float2), hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(pairs, ps->pairs, 2 * sizeof(unsigned long long), hipMemcpyDeviceToDevice, s);
+    const eppx_field_op op{};
+    for (int kind = 0; kind < 2 && e == hipSuccess; ++kind) {
+        for (int bi = 0; bi < 3 && e == hipSuccess; ++bi) {
+            double total = 0.0;
+            for (int t = 0; t <= trials && e == hipSuccess; ++t) {
+                if (kind == 0) e = hipMemcpyAsync(f.drho, drho, n * sizeof(float), hipMemcpyDeviceToDevice, s);
+                else e = hipMemcpyAsync(f.dvel, dvel, n * sizeof(float2), hipMemcpyDeviceToDevice, s);
+                if (e == hipSuccess) e = hipEventRecord(e0, s);
+                if (e == hipSuccess)
+                    e = kind == 0 ? launch_search(w, f, ps->n, EPPX_SEARCH_KERNEL_CONTINUITY, ph, 1, M, ps->ncount,
+                                                  &ps->pairs[0], &ps->pairs[1], ctx->d_flags, which, kBlocks[bi], s)
+                                  : launch_interaction(w, f, ps->n, EPPX_INTER_MOMENTUM, ph, op, 1, kBlocks[bi], s);
+                if (e == hipSuccess) e = hipEventRecord(e1, s);
+                if (e == hipSuccess) e = hipEventSynchronize(e1);
+                float ms = 0.0f;
+                if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+                if (t > 0) total += 1000.0 * ms;  // trial 0 warms up
+            }
+            us[3 * kind + bi] = trials > 0 ? total / trials : 0.0;
+        }
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(f.drho, drho, n * sizeof(float), hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(f.dvel, dvel, n * sizeof(float2), hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(ps->pairs, pairs, 2 * sizeof(unsigned long long), hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    dfree(drho);
+    dfree(dvel);
+    dfree(pairs);
+    if (e != hipSuccess) return hip_fail(ctx, e, "autotune");
+    int best_s = 1, best_m = 1;
+    for (int bi = 0; bi < 3; ++bi) {
+        if (us[bi] < us[best_s]) best_s = bi;
+        if (us[3 + bi] < us[3 + best_m]) best_m = bi;
+    }
+    if (trials > 0) {
+        ps->search_block = kBlocks[best_s];
+        ps->momentum_block = kBlocks[best_m];
+    }
+    return EPPX_OK;
 }
 
 }  // namespace
@@ -982,6 +1058,36 @@ int32_t eppx_step(eppx_ps *ps, eppx_ns *ns, const eppx_physics *phys, int32_t n_
     return EPPX_OK;
 }
 
+int32_t eppx_autotune(eppx_ps *ps, eppx_ns *ns, const eppx_physics *phys, int32_t trials, int32_t *blocks,
+                      double *us) {
+    if (!ps || !ns || !phys || trials < 0 || !blocks) return EPPX_E_ARG;
+    eppx_ctx *ctx = ps->ctx;
+    if (!ps->searched || ps->last_ns != ns)
+        return set_error(ctx, EPPX_E_STATE, "eppx_autotune: needs a prior eppx_search on this neighbour system");
+    Physics ph{};
+    int32_t rc = physics_arg(ctx, phys, sizeof(*phys), &ph);
+    if (rc) return rc;
+    ph.kernel = ps->last_kernel;
+    HIP_OK(ctx, hipSetDevice(ctx->device));
+    double t[6];
+    rc = tune_pair_blocks(ps, make_walk(ps, ns, ps->last_crit), ph, ns->M, 0, trials, t);
+    if (rc) return rc;
+    drop_graphs(ps);  // recorded steps carry the old block sizes
+    blocks[0] = ps->search_block;
+    blocks[1] = ps->momentum_block;
+    if (us) memcpy(us, t, sizeof(t));
+    return read_flags(ctx);
+}
+
+int32_t eppx_set_pair_blocks(eppx_ps *ps, int32_t search_block, int32_t momentum_block) {
+    auto ok = [](int32_t b) { return b == 128 || b == 256 || b == 512; };
+    if (!ps || !ok(search_block) || !ok(momentum_block)) return EPPX_E_ARG;
+    ps->search_block = search_block;
+    ps->momentum_block = momentum_block;
+    drop_graphs(ps);
+    return EPPX_OK;
+}
+
 int32_t eppx_pair_count(eppx_ps *ps, eppx_ns *ns, int64_t *out) {
     if (!ps || !ns || !out) return EPPX_E_ARG;
     eppx_ctx *ctx = ps->ctx;
@@ -1216,7 +1322,8 @@ int32_t batch_cell_build(eppx_batch *b, bool search, int32_t which) {
     const int fast = ctx->math_mode == EPPX_MATH_FAST;
     HIP_OK(ctx, timed(ctx, "search", [&] {
                return launch_search(w, ps->f(), ps->n, EPPX_SEARCH_KERNEL_CONTINUITY, b->ph, fast, b->ns.M,
-                                    ps->ncount, &ps->pairs[0], &ps->pairs[1], ctx->d_flags, which, s);
+                                    ps->ncount, &ps->pairs[0], &ps->pairs[1], ctx->d_flags, which, ps->search_block,
+                                    s);
            }));
     return EPPX_OK;
 }
@@ -1233,7 +1340,8 @@ int32_t batch_enqueue_step(eppx_batch *b) {
            }));
     const Walk w = batch_walk(b);
     HIP_OK(ctx, timed(ctx, "momentum", [&] {
-               return launch_interaction(w, ps->f(), ps->n, EPPX_INTER_MOMENTUM, b->ph, op, fast, s);
+               return launch_interaction(w, ps->f(), ps->n, EPPX_INTER_MOMENTUM, b->ph, op, fast, ps->momentum_block,
+                                         s);
            }));
     HIP_OK(ctx, timed(ctx, "gravity_move", [&] {
                return launch_self(ps->f(), ps->n, EPPX_SELF_GRAVITY_MOVE, b->ph, op, b->d_cases, s);
@@ -1532,6 +1640,22 @@ int32_t eppx_batch_remove(eppx_batch *b, int32_t case_id) {
     b->meta[case_id] = BatchCase{};
     b->cases[case_id] = CaseDesc{};
     if ((rc = relayout(b))) return rc;
+    return read_flags(ctx);
+}
+
+int32_t eppx_batch_autotune(eppx_batch *b, int32_t trials, int32_t *blocks, double *us) {
+    if (!b || trials < 0 || !blocks) return EPPX_E_ARG;
+    eppx_ctx *ctx = b->ctx;
+    HIP_OK(ctx, hipSetDevice(ctx->device));
+    double t[6] = {0, 0, 0, 0, 0, 0};
+    if (b->ph_set && b->ps->n) {
+        const int32_t rc = tune_pair_blocks(b->ps, batch_walk(b), b->ph, b->ns.M, 0, trials, t);
+        if (rc) return rc;
+    }
+    drop_batch_graphs(b);
+    blocks[0] = b->ps->search_block;
+    blocks[1] = b->ps->momentum_block;
+    if (us) memcpy(us, t, sizeof(t));
     return read_flags(ctx);
 }
 

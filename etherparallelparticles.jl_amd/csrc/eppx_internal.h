@@ -117,9 +117,11 @@ hipError_t launch_permute(Fields cur, Fields nxt, float4 *geo, int32_t n, int32_
                           const int32_t *key, const int32_t *tmpslot, const int32_t *start,
                           const int32_t *members, int32_t *n_alive, uint32_t *flags, hipStream_t s);
 // which = 0: the action runs for particles of cases with steps left; 1: of fresh cases.
+// block: threads per pair block (128, 256 or 512 in FAST math; EXACT uses 256).
 hipError_t launch_search(const Walk &w, Fields f, int32_t n, int32_t action, const Physics &ph, int32_t fast,
                          int32_t M, int32_t *ncount, unsigned long long *pairs_last,
-                         unsigned long long *pairs_total, uint32_t *flags, int32_t which, hipStream_t s);
+                         unsigned long long *pairs_total, uint32_t *flags, int32_t which, int32_t block,
+                         hipStream_t s);
 hipError_t launch_snapshot_mass(float4 *geo, Fields f, int32_t n, hipStream_t s);
 // cases == nullptr: every alive particle, dt from ph; otherwise particles of
 // cases with steps left, dt from their case.
@@ -128,7 +130,7 @@ hipError_t launch_self(Fields f, int32_t n, int32_t action, const Physics &ph, c
 hipError_t launch_tick(CaseDesc *cases, int32_t n_cases, hipStream_t s);
 hipError_t launch_set_case(CaseDesc *d, const CaseDesc &v, hipStream_t s);
 hipError_t launch_interaction(const Walk &w, Fields f, int32_t n, int32_t action, const Physics &ph,
-                              const eppx_field_op &op, int32_t fast, hipStream_t s);
+                              const eppx_field_op &op, int32_t fast, int32_t block, hipStream_t s);
 hipError_t launch_export(const Walk &w, Fields f, int32_t n, const int64_t *row_start, int32_t *js,
                          const int32_t *ncount, NeighbourExport ex, hipStream_t s);
 // Rows [0, c.n_capacity) of one case's reference tables into slots

@@ -24,8 +24,12 @@ namespace eppx {
 namespace {
 
 constexpr int kRankLimit = 1024;  // largest cell ordered by particle index
-constexpr int kPairBlock = 256;   // threads per pair-kernel block
-constexpr int kTileRow = 384;     // neighbour slots per row a pair block stages in LDS
+// Pair-kernel shapes the autotuner picks from: BLOCK threads per block (one
+// particle each) and the neighbour slots per row the block stages in LDS.
+template <int BLOCK>
+struct PairShape {
+    static constexpr int tile_row = BLOCK + BLOCK / 2;
+};
 
 // sigma of the 2D kernels: FT(10/7/pi) (CubicSpline.jl:14), FT(7/4/pi) (WendlandC2.jl:13)
 constexpr float kSigmaCubic = (float)(10.0 / 7.0 / 3.14159265358979323846);
@@ -376,18 +380,20 @@ __device__ __forceinline__ int walk_rows(const Walk &w, const CaseDesc &cd, cons
 }
 
 // Stages the geometry of a pair block's neighbourhood in LDS.  The block's
-// sorted slots [i0, i0 + kPairBlock) lie in cells c0 <= c <= c1; when these
+// sorted slots [i0, i0 + BLOCK) lie in cells c0 <= c <= c1; when these
 // belong to one case and share one cell row, the three neighbour rows of
 // every particle of the block are sub-ranges of the rows' slot ranges over
-// cells c0 - 1 .. c1 + 1, which are copied to tile (kTileRow slots per row).
+// cells c0 - 1 .. c1 + 1, which are copied to tile (tile_row slots per row).
 // Returns false, with nothing staged, when the block spans two cases or two
 // cell rows or a row range does not fit; the block then reads global memory.
 // Every thread of the block calls it (two barriers).
+template <int BLOCK>
 __device__ __forceinline__ bool stage_rows(const Walk &w, int i0, int n, float4 *tile, uint8_t *tix, int *info,
                                            TileRows &rows) {
+    constexpr int kTileRow = PairShape<BLOCK>::tile_row;
     const int t = threadIdx.x;
     if (t < 3) {
-        const int last = min(i0 + kPairBlock, n) - 1;
+        const int last = min(i0 + BLOCK, n) - 1;
         const CaseDesc cd = w.cases ? w.cases[w.cs[i0]] : w.one;
         const int nx = cd.g.n_x;
         const int c0 = w.cell[i0] - 1, c1 = w.cell[last] - 1;
@@ -408,7 +414,7 @@ __device__ __forceinline__ bool stage_rows(const Walk &w, int i0, int n, float4 
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
             const int b = info[r], len = info[3 + r];
-            for (int k = t; k < len; k += kPairBlock) {
+            for (int k = t; k < len; k += BLOCK) {
                 tile[r * kTileRow + k] = w.geo[b + k];
                 const uint32_t t1 = (uint32_t)w.tag[b + k] - 1u;
                 tix[r * kTileRow + k] = (uint8_t)(t1 < (uint32_t)kMaxTag ? t1 : 63u);
@@ -679,21 +685,22 @@ struct ContinuityGather {
 // Search pass: find the neighbours of every alive particle and run the fused
 // action (EPPX_SEARCH_KERNEL_CONTINUITY: sComputeKernelAndiContinuity!,
 // main.jl:74-90).
-template <bool FAST, int ACTION, int K, bool BATCH>
-__global__ __launch_bounds__(kPairBlock) void k_search(Walk w, Fields f, Physics ph, int32_t M, int32_t *ncount,
+template <bool FAST, int ACTION, int K, bool BATCH, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_search(Walk w, Fields f, Physics ph, int32_t M, int32_t *ncount,
                                                         unsigned long long *pairs_last,
                                                         unsigned long long *pairs_total, uint32_t *flags,
                                                         int32_t which) {
-    __shared__ int lds[kPairBlock / 64];
+    constexpr int kTileRow = PairShape<BLOCK>::tile_row;
+    __shared__ int lds[BLOCK / 64];
     __shared__ float4 tile[3 * kTileRow];
     __shared__ uint8_t tix[3 * kTileRow];
     __shared__ int info[9];
     const int n = *w.n_alive;
-    const int i0 = xcd_block(blockIdx.x, gridDim.x) * kPairBlock;
+    const int i0 = xcd_block(blockIdx.x, gridDim.x) * BLOCK;
     if (i0 >= n) return;  // the whole block
     const int i = i0 + threadIdx.x;
     TileRows rows;
-    const bool tiled = ACTION == EPPX_SEARCH_KERNEL_CONTINUITY && stage_rows(w, i0, n, tile, tix, info, rows);
+    const bool tiled = ACTION == EPPX_SEARCH_KERNEL_CONTINUITY && stage_rows<BLOCK>(w, i0, n, tile, tix, info, rows);
     int cnt = 0;
     const CaseDesc cd = BATCH ? w.cases[w.cs[min(i, n - 1)]] : w.one;
     // a batch runs the action for the cases that advance (which = 0) or wait for a first search (1)
@@ -809,16 +816,17 @@ __device__ __forceinline__ void momentum_pair(const Physics &ph, float2 &acc, bo
     }
 }
 
-template <bool FAST, int K, int VISC, bool BATCH>
-__global__ __launch_bounds__(kPairBlock) void k_momentum(Walk w, Fields f, Physics ph) {
+template <bool FAST, int K, int VISC, bool BATCH, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_momentum(Walk w, Fields f, Physics ph) {
+    constexpr int kTileRow = PairShape<BLOCK>::tile_row;
     __shared__ float4 tile[3 * kTileRow];
     __shared__ uint8_t tix[3 * kTileRow];
     __shared__ int info[9];
     const int n = *w.n_alive;
-    const int i0 = xcd_block(blockIdx.x, gridDim.x) * kPairBlock;
+    const int i0 = xcd_block(blockIdx.x, gridDim.x) * BLOCK;
     if (i0 >= n) return;  // the whole block
     TileRows rows;
-    const bool tiled = stage_rows(w, i0, n, tile, tix, info, rows);
+    const bool tiled = stage_rows<BLOCK>(w, i0, n, tile, tix, info, rows);
     const int i = i0 + threadIdx.x;
     if (i >= n) return;
     if (f.tag[i] != ph.fluid_tag) return;  // iMomentum! acts on fluid particles only
@@ -1199,37 +1207,56 @@ hipError_t launch_permute(Fields cur, Fields nxt, float4 *geo, int32_t n, int32_
     return hipGetLastError();
 }
 
-template <bool FAST, int K>
-static void launch_search_k(const Walk &w, Fields f, const Physics &ph, int32_t M, int32_t *ncount,
-                            unsigned long long *pairs_last, unsigned long long *pairs_total, uint32_t *flags,
-                            int32_t which, dim3 g, hipStream_t s) {
+template <bool FAST, int K, int BLOCK>
+static void launch_search_kb(const Walk &w, Fields f, const Physics &ph, int32_t n, int32_t M, int32_t *ncount,
+                             unsigned long long *pairs_last, unsigned long long *pairs_total, uint32_t *flags,
+                             int32_t which, hipStream_t s) {
     constexpr int KCONT = EPPX_SEARCH_KERNEL_CONTINUITY;
+    const dim3 g = grid_for(n, BLOCK);
     if (w.cases)
-        hipLaunchKernelGGL((k_search<FAST, KCONT, K, true>), g, dim3(kPairBlock), 0, s, w, f, ph, M, ncount,
+        hipLaunchKernelGGL((k_search<FAST, KCONT, K, true, BLOCK>), g, dim3(BLOCK), 0, s, w, f, ph, M, ncount,
                            pairs_last, pairs_total, flags, which);
     else
-        hipLaunchKernelGGL((k_search<FAST, KCONT, K, false>), g, dim3(kPairBlock), 0, s, w, f, ph, M, ncount,
+        hipLaunchKernelGGL((k_search<FAST, KCONT, K, false, BLOCK>), g, dim3(BLOCK), 0, s, w, f, ph, M, ncount,
                            pairs_last, pairs_total, flags, which);
+}
+
+// FAST kernels come in every tuned block size; EXACT ones (the parity mode) in 256 only.
+template <bool FAST, int K>
+static void launch_search_k(const Walk &w, Fields f, const Physics &ph, int32_t n, int32_t M, int32_t *ncount,
+                            unsigned long long *pairs_last, unsigned long long *pairs_total, uint32_t *flags,
+                            int32_t which, int32_t block, hipStream_t s) {
+    if (FAST && block == 128)
+        launch_search_kb<FAST, K, 128>(w, f, ph, n, M, ncount, pairs_last, pairs_total, flags, which, s);
+    else if (FAST && block == 512)
+        launch_search_kb<FAST, K, 512>(w, f, ph, n, M, ncount, pairs_last, pairs_total, flags, which, s);
+    else
+        launch_search_kb<FAST, K, 256>(w, f, ph, n, M, ncount, pairs_last, pairs_total, flags, which, s);
 }
 
 hipError_t launch_search(const Walk &w, Fields f, int32_t n, int32_t action, const Physics &ph, int32_t fast,
                          int32_t M, int32_t *ncount, unsigned long long *pairs_last,
-                         unsigned long long *pairs_total, uint32_t *flags, int32_t which, hipStream_t s) {
-    const dim3 g = grid_for(n, kPairBlock);
+                         unsigned long long *pairs_total, uint32_t *flags, int32_t which, int32_t block,
+                         hipStream_t s) {
     constexpr int KC = EPPX_KERNEL_CUBIC, KW = EPPX_KERNEL_WENDLAND_C2;
     if (action == EPPX_SEARCH_KERNEL_CONTINUITY) {
         const bool cubic = ph.kernel == KC;
-        if (fast && cubic) launch_search_k<true, KC>(w, f, ph, M, ncount, pairs_last, pairs_total, flags, which, g, s);
-        else if (fast) launch_search_k<true, KW>(w, f, ph, M, ncount, pairs_last, pairs_total, flags, which, g, s);
-        else if (cubic) launch_search_k<false, KC>(w, f, ph, M, ncount, pairs_last, pairs_total, flags, which, g, s);
-        else launch_search_k<false, KW>(w, f, ph, M, ncount, pairs_last, pairs_total, flags, which, g, s);
+        if (fast && cubic)
+            launch_search_k<true, KC>(w, f, ph, n, M, ncount, pairs_last, pairs_total, flags, which, block, s);
+        else if (fast)
+            launch_search_k<true, KW>(w, f, ph, n, M, ncount, pairs_last, pairs_total, flags, which, block, s);
+        else if (cubic)
+            launch_search_k<false, KC>(w, f, ph, n, M, ncount, pairs_last, pairs_total, flags, which, block, s);
+        else
+            launch_search_k<false, KW>(w, f, ph, n, M, ncount, pairs_last, pairs_total, flags, which, block, s);
     } else {
+        const dim3 g = grid_for(n, 256);
         if (w.cases)
-            hipLaunchKernelGGL((k_search<false, EPPX_SEARCH_NONE, KW, true>), g, dim3(kPairBlock), 0, s, w, f, ph, M,
+            hipLaunchKernelGGL((k_search<false, EPPX_SEARCH_NONE, KW, true, 256>), g, dim3(256), 0, s, w, f, ph, M,
                                ncount, pairs_last, pairs_total, flags, which);
         else
-            hipLaunchKernelGGL((k_search<false, EPPX_SEARCH_NONE, KW, false>), g, dim3(kPairBlock), 0, s, w, f, ph,
-                               M, ncount, pairs_last, pairs_total, flags, which);
+            hipLaunchKernelGGL((k_search<false, EPPX_SEARCH_NONE, KW, false, 256>), g, dim3(256), 0, s, w, f, ph, M,
+                               ncount, pairs_last, pairs_total, flags, which);
     }
     return hipGetLastError();
 }
@@ -1254,30 +1281,37 @@ hipError_t launch_self(Fields f, int32_t n, int32_t action, const Physics &ph, c
     return hipGetLastError();
 }
 
-template <bool FAST, int K, bool BATCH>
-static void launch_momentum_b(const Walk &w, Fields f, const Physics &ph, dim3 g, hipStream_t s) {
+template <bool FAST, int K, bool BATCH, int BLOCK>
+static void launch_momentum_b(const Walk &w, Fields f, const Physics &ph, int32_t n, hipStream_t s) {
+    const dim3 g = grid_for(n, BLOCK);
     if (ph.visc == EPPX_VISC_CLASSIC)
-        hipLaunchKernelGGL((k_momentum<FAST, K, EPPX_VISC_CLASSIC, BATCH>), g, dim3(kPairBlock), 0, s, w, f, ph);
+        hipLaunchKernelGGL((k_momentum<FAST, K, EPPX_VISC_CLASSIC, BATCH, BLOCK>), g, dim3(BLOCK), 0, s, w, f, ph);
     else
-        hipLaunchKernelGGL((k_momentum<FAST, K, EPPX_VISC_ARTIFICIAL, BATCH>), g, dim3(kPairBlock), 0, s, w, f, ph);
+        hipLaunchKernelGGL((k_momentum<FAST, K, EPPX_VISC_ARTIFICIAL, BATCH, BLOCK>), g, dim3(BLOCK), 0, s, w, f, ph);
+}
+
+template <bool FAST, int K, bool BATCH>
+static void launch_momentum_s(const Walk &w, Fields f, const Physics &ph, int32_t n, int32_t block, hipStream_t s) {
+    if (FAST && block == 128) launch_momentum_b<FAST, K, BATCH, 128>(w, f, ph, n, s);
+    else if (FAST && block == 512) launch_momentum_b<FAST, K, BATCH, 512>(w, f, ph, n, s);
+    else launch_momentum_b<FAST, K, BATCH, 256>(w, f, ph, n, s);
 }
 
 template <bool FAST, int K>
-static void launch_momentum_v(const Walk &w, Fields f, const Physics &ph, dim3 g, hipStream_t s) {
-    if (w.cases) launch_momentum_b<FAST, K, true>(w, f, ph, g, s);
-    else launch_momentum_b<FAST, K, false>(w, f, ph, g, s);
+static void launch_momentum_v(const Walk &w, Fields f, const Physics &ph, int32_t n, int32_t block, hipStream_t s) {
+    if (w.cases) launch_momentum_s<FAST, K, true>(w, f, ph, n, block, s);
+    else launch_momentum_s<FAST, K, false>(w, f, ph, n, block, s);
 }
 
 hipError_t launch_interaction(const Walk &w, Fields f, int32_t n, int32_t action, const Physics &ph,
-                              const eppx_field_op &op, int32_t fast, hipStream_t s) {
+                              const eppx_field_op &op, int32_t fast, int32_t block, hipStream_t s) {
     switch (action) {
         case EPPX_INTER_MOMENTUM: {
-            const dim3 g = grid_for(n, kPairBlock);
             const bool cubic = ph.kernel == EPPX_KERNEL_CUBIC;
-            if (fast && cubic) launch_momentum_v<true, EPPX_KERNEL_CUBIC>(w, f, ph, g, s);
-            else if (fast) launch_momentum_v<true, EPPX_KERNEL_WENDLAND_C2>(w, f, ph, g, s);
-            else if (cubic) launch_momentum_v<false, EPPX_KERNEL_CUBIC>(w, f, ph, g, s);
-            else launch_momentum_v<false, EPPX_KERNEL_WENDLAND_C2>(w, f, ph, g, s);
+            if (fast && cubic) launch_momentum_v<true, EPPX_KERNEL_CUBIC>(w, f, ph, n, block, s);
+            else if (fast) launch_momentum_v<true, EPPX_KERNEL_WENDLAND_C2>(w, f, ph, n, block, s);
+            else if (cubic) launch_momentum_v<false, EPPX_KERNEL_CUBIC>(w, f, ph, n, block, s);
+            else launch_momentum_v<false, EPPX_KERNEL_WENDLAND_C2>(w, f, ph, n, block, s);
             break;
         }
         case EPPX_INTER_FILTER:

@@ -107,3 +107,18 @@ def step(particle_system: ParticleSystem, neighbour_system: NeighbourSystem, phy
                                     int(filter_interval), int(step_offset)), ctx, "step")
     if sync:
         particle_system.parallel.synchronize()
+
+
+def autotune(particle_system: ParticleSystem, neighbour_system: NeighbourSystem, physics: SPH.Physics,
+             trials: int = 3) -> dict:
+    """Picks the pair-block sizes (128 / 256 / 512 threads) of the search action and the momentum pass by
+    timing each on the current state (which it leaves unchanged); FAST math only.  Returns the choice and the
+    timings in microseconds."""
+    p = physics.c()
+    blocks = (C.c_int32 * 2)()
+    us = (C.c_double * 6)()
+    ctx = particle_system.parallel.ctx
+    _lib.check(_lib.lib().eppx_autotune(particle_system.handle, neighbour_system.handle, C.byref(p), int(trials),
+                                        blocks, us), ctx, "autotune")
+    return {"search_block": blocks[0], "momentum_block": blocks[1],
+            "search_us": dict(zip((128, 256, 512), us[0:3])), "momentum_us": dict(zip((128, 256, 512), us[3:6]))}

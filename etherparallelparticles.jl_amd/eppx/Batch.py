@@ -99,6 +99,15 @@ class Batch:
         _lib.check(_lib.lib().eppx_batch_remove(self.handle, int(case_id)), self.parallel.ctx, "Batch.remove")
         self.hosts.pop(case_id, None)
 
+    def autotune(self, trials: int = 3) -> dict:
+        """Pair-block sizes for the current cases (see Algorithm.autotune)."""
+        blocks = (C.c_int32 * 2)()
+        us = (C.c_double * 6)()
+        _lib.check(_lib.lib().eppx_batch_autotune(self.handle, int(trials), blocks, us), self.parallel.ctx,
+                   "Batch.autotune")
+        return {"search_block": blocks[0], "momentum_block": blocks[1],
+                "search_us": dict(zip((128, 256, 512), us[0:3])), "momentum_us": dict(zip((128, 256, 512), us[3:6]))}
+
     def info(self):
         slots, cases = C.c_int32(), C.c_int32()
         _lib.check(_lib.lib().eppx_batch_info(self.handle, C.byref(slots), C.byref(cases)), self.parallel.ctx, "info")

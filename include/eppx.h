@@ -203,6 +203,21 @@ int32_t eppx_interaction(eppx_ps *ps, eppx_ns *ns, int32_t action, const void *p
 int32_t eppx_step(eppx_ps *ps, eppx_ns *ns, const eppx_physics *phys, int32_t n_steps, int32_t filter_interval,
                   int64_t step_offset);
 
+/* Autotuner (no counterpart in the reference, whose kernels take the
+ * backend's default work-group size): times the FAST search action and the
+ * momentum pass on the current state for pair blocks of 128, 256 and 512
+ * threads (trials each, after one warm-up), keeps the fastest for each
+ * kernel, and leaves the state as it was.  blocks[2] receives the choice
+ * (search, momentum); us[6], when given, the average microseconds (search
+ * then momentum, by block size).  EXACT math keeps 256.  Needs a prior
+ * eppx_search on ns (waits). */
+int32_t eppx_autotune(eppx_ps *ps, eppx_ns *ns, const eppx_physics *phys, int32_t trials, int32_t *blocks,
+                      double *us);
+
+/* Sets the pair-block sizes by hand (128, 256 or 512 threads each; FAST
+ * math; results do not depend on them). */
+int32_t eppx_set_pair_blocks(eppx_ps *ps, int32_t search_block, int32_t momentum_block);
+
 /* Σ nCount over alive particles after the last search on ns (waits). */
 int32_t eppx_pair_count(eppx_ps *ps, eppx_ns *ns, int64_t *out);
 /* Running Σ nCount over every search since the last reset (waits). */
@@ -260,6 +275,8 @@ int32_t eppx_batch_download(eppx_batch *b, int32_t case_id, int32_t *n_particles
 /* Releases a case's slots and case slot. */
 int32_t eppx_batch_remove(eppx_batch *b, int32_t case_id);
 int32_t eppx_batch_info(eppx_batch *b, int32_t *slots_in_use, int32_t *cases_in_use);
+/* eppx_autotune for the batch's current cases (waits). */
+int32_t eppx_batch_autotune(eppx_batch *b, int32_t trials, int32_t *blocks, double *us);
 /* Running Σ nCount over the batch's searches since the last reset (waits). */
 int32_t eppx_batch_pair_total(eppx_batch *b, int64_t *out, int32_t reset);
 
