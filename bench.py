@@ -205,8 +205,7 @@ def run_mix(args):
         "one_by_one": {"value": pairs_single / t_single, "seconds": t_single, "steps_seconds": steps_single},
         "batched": {"seconds": t_batch, "steps_seconds": t2 - t1, "add_seconds": (ta - t0),
                     "download_seconds": t3 - t2, "autotune_seconds": tb - ta,
-                    "pair_blocks": {"search": tune["search_block"], "momentum": tune["momentum_block"]}
-                    if tune else None},
+                    "pair_shape": {"search": tune["search"], "momentum": tune["momentum"]} if tune else None},
         "speedup_end_to_end": t_single / t_batch,
         "speedup_steps": steps_single / (t2 - t1),
         "latency_s": {"batched_p50": t_batch, "batched_p99": t_batch,
@@ -348,8 +347,8 @@ def main():
     log(f"warmup {args.warmup} steps done")
     tune = Algorithm.autotune(ps, ns, case.physics, trials=args.tune_trials) if args.tune_trials else None
     if tune:
-        log(f"autotune: search block {tune['search_block']}, momentum block {tune['momentum_block']} "
-            f"({tune['search_us']}, {tune['momentum_us']})")
+        log(f"autotune: search {tune['search']}, momentum {tune['momentum']} "
+            f"(us: {tune['search_us']}, {tune['momentum_us']})")
     total = C.c_int64()
     _lib.check(L.eppx_pair_total(ps.handle, C.byref(total), 1), par.ctx, "pair_total")
     fb = C.c_int64()
@@ -427,8 +426,7 @@ def main():
                                f"h={case.h_factor}dx, Tait EOS, artificial viscosity)",
                    "particles": case.n_particles, "cells": case.domain.get_n(), "math": args.math,
                    "pairs_per_step": pairs / args.steps,
-                   "pair_blocks": {"search": tune["search_block"], "momentum": tune["momentum_block"]}
-                   if tune else {"search": 256, "momentum": 256},
+                   "pair_shape": {"search": tune["search"], "momentum": tune["momentum"]} if tune else None,
                    "untiled_block_fraction": fb.value / (2 * args.steps) / blocks_per_launch},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,

@@ -83,7 +83,11 @@ struct eppx_ps {
     int32_t last_crit = 0, last_action = 0, last_kernel = 0, last_fluid = 0, last_wall = 0;
     bool searched = false;
     StepGraph graphs[2];
-    int32_t search_block = 256, momentum_block = 256;  // pair-block sizes (eppx_autotune picks them)
+    // pair-kernel shapes (eppx_autotune picks them): threads per block, and
+    // whether pass 2 reads its per-neighbour data from the LDS tile
+    int32_t search_block = 256, momentum_block = 256, search_stage = 0, momentum_stage = 0;
+    unsigned long long *masks = nullptr;  // the last search's accept masks (3 per slot)
+    bool masks_valid = false;             // they describe every alive slot's pairs
     // columns the kernels do not own, kept on the host between upload and download
     std::vector<int32_t> pass_int_cols, pass_float_cols;
     std::vector<int32_t> pass_int;
@@ -305,6 +309,9 @@ Walk make_walk(const eppx_ps *ps, const eppx_ns *ns, int32_t criterion) {
     w.n_alive = ps->n_alive;
     memcpy(w.adj, ns->adj, sizeof(w.adj));
     w.criterion = criterion;
+    w.masks = ps->masks;
+    w.mask_stride = ps->cap;
+    w.mask_mode = 0;
     return w;
 }
 
@@ -380,10 +387,12 @@ int32_t do_search(eppx_ps *ps, eppx_ns *ns, int32_t criterion, int32_t action, c
            }));
     ps->cur ^= 1;
     Walk w = make_walk(ps, ns, criterion);
+    w.mask_mode = action == EPPX_SEARCH_KERNEL_CONTINUITY ? 1 : 0;  // keep the masks for the interactions
     const int fast = ctx->math_mode == EPPX_MATH_FAST;
     HIP_OK(ctx, timed(ctx, "search", [&] {
                return launch_search(w, ps->f(), ps->n, action, ph, fast, ns->M, ps->ncount, &ps->pairs[0],
-                                    &ps->pairs[1], ctx->d_flags, 0, ps->search_block, s);
+                                    &ps->pairs[1], ctx->d_flags, 0, ps->search_block,
+                                    ps->search_stage, s);
            }));
     ps->last_ns = ns;
     ps->last_crit = criterion;
@@ -392,6 +401,7 @@ int32_t do_search(eppx_ps *ps, eppx_ns *ns, int32_t criterion, int32_t action, c
     ps->last_fluid = ph.fluid_tag;
     ps->last_wall = ph.wall_tag;
     ps->searched = true;
+    ps->masks_valid = action == EPPX_SEARCH_KERNEL_CONTINUITY;
     return EPPX_OK;
 }
 
@@ -425,30 +435,41 @@ int32_t do_interaction(eppx_ps *ps, eppx_ns *ns, int32_t action, Physics ph, con
     eppx_ctx *ctx = ps->ctx;
     ph.kernel = ps->last_kernel;  // the stored nW / nDW of the reference come from the search
     Walk w = make_walk(ps, ns, ps->last_crit);
+    // the search's masks hold the momentum pass's pairs when its pair classes are the search's
+    if (action == EPPX_INTER_MOMENTUM && ps->masks_valid && ph.fluid_tag == ps->last_fluid &&
+        ph.wall_tag == ps->last_wall)
+        w.mask_mode = 2;
     const int fast = ctx->math_mode == EPPX_MATH_FAST;
     const char *name = action == EPPX_INTER_MOMENTUM ? "momentum"
                        : action == EPPX_INTER_FILTER ? "filter_inter"
                                                      : "add_per_neighbour";
     HIP_OK(ctx, timed(ctx, name, [&] {
-               return launch_interaction(w, ps->f(), ps->n, action, ph, op, fast, ps->momentum_block, ctx->stream);
+               return launch_interaction(w, ps->f(), ps->n, action, ph, op, fast, ps->momentum_block,
+                                         ps->momentum_stage, ctx->stream);
            }));
     return refresh_snapshot_mass(ps, action == EPPX_INTER_ADD_PER_NEIGHBOUR ? op.field : -1);
 }
 
-// Autotuner of the pair-block size: times the FAST search action and the
-// momentum pass on the current state for each candidate block size (the
-// first trial of each warms up), keeps the fastest of each kernel, and
-// restores what the trials wrote (dDensity, dVelocity, the pair counters),
-// so the state does not change.  us[k] (k = 0..2 search, 3..5 momentum, for
-// 128 / 256 / 512 threads) receives the average microseconds.  EXACT math
-// keeps 256.
+// Autotuner of the pair-kernel shape: times the FAST search action and the
+// momentum pass on the current state for each candidate shape (256 or 512
+// threads per block; per-neighbour data gathered from HBM or staged in the
+// block's LDS tile), the first trial of each as a warm-up, keeps the fastest
+// of each kernel, and restores what the trials wrote (dDensity, dVelocity,
+// the pair counters), so the state does not change.  us[k] (k = 0..3 search,
+// 4..7 momentum; shapes (256, gathered), (512, gathered), (256, staged),
+// (512, staged)) receives the average microseconds.  EXACT math keeps
+// (256, gathered).
+constexpr int32_t kShapes = 4;
+constexpr int32_t kShapeBlock[kShapes] = {256, 512, 256, 512};
+constexpr int32_t kShapeStage[kShapes] = {0, 0, 1, 1};
+
 int32_t tune_pair_blocks(eppx_ps *ps, const Walk &w, const Physics &ph, int32_t M, int32_t which, int32_t trials,
-                         double *us) {
-    static const int32_t kBlocks[3] = {128, 256, 512};
+                         bool masks_valid, double *us) {
     eppx_ctx *ctx = ps->ctx;
-    for (int k = 0; k < 6; ++k) us[k] = 0.0;
+    for (int k = 0; k < 2 * kShapes; ++k) us[k] = 0.0;
     if (ctx->math_mode != EPPX_MATH_FAST || ps->n == 0) {
         ps->search_block = ps->momentum_block = 256;
+        ps->search_stage = ps->momentum_stage = 0;
         return EPPX_OK;
     }
     hipStream_t s = ctx->stream;
@@ -467,8 +488,10 @@ int32_t tune_pair_blocks(eppx_ps *ps, const Walk &w, const Physics &ph, int32_t 
     if (e == hipSuccess) e = hipMemcpyAsync(dvel, f.dvel, n * sizeof(float2), hipMemcpyDeviceToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(pairs, ps->pairs, 2 * sizeof(unsigned long long), hipMemcpyDeviceToDevice, s);
     const eppx_field_op op{};
+    Walk wm = w;  // the momentum pass reads the search's masks as it will in a step
+    wm.mask_mode = masks_valid ? 2 : 0;
     for (int kind = 0; kind < 2 && e == hipSuccess; ++kind) {
-        for (int bi = 0; bi < 3 && e == hipSuccess; ++bi) {
+        for (int v = 0; v < kShapes && e == hipSuccess; ++v) {
             double total = 0.0;
             for (int t = 0; t <= trials && e == hipSuccess; ++t) {
                 if (kind == 0) e = hipMemcpyAsync(f.drho, drho, n * sizeof(float), hipMemcpyDeviceToDevice, s);
@@ -476,15 +499,17 @@ int32_t tune_pair_blocks(eppx_ps *ps, const Walk &w, const Physics &ph, int32_t 
                 if (e == hipSuccess) e = hipEventRecord(e0, s);
                 if (e == hipSuccess)
                     e = kind == 0 ? launch_search(w, f, ps->n, EPPX_SEARCH_KERNEL_CONTINUITY, ph, 1, M, ps->ncount,
-                                                  &ps->pairs[0], &ps->pairs[1], ctx->d_flags, which, kBlocks[bi], s)
-                                  : launch_interaction(w, f, ps->n, EPPX_INTER_MOMENTUM, ph, op, 1, kBlocks[bi], s);
+                                                  &ps->pairs[0], &ps->pairs[1], ctx->d_flags, which, kShapeBlock[v],
+                                                  kShapeStage[v], s)
+                                  : launch_interaction(wm, f, ps->n, EPPX_INTER_MOMENTUM, ph, op, 1, kShapeBlock[v],
+                                                       kShapeStage[v], s);
                 if (e == hipSuccess) e = hipEventRecord(e1, s);
                 if (e == hipSuccess) e = hipEventSynchronize(e1);
                 float ms = 0.0f;
                 if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
                 if (t > 0) total += 1000.0 * ms;  // trial 0 warms up
             }
-            us[3 * kind + bi] = trials > 0 ? total / trials : 0.0;
+            us[kShapes * kind + v] = trials > 0 ? total / trials : 0.0;
         }
     }
     if (e == hipSuccess) e = hipMemcpyAsync(f.drho, drho, n * sizeof(float), hipMemcpyDeviceToDevice, s);
@@ -497,16 +522,25 @@ int32_t tune_pair_blocks(eppx_ps *ps, const Walk &w, const Physics &ph, int32_t 
     dfree(dvel);
     dfree(pairs);
     if (e != hipSuccess) return hip_fail(ctx, e, "autotune");
-    int best_s = 1, best_m = 1;
-    for (int bi = 0; bi < 3; ++bi) {
-        if (us[bi] < us[best_s]) best_s = bi;
-        if (us[3 + bi] < us[3 + best_m]) best_m = bi;
-    }
     if (trials > 0) {
-        ps->search_block = kBlocks[best_s];
-        ps->momentum_block = kBlocks[best_m];
+        int best_s = 0, best_m = 0;
+        for (int v = 1; v < kShapes; ++v) {
+            if (us[v] < us[best_s]) best_s = v;
+            if (us[kShapes + v] < us[kShapes + best_m]) best_m = v;
+        }
+        ps->search_block = kShapeBlock[best_s];
+        ps->search_stage = kShapeStage[best_s];
+        ps->momentum_block = kShapeBlock[best_m];
+        ps->momentum_stage = kShapeStage[best_m];
     }
     return EPPX_OK;
+}
+
+void shape_out(const eppx_ps *ps, int32_t *choice) {
+    choice[0] = ps->search_block;
+    choice[1] = ps->search_stage;
+    choice[2] = ps->momentum_block;
+    choice[3] = ps->momentum_stage;
 }
 
 }  // namespace
@@ -657,6 +691,7 @@ int32_t eppx_ps_create(eppx_ctx *ctx, int32_t n_capacity, const eppx_layout *lay
     if (e == hipSuccess) e = dalloc(&ps->members, (size_t)n_capacity);
     if (e == hipSuccess) e = dalloc(&ps->ncount, (size_t)n_capacity);
     if (e == hipSuccess) e = dalloc(&ps->n_alive, 2);
+    if (e == hipSuccess) e = dalloc(&ps->masks, 3 * (size_t)n_capacity);
     if (e == hipSuccess) e = dalloc(&ps->pairs, 2);
     if (e == hipSuccess) e = dalloc(&ps->h_range, 5);
     if (e == hipSuccess) e = hipMemsetAsync(ps->pairs, 0, 2 * sizeof(unsigned long long), ctx->stream);
@@ -683,7 +718,7 @@ int32_t eppx_ps_destroy(eppx_ps *ps) {
     dfree(ps->geo);
     dfree(ps->fallback);
     dfree(ps->counts); dfree(ps->start); dfree(ps->tile_sums);
-    dfree(ps->key); dfree(ps->tmpslot); dfree(ps->members); dfree(ps->ncount); dfree(ps->n_alive);
+    dfree(ps->key); dfree(ps->tmpslot); dfree(ps->members); dfree(ps->ncount); dfree(ps->n_alive); dfree(ps->masks);
     dfree(ps->pairs); dfree(ps->h_range);
     eppx_ctx *ctx = ps->ctx;
     delete ps;
@@ -749,6 +784,7 @@ int32_t eppx_ps_upload(eppx_ps *ps, int32_t n_particles, const int32_t *is_alive
     for (size_t k = 0; k < ps->pass_float_cols.size(); ++k)
         memcpy(&ps->pass_float[k * cap], float_props + (size_t)ps->pass_float_cols[k] * cap, cap * sizeof(float));
     ps->searched = false;
+    ps->masks_valid = false;
     ps->last_ns = nullptr;
     drop_graphs(ps);  // recorded kernel arguments (particle count, H bounds) may have changed
     return EPPX_OK;
@@ -1024,7 +1060,7 @@ int32_t eppx_step(eppx_ps *ps, eppx_ns *ns, const eppx_physics *phys, int32_t n_
     };
     for (int32_t k = 1; k <= n_steps; ++k) {
         const bool filter_now = filter_interval > 0 && (step_offset + k) % filter_interval == 0;
-        if (filter_now || ctx->timing) {  // filter steps and per-kernel timing run eagerly
+        if (filter_now || ctx->timing || !ps->masks_valid) {  // filter steps and per-kernel timing run eagerly
             if ((rc = enqueue_step(filter_now))) return rc;
             continue;
         }
@@ -1058,9 +1094,9 @@ int32_t eppx_step(eppx_ps *ps, eppx_ns *ns, const eppx_physics *phys, int32_t n_
     return EPPX_OK;
 }
 
-int32_t eppx_autotune(eppx_ps *ps, eppx_ns *ns, const eppx_physics *phys, int32_t trials, int32_t *blocks,
+int32_t eppx_autotune(eppx_ps *ps, eppx_ns *ns, const eppx_physics *phys, int32_t trials, int32_t *choice,
                       double *us) {
-    if (!ps || !ns || !phys || trials < 0 || !blocks) return EPPX_E_ARG;
+    if (!ps || !ns || !phys || trials < 0 || !choice) return EPPX_E_ARG;
     eppx_ctx *ctx = ps->ctx;
     if (!ps->searched || ps->last_ns != ns)
         return set_error(ctx, EPPX_E_STATE, "eppx_autotune: needs a prior eppx_search on this neighbour system");
@@ -1069,21 +1105,23 @@ int32_t eppx_autotune(eppx_ps *ps, eppx_ns *ns, const eppx_physics *phys, int32_
     if (rc) return rc;
     ph.kernel = ps->last_kernel;
     HIP_OK(ctx, hipSetDevice(ctx->device));
-    double t[6];
-    rc = tune_pair_blocks(ps, make_walk(ps, ns, ps->last_crit), ph, ns->M, 0, trials, t);
+    double t[2 * kShapes];
+    rc = tune_pair_blocks(ps, make_walk(ps, ns, ps->last_crit), ph, ns->M, 0, trials,
+                          ps->masks_valid && ph.fluid_tag == ps->last_fluid && ph.wall_tag == ps->last_wall, t);
     if (rc) return rc;
-    drop_graphs(ps);  // recorded steps carry the old block sizes
-    blocks[0] = ps->search_block;
-    blocks[1] = ps->momentum_block;
+    drop_graphs(ps);  // recorded steps carry the old shapes
+    shape_out(ps, choice);
     if (us) memcpy(us, t, sizeof(t));
     return read_flags(ctx);
 }
 
-int32_t eppx_set_pair_blocks(eppx_ps *ps, int32_t search_block, int32_t momentum_block) {
-    auto ok = [](int32_t b) { return b == 128 || b == 256 || b == 512; };
-    if (!ps || !ok(search_block) || !ok(momentum_block)) return EPPX_E_ARG;
-    ps->search_block = search_block;
-    ps->momentum_block = momentum_block;
+int32_t eppx_set_pair_shape(eppx_ps *ps, const int32_t *choice) {
+    auto ok = [](int32_t b, int32_t st) { return (b == 256 || b == 512) && (st == 0 || st == 1); };
+    if (!ps || !choice || !ok(choice[0], choice[1]) || !ok(choice[2], choice[3])) return EPPX_E_ARG;
+    ps->search_block = choice[0];
+    ps->search_stage = choice[1];
+    ps->momentum_block = choice[2];
+    ps->momentum_stage = choice[3];
     drop_graphs(ps);
     return EPPX_OK;
 }
@@ -1291,6 +1329,9 @@ Walk batch_walk(const eppx_batch *b) {
     w.n_alive = ps->n_alive;
     memcpy(w.adj, b->ns.adj, sizeof(w.adj));
     w.criterion = EPPX_CRIT_SYMMETRY;
+    w.masks = ps->masks;
+    w.mask_stride = ps->cap;
+    w.mask_mode = 0;
     return w;
 }
 
@@ -1317,14 +1358,19 @@ int32_t batch_cell_build(eppx_batch *b, bool search, int32_t which) {
                                      ps->n_alive, ctx->d_flags, s);
            }));
     ps->cur ^= 1;
-    if (!search) return EPPX_OK;
-    const Walk w = batch_walk(b);
+    if (!search) {
+        ps->masks_valid = false;  // slots moved, masks did not
+        return EPPX_OK;
+    }
+    Walk w = batch_walk(b);
+    w.mask_mode = 1;
     const int fast = ctx->math_mode == EPPX_MATH_FAST;
     HIP_OK(ctx, timed(ctx, "search", [&] {
                return launch_search(w, ps->f(), ps->n, EPPX_SEARCH_KERNEL_CONTINUITY, b->ph, fast, b->ns.M,
                                     ps->ncount, &ps->pairs[0], &ps->pairs[1], ctx->d_flags, which, ps->search_block,
-                                    s);
+                                    ps->search_stage, s);
            }));
+    ps->masks_valid = which == 0;  // a first search (1) leaves the other cases' moved slots without masks
     return EPPX_OK;
 }
 
@@ -1338,10 +1384,11 @@ int32_t batch_enqueue_step(eppx_batch *b) {
     HIP_OK(ctx, timed(ctx, "continuity_eos", [&] {
                return launch_self(ps->f(), ps->n, EPPX_SELF_CONTINUITY_EOS, b->ph, op, b->d_cases, s);
            }));
-    const Walk w = batch_walk(b);
+    Walk w = batch_walk(b);
+    w.mask_mode = ps->masks_valid ? 2 : 0;
     HIP_OK(ctx, timed(ctx, "momentum", [&] {
                return launch_interaction(w, ps->f(), ps->n, EPPX_INTER_MOMENTUM, b->ph, op, fast, ps->momentum_block,
-                                         s);
+                                         ps->momentum_stage, s);
            }));
     HIP_OK(ctx, timed(ctx, "gravity_move", [&] {
                return launch_self(ps->f(), ps->n, EPPX_SELF_GRAVITY_MOVE, b->ph, op, b->d_cases, s);
@@ -1359,8 +1406,9 @@ extern "C" {
 int32_t eppx_batch_bytes_per_particle(int32_t *out) {
     if (!out) return EPPX_E_ARG;
     // two buffers of 80 B (the 76 B of particle state and the case id), the
-    // 16 B search snapshot, key, provisional slot, cell member and count
-    *out = 2 * 80 + 16 + 4 * 4;
+    // 16 B search snapshot, key, provisional slot, cell member and count, and
+    // the search's three 64-bit accept masks
+    *out = 2 * 80 + 16 + 4 * 4 + 24;
     return EPPX_OK;
 }
 
@@ -1526,7 +1574,7 @@ int32_t eppx_batch_step(eppx_batch *b, int32_t n_steps) {
     if (!b->ph_set || ps->n == 0) return EPPX_OK;
     for (int32_t k = 0; k < n_steps; ++k) {
         int32_t rc;
-        if (ctx->timing) {
+        if (ctx->timing || !ps->masks_valid) {  // the step after a membership change finds its own pairs
             if ((rc = batch_enqueue_step(b))) return rc;
             continue;
         }
@@ -1643,18 +1691,17 @@ int32_t eppx_batch_remove(eppx_batch *b, int32_t case_id) {
     return read_flags(ctx);
 }
 
-int32_t eppx_batch_autotune(eppx_batch *b, int32_t trials, int32_t *blocks, double *us) {
-    if (!b || trials < 0 || !blocks) return EPPX_E_ARG;
+int32_t eppx_batch_autotune(eppx_batch *b, int32_t trials, int32_t *choice, double *us) {
+    if (!b || trials < 0 || !choice) return EPPX_E_ARG;
     eppx_ctx *ctx = b->ctx;
     HIP_OK(ctx, hipSetDevice(ctx->device));
-    double t[6] = {0, 0, 0, 0, 0, 0};
+    double t[2 * kShapes] = {};
     if (b->ph_set && b->ps->n) {
-        const int32_t rc = tune_pair_blocks(b->ps, batch_walk(b), b->ph, b->ns.M, 0, trials, t);
+        const int32_t rc = tune_pair_blocks(b->ps, batch_walk(b), b->ph, b->ns.M, 0, trials, b->ps->masks_valid, t);
         if (rc) return rc;
     }
     drop_batch_graphs(b);
-    blocks[0] = b->ps->search_block;
-    blocks[1] = b->ps->momentum_block;
+    shape_out(b->ps, choice);
     if (us) memcpy(us, t, sizeof(t));
     return read_flags(ctx);
 }

@@ -69,6 +69,11 @@ struct Walk {
     uint32_t adj[kMaxTag + 1];  // adj[t]: bit (u-1) set when tag t is adjacent to tag u
     int32_t criterion;
     uint32_t *fallback;       // blocks that walked global memory instead of LDS (counter)
+    // The search's accept masks of the three neighbour rows of each slot
+    // (masks[r * mask_stride + i]): mask_mode 1 stores them, 2 reads them
+    // instead of testing the candidates again (same pair set), 0 neither.
+    unsigned long long *masks;
+    int32_t mask_stride, mask_mode;
 };
 
 struct Physics {
@@ -117,11 +122,12 @@ hipError_t launch_permute(Fields cur, Fields nxt, float4 *geo, int32_t n, int32_
                           const int32_t *key, const int32_t *tmpslot, const int32_t *start,
                           const int32_t *members, int32_t *n_alive, uint32_t *flags, hipStream_t s);
 // which = 0: the action runs for particles of cases with steps left; 1: of fresh cases.
-// block: threads per pair block (128, 256 or 512 in FAST math; EXACT uses 256).
+// block: threads per pair block (256 or 512); stage: 1 when pass 2 reads
+// its per-neighbour data from the block's LDS tile.  EXACT math uses 256, 0.
 hipError_t launch_search(const Walk &w, Fields f, int32_t n, int32_t action, const Physics &ph, int32_t fast,
                          int32_t M, int32_t *ncount, unsigned long long *pairs_last,
                          unsigned long long *pairs_total, uint32_t *flags, int32_t which, int32_t block,
-                         hipStream_t s);
+                         int32_t stage, hipStream_t s);
 hipError_t launch_snapshot_mass(float4 *geo, Fields f, int32_t n, hipStream_t s);
 // cases == nullptr: every alive particle, dt from ph; otherwise particles of
 // cases with steps left, dt from their case.
@@ -130,7 +136,7 @@ hipError_t launch_self(Fields f, int32_t n, int32_t action, const Physics &ph, c
 hipError_t launch_tick(CaseDesc *cases, int32_t n_cases, hipStream_t s);
 hipError_t launch_set_case(CaseDesc *d, const CaseDesc &v, hipStream_t s);
 hipError_t launch_interaction(const Walk &w, Fields f, int32_t n, int32_t action, const Physics &ph,
-                              const eppx_field_op &op, int32_t fast, int32_t block, hipStream_t s);
+                              const eppx_field_op &op, int32_t fast, int32_t block, int32_t stage, hipStream_t s);
 hipError_t launch_export(const Walk &w, Fields f, int32_t n, const int64_t *row_start, int32_t *js,
                          const int32_t *ncount, NeighbourExport ex, hipStream_t s);
 // Rows [0, c.n_capacity) of one case's reference tables into slots

@@ -17,6 +17,7 @@
 #include <math.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "eppx_internal.h"
 
@@ -192,8 +193,9 @@ __device__ __forceinline__ void walk(const Walk &w, const CaseDesc &cd, int i, f
 //    cell-sorted order, so each row is one slot range [b_r, e_r) whose
 //    candidates are tested, four at a time, into a 64-bit accept mask.
 //  Pass 2: for every accepted neighbour j whose tag is in keep_tags (the
-//    tag filter is folded into the pass-1 masks), p = fetch(j, tag bit)
-//    gathers what the pair term needs from global memory and
+//    tag filter is folded into the pass-1 masks), p = fetch(j, k, tag bit,
+//    src) gathers what the pair term needs (slot k of src when src stages
+//    it, else global memory) and
 //    apply(j, tag bit, h_j, m_j, dx, dy, dr2, p) adds the term.  ORDERED
 //    visits them in the reference's list order (x offset outer, y offset
 //    inner, ascending slot inside a cell), so sums are bit-identical to the
@@ -239,7 +241,10 @@ __device__ __forceinline__ uint32_t accept_bits(const Walk &w, const CaseDesc &c
 // Candidate geometry of the row walk: slot j of neighbour row r, read from
 // global memory or from the block's LDS copy of its three row ranges.
 // get(k): the snapshot (x, y, h, m) of slot k; tbit(k): its tag bit.
+// kStage: what else a tile holds for pass 2 (0 nothing, 1 the (u, v, rho, p)
+// record, 2 its (u, v) half).
 struct GlobalRows {
+    static constexpr int kStage = 0;
     const float4 *geo;
     const int32_t *tag;
     __device__ __forceinline__ int off(int) const { return 0; }
@@ -247,9 +252,13 @@ struct GlobalRows {
     __device__ __forceinline__ uint32_t tbit(int k) const { return tag_bit(tag[k]); }
     __device__ __forceinline__ float4 at(int r, int j) const { return get(j + off(r)); }
 };
+template <int STAGE>
 struct TileRows {
+    static constexpr int kStage = STAGE;
     const float4 *tile;   // LDS: row r, slot j at tile[j + off(r)]
     const uint8_t *tix;   // LDS: tag - 1 of the same slot (63 outside 1..kMaxTag)
+    const float4 *dyn;    // LDS (STAGE 1): (u, v, rho, p) of the same slot
+    const float2 *uv;     // LDS (STAGE 2): (u, v) of the same slot
     int off0, off1, off2;
     __device__ __forceinline__ int off(int r) const { return sel3(r, off0, off1, off2); }
     __device__ __forceinline__ float4 get(int k) const { return tile[k]; }
@@ -304,7 +313,7 @@ __device__ __forceinline__ int walk_rows(const Walk &w, const CaseDesc &cd, cons
             ++cnt;
             if (tbj & keep_tags) {
                 const float4 gj = w.geo[j];
-                apply(j, tbj, gj.z, gj.w, dx, dy, dr2, fetch(j, tbj));
+                apply(j, tbj, gj.z, gj.w, dx, dy, dr2, fetch(j, j, tbj, GlobalRows{w.geo, w.tag}));
             }
         });
         return COUNT ? cnt : 0;
@@ -312,21 +321,33 @@ __device__ __forceinline__ int walk_rows(const Walk &w, const CaseDesc &cd, cons
     const uint32_t adj = adj_of(w, tag_bit(w.tag[i]));
     const uint32_t kadj = adj & keep_tags;
     const float2 xi = make_float2(gi.x, gi.y);
-    // The band and the tag test are skipped for the whole wave when no lane
-    // needs them: the band is empty and every tag present is in kadj.
-    const bool check = __ballot(cd.check_all || (cd.tags_present & ~kadj) != 0u) != 0ull;
-    unsigned long long m0, m1, m2;
-    if (check) {
-        m0 = accept_row<true>(w, cd, src, 0, b[0], e[0], xi, gi.z, kadj);
-        m1 = accept_row<true>(w, cd, src, 1, b[1], e[1], xi, gi.z, kadj);
-        m2 = accept_row<true>(w, cd, src, 2, b[2], e[2], xi, gi.z, kadj);
-    } else {
-        m0 = accept_row<false>(w, cd, src, 0, b[0], e[0], xi, gi.z, kadj);
-        m1 = accept_row<false>(w, cd, src, 1, b[1], e[1], xi, gi.z, kadj);
-        m2 = accept_row<false>(w, cd, src, 2, b[2], e[2], xi, gi.z, kadj);
-    }
     const unsigned long long self = 1ull << (i - b[1]);
-    m1 &= ~self;  // I != J
+    unsigned long long m0, m1, m2;
+    const size_t ms = (size_t)w.mask_stride;
+    if (!COUNT && w.mask_mode == 2) {  // the last search's masks: the same pair set, the same keep classes
+        m0 = w.masks[i];
+        m1 = w.masks[i + ms];
+        m2 = w.masks[i + 2 * ms];
+    } else {
+        // The band and the tag test are skipped for the whole wave when no
+        // lane needs them: the band is empty and every tag present is in kadj.
+        const bool check = __ballot(cd.check_all || (cd.tags_present & ~kadj) != 0u) != 0ull;
+        if (check) {
+            m0 = accept_row<true>(w, cd, src, 0, b[0], e[0], xi, gi.z, kadj);
+            m1 = accept_row<true>(w, cd, src, 1, b[1], e[1], xi, gi.z, kadj);
+            m2 = accept_row<true>(w, cd, src, 2, b[2], e[2], xi, gi.z, kadj);
+        } else {
+            m0 = accept_row<false>(w, cd, src, 0, b[0], e[0], xi, gi.z, kadj);
+            m1 = accept_row<false>(w, cd, src, 1, b[1], e[1], xi, gi.z, kadj);
+            m2 = accept_row<false>(w, cd, src, 2, b[2], e[2], xi, gi.z, kadj);
+        }
+        m1 &= ~self;  // I != J
+        if (w.mask_mode == 1) {
+            w.masks[i] = m0;
+            w.masks[i + ms] = m1;
+            w.masks[i + 2 * ms] = m2;
+        }
+    }
     int cnt = 0;
     if (COUNT) {
         if (adj & ~keep_tags) {  // some adjacent tags are counted but not visited
@@ -357,7 +378,7 @@ __device__ __forceinline__ int walk_rows(const Walk &w, const CaseDesc &cd, cons
                 const int j = cb + __builtin_ctzll(m);
                 m &= m - 1ull;
                 const uint32_t tbj = src.tbit(j + roff);
-                visit(j, roff, tbj, fetch(j, tbj));
+                visit(j, roff, tbj, fetch(j, j + roff, tbj, src));
             }
         }
     } else {
@@ -373,7 +394,7 @@ __device__ __forceinline__ int walk_rows(const Walk &w, const CaseDesc &cd, cons
             n0 &= n0 - 1ull;
             if (!n0) { n0 = n1; b0 = b1; o0 = o1; n1 = n2; b1 = b2; o1 = o2; n2 = 0ull; }
             const uint32_t tbj = src.tbit(j + jo);
-            visit(j, jo, tbj, fetch(j, tbj));
+            visit(j, jo, tbj, fetch(j, j + jo, tbj, src));
         }
     }
     return cnt;
@@ -387,9 +408,10 @@ __device__ __forceinline__ int walk_rows(const Walk &w, const CaseDesc &cd, cons
 // Returns false, with nothing staged, when the block spans two cases or two
 // cell rows or a row range does not fit; the block then reads global memory.
 // Every thread of the block calls it (two barriers).
-template <int BLOCK>
-__device__ __forceinline__ bool stage_rows(const Walk &w, int i0, int n, float4 *tile, uint8_t *tix, int *info,
-                                           TileRows &rows) {
+template <int BLOCK, int STAGE>
+__device__ __forceinline__ bool stage_rows(const Walk &w, const float4 *dyn, int i0, int n, float4 *tile,
+                                           uint8_t *tix, float4 *tdyn, float2 *tuv, int *info,
+                                           TileRows<STAGE> &rows) {
     constexpr int kTileRow = PairShape<BLOCK>::tile_row;
     const int t = threadIdx.x;
     if (t < 3) {
@@ -418,10 +440,14 @@ __device__ __forceinline__ bool stage_rows(const Walk &w, int i0, int n, float4 
                 tile[r * kTileRow + k] = w.geo[b + k];
                 const uint32_t t1 = (uint32_t)w.tag[b + k] - 1u;
                 tix[r * kTileRow + k] = (uint8_t)(t1 < (uint32_t)kMaxTag ? t1 : 63u);
+                if (STAGE == 1) tdyn[r * kTileRow + k] = dyn[b + k];
+                if (STAGE == 2) tuv[r * kTileRow + k] = reinterpret_cast<const float2 *>(dyn)[2 * (size_t)(b + k)];
             }
         }
         rows.tile = tile;
         rows.tix = tix;
+        rows.dyn = tdyn;
+        rows.uv = tuv;
         rows.off0 = -info[0];
         rows.off1 = kTileRow - info[1];
         rows.off2 = 2 * kTileRow - info[2];
@@ -685,7 +711,7 @@ struct ContinuityGather {
 // Search pass: find the neighbours of every alive particle and run the fused
 // action (EPPX_SEARCH_KERNEL_CONTINUITY: sComputeKernelAndiContinuity!,
 // main.jl:74-90).
-template <bool FAST, int ACTION, int K, bool BATCH, int BLOCK>
+template <bool FAST, int ACTION, int K, bool BATCH, int BLOCK, int STAGE>
 __global__ __launch_bounds__(BLOCK) void k_search(Walk w, Fields f, Physics ph, int32_t M, int32_t *ncount,
                                                         unsigned long long *pairs_last,
                                                         unsigned long long *pairs_total, uint32_t *flags,
@@ -694,13 +720,15 @@ __global__ __launch_bounds__(BLOCK) void k_search(Walk w, Fields f, Physics ph, 
     __shared__ int lds[BLOCK / 64];
     __shared__ float4 tile[3 * kTileRow];
     __shared__ uint8_t tix[3 * kTileRow];
+    __shared__ float2 tuv[STAGE == 2 ? 3 * kTileRow : 1];
     __shared__ int info[9];
     const int n = *w.n_alive;
     const int i0 = xcd_block(blockIdx.x, gridDim.x) * BLOCK;
     if (i0 >= n) return;  // the whole block
     const int i = i0 + threadIdx.x;
-    TileRows rows;
-    const bool tiled = ACTION == EPPX_SEARCH_KERNEL_CONTINUITY && stage_rows<BLOCK>(w, i0, n, tile, tix, info, rows);
+    TileRows<STAGE> rows;
+    const bool tiled = ACTION == EPPX_SEARCH_KERNEL_CONTINUITY &&
+                       stage_rows<BLOCK, STAGE>(w, f.dyn, i0, n, tile, tix, nullptr, tuv, info, rows);
     int cnt = 0;
     const CaseDesc cd = BATCH ? w.cases[w.cs[min(i, n - 1)]] : w.one;
     // a batch runs the action for the cases that advance (which = 0) or wait for a first search (1)
@@ -719,9 +747,10 @@ __global__ __launch_bounds__(BLOCK) void k_search(Walk w, Fields f, Physics ph, 
             const float2 *uv = reinterpret_cast<const float2 *>(f.dyn);  // (u, v) of slot j at uv[2 j]
             const uint32_t fluid_bit = tag_bit(ph.fluid_tag);
             // u_j, and V_j for mixed pairs only (fluid-fluid terms use m_j from the snapshot)
-            auto fetch = [&](int j, uint32_t tbj) {
+            auto fetch = [&](int j, int k, uint32_t tbj, const auto &src) {
                 ContinuityGather p;
-                p.uv = uv[2 * (size_t)j];
+                if constexpr (std::decay_t<decltype(src)>::kStage == 2) p.uv = src.uv[k];
+                else p.uv = uv[2 * (size_t)j];
                 p.vol = (i_fluid && tbj == fluid_bit) ? 0.0f : f.vol[j];
                 return p;
             };
@@ -816,17 +845,18 @@ __device__ __forceinline__ void momentum_pair(const Physics &ph, float2 &acc, bo
     }
 }
 
-template <bool FAST, int K, int VISC, bool BATCH, int BLOCK>
+template <bool FAST, int K, int VISC, bool BATCH, int BLOCK, int STAGE>
 __global__ __launch_bounds__(BLOCK) void k_momentum(Walk w, Fields f, Physics ph) {
     constexpr int kTileRow = PairShape<BLOCK>::tile_row;
     __shared__ float4 tile[3 * kTileRow];
     __shared__ uint8_t tix[3 * kTileRow];
+    __shared__ float4 tdyn[STAGE == 1 ? 3 * kTileRow : 1];
     __shared__ int info[9];
     const int n = *w.n_alive;
     const int i0 = xcd_block(blockIdx.x, gridDim.x) * BLOCK;
     if (i0 >= n) return;  // the whole block
-    TileRows rows;
-    const bool tiled = stage_rows<BLOCK>(w, i0, n, tile, tix, info, rows);
+    TileRows<STAGE> rows;
+    const bool tiled = stage_rows<BLOCK, STAGE>(w, f.dyn, i0, n, tile, tix, tdyn, nullptr, info, rows);
     const int i = i0 + threadIdx.x;
     if (i >= n) return;
     if (f.tag[i] != ph.fluid_tag) return;  // iMomentum! acts on fluid particles only
@@ -838,7 +868,10 @@ __global__ __launch_bounds__(BLOCK) void k_momentum(Walk w, Fields f, Physics ph
     float2 acc = f.dvel[i];
     const float pr_i = FAST ? di.w * frcp(di.z * di.z) : 0.0f;
     const uint32_t fluid_bit = tag_bit(ph.fluid_tag);
-    auto fetch = [&](int j, uint32_t) { return f.dyn[j]; };
+    auto fetch = [&](int j, int k, uint32_t, const auto &src) -> float4 {
+        if constexpr (std::decay_t<decltype(src)>::kStage == 1) return src.dyn[k];
+        else return f.dyn[j];
+    };
     auto term = [&](int, uint32_t tbj, float hj, float mj, float dx, float dy, float dr2, const float4 &dj) {
         momentum_pair<FAST, K, VISC>(ph, acc, tbj == fluid_bit, gi.z, hj, ui, di.z, di.w, pr_i,
                                      make_float2(dj.x, dj.y), dj.z, dj.w, mj, dx, dy, dr2);
@@ -1207,55 +1240,58 @@ hipError_t launch_permute(Fields cur, Fields nxt, float4 *geo, int32_t n, int32_
     return hipGetLastError();
 }
 
-template <bool FAST, int K, int BLOCK>
+template <bool FAST, int K, int BLOCK, int STAGE>
 static void launch_search_kb(const Walk &w, Fields f, const Physics &ph, int32_t n, int32_t M, int32_t *ncount,
                              unsigned long long *pairs_last, unsigned long long *pairs_total, uint32_t *flags,
                              int32_t which, hipStream_t s) {
     constexpr int KCONT = EPPX_SEARCH_KERNEL_CONTINUITY;
     const dim3 g = grid_for(n, BLOCK);
     if (w.cases)
-        hipLaunchKernelGGL((k_search<FAST, KCONT, K, true, BLOCK>), g, dim3(BLOCK), 0, s, w, f, ph, M, ncount,
+        hipLaunchKernelGGL((k_search<FAST, KCONT, K, true, BLOCK, STAGE>), g, dim3(BLOCK), 0, s, w, f, ph, M, ncount,
                            pairs_last, pairs_total, flags, which);
     else
-        hipLaunchKernelGGL((k_search<FAST, KCONT, K, false, BLOCK>), g, dim3(BLOCK), 0, s, w, f, ph, M, ncount,
+        hipLaunchKernelGGL((k_search<FAST, KCONT, K, false, BLOCK, STAGE>), g, dim3(BLOCK), 0, s, w, f, ph, M, ncount,
                            pairs_last, pairs_total, flags, which);
 }
 
-// FAST kernels come in every tuned block size; EXACT ones (the parity mode) in 256 only.
+// FAST kernels come in every tuned shape (256 or 512 threads; (u, v) staged
+// in LDS or gathered); EXACT ones (the parity mode) in 256 threads, gathered.
 template <bool FAST, int K>
 static void launch_search_k(const Walk &w, Fields f, const Physics &ph, int32_t n, int32_t M, int32_t *ncount,
                             unsigned long long *pairs_last, unsigned long long *pairs_total, uint32_t *flags,
-                            int32_t which, int32_t block, hipStream_t s) {
-    if (FAST && block == 128)
-        launch_search_kb<FAST, K, 128>(w, f, ph, n, M, ncount, pairs_last, pairs_total, flags, which, s);
+                            int32_t which, int32_t block, int32_t stage, hipStream_t s) {
+    if (FAST && block == 512 && stage)
+        launch_search_kb<FAST, K, 512, 2>(w, f, ph, n, M, ncount, pairs_last, pairs_total, flags, which, s);
     else if (FAST && block == 512)
-        launch_search_kb<FAST, K, 512>(w, f, ph, n, M, ncount, pairs_last, pairs_total, flags, which, s);
+        launch_search_kb<FAST, K, 512, 0>(w, f, ph, n, M, ncount, pairs_last, pairs_total, flags, which, s);
+    else if (FAST && stage)
+        launch_search_kb<FAST, K, 256, 2>(w, f, ph, n, M, ncount, pairs_last, pairs_total, flags, which, s);
     else
-        launch_search_kb<FAST, K, 256>(w, f, ph, n, M, ncount, pairs_last, pairs_total, flags, which, s);
+        launch_search_kb<FAST, K, 256, 0>(w, f, ph, n, M, ncount, pairs_last, pairs_total, flags, which, s);
 }
 
 hipError_t launch_search(const Walk &w, Fields f, int32_t n, int32_t action, const Physics &ph, int32_t fast,
                          int32_t M, int32_t *ncount, unsigned long long *pairs_last,
                          unsigned long long *pairs_total, uint32_t *flags, int32_t which, int32_t block,
-                         hipStream_t s) {
+                         int32_t stage, hipStream_t s) {
     constexpr int KC = EPPX_KERNEL_CUBIC, KW = EPPX_KERNEL_WENDLAND_C2;
     if (action == EPPX_SEARCH_KERNEL_CONTINUITY) {
         const bool cubic = ph.kernel == KC;
         if (fast && cubic)
-            launch_search_k<true, KC>(w, f, ph, n, M, ncount, pairs_last, pairs_total, flags, which, block, s);
+            launch_search_k<true, KC>(w, f, ph, n, M, ncount, pairs_last, pairs_total, flags, which, block, stage, s);
         else if (fast)
-            launch_search_k<true, KW>(w, f, ph, n, M, ncount, pairs_last, pairs_total, flags, which, block, s);
+            launch_search_k<true, KW>(w, f, ph, n, M, ncount, pairs_last, pairs_total, flags, which, block, stage, s);
         else if (cubic)
-            launch_search_k<false, KC>(w, f, ph, n, M, ncount, pairs_last, pairs_total, flags, which, block, s);
+            launch_search_k<false, KC>(w, f, ph, n, M, ncount, pairs_last, pairs_total, flags, which, block, stage, s);
         else
-            launch_search_k<false, KW>(w, f, ph, n, M, ncount, pairs_last, pairs_total, flags, which, block, s);
+            launch_search_k<false, KW>(w, f, ph, n, M, ncount, pairs_last, pairs_total, flags, which, block, stage, s);
     } else {
         const dim3 g = grid_for(n, 256);
         if (w.cases)
-            hipLaunchKernelGGL((k_search<false, EPPX_SEARCH_NONE, KW, true, 256>), g, dim3(256), 0, s, w, f, ph, M,
+            hipLaunchKernelGGL((k_search<false, EPPX_SEARCH_NONE, KW, true, 256, 0>), g, dim3(256), 0, s, w, f, ph, M,
                                ncount, pairs_last, pairs_total, flags, which);
         else
-            hipLaunchKernelGGL((k_search<false, EPPX_SEARCH_NONE, KW, false, 256>), g, dim3(256), 0, s, w, f, ph, M,
+            hipLaunchKernelGGL((k_search<false, EPPX_SEARCH_NONE, KW, false, 256, 0>), g, dim3(256), 0, s, w, f, ph, M,
                                ncount, pairs_last, pairs_total, flags, which);
     }
     return hipGetLastError();
@@ -1281,37 +1317,42 @@ hipError_t launch_self(Fields f, int32_t n, int32_t action, const Physics &ph, c
     return hipGetLastError();
 }
 
-template <bool FAST, int K, bool BATCH, int BLOCK>
+template <bool FAST, int K, bool BATCH, int BLOCK, int STAGE>
 static void launch_momentum_b(const Walk &w, Fields f, const Physics &ph, int32_t n, hipStream_t s) {
     const dim3 g = grid_for(n, BLOCK);
     if (ph.visc == EPPX_VISC_CLASSIC)
-        hipLaunchKernelGGL((k_momentum<FAST, K, EPPX_VISC_CLASSIC, BATCH, BLOCK>), g, dim3(BLOCK), 0, s, w, f, ph);
+        hipLaunchKernelGGL((k_momentum<FAST, K, EPPX_VISC_CLASSIC, BATCH, BLOCK, STAGE>), g, dim3(BLOCK), 0, s, w, f,
+                           ph);
     else
-        hipLaunchKernelGGL((k_momentum<FAST, K, EPPX_VISC_ARTIFICIAL, BATCH, BLOCK>), g, dim3(BLOCK), 0, s, w, f, ph);
+        hipLaunchKernelGGL((k_momentum<FAST, K, EPPX_VISC_ARTIFICIAL, BATCH, BLOCK, STAGE>), g, dim3(BLOCK), 0, s, w,
+                           f, ph);
 }
 
 template <bool FAST, int K, bool BATCH>
-static void launch_momentum_s(const Walk &w, Fields f, const Physics &ph, int32_t n, int32_t block, hipStream_t s) {
-    if (FAST && block == 128) launch_momentum_b<FAST, K, BATCH, 128>(w, f, ph, n, s);
-    else if (FAST && block == 512) launch_momentum_b<FAST, K, BATCH, 512>(w, f, ph, n, s);
-    else launch_momentum_b<FAST, K, BATCH, 256>(w, f, ph, n, s);
+static void launch_momentum_s(const Walk &w, Fields f, const Physics &ph, int32_t n, int32_t block, int32_t stage,
+                              hipStream_t s) {
+    if (FAST && block == 512 && stage) launch_momentum_b<FAST, K, BATCH, 512, 1>(w, f, ph, n, s);
+    else if (FAST && block == 512) launch_momentum_b<FAST, K, BATCH, 512, 0>(w, f, ph, n, s);
+    else if (FAST && stage) launch_momentum_b<FAST, K, BATCH, 256, 1>(w, f, ph, n, s);
+    else launch_momentum_b<FAST, K, BATCH, 256, 0>(w, f, ph, n, s);
 }
 
 template <bool FAST, int K>
-static void launch_momentum_v(const Walk &w, Fields f, const Physics &ph, int32_t n, int32_t block, hipStream_t s) {
-    if (w.cases) launch_momentum_s<FAST, K, true>(w, f, ph, n, block, s);
-    else launch_momentum_s<FAST, K, false>(w, f, ph, n, block, s);
+static void launch_momentum_v(const Walk &w, Fields f, const Physics &ph, int32_t n, int32_t block, int32_t stage,
+                              hipStream_t s) {
+    if (w.cases) launch_momentum_s<FAST, K, true>(w, f, ph, n, block, stage, s);
+    else launch_momentum_s<FAST, K, false>(w, f, ph, n, block, stage, s);
 }
 
 hipError_t launch_interaction(const Walk &w, Fields f, int32_t n, int32_t action, const Physics &ph,
-                              const eppx_field_op &op, int32_t fast, int32_t block, hipStream_t s) {
+                              const eppx_field_op &op, int32_t fast, int32_t block, int32_t stage, hipStream_t s) {
     switch (action) {
         case EPPX_INTER_MOMENTUM: {
             const bool cubic = ph.kernel == EPPX_KERNEL_CUBIC;
-            if (fast && cubic) launch_momentum_v<true, EPPX_KERNEL_CUBIC>(w, f, ph, n, block, s);
-            else if (fast) launch_momentum_v<true, EPPX_KERNEL_WENDLAND_C2>(w, f, ph, n, block, s);
-            else if (cubic) launch_momentum_v<false, EPPX_KERNEL_CUBIC>(w, f, ph, n, block, s);
-            else launch_momentum_v<false, EPPX_KERNEL_WENDLAND_C2>(w, f, ph, n, block, s);
+            if (fast && cubic) launch_momentum_v<true, EPPX_KERNEL_CUBIC>(w, f, ph, n, block, stage, s);
+            else if (fast) launch_momentum_v<true, EPPX_KERNEL_WENDLAND_C2>(w, f, ph, n, block, stage, s);
+            else if (cubic) launch_momentum_v<false, EPPX_KERNEL_CUBIC>(w, f, ph, n, block, stage, s);
+            else launch_momentum_v<false, EPPX_KERNEL_WENDLAND_C2>(w, f, ph, n, block, stage, s);
             break;
         }
         case EPPX_INTER_FILTER:

@@ -109,16 +109,32 @@ def step(particle_system: ParticleSystem, neighbour_system: NeighbourSystem, phy
         particle_system.parallel.synchronize()
 
 
+SHAPES = ((256, 0), (512, 0), (256, 1), (512, 1))  # (threads per block, per-neighbour data staged in LDS)
+
+
+def _shape_dict(choice, us):
+    return {"search": {"block": choice[0], "staged": bool(choice[1])},
+            "momentum": {"block": choice[2], "staged": bool(choice[3])},
+            "search_us": {f"{b}{'s' if st else 'g'}": us[k] for k, (b, st) in enumerate(SHAPES)},
+            "momentum_us": {f"{b}{'s' if st else 'g'}": us[4 + k] for k, (b, st) in enumerate(SHAPES)}}
+
+
 def autotune(particle_system: ParticleSystem, neighbour_system: NeighbourSystem, physics: SPH.Physics,
              trials: int = 3) -> dict:
-    """Picks the pair-block sizes (128 / 256 / 512 threads) of the search action and the momentum pass by
-    timing each on the current state (which it leaves unchanged); FAST math only.  Returns the choice and the
-    timings in microseconds."""
+    """Picks the pair-kernel shapes of the search action and the momentum pass (256 or 512 threads per block;
+    per-neighbour data gathered from HBM or staged in LDS) by timing each on the current state, which it leaves
+    unchanged; FAST math only.  Returns the choice and the timings in microseconds (keys 256g, 512g, 256s, 512s)."""
     p = physics.c()
-    blocks = (C.c_int32 * 2)()
-    us = (C.c_double * 6)()
+    choice = (C.c_int32 * 4)()
+    us = (C.c_double * 8)()
     ctx = particle_system.parallel.ctx
     _lib.check(_lib.lib().eppx_autotune(particle_system.handle, neighbour_system.handle, C.byref(p), int(trials),
-                                        blocks, us), ctx, "autotune")
-    return {"search_block": blocks[0], "momentum_block": blocks[1],
-            "search_us": dict(zip((128, 256, 512), us[0:3])), "momentum_us": dict(zip((128, 256, 512), us[3:6]))}
+                                        choice, us), ctx, "autotune")
+    return _shape_dict(choice, us)
+
+
+def set_pair_shape(particle_system: ParticleSystem, search=(256, 0), momentum=(256, 0)) -> None:
+    """Pins the pair-kernel shapes ((block, staged) each); results do not depend on them."""
+    choice = (C.c_int32 * 4)(search[0], int(search[1]), momentum[0], int(momentum[1]))
+    _lib.check(_lib.lib().eppx_set_pair_shape(particle_system.handle, choice), particle_system.parallel.ctx,
+               "set_pair_shape")

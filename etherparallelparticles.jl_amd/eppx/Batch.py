@@ -100,13 +100,13 @@ class Batch:
         self.hosts.pop(case_id, None)
 
     def autotune(self, trials: int = 3) -> dict:
-        """Pair-block sizes for the current cases (see Algorithm.autotune)."""
-        blocks = (C.c_int32 * 2)()
-        us = (C.c_double * 6)()
-        _lib.check(_lib.lib().eppx_batch_autotune(self.handle, int(trials), blocks, us), self.parallel.ctx,
+        """Pair-kernel shapes for the current cases (see Algorithm.autotune)."""
+        from .Algorithm import _shape_dict
+        choice = (C.c_int32 * 4)()
+        us = (C.c_double * 8)()
+        _lib.check(_lib.lib().eppx_batch_autotune(self.handle, int(trials), choice, us), self.parallel.ctx,
                    "Batch.autotune")
-        return {"search_block": blocks[0], "momentum_block": blocks[1],
-                "search_us": dict(zip((128, 256, 512), us[0:3])), "momentum_us": dict(zip((128, 256, 512), us[3:6]))}
+        return _shape_dict(choice, us)
 
     def info(self):
         slots, cases = C.c_int32(), C.c_int32()

@@ -120,7 +120,7 @@ def lib() -> C.CDLL:
             "eppx_batch_info": ([vp, P(i32), P(i32)], i32),
             "eppx_batch_pair_total": ([vp, P(i64), i32], i32),
             "eppx_autotune": ([vp, vp, P(Physics), i32, P(i32), P(C.c_double)], i32),
-            "eppx_set_pair_blocks": ([vp, i32, i32], i32),
+            "eppx_set_pair_shape": ([vp, P(i32)], i32),
             "eppx_batch_autotune": ([vp, i32, P(i32), P(C.c_double)], i32),
         }
         for name, (args, res) in sig.items():

@@ -205,18 +205,20 @@ int32_t eppx_step(eppx_ps *ps, eppx_ns *ns, const eppx_physics *phys, int32_t n_
 
 /* Autotuner (no counterpart in the reference, whose kernels take the
  * backend's default work-group size): times the FAST search action and the
- * momentum pass on the current state for pair blocks of 128, 256 and 512
- * threads (trials each, after one warm-up), keeps the fastest for each
- * kernel, and leaves the state as it was.  blocks[2] receives the choice
- * (search, momentum); us[6], when given, the average microseconds (search
- * then momentum, by block size).  EXACT math keeps 256.  Needs a prior
- * eppx_search on ns (waits). */
-int32_t eppx_autotune(eppx_ps *ps, eppx_ns *ns, const eppx_physics *phys, int32_t trials, int32_t *blocks,
+ * momentum pass on the current state for each pair-kernel shape (256 or 512
+ * threads per block; per-neighbour data gathered from HBM or staged in the
+ * block's LDS tile), trials each after one warm-up, keeps the fastest for
+ * each kernel, and leaves the state as it was.  choice[4] receives (search
+ * block, search staged, momentum block, momentum staged); us[8], when given,
+ * the average microseconds (search then momentum; shapes (256, gathered),
+ * (512, gathered), (256, staged), (512, staged)).  EXACT math keeps
+ * (256, gathered).  Needs a prior eppx_search on ns (waits). */
+int32_t eppx_autotune(eppx_ps *ps, eppx_ns *ns, const eppx_physics *phys, int32_t trials, int32_t *choice,
                       double *us);
 
-/* Sets the pair-block sizes by hand (128, 256 or 512 threads each; FAST
- * math; results do not depend on them). */
-int32_t eppx_set_pair_blocks(eppx_ps *ps, int32_t search_block, int32_t momentum_block);
+/* Sets the pair-kernel shapes by hand (choice as eppx_autotune returns it;
+ * FAST math; results do not depend on them). */
+int32_t eppx_set_pair_shape(eppx_ps *ps, const int32_t *choice);
 
 /* Σ nCount over alive particles after the last search on ns (waits). */
 int32_t eppx_pair_count(eppx_ps *ps, eppx_ns *ns, int64_t *out);
@@ -276,7 +278,7 @@ int32_t eppx_batch_download(eppx_batch *b, int32_t case_id, int32_t *n_particles
 int32_t eppx_batch_remove(eppx_batch *b, int32_t case_id);
 int32_t eppx_batch_info(eppx_batch *b, int32_t *slots_in_use, int32_t *cases_in_use);
 /* eppx_autotune for the batch's current cases (waits). */
-int32_t eppx_batch_autotune(eppx_batch *b, int32_t trials, int32_t *blocks, double *us);
+int32_t eppx_batch_autotune(eppx_batch *b, int32_t trials, int32_t *choice, double *us);
 /* Running Σ nCount over the batch's searches since the last reset (waits). */
 int32_t eppx_batch_pair_total(eppx_batch *b, int64_t *out, int32_t reset);
 

@@ -1,4 +1,5 @@
-"""Pair-block sizes (the autotuner's choices) change no result, and the
+"""Pair-kernel shapes (the autotuner's choices: threads per block, and
+per-neighbour data gathered or staged in LDS) change no result, and the
 autotuner leaves the state as it found it."""
 import ctypes as C
 
@@ -10,11 +11,10 @@ import parity_util as U
 pytestmark = pytest.mark.gpu
 
 
-def _run(parallel, case, blocks, steps):
-    from eppx import Algorithm, SPH, _lib
+def _run(parallel, case, shape, steps):
+    from eppx import Algorithm, SPH
     ps, ns = U.device(parallel, case, 64)
-    if blocks is not None:
-        _lib.check(_lib.lib().eppx_set_pair_blocks(ps.handle, *blocks), parallel.ctx, "blocks")
+    Algorithm.set_pair_shape(ps, *shape)
     Algorithm.search(ps, case.domain, ns, criterion=Algorithm.symmetrySearchCriterion,
                      action=SPH.sComputeKernelAndiContinuity(case.physics))
     Algorithm.step(ps, ns, case.physics, n_steps=steps)
@@ -24,13 +24,14 @@ def _run(parallel, case, blocks, steps):
     return host
 
 
-def test_block_sizes_give_identical_results(parallel):
-    """FAST math, 8 steps of the dx = 0.01 dam break with every block size."""
-    from eppx import cases
+def test_shapes_give_identical_results(parallel):
+    """FAST math, 8 steps of the dx = 0.01 dam break with every shape."""
+    from eppx import Algorithm, cases
     case = cases.dam_break(0.01, 2.0, M=64)
     parallel.set_math("fast")
     try:
-        runs = [_run(parallel, case, b, 8) for b in ((256, 256), (128, 512), (512, 128))]
+        shapes = [((256, 0), (256, 0))] + [(a, b) for a, b in zip(Algorithm.SHAPES[1:], Algorithm.SHAPES[::-1])]
+        runs = [_run(parallel, case, sh, 8) for sh in shapes]
     finally:
         parallel.set_math("exact")
     for other in runs[1:]:
@@ -53,8 +54,9 @@ def test_autotune_leaves_state_unchanged(parallel):
         from eppx import _lib
         _lib.check(_lib.lib().eppx_pair_total(ps.handle, C.byref(total0), 0), parallel.ctx, "total")
         choice = Algorithm.autotune(ps, ns, case.physics, trials=2)
-        assert choice["search_block"] in (128, 256, 512) and choice["momentum_block"] in (128, 256, 512)
+        assert choice["search"]["block"] in (256, 512) and choice["momentum"]["block"] in (256, 512)
         assert all(v > 0 for v in choice["search_us"].values())
+        assert all(v > 0 for v in choice["momentum_us"].values())
         after = U.download(ps)
         total1 = C.c_int64()
         _lib.check(_lib.lib().eppx_pair_total(ps.handle, C.byref(total1), 0), parallel.ctx, "total")
