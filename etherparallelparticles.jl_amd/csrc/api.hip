@@ -365,18 +365,23 @@ int32_t field_arg(eppx_ctx *ctx, const eppx_ps *ps, const void *params, size_t s
 }
 
 // clean! + insert + cell-order move + find + fused action.
-int32_t do_search(eppx_ps *ps, eppx_ns *ns, int32_t criterion, int32_t action, const Physics &ph) {
+// keys_done: the cell counts, keys and provisional slots of this search were
+// already taken (k_move_key in a step).
+int32_t do_search(eppx_ps *ps, eppx_ns *ns, int32_t criterion, int32_t action, const Physics &ph,
+                  bool keys_done = false) {
     eppx_ctx *ctx = ps->ctx;
     int32_t rc = ensure_cells(ps, ns->grid.n_cells);
     if (rc) return rc;
     const int32_t nc = ns->grid.n_cells;
     hipStream_t s = ctx->stream;
     Fields cur = ps->buf[ps->cur], nxt = ps->buf[ps->cur ^ 1];
-    HIP_OK(ctx, timed(ctx, "cell_reset", [&] { return launch_reset_search(ps->counts, nc + 2, ps->pairs, s); }));
-    HIP_OK(ctx, timed(ctx, "cell_key", [&] {
-               return launch_key_count(cur, ps->n, case_of(ps, ns, criterion), nullptr, nc, ps->counts, ps->key,
-                                       ps->tmpslot, ctx->d_flags, ns->M, s);
-           }));
+    if (!keys_done) {
+        HIP_OK(ctx, timed(ctx, "cell_reset", [&] { return launch_reset_search(ps->counts, nc + 2, ps->pairs, s); }));
+        HIP_OK(ctx, timed(ctx, "cell_key", [&] {
+                   return launch_key_count(cur, ps->n, case_of(ps, ns, criterion), nullptr, nc, ps->counts, ps->key,
+                                           ps->tmpslot, ctx->d_flags, ns->M, s);
+               }));
+    }
     HIP_OK(ctx, timed(ctx, "cell_scan", [&] { return launch_scan(ps->counts, nc + 2, ps->start, ps->tile_sums, s); }));
     HIP_OK(ctx, timed(ctx, "cell_members", [&] {
                return launch_fill_members(cur, ps->n, nc, ps->key, ps->tmpslot, ps->start, ps->members, s);
@@ -1046,12 +1051,21 @@ int32_t eppx_step(eppx_ps *ps, eppx_ns *ns, const eppx_physics *phys, int32_t n_
     HIP_OK(ctx, hipSetDevice(ctx->device));
     if ((rc = ensure_cells(ps, ns->grid.n_cells))) return rc;  // no allocation while a step is recorded
     const eppx_field_op op{};
+    // sContinuity! (with the coming search's counts zeroed), iMomentum!,
+    // sMomentum! (with the coming search's cell keys), then the search
+    const int32_t nc = ns->grid.n_cells;
+    hipStream_t s = ctx->stream;
     auto enqueue_step = [&](bool filter_now) -> int32_t {
         int32_t r;
-        if ((r = do_self(ps, EPPX_SELF_CONTINUITY_EOS, ph, op))) return r;
+        HIP_OK(ctx, timed(ctx, "continuity_eos", [&] {
+                   return launch_eos_reset(ps->f(), ps->n, ph, nullptr, ps->counts, nc + 2, &ps->pairs[0], s);
+               }));
         if ((r = do_interaction(ps, ns, EPPX_INTER_MOMENTUM, ph, op))) return r;
-        if ((r = do_self(ps, EPPX_SELF_GRAVITY_MOVE, ph, op))) return r;
-        if ((r = do_search(ps, ns, EPPX_CRIT_SYMMETRY, EPPX_SEARCH_KERNEL_CONTINUITY, ph))) return r;
+        HIP_OK(ctx, timed(ctx, "gravity_move", [&] {
+                   return launch_move_key(ps->f(), ps->n, ph, case_of(ps, ns, EPPX_CRIT_SYMMETRY), nullptr, nc,
+                                          ps->counts, ps->key, ps->tmpslot, ctx->d_flags, ns->M, s);
+               }));
+        if ((r = do_search(ps, ns, EPPX_CRIT_SYMMETRY, EPPX_SEARCH_KERNEL_CONTINUITY, ph, true))) return r;
         if (filter_now) {
             if ((r = do_interaction(ps, ns, EPPX_INTER_FILTER, ph, op))) return r;
             if ((r = do_self(ps, EPPX_SELF_FILTER, ph, op))) return r;
@@ -1338,17 +1352,19 @@ Walk batch_walk(const eppx_batch *b) {
 // The segmented cell build and (search) the fused kernel + continuity action
 // for the cases `which` selects (0: with steps left, 1: waiting for a first
 // search).
-int32_t batch_cell_build(eppx_batch *b, bool search, int32_t which) {
+int32_t batch_cell_build(eppx_batch *b, bool search, int32_t which, bool keys_done = false) {
     eppx_ps *ps = b->ps;
     eppx_ctx *ctx = b->ctx;
     hipStream_t s = ctx->stream;
     const int32_t nc = b->total_cells;
     Fields cur = ps->buf[ps->cur], nxt = ps->buf[ps->cur ^ 1];
-    HIP_OK(ctx, timed(ctx, "cell_reset", [&] { return launch_reset_search(ps->counts, nc + 2, ps->pairs, s); }));
-    HIP_OK(ctx, timed(ctx, "cell_key", [&] {
-               return launch_key_count(cur, ps->n, CaseDesc{}, b->d_cases, nc, ps->counts, ps->key, ps->tmpslot,
-                                       ctx->d_flags, b->ns.M, s);
-           }));
+    if (!keys_done) {
+        HIP_OK(ctx, timed(ctx, "cell_reset", [&] { return launch_reset_search(ps->counts, nc + 2, ps->pairs, s); }));
+        HIP_OK(ctx, timed(ctx, "cell_key", [&] {
+                   return launch_key_count(cur, ps->n, CaseDesc{}, b->d_cases, nc, ps->counts, ps->key, ps->tmpslot,
+                                           ctx->d_flags, b->ns.M, s);
+               }));
+    }
     HIP_OK(ctx, timed(ctx, "cell_scan", [&] { return launch_scan(ps->counts, nc + 2, ps->start, ps->tile_sums, s); }));
     HIP_OK(ctx, timed(ctx, "cell_members", [&] {
                return launch_fill_members(cur, ps->n, nc, ps->key, ps->tmpslot, ps->start, ps->members, s);
@@ -1382,7 +1398,8 @@ int32_t batch_enqueue_step(eppx_batch *b) {
     const eppx_field_op op{};
     const int fast = ctx->math_mode == EPPX_MATH_FAST;
     HIP_OK(ctx, timed(ctx, "continuity_eos", [&] {
-               return launch_self(ps->f(), ps->n, EPPX_SELF_CONTINUITY_EOS, b->ph, op, b->d_cases, s);
+               return launch_eos_reset(ps->f(), ps->n, b->ph, b->d_cases, ps->counts, b->total_cells + 2, &ps->pairs[0],
+                                       s);
            }));
     Walk w = batch_walk(b);
     w.mask_mode = ps->masks_valid ? 2 : 0;
@@ -1391,9 +1408,10 @@ int32_t batch_enqueue_step(eppx_batch *b) {
                                          ps->momentum_stage, s);
            }));
     HIP_OK(ctx, timed(ctx, "gravity_move", [&] {
-               return launch_self(ps->f(), ps->n, EPPX_SELF_GRAVITY_MOVE, b->ph, op, b->d_cases, s);
+               return launch_move_key(ps->f(), ps->n, b->ph, CaseDesc{}, b->d_cases, b->total_cells, ps->counts,
+                                      ps->key, ps->tmpslot, ctx->d_flags, b->ns.M, s);
            }));
-    int32_t rc = batch_cell_build(b, true, 0);
+    int32_t rc = batch_cell_build(b, true, 0, true);
     if (rc) return rc;
     HIP_OK(ctx, launch_tick(b->d_cases, b->max_cases, s));
     return EPPX_OK;

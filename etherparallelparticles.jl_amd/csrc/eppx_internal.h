@@ -134,6 +134,13 @@ hipError_t launch_snapshot_mass(float4 *geo, Fields f, int32_t n, hipStream_t s)
 hipError_t launch_self(Fields f, int32_t n, int32_t action, const Physics &ph, const eppx_field_op &op,
                        const CaseDesc *cases, hipStream_t s);
 hipError_t launch_tick(CaseDesc *cases, int32_t n_cases, hipStream_t s);
+// Fused kernels of the time step: continuity + EOS with the cell counts of the
+// coming search zeroed; gravity + move with that search's cell key and count.
+hipError_t launch_eos_reset(Fields f, int32_t n, const Physics &ph, const CaseDesc *cases, int32_t *counts,
+                            int32_t n_counts, unsigned long long *pairs_last, hipStream_t s);
+hipError_t launch_move_key(Fields f, int32_t n, const Physics &ph, const CaseDesc &one, const CaseDesc *cases,
+                           int32_t n_cells, int32_t *counts, int32_t *key, int32_t *tmpslot, uint32_t *flags,
+                           int32_t max_per_cell, hipStream_t s);
 hipError_t launch_set_case(CaseDesc *d, const CaseDesc &v, hipStream_t s);
 hipError_t launch_interaction(const Walk &w, Fields f, int32_t n, int32_t action, const Physics &ph,
                               const eppx_field_op &op, int32_t fast, int32_t block, int32_t stage, hipStream_t s);

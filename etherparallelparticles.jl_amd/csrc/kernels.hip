@@ -487,39 +487,38 @@ __global__ void k_reset_search(int32_t *counts, int32_t n, unsigned long long *p
 // key is the 0-based cell, n_cells for dead particles.  Counts are taken with
 // one atomic per distinct key in the wave (the slots of a sorted wave share
 // few cells), which also yields each particle's provisional slot.
-__global__ __launch_bounds__(kBlock) void k_key_count(Fields f, int32_t n, CaseDesc one, const CaseDesc *cases,
-                                                       int32_t n_cells, int32_t *counts, int32_t *key,
-                                                       int32_t *tmpslot, uint32_t *flags, int32_t max_per_cell) {
-    const int32_t s = blockIdx.x * kBlock + threadIdx.x;
-    const bool valid = s < n;
-    int32_t k = -1;
-    if (valid) {
-        const CaseDesc cd = cases ? cases[f.cs[s]] : one;
-        const Grid &g = cd.g;
-        k = cd.removed ? n_cells + 1 : n_cells;
-        if (!cd.removed && f.alive[s] == 1) {
-            int32_t c = f.cell[s];
-            // a cached cell outside this grid (caller data, or another domain) is recomputed
-            if (c < 0 || c > g.n_cells) c = -1;
-            if (!(f.mov[s] == 0 && c > 0)) {
-                const float2 x = f.pos[s];
-                if (g.first_x <= x.x && x.x <= g.last_x && g.first_y <= x.y && x.y <= g.last_y) {
-                    int32_t i = (int32_t)floorf((x.x - g.first_x) * g.gap_x_inv) + 1;
-                    i = max(1, min(g.n_x, i));
-                    int32_t j = (int32_t)floorf((x.y - g.first_y) * g.gap_y_inv) + 1;
-                    j = max(1, min(g.n_y, j));
-                    c = i + g.n_x * (j - 1);
-                    f.cell[s] = c;
-                } else {
-                    f.alive[s] = 0;
-                    f.cell[s] = 0;
-                    c = 0;
-                }
+// Cell key of slot s (updates its cell, and kills it outside its domain).
+__device__ __forceinline__ int32_t cell_key(Fields &f, int32_t s, const CaseDesc &cd, int32_t n_cells) {
+    const Grid &g = cd.g;
+    int32_t k = cd.removed ? n_cells + 1 : n_cells;
+    if (!cd.removed && f.alive[s] == 1) {
+        int32_t c = f.cell[s];
+        // a cached cell outside this grid (caller data, or another domain) is recomputed
+        if (c < 0 || c > g.n_cells) c = -1;
+        if (!(f.mov[s] == 0 && c > 0)) {
+            const float2 x = f.pos[s];
+            if (g.first_x <= x.x && x.x <= g.last_x && g.first_y <= x.y && x.y <= g.last_y) {
+                int32_t i = (int32_t)floorf((x.x - g.first_x) * g.gap_x_inv) + 1;
+                i = max(1, min(g.n_x, i));
+                int32_t j = (int32_t)floorf((x.y - g.first_y) * g.gap_y_inv) + 1;
+                j = max(1, min(g.n_y, j));
+                c = i + g.n_x * (j - 1);
+                f.cell[s] = c;
+            } else {
+                f.alive[s] = 0;
+                f.cell[s] = 0;
+                c = 0;
             }
-            if (c > 0) k = cd.cell_base + c - 1;
         }
-        key[s] = k;
+        if (c > 0) k = cd.cell_base + c - 1;
     }
+    return k;
+}
+
+// Per-key counts with one atomic per distinct key in the wave; returns the
+// lane's provisional slot within its key.  Every lane of the wave calls it.
+__device__ __forceinline__ int32_t count_key(bool valid, int32_t k, int32_t n_cells, int32_t *counts,
+                                             uint32_t *flags, int32_t max_per_cell) {
     unsigned long long active = __ballot(valid);
     int32_t slot = 0;
     while (active) {
@@ -536,6 +535,20 @@ __global__ __launch_bounds__(kBlock) void k_key_count(Fields f, int32_t n, CaseD
         if (valid && k == lk) slot = base + __popcll(same & ((1ull << lane_id()) - 1ull));
         active &= ~same;
     }
+    return slot;
+}
+
+__global__ __launch_bounds__(kBlock) void k_key_count(Fields f, int32_t n, CaseDesc one, const CaseDesc *cases,
+                                                       int32_t n_cells, int32_t *counts, int32_t *key,
+                                                       int32_t *tmpslot, uint32_t *flags, int32_t max_per_cell) {
+    const int32_t s = blockIdx.x * kBlock + threadIdx.x;
+    const bool valid = s < n;
+    int32_t k = -1;
+    if (valid) {
+        k = cell_key(f, s, cases ? cases[f.cs[s]] : one, n_cells);
+        key[s] = k;
+    }
+    const int32_t slot = count_key(valid, k, n_cells, counts, flags, max_per_cell);
     if (valid) tmpslot[s] = slot;
 }
 
@@ -937,6 +950,36 @@ __global__ __launch_bounds__(kBlock) void k_interaction(Walk w, Fields f, Physic
 //    sAccelerateMove! Motion.jl:61-75;
 //  EPPX_SELF_FILTER: sFilter main.jl:136-144 -> sKernelFilter! Filter.jl:24-38;
 //  EPPX_SELF_SCALE_FIELD: field[I] *= value.
+// sContinuity! (main.jl:92-105): rho += drho dt, drho = 0, V = m / rho, p = eos(rho).
+__device__ __forceinline__ void continuity_eos_one(Fields &f, int s, int tag, const Physics &ph) {
+    if (tag != ph.fluid_tag && tag != ph.wall_tag) return;
+    float4 d = f.dyn[s];
+    const float rho = d.z + f.drho[s] * ph.dt;
+    f.drho[s] = 0.0f;
+    f.vol[s] = f.mass[s] / rho;
+    d.z = rho;
+    d.w = eos(ph, rho);
+    f.dyn[s] = d;
+}
+
+// sMomentum! (main.jl:120-126): du += g, then x += (u + du dt / 2) dt and
+// u += du dt per component, du = 0.
+__device__ __forceinline__ void gravity_move_one(Fields &f, int s, int tag, const Physics &ph) {
+    if (tag != ph.fluid_tag) return;
+    float2 du = f.dvel[s];
+    du.x += ph.gx;
+    du.y += ph.gy;
+    float2 x = f.pos[s];
+    float4 d = f.dyn[s];
+    x.x += (d.x + du.x * ph.dt * 0.5f) * ph.dt;
+    d.x += du.x * ph.dt;
+    x.y += (d.y + du.y * ph.dt * 0.5f) * ph.dt;
+    d.y += du.y * ph.dt;
+    f.pos[s] = x;
+    f.dyn[s] = d;
+    f.dvel[s] = make_float2(0.0f, 0.0f);
+}
+
 template <int ACTION>
 __global__ __launch_bounds__(kBlock) void k_self(Fields f, int32_t n, Physics ph, eppx_field_op op,
                                                  const CaseDesc *cases) {
@@ -949,28 +992,9 @@ __global__ __launch_bounds__(kBlock) void k_self(Fields f, int32_t n, Physics ph
     }
     const int tag = f.tag[s];
     if (ACTION == EPPX_SELF_CONTINUITY_EOS) {
-        if (tag != ph.fluid_tag && tag != ph.wall_tag) return;
-        float4 d = f.dyn[s];
-        const float rho = d.z + f.drho[s] * ph.dt;
-        f.drho[s] = 0.0f;
-        f.vol[s] = f.mass[s] / rho;
-        d.z = rho;
-        d.w = eos(ph, rho);
-        f.dyn[s] = d;
+        continuity_eos_one(f, s, tag, ph);
     } else if (ACTION == EPPX_SELF_GRAVITY_MOVE) {
-        if (tag != ph.fluid_tag) return;
-        float2 du = f.dvel[s];
-        du.x += ph.gx;
-        du.y += ph.gy;
-        float2 x = f.pos[s];
-        float4 d = f.dyn[s];
-        x.x += (d.x + du.x * ph.dt * 0.5f) * ph.dt;
-        d.x += du.x * ph.dt;
-        x.y += (d.y + du.y * ph.dt * 0.5f) * ph.dt;
-        d.y += du.y * ph.dt;
-        f.pos[s] = x;
-        f.dyn[s] = d;
-        f.dvel[s] = make_float2(0.0f, 0.0f);
+        gravity_move_one(f, s, tag, ph);
     } else if (ACTION == EPPX_SELF_FILTER) {
         if (tag != ph.fluid_tag) return;
         const float hinv = 1.0f / f.h[s];
@@ -996,6 +1020,47 @@ __global__ __launch_bounds__(kBlock) void k_self(Fields f, int32_t n, Physics ph
 __global__ __launch_bounds__(kBlock) void k_snapshot_mass(float4 *geo, Fields f, int32_t n) {
     const int s = blockIdx.x * kBlock + threadIdx.x;
     if (s < n) geo[s].w = f.mass[s];
+}
+
+// The step's first pass: sContinuity! for every slot, and the cell counts
+// and last-search pair counter of the coming cell build zeroed (slots
+// s < n_counts).  Launched over max(n, n_counts) threads.
+__global__ __launch_bounds__(kBlock) void k_eos_reset(Fields f, int32_t n, Physics ph, const CaseDesc *cases,
+                                                       int32_t *counts, int32_t n_counts,
+                                                       unsigned long long *pairs_last) {
+    const int s = blockIdx.x * kBlock + threadIdx.x;
+    if (s < n_counts) counts[s] = 0;
+    if (s == 0) *pairs_last = 0ull;
+    if (s >= n || f.alive[s] != 1) return;
+    if (cases) {
+        const CaseDesc &cd = cases[f.cs[s]];
+        if (cd.steps_left <= 0) return;
+        ph.dt = cd.dt;
+    }
+    continuity_eos_one(f, s, f.tag[s], ph);
+}
+
+// sMomentum! fused with the cell key and count of the search that follows:
+// the key is taken from the position just written.
+__global__ __launch_bounds__(kBlock) void k_move_key(Fields f, int32_t n, Physics ph, CaseDesc one,
+                                                      const CaseDesc *cases, int32_t n_cells, int32_t *counts,
+                                                      int32_t *key, int32_t *tmpslot, uint32_t *flags,
+                                                      int32_t max_per_cell) {
+    const int32_t s = blockIdx.x * kBlock + threadIdx.x;
+    const bool valid = s < n;
+    int32_t k = -1;
+    if (valid) {
+        const CaseDesc cd = cases ? cases[f.cs[s]] : one;
+        if (f.alive[s] == 1 && cd.steps_left > 0) {
+            Physics p = ph;
+            if (cases) p.dt = cd.dt;
+            gravity_move_one(f, s, f.tag[s], p);
+        }
+        k = cell_key(f, s, cd, n_cells);
+        key[s] = k;
+    }
+    const int32_t slot = count_key(valid, k, n_cells, counts, flags, max_per_cell);
+    if (valid) tmpslot[s] = slot;
 }
 
 // Writes one case descriptor (a kernel, so that a recorded step may carry it).
@@ -1382,6 +1447,21 @@ hipError_t launch_from_ref(Fields f, int32_t offset, Columns c, const int32_t *i
 hipError_t launch_to_ref(Fields f, int32_t n, Columns c, int32_t *ip, float *fp, int32_t *is_alive,
                          int32_t *cell_index, int32_t cs, hipStream_t s) {
     hipLaunchKernelGGL(k_to_ref, grid_for(n), dim3(kBlock), 0, s, f, n, c, ip, fp, is_alive, cell_index, cs);
+    return hipGetLastError();
+}
+
+hipError_t launch_eos_reset(Fields f, int32_t n, const Physics &ph, const CaseDesc *cases, int32_t *counts,
+                            int32_t n_counts, unsigned long long *pairs_last, hipStream_t s) {
+    hipLaunchKernelGGL(k_eos_reset, grid_for(std::max(n, n_counts)), dim3(kBlock), 0, s, f, n, ph, cases, counts,
+                       n_counts, pairs_last);
+    return hipGetLastError();
+}
+
+hipError_t launch_move_key(Fields f, int32_t n, const Physics &ph, const CaseDesc &one, const CaseDesc *cases,
+                           int32_t n_cells, int32_t *counts, int32_t *key, int32_t *tmpslot, uint32_t *flags,
+                           int32_t max_per_cell, hipStream_t s) {
+    hipLaunchKernelGGL(k_move_key, grid_for(n), dim3(kBlock), 0, s, f, n, ph, one, cases, n_cells, counts, key,
+                       tmpslot, flags, max_per_cell);
     return hipGetLastError();
 }
 
