@@ -76,6 +76,7 @@ struct eppx_ps {
     uint32_t *fallback = nullptr;         // pair blocks that walked global memory
     float *h_range = nullptr;
     float rmin2 = -1.0f, rmax2 = INFINITY;  // symmetry-criterion bounds from the H range
+    float hinv_u = 0.0f;                    // one H for every alive particle: its pair hinv; else 0
     uint32_t tags_present = 0xffffffffu;    // tag bits of the uploaded particles
     bool odd_tags = true;                   // some tag outside 1..kMaxTag
     // search state
@@ -268,6 +269,15 @@ void symmetry_bounds(bool valid, float hmin_f, float hmax_f, float *rmin2, float
     }
 }
 
+// The pair hinv (H + H) / (2 H H) when every alive particle has one valid H
+// (the FAST pair terms then hoist it), else 0.
+float uniform_hinv(bool valid, float hmin, float hmax) {
+    if (!valid || !(hmin == hmax) || !(hmin >= 1e-30f) || !(hmin <= 1e30f)) return 0.0f;
+    const float h = hmin;
+    const float hinv = (h + h) / (2.0f * h * h);
+    return std::isfinite(hinv) && hinv > 0.0f ? hinv : 0.0f;
+}
+
 // Criterion bounds: the criterion accepts every dr2 < rmin2 and rejects
 // every dr2 > rmax2.  defaultSearchCriterion: dr2 <= gap_square, i.e.
 // dr2 < the next float up, with an empty band; symmetry: the bounds the H
@@ -291,6 +301,7 @@ CaseDesc case_of(const eppx_ps *ps, const eppx_ns *ns, int32_t criterion) {
     criterion_bounds(criterion, ns->grid, ps->rmin2, ps->rmax2, &c.rmin2, &c.rmax2);
     c.tags_present = ps->tags_present;
     c.check_all = (c.rmin2 <= c.rmax2) || ps->odd_tags;
+    c.hinv_u = ps->hinv_u;
     c.steps_left = 1;
     return c;
 }
@@ -781,6 +792,7 @@ int32_t eppx_ps_upload(eppx_ps *ps, int32_t n_particles, const int32_t *is_alive
     memcpy(&odd, &range[4], sizeof(odd));
     ps->odd_tags = odd != 0;
     symmetry_bounds(!bad && has(L.H), range[0], range[1], &ps->rmin2, &ps->rmax2);
+    ps->hinv_u = uniform_hinv(!bad && has(L.H), range[0], range[1]);
     // keep the columns the kernels do not own
     ps->pass_int.assign(ps->pass_int_cols.size() * cap, 0);
     ps->pass_float.assign(ps->pass_float_cols.size() * cap, 0.0f);
@@ -1532,6 +1544,7 @@ int32_t eppx_batch_add(eppx_batch *b, const eppx_domain2d *domain, const eppx_ph
     symmetry_bounds(!bad, hmin, hmax, &cd.rmin2, &cd.rmax2);
     cd.tags_present = tags;
     cd.check_all = (cd.rmin2 <= cd.rmax2) || odd;
+    cd.hinv_u = uniform_hinv(!bad, hmin, hmax);
     cd.dt = ph.dt;
     cd.steps_left = steps;
     cd.fresh = 1;

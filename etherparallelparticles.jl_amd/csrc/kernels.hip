@@ -89,6 +89,34 @@ __device__ __forceinline__ float grad_fast(float r, float hinv) {
     return (-kSigmaWendland * 0.625f) * h3 * q * (t * t * t);
 }
 
+// dw / r of the same kernel, for the FAST pair terms (rinv = 1 / r,
+// h3 = hinv^3).  Wendland C2's dw carries a factor q = r hinv, so dw / r
+// needs no reciprocal: (-sigma 5/8) h3 hinv (2 - q)^3.
+template <int K>
+__device__ __forceinline__ float grad_over_r_fast(float r, float rinv, float hinv, float h3) {
+    const float q = r * hinv;
+    if (K == EPPX_KERNEL_CUBIC) return grad_fast<K>(r, hinv) * rinv;
+    const float t = fmaxf(2.0f - q, 0.0f);
+    return ((-kSigmaWendland * 0.625f) * h3 * hinv) * (t * t * t);
+}
+
+// The pair's hinv = (h_i + h_j) / (2 h_i h_j) and hinv^3 for the FAST terms.
+// UNIFORM: every particle of the case has one H, whose hinv the caller has
+// (hinv_u, h3_u, loop invariant).
+template <bool UNIFORM>
+struct PairH {
+    float hinv, h3;
+    __device__ __forceinline__ PairH(float hinv_u, float h3_u, float hi, float hj) {
+        if (UNIFORM) {
+            hinv = hinv_u;
+            h3 = h3_u;
+        } else {
+            hinv = (hi + hj) * frcp(2.0f * hi * hj);
+            h3 = hinv * hinv * hinv;
+        }
+    }
+};
+
 // _value of CubicSpline.jl:17-31 / WendlandC2.jl:16-26 (2D), written order.
 __device__ __forceinline__ float value_exact(int kernel, float r, float hinv) {
     const float q = r * hinv;
@@ -382,19 +410,27 @@ __device__ __forceinline__ int walk_rows(const Walk &w, const CaseDesc &cd, cons
             }
         }
     } else {
-        // Rows without accepted neighbours are dropped first, so the loop
-        // moves to the next row at most once per visited neighbour.
-        unsigned long long n0 = m0, n1 = m1, n2 = m2;
-        int b0 = b[0], b1 = b[1], b2 = b[2];
-        int o0 = src.off(0), o1 = src.off(1), o2 = src.off(2);
-        if (!n1) { n1 = n2; b1 = b2; o1 = o2; n2 = 0ull; }
-        if (!n0) { n0 = n1; b0 = b1; o0 = o1; n1 = n2; b1 = b2; o1 = o2; n2 = 0ull; }
-        while (n0) {
-            const int j = b0 + __builtin_ctzll(n0), jo = o0;
-            n0 &= n0 - 1ull;
-            if (!n0) { n0 = n1; b0 = b1; o0 = o1; n1 = n2; b1 = b2; o1 = o2; n2 = 0ull; }
+        // Rows 0 and 2 (the rows below and above) share one loop: a particle
+        // low in its cell has many neighbours below and few above, so their
+        // sum varies little across a wave.  The middle row follows.
+        unsigned long long na = m0, nb = m2;
+        int ba = b[0], bb = b[2];
+        int oa = src.off(0), ob = src.off(2);
+        if (!na) { na = nb; ba = bb; oa = ob; nb = 0ull; }
+        while (na) {
+            const int j = ba + __builtin_ctzll(na), jo = oa;
+            na &= na - 1ull;
+            if (!na) { na = nb; ba = bb; oa = ob; nb = 0ull; }
             const uint32_t tbj = src.tbit(j + jo);
             visit(j, jo, tbj, fetch(j, j + jo, tbj, src));
+        }
+        unsigned long long nm = m1;
+        const int bm = b[1], om = src.off(1);
+        while (nm) {
+            const int j = bm + __builtin_ctzll(nm);
+            nm &= nm - 1ull;
+            const uint32_t tbj = src.tbit(j + om);
+            visit(j, om, tbj, fetch(j, j + om, tbj, src));
         }
     }
     return cnt;
@@ -484,9 +520,7 @@ __global__ void k_reset_search(int32_t *counts, int32_t n, unsigned long long *p
 // ------------------------------------------------------------- cell list
 // device_insertParticlesIntoCells! (NeighbourSearch.jl:12-47): cell of each
 // alive particle (cached for immovable ones), kill outside the domain.  The
-// key is the 0-based cell, n_cells for dead particles.  Counts are taken with
-// one atomic per distinct key in the wave (the slots of a sorted wave share
-// few cells), which also yields each particle's provisional slot.
+// key is the 0-based cell, n_cells for dead particles.
 // Cell key of slot s (updates its cell, and kills it outside its domain).
 __device__ __forceinline__ int32_t cell_key(Fields &f, int32_t s, const CaseDesc &cd, int32_t n_cells) {
     const Grid &g = cd.g;
@@ -515,27 +549,31 @@ __device__ __forceinline__ int32_t cell_key(Fields &f, int32_t s, const CaseDesc
     return k;
 }
 
-// Per-key counts with one atomic per distinct key in the wave; returns the
-// lane's provisional slot within its key.  Every lane of the wave calls it.
+// Per-key counts: the valid lanes of a wave split into runs of consecutive
+// lanes with equal keys (a sorted wave mostly spans a few cells); the first
+// lane of each run adds the run's length to its key's count, all runs in one
+// atomic instruction, and a lane's provisional slot is its run's base plus
+// its place in the run.  A key that appears in two runs of a wave gets two
+// bases.  Every lane of the wave calls it.
 __device__ __forceinline__ int32_t count_key(bool valid, int32_t k, int32_t n_cells, int32_t *counts,
                                              uint32_t *flags, int32_t max_per_cell) {
-    unsigned long long active = __ballot(valid);
-    int32_t slot = 0;
-    while (active) {
-        const int leader = __ffsll(active) - 1;
-        const int32_t lk = __shfl(k, leader);
-        const unsigned long long same = __ballot(k == lk) & active;
-        int32_t base = 0;
-        if (lane_id() == leader) {
-            const int32_t cnt = __popcll(same);
-            base = atomicAdd(&counts[lk], cnt);
-            if (lk < n_cells && base + cnt > max_per_cell) atomicOr(flags, kFlagCellOverflow);
-        }
-        base = __shfl(base, leader);
-        if (valid && k == lk) slot = base + __popcll(same & ((1ull << lane_id()) - 1ull));
-        active &= ~same;
+    const int lane = lane_id();
+    const int32_t prev = __shfl_up(k, 1);
+    const unsigned long long vmask = __ballot(valid);
+    const bool is_head = valid && (lane == 0 || prev != k || !((vmask >> (lane - 1)) & 1ull));
+    const unsigned long long heads = __ballot(is_head);
+    const unsigned long long upto = (2ull << lane) - 1ull;  // lanes 0..lane
+    // a valid lane that is no head follows a valid lane of its key, so a head lies at or below it
+    const int head = valid ? 63 - __clzll(heads & upto) : lane;
+    int32_t base = 0;
+    if (is_head) {
+        const unsigned long long stops = (heads | ~vmask) & ~upto;  // next head or invalid lane
+        const int32_t cnt = (stops ? __ffsll(stops) - 1 : 64) - lane;
+        base = atomicAdd(&counts[k], cnt);
+        if (k < n_cells && base + cnt > max_per_cell) atomicOr(flags, kFlagCellOverflow);
     }
-    return slot;
+    base = __shfl(base, head);
+    return valid ? base + (lane - head) : 0;
 }
 
 __global__ __launch_bounds__(kBlock) void k_key_count(Fields f, int32_t n, CaseDesc one, const CaseDesc *cases,
@@ -696,6 +734,16 @@ __global__ __launch_bounds__(kBlock) void k_permute(Fields cur, Fields nxt, floa
 // iValueGradient! + iClassicContinuity! / iBalancedContinuity!
 // (Library/Kernel.jl:114-128, Continuity.jl:23-47): one term of drho_i;
 // mj_or_rhoVj is m_j (fluid-fluid) or rho_i * V_j (fluid-wall, wall-fluid).
+template <int K, bool UNIFORM>
+__device__ __forceinline__ float continuity_fast(const PairH<UNIFORM> &ph, float2 ui, float2 uj, float mj_or_rhoVj,
+                                                 float dx, float dy, float dr2) {
+    const float rinv = frsq(dr2);
+    const float r = dr2 * rinv;
+    const float dwr = grad_over_r_fast<K>(r, rinv, ph.hinv, ph.h3);
+    const float vdotx = fmaf(dy, ui.y - uj.y, dx * (ui.x - uj.x));
+    return (mj_or_rhoVj * vdotx) * dwr;
+}
+
 template <bool FAST, int K>
 __device__ __forceinline__ float continuity_term(float hi, float hj, float2 ui, float2 uj, float mj_or_rhoVj,
                                                  float dx, float dy, float dr2) {
@@ -708,12 +756,7 @@ __device__ __forceinline__ float continuity_term(float hi, float hj, float2 ui, 
         vdotx += dy * (ui.y - uj.y);
         return mj_or_rhoVj * vdotx * dw / r;
     }
-    const float rinv = frsq(dr2);
-    const float r = dr2 * rinv;
-    const float hinv = (hi + hj) * frcp(2.0f * hi * hj);
-    const float dw = grad_fast<K>(r, hinv);
-    const float vdotx = fmaf(dy, ui.y - uj.y, dx * (ui.x - uj.x));
-    return (mj_or_rhoVj * vdotx) * (dw * rinv);
+    return continuity_fast<K>(PairH<false>(0.0f, 0.0f, hi, hj), ui, uj, mj_or_rhoVj, dx, dy, dr2);
 }
 
 struct ContinuityGather {
@@ -772,8 +815,25 @@ __global__ __launch_bounds__(BLOCK) void k_search(Walk w, Fields f, Physics ph, 
                 const float m = (i_fluid && tbj == fluid_bit) ? mj : rho_i * p.vol;
                 acc += continuity_term<FAST, K>(gi.z, hj, ui, p.uv, m, dx, dy, dr2);
             };
-            if (tiled) cnt = walk_rows<!FAST, true>(w, cd, rows, i, gi, keep, fetch, term);
-            else cnt = walk_rows<!FAST, true>(w, cd, GlobalRows{w.geo, w.tag}, i, gi, keep, fetch, term);
+            bool walked = false;
+            if constexpr (FAST) {
+                if (tiled && cd.hinv_u > 0.0f) {  // one H in the case: the pair's hinv is loop invariant
+                    const PairH<true> hu(cd.hinv_u, cd.hinv_u * cd.hinv_u * cd.hinv_u, 0.0f, 0.0f);
+                    auto term_u = [&](int, uint32_t tbj, float, float mj, float dx, float dy, float dr2,
+                                      const ContinuityGather &p) {
+                        const float m = (i_fluid && tbj == fluid_bit) ? mj : rho_i * p.vol;
+                        acc += continuity_fast<K>(hu, ui, p.uv, m, dx, dy, dr2);
+                    };
+                    cnt = walk_rows<false, true>(w, cd, rows, i, gi, keep, fetch, term_u);
+                    walked = true;
+                }
+            }
+            if (walked) {
+            } else if (tiled) {
+                cnt = walk_rows<!FAST, true>(w, cd, rows, i, gi, keep, fetch, term);
+            } else {
+                cnt = walk_rows<!FAST, true>(w, cd, GlobalRows{w.geo, w.tag}, i, gi, keep, fetch, term);
+            }
             f.drho[i] = acc;
         } else {
             walk(w, cd, i, gi, [&](int, uint32_t, float, float, float) { ++cnt; });
@@ -792,6 +852,42 @@ __global__ __launch_bounds__(BLOCK) void k_search(Walk w, Fields f, Physics ph, 
 // iBalancedPressure! (fluid-wall) (Pressure.jl:24-60), then
 // iClassicViscosity! or sArtificialViscosity! (Viscosity.jl:10-52), added to
 // the acceleration of fluid particle i.
+// FAST form of the same terms, written with dwr = dw / r:
+//  pressure: prho2 (-m_j dwr) (dx, dy);
+//  classic viscosity: 2 mu m_j r dw / (rho_i rho_j (r^2 + 0.01 h^2)) (du)
+//    = -2 mu q^2 (-m_j dwr) / (rho_i rho_j qa) (du), with qa = q^2 + 0.01
+//    (r^2 + 0.01 h^2 = qa / hinv^2);
+//  artificial viscosity: with v = min(vdotx, 0) hinv, phi = v / qa, and
+//    (beta phi - alpha c) phi (-m_j dwr) / (rho_bar rho_i rho_j) (dx, dy)
+//    = (beta v - alpha c qa) v (-m_j dwr) / (qa^2 rho_bar rho_i rho_j) (dx, dy),
+//    one reciprocal; a separating pair (vdotx > 0) gives v = 0.
+template <int K, int VISC, bool UNIFORM>
+__device__ __forceinline__ void momentum_fast(const Physics &ph, float2 &acc, bool classic, const PairH<UNIFORM> &h,
+                                              float2 ui, float rho_i, float p_i, float pr_i, float2 uj, float rho_j,
+                                              float p_j, float m_j, float dx, float dy, float dr2) {
+    const float rinv = frsq(dr2);
+    const float r = dr2 * rinv;
+    const float mdwr = -m_j * grad_over_r_fast<K>(r, rinv, h.hinv, h.h3);
+    const float rirj = rho_i * rho_j;
+    // classic: p_i / rho_i^2 + p_j / rho_j^2 (pr_i = p_i / rho_i^2); balanced: (p_i + p_j) / (rho_i rho_j)
+    float prho2 = classic ? fmaf(p_j, frcp(rho_j * rho_j), pr_i) : (p_i + p_j) * frcp(rirj);
+    prho2 = fmaf(fabsf(prho2), ph.coefficient, prho2);
+    float radial = prho2 * mdwr;
+    const float q = r * h.hinv;
+    const float qa = fmaf(q, q, 0.01f);
+    if (VISC == EPPX_VISC_CLASSIC) {
+        const float force = (-2.0f * ph.mu) * (q * q) * mdwr * frcp(rirj * qa);
+        acc.x = fmaf(force, ui.x - uj.x, acc.x);
+        acc.y = fmaf(force, ui.y - uj.y, acc.y);
+    } else {
+        const float v = fminf(fmaf(dy, ui.y - uj.y, dx * (ui.x - uj.x)), 0.0f) * h.hinv;
+        const float den = (qa * qa) * ((0.5f * (rho_i + rho_j)) * rirj);
+        radial = fmaf(fmaf(ph.beta, v, -(ph.alpha * ph.av_c) * qa) * v, mdwr * frcp(den), radial);
+    }
+    acc.x = fmaf(radial, dx, acc.x);
+    acc.y = fmaf(radial, dy, acc.y);
+}
+
 template <bool FAST, int K, int VISC>
 __device__ __forceinline__ void momentum_pair(const Physics &ph, float2 &acc, bool classic, float hi, float hj,
                                               float2 ui, float rho_i, float p_i, float pr_i, float2 uj, float rho_j,
@@ -828,34 +924,8 @@ __device__ __forceinline__ void momentum_pair(const Physics &ph, float2 &acc, bo
         }
         return;
     }
-    const float rinv = frsq(dr2);
-    const float r = dr2 * rinv;
-    const float hinv = (hi + hj) * frcp(2.0f * hi * hj);
-    const float dw = grad_fast<K>(r, hinv);
-    const float rirj = rho_i * rho_j;
-    float prho2 = classic ? fmaf(p_j, frcp(rho_j * rho_j), pr_i) : (p_i + p_j) * frcp(rirj);
-    prho2 = fmaf(fabsf(prho2), ph.coefficient, prho2);
-    const float mdw = -m_j * dw;
-    prho2 *= mdw * rinv;
-    acc.x = fmaf(prho2, dx, acc.x);
-    acc.y = fmaf(prho2, dy, acc.y);
-    // With h = 1 / hinv and q = r hinv: r^2 + 0.01 h^2 = (q^2 + 0.01) / hinv^2.
-    const float q = r * hinv;
-    const float qa = fmaf(q, q, 0.01f);
-    if (VISC == EPPX_VISC_CLASSIC) {
-        const float force = (2.0f * ph.mu) * m_j * r * dw * (hinv * hinv) * frcp(rirj * qa);
-        acc.x = fmaf(force, dux, acc.x);
-        acc.y = fmaf(force, duy, acc.y);
-    } else {
-        // phi = vdotx / (hinv (r^2 + 0.01 h^2)) = vdotx hinv / (q^2 + 0.01); a separating
-        // pair (vdotx > 0) contributes nothing, which phi = 0 gives without a branch.
-        const float vdotx = fmaf(dy, duy, dx * dux);
-        const float phi = fminf(vdotx, 0.0f) * hinv * frcp(qa);
-        const float force =
-            fmaf(ph.beta, phi, -ph.alpha * ph.av_c) * phi * (mdw * frcp((0.5f * (rho_i + rho_j)) * rirj * r));
-        acc.x = fmaf(force, dx, acc.x);
-        acc.y = fmaf(force, dy, acc.y);
-    }
+    momentum_fast<K, VISC>(ph, acc, classic, PairH<false>(0.0f, 0.0f, hi, hj), ui, rho_i, p_i, pr_i, uj, rho_j, p_j,
+                           m_j, dx, dy, dr2);
 }
 
 template <bool FAST, int K, int VISC, bool BATCH, int BLOCK, int STAGE>
@@ -890,8 +960,24 @@ __global__ __launch_bounds__(BLOCK) void k_momentum(Walk w, Fields f, Physics ph
                                      make_float2(dj.x, dj.y), dj.z, dj.w, mj, dx, dy, dr2);
     };
     const uint32_t keep = tag_bit(ph.fluid_tag) | tag_bit(ph.wall_tag);
-    if (tiled) walk_rows<!FAST, false>(w, cd, rows, i, gi, keep, fetch, term);
-    else walk_rows<!FAST, false>(w, cd, GlobalRows{w.geo, w.tag}, i, gi, keep, fetch, term);
+    bool walked = false;
+    if constexpr (FAST) {
+        if (tiled && cd.hinv_u > 0.0f) {  // one H in the case: the pair's hinv is loop invariant
+            const PairH<true> hu(cd.hinv_u, cd.hinv_u * cd.hinv_u * cd.hinv_u, 0.0f, 0.0f);
+            auto term_u = [&](int, uint32_t tbj, float, float mj, float dx, float dy, float dr2, const float4 &dj) {
+                momentum_fast<K, VISC>(ph, acc, tbj == fluid_bit, hu, ui, di.z, di.w, pr_i, make_float2(dj.x, dj.y),
+                                       dj.z, dj.w, mj, dx, dy, dr2);
+            };
+            walk_rows<false, false>(w, cd, rows, i, gi, keep, fetch, term_u);
+            walked = true;
+        }
+    }
+    if (walked) {
+    } else if (tiled) {
+        walk_rows<!FAST, false>(w, cd, rows, i, gi, keep, fetch, term);
+    } else {
+        walk_rows<!FAST, false>(w, cd, GlobalRows{w.geo, w.tag}, i, gi, keep, fetch, term);
+    }
     f.dvel[i] = acc;
 }
 
