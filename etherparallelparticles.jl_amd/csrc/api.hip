@@ -673,6 +673,9 @@ int32_t eppx_domain2d_locate(const eppx_domain2d *d, float x, float y, int32_t *
 
 int32_t eppx_ps_create(eppx_ctx *ctx, int32_t n_capacity, const eppx_layout *layout, eppx_ps **out) {
     if (!ctx || !layout || !out || n_capacity < 0) return set_error(ctx, EPPX_E_ARG, "eppx_ps_create: bad argument");
+    // the pair kernels address slot records with 32-bit byte offsets
+    if (n_capacity >= (1 << 28))
+        return set_error(ctx, EPPX_E_ARG, "eppx_ps_create: n_capacity must be below 2^28 (268,435,456)");
     *out = nullptr;
     const eppx_layout &L = *layout;
     if (L.n_int < 0 || L.n_float < 0 || L.max_neighbour_number < 0)

@@ -47,6 +47,14 @@ __device__ __forceinline__ int xcd_block(int b, int g) {
     const int x = b % kXcds, k = b / kXcds, per = g / kXcds, rem = g % kXcds;
     return x * per + min(x, rem) + k;
 }
+// Record k of a slot array, addressed by a 32-bit byte offset from the array
+// base (a system holds fewer than 2^28 slots, so k * 16 fits): the gather
+// then takes the scalar-base + vector-offset form, with no 64-bit address
+// arithmetic per pair.
+template <class T>
+__device__ __forceinline__ T ld_slot(const T *base, int k) {
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + (uint32_t)k * (uint32_t)sizeof(T));
+}
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
 
@@ -278,6 +286,7 @@ struct GlobalRows {
     __device__ __forceinline__ int off(int) const { return 0; }
     __device__ __forceinline__ float4 get(int k) const { return geo[k]; }
     __device__ __forceinline__ uint32_t tbit(int k) const { return tag_bit(tag[k]); }
+    __device__ __forceinline__ uint32_t tbit_adj(int k) const { return tag_bit(tag[k]); }
     __device__ __forceinline__ float4 at(int r, int j) const { return get(j + off(r)); }
 };
 template <int STAGE>
@@ -292,6 +301,8 @@ struct TileRows {
     __device__ __forceinline__ float4 get(int k) const { return tile[k]; }
     // a 64-bit shift: slot index 63 lands outside the low 32 bits, giving 0
     __device__ __forceinline__ uint32_t tbit(int k) const { return (uint32_t)(1ull << tix[k]); }
+    // tag bit of an accepted neighbour: its tag is adjacent, so in 1..kMaxTag
+    __device__ __forceinline__ uint32_t tbit_adj(int k) const { return 1u << tix[k]; }
     __device__ __forceinline__ float4 at(int r, int j) const { return get(j + off(r)); }
 };
 
@@ -405,7 +416,7 @@ __device__ __forceinline__ int walk_rows(const Walk &w, const CaseDesc &cd, cons
             while (m) {
                 const int j = cb + __builtin_ctzll(m);
                 m &= m - 1ull;
-                const uint32_t tbj = src.tbit(j + roff);
+                const uint32_t tbj = src.tbit_adj(j + roff);
                 visit(j, roff, tbj, fetch(j, j + roff, tbj, src));
             }
         }
@@ -421,7 +432,7 @@ __device__ __forceinline__ int walk_rows(const Walk &w, const CaseDesc &cd, cons
             const int j = ba + __builtin_ctzll(na), jo = oa;
             na &= na - 1ull;
             if (!na) { na = nb; ba = bb; oa = ob; nb = 0ull; }
-            const uint32_t tbj = src.tbit(j + jo);
+            const uint32_t tbj = src.tbit_adj(j + jo);
             visit(j, jo, tbj, fetch(j, j + jo, tbj, src));
         }
         unsigned long long nm = m1;
@@ -429,7 +440,7 @@ __device__ __forceinline__ int walk_rows(const Walk &w, const CaseDesc &cd, cons
         while (nm) {
             const int j = bm + __builtin_ctzll(nm);
             nm &= nm - 1ull;
-            const uint32_t tbj = src.tbit(j + om);
+            const uint32_t tbj = src.tbit_adj(j + om);
             visit(j, om, tbj, fetch(j, j + om, tbj, src));
         }
     }
@@ -806,8 +817,8 @@ __global__ __launch_bounds__(BLOCK) void k_search(Walk w, Fields f, Physics ph, 
             auto fetch = [&](int j, int k, uint32_t tbj, const auto &src) {
                 ContinuityGather p;
                 if constexpr (std::decay_t<decltype(src)>::kStage == 2) p.uv = src.uv[k];
-                else p.uv = uv[2 * (size_t)j];
-                p.vol = (i_fluid && tbj == fluid_bit) ? 0.0f : f.vol[j];
+                else p.uv = ld_slot(uv, 2 * j);
+                p.vol = (i_fluid && tbj == fluid_bit) ? 0.0f : ld_slot(f.vol, j);
                 return p;
             };
             auto term = [&](int, uint32_t tbj, float hj, float mj, float dx, float dy, float dr2,
@@ -953,7 +964,7 @@ __global__ __launch_bounds__(BLOCK) void k_momentum(Walk w, Fields f, Physics ph
     const uint32_t fluid_bit = tag_bit(ph.fluid_tag);
     auto fetch = [&](int j, int k, uint32_t, const auto &src) -> float4 {
         if constexpr (std::decay_t<decltype(src)>::kStage == 1) return src.dyn[k];
-        else return f.dyn[j];
+        else return ld_slot(f.dyn, j);
     };
     auto term = [&](int, uint32_t tbj, float hj, float mj, float dx, float dy, float dr2, const float4 &dj) {
         momentum_pair<FAST, K, VISC>(ph, acc, tbj == fluid_bit, gi.z, hj, ui, di.z, di.w, pr_i,
