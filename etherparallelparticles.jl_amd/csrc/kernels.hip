@@ -698,47 +698,62 @@ __global__ __launch_bounds__(kBlock) void k_permute(Fields cur, Fields nxt, floa
         n_alive[1] = start[n_cells + 1];  // alive and dead slots (released ones follow)
     }
     if (s >= n) return;
-    int32_t dest = s;
-    int32_t alive = cur.alive[s];
-    {
-        const int32_t k = key[s];
-        const int32_t b = start[k];
-        if (k < n_cells) {
-            const int32_t e = start[k + 1];
-            if (e - b <= kRankLimit) {
-                const int32_t me = cur.id[s];
-                int32_t r = 0;
-                for (int32_t q = b; q < e; ++q) r += members[q] < me;
-                dest = b + r;
-            } else {
-                dest = b + tmpslot[s];
-                atomicOr(flags, kFlagRankFallback);
-            }
-            alive = 1;
-        } else {
-            dest = b + tmpslot[s];
-            alive = cur.alive[s] == 1 ? 0 : cur.alive[s];
-        }
-    }
+    // Every load of the slot is issued before the first store (the two
+    // buffers could alias as far as the compiler knows, which would otherwise
+    // serialise each field's load and store), so the whole record is in
+    // flight while the rank is taken.
+    const int32_t k = key[s];
+    const int32_t me = cur.id[s];
+    const int32_t alive_in = cur.alive[s];
+    const int32_t tmp = tmpslot[s];
     const float2 x = cur.pos[s];
-    const float h = cur.h[s];
-    const int32_t tag = cur.tag[s];
+    const float4 dyn = cur.dyn[s];
+    const float2 dvel = cur.dvel[s];
+    const float mass = cur.mass[s], vol = cur.vol[s], drho = cur.drho[s], h = cur.h[s];
+    const float wv = cur.wv[s], wvrho = cur.wvrho[s];
+    const int32_t tag = cur.tag[s], mov = cur.mov[s], cell = cur.cell[s];
+    const int32_t cs = batch ? cur.cs[s] : 0;
+    const int32_t b = start[k];
+    int32_t dest, alive;
+    if (k < n_cells) {
+        const int32_t e = start[k + 1];
+        if (e - b <= kRankLimit) {
+            // four independent member loads in flight per iteration
+            int32_t r0 = 0, r1 = 0, r2 = 0, r3 = 0, q = b;
+            for (; q + 4 <= e; q += 4) {
+                const int32_t a0 = members[q], a1 = members[q + 1], a2 = members[q + 2], a3 = members[q + 3];
+                r0 += a0 < me;
+                r1 += a1 < me;
+                r2 += a2 < me;
+                r3 += a3 < me;
+            }
+            for (; q < e; ++q) r0 += members[q] < me;
+            dest = b + r0 + r1 + r2 + r3;
+        } else {
+            dest = b + tmp;
+            atomicOr(flags, kFlagRankFallback);
+        }
+        alive = 1;
+    } else {
+        dest = b + tmp;
+        alive = alive_in == 1 ? 0 : alive_in;
+    }
     nxt.pos[dest] = x;
-    geo[dest] = make_float4(x.x, x.y, h, cur.mass[s]);
-    nxt.dyn[dest] = cur.dyn[s];
-    nxt.dvel[dest] = cur.dvel[s];
-    nxt.mass[dest] = cur.mass[s];
-    nxt.vol[dest] = cur.vol[s];
-    nxt.drho[dest] = cur.drho[s];
+    geo[dest] = make_float4(x.x, x.y, h, mass);
+    nxt.dyn[dest] = dyn;
+    nxt.dvel[dest] = dvel;
+    nxt.mass[dest] = mass;
+    nxt.vol[dest] = vol;
+    nxt.drho[dest] = drho;
     nxt.h[dest] = h;
-    nxt.wv[dest] = cur.wv[s];
-    nxt.wvrho[dest] = cur.wvrho[s];
+    nxt.wv[dest] = wv;
+    nxt.wvrho[dest] = wvrho;
     nxt.tag[dest] = tag;
-    nxt.mov[dest] = cur.mov[s];
-    nxt.id[dest] = cur.id[s];
-    nxt.cell[dest] = cur.cell[s];
+    nxt.mov[dest] = mov;
+    nxt.id[dest] = me;
+    nxt.cell[dest] = cell;
     nxt.alive[dest] = alive;
-    if (batch) nxt.cs[dest] = cur.cs[s];  // a particle system's slots all hold case 0
+    if (batch) nxt.cs[dest] = cs;  // a particle system's slots all hold case 0
 }
 
 // ------------------------------------------------------------ pair physics
