@@ -533,15 +533,16 @@ __global__ void k_reset_search(int32_t *counts, int32_t n, unsigned long long *p
 // alive particle (cached for immovable ones), kill outside the domain.  The
 // key is the 0-based cell, n_cells for dead particles.
 // Cell key of slot s (updates its cell, and kills it outside its domain).
-__device__ __forceinline__ int32_t cell_key(Fields &f, int32_t s, const CaseDesc &cd, int32_t n_cells) {
+// alive, mov, cell and x are the slot's current values, already loaded.
+__device__ __forceinline__ int32_t cell_key_of(Fields &f, int32_t s, const CaseDesc &cd, int32_t n_cells,
+                                               int32_t alive, int32_t mov, int32_t cell, float2 x) {
     const Grid &g = cd.g;
     int32_t k = cd.removed ? n_cells + 1 : n_cells;
-    if (!cd.removed && f.alive[s] == 1) {
-        int32_t c = f.cell[s];
+    if (!cd.removed && alive == 1) {
+        int32_t c = cell;
         // a cached cell outside this grid (caller data, or another domain) is recomputed
         if (c < 0 || c > g.n_cells) c = -1;
-        if (!(f.mov[s] == 0 && c > 0)) {
-            const float2 x = f.pos[s];
+        if (!(mov == 0 && c > 0)) {
             if (g.first_x <= x.x && x.x <= g.last_x && g.first_y <= x.y && x.y <= g.last_y) {
                 int32_t i = (int32_t)floorf((x.x - g.first_x) * g.gap_x_inv) + 1;
                 i = max(1, min(g.n_x, i));
@@ -558,6 +559,9 @@ __device__ __forceinline__ int32_t cell_key(Fields &f, int32_t s, const CaseDesc
         if (c > 0) k = cd.cell_base + c - 1;
     }
     return k;
+}
+__device__ __forceinline__ int32_t cell_key(Fields &f, int32_t s, const CaseDesc &cd, int32_t n_cells) {
+    return cell_key_of(f, s, cd, n_cells, f.alive[s], f.mov[s], f.cell[s], f.pos[s]);
 }
 
 // Per-key counts: the valid lanes of a wave split into runs of consecutive
@@ -601,8 +605,9 @@ __global__ __launch_bounds__(kBlock) void k_key_count(Fields f, int32_t n, CaseD
     if (valid) tmpslot[s] = slot;
 }
 
-// Exclusive scan of cell counts: per-tile sums, one-block scan of the tile
-// sums, per-tile rescan.  No inter-block waiting.
+// Exclusive scan of cell counts: per-tile sums, then a per-tile rescan whose
+// block first adds up the sums of the tiles before it.  No inter-block
+// waiting.
 __global__ __launch_bounds__(kBlock) void k_scan_reduce(const int32_t *in, int32_t n, int32_t *tile_sums) {
     __shared__ int lds[kBlock / 64];
     const int64_t base = (int64_t)blockIdx.x * kScanTile;
@@ -637,26 +642,19 @@ __device__ __forceinline__ int block_inclusive_scan(int v, int *lds) {
     return v;
 }
 
-__global__ __launch_bounds__(1024) void k_scan_top(int32_t *tile_sums, int32_t n_tiles, int32_t *start, int32_t n) {
-    __shared__ int lds[16];
-    __shared__ int chunk_total;
-    int carry = 0;
-    for (int base = 0; base < n_tiles; base += 1024) {
-        const int idx = base + threadIdx.x;
-        const int v = idx < n_tiles ? tile_sums[idx] : 0;
-        const int inc = block_inclusive_scan(v, lds);
-        if (idx < n_tiles) tile_sums[idx] = carry + inc - v;
-        if (threadIdx.x == 1023) chunk_total = inc;
-        __syncthreads();
-        carry += chunk_total;
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) start[n] = carry;
-}
-
-__global__ __launch_bounds__(kBlock) void k_scan_down(const int32_t *in, int32_t n, const int32_t *tile_offsets,
+__global__ __launch_bounds__(kBlock) void k_scan_down(const int32_t *in, int32_t n, const int32_t *tile_sums,
                                                       int32_t *start) {
     __shared__ int lds[kBlock / 64];
+    __shared__ int tile_offset;
+    // the sum of the earlier tiles (their sums are at most a few thousand values)
+    int pre = 0;
+    for (int t = threadIdx.x; t < (int)blockIdx.x; t += kBlock) pre += tile_sums[t];
+    pre = block_sum_i32(pre, lds);
+    if (threadIdx.x == 0) {
+        tile_offset = pre;
+        if (blockIdx.x == gridDim.x - 1) start[n] = pre + tile_sums[blockIdx.x];  // the total
+    }
+    __syncthreads();
     constexpr int kPer = kScanTile / kBlock;
     const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kPer;
     int v[kPer];
@@ -667,7 +665,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(const int32_t *in, int32_t
         sum += v[k];
     }
     const int inc = block_inclusive_scan(sum, lds);
-    int run = tile_offsets[blockIdx.x] + inc - sum;
+    int run = tile_offset + inc - sum;
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
         if (base + k < n) start[base + k] = run;
@@ -1063,30 +1061,38 @@ __global__ __launch_bounds__(kBlock) void k_interaction(Walk w, Fields f, Physic
 //  EPPX_SELF_FILTER: sFilter main.jl:136-144 -> sKernelFilter! Filter.jl:24-38;
 //  EPPX_SELF_SCALE_FIELD: field[I] *= value.
 // sContinuity! (main.jl:92-105): rho += drho dt, drho = 0, V = m / rho, p = eos(rho).
+// On values in registers: d = (u, v, rho, p) updated; returns the volume.
+__device__ __forceinline__ float continuity_eos_regs(float4 &d, float drho, float mass, const Physics &ph) {
+    const float rho = d.z + drho * ph.dt;
+    d.z = rho;
+    d.w = eos(ph, rho);
+    return mass / rho;
+}
 __device__ __forceinline__ void continuity_eos_one(Fields &f, int s, int tag, const Physics &ph) {
     if (tag != ph.fluid_tag && tag != ph.wall_tag) return;
     float4 d = f.dyn[s];
-    const float rho = d.z + f.drho[s] * ph.dt;
+    const float vol = continuity_eos_regs(d, f.drho[s], f.mass[s], ph);
     f.drho[s] = 0.0f;
-    f.vol[s] = f.mass[s] / rho;
-    d.z = rho;
-    d.w = eos(ph, rho);
+    f.vol[s] = vol;
     f.dyn[s] = d;
 }
 
 // sMomentum! (main.jl:120-126): du += g, then x += (u + du dt / 2) dt and
 // u += du dt per component, du = 0.
-__device__ __forceinline__ void gravity_move_one(Fields &f, int s, int tag, const Physics &ph) {
-    if (tag != ph.fluid_tag) return;
-    float2 du = f.dvel[s];
+// sGravity! + sAccelerateMove! on values in registers.
+__device__ __forceinline__ void gravity_move_regs(float2 &x, float4 &d, float2 du, const Physics &ph) {
     du.x += ph.gx;
     du.y += ph.gy;
-    float2 x = f.pos[s];
-    float4 d = f.dyn[s];
     x.x += (d.x + du.x * ph.dt * 0.5f) * ph.dt;
     d.x += du.x * ph.dt;
     x.y += (d.y + du.y * ph.dt * 0.5f) * ph.dt;
     d.y += du.y * ph.dt;
+}
+__device__ __forceinline__ void gravity_move_one(Fields &f, int s, int tag, const Physics &ph) {
+    if (tag != ph.fluid_tag) return;
+    float2 x = f.pos[s];
+    float4 d = f.dyn[s];
+    gravity_move_regs(x, d, f.dvel[s], ph);
     f.pos[s] = x;
     f.dyn[s] = d;
     f.dvel[s] = make_float2(0.0f, 0.0f);
@@ -1143,13 +1149,22 @@ __global__ __launch_bounds__(kBlock) void k_eos_reset(Fields f, int32_t n, Physi
     const int s = blockIdx.x * kBlock + threadIdx.x;
     if (s < n_counts) counts[s] = 0;
     if (s == 0) *pairs_last = 0ull;
-    if (s >= n || f.alive[s] != 1) return;
+    if (s >= n) return;
+    // every load of the slot up front, so that they are in flight together
+    const int32_t alive = f.alive[s], tag = f.tag[s];
+    const int32_t cs = cases ? f.cs[s] : 0;
+    float4 d = f.dyn[s];
+    const float drho = f.drho[s], mass = f.mass[s];
+    if (alive != 1 || (tag != ph.fluid_tag && tag != ph.wall_tag)) return;
     if (cases) {
-        const CaseDesc &cd = cases[f.cs[s]];
+        const CaseDesc &cd = cases[cs];
         if (cd.steps_left <= 0) return;
         ph.dt = cd.dt;
     }
-    continuity_eos_one(f, s, f.tag[s], ph);
+    const float vol = continuity_eos_regs(d, drho, mass, ph);
+    f.drho[s] = 0.0f;
+    f.vol[s] = vol;
+    f.dyn[s] = d;
 }
 
 // sMomentum! fused with the cell key and count of the search that follows:
@@ -1162,13 +1177,22 @@ __global__ __launch_bounds__(kBlock) void k_move_key(Fields f, int32_t n, Physic
     const bool valid = s < n;
     int32_t k = -1;
     if (valid) {
-        const CaseDesc cd = cases ? cases[f.cs[s]] : one;
-        if (f.alive[s] == 1 && cd.steps_left > 0) {
+        // every load of the slot up front, so that they are in flight together
+        const int32_t alive = f.alive[s], tag = f.tag[s], mov = f.mov[s], cell = f.cell[s];
+        const int32_t cs = cases ? f.cs[s] : 0;
+        float2 x = f.pos[s];
+        float4 d = f.dyn[s];
+        const float2 du = f.dvel[s];
+        const CaseDesc cd = cases ? cases[cs] : one;
+        if (alive == 1 && cd.steps_left > 0 && tag == ph.fluid_tag) {
             Physics p = ph;
             if (cases) p.dt = cd.dt;
-            gravity_move_one(f, s, f.tag[s], p);
+            gravity_move_regs(x, d, du, p);
+            f.pos[s] = x;
+            f.dyn[s] = d;
+            f.dvel[s] = make_float2(0.0f, 0.0f);
         }
-        k = cell_key(f, s, cd, n_cells);
+        k = cell_key_of(f, s, cd, n_cells, alive, mov, cell, x);
         key[s] = k;
     }
     const int32_t slot = count_key(valid, k, n_cells, counts, flags, max_per_cell);
@@ -1397,7 +1421,6 @@ hipError_t launch_key_count(Fields f, int32_t n, const CaseDesc &one, const Case
 hipError_t launch_scan(const int32_t *counts, int32_t n, int32_t *start, int32_t *tile_sums, hipStream_t s) {
     const int32_t tiles = (n + kScanTile - 1) / kScanTile;
     hipLaunchKernelGGL(k_scan_reduce, dim3(tiles), dim3(kBlock), 0, s, counts, n, tile_sums);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, tile_sums, tiles, start, n);
     hipLaunchKernelGGL(k_scan_down, dim3(tiles), dim3(kBlock), 0, s, counts, n, tile_sums, start);
     return hipGetLastError();
 }
