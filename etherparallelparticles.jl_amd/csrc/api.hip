@@ -89,6 +89,10 @@ struct eppx_ps {
     int32_t search_block = 256, momentum_block = 256, search_stage = 0, momentum_stage = 0;
     unsigned long long *masks = nullptr;  // the last search's accept masks (3 per slot)
     bool masks_valid = false;             // they describe every alive slot's pairs
+    // SumWeight / SumWeightedDensity / SumWeightedPressure may be non-zero in
+    // some buffer (an upload brought non-zero values, or a filter or
+    // extrapolation interaction ran); while false the permute skips them
+    bool accum_live = false;
     // columns the kernels do not own, kept on the host between upload and download
     std::vector<int32_t> pass_int_cols, pass_float_cols;
     std::vector<int32_t> pass_int;
@@ -149,14 +153,14 @@ hipError_t alloc_fields(Fields &f, int32_t cap) {
         if (e == hipSuccess) e = dalloc(p, (size_t)cap);
     };
     a(&f.pos); a(&f.dvel); a(&f.dyn);
-    a(&f.mass); a(&f.vol); a(&f.drho); a(&f.h); a(&f.wv); a(&f.wvrho);
+    a(&f.mass); a(&f.vol); a(&f.drho); a(&f.h); a(&f.wv); a(&f.wvrho); a(&f.wvp);
     a(&f.tag); a(&f.mov); a(&f.id); a(&f.cell); a(&f.alive); a(&f.cs);
     return e;
 }
 
 void free_fields(Fields &f) {
     dfree(f.pos); dfree(f.dvel); dfree(f.dyn);
-    dfree(f.mass); dfree(f.vol); dfree(f.drho); dfree(f.h); dfree(f.wv); dfree(f.wvrho);
+    dfree(f.mass); dfree(f.vol); dfree(f.drho); dfree(f.h); dfree(f.wv); dfree(f.wvrho); dfree(f.wvp);
     dfree(f.tag); dfree(f.mov); dfree(f.id); dfree(f.cell); dfree(f.alive); dfree(f.cs);
 }
 
@@ -164,7 +168,8 @@ bool owned_float(const eppx_layout &L, int c) {
     auto in = [&](int start, int width) { return start >= 0 && c >= start && c < start + width; };
     return in(L.PositionVec, 2) || in(L.VelocityVec, 2) || in(L.dVelocityVec, 2) || in(L.Mass, 1) ||
            in(L.Volume, 1) || in(L.Density, 1) || in(L.dDensity, 1) || in(L.Pressure, 1) || in(L.H, 1) ||
-           in(L.SumWeight, 1) || in(L.SumWeightedDensity, 1) || in(L.nRVec, 2 * L.max_neighbour_number) ||
+           in(L.SumWeight, 1) || in(L.SumWeightedDensity, 1) || in(L.SumWeightedPressure, 1) ||
+           in(L.nRVec, 2 * L.max_neighbour_number) ||
            in(L.nR, L.max_neighbour_number) || in(L.nW, L.max_neighbour_number) ||
            in(L.nDW, L.max_neighbour_number) || in(L.nHInv, L.max_neighbour_number);
 }
@@ -177,7 +182,8 @@ bool owned_int(const eppx_layout &L, int c) {
 Columns columns_of(const eppx_ps *ps) {
     const eppx_layout &L = ps->layout;
     return Columns{ps->cap, L.n_int, L.n_float, L.Tag, L.IsMovable, L.PositionVec, L.VelocityVec, L.dVelocityVec,
-                   L.Mass, L.Volume, L.Density, L.dDensity, L.Pressure, L.H, L.SumWeight, L.SumWeightedDensity};
+                   L.Mass, L.Volume, L.Density, L.dDensity, L.Pressure, L.H, L.SumWeight, L.SumWeightedDensity,
+                   L.SumWeightedPressure};
 }
 
 // Event-bracketed launch when the context times kernels.
@@ -399,7 +405,7 @@ int32_t do_search(eppx_ps *ps, eppx_ns *ns, int32_t criterion, int32_t action, c
            }));
     HIP_OK(ctx, timed(ctx, "cell_permute", [&] {
                return launch_permute(cur, nxt, ps->geo, ps->n, 0, nc, ps->key, ps->tmpslot, ps->start,
-                                     ps->members, ps->n_alive, ctx->d_flags, s);
+                                     ps->members, ps->n_alive, ctx->d_flags, ps->accum_live ? 1 : 0, s);
            }));
     ps->cur ^= 1;
     Walk w = make_walk(ps, ns, criterion);
@@ -442,6 +448,7 @@ int32_t do_self(eppx_ps *ps, int32_t action, const Physics &ph, const eppx_field
     const char *name = action == EPPX_SELF_CONTINUITY_EOS ? "continuity_eos"
                        : action == EPPX_SELF_GRAVITY_MOVE ? "gravity_move"
                        : action == EPPX_SELF_FILTER       ? "filter_self"
+                       : action == EPPX_SELF_EXTRAPOLATE_PRESSURE ? "extrapolate_self"
                                                           : "scale_field";
     HIP_OK(ctx, timed(ctx, name, [&] { return launch_self(ps->f(), ps->n, action, ph, op, nullptr, ctx->stream); }));
     return refresh_snapshot_mass(ps, action == EPPX_SELF_SCALE_FIELD ? op.field : -1);
@@ -449,6 +456,10 @@ int32_t do_self(eppx_ps *ps, int32_t action, const Physics &ph, const eppx_field
 
 int32_t do_interaction(eppx_ps *ps, eppx_ns *ns, int32_t action, Physics ph, const eppx_field_op &op) {
     eppx_ctx *ctx = ps->ctx;
+    if ((action == EPPX_INTER_FILTER || action == EPPX_INTER_EXTRAPOLATE_PRESSURE) && !ps->accum_live) {
+        ps->accum_live = true;  // from now on the permute moves the sums (a recorded step moved none)
+        drop_graphs(ps);
+    }
     ph.kernel = ps->last_kernel;  // the stored nW / nDW of the reference come from the search
     Walk w = make_walk(ps, ns, ps->last_crit);
     // the search's masks hold the momentum pass's pairs when its pair classes are the search's
@@ -456,9 +467,10 @@ int32_t do_interaction(eppx_ps *ps, eppx_ns *ns, int32_t action, Physics ph, con
         ph.wall_tag == ps->last_wall)
         w.mask_mode = 2;
     const int fast = ctx->math_mode == EPPX_MATH_FAST;
-    const char *name = action == EPPX_INTER_MOMENTUM ? "momentum"
-                       : action == EPPX_INTER_FILTER ? "filter_inter"
-                                                     : "add_per_neighbour";
+    const char *name = action == EPPX_INTER_MOMENTUM                ? "momentum"
+                       : action == EPPX_INTER_FILTER                ? "filter_inter"
+                       : action == EPPX_INTER_EXTRAPOLATE_PRESSURE ? "extrapolate_inter"
+                                                                    : "add_per_neighbour";
     HIP_OK(ctx, timed(ctx, name, [&] {
                return launch_interaction(w, ps->f(), ps->n, action, ph, op, fast, ps->momentum_block,
                                          ps->momentum_stage, ctx->stream);
@@ -687,7 +699,8 @@ int32_t eppx_ps_create(eppx_ctx *ctx, int32_t n_capacity, const eppx_layout *lay
         !fits(L.dVelocityVec, 2, L.n_float) || !fits(L.Mass, 1, L.n_float) || !fits(L.Volume, 1, L.n_float) ||
         !fits(L.Density, 1, L.n_float) || !fits(L.dDensity, 1, L.n_float) || !fits(L.Pressure, 1, L.n_float) ||
         !fits(L.H, 1, L.n_float) || !fits(L.Gap, 1, L.n_float) || !fits(L.SumWeight, 1, L.n_float) ||
-        !fits(L.SumWeightedDensity, 1, L.n_float) || !fits(L.nRVec, 2 * M, L.n_float) ||
+        !fits(L.SumWeightedDensity, 1, L.n_float) || !fits(L.SumWeightedPressure, 1, L.n_float) ||
+        !fits(L.nRVec, 2 * M, L.n_float) ||
         !fits(L.nR, M, L.n_float) || !fits(L.nW, M, L.n_float) || !fits(L.nDW, M, L.n_float) ||
         !fits(L.nHInv, M, L.n_float))
         return set_error(ctx, EPPX_E_ARG, "eppx_ps_create: a column lies outside its table");
@@ -715,6 +728,12 @@ int32_t eppx_ps_create(eppx_ctx *ctx, int32_t n_capacity, const eppx_layout *lay
     if (e == hipSuccess) e = dalloc(&ps->h_range, 5);
     if (e == hipSuccess) e = hipMemsetAsync(ps->pairs, 0, 2 * sizeof(unsigned long long), ctx->stream);
     if (e == hipSuccess) e = hipMemsetAsync(ps->n_alive, 0, 2 * sizeof(int32_t), ctx->stream);
+    // the filter and extrapolation sums start at zero in both buffers (the
+    // permute does not move them while they stay zero)
+    for (int b = 0; b < 2 && e == hipSuccess; ++b)
+        for (float *p : {ps->buf[b].wv, ps->buf[b].wvrho, ps->buf[b].wvp})
+            if (e == hipSuccess && n_capacity)
+                e = hipMemsetAsync(p, 0, (size_t)n_capacity * sizeof(float), ctx->stream);
     if (e != hipSuccess) {
         eppx_ps_destroy(ps);
         return hip_fail(ctx, e, "eppx_ps_create");
@@ -744,6 +763,18 @@ int32_t eppx_ps_destroy(eppx_ps *ps) {
     ctx->children -= 1;
     release_ctx(ctx);
     return EPPX_OK;
+}
+
+// Any non-zero value (NaN included) in rows [0, rows) of the SumWeight,
+// SumWeightedDensity and SumWeightedPressure columns of a float table.
+bool accumulators_nonzero(const eppx_layout &L, const float *fp, size_t rows) {
+    for (int32_t c : {L.SumWeight, L.SumWeightedDensity, L.SumWeightedPressure}) {
+        if (c < 0) continue;
+        const float *col = fp + rows * (size_t)c;
+        for (size_t r = 0; r < rows; ++r)
+            if (!(col[r] == 0.0f)) return true;
+    }
+    return false;
 }
 
 int32_t eppx_ps_upload(eppx_ps *ps, int32_t n_particles, const int32_t *is_alive, const int32_t *cell_index,
@@ -796,6 +827,7 @@ int32_t eppx_ps_upload(eppx_ps *ps, int32_t n_particles, const int32_t *is_alive
     ps->odd_tags = odd != 0;
     symmetry_bounds(!bad && has(L.H), range[0], range[1], &ps->rmin2, &ps->rmax2);
     ps->hinv_u = uniform_hinv(!bad && has(L.H), range[0], range[1]);
+    ps->accum_live = accumulators_nonzero(L, float_props, cap);  // both buffers now hold these rows
     // keep the columns the kernels do not own
     ps->pass_int.assign(ps->pass_int_cols.size() * cap, 0);
     ps->pass_float.assign(ps->pass_float_cols.size() * cap, 0.0f);
@@ -994,6 +1026,13 @@ int32_t eppx_selfaction(eppx_ps *ps, int32_t action, const void *params, size_t 
                                         {L.Pressure, "Pressure"}},
                                    "sFilter");
             break;
+        case EPPX_SELF_EXTRAPOLATE_PRESSURE:
+            rc = physics_arg(ctx, params, params_size, &ph);
+            if (!rc)
+                rc = check_columns(ps, {{L.SumWeight, "SumWeight"}, {L.SumWeightedPressure, "SumWeightedPressure"},
+                                        {L.Pressure, "Pressure"}},
+                                   "sExtrapolatePressure");
+            break;
         case EPPX_SELF_SCALE_FIELD:
             rc = field_arg(ctx, ps, params, params_size, &op);
             break;
@@ -1017,6 +1056,7 @@ int32_t eppx_interaction(eppx_ps *ps, eppx_ns *ns, int32_t action, const void *p
     switch (action) {
         case EPPX_INTER_MOMENTUM:
         case EPPX_INTER_FILTER:
+        case EPPX_INTER_EXTRAPOLATE_PRESSURE:
             if (ps->last_action != EPPX_SEARCH_KERNEL_CONTINUITY)
                 return set_error(ctx, EPPX_E_STATE,
                                  "eppx_interaction: kernel values come from a search with "
@@ -1030,6 +1070,10 @@ int32_t eppx_interaction(eppx_ps *ps, eppx_ns *ns, int32_t action, const void *p
                 rc = check_columns(ps, {{L.SumWeight, "SumWeight"}, {L.SumWeightedDensity, "SumWeightedDensity"},
                                         {L.Volume, "Volume"}, {L.Mass, "Mass"}},
                                    "iFilter");
+            if (!rc && action == EPPX_INTER_EXTRAPOLATE_PRESSURE)
+                rc = check_columns(ps, {{L.SumWeight, "SumWeight"}, {L.SumWeightedPressure, "SumWeightedPressure"},
+                                        {L.Volume, "Volume"}, {L.Density, "Density"}, {L.Pressure, "Pressure"}},
+                                   "iExtrapolatePressure");
             break;
         case EPPX_INTER_ADD_PER_NEIGHBOUR:
             rc = field_arg(ctx, ps, params, params_size, &op);
@@ -1386,7 +1430,7 @@ int32_t batch_cell_build(eppx_batch *b, bool search, int32_t which, bool keys_do
            }));
     HIP_OK(ctx, timed(ctx, "cell_permute", [&] {
                return launch_permute(cur, nxt, ps->geo, ps->n, 1, nc, ps->key, ps->tmpslot, ps->start, ps->members,
-                                     ps->n_alive, ctx->d_flags, s);
+                                     ps->n_alive, ctx->d_flags, ps->accum_live ? 1 : 0, s);
            }));
     ps->cur ^= 1;
     if (!search) {
@@ -1569,6 +1613,10 @@ int32_t eppx_batch_add(eppx_batch *b, const eppx_domain2d *domain, const eppx_ph
         Columns cols = columns_of(ps);
         cols.n_capacity = n_particles;
         if (e == hipSuccess) e = launch_from_ref(ps->f(), ps->n, cols, d_ip, d_fp, d_alive, d_cell, c, s);
+        if (e == hipSuccess && !ps->accum_live && accumulators_nonzero(L, float_props, rows)) {
+            ps->accum_live = true;  // the permute moves the sums from now on
+            drop_batch_graphs(b);
+        }
         if (e == hipSuccess) e = hipStreamSynchronize(s);
         dfree(d_ip);
         dfree(d_fp);

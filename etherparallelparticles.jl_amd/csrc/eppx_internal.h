@@ -51,7 +51,7 @@ struct CaseDesc {
 struct Fields {
     float2 *pos, *dvel;
     float4 *dyn;  // (u, v, density, pressure): what a neighbour's pair term reads, one 16-byte record
-    float *mass, *vol, *drho, *h, *wv, *wvrho;
+    float *mass, *vol, *drho, *h, *wv, *wvrho, *wvp;  // wvp: SumWeightedPressure
     int32_t *tag, *mov, *id, *cell, *alive, *cs;
 };
 
@@ -92,7 +92,7 @@ inline Physics to_physics(const eppx_physics &p) {
 struct Columns {
     int32_t n_capacity, n_int, n_float;
     int32_t Tag, IsMovable, PositionVec, VelocityVec, dVelocityVec, Mass, Volume, Density, dDensity, Pressure, H;
-    int32_t SumWeight, SumWeightedDensity;
+    int32_t SumWeight, SumWeightedDensity, SumWeightedPressure;
 };
 
 // Neighbour-column export targets (reference layout, column-major) for the
@@ -119,9 +119,12 @@ hipError_t launch_key_count(Fields f, int32_t n, const CaseDesc &one, const Case
 hipError_t launch_scan(const int32_t *counts, int32_t n, int32_t *start, int32_t *block_sums, hipStream_t s);
 hipError_t launch_fill_members(const Fields f, int32_t n, int32_t n_cells, const int32_t *key,
                                const int32_t *tmpslot, const int32_t *start, int32_t *members, hipStream_t s);
+// move_accum = 0: SumWeight, SumWeightedDensity and SumWeightedPressure are
+// zero in both buffers and are not moved.
 hipError_t launch_permute(Fields cur, Fields nxt, float4 *geo, int32_t n, int32_t batch, int32_t n_cells,
                           const int32_t *key, const int32_t *tmpslot, const int32_t *start,
-                          const int32_t *members, int32_t *n_alive, uint32_t *flags, hipStream_t s);
+                          const int32_t *members, int32_t *n_alive, uint32_t *flags, int32_t move_accum,
+                          hipStream_t s);
 // which = 0: the action runs for particles of cases with steps left; 1: of fresh cases.
 // block: threads per pair block (256 or 512); stage: 1 when pass 2 reads
 // its per-neighbour data from the block's LDS tile.  EXACT math uses 256, 0.

@@ -689,7 +689,7 @@ __global__ __launch_bounds__(kBlock) void k_fill_members(Fields f, int32_t n, in
 __global__ __launch_bounds__(kBlock) void k_permute(Fields cur, Fields nxt, float4 *geo, int32_t n, int32_t batch,
                                                      int32_t n_cells, const int32_t *key, const int32_t *tmpslot,
                                                      const int32_t *start, const int32_t *members,
-                                                     int32_t *n_alive, uint32_t *flags) {
+                                                     int32_t *n_alive, uint32_t *flags, int32_t move_accum) {
     const int32_t s = blockIdx.x * kBlock + threadIdx.x;
     if (s == 0) {
         n_alive[0] = start[n_cells];      // alive slots
@@ -708,7 +708,9 @@ __global__ __launch_bounds__(kBlock) void k_permute(Fields cur, Fields nxt, floa
     const float4 dyn = cur.dyn[s];
     const float2 dvel = cur.dvel[s];
     const float mass = cur.mass[s], vol = cur.vol[s], drho = cur.drho[s], h = cur.h[s];
-    const float wv = cur.wv[s], wvrho = cur.wvrho[s];
+    // the filter and extrapolation sums, when they may be non-zero
+    const float wv = move_accum ? cur.wv[s] : 0.0f, wvrho = move_accum ? cur.wvrho[s] : 0.0f;
+    const float wvp = move_accum ? cur.wvp[s] : 0.0f;
     const int32_t tag = cur.tag[s], mov = cur.mov[s], cell = cur.cell[s];
     const int32_t cs = batch ? cur.cs[s] : 0;
     const int32_t b = start[k];
@@ -744,8 +746,11 @@ __global__ __launch_bounds__(kBlock) void k_permute(Fields cur, Fields nxt, floa
     nxt.vol[dest] = vol;
     nxt.drho[dest] = drho;
     nxt.h[dest] = h;
-    nxt.wv[dest] = wv;
-    nxt.wvrho[dest] = wvrho;
+    if (move_accum) {
+        nxt.wv[dest] = wv;
+        nxt.wvrho[dest] = wvrho;
+        nxt.wvp[dest] = wvp;
+    }
     nxt.tag[dest] = tag;
     nxt.mov[dest] = mov;
     nxt.id[dest] = me;
@@ -1023,6 +1028,12 @@ __device__ __forceinline__ float *field_column(const Fields &f, const eppx_field
     }
 }
 
+// Julia's max(x, y) of two floats: NaN if either is NaN, and +0 above -0.
+__device__ __forceinline__ float jl_max(float x, float y) {
+    const bool take_y = (y > x) || (signbit(y) < signbit(x));
+    return take_y ? (isnan(x) ? x : y) : (isnan(y) ? y : x);
+}
+
 template <int ACTION>
 __global__ __launch_bounds__(kBlock) void k_interaction(Walk w, Fields f, Physics ph, eppx_field_op op) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
@@ -1043,6 +1054,26 @@ __global__ __launch_bounds__(kBlock) void k_interaction(Walk w, Fields f, Physic
         });
         f.wv[i] = wv;
         f.wvrho[i] = wvrho;
+    } else if (ACTION == EPPX_INTER_EXTRAPOLATE_PRESSURE) {
+        // iExtrapolatePressure! (Pressure.jl:81-101) for wall i and fluid j, with
+        // w the pair's kernel value (the search's nW), p0 and g from ph
+        if (ti != ph.wall_tag) return;
+        float wv = f.wv[i], wvp = f.wvp[i];
+        const uint32_t fluid_bit = tag_bit(ph.fluid_tag);
+        walk(w, cd, i, gi, [&](int j, uint32_t tbj, float dx, float dy, float dr2) {
+            if (tbj != fluid_bit) return;
+            const float r = sqrtf(dr2);
+            const float wgt = value_exact(ph.kernel, r, invharmonic(gi.z, w.geo[j].z));
+            float g_dot_x = 0.0f;
+            g_dot_x += dx * ph.gx;
+            g_dot_x += dy * ph.gy;
+            const float4 dj = f.dyn[j];
+            const float wvj = wgt * f.vol[j];
+            wv += wvj;
+            wvp += wvj * (jl_max(dj.w, ph.p0) + jl_max(dj.z * g_dot_x, ph.p0));
+        });
+        f.wv[i] = wv;
+        f.wvp[i] = wvp;
     } else {
         int stride;
         float *col = field_column(f, op, stride);
@@ -1059,6 +1090,8 @@ __global__ __launch_bounds__(kBlock) void k_interaction(Walk w, Fields f, Physic
 //  EPPX_SELF_GRAVITY_MOVE: sMomentum! main.jl:120-126 -> sGravity! State.jl:35-47,
 //    sAccelerateMove! Motion.jl:61-75;
 //  EPPX_SELF_FILTER: sFilter main.jl:136-144 -> sKernelFilter! Filter.jl:24-38;
+//  EPPX_SELF_EXTRAPOLATE_PRESSURE: sExtrapolatePressure! (Pressure.jl:125-143) on
+//    wall particles: p = wv_p / wv and both sums cleared when wv > 0, else p = p0;
 //  EPPX_SELF_SCALE_FIELD: field[I] *= value.
 // sContinuity! (main.jl:92-105): rho += drho dt, drho = 0, V = m / rho, p = eos(rho).
 // On values in registers: d = (u, v, rho, p) updated; returns the volume.
@@ -1126,6 +1159,18 @@ __global__ __launch_bounds__(kBlock) void k_self(Fields f, int32_t n, Physics ph
         float4 d = f.dyn[s];
         d.z = rho;
         d.w = eos(ph, rho);
+        f.dyn[s] = d;
+    } else if (ACTION == EPPX_SELF_EXTRAPOLATE_PRESSURE) {
+        if (tag != ph.wall_tag) return;
+        const float wv = f.wv[s];
+        float4 d = f.dyn[s];
+        if (wv > 0.0f) {
+            d.w = f.wvp[s] / wv;
+            f.wv[s] = 0.0f;
+            f.wvp[s] = 0.0f;
+        } else {
+            d.w = ph.p0;
+        }
         f.dyn[s] = d;
     } else {
         int stride;
@@ -1270,6 +1315,7 @@ __global__ __launch_bounds__(kBlock) void k_from_ref(Fields f, int32_t offset, C
     f.h[s] = fcol(c.H);
     f.wv[s] = fcol(c.SumWeight);
     f.wvrho[s] = fcol(c.SumWeightedDensity);
+    f.wvp[s] = fcol(c.SumWeightedPressure);
     f.tag[s] = icol(c.Tag);
     f.mov[s] = icol(c.IsMovable);
     f.id[s] = row;
@@ -1305,6 +1351,7 @@ __global__ __launch_bounds__(kBlock) void k_to_ref(Fields f, int32_t n, Columns 
     put(c.H, f.h[s]);
     put(c.SumWeight, f.wv[s]);
     put(c.SumWeightedDensity, f.wvrho[s]);
+    put(c.SumWeightedPressure, f.wvp[s]);
     if (c.Tag >= 0) ip[row + C * c.Tag] = f.tag[s];
     if (c.IsMovable >= 0) ip[row + C * c.IsMovable] = f.mov[s];
     is_alive[row] = f.alive[s];
@@ -1434,9 +1481,10 @@ hipError_t launch_fill_members(const Fields f, int32_t n, int32_t n_cells, const
 
 hipError_t launch_permute(Fields cur, Fields nxt, float4 *geo, int32_t n, int32_t batch, int32_t n_cells,
                           const int32_t *key, const int32_t *tmpslot, const int32_t *start,
-                          const int32_t *members, int32_t *n_alive, uint32_t *flags, hipStream_t s) {
+                          const int32_t *members, int32_t *n_alive, uint32_t *flags, int32_t move_accum,
+                          hipStream_t s) {
     hipLaunchKernelGGL(k_permute, grid_for(n), dim3(kBlock), 0, s, cur, nxt, geo, n, batch, n_cells, key, tmpslot,
-                       start, members, n_alive, flags);
+                       start, members, n_alive, flags, move_accum);
     return hipGetLastError();
 }
 
@@ -1510,6 +1558,9 @@ hipError_t launch_self(Fields f, int32_t n, int32_t action, const Physics &ph, c
         case EPPX_SELF_FILTER:
             hipLaunchKernelGGL(k_self<EPPX_SELF_FILTER>, g, dim3(kBlock), 0, s, f, n, ph, op, cases);
             break;
+        case EPPX_SELF_EXTRAPOLATE_PRESSURE:
+            hipLaunchKernelGGL(k_self<EPPX_SELF_EXTRAPOLATE_PRESSURE>, g, dim3(kBlock), 0, s, f, n, ph, op, cases);
+            break;
         default:
             hipLaunchKernelGGL(k_self<EPPX_SELF_SCALE_FIELD>, g, dim3(kBlock), 0, s, f, n, ph, op, cases);
             break;
@@ -1557,6 +1608,10 @@ hipError_t launch_interaction(const Walk &w, Fields f, int32_t n, int32_t action
         }
         case EPPX_INTER_FILTER:
             hipLaunchKernelGGL(k_interaction<EPPX_INTER_FILTER>, grid_for(n), dim3(kBlock), 0, s, w, f, ph, op);
+            break;
+        case EPPX_INTER_EXTRAPOLATE_PRESSURE:
+            hipLaunchKernelGGL(k_interaction<EPPX_INTER_EXTRAPOLATE_PRESSURE>, grid_for(n), dim3(kBlock), 0, s, w, f,
+                               ph, op);
             break;
         default:
             hipLaunchKernelGGL(k_interaction<EPPX_INTER_ADD_PER_NEIGHBOUR>, grid_for(n), dim3(kBlock), 0, s, w, f, ph,

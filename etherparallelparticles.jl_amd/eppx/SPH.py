@@ -129,6 +129,21 @@ def sFilter(physics: Physics) -> Action:
     return Action("self", _lib.SELF_FILTER, physics, name="sFilter")
 
 
+def iExtrapolatePressure(physics: Physics) -> Action:
+    """SPH.Library.iExtrapolatePressure! (Pressure.jl:81-101) for wall i and
+    fluid j, with w = the pair's kernel value, p0 = physics.p0 and
+    g = (physics.gx, physics.gy): SumWeight += w V_j, SumWeightedPressure +=
+    w V_j (max(p_j, p0) + max(rho_j g . r_ij, p0))."""
+    return Action("inter", _lib.INTER_EXTRAPOLATE_PRESSURE, physics, name="iExtrapolatePressure!")
+
+
+def sExtrapolatePressure(physics: Physics) -> Action:
+    """SPH.Library.sExtrapolatePressure! (Pressure.jl:125-143) on wall
+    particles: p = SumWeightedPressure / SumWeight and both cleared when
+    SumWeight > 0, else p = p0."""
+    return Action("self", _lib.SELF_EXTRAPOLATE_PRESSURE, physics, name="sExtrapolatePressure!")
+
+
 def ScaleField(column: str, component: int, value: float) -> Action:
     """field(I) *= value for every alive particle."""
     return Action("self", _lib.SELF_SCALE_FIELD, op=(column, component, value), name=f"scale {column}")

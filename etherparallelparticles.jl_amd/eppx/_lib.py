@@ -21,8 +21,8 @@ KERNEL_CUBIC, KERNEL_WENDLAND_C2 = 0, 1
 EOS_LINEAR, EOS_TAIT = 0, 1
 VISC_CLASSIC, VISC_ARTIFICIAL = 0, 1
 SEARCH_NONE, SEARCH_KERNEL_CONTINUITY = 0, 1
-SELF_CONTINUITY_EOS, SELF_GRAVITY_MOVE, SELF_FILTER, SELF_SCALE_FIELD = 1, 2, 3, 10
-INTER_MOMENTUM, INTER_FILTER, INTER_ADD_PER_NEIGHBOUR = 1, 2, 10
+SELF_CONTINUITY_EOS, SELF_GRAVITY_MOVE, SELF_FILTER, SELF_EXTRAPOLATE_PRESSURE, SELF_SCALE_FIELD = 1, 2, 3, 4, 10
+INTER_MOMENTUM, INTER_FILTER, INTER_EXTRAPOLATE_PRESSURE, INTER_ADD_PER_NEIGHBOUR = 1, 2, 3, 10
 (FIELD_POSITION, FIELD_VELOCITY, FIELD_DVELOCITY, FIELD_MASS, FIELD_VOLUME, FIELD_DENSITY, FIELD_DDENSITY,
  FIELD_PRESSURE) = range(8)
 MATH_EXACT, MATH_FAST = 0, 1
@@ -44,7 +44,8 @@ class Domain2D(C.Structure):
 
 LAYOUT_FIELDS = ("n_int", "n_float", "max_neighbour_number", "Tag", "IsMovable", "nCount", "nIndex",
                  "PositionVec", "VelocityVec", "dVelocityVec", "Mass", "Volume", "Density", "dDensity",
-                 "Pressure", "H", "Gap", "SumWeight", "SumWeightedDensity", "nRVec", "nR", "nW", "nDW", "nHInv")
+                 "Pressure", "H", "Gap", "SumWeight", "SumWeightedDensity", "nRVec", "nR", "nW", "nDW", "nHInv",
+                 "SumWeightedPressure")
 
 
 class Layout(C.Structure):

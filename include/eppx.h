@@ -67,6 +67,7 @@ typedef struct {
     int32_t PositionVec, VelocityVec, dVelocityVec, Mass, Volume, Density, dDensity, Pressure, H, Gap;
     int32_t SumWeight, SumWeightedDensity;                    /* float columns */
     int32_t nRVec, nR, nW, nDW, nHInv;                        /* neighbour cols */
+    int32_t SumWeightedPressure;                              /* float column   */
 } eppx_layout;
 
 /* Search criteria (src/Algorithm/NeighbourSearch.jl:141-164). */
@@ -89,11 +90,15 @@ enum { /* self actions */
     EPPX_SELF_CONTINUITY_EOS = 1,       /* sContinuity!, main.jl:92-105                     */
     EPPX_SELF_GRAVITY_MOVE = 2,         /* sMomentum!, main.jl:120-126                      */
     EPPX_SELF_FILTER = 3,               /* sFilter, main.jl:136-144                         */
+    EPPX_SELF_EXTRAPOLATE_PRESSURE = 4, /* sExtrapolatePressure! on wall particles,
+                                           Pressure.jl:125-143                             */
     EPPX_SELF_SCALE_FIELD = 10          /* field[I] *= value                                */
 };
 enum { /* interactions */
     EPPX_INTER_MOMENTUM = 1,            /* iMomentum!, main.jl:107-118                      */
     EPPX_INTER_FILTER = 2,              /* iFilter, main.jl:128-134                         */
+    EPPX_INTER_EXTRAPOLATE_PRESSURE = 3,/* iExtrapolatePressure! (2D) for wall i, fluid j,
+                                           w = the pair's kernel value, Pressure.jl:81-101 */
     EPPX_INTER_ADD_PER_NEIGHBOUR = 10   /* field[I] += value once per neighbour             */
 };
 

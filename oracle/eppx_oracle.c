@@ -374,6 +374,19 @@ static void self_one(const or_layout *L, int32_t I, int32_t *IP, float *FP, int3
         FPC(L, FP, I, L->SumWeightedDensity) = 0.0f;
         FPC(L, FP, I, L->Volume) = FPC(L, FP, I, L->Mass) / FPC(L, FP, I, L->Density);
         FPC(L, FP, I, L->Pressure) = eos(P, FPC(L, FP, I, L->Density));
+    } else if (action == 4) {
+        /* sExtrapolatePressure! Pressure.jl:125-143 (on wall particles) */
+        if (tag != P->wall_tag) return;
+        float wv = FPC(L, FP, I, L->SumWeight);
+        if (wv > 0.0f) {
+            FPC(L, FP, I, L->Pressure) = FPC(L, FP, I, L->SumWeightedPressure) / wv;
+            if (OR_MUTATION != 11) {
+                FPC(L, FP, I, L->SumWeight) = 0.0f;
+                FPC(L, FP, I, L->SumWeightedPressure) = 0.0f;
+            }
+        } else {
+            FPC(L, FP, I, L->Pressure) = P->p0;
+        }
     } else if (action == 10) {
         FPC(L, FP, I, column) *= value;
     }
@@ -458,6 +471,31 @@ static void filter_pair(const or_layout *L, int32_t I, int32_t NI, float *FP, co
     FPC(L, FP, I, L->SumWeightedDensity) += w * FPC(L, FP, J, L->Mass);
 }
 
+/* Julia's max(x, y) of two floats (Base.max for IEEE floats): NaN when
+ * either is NaN, and +0 above -0. */
+static float jl_max(float x, float y) {
+    int take_y = (y > x) || (signbit(y) < signbit(x));
+    return take_y ? (isnan(x) ? x : y) : (isnan(y) ? y : x);
+}
+
+/* iExtrapolatePressure! (2D) Pressure.jl:81-101 for wall I and fluid J, with
+ * w = the pair's kernel value stored by the search (nW), p0 and g from P. */
+static void extrapolate_pair(const or_layout *L, int32_t I, int32_t NI, float *FP, const int32_t *IP,
+                             const or_physics *P) {
+    int32_t J = IPC(L, IP, I, L->nIndex + NI) - 1;
+    if (IPC(L, IP, I, L->Tag) != P->wall_tag || IPC(L, IP, J, L->Tag) != P->fluid_tag) return;
+    float w = FPC(L, FP, I, L->nW + NI);
+    float rx = FPC(L, FP, I, L->nRVec + 2 * NI), ry = FPC(L, FP, I, L->nRVec + 2 * NI + 1);
+    float g_dot_x = 0.0f;
+    g_dot_x += rx * P->gx;
+    g_dot_x += ry * P->gy;
+    if (OR_MUTATION == 10) g_dot_x = -g_dot_x;
+    float wv = w * FPC(L, FP, J, L->Volume);
+    FPC(L, FP, I, L->SumWeight) += wv;
+    FPC(L, FP, I, L->SumWeightedPressure) +=
+        wv * (jl_max(FPC(L, FP, J, L->Pressure), P->p0) + jl_max(FPC(L, FP, J, L->Density) * g_dot_x, P->p0));
+}
+
 void or_interaction(const or_layout *L, int32_t n_particles, const int32_t *is_alive, int32_t *IP, float *FP,
                     int32_t action, const or_physics *phys, int32_t column, float value, int32_t n_threads) {
 #ifdef _OPENMP
@@ -473,6 +511,7 @@ void or_interaction(const or_layout *L, int32_t n_particles, const int32_t *is_a
         for (int32_t NI = 0; NI < n; ++NI) {
             if (action == 1) momentum_pair(L, I, NI, FP, IP, phys);
             else if (action == 2) filter_pair(L, I, NI, FP, IP, phys);
+            else if (action == 3) extrapolate_pair(L, I, NI, FP, IP, phys);
             else if (action == 10) FPC(L, FP, I, column) += value;
         }
     }

@@ -41,6 +41,7 @@ typedef struct {
     int32_t PositionVec, VelocityVec, dVelocityVec, Mass, Volume, Density, dDensity, Pressure, H, Gap;
     int32_t SumWeight, SumWeightedDensity;
     int32_t nRVec, nR, nW, nDW, nHInv;
+    int32_t SumWeightedPressure;
 } or_layout;
 
 enum { OR_KERNEL_CUBIC = 0, OR_KERNEL_WENDLAND_C2 = 1 };
@@ -99,12 +100,14 @@ int32_t or_search(const or_layout *L, int32_t n_particles, int32_t *is_alive, in
                   int32_t serial_insert, int32_t n_threads);
 
 /* Self actions: 1 continuity+volume+EOS (sContinuity!), 2 gravity+move
- * (sMomentum!), 3 filter finish (sFilter), 10 multiply float column
+ * (sMomentum!), 3 filter finish (sFilter), 4 extrapolated wall pressure
+ * (sExtrapolatePressure! on wall particles), 10 multiply float column
  * `column` by `value`. */
 void or_selfaction(const or_layout *L, int32_t n_particles, const int32_t *is_alive, int32_t *IP, float *FP,
                    int32_t action, const or_physics *phys, int32_t column, float value, int32_t n_threads);
 
 /* Interactions: 1 momentum (iMomentum!), 2 filter sums (iFilter),
+ * 3 extrapolation sums (iExtrapolatePressure! for wall I, fluid J),
  * 10 add `value` to float column `column` once per neighbour. */
 void or_interaction(const or_layout *L, int32_t n_particles, const int32_t *is_alive, int32_t *IP, float *FP,
                     int32_t action, const or_physics *phys, int32_t column, float value, int32_t n_threads);

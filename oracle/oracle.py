@@ -47,7 +47,8 @@ class Domain2D(C.Structure):
 
 LAYOUT_FIELDS = ("n_capacity", "n_int", "n_float", "max_neighbour_number", "Tag", "IsMovable", "nCount", "nIndex",
                  "PositionVec", "VelocityVec", "dVelocityVec", "Mass", "Volume", "Density", "dDensity", "Pressure",
-                 "H", "Gap", "SumWeight", "SumWeightedDensity", "nRVec", "nR", "nW", "nDW", "nHInv")
+                 "H", "Gap", "SumWeight", "SumWeightedDensity", "nRVec", "nR", "nW", "nDW", "nHInv",
+                 "SumWeightedPressure")
 
 
 class Layout(C.Structure):

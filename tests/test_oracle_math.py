@@ -401,3 +401,71 @@ def test_shepard_filter_scales_with_mass(oracle):
     b.FP[: b.n, b.L.Density] *= 3.0
     ra, rb = shepard(oracle, a, phys(oracle)), shepard(oracle, b, phys(oracle))
     assert np.max(np.abs(rb / (3.0 * ra) - 1.0)) < 1e-6
+
+
+# ------------------------------------------------ extrapolated wall pressure
+
+def hydrostatic_walls(O, variant=None, wall_rows=6):
+    """A 20 x 20 lattice whose bottom rows are walls; the fluid above holds the
+    hydrostatic pressure p = rho0 g (H - y) of a free surface at H."""
+    nx, ny, dx = 20, 20, 0.01
+    tags = np.ones(nx * ny, np.int32)
+    tags[: wall_rows * nx] = 2
+    s, x, y = lattice(O, nx, ny, dx, tags=tags, variant=variant)
+    H = ny * dx
+    fluid = tags == 1
+    s.FP[: s.n, s.L.Pressure][fluid] = (1000.0 * 9.81 * (H - y[fluid])).astype(np.float32)
+    return s, x, y, fluid, H
+
+
+def extrapolate(O, s, ph):
+    assert s.search(O.CRIT_SYMMETRY, 1, ph) == 0
+    s.interaction(3, ph)
+    s.selfaction(4, ph)
+    return s.FP[: s.n, s.L.Pressure].astype(np.float64)
+
+
+def test_extrapolated_wall_pressure_is_hydrostatic(oracle):
+    """Walls within reach of the fluid take rho0 g (H - y_wall), the hydrostatic
+    pressure carried down to their depth; walls out of reach keep p0 and their
+    sums; the sums of the others are cleared."""
+    s, x, y, fluid, H = hydrostatic_walls(oracle)
+    p = extrapolate(oracle, s, phys(oracle))
+    wall = ~fluid
+    # fluid starts at y = 6.5 dx and W vanishes at 2h = 4 dx, so rows from 3.5 dx up see it
+    reach = wall & (y > 0.03)
+    want = 1000.0 * 9.81 * (H - y)
+    assert reach.sum() > 0 and (wall & ~reach).sum() > 0
+    assert np.max(np.abs(p[reach] / want[reach] - 1.0)) < 1e-5
+    assert (p[wall & ~reach] == 0.0).all()
+    assert (s.FP[: s.n, s.L.SumWeight] == 0.0).all() and (s.FP[: s.n, s.L.SumWeightedPressure] == 0.0).all()
+
+
+def test_extrapolated_wall_pressure_catches_flipped_gravity(oracle):
+    s, x, y, fluid, H = hydrostatic_walls(oracle, variant=oracle.mutant(10))
+    p = extrapolate(oracle, s, phys(oracle))
+    reach = ~fluid & (y > 0.03)
+    want = 1000.0 * 9.81 * (H - y)
+    assert np.max(np.abs(p[reach] / want[reach] - 1.0)) > 1e-2
+
+
+def test_extrapolated_wall_pressure_catches_uncleared_sums(oracle):
+    s, *_ = hydrostatic_walls(oracle, variant=oracle.mutant(11))
+    extrapolate(oracle, s, phys(oracle))
+    assert s.FP[: s.n, s.L.SumWeight].max() > 0.0
+
+
+def test_extrapolated_wall_pressure_clamps_and_propagates_nan(oracle):
+    """max(p_j, p0): suction in the fluid gives p0 on the wall; Julia's max
+    keeps a NaN, so a NaN fluid pressure reaches the walls next to it only."""
+    s, x, y, fluid, H = hydrostatic_walls(oracle)
+    s.FP[: s.n, s.L.Pressure][fluid] = -500.0
+    p = extrapolate(oracle, s, phys(oracle, gy=0.0))
+    assert (p[~fluid] == 0.0).all()
+    s, x, y, fluid, H = hydrostatic_walls(oracle)
+    k = int(np.flatnonzero(fluid & (y < 0.07) & (x > 0.09) & (x < 0.10))[0])  # first fluid row, mid x
+    s.FP[k, s.L.Pressure] = np.nan
+    p = extrapolate(oracle, s, phys(oracle))
+    near = ~fluid & ((x - x[k]) ** 2 + (y - y[k]) ** 2 < (0.04) ** 2)
+    far = ~fluid & (np.abs(x - x[k]) > 0.05)
+    assert np.isnan(p[near]).any() and np.isfinite(p[far]).all()
