@@ -454,11 +454,12 @@ __device__ __forceinline__ int walk_rows(const Walk &w, const CaseDesc &cd, cons
 // cells c0 - 1 .. c1 + 1, which are copied to tile (tile_row slots per row).
 // Returns false, with nothing staged, when the block spans two cases or two
 // cell rows or a row range does not fit; the block then reads global memory.
-// Every thread of the block calls it (two barriers).
+// only_tag: set when every staged slot (the block's own particles among them)
+// carries the tag `tag`.  Every thread of the block calls it (two barriers).
 template <int BLOCK, int STAGE>
 __device__ __forceinline__ bool stage_rows(const Walk &w, const float4 *dyn, int i0, int n, float4 *tile,
                                            uint8_t *tix, float4 *tdyn, float2 *tuv, int *info,
-                                           TileRows<STAGE> &rows) {
+                                           TileRows<STAGE> &rows, int tag, bool &only_tag) {
     constexpr int kTileRow = PairShape<BLOCK>::tile_row;
     const int t = threadIdx.x;
     if (t < 3) {
@@ -479,13 +480,16 @@ __device__ __forceinline__ bool stage_rows(const Walk &w, const float4 *dyn, int
     }
     __syncthreads();
     const bool ok = info[6] && info[7] && info[8];
+    int other = 0;  // this thread staged a slot of another tag
     if (ok) {
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
             const int b = info[r], len = info[3 + r];
             for (int k = t; k < len; k += BLOCK) {
                 tile[r * kTileRow + k] = w.geo[b + k];
-                const uint32_t t1 = (uint32_t)w.tag[b + k] - 1u;
+                const int tg = w.tag[b + k];
+                other |= tg != tag;
+                const uint32_t t1 = (uint32_t)tg - 1u;
                 tix[r * kTileRow + k] = (uint8_t)(t1 < (uint32_t)kMaxTag ? t1 : 63u);
                 if (STAGE == 1) tdyn[r * kTileRow + k] = dyn[b + k];
                 if (STAGE == 2) tuv[r * kTileRow + k] = reinterpret_cast<const float2 *>(dyn)[2 * (size_t)(b + k)];
@@ -501,7 +505,7 @@ __device__ __forceinline__ bool stage_rows(const Walk &w, const float4 *dyn, int
     } else if (t == 0) {
         atomicAdd(w.fallback, 1u);
     }
-    __syncthreads();
+    only_tag = ok && !__syncthreads_or(other);
     return ok;
 }
 
@@ -812,8 +816,10 @@ __global__ __launch_bounds__(BLOCK) void k_search(Walk w, Fields f, Physics ph, 
     if (i0 >= n) return;  // the whole block
     const int i = i0 + threadIdx.x;
     TileRows<STAGE> rows;
+    bool all_fluid = false;  // every particle of the block's neighbourhood is fluid
     const bool tiled = ACTION == EPPX_SEARCH_KERNEL_CONTINUITY &&
-                       stage_rows<BLOCK, STAGE>(w, f.dyn, i0, n, tile, tix, nullptr, tuv, info, rows);
+                       stage_rows<BLOCK, STAGE>(w, f.dyn, i0, n, tile, tix, nullptr, tuv, info, rows, ph.fluid_tag,
+                                                all_fluid);
     int cnt = 0;
     const CaseDesc cd = BATCH ? w.cases[w.cs[min(i, n - 1)]] : w.one;
     // a batch runs the action for the cases that advance (which = 0) or wait for a first search (1)
@@ -848,12 +854,27 @@ __global__ __launch_bounds__(BLOCK) void k_search(Walk w, Fields f, Physics ph, 
             if constexpr (FAST) {
                 if (tiled && cd.hinv_u > 0.0f) {  // one H in the case: the pair's hinv is loop invariant
                     const PairH<true> hu(cd.hinv_u, cd.hinv_u * cd.hinv_u * cd.hinv_u, 0.0f, 0.0f);
-                    auto term_u = [&](int, uint32_t tbj, float, float mj, float dx, float dy, float dr2,
-                                      const ContinuityGather &p) {
-                        const float m = (i_fluid && tbj == fluid_bit) ? mj : rho_i * p.vol;
-                        acc += continuity_fast<K>(hu, ui, p.uv, m, dx, dy, dr2);
-                    };
-                    cnt = walk_rows<false, true>(w, cd, rows, i, gi, keep, fetch, term_u);
+                    if (all_fluid) {  // fluid-fluid pairs only (classic continuity): no tag per pair
+                        auto fetch_f = [&](int j, int k, uint32_t, const auto &src) {
+                            ContinuityGather p;
+                            if constexpr (std::decay_t<decltype(src)>::kStage == 2) p.uv = src.uv[k];
+                            else p.uv = ld_slot(uv, 2 * j);
+                            p.vol = 0.0f;
+                            return p;
+                        };
+                        auto term_f = [&](int, uint32_t, float, float mj, float dx, float dy, float dr2,
+                                          const ContinuityGather &p) {
+                            acc += continuity_fast<K>(hu, ui, p.uv, mj, dx, dy, dr2);
+                        };
+                        cnt = walk_rows<false, true>(w, cd, rows, i, gi, keep, fetch_f, term_f);
+                    } else {
+                        auto term_u = [&](int, uint32_t tbj, float, float mj, float dx, float dy, float dr2,
+                                          const ContinuityGather &p) {
+                            const float m = (i_fluid && tbj == fluid_bit) ? mj : rho_i * p.vol;
+                            acc += continuity_fast<K>(hu, ui, p.uv, m, dx, dy, dr2);
+                        };
+                        cnt = walk_rows<false, true>(w, cd, rows, i, gi, keep, fetch, term_u);
+                    }
                     walked = true;
                 }
             }
@@ -968,7 +989,9 @@ __global__ __launch_bounds__(BLOCK) void k_momentum(Walk w, Fields f, Physics ph
     const int i0 = xcd_block(blockIdx.x, gridDim.x) * BLOCK;
     if (i0 >= n) return;  // the whole block
     TileRows<STAGE> rows;
-    const bool tiled = stage_rows<BLOCK, STAGE>(w, f.dyn, i0, n, tile, tix, tdyn, nullptr, info, rows);
+    bool all_fluid = false;  // every particle of the block's neighbourhood is fluid
+    const bool tiled =
+        stage_rows<BLOCK, STAGE>(w, f.dyn, i0, n, tile, tix, tdyn, nullptr, info, rows, ph.fluid_tag, all_fluid);
     const int i = i0 + threadIdx.x;
     if (i >= n) return;
     if (f.tag[i] != ph.fluid_tag) return;  // iMomentum! acts on fluid particles only
@@ -993,11 +1016,20 @@ __global__ __launch_bounds__(BLOCK) void k_momentum(Walk w, Fields f, Physics ph
     if constexpr (FAST) {
         if (tiled && cd.hinv_u > 0.0f) {  // one H in the case: the pair's hinv is loop invariant
             const PairH<true> hu(cd.hinv_u, cd.hinv_u * cd.hinv_u * cd.hinv_u, 0.0f, 0.0f);
-            auto term_u = [&](int, uint32_t tbj, float, float mj, float dx, float dy, float dr2, const float4 &dj) {
-                momentum_fast<K, VISC>(ph, acc, tbj == fluid_bit, hu, ui, di.z, di.w, pr_i, make_float2(dj.x, dj.y),
-                                       dj.z, dj.w, mj, dx, dy, dr2);
-            };
-            walk_rows<false, false>(w, cd, rows, i, gi, keep, fetch, term_u);
+            if (all_fluid) {  // fluid-fluid pairs only (classic pressure): no tag per pair
+                auto term_f = [&](int, uint32_t, float, float mj, float dx, float dy, float dr2, const float4 &dj) {
+                    momentum_fast<K, VISC>(ph, acc, true, hu, ui, di.z, di.w, pr_i, make_float2(dj.x, dj.y), dj.z,
+                                           dj.w, mj, dx, dy, dr2);
+                };
+                walk_rows<false, false>(w, cd, rows, i, gi, keep, fetch, term_f);
+            } else {
+                auto term_u = [&](int, uint32_t tbj, float, float mj, float dx, float dy, float dr2,
+                                  const float4 &dj) {
+                    momentum_fast<K, VISC>(ph, acc, tbj == fluid_bit, hu, ui, di.z, di.w, pr_i,
+                                           make_float2(dj.x, dj.y), dj.z, dj.w, mj, dx, dy, dr2);
+                };
+                walk_rows<false, false>(w, cd, rows, i, gi, keep, fetch, term_u);
+            }
             walked = true;
         }
     }
