@@ -83,7 +83,11 @@ struct eppx_ps {
     eppx_ns *last_ns = nullptr;
     int32_t last_crit = 0, last_action = 0, last_kernel = 0, last_fluid = 0, last_wall = 0;
     bool searched = false;
-    StepGraph graphs[2];
+    // recorded steps, per buffer parity and per kind of step (StepKind)
+    StepGraph graphs[2][4];
+    // the fused step's records and volumes written aside (allocated on first use)
+    float4 *dyn_new = nullptr;
+    float *vol_new = nullptr;
     // pair-kernel shapes (eppx_autotune picks them): threads per block, and
     // whether pass 2 reads its per-neighbour data from the LDS tile
     int32_t search_block = 256, momentum_block = 256, search_stage = 0, momentum_stage = 0;
@@ -385,7 +389,8 @@ int32_t field_arg(eppx_ctx *ctx, const eppx_ps *ps, const void *params, size_t s
 // keys_done: the cell counts, keys and provisional slots of this search were
 // already taken (k_move_key in a step).
 int32_t do_search(eppx_ps *ps, eppx_ns *ns, int32_t criterion, int32_t action, const Physics &ph,
-                  bool keys_done = false) {
+                  bool keys_done = false, const float4 *dyn_src = nullptr, const float *vol_src = nullptr,
+                  const EosOut *eos = nullptr) {
     eppx_ctx *ctx = ps->ctx;
     int32_t rc = ensure_cells(ps, ns->grid.n_cells);
     if (rc) return rc;
@@ -405,13 +410,17 @@ int32_t do_search(eppx_ps *ps, eppx_ns *ns, int32_t criterion, int32_t action, c
            }));
     HIP_OK(ctx, timed(ctx, "cell_permute", [&] {
                return launch_permute(cur, nxt, ps->geo, ps->n, 0, nc, ps->key, ps->tmpslot, ps->start,
-                                     ps->members, ps->n_alive, ctx->d_flags, ps->accum_live ? 1 : 0, s);
+                                     ps->members, ps->n_alive, ctx->d_flags, ps->accum_live ? 1 : 0, dyn_src,
+                                     vol_src, s);
            }));
     ps->cur ^= 1;
     Walk w = make_walk(ps, ns, criterion);
     w.mask_mode = action == EPPX_SEARCH_KERNEL_CONTINUITY ? 1 : 0;  // keep the masks for the interactions
     const int fast = ctx->math_mode == EPPX_MATH_FAST;
     HIP_OK(ctx, timed(ctx, "search", [&] {
+               if (eos)
+                   return launch_search_eos(w, ps->f(), ps->n, ph, *eos, fast, ns->M, ps->ncount, &ps->pairs[0],
+                                            &ps->pairs[1], ctx->d_flags, ps->search_block, s);
                return launch_search(w, ps->f(), ps->n, action, ph, fast, ns->M, ps->ncount, &ps->pairs[0],
                                     &ps->pairs[1], ctx->d_flags, 0, ps->search_block,
                                     ps->search_stage, s);
@@ -428,11 +437,15 @@ int32_t do_search(eppx_ps *ps, eppx_ns *ns, int32_t criterion, int32_t action, c
 }
 
 void drop_graphs(eppx_ps *ps) {
-    if (ps->graphs[0].exec || ps->graphs[1].exec) (void)hipStreamSynchronize(ps->ctx->stream);
-    for (auto &g : ps->graphs) {
-        if (g.exec) (void)hipGraphExecDestroy(g.exec);
-        g = StepGraph{};
-    }
+    bool any = false;
+    for (auto &row : ps->graphs)
+        for (auto &g : row) any = any || g.exec;
+    if (any) (void)hipStreamSynchronize(ps->ctx->stream);
+    for (auto &row : ps->graphs)
+        for (auto &g : row) {
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+            g = StepGraph{};
+        }
 }
 
 // The pair kernels read m_j from the search snapshot; an operation that
@@ -476,6 +489,20 @@ int32_t do_interaction(eppx_ps *ps, eppx_ns *ns, int32_t action, Physics ph, con
                                          ps->momentum_stage, ctx->stream);
            }));
     return refresh_snapshot_mass(ps, action == EPPX_INTER_ADD_PER_NEIGHBOUR ? op.field : -1);
+}
+
+// iMomentum! with sMomentum! and the coming search's cell keys fused (a
+// particle system's time step; the search's masks give the pair set).
+int32_t do_momentum_move(eppx_ps *ps, eppx_ns *ns, Physics ph, const MoveKey &mk) {
+    eppx_ctx *ctx = ps->ctx;
+    ph.kernel = ps->last_kernel;
+    Walk w = make_walk(ps, ns, ps->last_crit);
+    if (ps->masks_valid && ph.fluid_tag == ps->last_fluid && ph.wall_tag == ps->last_wall) w.mask_mode = 2;
+    const int fast = ctx->math_mode == EPPX_MATH_FAST;
+    HIP_OK(ctx, timed(ctx, "momentum", [&] {
+               return launch_momentum_move(w, ps->f(), ps->n, ph, mk, fast, ps->momentum_block, ctx->stream);
+           }));
+    return EPPX_OK;
 }
 
 // Autotuner of the pair-kernel shape: times the FAST search action and the
@@ -758,6 +785,7 @@ int32_t eppx_ps_destroy(eppx_ps *ps) {
     dfree(ps->counts); dfree(ps->start); dfree(ps->tile_sums);
     dfree(ps->key); dfree(ps->tmpslot); dfree(ps->members); dfree(ps->ncount); dfree(ps->n_alive); dfree(ps->masks);
     dfree(ps->pairs); dfree(ps->h_range);
+    dfree(ps->dyn_new); dfree(ps->vol_new);
     eppx_ctx *ctx = ps->ctx;
     delete ps;
     ctx->children -= 1;
@@ -1110,40 +1138,67 @@ int32_t eppx_step(eppx_ps *ps, eppx_ns *ns, const eppx_physics *phys, int32_t n_
     HIP_OK(ctx, hipSetDevice(ctx->device));
     if ((rc = ensure_cells(ps, ns->grid.n_cells))) return rc;  // no allocation while a step is recorded
     const eppx_field_op op{};
-    // sContinuity! (with the coming search's counts zeroed), iMomentum!,
-    // sMomentum! (with the coming search's cell keys), then the search
     const int32_t nc = ns->grid.n_cells;
     hipStream_t s = ctx->stream;
-    auto enqueue_step = [&](bool filter_now) -> int32_t {
+    // The fused step (gathered pair shapes): momentum runs sMomentum! and the
+    // cell keys in its epilogue, and the search may run the next step's
+    // sContinuity! in its own; their outputs for the permute go to dyn_new /
+    // vol_new.
+    const bool fusable = ps->search_stage == 0 && ps->momentum_stage == 0;
+    if (fusable && !ps->dyn_new) {  // no allocation while a step is recorded
+        HIP_OK(ctx, dalloc(&ps->dyn_new, (size_t)ps->cap));
+        HIP_OK(ctx, dalloc(&ps->vol_new, (size_t)ps->cap));
+    }
+    // eos_first: the step starts with sContinuity! (first step of the call,
+    // or after a step whose search did not apply it); fuse_eos: its search
+    // applies the next step's.
+    auto enqueue_step = [&](bool eos_first, bool fuse_eos, bool filter_now) -> int32_t {
         int32_t r;
-        HIP_OK(ctx, timed(ctx, "continuity_eos", [&] {
-                   return launch_eos_reset(ps->f(), ps->n, ph, nullptr, ps->counts, nc + 2, &ps->pairs[0], s);
-               }));
-        if ((r = do_interaction(ps, ns, EPPX_INTER_MOMENTUM, ph, op))) return r;
-        HIP_OK(ctx, timed(ctx, "gravity_move", [&] {
-                   return launch_move_key(ps->f(), ps->n, ph, case_of(ps, ns, EPPX_CRIT_SYMMETRY), nullptr, nc,
-                                          ps->counts, ps->key, ps->tmpslot, ctx->d_flags, ns->M, s);
-               }));
-        if ((r = do_search(ps, ns, EPPX_CRIT_SYMMETRY, EPPX_SEARCH_KERNEL_CONTINUITY, ph, true))) return r;
+        if (eos_first)
+            HIP_OK(ctx, timed(ctx, "continuity_eos", [&] {
+                       return launch_eos_reset(ps->f(), ps->n, ph, nullptr, ps->counts, nc + 2, &ps->pairs[0], s);
+                   }));
+        if (fusable) {
+            const MoveKey mk{ps->n, nc, ns->M, ps->counts, ps->key, ps->tmpslot, ctx->d_flags, ps->dyn_new,
+                             &ps->pairs[0]};
+            if ((r = do_momentum_move(ps, ns, ph, mk))) return r;
+            const EosOut eo{ps->n, nc + 2, ps->vol_new, ps->counts};
+            if ((r = do_search(ps, ns, EPPX_CRIT_SYMMETRY, EPPX_SEARCH_KERNEL_CONTINUITY, ph, true, ps->dyn_new,
+                               eos_first ? nullptr : ps->vol_new, fuse_eos ? &eo : nullptr)))
+                return r;
+        } else {
+            if ((r = do_interaction(ps, ns, EPPX_INTER_MOMENTUM, ph, op))) return r;
+            HIP_OK(ctx, timed(ctx, "gravity_move", [&] {
+                       return launch_move_key(ps->f(), ps->n, ph, case_of(ps, ns, EPPX_CRIT_SYMMETRY), nullptr, nc,
+                                              ps->counts, ps->key, ps->tmpslot, ctx->d_flags, ns->M, s);
+                   }));
+            if ((r = do_search(ps, ns, EPPX_CRIT_SYMMETRY, EPPX_SEARCH_KERNEL_CONTINUITY, ph, true))) return r;
+        }
         if (filter_now) {
             if ((r = do_interaction(ps, ns, EPPX_INTER_FILTER, ph, op))) return r;
             if ((r = do_self(ps, EPPX_SELF_FILTER, ph, op))) return r;
         }
         return EPPX_OK;
     };
+    bool eos_pending = true;
     for (int32_t k = 1; k <= n_steps; ++k) {
         const bool filter_now = filter_interval > 0 && (step_offset + k) % filter_interval == 0;
+        // the last step of the call leaves the state as the reference's loop
+        // body does (its dDensity not yet applied); a filter step too
+        const bool fuse = fusable && k < n_steps && !filter_now;
+        const bool eos_first = eos_pending;
+        eos_pending = !fuse;
         if (filter_now || ctx->timing || !ps->masks_valid) {  // filter steps and per-kernel timing run eagerly
-            if ((rc = enqueue_step(filter_now))) return rc;
+            if ((rc = enqueue_step(eos_first, fuse, filter_now))) return rc;
             continue;
         }
-        StepGraph &g = ps->graphs[ps->cur];
+        StepGraph &g = ps->graphs[ps->cur][(eos_first ? 0 : 2) + (fuse ? 1 : 0)];
         if (!g.exec || g.ns_serial != ns->serial || g.math != ctx->math_mode || memcmp(&g.ph, &ph, sizeof(ph)) != 0) {
             if (g.exec) (void)hipGraphExecDestroy(g.exec);
             g = StepGraph{};
             const int cur0 = ps->cur;
             HIP_OK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
-            rc = enqueue_step(false);
+            rc = enqueue_step(eos_first, fuse, false);
             hipGraph_t graph = nullptr;
             const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
             ps->cur = cur0;  // recorded, not run yet
@@ -1430,7 +1485,7 @@ int32_t batch_cell_build(eppx_batch *b, bool search, int32_t which, bool keys_do
            }));
     HIP_OK(ctx, timed(ctx, "cell_permute", [&] {
                return launch_permute(cur, nxt, ps->geo, ps->n, 1, nc, ps->key, ps->tmpslot, ps->start, ps->members,
-                                     ps->n_alive, ctx->d_flags, ps->accum_live ? 1 : 0, s);
+                                     ps->n_alive, ctx->d_flags, ps->accum_live ? 1 : 0, nullptr, nullptr, s);
            }));
     ps->cur ^= 1;
     if (!search) {

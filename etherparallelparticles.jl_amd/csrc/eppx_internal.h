@@ -120,11 +120,13 @@ hipError_t launch_scan(const int32_t *counts, int32_t n, int32_t *start, int32_t
 hipError_t launch_fill_members(const Fields f, int32_t n, int32_t n_cells, const int32_t *key,
                                const int32_t *tmpslot, const int32_t *start, int32_t *members, hipStream_t s);
 // move_accum = 0: SumWeight, SumWeightedDensity and SumWeightedPressure are
-// zero in both buffers and are not moved.
+// zero in both buffers and are not moved.  dyn_src / vol_src (null: cur's)
+// hold the slots' (u, v, rho, p) records and volumes when a fused kernel
+// wrote them aside.
 hipError_t launch_permute(Fields cur, Fields nxt, float4 *geo, int32_t n, int32_t batch, int32_t n_cells,
                           const int32_t *key, const int32_t *tmpslot, const int32_t *start,
                           const int32_t *members, int32_t *n_alive, uint32_t *flags, int32_t move_accum,
-                          hipStream_t s);
+                          const float4 *dyn_src, const float *vol_src, hipStream_t s);
 // which = 0: the action runs for particles of cases with steps left; 1: of fresh cases.
 // block: threads per pair block (256 or 512); stage: 1 when pass 2 reads
 // its per-neighbour data from the block's LDS tile.  EXACT math uses 256, 0.
@@ -146,6 +148,32 @@ hipError_t launch_move_key(Fields f, int32_t n, const Physics &ph, const CaseDes
                            int32_t n_cells, int32_t *counts, int32_t *key, int32_t *tmpslot, uint32_t *flags,
                            int32_t max_per_cell, hipStream_t s);
 hipError_t launch_set_case(CaseDesc *d, const CaseDesc &v, hipStream_t s);
+
+// Fused time step of a particle system (256 / 512 threads, gathered shapes).
+// Momentum with gravity + accelerate-move and the coming search's cell keys
+// and counts in its epilogue: the moved (u, v, rho, p) records go to dyn_new
+// (the neighbours still read the old ones), the positions, dvel, cell and
+// alive in place; pairs_last is zeroed.
+struct MoveKey {
+    int32_t n_slots, n_cells, max_per_cell;
+    int32_t *counts, *key, *tmpslot;
+    uint32_t *flags;
+    float4 *dyn_new;
+    unsigned long long *pairs_last;
+};
+hipError_t launch_momentum_move(const Walk &w, Fields f, int32_t n, const Physics &ph, const MoveKey &mk,
+                                int32_t fast, int32_t block, hipStream_t s);
+// The search with the next step's sContinuity! in its epilogue: rho, p and
+// dDensity in place, the volumes (of every slot) to vol_new (the neighbours
+// still read the old ones), and the cell counts zeroed for the next step.
+struct EosOut {
+    int32_t n_slots, n_counts;
+    float *vol_new;
+    int32_t *counts;
+};
+hipError_t launch_search_eos(const Walk &w, Fields f, int32_t n, const Physics &ph, const EosOut &eo, int32_t fast,
+                             int32_t M, int32_t *ncount, unsigned long long *pairs_last,
+                             unsigned long long *pairs_total, uint32_t *flags, int32_t block, hipStream_t s);
 hipError_t launch_interaction(const Walk &w, Fields f, int32_t n, int32_t action, const Physics &ph,
                               const eppx_field_op &op, int32_t fast, int32_t block, int32_t stage, hipStream_t s);
 hipError_t launch_export(const Walk &w, Fields f, int32_t n, const int64_t *row_start, int32_t *js,

@@ -162,6 +162,25 @@ __device__ __forceinline__ float eos(const Physics &ph, float rho) {
     return ph.tait_b * (t * t * x - 1.0f) + ph.p0;
 }
 
+// sContinuity! (main.jl:92-105) on values in registers: rho += drho dt,
+// p = eos(rho); d = (u, v, rho, p) updated; returns the volume m / rho.
+__device__ __forceinline__ float continuity_eos_regs(float4 &d, float drho, float mass, const Physics &ph) {
+    const float rho = d.z + drho * ph.dt;
+    d.z = rho;
+    d.w = eos(ph, rho);
+    return mass / rho;
+}
+// sMomentum! (main.jl:120-126) on values in registers: du += g, then per
+// component x += (u + du dt / 2) dt and u += du dt.
+__device__ __forceinline__ void gravity_move_regs(float2 &x, float4 &d, float2 du, const Physics &ph) {
+    du.x += ph.gx;
+    du.y += ph.gy;
+    x.x += (d.x + du.x * ph.dt * 0.5f) * ph.dt;
+    d.x += du.x * ph.dt;
+    x.y += (d.y + du.y * ph.dt * 0.5f) * ph.dt;
+    d.y += du.y * ph.dt;
+}
+
 // Tags enter the neighbour walk as the bit 1 << (t - 1); a tag outside
 // 1..kMaxTag gets 0, which no tag is adjacent to.
 __device__ __forceinline__ uint32_t tag_bit(int t) {
@@ -693,7 +712,8 @@ __global__ __launch_bounds__(kBlock) void k_fill_members(Fields f, int32_t n, in
 __global__ __launch_bounds__(kBlock) void k_permute(Fields cur, Fields nxt, float4 *geo, int32_t n, int32_t batch,
                                                      int32_t n_cells, const int32_t *key, const int32_t *tmpslot,
                                                      const int32_t *start, const int32_t *members,
-                                                     int32_t *n_alive, uint32_t *flags, int32_t move_accum) {
+                                                     int32_t *n_alive, uint32_t *flags, int32_t move_accum,
+                                                     const float4 *dyn_src, const float *vol_src) {
     const int32_t s = blockIdx.x * kBlock + threadIdx.x;
     if (s == 0) {
         n_alive[0] = start[n_cells];      // alive slots
@@ -709,9 +729,9 @@ __global__ __launch_bounds__(kBlock) void k_permute(Fields cur, Fields nxt, floa
     const int32_t alive_in = cur.alive[s];
     const int32_t tmp = tmpslot[s];
     const float2 x = cur.pos[s];
-    const float4 dyn = cur.dyn[s];
+    const float4 dyn = (dyn_src ? dyn_src : cur.dyn)[s];
     const float2 dvel = cur.dvel[s];
-    const float mass = cur.mass[s], vol = cur.vol[s], drho = cur.drho[s], h = cur.h[s];
+    const float mass = cur.mass[s], vol = (vol_src ? vol_src : cur.vol)[s], drho = cur.drho[s], h = cur.h[s];
     // the filter and extrapolation sums, when they may be non-zero
     const float wv = move_accum ? cur.wv[s] : 0.0f, wvrho = move_accum ? cur.wvrho[s] : 0.0f;
     const float wvp = move_accum ? cur.wvp[s] : 0.0f;
@@ -800,11 +820,17 @@ struct ContinuityGather {
 // Search pass: find the neighbours of every alive particle and run the fused
 // action (EPPX_SEARCH_KERNEL_CONTINUITY: sComputeKernelAndiContinuity!,
 // main.jl:74-90).
-template <bool FAST, int ACTION, int K, bool BATCH, int BLOCK, int STAGE>
+// EOS (particle systems, in a time step, with the continuity action): the
+// epilogue also applies the next step's sContinuity! (main.jl:92-105) to the
+// particle's new dDensity -- rho, p and dDensity in place (the pair terms of
+// other threads read only (u, v) of a record), the volume to eo.vol_new
+// (they read V_j), for every slot -- and zeroes the cell counts of the next
+// step (the scan has consumed them).
+template <bool FAST, int ACTION, int K, bool BATCH, int BLOCK, int STAGE, bool EOS>
 __global__ __launch_bounds__(BLOCK) void k_search(Walk w, Fields f, Physics ph, int32_t M, int32_t *ncount,
                                                         unsigned long long *pairs_last,
                                                         unsigned long long *pairs_total, uint32_t *flags,
-                                                        int32_t which) {
+                                                        int32_t which, EosOut eo) {
     constexpr int kTileRow = PairShape<BLOCK>::tile_row;
     __shared__ int lds[BLOCK / 64];
     __shared__ float4 tile[3 * kTileRow];
@@ -813,8 +839,17 @@ __global__ __launch_bounds__(BLOCK) void k_search(Walk w, Fields f, Physics ph, 
     __shared__ int info[9];
     const int n = *w.n_alive;
     const int i0 = xcd_block(blockIdx.x, gridDim.x) * BLOCK;
+    if constexpr (EOS) {
+        for (int c = blockIdx.x * BLOCK + threadIdx.x; c < eo.n_counts; c += gridDim.x * BLOCK) eo.counts[c] = 0;
+        if (i0 >= n) {  // slots of dead particles only: their volumes as they are
+            const int s = i0 + threadIdx.x;
+            if (s < eo.n_slots) eo.vol_new[s] = f.vol[s];
+            return;
+        }
+    }
     if (i0 >= n) return;  // the whole block
     const int i = i0 + threadIdx.x;
+    if (EOS && i >= n && i < eo.n_slots) eo.vol_new[i] = f.vol[i];
     TileRows<STAGE> rows;
     bool all_fluid = false;  // every particle of the block's neighbourhood is fluid
     const bool tiled = ACTION == EPPX_SEARCH_KERNEL_CONTINUITY &&
@@ -884,7 +919,16 @@ __global__ __launch_bounds__(BLOCK) void k_search(Walk w, Fields f, Physics ph, 
             } else {
                 cnt = walk_rows<!FAST, true>(w, cd, GlobalRows{w.geo, w.tag}, i, gi, keep, fetch, term);
             }
-            f.drho[i] = acc;
+            if (EOS && (i_fluid || i_wall)) {
+                float4 d = di;
+                const float vol = continuity_eos_regs(d, acc, gi.w, ph);  // gi.w: Mass (the snapshot follows it)
+                f.drho[i] = 0.0f;
+                reinterpret_cast<float2 *>(f.dyn)[2 * (size_t)i + 1] = make_float2(d.z, d.w);  // (rho, p) half
+                eo.vol_new[i] = vol;
+            } else {
+                f.drho[i] = acc;
+                if (EOS) eo.vol_new[i] = f.vol[i];
+            }
         } else {
             walk(w, cd, i, gi, [&](int, uint32_t, float, float, float) { ++cnt; });
         }
@@ -978,8 +1022,11 @@ __device__ __forceinline__ void momentum_pair(const Physics &ph, float2 &acc, bo
                            m_j, dx, dy, dr2);
 }
 
-template <bool FAST, int K, int VISC, bool BATCH, int BLOCK, int STAGE>
-__global__ __launch_bounds__(BLOCK) void k_momentum(Walk w, Fields f, Physics ph) {
+// MOVE (particle systems, in a time step): the epilogue also runs
+// sMomentum! (gravity + accelerate-move) on the result and takes the cell key
+// and count of every slot for the coming search (MoveKey).
+template <bool FAST, int K, int VISC, bool BATCH, int BLOCK, int STAGE, bool MOVE>
+__global__ __launch_bounds__(BLOCK) void k_momentum(Walk w, Fields f, Physics ph, MoveKey mk) {
     constexpr int kTileRow = PairShape<BLOCK>::tile_row;
     __shared__ float4 tile[3 * kTileRow];
     __shared__ uint8_t tix[3 * kTileRow];
@@ -987,20 +1034,36 @@ __global__ __launch_bounds__(BLOCK) void k_momentum(Walk w, Fields f, Physics ph
     __shared__ int info[9];
     const int n = *w.n_alive;
     const int i0 = xcd_block(blockIdx.x, gridDim.x) * BLOCK;
-    if (i0 >= n) return;  // the whole block
+    if (!MOVE && i0 >= n) return;  // the whole block
+    if (MOVE && blockIdx.x == 0 && threadIdx.x == 0) *mk.pairs_last = 0ull;
     TileRows<STAGE> rows;
     bool all_fluid = false;  // every particle of the block's neighbourhood is fluid
-    const bool tiled =
-        stage_rows<BLOCK, STAGE>(w, f.dyn, i0, n, tile, tix, tdyn, nullptr, info, rows, ph.fluid_tag, all_fluid);
+    bool tiled = false;
+    if (i0 < n)  // the same for the whole block
+        tiled = stage_rows<BLOCK, STAGE>(w, f.dyn, i0, n, tile, tix, tdyn, nullptr, info, rows, ph.fluid_tag,
+                                         all_fluid);
     const int i = i0 + threadIdx.x;
-    if (i >= n) return;
-    if (f.tag[i] != ph.fluid_tag) return;  // iMomentum! acts on fluid particles only
-    const CaseDesc cd = BATCH ? w.cases[w.cs[i]] : w.one;
-    if (cd.steps_left <= 0) return;  // a batch case that has finished
+    if (!MOVE && i >= n) return;
+    const CaseDesc cd = BATCH ? w.cases[w.cs[min(i, n - 1)]] : w.one;
+    // iMomentum! acts on fluid particles only (of cases that advance)
+    const bool act = i < n && f.tag[i] == ph.fluid_tag && cd.steps_left > 0;
+    if (!MOVE && !act) return;
+    float4 di = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float2 acc = make_float2(0.0f, 0.0f);
+    // MOVE: the epilogue's loads are issued now, so they are in flight during the walk
+    int32_t e_alive = 0, e_mov = 0, e_cell = 0;
+    float2 e_x = make_float2(0.0f, 0.0f);
+    if (MOVE && i < mk.n_slots) {
+        e_alive = f.alive[i];
+        e_mov = f.mov[i];
+        e_cell = f.cell[i];
+        e_x = f.pos[i];
+    }
+    if (act) {
     const float4 gi = w.geo[i];
-    const float4 di = f.dyn[i];
+    di = f.dyn[i];
     const float2 ui = make_float2(di.x, di.y);
-    float2 acc = f.dvel[i];
+    acc = f.dvel[i];
     const float pr_i = FAST ? di.w * frcp(di.z * di.z) : 0.0f;
     const uint32_t fluid_bit = tag_bit(ph.fluid_tag);
     auto fetch = [&](int j, int k, uint32_t, const auto &src) -> float4 {
@@ -1039,7 +1102,30 @@ __global__ __launch_bounds__(BLOCK) void k_momentum(Walk w, Fields f, Physics ph
     } else {
         walk_rows<!FAST, false>(w, cd, GlobalRows{w.geo, w.tag}, i, gi, keep, fetch, term);
     }
-    f.dvel[i] = acc;
+    if (!MOVE) f.dvel[i] = acc;
+    }
+    if constexpr (MOVE) {
+        // The neighbours of other threads still read this slot's record, so
+        // the moved one goes to dyn_new; positions, dvel, cell and alive are
+        // not read by the pair terms.
+        const bool valid = i < mk.n_slots;
+        int32_t k = -1;
+        if (valid) {
+            const int32_t alive = e_alive, mov = e_mov, cell = e_cell;
+            float2 x = e_x;
+            float4 d = act ? di : f.dyn[i];
+            if (act) {
+                gravity_move_regs(x, d, acc, ph);  // du = the momentum result
+                f.pos[i] = x;
+                f.dvel[i] = make_float2(0.0f, 0.0f);
+            }
+            mk.dyn_new[i] = d;
+            k = cell_key_of(f, i, cd, mk.n_cells, alive, mov, cell, x);
+            mk.key[i] = k;
+        }
+        const int32_t slot = count_key(valid, k, mk.n_cells, mk.counts, mk.flags, mk.max_per_cell);
+        if (valid) mk.tmpslot[i] = slot;
+    }
 }
 
 // Other interactions over the last search's lists (device_interaction!,
@@ -1126,13 +1212,6 @@ __global__ __launch_bounds__(kBlock) void k_interaction(Walk w, Fields f, Physic
 //    wall particles: p = wv_p / wv and both sums cleared when wv > 0, else p = p0;
 //  EPPX_SELF_SCALE_FIELD: field[I] *= value.
 // sContinuity! (main.jl:92-105): rho += drho dt, drho = 0, V = m / rho, p = eos(rho).
-// On values in registers: d = (u, v, rho, p) updated; returns the volume.
-__device__ __forceinline__ float continuity_eos_regs(float4 &d, float drho, float mass, const Physics &ph) {
-    const float rho = d.z + drho * ph.dt;
-    d.z = rho;
-    d.w = eos(ph, rho);
-    return mass / rho;
-}
 __device__ __forceinline__ void continuity_eos_one(Fields &f, int s, int tag, const Physics &ph) {
     if (tag != ph.fluid_tag && tag != ph.wall_tag) return;
     float4 d = f.dyn[s];
@@ -1144,15 +1223,6 @@ __device__ __forceinline__ void continuity_eos_one(Fields &f, int s, int tag, co
 
 // sMomentum! (main.jl:120-126): du += g, then x += (u + du dt / 2) dt and
 // u += du dt per component, du = 0.
-// sGravity! + sAccelerateMove! on values in registers.
-__device__ __forceinline__ void gravity_move_regs(float2 &x, float4 &d, float2 du, const Physics &ph) {
-    du.x += ph.gx;
-    du.y += ph.gy;
-    x.x += (d.x + du.x * ph.dt * 0.5f) * ph.dt;
-    d.x += du.x * ph.dt;
-    x.y += (d.y + du.y * ph.dt * 0.5f) * ph.dt;
-    d.y += du.y * ph.dt;
-}
 __device__ __forceinline__ void gravity_move_one(Fields &f, int s, int tag, const Physics &ph) {
     if (tag != ph.fluid_tag) return;
     float2 x = f.pos[s];
@@ -1514,9 +1584,9 @@ hipError_t launch_fill_members(const Fields f, int32_t n, int32_t n_cells, const
 hipError_t launch_permute(Fields cur, Fields nxt, float4 *geo, int32_t n, int32_t batch, int32_t n_cells,
                           const int32_t *key, const int32_t *tmpslot, const int32_t *start,
                           const int32_t *members, int32_t *n_alive, uint32_t *flags, int32_t move_accum,
-                          hipStream_t s) {
+                          const float4 *dyn_src, const float *vol_src, hipStream_t s) {
     hipLaunchKernelGGL(k_permute, grid_for(n), dim3(kBlock), 0, s, cur, nxt, geo, n, batch, n_cells, key, tmpslot,
-                       start, members, n_alive, flags, move_accum);
+                       start, members, n_alive, flags, move_accum, dyn_src, vol_src);
     return hipGetLastError();
 }
 
@@ -1526,12 +1596,13 @@ static void launch_search_kb(const Walk &w, Fields f, const Physics &ph, int32_t
                              int32_t which, hipStream_t s) {
     constexpr int KCONT = EPPX_SEARCH_KERNEL_CONTINUITY;
     const dim3 g = grid_for(n, BLOCK);
+    const EosOut none{};
     if (w.cases)
-        hipLaunchKernelGGL((k_search<FAST, KCONT, K, true, BLOCK, STAGE>), g, dim3(BLOCK), 0, s, w, f, ph, M, ncount,
-                           pairs_last, pairs_total, flags, which);
+        hipLaunchKernelGGL((k_search<FAST, KCONT, K, true, BLOCK, STAGE, false>), g, dim3(BLOCK), 0, s, w, f, ph, M,
+                           ncount, pairs_last, pairs_total, flags, which, none);
     else
-        hipLaunchKernelGGL((k_search<FAST, KCONT, K, false, BLOCK, STAGE>), g, dim3(BLOCK), 0, s, w, f, ph, M, ncount,
-                           pairs_last, pairs_total, flags, which);
+        hipLaunchKernelGGL((k_search<FAST, KCONT, K, false, BLOCK, STAGE, false>), g, dim3(BLOCK), 0, s, w, f, ph, M,
+                           ncount, pairs_last, pairs_total, flags, which, none);
 }
 
 // FAST kernels come in every tuned shape (256 or 512 threads; (u, v) staged
@@ -1567,13 +1638,42 @@ hipError_t launch_search(const Walk &w, Fields f, int32_t n, int32_t action, con
             launch_search_k<false, KW>(w, f, ph, n, M, ncount, pairs_last, pairs_total, flags, which, block, stage, s);
     } else {
         const dim3 g = grid_for(n, 256);
+        const EosOut none{};
         if (w.cases)
-            hipLaunchKernelGGL((k_search<false, EPPX_SEARCH_NONE, KW, true, 256, 0>), g, dim3(256), 0, s, w, f, ph, M,
-                               ncount, pairs_last, pairs_total, flags, which);
+            hipLaunchKernelGGL((k_search<false, EPPX_SEARCH_NONE, KW, true, 256, 0, false>), g, dim3(256), 0, s, w, f,
+                               ph, M, ncount, pairs_last, pairs_total, flags, which, none);
         else
-            hipLaunchKernelGGL((k_search<false, EPPX_SEARCH_NONE, KW, false, 256, 0>), g, dim3(256), 0, s, w, f, ph, M,
-                               ncount, pairs_last, pairs_total, flags, which);
+            hipLaunchKernelGGL((k_search<false, EPPX_SEARCH_NONE, KW, false, 256, 0, false>), g, dim3(256), 0, s, w, f,
+                               ph, M, ncount, pairs_last, pairs_total, flags, which, none);
     }
+    return hipGetLastError();
+}
+
+template <bool FAST, int K, int BLOCK>
+static void launch_search_eos_kb(const Walk &w, Fields f, const Physics &ph, const EosOut &eo, int32_t n, int32_t M,
+                                 int32_t *ncount, unsigned long long *pairs_last, unsigned long long *pairs_total,
+                                 uint32_t *flags, hipStream_t s) {
+    hipLaunchKernelGGL((k_search<FAST, EPPX_SEARCH_KERNEL_CONTINUITY, K, false, BLOCK, 0, true>), grid_for(n, BLOCK),
+                       dim3(BLOCK), 0, s, w, f, ph, M, ncount, pairs_last, pairs_total, flags, 0, eo);
+}
+
+hipError_t launch_search_eos(const Walk &w, Fields f, int32_t n, const Physics &ph, const EosOut &eo, int32_t fast,
+                             int32_t M, int32_t *ncount, unsigned long long *pairs_last,
+                             unsigned long long *pairs_total, uint32_t *flags, int32_t block, hipStream_t s) {
+    constexpr int KC = EPPX_KERNEL_CUBIC, KW = EPPX_KERNEL_WENDLAND_C2;
+    const bool cubic = ph.kernel == KC;
+    if (fast && block == 512 && cubic)
+        launch_search_eos_kb<true, KC, 512>(w, f, ph, eo, n, M, ncount, pairs_last, pairs_total, flags, s);
+    else if (fast && block == 512)
+        launch_search_eos_kb<true, KW, 512>(w, f, ph, eo, n, M, ncount, pairs_last, pairs_total, flags, s);
+    else if (fast && cubic)
+        launch_search_eos_kb<true, KC, 256>(w, f, ph, eo, n, M, ncount, pairs_last, pairs_total, flags, s);
+    else if (fast)
+        launch_search_eos_kb<true, KW, 256>(w, f, ph, eo, n, M, ncount, pairs_last, pairs_total, flags, s);
+    else if (cubic)
+        launch_search_eos_kb<false, KC, 256>(w, f, ph, eo, n, M, ncount, pairs_last, pairs_total, flags, s);
+    else
+        launch_search_eos_kb<false, KW, 256>(w, f, ph, eo, n, M, ncount, pairs_last, pairs_total, flags, s);
     return hipGetLastError();
 }
 
@@ -1603,12 +1703,13 @@ hipError_t launch_self(Fields f, int32_t n, int32_t action, const Physics &ph, c
 template <bool FAST, int K, bool BATCH, int BLOCK, int STAGE>
 static void launch_momentum_b(const Walk &w, Fields f, const Physics &ph, int32_t n, hipStream_t s) {
     const dim3 g = grid_for(n, BLOCK);
+    const MoveKey none{};
     if (ph.visc == EPPX_VISC_CLASSIC)
-        hipLaunchKernelGGL((k_momentum<FAST, K, EPPX_VISC_CLASSIC, BATCH, BLOCK, STAGE>), g, dim3(BLOCK), 0, s, w, f,
-                           ph);
+        hipLaunchKernelGGL((k_momentum<FAST, K, EPPX_VISC_CLASSIC, BATCH, BLOCK, STAGE, false>), g, dim3(BLOCK), 0, s,
+                           w, f, ph, none);
     else
-        hipLaunchKernelGGL((k_momentum<FAST, K, EPPX_VISC_ARTIFICIAL, BATCH, BLOCK, STAGE>), g, dim3(BLOCK), 0, s, w,
-                           f, ph);
+        hipLaunchKernelGGL((k_momentum<FAST, K, EPPX_VISC_ARTIFICIAL, BATCH, BLOCK, STAGE, false>), g, dim3(BLOCK), 0,
+                           s, w, f, ph, none);
 }
 
 template <bool FAST, int K, bool BATCH>
@@ -1625,6 +1726,31 @@ static void launch_momentum_v(const Walk &w, Fields f, const Physics &ph, int32_
                               hipStream_t s) {
     if (w.cases) launch_momentum_s<FAST, K, true>(w, f, ph, n, block, stage, s);
     else launch_momentum_s<FAST, K, false>(w, f, ph, n, block, stage, s);
+}
+
+template <bool FAST, int K, int BLOCK>
+static void launch_momentum_move_kb(const Walk &w, Fields f, const Physics &ph, int32_t n, const MoveKey &mk,
+                                    hipStream_t s) {
+    const dim3 g = grid_for(n, BLOCK);
+    if (ph.visc == EPPX_VISC_CLASSIC)
+        hipLaunchKernelGGL((k_momentum<FAST, K, EPPX_VISC_CLASSIC, false, BLOCK, 0, true>), g, dim3(BLOCK), 0, s, w, f,
+                           ph, mk);
+    else
+        hipLaunchKernelGGL((k_momentum<FAST, K, EPPX_VISC_ARTIFICIAL, false, BLOCK, 0, true>), g, dim3(BLOCK), 0, s, w,
+                           f, ph, mk);
+}
+
+hipError_t launch_momentum_move(const Walk &w, Fields f, int32_t n, const Physics &ph, const MoveKey &mk,
+                                int32_t fast, int32_t block, hipStream_t s) {
+    constexpr int KC = EPPX_KERNEL_CUBIC, KW = EPPX_KERNEL_WENDLAND_C2;
+    const bool cubic = ph.kernel == KC;
+    if (fast && block == 512 && cubic) launch_momentum_move_kb<true, KC, 512>(w, f, ph, n, mk, s);
+    else if (fast && block == 512) launch_momentum_move_kb<true, KW, 512>(w, f, ph, n, mk, s);
+    else if (fast && cubic) launch_momentum_move_kb<true, KC, 256>(w, f, ph, n, mk, s);
+    else if (fast) launch_momentum_move_kb<true, KW, 256>(w, f, ph, n, mk, s);
+    else if (cubic) launch_momentum_move_kb<false, KC, 256>(w, f, ph, n, mk, s);
+    else launch_momentum_move_kb<false, KW, 256>(w, f, ph, n, mk, s);
+    return hipGetLastError();
 }
 
 hipError_t launch_interaction(const Walk &w, Fields f, int32_t n, int32_t action, const Physics &ph,

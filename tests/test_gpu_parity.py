@@ -236,3 +236,25 @@ def test_zero_particles(parallel, small):
     Algorithm.step(ps, ns, small.physics, n_steps=2)
     off, js = U.export_pairs(ps, ns)
     assert off.tolist() == [0] and js.size == 0
+
+
+def test_steps_split_over_calls_exact(parallel, small):
+    """The same 12 steps (filter every 4) in calls of 1, 3, 1 and 7 steps: a
+    call's first step applies sContinuity! itself, later ones find it applied
+    by the previous search, the last leaves it pending -- bit-identical to the
+    restatement, with the same pairs."""
+    from eppx import Algorithm
+    ph = small.physics
+    oph = U.oracle_physics(ph)
+    ps, ns, ref = U.search_both(parallel, small)
+    done = 0
+    for n in (1, 3, 1, 7):
+        Algorithm.step(ps, ns, ph, n_steps=n, filter_interval=4, step_offset=done)
+        done += n
+    err, pairs = ref.run_steps(12, oph, filter_interval=4)
+    assert err == 0
+    host = U.download(ps)
+    U.assert_state_equal(host, ref, names=U.STATE_FLOAT + ("SumWeight", "SumWeightedDensity"))
+    U.assert_neighbour_columns_equal(host, ref)
+    off, _ = U.export_pairs(ps, ns)
+    assert off[-1] == pairs[-1]
