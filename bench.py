@@ -33,10 +33,14 @@ HBM_PEAK_GBS = 8000.0
 # the cell-order move are implementation cost and are not counted), plus the
 # cell-start array read once per cell.
 ALG_BYTES = {
-    # pos_nb 8, vel 8, mass 4, vol 4, rho 4, drho 4 (rmw), tag 4, h 4, cell 4 in; drho 4, count 4 out
-    "search": (52, 4),
-    # pos_nb 8, vel 8, rho 4, p 4, mass 4, tag 4, h 4, cell 4, dvel 8 in; dvel 8 out
-    "momentum": (56, 4),
+    # the search with the next step's continuity fused (every step of a call but its last):
+    # pos_nb 8, vel 8, mass 4, vol 4, rho 4, drho 4 (rmw), tag 4, h 4, cell 4 in; drho 4, count 4 out;
+    # rho 4, p 4, vol 4 out; 4 B/cell of cell table read and 4 B/cell of counts zeroed
+    "search": (64, 8),
+    # momentum with gravity + move and the cell keys fused: pos_nb 8, vel 8, rho 4, p 4, mass 4, tag 4,
+    # h 4, cell 4, dvel 8 in; dvel 8 out; then pos 8, alive 4, mov 4, cell 4 in; pos 8, vel 8, key 4,
+    # slot 4 out; 4 B/cell of counts
+    "momentum": (100, 4),
     # alive 4, tag 4, rho 4, drho 4, mass 4 in; rho 4, drho 4, vol 4, p 4 out
     "continuity_eos": (36, 0),
     # alive 4, tag 4, dvel 8, pos 8, vel 8 in; dvel 8, pos 8, vel 8 out
@@ -307,6 +311,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sample length")
     ap.add_argument("--profile-steps", type=int, default=50, help="eager steps timed kernel by kernel")
     ap.add_argument("--tune-trials", type=int, default=3, help="autotuner trials per block size (0: no tuning)")
+    ap.add_argument("--pair-shape", default=None, metavar="SB,SS,MB,MS",
+                    help="pin the pair-kernel shapes instead of tuning: search block, staged, momentum block, staged")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run device copies of BYTES (profiler calibration)")
@@ -345,7 +351,12 @@ def main():
     L = _lib.lib()
     Algorithm.step(ps, ns, case.physics, n_steps=args.warmup)
     log(f"warmup {args.warmup} steps done")
-    tune = Algorithm.autotune(ps, ns, case.physics, trials=args.tune_trials) if args.tune_trials else None
+    if args.pair_shape:
+        sb, ss, mb, ms = (int(v) for v in args.pair_shape.split(","))
+        Algorithm.set_pair_shape(ps, search=(sb, ss), momentum=(mb, ms))
+        tune = None
+    else:
+        tune = Algorithm.autotune(ps, ns, case.physics, trials=args.tune_trials) if args.tune_trials else None
     if tune:
         log(f"autotune: search {tune['search']}, momentum {tune['momentum']} "
             f"(us: {tune['search_us']}, {tune['momentum_us']})")

@@ -1144,8 +1144,8 @@ int32_t eppx_step(eppx_ps *ps, eppx_ns *ns, const eppx_physics *phys, int32_t n_
     // cell keys in its epilogue, and the search may run the next step's
     // sContinuity! in its own; their outputs for the permute go to dyn_new /
     // vol_new.
-    const bool fusable = ps->search_stage == 0 && ps->momentum_stage == 0;
-    if (fusable && !ps->dyn_new) {  // no allocation while a step is recorded
+    const bool fuse_move = ps->momentum_stage == 0, fuse_eos_ok = ps->search_stage == 0;
+    if ((fuse_move || fuse_eos_ok) && !ps->dyn_new) {  // no allocation while a step is recorded
         HIP_OK(ctx, dalloc(&ps->dyn_new, (size_t)ps->cap));
         HIP_OK(ctx, dalloc(&ps->vol_new, (size_t)ps->cap));
     }
@@ -1158,22 +1158,22 @@ int32_t eppx_step(eppx_ps *ps, eppx_ns *ns, const eppx_physics *phys, int32_t n_
             HIP_OK(ctx, timed(ctx, "continuity_eos", [&] {
                        return launch_eos_reset(ps->f(), ps->n, ph, nullptr, ps->counts, nc + 2, &ps->pairs[0], s);
                    }));
-        if (fusable) {
+        if (fuse_move) {
             const MoveKey mk{ps->n, nc, ns->M, ps->counts, ps->key, ps->tmpslot, ctx->d_flags, ps->dyn_new,
                              &ps->pairs[0]};
             if ((r = do_momentum_move(ps, ns, ph, mk))) return r;
-            const EosOut eo{ps->n, nc + 2, ps->vol_new, ps->counts};
-            if ((r = do_search(ps, ns, EPPX_CRIT_SYMMETRY, EPPX_SEARCH_KERNEL_CONTINUITY, ph, true, ps->dyn_new,
-                               eos_first ? nullptr : ps->vol_new, fuse_eos ? &eo : nullptr)))
-                return r;
         } else {
             if ((r = do_interaction(ps, ns, EPPX_INTER_MOMENTUM, ph, op))) return r;
             HIP_OK(ctx, timed(ctx, "gravity_move", [&] {
                        return launch_move_key(ps->f(), ps->n, ph, case_of(ps, ns, EPPX_CRIT_SYMMETRY), nullptr, nc,
                                               ps->counts, ps->key, ps->tmpslot, ctx->d_flags, ns->M, s);
                    }));
-            if ((r = do_search(ps, ns, EPPX_CRIT_SYMMETRY, EPPX_SEARCH_KERNEL_CONTINUITY, ph, true))) return r;
         }
+        const EosOut eo{ps->n, nc + 2, ps->vol_new, ps->counts};
+        if ((r = do_search(ps, ns, EPPX_CRIT_SYMMETRY, EPPX_SEARCH_KERNEL_CONTINUITY, ph, true,
+                           fuse_move ? ps->dyn_new : nullptr, eos_first ? nullptr : ps->vol_new,
+                           fuse_eos ? &eo : nullptr)))
+            return r;
         if (filter_now) {
             if ((r = do_interaction(ps, ns, EPPX_INTER_FILTER, ph, op))) return r;
             if ((r = do_self(ps, EPPX_SELF_FILTER, ph, op))) return r;
@@ -1185,7 +1185,7 @@ int32_t eppx_step(eppx_ps *ps, eppx_ns *ns, const eppx_physics *phys, int32_t n_
         const bool filter_now = filter_interval > 0 && (step_offset + k) % filter_interval == 0;
         // the last step of the call leaves the state as the reference's loop
         // body does (its dDensity not yet applied); a filter step too
-        const bool fuse = fusable && k < n_steps && !filter_now;
+        const bool fuse = fuse_eos_ok && k < n_steps && !filter_now;
         const bool eos_first = eos_pending;
         eos_pending = !fuse;
         if (filter_now || ctx->timing || !ps->masks_valid) {  // filter steps and per-kernel timing run eagerly
