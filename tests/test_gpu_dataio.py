@@ -45,3 +45,39 @@ def test_config_driven_case_matches_direct(parallel, small, tmp_path):
     err, _ = ref.run_steps(n, U.oracle_physics(small.physics))
     assert err == 0
     U.assert_state_equal(b, ref)
+
+
+def test_collapse_dry_driver_matches_direct_run(parallel, tmp_path):
+    """examples/collapse_dry.py (casegen, then main with the filter every 20
+    steps and an output every 40): the written state equals a direct run."""
+    import importlib.util
+    import os
+    from eppx import Algorithm, DataIO, SPH, cases
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "etherparallelparticles.jl_amd",
+                        "examples", "collapse_dry.py")
+    spec = importlib.util.spec_from_file_location("collapse_dry", path)
+    ex = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(ex)
+    out = str(tmp_path / "run")
+    ex.casegen(out, dx=0.02)
+    app = ex.main(out, 60, filter_interval=20, output_interval=40, log=lambda *_: None)
+    assert app["TMSTEP"] == 40 and app["WriteStep"] == 1  # the output of step 40
+    case = cases.dam_break(0.02, 1.5, SPH.CubicSpline, M=40)
+    ps, ns = U.device(parallel, case)
+    Algorithm.search(ps, case.domain, ns, criterion=Algorithm.symmetrySearchCriterion,
+                     action=SPH.sComputeKernelAndiContinuity(case.physics))
+    Algorithm.step(ps, ns, case.physics, n_steps=40, filter_interval=20)
+    direct = U.download(ps)
+    w = DataIO.WriterFrom(DataIO.parse(os.path.join(out, "config", "config.json")))
+    saved = Class_host_like(direct)
+    DataIO.load_into(w, saved, 1)
+    keep = direct.is_alive_[: direct.n_particles_] == 1
+    assert np.array_equal(saved.float_properties_[: saved.n_particles_].view(np.uint32),
+                          direct.float_properties_[: direct.n_particles_][keep].view(np.uint32))
+    assert np.array_equal(saved.int_properties_[: saved.n_particles_],
+                          direct.int_properties_[: direct.n_particles_][keep])
+
+
+def Class_host_like(h):
+    from eppx import Class
+    return Class.HostParticleSystem(h.n_capacity, h.named_index_, h.basic_parameters_)
