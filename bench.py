@@ -309,6 +309,8 @@ def main():
     ap.add_argument("--serve-chunk", type=int, default=100, help="largest number of steps between polls")
     ap.add_argument("--math", choices=["exact", "fast"], default="fast")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sample length")
+    ap.add_argument("--cpu-seconds-1t", type=float, default=5.0,
+                    help="single-thread CPU baseline sample length (0: skip)")
     ap.add_argument("--profile-steps", type=int, default=50, help="eager steps timed kernel by kernel")
     ap.add_argument("--tune-trials", type=int, default=3, help="autotuner trials per block size (0: no tuning)")
     ap.add_argument("--pair-shape", default=None, metavar="SB,SS,MB,MS",
@@ -416,6 +418,11 @@ def main():
                "sample": f"{nsteps} steps of the same {case.n_particles}-particle dam break after one "
                          f"warm-up step, C/OpenMP restatement of the reference algorithm on the reference layout "
                          f"(per-cell atomic insert, materialised M=64 neighbour lists), {secs:.1f} s"}
+        if args.cpu_seconds_1t > 0 and threads > 1:
+            rate1, secs1, n1 = cpu_baseline(case, args.cpu_seconds_1t, 1)
+            cpu["single_thread"] = {"value": rate1, "cores": 1,
+                                    "sample": f"{n1} steps after one warm-up step on one thread, {secs1:.1f} s"}
+            log(f"cpu baseline: {rate / 1e9:.3f} G/s on {threads} threads, {rate1 / 1e9:.3f} G/s on 1")
 
     ms = 1000.0 * elapsed / args.steps
     out = {
