@@ -467,60 +467,80 @@ __device__ __forceinline__ int walk_rows(const Walk &w, const CaseDesc &cd, cons
 }
 
 // Stages the geometry of a pair block's neighbourhood in LDS.  The block's
-// sorted slots [i0, i0 + BLOCK) lie in cells c0 <= c <= c1; when these
-// belong to one case and share one cell row, the three neighbour rows of
-// every particle of the block are sub-ranges of the rows' slot ranges over
-// cells c0 - 1 .. c1 + 1, which are copied to tile (tile_row slots per row).
-// Returns false, with nothing staged, when the block spans two cases or two
-// cell rows or a row range does not fit; the block then reads global memory.
-// only_tag: set when every staged slot (the block's own particles among them)
-// carries the tag `tag`.  Every thread of the block calls it (two barriers).
+// sorted slots [i0, i0 + BLOCK) lie in cells c0 <= c <= c1 of one case, in
+// one cell row or in two consecutive ones.  The slots of each cell row (a
+// group) have their three neighbour rows inside the rows' slot ranges over
+// the group's cells widened by one on either side; those ranges (three per
+// group, at most six) are copied to the tile one after another, and each
+// thread's row offsets point into its own group's ranges.  Returns false,
+// with nothing staged, when the block spans two cases or more than two cell
+// rows or the ranges do not fit the tile; the block then reads global
+// memory.  only_tag: set when every staged slot (the block's own particles
+// among them) carries the tag `tag`.  info: kStageInfo ints of LDS.  Every
+// thread of the block calls it (two barriers).
+constexpr int kStageInfo = 16;
 template <int BLOCK, int STAGE>
 __device__ __forceinline__ bool stage_rows(const Walk &w, const float4 *dyn, int i0, int n, float4 *tile,
                                            uint8_t *tix, float4 *tdyn, float2 *tuv, int *info,
                                            TileRows<STAGE> &rows, int tag, bool &only_tag) {
-    constexpr int kTileRow = PairShape<BLOCK>::tile_row;
+    constexpr int kCap = 3 * PairShape<BLOCK>::tile_row;
     const int t = threadIdx.x;
-    if (t < 3) {
+    if (t < 6) {
+        const int g = t / 3, r = t % 3;  // group (cell row) and neighbour row
         const int last = min(i0 + BLOCK, n) - 1;
         const CaseDesc cd = w.cases ? w.cases[w.cs[i0]] : w.one;
         const int nx = cd.g.n_x;
         const int c0 = w.cell[i0] - 1, c1 = w.cell[last] - 1;
-        const int cy = c0 / nx, jj = cy + t - 1;
-        int b = 0, len = 0, ok = c1 / nx == cy && (!w.cases || w.cs[last] == w.cs[i0]);
-        if (ok && jj >= 0 && jj < cd.g.n_y) {
-            b = w.start[cd.cell_base + max(c0 % nx - 1, 0) + nx * jj];
-            len = w.start[cd.cell_base + min(c1 % nx + 1, nx - 1) + 1 + nx * jj] - b;
-            ok = len <= kTileRow;
+        const int cy0 = c0 / nx, cy1 = c1 / nx;
+        const int ok = (cy1 == cy0 || cy1 == cy0 + 1) && (!w.cases || w.cs[last] == w.cs[i0]);
+        int b = 0, len = 0;
+        if (ok && (g == 0 || cy1 != cy0)) {
+            const int xa = g == 0 ? c0 % nx : 0;                          // the group's first x cell
+            const int xb = (g == 1 || cy1 == cy0) ? c1 % nx : nx - 1;     // and its last
+            const int jj = cy0 + g + r - 1;
+            if (jj >= 0 && jj < cd.g.n_y) {
+                b = w.start[cd.cell_base + max(xa - 1, 0) + nx * jj];
+                len = w.start[cd.cell_base + min(xb + 1, nx - 1) + 1 + nx * jj] - b;
+            }
         }
         info[t] = b;
-        info[3 + t] = len;
-        info[6 + t] = ok;
+        info[6 + t] = len;
+        if (t == 0) {
+            info[12] = ok;
+            info[13] = cy0;
+            info[14] = nx;
+        }
     }
     __syncthreads();
-    const bool ok = info[6] && info[7] && info[8];
+    // block-uniform: range starts and lengths, and where each range goes in the tile
+    auto u = [&](int k) { return __builtin_amdgcn_readfirstlane(info[k]); };
+    const int d1 = u(6), d2 = d1 + u(7), d3 = d2 + u(8), d4 = d3 + u(9), d5 = d4 + u(10), total = d5 + u(11);
+    const bool ok = u(12) && total <= kCap;
     int other = 0;  // this thread staged a slot of another tag
     if (ok) {
 #pragma unroll
-        for (int r = 0; r < 3; ++r) {
-            const int b = info[r], len = info[3 + r];
-            for (int k = t; k < len; k += BLOCK) {
-                tile[r * kTileRow + k] = w.geo[b + k];
-                const int tg = w.tag[b + k];
+        for (int k = 0; k < 6; ++k) {
+            const int b = u(k), len = u(6 + k);
+            const int d = k == 0 ? 0 : k == 1 ? d1 : k == 2 ? d2 : k == 3 ? d3 : k == 4 ? d4 : d5;
+            for (int m = t; m < len; m += BLOCK) {
+                tile[d + m] = w.geo[b + m];
+                const int tg = w.tag[b + m];
                 other |= tg != tag;
                 const uint32_t t1 = (uint32_t)tg - 1u;
-                tix[r * kTileRow + k] = (uint8_t)(t1 < (uint32_t)kMaxTag ? t1 : 63u);
-                if (STAGE == 1) tdyn[r * kTileRow + k] = dyn[b + k];
-                if (STAGE == 2) tuv[r * kTileRow + k] = reinterpret_cast<const float2 *>(dyn)[2 * (size_t)(b + k)];
+                tix[d + m] = (uint8_t)(t1 < (uint32_t)kMaxTag ? t1 : 63u);
+                if (STAGE == 1) tdyn[d + m] = dyn[b + m];
+                if (STAGE == 2) tuv[d + m] = reinterpret_cast<const float2 *>(dyn)[2 * (size_t)(b + m)];
             }
         }
+        const int i = min(i0 + t, n - 1);
+        const bool second = (w.cell[i] - 1) / u(14) != u(13);  // this thread's slot is in the second row
         rows.tile = tile;
         rows.tix = tix;
         rows.dyn = tdyn;
         rows.uv = tuv;
-        rows.off0 = -info[0];
-        rows.off1 = kTileRow - info[1];
-        rows.off2 = 2 * kTileRow - info[2];
+        rows.off0 = second ? d3 - u(3) : -u(0);
+        rows.off1 = second ? d4 - u(4) : d1 - u(1);
+        rows.off2 = second ? d5 - u(5) : d2 - u(2);
     } else if (t == 0) {
         atomicAdd(w.fallback, 1u);
     }
@@ -836,7 +856,7 @@ __global__ __launch_bounds__(BLOCK) void k_search(Walk w, Fields f, Physics ph, 
     __shared__ float4 tile[3 * kTileRow];
     __shared__ uint8_t tix[3 * kTileRow];
     __shared__ float2 tuv[STAGE == 2 ? 3 * kTileRow : 1];
-    __shared__ int info[9];
+    __shared__ int info[kStageInfo];
     const int n = *w.n_alive;
     const int i0 = xcd_block(blockIdx.x, gridDim.x) * BLOCK;
     if constexpr (EOS) {
@@ -1031,7 +1051,7 @@ __global__ __launch_bounds__(BLOCK) void k_momentum(Walk w, Fields f, Physics ph
     __shared__ float4 tile[3 * kTileRow];
     __shared__ uint8_t tix[3 * kTileRow];
     __shared__ float4 tdyn[STAGE == 1 ? 3 * kTileRow : 1];
-    __shared__ int info[9];
+    __shared__ int info[kStageInfo];
     const int n = *w.n_alive;
     const int i0 = xcd_block(blockIdx.x, gridDim.x) * BLOCK;
     if (!MOVE && i0 >= n) return;  // the whole block
