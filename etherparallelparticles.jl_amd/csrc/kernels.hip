@@ -1070,15 +1070,7 @@ __global__ __launch_bounds__(BLOCK) void k_momentum(Walk w, Fields f, Physics ph
     if (!MOVE && !act) return;
     float4 di = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     float2 acc = make_float2(0.0f, 0.0f);
-    // MOVE: the epilogue's loads are issued now, so they are in flight during the walk
-    int32_t e_alive = 0, e_mov = 0, e_cell = 0;
-    float2 e_x = make_float2(0.0f, 0.0f);
-    if (MOVE && i < mk.n_slots) {
-        e_alive = f.alive[i];
-        e_mov = f.mov[i];
-        e_cell = f.cell[i];
-        e_x = f.pos[i];
-    }
+
     if (act) {
     const float4 gi = w.geo[i];
     di = f.dyn[i];
@@ -1131,9 +1123,9 @@ __global__ __launch_bounds__(BLOCK) void k_momentum(Walk w, Fields f, Physics ph
         const bool valid = i < mk.n_slots;
         int32_t k = -1;
         if (valid) {
-            const int32_t alive = e_alive, mov = e_mov, cell = e_cell;
-            float2 x = e_x;
-            float4 d = act ? di : f.dyn[i];
+            const int32_t alive = f.alive[i], mov = f.mov[i], cell = f.cell[i];
+            float2 x = f.pos[i];
+            float4 d = f.dyn[i];
             if (act) {
                 gravity_move_regs(x, d, acc, ph);  // du = the momentum result
                 f.pos[i] = x;
