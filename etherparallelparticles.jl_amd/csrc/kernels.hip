@@ -847,7 +847,7 @@ struct ContinuityGather {
 // (they read V_j), for every slot -- and zeroes the cell counts of the next
 // step (the scan has consumed them).
 template <bool FAST, int ACTION, int K, bool BATCH, int BLOCK, int STAGE, bool EOS>
-__global__ __launch_bounds__(BLOCK) void k_search(Walk w, Fields f, Physics ph, int32_t M, int32_t *ncount,
+__global__ __launch_bounds__(BLOCK, 8) void k_search(Walk w, Fields f, Physics ph, int32_t M, int32_t *ncount,
                                                         unsigned long long *pairs_last,
                                                         unsigned long long *pairs_total, uint32_t *flags,
                                                         int32_t which, EosOut eo) {
