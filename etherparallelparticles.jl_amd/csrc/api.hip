@@ -252,7 +252,11 @@ int32_t ensure_cells(eppx_ps *ps, int32_t n_cells) {
     HIP_OK(ctx, dalloc(&ps->counts, (size_t)nb));
     HIP_OK(ctx, dalloc(&ps->start, (size_t)nb + 1));
     HIP_OK(ctx, dalloc(&ps->tile_sums, (size_t)((nb + kScanTile - 1) / kScanTile)));
+    // empty cells until the next cell build: a walk before it finds no neighbours
+    // instead of reading uninitialised ranges
+    HIP_OK(ctx, hipMemsetAsync(ps->start, 0, ((size_t)nb + 1) * sizeof(int32_t), ctx->stream));
     ps->n_cells = n_cells;
+    ps->masks_valid = false;
     return EPPX_OK;
 }
 
@@ -1825,6 +1829,9 @@ int32_t eppx_batch_remove(eppx_batch *b, int32_t case_id) {
     b->meta[case_id] = BatchCase{};
     b->cases[case_id] = CaseDesc{};
     if ((rc = relayout(b))) return rc;
+    // the cases after the removed one moved to lower cells: the cell list of
+    // the new layout, before a step walks it
+    if (ps->n && (rc = batch_cell_build(b, false, 0))) return rc;
     return read_flags(ctx);
 }
 

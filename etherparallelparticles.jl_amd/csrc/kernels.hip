@@ -515,7 +515,8 @@ __device__ __forceinline__ bool stage_rows(const Walk &w, const float4 *dyn, int
     // block-uniform: range starts and lengths, and where each range goes in the tile
     auto u = [&](int k) { return __builtin_amdgcn_readfirstlane(info[k]); };
     const int d1 = u(6), d2 = d1 + u(7), d3 = d2 + u(8), d4 = d3 + u(9), d5 = d4 + u(10), total = d5 + u(11);
-    const bool ok = u(12) && total <= kCap;
+    const bool ok = u(12) && total <= kCap &&
+                    (u(6) | u(7) | u(8) | u(9) | u(10) | u(11)) >= 0;  // no range runs backwards
     int other = 0;  // this thread staged a slot of another tag
     if (ok) {
 #pragma unroll

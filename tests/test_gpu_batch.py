@@ -77,6 +77,29 @@ def test_batch_add_and_remove_between_steps(parallel):
         b.close()
 
 
+def test_batch_remove_middle_case_then_step(parallel):
+    """Removing a case that is not the last moves the later cases to lower
+    cells: the steps after it (with no add between) walk the cell list of the
+    new layout and the remaining cases go on as if alone."""
+    cs = [_case(*s) for s in SPECS]  # budgets 12, 7, 20
+    b = _batch(parallel, cs)
+    try:
+        ids = [b.add(c.domain, c.physics, c.host, 20) for c in cs]
+        b.step(5)
+        b.remove(ids[1])
+        assert b.info() == (cs[0].n_particles + cs[2].n_particles, 2)
+        b.step(6)
+        left = b.steps_left()
+        assert left[ids[0]] == 9 and left[ids[2]] == 9
+        _assert_case(b.download(ids[0]), _oracle_after(cs[0], 11))
+        _assert_case(b.download(ids[2]), _oracle_after(cs[2], 11))
+        b.remove(ids[0])
+        b.step(3)
+        _assert_case(b.download(ids[2]), _oracle_after(cs[2], 14))
+    finally:
+        b.close()
+
+
 def test_batch_fast_equals_alone(parallel):
     """FAST math: every case of the batch equals the same case run alone
     through the particle-system path (tolerance: none; identical bits)."""
