@@ -55,6 +55,10 @@ template <class T>
 __device__ __forceinline__ T ld_slot(const T *base, int k) {
     return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + (uint32_t)k * (uint32_t)sizeof(T));
 }
+template <class T>
+__device__ __forceinline__ void st_slot(T *base, int k, T v) {
+    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + (uint32_t)k * (uint32_t)sizeof(T)) = v;
+}
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
 
@@ -848,10 +852,12 @@ struct ContinuityGather {
 // (they read V_j), for every slot -- and zeroes the cell counts of the next
 // step (the scan has consumed them).
 template <bool FAST, int ACTION, int K, bool BATCH, int BLOCK, int STAGE, bool EOS>
-__global__ __launch_bounds__(BLOCK, 8) void k_search(Walk w, Fields f, Physics ph, int32_t M, int32_t *ncount,
-                                                        unsigned long long *pairs_last,
-                                                        unsigned long long *pairs_total, uint32_t *flags,
-                                                        int32_t which, EosOut eo) {
+// Eight waves per SIMD for one-case launches (they spill only on rare paths);
+// the batch kernels need more registers and keep the compiler's choice.
+__global__ __launch_bounds__(BLOCK, BATCH ? 1 : 8) void k_search(Walk w, Fields f, Physics ph, int32_t M,
+                                                                 int32_t *ncount, unsigned long long *pairs_last,
+                                                                 unsigned long long *pairs_total, uint32_t *flags,
+                                                                 int32_t which, EosOut eo) {
     constexpr int kTileRow = PairShape<BLOCK>::tile_row;
     __shared__ int lds[BLOCK / 64];
     __shared__ float4 tile[3 * kTileRow];
@@ -887,7 +893,7 @@ __global__ __launch_bounds__(BLOCK, 8) void k_search(Walk w, Fields f, Physics p
             const float4 di = f.dyn[i];
             const float2 ui = make_float2(di.x, di.y);
             const float rho_i = di.z;
-            float acc = f.drho[i];
+            float acc = ld_slot(f.drho, i);
             // pair classes of the continuity action: fluid-fluid (classic), fluid-wall / wall-fluid (balanced)
             const uint32_t keep = i_fluid ? tag_bit(ph.fluid_tag) | tag_bit(ph.wall_tag)
                                           : (i_wall ? tag_bit(ph.fluid_tag) : 0u);
@@ -943,12 +949,13 @@ __global__ __launch_bounds__(BLOCK, 8) void k_search(Walk w, Fields f, Physics p
             if (EOS && (i_fluid || i_wall)) {
                 float4 d = di;
                 const float vol = continuity_eos_regs(d, acc, gi.w, ph);  // gi.w: Mass (the snapshot follows it)
-                f.drho[i] = 0.0f;
-                reinterpret_cast<float2 *>(f.dyn)[2 * (size_t)i + 1] = make_float2(d.z, d.w);  // (rho, p) half
-                eo.vol_new[i] = vol;
+                // 32-bit slot offsets from the scalar bases: nothing 64-bit per lane outlives the walk
+                st_slot(f.drho, i, 0.0f);
+                st_slot(reinterpret_cast<float2 *>(f.dyn), 2 * i + 1, make_float2(d.z, d.w));  // (rho, p) half
+                st_slot(eo.vol_new, i, vol);
             } else {
-                f.drho[i] = acc;
-                if (EOS) eo.vol_new[i] = f.vol[i];
+                st_slot(f.drho, i, acc);
+                if (EOS) st_slot(eo.vol_new, i, ld_slot(f.vol, i));
             }
         } else {
             walk(w, cd, i, gi, [&](int, uint32_t, float, float, float) { ++cnt; });
@@ -1047,7 +1054,7 @@ __device__ __forceinline__ void momentum_pair(const Physics &ph, float2 &acc, bo
 // sMomentum! (gravity + accelerate-move) on the result and takes the cell key
 // and count of every slot for the coming search (MoveKey).
 template <bool FAST, int K, int VISC, bool BATCH, int BLOCK, int STAGE, bool MOVE>
-__global__ __launch_bounds__(BLOCK) void k_momentum(Walk w, Fields f, Physics ph, MoveKey mk) {
+__global__ __launch_bounds__(BLOCK, BATCH ? 1 : 8) void k_momentum(Walk w, Fields f, Physics ph, MoveKey mk) {
     constexpr int kTileRow = PairShape<BLOCK>::tile_row;
     __shared__ float4 tile[3 * kTileRow];
     __shared__ uint8_t tix[3 * kTileRow];
