@@ -273,9 +273,12 @@ __device__ __forceinline__ int sel3(int r, int a, int b, int c) { return r == 0 
 // band is empty and every candidate tag is adjacent.  With CHECK, candidates
 // in the band take the exact criterion (one branch per group, rarely taken)
 // and non-adjacent tags are cleared.
-template <bool CHECK, int N>
+// g: the candidates' (x, y); their h comes from src (slots k0 + k), read only
+// in the band.
+template <bool CHECK, int N, class Src>
 __device__ __forceinline__ uint32_t accept_bits(const Walk &w, const CaseDesc &cd, float2 xi, float hi, uint32_t adj,
-                                                const float4 (&g)[N], const uint32_t (&tb)[N]) {
+                                                const float2 (&g)[N], const uint32_t (&tb)[N], const Src &src,
+                                                int k0) {
     float d[N];
     uint32_t ok = 0u, band = 0u;
 #pragma unroll
@@ -289,7 +292,7 @@ __device__ __forceinline__ uint32_t accept_bits(const Walk &w, const CaseDesc &c
         if (band) {
 #pragma unroll
             for (int k = 0; k < N; ++k)
-                if ((band >> k) & 1u) ok |= (uint32_t)criterion_exact(w, cd, hi, g[k].z, d[k]) << k;
+                if ((band >> k) & 1u) ok |= (uint32_t)criterion_exact(w, cd, hi, src.h_of(k0 + k), d[k]) << k;
         }
 #pragma unroll
         for (int k = 0; k < N; ++k) ok &= ~((uint32_t)((adj & tb[k]) == 0u) << k);
@@ -299,7 +302,8 @@ __device__ __forceinline__ uint32_t accept_bits(const Walk &w, const CaseDesc &c
 
 // Candidate geometry of the row walk: slot j of neighbour row r, read from
 // global memory or from the block's LDS copy of its three row ranges.
-// get(k): the snapshot (x, y, h, m) of slot k; tbit(k): its tag bit.
+// get(k): the snapshot (x, y, h, m) of slot k; xy_of(k), h_of(k): its parts
+// (pass 1 reads (x, y) only); tbit(k): its tag bit.
 // kStage: what else a tile holds for pass 2 (0 nothing, 1 the (u, v, rho, p)
 // record, 2 its (u, v) half).
 struct GlobalRows {
@@ -308,6 +312,8 @@ struct GlobalRows {
     const int32_t *tag;
     __device__ __forceinline__ int off(int) const { return 0; }
     __device__ __forceinline__ float4 get(int k) const { return geo[k]; }
+    __device__ __forceinline__ float2 xy_of(int k) const { return ld_slot(reinterpret_cast<const float2 *>(geo), 2 * k); }
+    __device__ __forceinline__ float h_of(int k) const { return geo[k].z; }
     __device__ __forceinline__ uint32_t tbit(int k) const { return tag_bit(tag[k]); }
     __device__ __forceinline__ uint32_t tbit_adj(int k) const { return tag_bit(tag[k]); }
     __device__ __forceinline__ float4 at(int r, int j) const { return get(j + off(r)); }
@@ -315,13 +321,19 @@ struct GlobalRows {
 template <int STAGE>
 struct TileRows {
     static constexpr int kStage = STAGE;
-    const float4 *tile;   // LDS: row r, slot j at tile[j + off(r)]
+    const float2 *xy;     // LDS: (x, y) of row r, slot j at xy[j + off(r)]
+    const float2 *hm;     // LDS: (h, m) of the same slot
     const uint8_t *tix;   // LDS: tag - 1 of the same slot (63 outside 1..kMaxTag)
     const float4 *dyn;    // LDS (STAGE 1): (u, v, rho, p) of the same slot
     const float2 *uv;     // LDS (STAGE 2): (u, v) of the same slot
     int off0, off1, off2;
     __device__ __forceinline__ int off(int r) const { return sel3(r, off0, off1, off2); }
-    __device__ __forceinline__ float4 get(int k) const { return tile[k]; }
+    __device__ __forceinline__ float4 get(int k) const {
+        const float2 a = xy[k], b = hm[k];
+        return make_float4(a.x, a.y, b.x, b.y);
+    }
+    __device__ __forceinline__ float2 xy_of(int k) const { return xy[k]; }
+    __device__ __forceinline__ float h_of(int k) const { return hm[k].x; }
     // a 64-bit shift: slot index 63 lands outside the low 32 bits, giving 0
     __device__ __forceinline__ uint32_t tbit(int k) const { return (uint32_t)(1ull << tix[k]); }
     // tag bit of an accepted neighbour: its tag is adjacent, so in 1..kMaxTag
@@ -336,18 +348,18 @@ __device__ __forceinline__ unsigned long long accept_row(const Walk &w, const Ca
     int j = b;
     const int o = src.off(r);
     for (; j + 4 <= e; j += 4) {  // four independent loads in flight per iteration
-        const float4 g[4] = {src.get(j + o), src.get(j + 1 + o), src.get(j + 2 + o), src.get(j + 3 + o)};
+        const float2 g[4] = {src.xy_of(j + o), src.xy_of(j + 1 + o), src.xy_of(j + 2 + o), src.xy_of(j + 3 + o)};
         uint32_t tb[4] = {0u, 0u, 0u, 0u};
         if (CHECK) {
 #pragma unroll
             for (int k = 0; k < 4; ++k) tb[k] = src.tbit(j + k + o);
         }
-        m |= (unsigned long long)accept_bits<CHECK, 4>(w, cd, xi, hi, adj, g, tb) << (j - b);
+        m |= (unsigned long long)accept_bits<CHECK, 4>(w, cd, xi, hi, adj, g, tb, src, j + o) << (j - b);
     }
     for (; j < e; ++j) {
-        const float4 g[1] = {src.get(j + o)};
+        const float2 g[1] = {src.xy_of(j + o)};
         const uint32_t tb[1] = {CHECK ? src.tbit(j + o) : 0u};
-        m |= (unsigned long long)accept_bits<CHECK, 1>(w, cd, xi, hi, adj, g, tb) << (j - b);
+        m |= (unsigned long long)accept_bits<CHECK, 1>(w, cd, xi, hi, adj, g, tb, src, j + o) << (j - b);
     }
     return m;
 }
@@ -484,8 +496,8 @@ __device__ __forceinline__ int walk_rows(const Walk &w, const CaseDesc &cd, cons
 // thread of the block calls it (two barriers).
 constexpr int kStageInfo = 16;
 template <int BLOCK, int STAGE>
-__device__ __forceinline__ bool stage_rows(const Walk &w, const float4 *dyn, int i0, int n, float4 *tile,
-                                           uint8_t *tix, float4 *tdyn, float2 *tuv, int *info,
+__device__ __forceinline__ bool stage_rows(const Walk &w, const float4 *dyn, int i0, int n, float2 *txy,
+                                           float2 *thm, uint8_t *tix, float4 *tdyn, float2 *tuv, int *info,
                                            TileRows<STAGE> &rows, int tag, bool &only_tag) {
     constexpr int kCap = 3 * PairShape<BLOCK>::tile_row;
     const int t = threadIdx.x;
@@ -528,7 +540,9 @@ __device__ __forceinline__ bool stage_rows(const Walk &w, const float4 *dyn, int
             const int b = u(k), len = u(6 + k);
             const int d = k == 0 ? 0 : k == 1 ? d1 : k == 2 ? d2 : k == 3 ? d3 : k == 4 ? d4 : d5;
             for (int m = t; m < len; m += BLOCK) {
-                tile[d + m] = w.geo[b + m];
+                const float4 g = w.geo[b + m];
+                txy[d + m] = make_float2(g.x, g.y);
+                thm[d + m] = make_float2(g.z, g.w);
                 const int tg = w.tag[b + m];
                 other |= tg != tag;
                 const uint32_t t1 = (uint32_t)tg - 1u;
@@ -539,7 +553,8 @@ __device__ __forceinline__ bool stage_rows(const Walk &w, const float4 *dyn, int
         }
         const int i = min(i0 + t, n - 1);
         const bool second = (w.cell[i] - 1) / u(14) != u(13);  // this thread's slot is in the second row
-        rows.tile = tile;
+        rows.xy = txy;
+        rows.hm = thm;
         rows.tix = tix;
         rows.dyn = tdyn;
         rows.uv = tuv;
@@ -860,7 +875,8 @@ __global__ __launch_bounds__(BLOCK, BATCH ? 1 : 8) void k_search(Walk w, Fields 
                                                                  int32_t which, EosOut eo) {
     constexpr int kTileRow = PairShape<BLOCK>::tile_row;
     __shared__ int lds[BLOCK / 64];
-    __shared__ float4 tile[3 * kTileRow];
+    __shared__ float2 txy[3 * kTileRow];
+    __shared__ float2 thm[3 * kTileRow];
     __shared__ uint8_t tix[3 * kTileRow];
     __shared__ float2 tuv[STAGE == 2 ? 3 * kTileRow : 1];
     __shared__ int info[kStageInfo];
@@ -880,7 +896,7 @@ __global__ __launch_bounds__(BLOCK, BATCH ? 1 : 8) void k_search(Walk w, Fields 
     TileRows<STAGE> rows;
     bool all_fluid = false;  // every particle of the block's neighbourhood is fluid
     const bool tiled = ACTION == EPPX_SEARCH_KERNEL_CONTINUITY &&
-                       stage_rows<BLOCK, STAGE>(w, f.dyn, i0, n, tile, tix, nullptr, tuv, info, rows, ph.fluid_tag,
+                       stage_rows<BLOCK, STAGE>(w, f.dyn, i0, n, txy, thm, tix, nullptr, tuv, info, rows, ph.fluid_tag,
                                                 all_fluid);
     int cnt = 0;
     const CaseDesc cd = BATCH ? w.cases[w.cs[min(i, n - 1)]] : w.one;
@@ -1056,7 +1072,8 @@ __device__ __forceinline__ void momentum_pair(const Physics &ph, float2 &acc, bo
 template <bool FAST, int K, int VISC, bool BATCH, int BLOCK, int STAGE, bool MOVE>
 __global__ __launch_bounds__(BLOCK, BATCH ? 1 : 8) void k_momentum(Walk w, Fields f, Physics ph, MoveKey mk) {
     constexpr int kTileRow = PairShape<BLOCK>::tile_row;
-    __shared__ float4 tile[3 * kTileRow];
+    __shared__ float2 txy[3 * kTileRow];
+    __shared__ float2 thm[3 * kTileRow];
     __shared__ uint8_t tix[3 * kTileRow];
     __shared__ float4 tdyn[STAGE == 1 ? 3 * kTileRow : 1];
     __shared__ int info[kStageInfo];
@@ -1068,7 +1085,7 @@ __global__ __launch_bounds__(BLOCK, BATCH ? 1 : 8) void k_momentum(Walk w, Field
     bool all_fluid = false;  // every particle of the block's neighbourhood is fluid
     bool tiled = false;
     if (i0 < n)  // the same for the whole block
-        tiled = stage_rows<BLOCK, STAGE>(w, f.dyn, i0, n, tile, tix, tdyn, nullptr, info, rows, ph.fluid_tag,
+        tiled = stage_rows<BLOCK, STAGE>(w, f.dyn, i0, n, txy, thm, tix, tdyn, nullptr, info, rows, ph.fluid_tag,
                                          all_fluid);
     const int i = i0 + threadIdx.x;
     if (!MOVE && i >= n) return;
