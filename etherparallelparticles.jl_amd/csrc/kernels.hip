@@ -466,9 +466,10 @@ __device__ __forceinline__ int walk_rows(const Walk &w, const CaseDesc &cd, cons
         while (na) {
             const int j = ba + __builtin_ctzll(na), jo = oa;
             na &= na - 1ull;
-            if (!na) { na = nb; ba = bb; oa = ob; nb = 0ull; }
             const uint32_t tbj = src.tbit_adj(j + jo);
             visit(j, jo, tbj, fetch(j, j + jo, tbj, src));
+            // the switch after the term: nothing of this pair is live across it
+            if (!na) { na = nb; ba = bb; oa = ob; nb = 0ull; }
         }
         unsigned long long nm = m1;
         const int bm = b[1], om = src.off(1);
