@@ -62,6 +62,29 @@ __device__ __forceinline__ void st_slot(T *base, int k, T v) {
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
 
+// Row cy = c / n and column cx = c % n of cell c on a grid n cells wide.  For
+// 0 <= c < 2^22 the float estimate c * rcp(n) is within one of the quotient
+// (rcp within an ulp, the product rounded once), so one remainder correction
+// makes it exact; larger cells take the integer division.
+__device__ __forceinline__ void cell_xy(int c, int n, int &cx, int &cy) {
+    if (c >= 0 && c < (1 << 22)) {
+        int q = (int)((float)c * frcp((float)n));
+        int r = c - q * n;
+        if (r < 0) {
+            --q;
+            r += n;
+        } else if (r >= n) {
+            ++q;
+            r -= n;
+        }
+        cy = q;
+        cx = r;
+    } else {
+        cy = c / n;
+        cx = c - cy * n;
+    }
+}
+
 // Mean.harmonic / invharmonic (src/Math/Mean.jl:28-34)
 __device__ __forceinline__ float harmonic(float a, float b) { return 2.0f * a * b / (a + b); }
 __device__ __forceinline__ float invharmonic(float a, float b) { return (a + b) / (2.0f * a * b); }
@@ -369,7 +392,8 @@ __device__ __forceinline__ int walk_rows(const Walk &w, const CaseDesc &cd, cons
                                          uint32_t keep_tags, Fetch &&fetch, Apply &&apply) {
     const int c = w.cell[i] - 1;
     const int nx = cd.g.n_x, base = cd.cell_base;
-    const int cx = c % nx, cy = c / nx;
+    int cx, cy;
+    cell_xy(c, nx, cx, cy);
     const int x0 = max(cx - 1, 0), x1 = min(cx + 1, nx - 1);
     int b[3], e[3];
 #pragma unroll
@@ -553,7 +577,9 @@ __device__ __forceinline__ bool stage_rows(const Walk &w, const float4 *dyn, int
             }
         }
         const int i = min(i0 + t, n - 1);
-        const bool second = (w.cell[i] - 1) / u(14) != u(13);  // this thread's slot is in the second row
+        int cxi, cyi;
+        cell_xy(w.cell[i] - 1, u(14), cxi, cyi);
+        const bool second = cyi != u(13);  // this thread's slot is in the second row
         rows.xy = txy;
         rows.hm = thm;
         rows.tix = tix;
@@ -767,7 +793,6 @@ __global__ __launch_bounds__(kBlock) void k_permute(Fields cur, Fields nxt, floa
     // flight while the rank is taken.
     const int32_t k = key[s];
     const int32_t me = cur.id[s];
-    const int32_t alive_in = cur.alive[s];
     const int32_t tmp = tmpslot[s];
     const float2 x = cur.pos[s];
     const float4 dyn = (dyn_src ? dyn_src : cur.dyn)[s];
@@ -776,7 +801,11 @@ __global__ __launch_bounds__(kBlock) void k_permute(Fields cur, Fields nxt, floa
     // the filter and extrapolation sums, when they may be non-zero
     const float wv = move_accum ? cur.wv[s] : 0.0f, wvrho = move_accum ? cur.wvrho[s] : 0.0f;
     const float wvp = move_accum ? cur.wvp[s] : 0.0f;
-    const int32_t tag = cur.tag[s], mov = cur.mov[s], cell = cur.cell[s];
+    const int32_t tag = cur.tag[s], mov = cur.mov[s];
+    // an alive particle of a particle system sits in cell key + 1 (cell_key_of);
+    // the cell of a batch slot or a dead one, and the state of a dead one, are read
+    const bool plain = !batch && k < n_cells;
+    const int32_t cell = plain ? k + 1 : cur.cell[s];
     const int32_t cs = batch ? cur.cs[s] : 0;
     const int32_t b = start[k];
     int32_t dest, alive;
@@ -801,6 +830,7 @@ __global__ __launch_bounds__(kBlock) void k_permute(Fields cur, Fields nxt, floa
         alive = 1;
     } else {
         dest = b + tmp;
+        const int32_t alive_in = cur.alive[s];
         alive = alive_in == 1 ? 0 : alive_in;
     }
     nxt.pos[dest] = x;
