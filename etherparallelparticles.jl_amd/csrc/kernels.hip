@@ -152,6 +152,35 @@ struct PairH {
     }
 };
 
+// One H in the case (FAST): every pair term is the kernel's constant factor
+// kfac times the rest, so the pair sums carry the rest and the caller adds
+// kfac x sum once.  shape(r, rinv) = dw / r / kfac:
+//   cubic spline: kfac = sigma 0.75 hinv^3, shape = q (3q - 4) or -(2 - q)^2, over r;
+//   Wendland C2:  kfac = -sigma 0.625 hinv^4, shape = (2 - q)^3.
+// The artificial viscosity's hinv (v = min(u.x, 0) hinv) and the 1/2 of the
+// mean density fold into its two constants: av_b = 2 beta hinv^2,
+// av_a = 2 alpha c hinv.
+template <int K>
+struct UniformTerms {
+    float hinv, kfac, av_b, av_a;
+    __device__ __forceinline__ UniformTerms(float hinv_u, const Physics &ph) {
+        hinv = hinv_u;
+        const float h3 = hinv * hinv * hinv;
+        kfac = K == EPPX_KERNEL_CUBIC ? (kSigmaCubic * 0.75f) * h3 : ((-kSigmaWendland * 0.625f) * h3) * hinv;
+        av_b = ph.beta * (2.0f * hinv * hinv);
+        av_a = (ph.alpha * ph.av_c) * (2.0f * hinv);
+    }
+    __device__ __forceinline__ float shape(float r, float rinv) const {
+        const float q = r * hinv;
+        if (K == EPPX_KERNEL_CUBIC) {
+            const float t = 2.0f - q;
+            return (q < 1.0f ? q * fmaf(3.0f, q, -4.0f) : (q < 2.0f ? -(t * t) : 0.0f)) * rinv;
+        }
+        const float t = fmaxf(2.0f - q, 0.0f);
+        return t * (t * t);
+    }
+};
+
 // _value of CubicSpline.jl:17-31 / WendlandC2.jl:16-26 (2D), written order.
 __device__ __forceinline__ float value_exact(int kernel, float r, float hinv) {
     const float q = r * hinv;
@@ -868,6 +897,15 @@ __device__ __forceinline__ float continuity_fast(const PairH<UNIFORM> &ph, float
     return (mj_or_rhoVj * vdotx) * dwr;
 }
 
+// The same term over kfac (one H in the case).
+template <int K>
+__device__ __forceinline__ float continuity_fast_u(const UniformTerms<K> &ut, float2 ui, float2 uj, float mj_or_rhoVj,
+                                                   float dx, float dy, float dr2) {
+    const float rinv = frsq(dr2);
+    const float vdotx = fmaf(dy, ui.y - uj.y, dx * (ui.x - uj.x));
+    return (mj_or_rhoVj * vdotx) * ut.shape(dr2 * rinv, rinv);
+}
+
 template <bool FAST, int K>
 __device__ __forceinline__ float continuity_term(float hi, float hj, float2 ui, float2 uj, float mj_or_rhoVj,
                                                  float dx, float dy, float dr2) {
@@ -961,9 +999,13 @@ __global__ __launch_bounds__(BLOCK, BATCH ? 1 : 8) void k_search(Walk w, Fields 
             };
             bool walked = false;
             if constexpr (FAST) {
-                if (tiled && cd.hinv_u > 0.0f) {  // one H in the case: the pair's hinv is loop invariant
-                    const PairH<true> hu(cd.hinv_u, cd.hinv_u * cd.hinv_u * cd.hinv_u, 0.0f, 0.0f);
-                    if (all_fluid) {  // fluid-fluid pairs only (classic continuity): no tag per pair
+                // One H in the case: the pair's hinv is loop invariant.  Untiled
+                // blocks take the same form (through global memory), so every
+                // pair shape and a case alone or in a batch give the same bits.
+                if (cd.hinv_u > 0.0f) {
+                    const UniformTerms<K> ut(cd.hinv_u, ph);
+                    float sum = 0.0f;  // the pair terms over kfac
+                    if (all_fluid) {  // fluid-fluid pairs only (classic continuity): no tag per pair (tiled)
                         auto fetch_f = [&](int j, int k, uint32_t, const auto &src) {
                             ContinuityGather p;
                             if constexpr (std::decay_t<decltype(src)>::kStage == 2) p.uv = src.uv[k];
@@ -973,17 +1015,19 @@ __global__ __launch_bounds__(BLOCK, BATCH ? 1 : 8) void k_search(Walk w, Fields 
                         };
                         auto term_f = [&](int, uint32_t, float, float mj, float dx, float dy, float dr2,
                                           const ContinuityGather &p) {
-                            acc += continuity_fast<K>(hu, ui, p.uv, mj, dx, dy, dr2);
+                            sum += continuity_fast_u<K>(ut, ui, p.uv, mj, dx, dy, dr2);
                         };
                         cnt = walk_rows<false, true>(w, cd, rows, i, gi, keep, fetch_f, term_f);
                     } else {
                         auto term_u = [&](int, uint32_t tbj, float, float mj, float dx, float dy, float dr2,
                                           const ContinuityGather &p) {
                             const float m = (i_fluid && tbj == fluid_bit) ? mj : rho_i * p.vol;
-                            acc += continuity_fast<K>(hu, ui, p.uv, m, dx, dy, dr2);
+                            sum += continuity_fast_u<K>(ut, ui, p.uv, m, dx, dy, dr2);
                         };
-                        cnt = walk_rows<false, true>(w, cd, rows, i, gi, keep, fetch, term_u);
+                        if (tiled) cnt = walk_rows<false, true>(w, cd, rows, i, gi, keep, fetch, term_u);
+                        else cnt = walk_rows<false, true>(w, cd, GlobalRows{w.geo, w.tag}, i, gi, keep, fetch, term_u);
                     }
+                    acc = fmaf(ut.kfac, sum, acc);
                     walked = true;
                 }
             }
@@ -1055,6 +1099,36 @@ __device__ __forceinline__ void momentum_fast(const Physics &ph, float2 &acc, bo
     }
     acc.x = fmaf(radial, dx, acc.x);
     acc.y = fmaf(radial, dy, acc.y);
+}
+
+// The same terms over kfac (one H in the case), added to sum:
+//   artificial viscosity (beta v - alpha c qa) v (-m_j dwr) / (qa^2 rho_bar rho_i rho_j)
+//     = (av_b m - av_a qa) m (-m_j dwr) / (qa^2 (rho_i + rho_j) rho_i rho_j), m = min(u.x, 0).
+template <int K, int VISC>
+__device__ __forceinline__ void momentum_fast_u(const Physics &ph, float2 &sum, bool classic,
+                                                const UniformTerms<K> &ut, float2 ui, float rho_i, float p_i,
+                                                float pr_i, float2 uj, float rho_j, float p_j, float m_j, float dx,
+                                                float dy, float dr2) {
+    const float rinv = frsq(dr2);
+    const float r = dr2 * rinv;
+    const float mdwr = -m_j * ut.shape(r, rinv);
+    const float rirj = rho_i * rho_j;
+    float prho2 = classic ? fmaf(p_j, frcp(rho_j * rho_j), pr_i) : (p_i + p_j) * frcp(rirj);
+    prho2 = fmaf(fabsf(prho2), ph.coefficient, prho2);
+    float radial = prho2 * mdwr;
+    const float q = r * ut.hinv;
+    const float qa = fmaf(q, q, 0.01f);
+    if (VISC == EPPX_VISC_CLASSIC) {
+        const float force = (-2.0f * ph.mu) * (q * q) * mdwr * frcp(rirj * qa);
+        sum.x = fmaf(force, ui.x - uj.x, sum.x);
+        sum.y = fmaf(force, ui.y - uj.y, sum.y);
+    } else {
+        const float m = fminf(fmaf(dy, ui.y - uj.y, dx * (ui.x - uj.x)), 0.0f);
+        const float den = (qa * qa) * ((rho_i + rho_j) * rirj);
+        radial = fmaf(fmaf(ut.av_b, m, -ut.av_a * qa) * m, mdwr * frcp(den), radial);
+    }
+    sum.x = fmaf(radial, dx, sum.x);
+    sum.y = fmaf(radial, dy, sum.y);
 }
 
 template <bool FAST, int K, int VISC>
@@ -1145,22 +1219,29 @@ __global__ __launch_bounds__(BLOCK, BATCH ? 1 : 8) void k_momentum(Walk w, Field
     const uint32_t keep = tag_bit(ph.fluid_tag) | tag_bit(ph.wall_tag);
     bool walked = false;
     if constexpr (FAST) {
-        if (tiled && cd.hinv_u > 0.0f) {  // one H in the case: the pair's hinv is loop invariant
-            const PairH<true> hu(cd.hinv_u, cd.hinv_u * cd.hinv_u * cd.hinv_u, 0.0f, 0.0f);
-            if (all_fluid) {  // fluid-fluid pairs only (classic pressure): no tag per pair
+        // One H in the case: the pair's hinv is loop invariant.  Untiled blocks
+        // take the same form (through global memory), so every pair shape and
+        // a case alone or in a batch give the same bits.
+        if (cd.hinv_u > 0.0f) {
+            const UniformTerms<K> ut(cd.hinv_u, ph);
+            float2 sum = make_float2(0.0f, 0.0f);  // the pair terms over kfac
+            if (all_fluid) {  // fluid-fluid pairs only (classic pressure): no tag per pair (tiled blocks)
                 auto term_f = [&](int, uint32_t, float, float mj, float dx, float dy, float dr2, const float4 &dj) {
-                    momentum_fast<K, VISC>(ph, acc, true, hu, ui, di.z, di.w, pr_i, make_float2(dj.x, dj.y), dj.z,
-                                           dj.w, mj, dx, dy, dr2);
+                    momentum_fast_u<K, VISC>(ph, sum, true, ut, ui, di.z, di.w, pr_i, make_float2(dj.x, dj.y), dj.z,
+                                             dj.w, mj, dx, dy, dr2);
                 };
                 walk_rows<false, false>(w, cd, rows, i, gi, keep, fetch, term_f);
             } else {
                 auto term_u = [&](int, uint32_t tbj, float, float mj, float dx, float dy, float dr2,
                                   const float4 &dj) {
-                    momentum_fast<K, VISC>(ph, acc, tbj == fluid_bit, hu, ui, di.z, di.w, pr_i,
-                                           make_float2(dj.x, dj.y), dj.z, dj.w, mj, dx, dy, dr2);
+                    momentum_fast_u<K, VISC>(ph, sum, tbj == fluid_bit, ut, ui, di.z, di.w, pr_i,
+                                             make_float2(dj.x, dj.y), dj.z, dj.w, mj, dx, dy, dr2);
                 };
-                walk_rows<false, false>(w, cd, rows, i, gi, keep, fetch, term_u);
+                if (tiled) walk_rows<false, false>(w, cd, rows, i, gi, keep, fetch, term_u);
+                else walk_rows<false, false>(w, cd, GlobalRows{w.geo, w.tag}, i, gi, keep, fetch, term_u);
             }
+            acc.x = fmaf(ut.kfac, sum.x, acc.x);
+            acc.y = fmaf(ut.kfac, sum.y, acc.y);
             walked = true;
         }
     }
