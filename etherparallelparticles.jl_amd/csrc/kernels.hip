@@ -135,20 +135,13 @@ __device__ __forceinline__ float grad_over_r_fast(float r, float rinv, float hin
     return ((-kSigmaWendland * 0.625f) * h3 * hinv) * (t * t * t);
 }
 
-// The pair's hinv = (h_i + h_j) / (2 h_i h_j) and hinv^3 for the FAST terms.
-// UNIFORM: every particle of the case has one H, whose hinv the caller has
-// (hinv_u, h3_u, loop invariant).
-template <bool UNIFORM>
+// The pair's hinv = (h_i + h_j) / (2 h_i h_j) and hinv^3 for the FAST terms
+// (a case with one H takes UniformTerms instead).
 struct PairH {
     float hinv, h3;
-    __device__ __forceinline__ PairH(float hinv_u, float h3_u, float hi, float hj) {
-        if (UNIFORM) {
-            hinv = hinv_u;
-            h3 = h3_u;
-        } else {
-            hinv = (hi + hj) * frcp(2.0f * hi * hj);
-            h3 = hinv * hinv * hinv;
-        }
+    __device__ __forceinline__ PairH(float hi, float hj) {
+        hinv = (hi + hj) * frcp(2.0f * hi * hj);
+        h3 = hinv * hinv * hinv;
     }
 };
 
@@ -887,8 +880,8 @@ __global__ __launch_bounds__(kBlock) void k_permute(Fields cur, Fields nxt, floa
 // iValueGradient! + iClassicContinuity! / iBalancedContinuity!
 // (Library/Kernel.jl:114-128, Continuity.jl:23-47): one term of drho_i;
 // mj_or_rhoVj is m_j (fluid-fluid) or rho_i * V_j (fluid-wall, wall-fluid).
-template <int K, bool UNIFORM>
-__device__ __forceinline__ float continuity_fast(const PairH<UNIFORM> &ph, float2 ui, float2 uj, float mj_or_rhoVj,
+template <int K>
+__device__ __forceinline__ float continuity_fast(const PairH &ph, float2 ui, float2 uj, float mj_or_rhoVj,
                                                  float dx, float dy, float dr2) {
     const float rinv = frsq(dr2);
     const float r = dr2 * rinv;
@@ -906,19 +899,23 @@ __device__ __forceinline__ float continuity_fast_u(const UniformTerms<K> &ut, fl
     return (mj_or_rhoVj * vdotx) * ut.shape(dr2 * rinv, rinv);
 }
 
+// The reference's written order, given the pair's hinv = invharmonic(h_i, h_j).
+template <int K>
+__device__ __forceinline__ float continuity_exact(float hinv, float2 ui, float2 uj, float mj_or_rhoVj, float dx,
+                                                  float dy, float dr2) {
+    const float r = sqrtf(dr2);
+    const float dw = grad_exact<K>(r, hinv);
+    float vdotx = 0.0f;
+    vdotx += dx * (ui.x - uj.x);
+    vdotx += dy * (ui.y - uj.y);
+    return mj_or_rhoVj * vdotx * dw / r;
+}
+
 template <bool FAST, int K>
 __device__ __forceinline__ float continuity_term(float hi, float hj, float2 ui, float2 uj, float mj_or_rhoVj,
                                                  float dx, float dy, float dr2) {
-    if (!FAST) {
-        const float r = sqrtf(dr2);
-        const float hinv = invharmonic(hi, hj);
-        const float dw = grad_exact<K>(r, hinv);
-        float vdotx = 0.0f;
-        vdotx += dx * (ui.x - uj.x);
-        vdotx += dy * (ui.y - uj.y);
-        return mj_or_rhoVj * vdotx * dw / r;
-    }
-    return continuity_fast<K>(PairH<false>(0.0f, 0.0f, hi, hj), ui, uj, mj_or_rhoVj, dx, dy, dr2);
+    if (!FAST) return continuity_exact<K>(invharmonic(hi, hj), ui, uj, mj_or_rhoVj, dx, dy, dr2);
+    return continuity_fast<K>(PairH(hi, hj), ui, uj, mj_or_rhoVj, dx, dy, dr2);
 }
 
 struct ContinuityGather {
@@ -1030,6 +1027,19 @@ __global__ __launch_bounds__(BLOCK, BATCH ? 1 : 8) void k_search(Walk w, Fields 
                     acc = fmaf(ut.kfac, sum, acc);
                     walked = true;
                 }
+            } else {
+                if (cd.hinv_u > 0.0f) {
+                    // one H in the case: the pair's hinv is the particle's own (the same bits, once)
+                    const float hinv_e = invharmonic(gi.z, gi.z);
+                    auto term_e = [&](int, uint32_t tbj, float, float mj, float dx, float dy, float dr2,
+                                      const ContinuityGather &p) {
+                        const float m = (i_fluid && tbj == fluid_bit) ? mj : rho_i * p.vol;
+                        acc += continuity_exact<K>(hinv_e, ui, p.uv, m, dx, dy, dr2);
+                    };
+                    if (tiled) cnt = walk_rows<true, true>(w, cd, rows, i, gi, keep, fetch, term_e);
+                    else cnt = walk_rows<true, true>(w, cd, GlobalRows{w.geo, w.tag}, i, gi, keep, fetch, term_e);
+                    walked = true;
+                }
             }
             if (walked) {
             } else if (tiled) {
@@ -1074,8 +1084,8 @@ __global__ __launch_bounds__(BLOCK, BATCH ? 1 : 8) void k_search(Walk w, Fields 
 //    (beta phi - alpha c) phi (-m_j dwr) / (rho_bar rho_i rho_j) (dx, dy)
 //    = (beta v - alpha c qa) v (-m_j dwr) / (qa^2 rho_bar rho_i rho_j) (dx, dy),
 //    one reciprocal; a separating pair (vdotx > 0) gives v = 0.
-template <int K, int VISC, bool UNIFORM>
-__device__ __forceinline__ void momentum_fast(const Physics &ph, float2 &acc, bool classic, const PairH<UNIFORM> &h,
+template <int K, int VISC>
+__device__ __forceinline__ void momentum_fast(const Physics &ph, float2 &acc, bool classic, const PairH &h,
                                               float2 ui, float rho_i, float p_i, float pr_i, float2 uj, float rho_j,
                                               float p_j, float m_j, float dx, float dy, float dr2) {
     const float rinv = frsq(dr2);
@@ -1131,43 +1141,53 @@ __device__ __forceinline__ void momentum_fast_u(const Physics &ph, float2 &sum, 
     sum.y = fmaf(radial, dy, sum.y);
 }
 
+// The reference's written order (Pressure.jl:24-60, Viscosity.jl:10-52),
+// given the pair's hinv = invharmonic(h_i, h_j) and, when the case has one H,
+// h = 1 / hinv (h_or_0 = 0: taken here, for the pairs that need it).
+template <int K, int VISC>
+__device__ __forceinline__ void momentum_exact(const Physics &ph, float2 &acc, bool classic, float hinv, float h_or_0,
+                                               float2 ui, float rho_i, float p_i, float2 uj, float rho_j, float p_j,
+                                               float m_j, float dx, float dy, float dr2) {
+    const float dux = ui.x - uj.x, duy = ui.y - uj.y;
+    const float r = sqrtf(dr2);
+    const float dw = grad_exact<K>(r, hinv);
+    float prho2 = classic ? p_i / (rho_i * rho_i) + p_j / (rho_j * rho_j) : (p_i + p_j) / (rho_i * rho_j);
+    prho2 += fabsf(prho2) * ph.coefficient;
+    prho2 *= -m_j * dw / r;
+    acc.x += prho2 * dx;
+    acc.y += prho2 * dy;
+    if (VISC == EPPX_VISC_CLASSIC) {
+        const float h = h_or_0 > 0.0f ? h_or_0 : 1.0f / hinv;
+        const float az = r * r + 0.01f * h * h;
+        const float force = 2.0f * ph.mu * m_j * r * dw / (rho_i * rho_j * az);
+        acc.x += force * dux;
+        acc.y += force * duy;
+    } else {
+        float vdotx = 0.0f;
+        vdotx += dx * dux;
+        vdotx += dy * duy;
+        if (vdotx > 0.0f) return;
+        const float rho = (rho_i + rho_j) * 0.5f;
+        const float h = h_or_0 > 0.0f ? h_or_0 : 1.0f / hinv;
+        const float az = r * r + 0.01f * h * h;
+        const float phi = vdotx / (hinv * az);
+        float force = (-ph.alpha * ph.av_c + ph.beta * phi) * phi / rho;
+        force *= -m_j / (rho_i * rho_j * r) * dw;
+        acc.x += force * dx;
+        acc.y += force * dy;
+    }
+}
+
 template <bool FAST, int K, int VISC>
 __device__ __forceinline__ void momentum_pair(const Physics &ph, float2 &acc, bool classic, float hi, float hj,
                                               float2 ui, float rho_i, float p_i, float pr_i, float2 uj, float rho_j,
                                               float p_j, float m_j, float dx, float dy, float dr2) {
-    const float dux = ui.x - uj.x, duy = ui.y - uj.y;
     if (!FAST) {
-        const float r = sqrtf(dr2);
-        const float hinv = invharmonic(hi, hj);
-        const float dw = grad_exact<K>(r, hinv);
-        float prho2 = classic ? p_i / (rho_i * rho_i) + p_j / (rho_j * rho_j) : (p_i + p_j) / (rho_i * rho_j);
-        prho2 += fabsf(prho2) * ph.coefficient;
-        prho2 *= -m_j * dw / r;
-        acc.x += prho2 * dx;
-        acc.y += prho2 * dy;
-        if (VISC == EPPX_VISC_CLASSIC) {
-            const float h = 1.0f / hinv;
-            const float az = r * r + 0.01f * h * h;
-            const float force = 2.0f * ph.mu * m_j * r * dw / (rho_i * rho_j * az);
-            acc.x += force * dux;
-            acc.y += force * duy;
-        } else {
-            float vdotx = 0.0f;
-            vdotx += dx * dux;
-            vdotx += dy * duy;
-            if (vdotx > 0.0f) return;
-            const float rho = (rho_i + rho_j) * 0.5f;
-            const float h = 1.0f / hinv;
-            const float az = r * r + 0.01f * h * h;
-            const float phi = vdotx / (hinv * az);
-            float force = (-ph.alpha * ph.av_c + ph.beta * phi) * phi / rho;
-            force *= -m_j / (rho_i * rho_j * r) * dw;
-            acc.x += force * dx;
-            acc.y += force * dy;
-        }
+        momentum_exact<K, VISC>(ph, acc, classic, invharmonic(hi, hj), 0.0f, ui, rho_i, p_i, uj, rho_j, p_j, m_j, dx,
+                                dy, dr2);
         return;
     }
-    momentum_fast<K, VISC>(ph, acc, classic, PairH<false>(0.0f, 0.0f, hi, hj), ui, rho_i, p_i, pr_i, uj, rho_j, p_j,
+    momentum_fast<K, VISC>(ph, acc, classic, PairH(hi, hj), ui, rho_i, p_i, pr_i, uj, rho_j, p_j,
                            m_j, dx, dy, dr2);
 }
 
@@ -1242,6 +1262,19 @@ __global__ __launch_bounds__(BLOCK, BATCH ? 1 : 8) void k_momentum(Walk w, Field
             }
             acc.x = fmaf(ut.kfac, sum.x, acc.x);
             acc.y = fmaf(ut.kfac, sum.y, acc.y);
+            walked = true;
+        }
+    } else {
+        if (cd.hinv_u > 0.0f) {
+            // one H in the case: the pair's hinv, and its h, are the particle's own
+            // (the same bits as per pair, once)
+            const float hinv_e = invharmonic(gi.z, gi.z), h_e = 1.0f / hinv_e;
+            auto term_e = [&](int, uint32_t tbj, float, float mj, float dx, float dy, float dr2, const float4 &dj) {
+                momentum_exact<K, VISC>(ph, acc, tbj == fluid_bit, hinv_e, h_e, ui, di.z, di.w,
+                                        make_float2(dj.x, dj.y), dj.z, dj.w, mj, dx, dy, dr2);
+            };
+            if (tiled) walk_rows<true, false>(w, cd, rows, i, gi, keep, fetch, term_e);
+            else walk_rows<true, false>(w, cd, GlobalRows{w.geo, w.tag}, i, gi, keep, fetch, term_e);
             walked = true;
         }
     }
