@@ -1142,16 +1142,17 @@ __device__ __forceinline__ void momentum_fast_u(const Physics &ph, float2 &sum, 
 }
 
 // The reference's written order (Pressure.jl:24-60, Viscosity.jl:10-52),
-// given the pair's hinv = invharmonic(h_i, h_j) and, when the case has one H,
-// h = 1 / hinv (h_or_0 = 0: taken here, for the pairs that need it).
+// given the pair's hinv = invharmonic(h_i, h_j), the particle's
+// pr_i = p_i / (rho_i rho_i) and, when the case has one H, h = 1 / hinv
+// (h_or_0 = 0: taken here, for the pairs that need it).
 template <int K, int VISC>
 __device__ __forceinline__ void momentum_exact(const Physics &ph, float2 &acc, bool classic, float hinv, float h_or_0,
-                                               float2 ui, float rho_i, float p_i, float2 uj, float rho_j, float p_j,
-                                               float m_j, float dx, float dy, float dr2) {
+                                               float2 ui, float rho_i, float p_i, float pr_i, float2 uj, float rho_j,
+                                               float p_j, float m_j, float dx, float dy, float dr2) {
     const float dux = ui.x - uj.x, duy = ui.y - uj.y;
     const float r = sqrtf(dr2);
     const float dw = grad_exact<K>(r, hinv);
-    float prho2 = classic ? p_i / (rho_i * rho_i) + p_j / (rho_j * rho_j) : (p_i + p_j) / (rho_i * rho_j);
+    float prho2 = classic ? pr_i + p_j / (rho_j * rho_j) : (p_i + p_j) / (rho_i * rho_j);
     prho2 += fabsf(prho2) * ph.coefficient;
     prho2 *= -m_j * dw / r;
     acc.x += prho2 * dx;
@@ -1183,8 +1184,8 @@ __device__ __forceinline__ void momentum_pair(const Physics &ph, float2 &acc, bo
                                               float2 ui, float rho_i, float p_i, float pr_i, float2 uj, float rho_j,
                                               float p_j, float m_j, float dx, float dy, float dr2) {
     if (!FAST) {
-        momentum_exact<K, VISC>(ph, acc, classic, invharmonic(hi, hj), 0.0f, ui, rho_i, p_i, uj, rho_j, p_j, m_j, dx,
-                                dy, dr2);
+        momentum_exact<K, VISC>(ph, acc, classic, invharmonic(hi, hj), 0.0f, ui, rho_i, p_i, pr_i, uj, rho_j, p_j, m_j,
+                                dx, dy, dr2);
         return;
     }
     momentum_fast<K, VISC>(ph, acc, classic, PairH(hi, hj), ui, rho_i, p_i, pr_i, uj, rho_j, p_j,
@@ -1226,7 +1227,8 @@ __global__ __launch_bounds__(BLOCK, BATCH ? 1 : 8) void k_momentum(Walk w, Field
     di = f.dyn[i];
     const float2 ui = make_float2(di.x, di.y);
     acc = f.dvel[i];
-    const float pr_i = FAST ? di.w * frcp(di.z * di.z) : 0.0f;
+    // p_i / rho_i^2 of the classic pressure, once (EXACT: the reference's expression, the same bits)
+    const float pr_i = FAST ? di.w * frcp(di.z * di.z) : di.w / (di.z * di.z);
     const uint32_t fluid_bit = tag_bit(ph.fluid_tag);
     auto fetch = [&](int j, int k, uint32_t, const auto &src) -> float4 {
         if constexpr (std::decay_t<decltype(src)>::kStage == 1) return src.dyn[k];
@@ -1270,7 +1272,7 @@ __global__ __launch_bounds__(BLOCK, BATCH ? 1 : 8) void k_momentum(Walk w, Field
             // (the same bits as per pair, once)
             const float hinv_e = invharmonic(gi.z, gi.z), h_e = 1.0f / hinv_e;
             auto term_e = [&](int, uint32_t tbj, float, float mj, float dx, float dy, float dr2, const float4 &dj) {
-                momentum_exact<K, VISC>(ph, acc, tbj == fluid_bit, hinv_e, h_e, ui, di.z, di.w,
+                momentum_exact<K, VISC>(ph, acc, tbj == fluid_bit, hinv_e, h_e, ui, di.z, di.w, pr_i,
                                         make_float2(dj.x, dj.y), dj.z, dj.w, mj, dx, dy, dr2);
             };
             if (tiled) walk_rows<true, false>(w, cd, rows, i, gi, keep, fetch, term_e);
