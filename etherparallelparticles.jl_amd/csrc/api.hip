@@ -392,9 +392,11 @@ int32_t field_arg(eppx_ctx *ctx, const eppx_ps *ps, const void *params, size_t s
 // clean! + insert + cell-order move + find + fused action.
 // keys_done: the cell counts, keys and provisional slots of this search were
 // already taken (k_move_key in a step).
+// drho_zero (a step's searches): dDensity is zero on every alive slot of the
+// current buffer and, with permute_no_drho, of the other one as well.
 int32_t do_search(eppx_ps *ps, eppx_ns *ns, int32_t criterion, int32_t action, const Physics &ph,
                   bool keys_done = false, const float4 *dyn_src = nullptr, const float *vol_src = nullptr,
-                  const EosOut *eos = nullptr) {
+                  const EosOut *eos = nullptr, bool drho_zero = false, bool permute_no_drho = false) {
     eppx_ctx *ctx = ps->ctx;
     int32_t rc = ensure_cells(ps, ns->grid.n_cells);
     if (rc) return rc;
@@ -414,11 +416,13 @@ int32_t do_search(eppx_ps *ps, eppx_ns *ns, int32_t criterion, int32_t action, c
            }));
     HIP_OK(ctx, timed(ctx, "cell_permute", [&] {
                return launch_permute(cur, nxt, ps->geo, ps->n, 0, nc, ps->key, ps->tmpslot, ps->start,
-                                     ps->members, ps->n_alive, ctx->d_flags, ps->accum_live ? 1 : 0, dyn_src,
-                                     vol_src, s);
+                                     ps->members, ps->n_alive, ctx->d_flags,
+                                     (ps->accum_live ? kPermuteSums : 0) | (permute_no_drho ? kPermuteNoDrho : 0),
+                                     dyn_src, vol_src, s);
            }));
     ps->cur ^= 1;
     Walk w = make_walk(ps, ns, criterion);
+    w.drho_zero = drho_zero && action == EPPX_SEARCH_KERNEL_CONTINUITY;
     w.mask_mode = action == EPPX_SEARCH_KERNEL_CONTINUITY ? 1 : 0;  // keep the masks for the interactions
     const int fast = ctx->math_mode == EPPX_MATH_FAST;
     HIP_OK(ctx, timed(ctx, "search", [&] {
@@ -1153,6 +1157,14 @@ int32_t eppx_step(eppx_ps *ps, eppx_ns *ns, const eppx_physics *phys, int32_t n_
         HIP_OK(ctx, dalloc(&ps->dyn_new, (size_t)ps->cap));
         HIP_OK(ctx, dalloc(&ps->vol_new, (size_t)ps->cap));
     }
+    // With every particle fluid or wall, the step's sContinuity! leaves
+    // dDensity zero on every alive slot, and so does the fused search that
+    // applies the next one: from the call's first permute on, both buffers
+    // hold zeros there (slots only die), so the searches start their sums at
+    // zero and the later permutes leave dDensity alone.
+    auto tag_bit_h = [](int32_t t) { return (uint32_t)t - 1u < (uint32_t)kMaxTag ? 1u << (t - 1) : 0u; };
+    const bool drho_zero = !ps->odd_tags &&
+                           (ps->tags_present & ~(tag_bit_h(ph.fluid_tag) | tag_bit_h(ph.wall_tag))) == 0u;
     // eos_first: the step starts with sContinuity! (first step of the call,
     // or after a step whose search did not apply it); fuse_eos: its search
     // applies the next step's.
@@ -1176,7 +1188,7 @@ int32_t eppx_step(eppx_ps *ps, eppx_ns *ns, const eppx_physics *phys, int32_t n_
         const EosOut eo{ps->n, nc + 2, ps->vol_new, ps->counts};
         if ((r = do_search(ps, ns, EPPX_CRIT_SYMMETRY, EPPX_SEARCH_KERNEL_CONTINUITY, ph, true,
                            fuse_move ? ps->dyn_new : nullptr, eos_first ? nullptr : ps->vol_new,
-                           fuse_eos ? &eo : nullptr)))
+                           fuse_eos ? &eo : nullptr, drho_zero, drho_zero && !eos_first)))
             return r;
         if (filter_now) {
             if ((r = do_interaction(ps, ns, EPPX_INTER_FILTER, ph, op))) return r;
@@ -1489,7 +1501,8 @@ int32_t batch_cell_build(eppx_batch *b, bool search, int32_t which, bool keys_do
            }));
     HIP_OK(ctx, timed(ctx, "cell_permute", [&] {
                return launch_permute(cur, nxt, ps->geo, ps->n, 1, nc, ps->key, ps->tmpslot, ps->start, ps->members,
-                                     ps->n_alive, ctx->d_flags, ps->accum_live ? 1 : 0, nullptr, nullptr, s);
+                                     ps->n_alive, ctx->d_flags, ps->accum_live ? kPermuteSums : 0, nullptr, nullptr,
+                                     s);
            }));
     ps->cur ^= 1;
     if (!search) {

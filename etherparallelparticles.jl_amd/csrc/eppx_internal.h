@@ -75,6 +75,10 @@ struct Walk {
     // instead of testing the candidates again (same pair set), 0 neither.
     unsigned long long *masks;
     int32_t mask_stride, mask_mode;
+    // 1: dDensity is zero on every alive slot (a step's searches after its
+    // reset): the continuity sum starts at zero and the fused EOS search
+    // leaves the zero in place
+    int32_t drho_zero;
 };
 
 struct Physics {
@@ -119,13 +123,15 @@ hipError_t launch_key_count(Fields f, int32_t n, const CaseDesc &one, const Case
 hipError_t launch_scan(const int32_t *counts, int32_t n, int32_t *start, int32_t *block_sums, hipStream_t s);
 hipError_t launch_fill_members(const Fields f, int32_t n, int32_t n_cells, const int32_t *key,
                                const int32_t *tmpslot, const int32_t *start, int32_t *members, hipStream_t s);
-// move_accum = 0: SumWeight, SumWeightedDensity and SumWeightedPressure are
-// zero in both buffers and are not moved.  dyn_src / vol_src (null: cur's)
-// hold the slots' (u, v, rho, p) records and volumes when a fused kernel
-// wrote them aside.
+// opts: kPermuteSums moves SumWeight, SumWeightedDensity and
+// SumWeightedPressure (without it they are zero in both buffers and stay);
+// kPermuteNoDrho leaves dDensity of the alive slots alone (it is zero there
+// in both buffers).  dyn_src / vol_src (null: cur's) hold the slots'
+// (u, v, rho, p) records and volumes when a fused kernel wrote them aside.
+constexpr int32_t kPermuteSums = 1, kPermuteNoDrho = 2;
 hipError_t launch_permute(Fields cur, Fields nxt, float4 *geo, int32_t n, int32_t batch, int32_t n_cells,
                           const int32_t *key, const int32_t *tmpslot, const int32_t *start,
-                          const int32_t *members, int32_t *n_alive, uint32_t *flags, int32_t move_accum,
+                          const int32_t *members, int32_t *n_alive, uint32_t *flags, int32_t opts,
                           const float4 *dyn_src, const float *vol_src, hipStream_t s);
 // which = 0: the action runs for particles of cases with steps left; 1: of fresh cases.
 // block: threads per pair block (256 or 512); stage: 1 when pass 2 reads

@@ -801,7 +801,7 @@ __global__ __launch_bounds__(kBlock) void k_fill_members(Fields f, int32_t n, in
 __global__ __launch_bounds__(kBlock) void k_permute(Fields cur, Fields nxt, float4 *geo, int32_t n, int32_t batch,
                                                      int32_t n_cells, const int32_t *key, const int32_t *tmpslot,
                                                      const int32_t *start, const int32_t *members,
-                                                     int32_t *n_alive, uint32_t *flags, int32_t move_accum,
+                                                     int32_t *n_alive, uint32_t *flags, int32_t opts,
                                                      const float4 *dyn_src, const float *vol_src) {
     const int32_t s = blockIdx.x * kBlock + threadIdx.x;
     if (s == 0) {
@@ -819,7 +819,10 @@ __global__ __launch_bounds__(kBlock) void k_permute(Fields cur, Fields nxt, floa
     const float2 x = cur.pos[s];
     const float4 dyn = (dyn_src ? dyn_src : cur.dyn)[s];
     const float2 dvel = cur.dvel[s];
-    const float mass = cur.mass[s], vol = (vol_src ? vol_src : cur.vol)[s], drho = cur.drho[s], h = cur.h[s];
+    const bool move_accum = opts & kPermuteSums;
+    const bool move_drho = !(opts & kPermuteNoDrho) || k >= n_cells;  // dead slots keep theirs
+    const float mass = cur.mass[s], vol = (vol_src ? vol_src : cur.vol)[s], h = cur.h[s];
+    const float drho = move_drho ? cur.drho[s] : 0.0f;
     // the filter and extrapolation sums, when they may be non-zero
     const float wv = move_accum ? cur.wv[s] : 0.0f, wvrho = move_accum ? cur.wvrho[s] : 0.0f;
     const float wvp = move_accum ? cur.wvp[s] : 0.0f;
@@ -861,7 +864,7 @@ __global__ __launch_bounds__(kBlock) void k_permute(Fields cur, Fields nxt, floa
     nxt.dvel[dest] = dvel;
     nxt.mass[dest] = mass;
     nxt.vol[dest] = vol;
-    nxt.drho[dest] = drho;
+    if (move_drho) nxt.drho[dest] = drho;
     nxt.h[dest] = h;
     if (move_accum) {
         nxt.wv[dest] = wv;
@@ -975,7 +978,7 @@ __global__ __launch_bounds__(BLOCK, BATCH ? 1 : 8) void k_search(Walk w, Fields 
             const float4 di = f.dyn[i];
             const float2 ui = make_float2(di.x, di.y);
             const float rho_i = di.z;
-            float acc = ld_slot(f.drho, i);
+            float acc = w.drho_zero ? 0.0f : ld_slot(f.drho, i);
             // pair classes of the continuity action: fluid-fluid (classic), fluid-wall / wall-fluid (balanced)
             const uint32_t keep = i_fluid ? tag_bit(ph.fluid_tag) | tag_bit(ph.wall_tag)
                                           : (i_wall ? tag_bit(ph.fluid_tag) : 0u);
@@ -1051,7 +1054,7 @@ __global__ __launch_bounds__(BLOCK, BATCH ? 1 : 8) void k_search(Walk w, Fields 
                 float4 d = di;
                 const float vol = continuity_eos_regs(d, acc, gi.w, ph);  // gi.w: Mass (the snapshot follows it)
                 // 32-bit slot offsets from the scalar bases: nothing 64-bit per lane outlives the walk
-                st_slot(f.drho, i, 0.0f);
+                if (!w.drho_zero) st_slot(f.drho, i, 0.0f);
                 st_slot(reinterpret_cast<float2 *>(f.dyn), 2 * i + 1, make_float2(d.z, d.w));  // (rho, p) half
                 st_slot(eo.vol_new, i, vol);
             } else {
@@ -1767,10 +1770,10 @@ hipError_t launch_fill_members(const Fields f, int32_t n, int32_t n_cells, const
 
 hipError_t launch_permute(Fields cur, Fields nxt, float4 *geo, int32_t n, int32_t batch, int32_t n_cells,
                           const int32_t *key, const int32_t *tmpslot, const int32_t *start,
-                          const int32_t *members, int32_t *n_alive, uint32_t *flags, int32_t move_accum,
+                          const int32_t *members, int32_t *n_alive, uint32_t *flags, int32_t opts,
                           const float4 *dyn_src, const float *vol_src, hipStream_t s) {
     hipLaunchKernelGGL(k_permute, grid_for(n), dim3(kBlock), 0, s, cur, nxt, geo, n, batch, n_cells, key, tmpslot,
-                       start, members, n_alive, flags, move_accum, dyn_src, vol_src);
+                       start, members, n_alive, flags, opts, dyn_src, vol_src);
     return hipGetLastError();
 }
 

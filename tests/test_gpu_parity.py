@@ -296,6 +296,34 @@ def test_steps_split_over_calls_exact(parallel, small):
     assert off[-1] == pairs[-1]
 
 
+def test_particles_dying_inside_a_call_exact(parallel):
+    """Fluid particles thrown out of the domain die in the middle of a
+    10-step call (the later steps leave dDensity of the alive slots alone):
+    every particle's state, the dead ones' included, is bit-identical to the
+    restatement, and the pairs are the same."""
+    from eppx import Algorithm, cases
+    case = cases.small(M=64)
+    h = case.host
+    L = h.named_index_.float_table.index
+    fluid = np.flatnonzero(h.int_properties_[: h.n_particles_, h.named_index_.int_table.index["Tag"] - 1] == 1)
+    u = L["VelocityVec"] - 1
+    throw = fluid[::700][:6]  # a few fluid particles across the column
+    h.float_properties_[throw, u] = 2.0e4  # m/s: about 1 m per step, out of the tank within a few steps
+    ph = case.physics
+    oph = U.oracle_physics(ph)
+    ps, ns, ref = U.search_both(parallel, case, M=64)  # a thrown particle may land among others
+    alive0 = ps.get_alive_n_particles()
+    Algorithm.step(ps, ns, ph, n_steps=10)
+    err, pairs = ref.run_steps(10, oph)
+    assert err == 0
+    host = U.download(ps)
+    assert ps.get_alive_n_particles() < alive0  # some died during the call
+    U.assert_state_equal(host, ref, alive_only=False)
+    U.assert_neighbour_columns_equal(host, ref)
+    off, _ = U.export_pairs(ps, ns)
+    assert off[-1] == pairs[-1]
+
+
 def _fast_steps_errors(parallel, case, n):
     from eppx import Algorithm
     parallel.set_math("fast")
