@@ -84,7 +84,7 @@ struct eppx_ps {
     int32_t last_crit = 0, last_action = 0, last_kernel = 0, last_fluid = 0, last_wall = 0;
     bool searched = false;
     // recorded steps, per buffer parity and per kind of step (StepKind)
-    StepGraph graphs[2][4];
+    StepGraph graphs[2][8];
     // the fused step's records and volumes written aside (allocated on first use)
     float4 *dyn_new = nullptr;
     float *vol_new = nullptr;
@@ -97,6 +97,10 @@ struct eppx_ps {
     // some buffer (an upload brought non-zero values, or a filter or
     // extrapolation interaction ran); while false the permute skips them
     bool accum_live = false;
+    // dVelocity of the alive particles: -2 zero on all of them, t >= 0 zero
+    // except on particles of tag t (the fluid of the momentum passes since),
+    // -1 unknown
+    int32_t dvel_nonzero_tag = -1;
     // columns the kernels do not own, kept on the host between upload and download
     std::vector<int32_t> pass_int_cols, pass_float_cols;
     std::vector<int32_t> pass_int;
@@ -393,10 +397,12 @@ int32_t field_arg(eppx_ctx *ctx, const eppx_ps *ps, const void *params, size_t s
 // keys_done: the cell counts, keys and provisional slots of this search were
 // already taken (k_move_key in a step).
 // drho_zero (a step's searches): dDensity is zero on every alive slot of the
-// current buffer and, with permute_no_drho, of the other one as well.
+// current buffer and, with permute_no_drho, of the other one as well;
+// permute_no_dvel: dVelocity is zero on every alive slot of both.
 int32_t do_search(eppx_ps *ps, eppx_ns *ns, int32_t criterion, int32_t action, const Physics &ph,
                   bool keys_done = false, const float4 *dyn_src = nullptr, const float *vol_src = nullptr,
-                  const EosOut *eos = nullptr, bool drho_zero = false, bool permute_no_drho = false) {
+                  const EosOut *eos = nullptr, bool drho_zero = false, bool permute_no_drho = false,
+                  bool permute_no_dvel = false) {
     eppx_ctx *ctx = ps->ctx;
     int32_t rc = ensure_cells(ps, ns->grid.n_cells);
     if (rc) return rc;
@@ -417,7 +423,8 @@ int32_t do_search(eppx_ps *ps, eppx_ns *ns, int32_t criterion, int32_t action, c
     HIP_OK(ctx, timed(ctx, "cell_permute", [&] {
                return launch_permute(cur, nxt, ps->geo, ps->n, 0, nc, ps->key, ps->tmpslot, ps->start,
                                      ps->members, ps->n_alive, ctx->d_flags,
-                                     (ps->accum_live ? kPermuteSums : 0) | (permute_no_drho ? kPermuteNoDrho : 0),
+                                     (ps->accum_live ? kPermuteSums : 0) | (permute_no_drho ? kPermuteNoDrho : 0) |
+                                         (permute_no_dvel ? kPermuteNoDvel : 0),
                                      dyn_src, vol_src, s);
            }));
     ps->cur ^= 1;
@@ -464,8 +471,20 @@ int32_t refresh_snapshot_mass(eppx_ps *ps, int32_t field) {
     return EPPX_OK;
 }
 
+// dVelocity after a momentum pass (it writes the fluid's), a move (it zeroes
+// the fluid's) or a field op on it (anything).
+void note_dvel_momentum(eppx_ps *ps, int32_t fluid_tag) {
+    const int32_t t = ps->dvel_nonzero_tag;
+    ps->dvel_nonzero_tag = (t == -2 || t == fluid_tag) ? fluid_tag : -1;
+}
+void note_dvel_moved(eppx_ps *ps, int32_t fluid_tag) {
+    if (ps->dvel_nonzero_tag == fluid_tag) ps->dvel_nonzero_tag = -2;
+}
+
 int32_t do_self(eppx_ps *ps, int32_t action, const Physics &ph, const eppx_field_op &op) {
     eppx_ctx *ctx = ps->ctx;
+    if (action == EPPX_SELF_GRAVITY_MOVE) note_dvel_moved(ps, ph.fluid_tag);
+    if (action == EPPX_SELF_SCALE_FIELD && op.field == EPPX_FIELD_DVELOCITY) ps->dvel_nonzero_tag = -1;
     const char *name = action == EPPX_SELF_CONTINUITY_EOS ? "continuity_eos"
                        : action == EPPX_SELF_GRAVITY_MOVE ? "gravity_move"
                        : action == EPPX_SELF_FILTER       ? "filter_self"
@@ -477,6 +496,8 @@ int32_t do_self(eppx_ps *ps, int32_t action, const Physics &ph, const eppx_field
 
 int32_t do_interaction(eppx_ps *ps, eppx_ns *ns, int32_t action, Physics ph, const eppx_field_op &op) {
     eppx_ctx *ctx = ps->ctx;
+    if (action == EPPX_INTER_MOMENTUM) note_dvel_momentum(ps, ph.fluid_tag);
+    if (action == EPPX_INTER_ADD_PER_NEIGHBOUR && op.field == EPPX_FIELD_DVELOCITY) ps->dvel_nonzero_tag = -1;
     if ((action == EPPX_INTER_FILTER || action == EPPX_INTER_EXTRAPOLATE_PRESSURE) && !ps->accum_live) {
         ps->accum_live = true;  // from now on the permute moves the sums (a recorded step moved none)
         drop_graphs(ps);
@@ -803,6 +824,16 @@ int32_t eppx_ps_destroy(eppx_ps *ps) {
 
 // Any non-zero value (NaN included) in rows [0, rows) of the SumWeight,
 // SumWeightedDensity and SumWeightedPressure columns of a float table.
+bool columns_zero(int32_t c0, int32_t width, const float *fp, size_t rows) {
+    if (c0 < 0) return true;
+    for (int32_t c = c0; c < c0 + width; ++c) {
+        const float *col = fp + rows * (size_t)c;
+        for (size_t r = 0; r < rows; ++r)
+            if (!(col[r] == 0.0f)) return false;
+    }
+    return true;
+}
+
 bool accumulators_nonzero(const eppx_layout &L, const float *fp, size_t rows) {
     for (int32_t c : {L.SumWeight, L.SumWeightedDensity, L.SumWeightedPressure}) {
         if (c < 0) continue;
@@ -864,6 +895,7 @@ int32_t eppx_ps_upload(eppx_ps *ps, int32_t n_particles, const int32_t *is_alive
     symmetry_bounds(!bad && has(L.H), range[0], range[1], &ps->rmin2, &ps->rmax2);
     ps->hinv_u = uniform_hinv(!bad && has(L.H), range[0], range[1]);
     ps->accum_live = accumulators_nonzero(L, float_props, cap);  // both buffers now hold these rows
+    ps->dvel_nonzero_tag = columns_zero(L.dVelocityVec, 2, float_props, cap) ? -2 : -1;
     // keep the columns the kernels do not own
     ps->pass_int.assign(ps->pass_int_cols.size() * cap, 0);
     ps->pass_float.assign(ps->pass_float_cols.size() * cap, 0.0f);
@@ -1165,10 +1197,15 @@ int32_t eppx_step(eppx_ps *ps, eppx_ns *ns, const eppx_physics *phys, int32_t n_
     auto tag_bit_h = [](int32_t t) { return (uint32_t)t - 1u < (uint32_t)kMaxTag ? 1u << (t - 1) : 0u; };
     const bool drho_zero = !ps->odd_tags &&
                            (ps->tags_present & ~(tag_bit_h(ph.fluid_tag) | tag_bit_h(ph.wall_tag))) == 0u;
+    // With dVelocity zero on every particle but the fluid, the first step's
+    // move zeroes the fluid's and the permute copies the zeros: after it both
+    // buffers hold zeros on every alive slot, so the later steps start the
+    // momentum sums at zero and neither the move nor the permute writes them.
+    const bool dvel_skip = fuse_move && (ps->dvel_nonzero_tag == -2 || ps->dvel_nonzero_tag == ph.fluid_tag);
     // eos_first: the step starts with sContinuity! (first step of the call,
     // or after a step whose search did not apply it); fuse_eos: its search
-    // applies the next step's.
-    auto enqueue_step = [&](bool eos_first, bool fuse_eos, bool filter_now) -> int32_t {
+    // applies the next step's; dvel_zero: dVelocity is zero on every alive slot.
+    auto enqueue_step = [&](bool eos_first, bool fuse_eos, bool filter_now, bool dvel_zero) -> int32_t {
         int32_t r;
         if (eos_first)
             HIP_OK(ctx, timed(ctx, "continuity_eos", [&] {
@@ -1176,7 +1213,7 @@ int32_t eppx_step(eppx_ps *ps, eppx_ns *ns, const eppx_physics *phys, int32_t n_
                    }));
         if (fuse_move) {
             const MoveKey mk{ps->n, nc, ns->M, ps->counts, ps->key, ps->tmpslot, ctx->d_flags, ps->dyn_new,
-                             &ps->pairs[0]};
+                             &ps->pairs[0], dvel_zero ? 1 : 0};
             if ((r = do_momentum_move(ps, ns, ph, mk))) return r;
         } else {
             if ((r = do_interaction(ps, ns, EPPX_INTER_MOMENTUM, ph, op))) return r;
@@ -1188,7 +1225,7 @@ int32_t eppx_step(eppx_ps *ps, eppx_ns *ns, const eppx_physics *phys, int32_t n_
         const EosOut eo{ps->n, nc + 2, ps->vol_new, ps->counts};
         if ((r = do_search(ps, ns, EPPX_CRIT_SYMMETRY, EPPX_SEARCH_KERNEL_CONTINUITY, ph, true,
                            fuse_move ? ps->dyn_new : nullptr, eos_first ? nullptr : ps->vol_new,
-                           fuse_eos ? &eo : nullptr, drho_zero, drho_zero && !eos_first)))
+                           fuse_eos ? &eo : nullptr, drho_zero, drho_zero && !eos_first, dvel_zero)))
             return r;
         if (filter_now) {
             if ((r = do_interaction(ps, ns, EPPX_INTER_FILTER, ph, op))) return r;
@@ -1204,17 +1241,18 @@ int32_t eppx_step(eppx_ps *ps, eppx_ns *ns, const eppx_physics *phys, int32_t n_
         const bool fuse = fuse_eos_ok && k < n_steps && !filter_now;
         const bool eos_first = eos_pending;
         eos_pending = !fuse;
+        const bool dvel_zero = dvel_skip && k > 1;
         if (filter_now || ctx->timing || !ps->masks_valid) {  // filter steps and per-kernel timing run eagerly
-            if ((rc = enqueue_step(eos_first, fuse, filter_now))) return rc;
+            if ((rc = enqueue_step(eos_first, fuse, filter_now, dvel_zero))) return rc;
             continue;
         }
-        StepGraph &g = ps->graphs[ps->cur][(eos_first ? 0 : 2) + (fuse ? 1 : 0)];
+        StepGraph &g = ps->graphs[ps->cur][(eos_first ? 0 : 2) + (fuse ? 1 : 0) + (dvel_zero ? 4 : 0)];
         if (!g.exec || g.ns_serial != ns->serial || g.math != ctx->math_mode || memcmp(&g.ph, &ph, sizeof(ph)) != 0) {
             if (g.exec) (void)hipGraphExecDestroy(g.exec);
             g = StepGraph{};
             const int cur0 = ps->cur;
             HIP_OK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
-            rc = enqueue_step(eos_first, fuse, false);
+            rc = enqueue_step(eos_first, fuse, false, dvel_zero);
             hipGraph_t graph = nullptr;
             const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
             ps->cur = cur0;  // recorded, not run yet
@@ -1235,6 +1273,7 @@ int32_t eppx_step(eppx_ps *ps, eppx_ns *ns, const eppx_physics *phys, int32_t n_
         HIP_OK(ctx, hipGraphLaunch(g.exec, ctx->stream));
         ps->cur ^= 1;
     }
+    if (n_steps > 0) note_dvel_moved(ps, ph.fluid_tag);  // every step ends with the fluid's dVelocity zeroed
     return EPPX_OK;
 }
 

@@ -125,10 +125,10 @@ hipError_t launch_fill_members(const Fields f, int32_t n, int32_t n_cells, const
                                const int32_t *tmpslot, const int32_t *start, int32_t *members, hipStream_t s);
 // opts: kPermuteSums moves SumWeight, SumWeightedDensity and
 // SumWeightedPressure (without it they are zero in both buffers and stay);
-// kPermuteNoDrho leaves dDensity of the alive slots alone (it is zero there
-// in both buffers).  dyn_src / vol_src (null: cur's) hold the slots'
+// kPermuteNoDrho / kPermuteNoDvel leave dDensity / dVelocity of the alive
+// slots alone (zero there in both buffers).  dyn_src / vol_src (null: cur's) hold the slots'
 // (u, v, rho, p) records and volumes when a fused kernel wrote them aside.
-constexpr int32_t kPermuteSums = 1, kPermuteNoDrho = 2;
+constexpr int32_t kPermuteSums = 1, kPermuteNoDrho = 2, kPermuteNoDvel = 4;
 hipError_t launch_permute(Fields cur, Fields nxt, float4 *geo, int32_t n, int32_t batch, int32_t n_cells,
                           const int32_t *key, const int32_t *tmpslot, const int32_t *start,
                           const int32_t *members, int32_t *n_alive, uint32_t *flags, int32_t opts,
@@ -166,6 +166,9 @@ struct MoveKey {
     uint32_t *flags;
     float4 *dyn_new;
     unsigned long long *pairs_last;
+    // 1: dVelocity is zero on every alive slot (a step after the first of a
+    // call): the sum starts at zero and the move leaves the zero in place
+    int32_t dvel_zero;
 };
 hipError_t launch_momentum_move(const Walk &w, Fields f, int32_t n, const Physics &ph, const MoveKey &mk,
                                 int32_t fast, int32_t block, hipStream_t s);

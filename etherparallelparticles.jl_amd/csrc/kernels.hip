@@ -818,9 +818,10 @@ __global__ __launch_bounds__(kBlock) void k_permute(Fields cur, Fields nxt, floa
     const int32_t tmp = tmpslot[s];
     const float2 x = cur.pos[s];
     const float4 dyn = (dyn_src ? dyn_src : cur.dyn)[s];
-    const float2 dvel = cur.dvel[s];
     const bool move_accum = opts & kPermuteSums;
     const bool move_drho = !(opts & kPermuteNoDrho) || k >= n_cells;  // dead slots keep theirs
+    const bool move_dvel = !(opts & kPermuteNoDvel) || k >= n_cells;
+    const float2 dvel = move_dvel ? cur.dvel[s] : make_float2(0.0f, 0.0f);
     const float mass = cur.mass[s], vol = (vol_src ? vol_src : cur.vol)[s], h = cur.h[s];
     const float drho = move_drho ? cur.drho[s] : 0.0f;
     // the filter and extrapolation sums, when they may be non-zero
@@ -861,7 +862,7 @@ __global__ __launch_bounds__(kBlock) void k_permute(Fields cur, Fields nxt, floa
     nxt.pos[dest] = x;
     geo[dest] = make_float4(x.x, x.y, h, mass);
     nxt.dyn[dest] = dyn;
-    nxt.dvel[dest] = dvel;
+    if (move_dvel) nxt.dvel[dest] = dvel;
     nxt.mass[dest] = mass;
     nxt.vol[dest] = vol;
     if (move_drho) nxt.drho[dest] = drho;
@@ -1229,7 +1230,7 @@ __global__ __launch_bounds__(BLOCK, BATCH ? 1 : 8) void k_momentum(Walk w, Field
     const float4 gi = w.geo[i];
     di = f.dyn[i];
     const float2 ui = make_float2(di.x, di.y);
-    acc = f.dvel[i];
+    acc = (MOVE && mk.dvel_zero) ? make_float2(0.0f, 0.0f) : f.dvel[i];
     // p_i / rho_i^2 of the classic pressure, once (EXACT: the reference's expression, the same bits)
     const float pr_i = FAST ? di.w * frcp(di.z * di.z) : di.w / (di.z * di.z);
     const uint32_t fluid_bit = tag_bit(ph.fluid_tag);
@@ -1304,7 +1305,7 @@ __global__ __launch_bounds__(BLOCK, BATCH ? 1 : 8) void k_momentum(Walk w, Field
             if (act) {
                 gravity_move_regs(x, d, acc, ph);  // du = the momentum result
                 f.pos[i] = x;
-                f.dvel[i] = make_float2(0.0f, 0.0f);
+                if (!mk.dvel_zero) f.dvel[i] = make_float2(0.0f, 0.0f);
             }
             mk.dyn_new[i] = d;
             k = cell_key_of(f, i, cd, mk.n_cells, alive, mov, cell, x);
