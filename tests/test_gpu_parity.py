@@ -324,6 +324,34 @@ def test_particles_dying_inside_a_call_exact(parallel):
     assert off[-1] == pairs[-1]
 
 
+@pytest.mark.parametrize("start", ["walls_moving_dvel", "momentum_before_step"])
+def test_step_calls_with_nonzero_dvelocity_exact(parallel, start):
+    """A step call whose first step finds non-zero dVelocity: on wall particles
+    (the momentum pass never writes theirs, so it stays and the permutes must
+    move it in every step), or on the fluid after a lone iMomentum! (the first
+    step adds to it).  6 steps, bit-identical to the restatement."""
+    from eppx import Algorithm, SPH, cases
+    case = cases.small(M=40)
+    h = case.host
+    ph = case.physics
+    oph = U.oracle_physics(ph)
+    if start == "walls_moving_dvel":
+        tag = h.int_properties_[: h.n_particles_, h.named_index_.int_table.index["Tag"] - 1]
+        du = h.named_index_.float_table.index["dVelocityVec"] - 1
+        walls = np.flatnonzero(tag == 2)[::50]
+        h.float_properties_[walls, du] = 0.25
+        h.float_properties_[walls, du + 1] = -0.5
+    ps, ns, ref = U.search_both(parallel, case)
+    if start == "momentum_before_step":
+        Algorithm.interaction(ps, SPH.iMomentum(ph))
+        ref.interaction(1, oph)
+    for n in (1, 6):  # a one-step call, then one whose later steps may skip dVelocity
+        Algorithm.step(ps, ns, ph, n_steps=n)
+        err, _ = ref.run_steps(n, oph)
+        assert err == 0
+        U.assert_state_equal(U.download(ps), ref, alive_only=False)
+
+
 def _fast_steps_errors(parallel, case, n):
     from eppx import Algorithm
     parallel.set_math("fast")
