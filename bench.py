@@ -33,14 +33,16 @@ HBM_PEAK_GBS = 8000.0
 # the cell-order move are implementation cost and are not counted), plus the
 # cell-start array read once per cell.
 ALG_BYTES = {
-    # the search with the next step's continuity fused (every step of a call but its last):
-    # pos_nb 8, vel 8, mass 4, vol 4, rho 4, drho 4 (rmw), tag 4, h 4, cell 4 in; drho 4, count 4 out;
-    # rho 4, p 4, vol 4 out; 4 B/cell of cell table read and 4 B/cell of counts zeroed
-    "search": (64, 8),
-    # momentum with gravity + move and the cell keys fused: pos_nb 8, vel 8, rho 4, p 4, mass 4, tag 4,
-    # h 4, cell 4, dvel 8 in; dvel 8 out; then pos 8, alive 4, mov 4, cell 4 in; pos 8, vel 8, key 4,
-    # slot 4 out; 4 B/cell of counts
-    "momentum": (100, 4),
+    # the search with the next step's continuity fused (every step of a call but its last; dDensity
+    # is zero there and is neither read nor written): pos_nb 8, vel 8, mass 4, vol 4, rho 4, tag 4,
+    # h 4, cell 4 in; count 4 out; rho 4, p 4, vol 4 out -- the 56 B of the step's pass after the
+    # move -- plus 4 B/cell of cell table read and 4 B/cell of counts zeroed
+    "search": (56, 8),
+    # momentum with gravity + move and the cell keys fused (the steps after a call's first: dVelocity
+    # is zero and is neither read nor written): pos_nb 8, vel 8, rho 4, p 4, mass 4, tag 4, h 4,
+    # cell 4 in; then pos 8, alive 4, mov 4, cell 4 in; pos 8, vel 8, key 4, slot 4 out; 4 B/cell
+    # of counts
+    "momentum": (84, 4),
     # alive 4, tag 4, rho 4, drho 4, mass 4 in; rho 4, drho 4, vol 4, p 4 out
     "continuity_eos": (36, 0),
     # alive 4, tag 4, dvel 8, pos 8, vel 8 in; dvel 8, pos 8, vel 8 out
