@@ -1248,7 +1248,10 @@ int32_t eppx_step(eppx_ps *ps, eppx_ns *ns, const eppx_physics *phys, int32_t n_
         }
         StepGraph &g = ps->graphs[ps->cur][(eos_first ? 0 : 2) + (fuse ? 1 : 0) + (dvel_zero ? 4 : 0)];
         if (!g.exec || g.ns_serial != ns->serial || g.math != ctx->math_mode || memcmp(&g.ph, &ph, sizeof(ph)) != 0) {
-            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+            if (g.exec) {
+                (void)hipStreamSynchronize(ctx->stream);  // an earlier launch of it may still run
+                (void)hipGraphExecDestroy(g.exec);
+            }
             g = StepGraph{};
             const int cur0 = ps->cur;
             HIP_OK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
@@ -1773,7 +1776,10 @@ int32_t eppx_batch_step(eppx_batch *b, int32_t n_steps) {
         }
         StepGraph &g = b->graphs[ps->cur];
         if (!g.exec || g.ns_serial != b->generation || g.math != ctx->math_mode) {
-            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+            if (g.exec) {
+                (void)hipStreamSynchronize(ctx->stream);  // an earlier launch of it may still run
+                (void)hipGraphExecDestroy(g.exec);
+            }
             g = StepGraph{};
             const int cur0 = ps->cur;
             HIP_OK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
