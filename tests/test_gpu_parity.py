@@ -209,6 +209,53 @@ def test_fast_mode_within_tolerance_wendland_mid_run(parallel):
     assert du_err <= 1e-5
 
 
+def test_large_fast_steps_within_tolerance(parallel):
+    """The benchmark case at full size (2,030,018 particles): one call of 10
+    fused FAST steps (the liveness skips and hoisted one-H terms included)
+    against the same call in EXACT math, which the other tests hold
+    bit-identical to the restatement.  The same particles alive, Mass exact,
+    the same pair count (pair sets are exact in both modes while positions
+    agree), density within 1e-5 relative, positions within 1e-6 m."""
+    import ctypes as C
+    from eppx import Algorithm, SPH, _lib, cases
+    case = cases.large(M=64)
+    L = _lib.lib()
+    out = {}
+    for math in ("exact", "fast"):
+        parallel.set_math(math)
+        try:
+            ps, ns = U.device(parallel, case)
+            Algorithm.search(ps, case.domain, ns, criterion=Algorithm.symmetrySearchCriterion,
+                             action=SPH.sComputeKernelAndiContinuity(case.physics))
+            total = C.c_int64()
+            _lib.check(L.eppx_pair_total(ps.handle, C.byref(total), 1), parallel.ctx, "pair_total")
+            Algorithm.step(ps, ns, case.physics, n_steps=10)
+            _lib.check(L.eppx_pair_total(ps.handle, C.byref(total), 1), parallel.ctx, "pair_total")
+            host = U.download(ps)
+            lay = ps.layout
+            n = case.n_particles
+            out[math] = (host.is_alive_[:n].copy(), host.float_properties_[:n, lay.Mass].copy(),
+                         host.float_properties_[:n, lay.Density].astype(np.float64),
+                         host.float_properties_[:n, lay.PositionVec: lay.PositionVec + 2].astype(np.float64),
+                         total.value)
+            del host
+            ps.close()
+            ns.close()
+        finally:
+            parallel.set_math("exact")
+    (ae, me, re_, xe, pe), (af, mf, rf, xf, pf) = out["exact"], out["fast"]
+    assert np.array_equal(ae, af) and (ae == 1).all()
+    assert np.array_equal(me, mf)
+    assert pe == pf and pe > 0
+    alive = ae == 1
+    rho_err = np.max(np.abs(rf[alive] - re_[alive]) / np.abs(re_[alive]))
+    pos_err = np.max(np.abs(xf[alive] - xe[alive]))
+    print(f"2M case, 10 steps FAST vs EXACT: density {rho_err:.3g} relative, positions {pos_err:.3g} m, "
+          f"{pe} pairs over the steps")
+    assert rho_err <= 1e-5
+    assert pos_err <= 1e-6
+
+
 def test_large_first_search_exact(parallel):
     """2,030,018 particles (dx = 0.001): every pair list and drho identical."""
     from eppx import cases
