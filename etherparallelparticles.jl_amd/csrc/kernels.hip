@@ -1200,7 +1200,9 @@ __device__ __forceinline__ void momentum_pair(const Physics &ph, float2 &acc, bo
 // sMomentum! (gravity + accelerate-move) on the result and takes the cell key
 // and count of every slot for the coming search (MoveKey).
 template <bool FAST, int K, int VISC, bool BATCH, int BLOCK, int STAGE, bool MOVE>
-__global__ __launch_bounds__(BLOCK, BATCH ? 1 : 8) void k_momentum(Walk w, Fields f, Physics ph, MoveKey mk) {
+// Eight waves per SIMD; the EXACT batch kernels would spill at that bound and keep the compiler's choice.
+__global__ __launch_bounds__(BLOCK, (BATCH && !FAST) ? 1 : 8) void k_momentum(Walk w, Fields f, Physics ph,
+                                                                               MoveKey mk) {
     constexpr int kTileRow = PairShape<BLOCK>::tile_row;
     __shared__ float2 txy[3 * kTileRow];
     __shared__ float2 thm[3 * kTileRow];
