@@ -183,6 +183,23 @@ struct EosOut {
 hipError_t launch_search_eos(const Walk &w, Fields f, int32_t n, const Physics &ph, const EosOut &eo, int32_t fast,
                              int32_t M, int32_t *ncount, unsigned long long *pairs_last,
                              unsigned long long *pairs_total, uint32_t *flags, int32_t block, hipStream_t s);
+// The pair-kernel instantiations, one translation unit per math mode and
+// kernel function (search.hip, momentum.hip compiled with EPPX_TU_FAST and
+// EPPX_TU_KERNEL).  eos non-null: the fused next-step continuity (particle
+// systems); mk non-null: the fused move and cell keys (particle systems).
+struct PairLaunch {
+    int32_t n, M;
+    int32_t *ncount;
+    unsigned long long *pairs_last, *pairs_total;
+    uint32_t *flags;
+    int32_t which, block, stage;
+    const EosOut *eos;
+};
+template <bool FAST, int K>
+hipError_t launch_search_pair(const Walk &w, Fields f, const Physics &ph, const PairLaunch &pl, hipStream_t s);
+template <bool FAST, int K>
+hipError_t launch_momentum_pair(const Walk &w, Fields f, const Physics &ph, int32_t n, int32_t block, int32_t stage,
+                                const MoveKey *mk, hipStream_t s);
 hipError_t launch_interaction(const Walk &w, Fields f, int32_t n, int32_t action, const Physics &ph,
                               const eppx_field_op &op, int32_t fast, int32_t block, int32_t stage, hipStream_t s);
 hipError_t launch_export(const Walk &w, Fields f, int32_t n, const int64_t *row_start, int32_t *js,
