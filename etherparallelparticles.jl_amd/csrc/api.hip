@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdio>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -107,6 +108,45 @@ struct eppx_ps {
     std::vector<float> pass_float;
     Fields &f() { return buf[cur]; }
 };
+
+// Fixed device scratch of eppx_ps_download_noalloc: the staging of the
+// reference tables, for callers that cannot allocate device memory.  Every
+// device the library runs on holds one copy for the life of the process.
+// In host code `scratch` names a host-side stand-in, not a device address:
+// the address of each device's copy is taken once per device, with that
+// device current (eppx_ctx_create), and only that address reaches kernels
+// and copies.  Calls on one device share it, so overlapping calls race.
+constexpr size_t kScratchBytes = (size_t)128 << 20;  // 128 MiB
+constexpr size_t kScratchWords = kScratchBytes / sizeof(uint32_t);
+static __device__ __align__(256) uint32_t scratch[kScratchWords];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+uint32_t *g_scratch_ptr[kMaxDevices] = {};
+bool g_scratch_tried[kMaxDevices] = {};
+
+// Takes the device address of `scratch` on `device` (current) once; returns
+// the hipError_t of hipGetSymbolAddress on the first call for the device.
+hipError_t init_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_tried[device]) return hipSuccess;
+    g_scratch_tried[device] = true;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e == hipSuccess) g_scratch_ptr[device] = static_cast<uint32_t *>(p);
+    return e;
+}
+
+uint32_t *scratch_ptr(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    return g_scratch_ptr[device];
+}
+
+}  // namespace
 
 namespace {
 
@@ -649,6 +689,9 @@ int32_t eppx_ctx_create(int32_t device, eppx_ctx **out) {
         delete ctx;
         return EPPX_E_HIP;
     }
+    // the no-alloc download's scratch address on this device (once per device; a
+    // failure is reported by that entry point, which then has no address)
+    (void)init_scratch(device);
     *out = ctx;
     return EPPX_OK;
 }
@@ -910,26 +953,37 @@ int32_t eppx_ps_upload(eppx_ps *ps, int32_t n_particles, const int32_t *is_alive
     return EPPX_OK;
 }
 
-int32_t eppx_ps_download(eppx_ps *ps, eppx_ns *ns, int32_t *n_particles, int32_t *is_alive, int32_t *cell_index,
-                         int32_t *int_props, float *float_props) {
-    if (!ps) return EPPX_E_ARG;
+}  // extern "C"
+
+namespace {
+
+// Device staging of a download: the reference tables in their host layout.
+struct Staging {
+    int32_t *ip, *alive, *cell;
+    float *fp;
+};
+
+// Download of ps into the reference tables through device staging `st`
+// (cap * (n_int + n_float + 2) words); fill: clear the staging with a fill
+// kernel instead of hipMemsetAsync.
+int32_t download_via(eppx_ps *ps, eppx_ns *ns, int32_t *is_alive, int32_t *cell_index, int32_t *int_props,
+                     float *float_props, const Staging &st, bool fill) {
     eppx_ctx *ctx = ps->ctx;
     const eppx_layout &L = ps->layout;
-    if (!is_alive || !cell_index || (L.n_int && !int_props) || (L.n_float && !float_props))
-        return set_error(ctx, EPPX_E_ARG, "eppx_ps_download: bad argument");
-    HIP_OK(ctx, hipSetDevice(ctx->device));
     const size_t cap = (size_t)ps->cap;
-    if (n_particles) *n_particles = ps->n;
-    if (!cap) return read_flags(ctx);
-    int32_t *d_ip = nullptr, *d_alive = nullptr, *d_cell = nullptr;
-    float *d_fp = nullptr;
-    HIP_OK(ctx, dalloc(&d_ip, cap * (size_t)L.n_int));
-    HIP_OK(ctx, dalloc(&d_fp, cap * (size_t)L.n_float));
-    HIP_OK(ctx, dalloc(&d_alive, cap));
-    HIP_OK(ctx, dalloc(&d_cell, cap));
+    int32_t *d_ip = st.ip, *d_alive = st.alive, *d_cell = st.cell;
+    float *d_fp = st.fp;
     hipStream_t s = ctx->stream;
-    hipError_t e = hipMemsetAsync(d_ip, 0, std::max<size_t>(1, cap * L.n_int) * sizeof(int32_t), s);
-    if (e == hipSuccess) e = hipMemsetAsync(d_fp, 0, std::max<size_t>(1, cap * L.n_float) * sizeof(float), s);
+    hipError_t e = hipSuccess;
+    if (fill) {
+        // int 0 and float +0.0 share their bits
+        if (cap * L.n_int) e = launch_fill_i32(d_ip, 0, (int64_t)(cap * L.n_int), s);
+        if (e == hipSuccess && cap * L.n_float)
+            e = launch_fill_i32(reinterpret_cast<int32_t *>(d_fp), 0, (int64_t)(cap * L.n_float), s);
+    } else {
+        e = hipMemsetAsync(d_ip, 0, std::max<size_t>(1, cap * L.n_int) * sizeof(int32_t), s);
+        if (e == hipSuccess) e = hipMemsetAsync(d_fp, 0, std::max<size_t>(1, cap * L.n_float) * sizeof(float), s);
+    }
     if (e == hipSuccess) e = launch_to_ref(ps->f(), ps->cap, columns_of(ps), d_ip, d_fp, d_alive, d_cell, -1, s);
     const int32_t M = L.max_neighbour_number;
     if (e == hipSuccess && ns && ps->searched && ps->last_ns == ns &&
@@ -957,10 +1011,6 @@ int32_t eppx_ps_download(eppx_ps *ps, eppx_ns *ns, int32_t *n_particles, int32_t
     if (e == hipSuccess) e = hipMemcpyAsync(is_alive, d_alive, cap * sizeof(int32_t), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipMemcpyAsync(cell_index, d_cell, cap * sizeof(int32_t), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
-    dfree(d_ip);
-    dfree(d_fp);
-    dfree(d_alive);
-    dfree(d_cell);
     if (e != hipSuccess) return hip_fail(ctx, e, "eppx_ps_download");
     if (ps->pass_int.size() == ps->pass_int_cols.size() * cap)
         for (size_t k = 0; k < ps->pass_int_cols.size(); ++k)
@@ -968,7 +1018,77 @@ int32_t eppx_ps_download(eppx_ps *ps, eppx_ns *ns, int32_t *n_particles, int32_t
     if (ps->pass_float.size() == ps->pass_float_cols.size() * cap)
         for (size_t k = 0; k < ps->pass_float_cols.size(); ++k)
             memcpy(float_props + (size_t)ps->pass_float_cols[k] * cap, &ps->pass_float[k * cap], cap * sizeof(float));
-    return read_flags(ctx);
+    return EPPX_OK;
+}
+
+int32_t download_args(eppx_ps *ps, int32_t *is_alive, int32_t *cell_index, int32_t *int_props, float *float_props) {
+    const eppx_layout &L = ps->layout;
+    if (!is_alive || !cell_index || (L.n_int && !int_props) || (L.n_float && !float_props))
+        return set_error(ps->ctx, EPPX_E_ARG, "eppx_ps_download: bad argument");
+    return EPPX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t eppx_ps_download(eppx_ps *ps, eppx_ns *ns, int32_t *n_particles, int32_t *is_alive, int32_t *cell_index,
+                         int32_t *int_props, float *float_props) {
+    if (!ps) return EPPX_E_ARG;
+    eppx_ctx *ctx = ps->ctx;
+    const eppx_layout &L = ps->layout;
+    int32_t rc = download_args(ps, is_alive, cell_index, int_props, float_props);
+    if (rc) return rc;
+    HIP_OK(ctx, hipSetDevice(ctx->device));
+    const size_t cap = (size_t)ps->cap;
+    if (n_particles) *n_particles = ps->n;
+    if (!cap) return read_flags(ctx);
+    Staging st{};
+    hipError_t e = dalloc(&st.ip, cap * (size_t)L.n_int);
+    if (e == hipSuccess) e = dalloc(&st.fp, cap * (size_t)L.n_float);
+    if (e == hipSuccess) e = dalloc(&st.alive, cap);
+    if (e == hipSuccess) e = dalloc(&st.cell, cap);
+    rc = e == hipSuccess ? download_via(ps, ns, is_alive, cell_index, int_props, float_props, st, false)
+                         : hip_fail(ctx, e, "eppx_ps_download");
+    dfree(st.ip);
+    dfree(st.fp);
+    dfree(st.alive);
+    dfree(st.cell);
+    return rc ? rc : read_flags(ctx);
+}
+
+int32_t eppx_ps_download_noalloc(eppx_ps *ps, eppx_ns *ns, int32_t *n_particles, int32_t *is_alive,
+                                 int32_t *cell_index, int32_t *int_props, float *float_props) {
+    if (!ps) return EPPX_E_ARG;
+    eppx_ctx *ctx = ps->ctx;
+    const eppx_layout &L = ps->layout;
+    int32_t rc = download_args(ps, is_alive, cell_index, int_props, float_props);
+    if (rc) return rc;
+    const size_t cap = (size_t)ps->cap;
+    // staging words: the int and float tables, is_alive and cell_index, each table 64-word aligned
+    auto up = [](size_t w) { return (w + 63) & ~(size_t)63; };
+    const size_t w_ip = up(cap * (size_t)L.n_int), w_fp = up(cap * (size_t)L.n_float), w_v = up(cap);
+    const size_t words = w_ip + w_fp + 2 * w_v;
+    if (words > kScratchWords) {
+        char buf[256];
+        snprintf(buf, sizeof(buf),
+                 "eppx_ps_download_noalloc: the tables need %zu staging words (capacity %zu x %d columns + 2); "
+                 "the fixed device scratch holds %zu (use eppx_ps_download)",
+                 words, cap, L.n_int + L.n_float, (size_t)kScratchWords);
+        return set_error(ctx, EPPX_E_ARG, buf);
+    }
+    HIP_OK(ctx, hipSetDevice(ctx->device));
+    if (n_particles) *n_particles = ps->n;
+    if (!cap) return read_flags(ctx);
+    uint32_t *base = scratch_ptr(ctx->device);
+    if (!base) return set_error(ctx, EPPX_E_HIP, "eppx_ps_download_noalloc: no device address for the scratch buffer");
+    Staging st{};
+    st.ip = reinterpret_cast<int32_t *>(base);
+    st.fp = reinterpret_cast<float *>(base + w_ip);
+    st.alive = reinterpret_cast<int32_t *>(base + w_ip + w_fp);
+    st.cell = reinterpret_cast<int32_t *>(base + w_ip + w_fp + w_v);
+    rc = download_via(ps, ns, is_alive, cell_index, int_props, float_props, st, true);
+    return rc ? rc : read_flags(ctx);
 }
 
 int32_t eppx_ps_alive_count(eppx_ps *ps, int32_t *out) {

@@ -94,6 +94,7 @@ def lib() -> C.CDLL:
             "eppx_ps_destroy": ([vp], i32),
             "eppx_ps_upload": ([vp, i32, P(i32), P(i32), P(i32), P(C.c_float)], i32),
             "eppx_ps_download": ([vp, vp, P(i32), P(i32), P(i32), P(i32), P(C.c_float)], i32),
+            "eppx_ps_download_noalloc": ([vp, vp, P(i32), P(i32), P(i32), P(i32), P(C.c_float)], i32),
             "eppx_ps_alive_count": ([vp, P(i32)], i32),
             "eppx_ns_create": ([vp, P(Domain2D), P(i32), i32, i32, P(vp)], i32),
             "eppx_ns_destroy": ([vp], i32),

@@ -170,6 +170,17 @@ int32_t eppx_ps_upload(eppx_ps *ps, int32_t n_particles, const int32_t *is_alive
  * where the last search action was EPPX_SEARCH_KERNEL_CONTINUITY). */
 int32_t eppx_ps_download(eppx_ps *ps, eppx_ns *ns, int32_t *n_particles, int32_t *is_alive, int32_t *cell_index,
                          int32_t *int_props, float *float_props);
+/* No counterpart in the reference.  eppx_ps_download for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving or
+ * output loops, harnesses that forbid hipMalloc): the same arguments and
+ * results, staged through a fixed 128 MiB device buffer that the library
+ * holds on every device it runs on (no hipMalloc / hipFree per call).  Fails
+ * with EPPX_E_ARG when the tables need more staging than that:
+ * n_capacity * (n_int + n_float + 2) 4-byte words, each table rounded up to
+ * 64 words, must not exceed 33,554,432.  Calls on one device share the
+ * buffer: calls that overlap on one device race (waits). */
+int32_t eppx_ps_download_noalloc(eppx_ps *ps, eppx_ns *ns, int32_t *n_particles, int32_t *is_alive,
+                                 int32_t *cell_index, int32_t *int_props, float *float_props);
 /* Class.get_alive_n_particles, ParticleSystem.jl:154-158 (waits). */
 int32_t eppx_ps_alive_count(eppx_ps *ps, int32_t *out);
 
