@@ -815,8 +815,8 @@ int32_t eppx_ps_create(eppx_ctx *ctx, int32_t n_capacity, const eppx_layout *lay
     hipError_t e = alloc_fields(ps->buf[0], n_capacity);
     if (e == hipSuccess) e = alloc_fields(ps->buf[1], n_capacity);
     if (e == hipSuccess) e = dalloc(&ps->geo, (size_t)n_capacity);
-    if (e == hipSuccess) e = dalloc(&ps->fallback, 1);
-    if (e == hipSuccess) e = hipMemsetAsync(ps->fallback, 0, sizeof(uint32_t), ctx->stream);
+    if (e == hipSuccess) e = dalloc(&ps->fallback, 2);
+    if (e == hipSuccess) e = hipMemsetAsync(ps->fallback, 0, 2 * sizeof(uint32_t), ctx->stream);
     if (e == hipSuccess) e = dalloc(&ps->key, (size_t)n_capacity);
     if (e == hipSuccess) e = dalloc(&ps->tmpslot, (size_t)n_capacity);
     if (e == hipSuccess) e = dalloc(&ps->members, (size_t)n_capacity);
@@ -1510,6 +1510,18 @@ int32_t eppx_pair_fallback_blocks(eppx_ps *ps, int64_t *out, int32_t reset) {
     uint32_t v = 0;
     HIP_OK(ctx, hipMemcpyAsync(&v, ps->fallback, sizeof(v), hipMemcpyDeviceToHost, ctx->stream));
     if (reset) HIP_OK(ctx, hipMemsetAsync(ps->fallback, 0, sizeof(uint32_t), ctx->stream));
+    HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
+    *out = v;
+    return EPPX_OK;
+}
+
+int32_t eppx_pair_row_fallback(eppx_ps *ps, int64_t *out, int32_t reset) {
+    if (!ps || !out) return EPPX_E_ARG;
+    eppx_ctx *ctx = ps->ctx;
+    HIP_OK(ctx, hipSetDevice(ctx->device));
+    uint32_t v = 0;
+    HIP_OK(ctx, hipMemcpyAsync(&v, ps->fallback + 1, sizeof(v), hipMemcpyDeviceToHost, ctx->stream));
+    if (reset) HIP_OK(ctx, hipMemsetAsync(ps->fallback + 1, 0, sizeof(uint32_t), ctx->stream));
     HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
     *out = v;
     return EPPX_OK;

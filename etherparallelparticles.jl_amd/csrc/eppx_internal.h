@@ -69,7 +69,8 @@ struct Walk {
     const int32_t *n_alive;   // device scalars: [0] alive slots, [1] kept slots
     uint32_t adj[kMaxTag + 1];  // adj[t]: bit (u-1) set when tag t is adjacent to tag u
     int32_t criterion;
-    uint32_t *fallback;       // blocks that walked global memory instead of LDS (counter)
+    uint32_t *fallback;       // counters: [0] blocks that read global memory instead of LDS,
+                              // [1] particle walks whose neighbour rows exceed 64 slots
     // The search's accept masks of the three neighbour rows of each slot
     // (masks[r * mask_stride + i]): mask_mode 1 stores them, 2 reads them
     // instead of testing the candidates again (same pair set), 0 neither.

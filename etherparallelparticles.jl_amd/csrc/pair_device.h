@@ -432,6 +432,9 @@ __device__ __forceinline__ int walk_rows(const Walk &w, const CaseDesc &cd, cons
         }
     }
     if (e[0] - b[0] > 64 || e[1] - b[1] > 64 || e[2] - b[2] > 64) {
+        // counted once per wave: the lanes that take this path
+        const unsigned long long lanes = __ballot(1);
+        if (lane_id() == __ffsll((long long)lanes) - 1) atomicAdd(&w.fallback[1], (uint32_t)__popcll(lanes));
         int cnt = 0;
         walk(w, cd, i, gi, [&](int j, uint32_t tbj, float dx, float dy, float dr2) {
             ++cnt;

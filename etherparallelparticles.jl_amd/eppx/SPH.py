@@ -75,6 +75,21 @@ def dam_break_physics(dx: float, h_factor: float, kernel=WendlandC2, g: float = 
                    gx=0.0, gy=-g, alpha=alpha, beta=beta, av_c=float(np.float32(c0)))
 
 
+def example_physics(dx: float, h_factor: float = 1.5, kernel=CubicSpline, c0: float = 120.0, g: float = 9.8,
+                    mu0: float = 1e-3, rho0: float = 1000.0, p0: float = 0.0) -> Physics:
+    """The reference example's own constants (example/collapse_dry/same/main.jl:57-70, casegen.jl:71-79):
+    linear EOS p = c0^2 (rho - rho0) + p0 with c0 = 120, classic viscosity with mu0 = 1e-3
+    (main.jl:110,114 -> Viscosity.jl:10-26), g = 9.8 and dt = parallel(0.1 h0 / c0) * 2 -- h0 and c0
+    are Float32 parameters, the product and quotient run in Float64, the result is rounded to Float32
+    and then doubled."""
+    f32 = np.float32
+    h0, c = f32(h_factor * dx), f32(c0)
+    dt = f32(f32(0.1 * np.float64(h0) / np.float64(c)) * f32(2))
+    return Physics(kernel=kernel.id, eos=_lib.EOS_LINEAR, visc=_lib.VISC_CLASSIC, dt=float(dt), rho0=rho0, p0=p0,
+                   c0=float(c), tait_b=0.0, gx=0.0, gy=float(f32(-g)), mu=float(f32(mu0)), alpha=0.0, beta=0.0,
+                   av_c=0.0)
+
+
 # ------------------------------------------------------------------ actions
 
 

@@ -108,6 +108,7 @@ def lib() -> C.CDLL:
             "eppx_export_pairs": ([vp, vp, P(i64), P(i32), i64], i32),
             "eppx_check_finite": ([vp], i32),
             "eppx_pair_fallback_blocks": ([vp, P(i64), i32], i32),
+            "eppx_pair_row_fallback": ([vp, P(i64), i32], i32),
             "eppx_set_timing": ([vp, i32], i32),
             "eppx_kernel_time": ([vp, C.c_char_p, P(C.c_double), P(i64), i32], i32),
             "eppx_batch_bytes_per_particle": ([P(i32)], i32),

@@ -249,6 +249,12 @@ int32_t eppx_export_pairs(eppx_ps *ps, eppx_ns *ns, int64_t *offsets, int32_t *j
  * their neighbourhood in on-chip memory and read it from HBM instead, since
  * the last reset (a performance diagnostic; results do not depend on it). */
 int32_t eppx_pair_fallback_blocks(eppx_ps *ps, int64_t *out, int32_t reset);
+/* Number of particle walks (search and momentum passes) whose neighbour rows
+ * held more than 64 slots each, since the last reset: those walks test the
+ * candidates one at a time from HBM instead of through 64-bit accept masks
+ * (cells much larger than 2h, or crowded cells; a performance diagnostic,
+ * results do not depend on it). */
+int32_t eppx_pair_row_fallback(eppx_ps *ps, int64_t *out, int32_t reset);
 /* Returns EPPX_E_NONFINITE when a live particle's density, pressure,
  * position or velocity is not finite (waits). */
 int32_t eppx_check_finite(eppx_ps *ps);

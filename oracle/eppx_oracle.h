@@ -112,6 +112,18 @@ void or_selfaction(const or_layout *L, int32_t n_particles, const int32_t *is_al
 void or_interaction(const or_layout *L, int32_t n_particles, const int32_t *is_alive, int32_t *IP, float *FP,
                     int32_t action, const or_physics *phys, int32_t column, float value, int32_t n_threads);
 
+/* Summation order of the per-neighbour sums (continuity, momentum): 0 (the
+ * default) the reference's list order, 1 reversed.  Global to the library. */
+void or_set_pair_order(int32_t reversed);
+
+/* Magnitude bound over I's neighbours of the continuity terms (what = 0,
+ * out[I]; after a search with the kernel + continuity action) or the
+ * momentum terms (what = 1, out[2 I + k]; before iMomentum!): each term with
+ * its inner sums of products taken in absolute value (the scale of the
+ * float32 rounding of the sum in any order). */
+void or_pair_term_magnitude(const or_layout *L, int32_t n_particles, const int32_t *is_alive, int32_t *IP,
+                            float *FP, const or_physics *phys, int32_t what, double *out);
+
 /* n_steps of the dam-break loop body (main.jl:166-183): sContinuity,
  * iMomentum, sMomentum, search (+ filter every filter_interval steps when
  * filter_interval > 0).  Returns the OR of search error bits; writes the

@@ -102,6 +102,9 @@ def _bind(L):
     L.or_run_steps.argtypes = [P(Layout), C.c_int32, i32p, i32p, i32p, f32p, P(NeighbourSystem),
                                P(Physics), C.c_int32, C.c_int32, C.c_int32, C.c_int32, P(C.c_int64)]
     L.or_run_steps.restype = C.c_int32
+    L.or_set_pair_order.argtypes = [C.c_int32]
+    L.or_pair_term_magnitude.argtypes = [P(Layout), C.c_int32, i32p, i32p, f32p, P(Physics), C.c_int32,
+                                         P(C.c_double)]
     return L
 
 
@@ -315,6 +318,18 @@ class System:
                                  _ptr(self.FP, C.c_float), C.byref(self.ns), C.byref(phys), n_steps,
                                  filter_interval, int(serial), threads, _ptr(pairs, C.c_int64))
         return err, pairs[:n_steps]
+
+    def term_magnitude(self, what, phys: Physics):
+        """Sum of |term| over each particle's neighbours: what 0 the continuity terms (n,), 1 the momentum
+        terms (n, 2)."""
+        out = np.zeros((self.n, 2) if what else self.n, np.float64)
+        self.lib.or_pair_term_magnitude(*self._args(), _ptr(self.IP, C.c_int32), _ptr(self.FP, C.c_float),
+                                        C.byref(phys), int(what), _ptr(out, C.c_double))
+        return out
+
+    def set_pair_order(self, reversed_):
+        """Summation order of this library's per-neighbour sums (0 list order, 1 reversed)."""
+        self.lib.or_set_pair_order(int(reversed_))
 
     # -------------------------------------------------------------- views
     def count(self):

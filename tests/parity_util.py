@@ -106,3 +106,151 @@ def reference_pairs_in_order(ref):
     lists = ref.IP[: ref.n, L.nIndex: L.nIndex + M] - 1
     mask = np.arange(M)[None, :] < cnt[:, None]
     return off, lists[mask].astype(np.int32)
+
+
+def per_particle_bar(got, exp, floor, rel=1e-5):
+    """FAST-mode bar per entry: |got - exp| <= rel |exp| + floor.  Returns the
+    largest |got - exp| / (rel |exp| + floor) (<= 1 passes) and how many
+    entries have |exp| below floor / rel, where the absolute floor dominates."""
+    got = np.asarray(got, np.float64)
+    exp = np.asarray(exp, np.float64)
+    return (float(np.max(np.abs(got - exp) / (rel * np.abs(exp) + floor), initial=0.0)),
+            int(np.sum(np.abs(exp) < floor / rel)))
+
+
+def developed(parallel, case, n_steps, math="exact", filter_interval=0):
+    """The case after n_steps of its loop on the device: a flow well past its
+    start (pressure built up, the column collapsing), used as the common input
+    of the FAST-against-restatement comparisons.  Returns a copy of the case
+    whose host tables hold that state, with dDensity zeroed: the search that
+    both sides run first recomputes the pending dDensity from the same
+    positions and velocities."""
+    import dataclasses
+    parallel.set_math(math)
+    try:
+        ps, ns = device(parallel, case)
+        Algorithm.search(ps, case.domain, ns, criterion=Algorithm.symmetrySearchCriterion,
+                         action=SPH.sComputeKernelAndiContinuity(case.physics))
+        Algorithm.step(ps, ns, case.physics, n_steps=n_steps, filter_interval=filter_interval)
+        host = download(ps)
+        ps.close()
+        ns.close()
+    finally:
+        parallel.set_math("exact")
+    host.float_properties_[:, host.named_index_.float_table.index["dDensity"] - 1] = 0.0
+    return dataclasses.replace(case, host=host)
+
+
+def bar_ratio(got, exp, bound):
+    """max |got - exp| / bound over the entries (<= 1 passes)."""
+    got = np.asarray(got, np.float64)
+    exp = np.asarray(exp, np.float64)
+    d = np.abs(got - exp)
+    bound = np.asarray(bound, np.float64)
+    ok = bound > 0
+    if (d[~ok] > 0).any():
+        return float("inf")
+    return float(np.max(d[ok] / bound[ok], initial=0.0))
+
+
+def fast_momentum_pass(parallel, case, shapes=None):
+    """FAST math against the restatement from the same tables, per particle.
+
+    (a) A FAST search (kernel + continuity) and sContinuity!: dDensity and
+        density.
+    (b) An EXACT search and sContinuity! (bit-identical to the restatement's),
+        then iMomentum! in FAST math: the acceleration from identical inputs.
+    A float32 sum evaluated in another order or with other roundings of its
+    terms (the device's row order, FMA, hardware reciprocals) differs from the
+    restatement's by a small multiple of eps times the magnitude bound of the
+    sum (each term with its inner sums of products in absolute value; oracle
+    or_pair_term_magnitude).  The bars are 1e-5 of that bound for dDensity
+    and acceleration, 1e-5 relative for density.  For scale, the restatement's
+    momentum pass is also summed in reversed neighbour order (exact
+    arithmetic, only the order differs -- the reference's GPU lists have no
+    fixed order) and measured against the same bar.  Returns the
+    measurements."""
+    ph = case.physics
+    oph = oracle_physics(ph)
+    try:
+        parallel.set_math("fast")
+        ps, ns = device(parallel, case)
+        if shapes is not None:
+            Algorithm.set_pair_shape(ps, search=shapes["search"], momentum=shapes["momentum"])
+        Algorithm.search(ps, case.domain, ns, criterion=Algorithm.symmetrySearchCriterion,
+                         action=SPH.sComputeKernelAndiContinuity(ph))
+        after_search = download(ps)
+        Algorithm.selfaction(ps, SPH.sContinuity(ph))
+        after_continuity = download(ps)
+        ps.close()
+        ns.close()
+        parallel.set_math("exact")
+        ps, ns = device(parallel, case)
+        if shapes is not None:
+            Algorithm.set_pair_shape(ps, search=shapes["search"], momentum=shapes["momentum"])
+        Algorithm.search(ps, case.domain, ns, criterion=Algorithm.symmetrySearchCriterion,
+                         action=SPH.sComputeKernelAndiContinuity(ph))
+        Algorithm.selfaction(ps, SPH.sContinuity(ph))
+        parallel.set_math("fast")
+        Algorithm.interaction(ps, SPH.iMomentum(ph))
+        after_momentum = download(ps)
+        ps.close()
+        ns.close()
+    finally:
+        parallel.set_math("exact")
+    ref = reference(case)
+    assert ref.search(O.CRIT_SYMMETRY, 1, oph) == 0
+    L = ref.L
+    n = ref.n
+    alive = ref.is_alive[:n] == 1
+    fluid = alive & (ref.IP[:n, L.Tag] == ph.fluid_tag)
+    drho = bar_ratio(after_search.float_properties_[:n][alive, L.dDensity], ref.FP[:n][alive, L.dDensity],
+                     1e-5 * ref.term_magnitude(0, oph)[alive])
+    ref.selfaction(1, oph)
+    rho_g = after_continuity.float_properties_[:n][alive, L.Density].astype(np.float64)
+    rho_r = ref.FP[:n][alive, L.Density].astype(np.float64)
+    a_m = ref.term_magnitude(1, oph)[fluid]
+    du_c = slice(L.dVelocityVec, L.dVelocityVec + 2)
+    before = ref.FP[:, du_c].copy()
+    ref.set_pair_order(1)
+    ref.interaction(1, oph)
+    du_rev = ref.FP[:n][fluid, du_c].copy()
+    ref.FP[:, du_c] = before
+    ref.set_pair_order(0)
+    ref.interaction(1, oph)
+    du_r = ref.FP[:n][fluid, du_c]
+    du_g = after_momentum.float_properties_[:n][fluid, du_c]
+    old_bar, floored = per_particle_bar(du_g, du_r, 1e-5 * abs(float(np.float32(ph.gy))))
+    return {"rho": float(np.max(np.abs(rho_g - rho_r) / np.abs(rho_r))), "drho": drho,
+            "du": bar_ratio(du_g, du_r, 1e-5 * a_m), "du_reordered": bar_ratio(du_rev, du_r, 1e-5 * a_m),
+            "du_rel_floor": old_bar, "under_floor": floored, "entries": int(du_r.size),
+            "du_max": float(np.max(np.abs(du_r))), "cancellation": float(np.max(a_m / (np.abs(du_r) + 1e-30)))}
+
+
+def describe_pass(m):
+    return (f"density {m['rho']:.3g} relative; dDensity {m['drho']:.3g} of the bar; acceleration "
+            f"{m['du']:.3g} of the bar (reversed-order restatement: {m['du_reordered']:.3g}); against "
+            f"1e-5 |du| + 1e-5 g instead: {m['du_rel_floor']:.3g} ({m['under_floor']} of {m['entries']} under "
+            f"the floor); |du| <= {m['du_max']:.3g}, bound / |du| up to {m['cancellation']:.3g}")
+
+
+def reference_momentum_magnitude(case):
+    """sum |momentum terms| per particle and component (n, 2) of the first
+    momentum pass from the case's tables (search, then sContinuity!)."""
+    oph = oracle_physics(case.physics)
+    ref = reference(case)
+    assert ref.search(O.CRIT_SYMMETRY, 1, oph) == 0
+    ref.selfaction(1, oph)
+    return ref.term_magnitude(1, oph)
+
+
+def flow_summary(case):
+    """Largest speed, largest pressure and surge front (largest x of the fluid) of a case's tables."""
+    h = case.host
+    n = h.n_particles_
+    fi = h.named_index_.float_table.index
+    tag = h.int_properties_[:n, h.named_index_.int_table.index["Tag"] - 1]
+    fluid = (h.is_alive_[:n] == 1) & (tag == 1)
+    u = h.float_properties_[:n, fi["VelocityVec"] - 1: fi["VelocityVec"] + 1][fluid].astype(np.float64)
+    return (float(np.max(np.hypot(u[:, 0], u[:, 1]))), float(np.max(h.float_properties_[:n, fi["Pressure"] - 1])),
+            float(np.max(h.float_properties_[:n, fi["PositionVec"] - 1][fluid])))

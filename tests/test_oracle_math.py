@@ -469,3 +469,180 @@ def test_extrapolated_wall_pressure_clamps_and_propagates_nan(oracle):
     near = ~fluid & ((x - x[k]) ** 2 + (y - y[k]) ** 2 < (0.04) ** 2)
     far = ~fluid & (np.abs(x - x[k]) > 0.05)
     assert np.isnan(p[near]).any() and np.isfinite(p[far]).all()
+
+
+# ------------------------------------- the reference example's own physics
+# main.jl:57-70: linear EOS p = c0^2 (rho - rho0) + p0 with c0 = 120, and
+# iClassicViscosity! with mu0 (main.jl:110,114 -> Viscosity.jl:10-26).
+
+def linear_eos_values(O, variant=None):
+    s = pair_system(O, variant)
+    s.FP[:2, s.L.Density] = 1000.0
+    s.FP[:2, s.L.dDensity] = [0.0, 1.0e4]                # rho -> 1000 and 1001 with dt = 1e-4
+    ph = phys(O, eos=0, c0=120.0, p0=250.0, dt=1e-4)
+    s.selfaction(1, ph)
+    return s.FP[:2, s.L.Density].copy(), s.FP[:2, s.L.Pressure].copy()
+
+
+def test_linear_eos_closed_form(oracle):
+    """rho0 maps to p0 exactly; one unit of density above rho0 adds c0^2."""
+    rho, p = linear_eos_values(oracle)
+    assert rho.tolist() == [1000.0, 1001.0]
+    assert p[0] == 250.0
+    assert p[1] == 250.0 + 120.0 * 120.0
+
+
+def test_linear_eos_check_catches_unsquared_sound_speed(oracle):
+    _, p = linear_eos_values(oracle, oracle.mutant(12))
+    assert p[1] != 250.0 + 120.0 * 120.0
+
+
+def classic_viscosity_residual(O, variant=None):
+    """p = 0: classic viscosity alone; the pair's exchange m_i du_i + m_j du_j cancels."""
+    s = pair_system(O, variant, p=(0.0, 0.0))
+    ph = phys(O, visc=0, mu=1e-3)
+    s.search(O.CRIT_SYMMETRY, 1, ph)
+    s.interaction(1, ph)
+    du = s.FP[:2, s.L.dVelocityVec: s.L.dVelocityVec + 2].astype(np.float64)
+    m = s.FP[:2, s.L.Mass].astype(np.float64)
+    total = m[0] * du[0] + m[1] * du[1]
+    return np.max(np.abs(total)) / np.max(np.abs(m[:, None] * du)), du
+
+
+def test_classic_viscosity_antisymmetric(oracle):
+    res, du = classic_viscosity_residual(oracle)
+    assert np.max(np.abs(du)) > 0.0 and res < 1e-5
+    u_rel = np.array([0.3, 0.1]) - np.array([-0.2, 0.05])
+    assert float(np.dot(du[0], u_rel)) < 0.0                     # it slows the relative motion
+
+
+def test_classic_viscosity_antisymmetry_catches_transposed_mass(oracle):
+    assert classic_viscosity_residual(oracle, oracle.mutant(14))[0] > 1e-2
+
+
+def shear_decay_ratio(O, variant=None):
+    """u_x = U sin(k y) on a lattice, p = 0, rho = rho0: the SPH classic-viscosity
+    acceleration is nu times the Laplacian (Morris' form), so interior particles
+    decay at du_x / u_x = -nu k^2 with nu = mu / rho0.  Returns the median of
+    (du_x / u_x) / (-nu k^2) over interior particles away from the nodes."""
+    nx, ny = 40, 80
+    dx, lam, U, mu = 0.01, 40 * 0.01, 0.5, 1e-3
+    k = 2 * math.pi / lam
+    s, x, y = lattice(O, nx, ny, dx, h_factor=1.5, variant=variant,
+                      vel=lambda x, y: ((U * np.sin(k * y)).astype(np.float32), 0 * y))
+    ph = phys(O, kernel=0, visc=0, mu=mu, eos=0, c0=120.0)
+    s.search(O.CRIT_SYMMETRY, 1, ph)
+    s.interaction(1, ph)
+    du = s.FP[: s.n, s.L.dVelocityVec].astype(np.float64)
+    dv = s.FP[: s.n, s.L.dVelocityVec + 1].astype(np.float64)
+    u = s.FP[: s.n, s.L.VelocityVec].astype(np.float64)
+    sel = interior(x, y, nx, ny, dx) & (np.abs(u) > 0.5 * U)
+    assert sel.sum() > 100
+    assert np.max(np.abs(dv[sel])) < 1e-3 * np.max(np.abs(du[sel]))   # no cross-stream acceleration
+    return float(np.median((du[sel] / u[sel]) / (-(mu / 1000.0) * k * k)))
+
+
+def test_classic_viscosity_shear_layer_decay_rate(oracle):
+    assert abs(shear_decay_ratio(oracle) - 1.0) < 0.05
+
+
+def test_shear_decay_catches_dropped_factor_two(oracle):
+    assert abs(shear_decay_ratio(oracle, oracle.mutant(13)) - 1.0) > 0.3
+
+
+# ------------------------------------- sum-of-|terms| and summation order
+# (the FAST-mode bars of the GPU tests are 1e-5 of these magnitudes)
+
+def wendland_sensitivity(q):
+    """(2 |G| + |q G'|) / |G| for the Wendland C2 gradient shape G = q (2 - q)^3."""
+    g = q * (2 - q) ** 3
+    gp = (2 - q) ** 2 * (2 - 4 * q)
+    return (2 * abs(g) + abs(q * gp)) / abs(g)
+
+
+def test_term_magnitude_of_a_single_pair_is_its_term(oracle):
+    """One neighbour each, motion along x (no inner cancellation), p = 0: du
+    is one artificial-viscosity term and dDensity one continuity term, both
+    proportional to f = dw / r, so their bounds are |term| (|f| + r |f'|) / |f|
+    = |term| (2 |G| + |q G'|) / |G| (closed form of the Wendland shape; with
+    a flat EOS, so that the pressure share of one density ulp is zero); with
+    pressure the bound is at least |du|."""
+    O = oracle
+    ph = phys(O)
+    s = pair_system(O, p=(0.0, 0.0), u=((0.3, 0.0), (-0.3, 0.0)))
+    s.search(O.CRIT_SYMMETRY, 1, ph)
+    q = float(s.FP[0, s.L.nR]) * float(s.FP[0, s.L.nHInv])
+    k = wendland_sensitivity(q)
+    c = s.term_magnitude(0, ph)
+    drho = np.abs(s.FP[:2, s.L.dDensity].astype(np.float64))
+    assert drho.min() > 0 and np.allclose(c / drho, k, rtol=1e-5)
+    flat = phys(O, tait_b=0.0)  # dp / drho = 0: no pressure-ulp share in the bound
+    a = s.term_magnitude(1, flat)
+    s.interaction(1, flat)
+    du = np.abs(s.FP[:2, s.L.dVelocityVec: s.L.dVelocityVec + 2].astype(np.float64))
+    assert du[:, 0].min() > 0 and np.allclose(a[:, 0] / du[:, 0], k, rtol=1e-5)
+    s = pair_system(O)
+    s.search(O.CRIT_SYMMETRY, 1, ph)
+    a = s.term_magnitude(1, ph)
+    s.interaction(1, ph)
+    du = np.abs(s.FP[:2, s.L.dVelocityVec: s.L.dVelocityVec + 2].astype(np.float64))
+    assert (a >= du).all() and a.max() > 0
+
+
+def test_term_magnitude_grows_at_the_support_edge(oracle):
+    """Near q = 2 the Wendland gradient's relative sensitivity to r grows like
+    3 / (2 - q): a pair at q = 1.999 gets a bound ~1000x its term."""
+    O = oracle
+    ph = phys(O)
+    s = pair_system(O, p=(0.0, 0.0), u=((0.3, 0.0), (-0.3, 0.0)))
+    hinv = float(O.lib().or_invharmonic(0.010, 0.011))
+    r = 1.999 / hinv
+    s.FP[1, s.L.PositionVec] = np.float32(0.100 + r)
+    s.FP[1, s.L.PositionVec + 1] = np.float32(0.100)
+    s.search(O.CRIT_SYMMETRY, 1, ph)
+    assert s.count().tolist() == [1, 1]
+    q = float(s.FP[0, s.L.nR]) * float(s.FP[0, s.L.nHInv])
+    c = s.term_magnitude(0, ph)
+    drho = np.abs(s.FP[:2, s.L.dDensity].astype(np.float64))
+    assert wendland_sensitivity(q) > 500 and np.allclose(c / drho, wendland_sensitivity(q), rtol=1e-3)
+
+
+def test_reversed_order_changes_only_rounding(oracle):
+    """A lattice with a velocity field and a hydrostatic pressure: momentum
+    summed in reversed neighbour order differs from the list order by
+    rounding only (within 1e-6 of sum |terms|), and not at all for the
+    two-particle system."""
+    O = oracle
+    nx = ny = 24
+    dx = 0.01
+    s, x, y = lattice(O, nx, ny, dx, vel=lambda x, y: ((0.3 * np.sin(9 * y)).astype(np.float32),
+                                                       (0.2 * np.cos(7 * x)).astype(np.float32)),
+                      p=lambda x, y: (1e4 * (1.0 - y)).astype(np.float32))
+    ph = phys(O)
+    s.search(O.CRIT_SYMMETRY, 1, ph)
+    a = s.term_magnitude(1, ph)
+    c = slice(s.L.dVelocityVec, s.L.dVelocityVec + 2)
+    before = s.FP[:, c].copy()
+    s.interaction(1, ph)
+    fwd = s.FP[: s.n, c].astype(np.float64)
+    s.FP[:, c] = before
+    s.set_pair_order(1)
+    try:
+        s.interaction(1, ph)
+    finally:
+        s.set_pair_order(0)
+    rev = s.FP[: s.n, c].astype(np.float64)
+    assert (fwd != rev).any()                                      # the order matters to the bits
+    assert np.max(np.abs(fwd - rev) / a) < 1e-6
+    p2 = pair_system(O)
+    p2.search(O.CRIT_SYMMETRY, 1, ph)
+    p2.set_pair_order(1)
+    try:
+        p2.interaction(1, ph)
+    finally:
+        p2.set_pair_order(0)
+    rev2 = p2.FP[:2, c].copy()
+    p3 = pair_system(O)
+    p3.search(O.CRIT_SYMMETRY, 1, ph)
+    p3.interaction(1, ph)
+    assert np.array_equal(rev2, p3.FP[:2, c])
