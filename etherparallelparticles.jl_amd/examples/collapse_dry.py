@@ -7,8 +7,10 @@ dam-break loop with the Shepard filter every 20 steps and an output every
     python etherparallelparticles.jl_amd/examples/collapse_dry.py main OUT [--steps N]
 
 The physics constants live in the driver, as in main.jl (they are not part of
-the configuration): Tait EOS, artificial viscosity, dt = 0.1 h / c0, the
-kernel and h of the case.
+the configuration).  --physics baseline (the default): Tait EOS, artificial
+viscosity, dt = 0.1 h / c0, g = 9.81.  --physics example: main.jl's own
+constants (main.jl:57-70): linear EOS with c0 = 120, classic viscosity with
+mu0 = 1e-3, g = 9.8, dt = 0.2 h0 / c0.  Both use the kernel and h of the case.
 """
 from __future__ import annotations
 
@@ -24,7 +26,15 @@ from eppx import Algorithm, Class, DataIO, SPH, cases  # noqa: E402
 KERNELS = {"cubic": SPH.CubicSpline, "wendland": SPH.WendlandC2}
 
 
-def casegen(out: str, dx: float = 0.02, h_factor: float = 1.5, kernel: str = "cubic", M: int = 40) -> str:
+def driver_physics(drv: dict) -> SPH.Physics:
+    """The step's constants: main.jl's own (physics "example") or the benchmark's (the default)."""
+    if drv.get("physics", "baseline") == "example":
+        return SPH.example_physics(drv["dx"], drv["h_factor"], KERNELS[drv["kernel"]])
+    return SPH.dam_break_physics(drv["dx"], drv["h_factor"], KERNELS[drv["kernel"]])
+
+
+def casegen(out: str, dx: float = 0.02, h_factor: float = 1.5, kernel: str = "cubic", M: int = 40,
+            physics: str = "baseline") -> str:
     """casegen.jl: the dam break as a configuration (config/config.json) and
     the raw file of write step 0.  Returns the configuration's path."""
     case = cases.dam_break(dx, h_factor, KERNELS[kernel], M=M)
@@ -33,7 +43,8 @@ def casegen(out: str, dx: float = 0.02, h_factor: float = 1.5, kernel: str = "cu
     ns = type("NeighbourSystemRecord", (), {"active_pair_": Class.ActivePair(cases.ACTIVE_PAIR),
                                              "max_neighbour_number": M})()
     config = DataIO.config(par, case.domain, case.host, ns, writer)
-    config["driver"] = {"dx": dx, "h_factor": h_factor, "kernel": kernel}  # the physics main.jl hard-codes
+    # the physics main.jl hard-codes
+    config["driver"] = {"dx": dx, "h_factor": h_factor, "kernel": kernel, "physics": physics}
     DataIO.save(writer, case.host, DataIO.appendix())
     return DataIO.writeConfig(writer, config)
 
@@ -54,7 +65,7 @@ def main(out: str, total_steps: int, filter_interval: int = 20, output_interval:
     appendix = DataIO.load_into(writer, host, step)
     Class.syncto(ps, host)
     ns = DataIO.NeighbourSystem(config, par, domain)
-    phys = SPH.dam_break_physics(drv["dx"], drv["h_factor"], KERNELS[drv["kernel"]])
+    phys = driver_physics(drv)
     t, tm, write_step = float(appendix["TimeValue"]), int(appendix["TMSTEP"]), int(appendix["WriteStep"])
     Algorithm.search(ps, domain, ns, criterion=Algorithm.symmetrySearchCriterion,
                      action=SPH.sComputeKernelAndiContinuity(phys))
@@ -80,8 +91,10 @@ if __name__ == "__main__":
     ap.add_argument("--dx", type=float, default=0.02)
     ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--math", choices=["exact", "fast"], default="exact")
+    ap.add_argument("--physics", choices=["baseline", "example"], default="baseline",
+                    help="casegen: the step's constants (the benchmark's, or main.jl's own)")
     a = ap.parse_args()
     if a.what == "casegen":
-        print(casegen(a.out, dx=a.dx))
+        print(casegen(a.out, dx=a.dx, physics=a.physics))
     else:
         main(a.out, a.steps, math=a.math)

@@ -172,3 +172,18 @@ def test_casegen_writes_a_loadable_case(tmp_path):
     assert app["WriteStep"] == 0 and back.n_particles_ == case.n_particles
     assert np.array_equal(back.float_properties_, case.host.float_properties_)
     assert np.array_equal(back.int_properties_, case.host.int_properties_)
+
+
+def test_casegen_records_the_driver_physics(tmp_path):
+    """--physics example: main.jl's constants (linear EOS, c0 = 120, classic
+    viscosity mu0 = 1e-3, g = 9.8, dt = 0.2 h0 / c0) reach the driver."""
+    from eppx import SPH, _lib
+    ex = _example()
+    c = DataIO.parse(ex.casegen(str(tmp_path / "case"), dx=0.02, physics="example"))
+    ph = ex.driver_physics(c["driver"])
+    assert ph.eos == _lib.EOS_LINEAR and ph.visc == _lib.VISC_CLASSIC
+    assert np.float32(ph.c0) == np.float32(120.0) and np.float32(ph.gy) == np.float32(-9.8)
+    assert np.float32(ph.mu) == np.float32(1e-3)
+    assert np.float32(ph.dt) == np.float32(np.float32(0.1 * np.float64(np.float32(0.03)) / 120.0) * 2)
+    base = ex.driver_physics(DataIO.parse(ex.casegen(str(tmp_path / "b"), dx=0.02))["driver"])
+    assert base.eos == _lib.EOS_TAIT and base.visc == _lib.VISC_ARTIFICIAL

@@ -47,9 +47,12 @@ def test_config_driven_case_matches_direct(parallel, small, tmp_path):
     U.assert_state_equal(b, ref)
 
 
-def test_collapse_dry_driver_matches_direct_run(parallel, tmp_path):
+@pytest.mark.parametrize("physics", ["baseline", "example"])
+def test_collapse_dry_driver_matches_direct_run(parallel, tmp_path, physics):
     """examples/collapse_dry.py (casegen, then main with the filter every 20
-    steps and an output every 40): the written state equals a direct run."""
+    steps and an output every 40), with the benchmark's constants or
+    main.jl's own (linear EOS, classic viscosity): the written state equals a
+    direct run."""
     import importlib.util
     import os
     from eppx import Algorithm, DataIO, SPH, cases
@@ -59,14 +62,15 @@ def test_collapse_dry_driver_matches_direct_run(parallel, tmp_path):
     ex = importlib.util.module_from_spec(spec)
     spec.loader.exec_module(ex)
     out = str(tmp_path / "run")
-    ex.casegen(out, dx=0.02)
+    ex.casegen(out, dx=0.02, physics=physics)
     app = ex.main(out, 60, filter_interval=20, output_interval=40, log=lambda *_: None)
     assert app["TMSTEP"] == 40 and app["WriteStep"] == 1  # the output of step 40
     case = cases.dam_break(0.02, 1.5, SPH.CubicSpline, M=40)
+    ph = SPH.example_physics(0.02) if physics == "example" else case.physics
     ps, ns = U.device(parallel, case)
     Algorithm.search(ps, case.domain, ns, criterion=Algorithm.symmetrySearchCriterion,
-                     action=SPH.sComputeKernelAndiContinuity(case.physics))
-    Algorithm.step(ps, ns, case.physics, n_steps=40, filter_interval=20)
+                     action=SPH.sComputeKernelAndiContinuity(ph))
+    Algorithm.step(ps, ns, ph, n_steps=40, filter_interval=20)
     direct = U.download(ps)
     w = DataIO.WriterFrom(DataIO.parse(os.path.join(out, "config", "config.json")))
     saved = Class_host_like(direct)
