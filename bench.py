@@ -1,6 +1,7 @@
 """Benchmark: pair-interactions/s of the 2D dam break on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload large|small] [--math exact|fast]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload large|small|mix|serve|serve-sweep]
+                    [--math exact|fast]
 
 Workload "large" (default): 1.0 x 2.0 m water column in a 4 x 3 m tank,
 dx = 0.001 (2,000,000 fluid + 30,018 wall particles), Wendland C2, h = 2dx,
@@ -13,7 +14,10 @@ directed pairs of each step's search (the reference's sum of nCount) / wall
 time of those steps.  roofline: the kernel with the largest share of device
 time, its algorithmic bytes per launch / its average HIP-event duration.
 cpu_baseline: the C restatement of the reference algorithm (reference layout,
-OpenMP over the host's cores) on a bounded sample of the same workload.
+OpenMP over the host's cores) on a bounded sample of the same workload, and
+the small dam break's full 1000 steps.  The default line also carries the
+64-case uneven mix ("mix": batched against one by one) and the serving stream
+("serve": p50 / p99 time to completion beside the aggregate rate).
 """
 import argparse
 import ctypes as C
@@ -23,6 +27,10 @@ import sys
 import time
 
 import numpy as np
+
+# the CPU baseline's OpenMP threads stay on their cores (read when the runtime starts)
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
 
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "etherparallelparticles.jl_amd"))
@@ -91,6 +99,49 @@ def cpu_baseline(case, seconds, threads, max_steps=200):
     return total / dt, dt, steps
 
 
+def cpu_info(threads: int) -> dict:
+    """Model, logical CPUs and physical cores of the CPUs this process may run on."""
+    cpus = sorted(os.sched_getaffinity(0))
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    cores = set()
+    for c in cpus:
+        try:
+            base = f"/sys/devices/system/cpu/cpu{c}/topology/"
+            with open(base + "physical_package_id") as f1, open(base + "core_id") as f2:
+                cores.add((f1.read().strip(), f2.read().strip()))
+        except OSError:
+            cores.add(("?", str(c)))
+    return {"model": model, "logical_cpus": len(cpus), "physical_cores": len(cores), "threads_used": threads,
+            "omp_proc_bind": os.environ.get("OMP_PROC_BIND"), "omp_places": os.environ.get("OMP_PLACES")}
+
+
+def cpu_small_full(threads: int, steps: int = 1000) -> dict:
+    """The small dam break (6,518 particles, cubic spline) run in full on the CPU restatement."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle as O
+    from eppx import cases
+    c = cases.small(M=40)
+    h = c.host
+    ref = O.System(h.int_properties_, h.float_properties_, h.is_alive_, h.cell_index_, h.n_particles_,
+                   O.make_layout(c.int_nt, c.float_nt, h.n_capacity), O.domain2d(*c.domain.args),
+                   np.array([[1, 1], [1, 0]]), 40)
+    ph = O.physics(**c.physics.as_dict())
+    ref.search(O.CRIT_SYMMETRY, 1, ph, serial=False, threads=threads)
+    t0 = time.perf_counter()
+    err, pairs = ref.run_steps(steps, ph, serial=False, threads=threads)
+    dt = time.perf_counter() - t0
+    return {"steps": steps, "seconds": dt, "value": float(pairs.sum()) / dt, "unit": "pair-interactions/s",
+            "steps_per_s": steps / dt, "threads": threads}
+
+
 def copy_calibration(n_bytes: int, repeats: int = 3) -> None:
     """Device-to-device copies of n_bytes (HIP's own copy kernel): the known-size
     reference that scales rocprofv3's FETCH_SIZE / WRITE_SIZE readings."""
@@ -127,29 +178,17 @@ def _warm_up(par):
     ns.close()
 
 
-def _percentile(xs, q):
-    xs = sorted(xs)
-    if not xs:
-        return None
-    k = (len(xs) - 1) * q
-    lo = int(k)
-    hi = min(lo + 1, len(xs) - 1)
-    return xs[lo] + (xs[hi] - xs[lo]) * (k - lo)
-
-
-def run_mix(args):
-    """64 uneven dam-break cases (seed 0), 500 steps each: one batch (every case in one segmented cell list,
-    one launch per kernel per step, recorded graph) against the same cases one by one on the particle-system
-    path.  Both timings cover upload, first search, the steps and download."""
+def run_mix(args, par):
+    """64 uneven dam-break cases (seed 0), mix_steps steps each: one batch (every case in one segmented cell
+    list, one launch per kernel per step, recorded graph) against the same cases one by one on the
+    particle-system path.  Both timings cover upload, first search, the steps and download.  Returns the
+    measurement."""
     import eppx
-    from eppx import Algorithm, Batch, Class, SPH, cases
+    from eppx import Algorithm, Batch, Class, SPH, Serve, cases
     cs = cases.mix_cases(0, steps=args.mix_steps, M=1)
     total_n = sum(c.n_particles for c in cs)
     log(f"mix: {len(cs)} cases, {total_n} particles ({min(c.n_particles for c in cs)}..."
         f"{max(c.n_particles for c in cs)}), {args.mix_steps} steps each")
-    par = eppx.Environment.Parallel(device=0, math=args.math)
-    _warm_up(par)
-
     par.synchronize()
     t0 = time.perf_counter()
     b = Batch.Batch(par, cs[0].host, total_n, len(cs), cases.ACTIVE_PAIR, max_neighbour_number=64)
@@ -198,14 +237,14 @@ def run_mix(args):
     if pairs_single != pairs_batch:
         raise RuntimeError(f"pair counts differ: batch {pairs_batch}, one by one {pairs_single}")
     t_batch = (t3 - t0) - (tb - ta)  # the autotuner's trials are reported apart
-    out = {
+    return {
         "metric": "aggregate pair-interactions/s, 64-case uneven 2D dam-break mix, batched",
         "value": pairs_batch / t_batch, "unit": "pair-interactions/s", "n_gpus": 1, "steps": args.mix_steps,
         "warmup": 0, "ms_per_step": 1000.0 * (t2 - t1) / args.mix_steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"64 dam-break cases from seed 0 (dx in {{0.02, 0.01, 0.005, 0.0025}}, width "
                                f"U[0.5, 1.5], aspect U[0.5, 2], Wendland C2, h = 2dx), {args.mix_steps} steps each",
-                   "particles": total_n, "cases": len(cs), "math": args.math,
+                   "particles": total_n, "cases": len(cs), "math": args.math, "seed": 0,
                    "case_particles_min": min(c.n_particles for c in cs),
                    "case_particles_max": max(c.n_particles for c in cs)},
         "one_by_one": {"value": pairs_single / t_single, "seconds": t_single, "steps_seconds": steps_single},
@@ -215,86 +254,46 @@ def run_mix(args):
         "speedup_end_to_end": t_single / t_batch,
         "speedup_steps": steps_single / (t2 - t1),
         "latency_s": {"batched_p50": t_batch, "batched_p99": t_batch,
-                      "one_by_one_p50": _percentile(done, 0.5), "one_by_one_p99": _percentile(done, 0.99)},
-        "pairs": pairs_batch,
+                      "one_by_one_p50": Serve.percentile(done, 0.5), "one_by_one_p99": Serve.percentile(done, 0.99)},
+        "pairs": pairs_batch, "pairs_one_by_one": pairs_single, "pairs_equal": pairs_single == pairs_batch,
     }
-    print(json.dumps(out), flush=True)
-    par.close()
 
 
-def run_serve(args):
-    """Serving stream (seed 1): Poisson arrivals of dam-break requests (N log-uniform in [1e4, 1e6], dx from a
-    1 x 2 m column, steps U{200..2000}).  Continuous batching: requests are admitted as slots and case slots free
-    up (admission by device bytes per particle against a fixed HBM budget), run together in one batch, and leave
-    when their budget is done.  Reports p50 / p99 time to completion and aggregate pair-interactions/s."""
-    import eppx
-    from eppx import Batch, SPH, cases
-    reqs = cases.serving_stream(1, n_requests=args.serve_requests, rate_hz=args.serve_rate)
-    hosts = [cases.dam_break(dx, 2.0, SPH.WendlandC2, M=1, fluid_width=1.0, fluid_height=2.0) for _, _, dx, _ in reqs]
-    bpp = Batch.bytes_per_particle()
-    slots = int(args.serve_budget_gb * 1e9) // bpp
-    biggest = max(h.n_particles for h in hosts)
-    if slots < biggest:
-        raise SystemExit(f"HBM budget too small for the largest request ({biggest} particles)")
-    log(f"serve: {len(reqs)} requests over {reqs[-1][0]:.1f} s, {sum(h.n_particles for h in hosts)} particles, "
-        f"budget {args.serve_budget_gb} GB = {slots} slots of {bpp} B")
-    par = eppx.Environment.Parallel(device=0, math=args.math)
-    _warm_up(par)
-    b = Batch.Batch(par, hosts[0].host, slots, args.serve_slots, cases.ACTIVE_PAIR, max_neighbour_number=64)
-    pending = list(range(len(reqs)))
-    active = {}  # case id -> request
-    used = 0
-    latency, pairs, chunks = [None] * len(reqs), 0, 0
-    par.synchronize()
-    t0 = time.perf_counter()
-    while pending or active:
-        now = time.perf_counter() - t0
-        while pending and reqs[pending[0]][0] <= now:  # admit in arrival order while it fits
-            r = pending[0]
-            n = hosts[r].n_particles
-            if used + n > slots or len(active) >= args.serve_slots:
-                break
-            cid = b.add(hosts[r].domain, hosts[r].physics, hosts[r].host, reqs[r][3])
-            active[cid] = r
-            used += n
-            pending.pop(0)
-        if not active:
-            time.sleep(max(0.0, reqs[pending[0]][0] - (time.perf_counter() - t0)))
-            continue
-        left = b.steps_left()
-        k = int(min(left[c] for c in active))
-        if k > 0:
-            b.pair_total(reset=True)
-            b.step(min(k, args.serve_chunk), sync=False)
-            pairs += b.pair_total(reset=True)
-            chunks += 1
-            left = b.steps_left()
-        for cid in [c for c in active if left[c] == 0]:
-            b.download(cid)
-            r = active.pop(cid)
-            latency[r] = time.perf_counter() - t0 - reqs[r][0]
-            used -= hosts[r].n_particles
-            b.remove(cid)
-    makespan = time.perf_counter() - t0
-    b.close()
-    lat = [x for x in latency if x is not None]
-    out = {
+def run_serve(args, par, rate):
+    """Serving stream (seed 1): Poisson arrivals at `rate` per second of dam-break requests (N log-uniform in
+    [1e4, 1e6], dx from a 1 x 2 m column, steps U{200..2000}), served by eppx.Serve (continuous batching:
+    admission by device bytes per particle against a fixed HBM budget, cases join between chunks and leave
+    when done).  Returns p50 / p99 time to completion and the aggregate pair-interactions/s."""
+    from eppx import Serve, cases
+    reqs = cases.serving_requests(1, n_requests=args.serve_requests, rate_hz=rate)
+    budget = int(args.serve_budget_gb * 1e9)
+    log(f"serve: {len(reqs)} requests at {rate}/s over {reqs[-1].arrival:.2f} s, "
+        f"{sum(r.n_particles for r in reqs)} particles, budget {args.serve_budget_gb} GB")
+    srv = Serve.Server(par, reqs[0].host, budget, args.serve_slots, cases.ACTIVE_PAIR, max_neighbour_number=64,
+                       chunk=args.serve_chunk)
+    try:
+        par.synchronize()
+        results, stats = srv.run(reqs)
+    finally:
+        srv.close()
+    s = Serve.summary(results, stats)
+    log(f"serve at {rate}/s: p50 {s['p50']:.3f} s, p99 {s['p99']:.3f} s, makespan {s['makespan_s']:.2f} s, "
+        f"{s['pair_interactions_per_s'] / 1e9:.1f} G pair-interactions/s")
+    return {
         "metric": "serving time-to-completion p50/p99, Poisson 2D dam-break stream (continuous batching)",
-        "value": pairs / makespan, "unit": "pair-interactions/s", "n_gpus": 1, "steps": None, "warmup": 0,
-        "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-        "data": "synthetic",
-        "config": {"workload": f"{len(reqs)} requests, seed 1, Poisson at {args.serve_rate}/s, N log-uniform in "
-                               f"[1e4, 1e6] (dx = sqrt(2/N)), steps U{{200..2000}}, Wendland C2, h = 2dx",
-                   "math": args.math, "particles": sum(h.n_particles for h in hosts),
-                   "steps_total": sum(r[3] for r in reqs)},
-        "latency_s": {"p50": _percentile(lat, 0.5), "p99": _percentile(lat, 0.99), "mean": sum(lat) / len(lat),
-                      "max": max(lat)},
-        "admission": {"bytes_per_particle": bpp, "budget_bytes": slots * bpp, "slots": slots,
-                      "case_slots": args.serve_slots},
-        "makespan_s": makespan, "chunks": chunks, "pairs": pairs,
+        "value": s["pair_interactions_per_s"], "unit": "pair-interactions/s", "n_gpus": 1, "steps": None,
+        "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{len(reqs)} requests, seed 1, Poisson at {rate}/s, N log-uniform in [1e4, 1e6] "
+                               f"(dx = sqrt(2/N)), steps U{{200..2000}}, Wendland C2, h = 2dx",
+                   "seed": 1, "rate_per_s": rate, "math": args.math,
+                   "particles": sum(r.n_particles for r in reqs), "steps_total": sum(r.steps for r in reqs)},
+        "latency_s": {"p50": s["p50"], "p99": s["p99"], "mean": s["mean"], "max": s["max"]},
+        "admission": {"bytes_per_particle": srv.bytes_per_slot, "budget_bytes": srv.slots * srv.bytes_per_slot,
+                      "slots": srv.slots, "case_slots": args.serve_slots, "peak_slots": s["peak_slots"],
+                      "peak_cases": s["peak_cases"]},
+        "makespan_s": s["makespan_s"], "chunks": s["chunks"], "pairs": s["pairs"],
     }
-    print(json.dumps(out), flush=True)
-    par.close()
 
 
 def main():
@@ -302,10 +301,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", choices=["large", "small", "mix", "serve"], default="large")
+    ap.add_argument("--workload", choices=["large", "small", "mix", "serve", "serve-sweep"], default="large")
     ap.add_argument("--mix-steps", type=int, default=500)
     ap.add_argument("--serve-requests", type=int, default=32)
-    ap.add_argument("--serve-rate", type=float, default=4.0, help="Poisson arrival rate (1/s)")
+    ap.add_argument("--serve-rate", type=float, default=32.0, help="Poisson arrival rate (1/s)")
+    ap.add_argument("--serve-sweep", default="4,8,16,32,64", help="arrival rates of --workload serve-sweep")
+    ap.add_argument("--no-mix", action="store_true", help="default run: leave out the 64-case mix")
+    ap.add_argument("--no-serve", action="store_true", help="default run: leave out the serving stream")
     ap.add_argument("--serve-budget-gb", type=float, default=2.0, help="HBM admitted to resident cases")
     ap.add_argument("--serve-slots", type=int, default=32, help="cases resident at once")
     ap.add_argument("--serve-chunk", type=int, default=100, help="largest number of steps between polls")
@@ -334,10 +336,24 @@ def main():
 
     if not os.path.exists(eppx.library_path()):
         eppx.build()
-    if args.workload == "mix":
-        return run_mix(args)
-    if args.workload == "serve":
-        return run_serve(args)
+    if args.workload in ("mix", "serve", "serve-sweep"):
+        par = eppx.Environment.Parallel(device=0, math=args.math)
+        _warm_up(par)
+        if args.workload == "mix":
+            out = run_mix(args, par)
+        elif args.workload == "serve":
+            out = run_serve(args, par, args.serve_rate)
+        else:
+            sweep = [run_serve(args, par, float(r)) for r in args.serve_sweep.split(",")]
+            out = dict(sweep[-1])
+            out["metric"] = "serving time-to-completion p50/p99 against arrival rate, Poisson 2D dam-break stream"
+            out["sweep"] = [{"rate_per_s": x["config"]["rate_per_s"], "p50_s": x["latency_s"]["p50"],
+                             "p99_s": x["latency_s"]["p99"], "pair_interactions_per_s": x["value"],
+                             "makespan_s": x["makespan_s"], "peak_cases": x["admission"]["peak_cases"]}
+                            for x in sweep]
+        print(json.dumps(out), flush=True)
+        par.close()
+        return
     t_setup = time.perf_counter()
     if args.workload == "large":
         case = cases.dam_break(0.001, 2.0, SPH.WendlandC2, M=1)   # slim host tables; device never stores lists
@@ -403,13 +419,20 @@ def main():
     # (x, u, rho, p, m, tag, h in; x, u out), plus the 12 B/cell of the cell table
     step_bytes = 108 * case.n_particles + 12 * case.domain.get_n()
 
-    traffic = valu = None
+    # HBM bytes per launch and VALU busy of the dominant kernel, from the rocprofv3 counter runs committed under
+    # profiles/ (counters cannot be read inside this run); the file names the commit it was taken at
+    traffic = valu = prof_commit = None
     tpath = os.path.join(REPO, "profiles", f"traffic_{dominant}_{args.workload}_{args.math}.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             prof = json.load(f)
         traffic = prof.get("hbm_bytes_per_launch")
-        valu = prof.get("valu_fraction")
+        valu = prof.get("valu_busy")
+        prof_commit = prof.get("commit")
+    frac = achieved / HBM_PEAK_GBS
+    limiter = None
+    if valu is not None:
+        limiter = "valu_issue" if valu >= 0.6 else ("hbm" if frac >= 0.6 else "latency")
 
     cpu = None
     if not args.no_cpu:
@@ -420,11 +443,14 @@ def main():
                "sample": f"{nsteps} steps of the same {case.n_particles}-particle dam break after one "
                          f"warm-up step, C/OpenMP restatement of the reference algorithm on the reference layout "
                          f"(per-cell atomic insert, materialised M=64 neighbour lists), {secs:.1f} s"}
+        cpu["host"] = cpu_info(threads)
         if args.cpu_seconds_1t > 0 and threads > 1:
             rate1, secs1, n1 = cpu_baseline(case, args.cpu_seconds_1t, 1)
             cpu["single_thread"] = {"value": rate1, "cores": 1,
                                     "sample": f"{n1} steps after one warm-up step on one thread, {secs1:.1f} s"}
             log(f"cpu baseline: {rate / 1e9:.3f} G/s on {threads} threads, {rate1 / 1e9:.3f} G/s on 1")
+        cpu["small_dam_break_1000_steps"] = cpu_small_full(threads)
+        log(f"cpu small dam break, 1000 steps: {cpu['small_dam_break_1000_steps']['seconds']:.2f} s")
 
     ms = 1000.0 * elapsed / args.steps
     out = {
@@ -449,13 +475,25 @@ def main():
                    "pair_shape": {"search": tune["search"], "momentum": tune["momentum"]} if tune else None,
                    "untiled_block_fraction": fb.value / (2 * args.steps) / blocks_per_launch},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "alg_bytes_per_launch": alg_bytes, "avg_launch_us": avg_us, "valu_fraction": valu},
+                     "frac": frac, "traffic": traffic,
+                     "alg_bytes_per_launch": alg_bytes, "avg_launch_us": avg_us, "valu_busy": valu,
+                     "limiter": limiter, "counters_commit": prof_commit},
         "step_roofline": {"alg_bytes_per_step": step_bytes, "device_us_per_step": device_us_per_step,
                           "hbm_frac": step_bytes / (device_us_per_step * 1e-6) / 1e9 / HBM_PEAK_GBS},
         "kernels_us": {k: round(v[0], 2) for k, v in kt.items()},
         "cpu_baseline": cpu,
     }
+    if args.workload == "large" and not args.no_mix:
+        m = run_mix(args, par)
+        out["mix"] = {"value": m["value"], "unit": m["unit"], "one_by_one_value": m["one_by_one"]["value"],
+                      "speedup_end_to_end": m["speedup_end_to_end"], "speedup_steps": m["speedup_steps"],
+                      "batched_seconds": m["batched"]["seconds"], "one_by_one_seconds": m["one_by_one"]["seconds"],
+                      "ms_per_batch_step": m["ms_per_step"], "pairs": m["pairs"], "pairs_equal": m["pairs_equal"],
+                      "config": m["config"]}
+    if args.workload == "large" and not args.no_serve:
+        v = run_serve(args, par, args.serve_rate)
+        out["serve"] = {"value": v["value"], "unit": v["unit"], "latency_s": v["latency_s"],
+                        "makespan_s": v["makespan_s"], "admission": v["admission"], "config": v["config"]}
     print(json.dumps(out), flush=True)
     par.close()
 

@@ -127,16 +127,28 @@ def mix_cases(seed: int = 0, n_cases: int = 64, steps: int = 500, M: int = 64):
 
 
 def serving_stream(seed: int = 1, n_requests: int = 32, rate_hz: float = 4.0, n_min: float = 1e4,
-                   n_max: float = 1e6):
+                   n_max: float = 1e6, steps_min: int = 200, steps_max: int = 2000):
     """(arrival_s, n_target, dx, steps) of the Poisson serving stream: exponential
     inter-arrivals at rate_hz, N log-uniform in [n_min, n_max], dx = sqrt(2 / N)
-    (1 x 2 m column), steps uniform in {200..2000}."""
+    (1 x 2 m column), steps uniform in {steps_min..steps_max}."""
     rng = np.random.default_rng(seed)
     t, out = 0.0, []
     for _ in range(n_requests):
         t += float(rng.exponential(1.0 / rate_hz))
         n = float(math.exp(rng.uniform(math.log(n_min), math.log(n_max))))
         dx = math.sqrt(2.0 / n)
-        steps = int(rng.integers(200, 2001))
+        steps = int(rng.integers(steps_min, steps_max + 1))
         out.append((t, n, dx, steps))
+    return out
+
+
+def serving_requests(seed: int = 1, n_requests: int = 32, rate_hz: float = 4.0, n_min: float = 1e4,
+                     n_max: float = 1e6, steps_min: int = 200, steps_max: int = 2000, M: int = 1):
+    """The serving stream as Serve.Request objects: each a 1 x 2 m column dam
+    break at its dx (Wendland C2, h = 2dx, the BASELINE physics)."""
+    from .Serve import Request
+    out = []
+    for t, _, dx, steps in serving_stream(seed, n_requests, rate_hz, n_min, n_max, steps_min, steps_max):
+        c = dam_break(dx, 2.0, SPH.WendlandC2, M=M, fluid_width=1.0, fluid_height=2.0, steps=steps)
+        out.append(Request(t, c.domain, c.physics, c.host, steps, name=f"dx={dx:.5f}"))
     return out
