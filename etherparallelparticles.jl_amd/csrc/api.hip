@@ -78,6 +78,7 @@ struct eppx_ps {
     float *h_range = nullptr;
     float rmin2 = -1.0f, rmax2 = INFINITY;  // symmetry-criterion bounds from the H range
     float hinv_u = 0.0f;                    // one H for every alive particle: its pair hinv; else 0
+    float m_u = 0.0f;                       // one Mass for every alive particle: that mass; else 0
     uint32_t tags_present = 0xffffffffu;    // tag bits of the uploaded particles
     bool odd_tags = true;                   // some tag outside 1..kMaxTag
     // search state
@@ -336,6 +337,12 @@ float uniform_hinv(bool valid, float hmin, float hmax) {
     return std::isfinite(hinv) && hinv > 0.0f ? hinv : 0.0f;
 }
 
+// The one Mass of every alive particle (the FAST terms then hoist it), else 0.
+float uniform_mass(bool valid, float mmin, float mmax) {
+    if (!valid || !(mmin == mmax) || !(mmin > 0.0f) || !std::isfinite(mmin)) return 0.0f;
+    return mmin;
+}
+
 // Criterion bounds: the criterion accepts every dr2 < rmin2 and rejects
 // every dr2 > rmax2.  defaultSearchCriterion: dr2 <= gap_square, i.e.
 // dr2 < the next float up, with an empty band; symmetry: the bounds the H
@@ -360,6 +367,7 @@ CaseDesc case_of(const eppx_ps *ps, const eppx_ns *ns, int32_t criterion) {
     c.tags_present = ps->tags_present;
     c.check_all = (c.rmin2 <= c.rmax2) || ps->odd_tags;
     c.hinv_u = ps->hinv_u;
+    c.m_u = ps->m_u;
     c.steps_left = 1;
     return c;
 }
@@ -504,9 +512,16 @@ void drop_graphs(eppx_ps *ps) {
 }
 
 // The pair kernels read m_j from the search snapshot; an operation that
-// changes Mass after the search refreshes it there too.
+// changes Mass after the search refreshes it there too.  The masses may no
+// longer be one value, so the FAST terms stop hoisting it (recorded steps
+// carry the old case table and are dropped).
 int32_t refresh_snapshot_mass(eppx_ps *ps, int32_t field) {
-    if (field != EPPX_FIELD_MASS || !ps->searched) return EPPX_OK;
+    if (field != EPPX_FIELD_MASS) return EPPX_OK;
+    if (ps->m_u != 0.0f) {
+        ps->m_u = 0.0f;
+        drop_graphs(ps);
+    }
+    if (!ps->searched) return EPPX_OK;
     HIP_OK(ps->ctx, launch_snapshot_mass(ps->geo, ps->f(), ps->n, ps->ctx->stream));
     return EPPX_OK;
 }
@@ -824,7 +839,7 @@ int32_t eppx_ps_create(eppx_ctx *ctx, int32_t n_capacity, const eppx_layout *lay
     if (e == hipSuccess) e = dalloc(&ps->n_alive, 2);
     if (e == hipSuccess) e = dalloc(&ps->masks, 3 * (size_t)n_capacity);
     if (e == hipSuccess) e = dalloc(&ps->pairs, 2);
-    if (e == hipSuccess) e = dalloc(&ps->h_range, 5);
+    if (e == hipSuccess) e = dalloc(&ps->h_range, 8);
     if (e == hipSuccess) e = hipMemsetAsync(ps->pairs, 0, 2 * sizeof(unsigned long long), ctx->stream);
     if (e == hipSuccess) e = hipMemsetAsync(ps->n_alive, 0, 2 * sizeof(int32_t), ctx->stream);
     // the filter and extrapolation sums start at zero in both buffers (the
@@ -919,8 +934,8 @@ int32_t eppx_ps_upload(eppx_ps *ps, int32_t n_particles, const int32_t *is_alive
     ps->n = n_particles;
     // H range over alive particles for the criterion's exact-safe bounds, and
     // the tags present.
-    const float init[5] = {INFINITY, 0.0f, 0.0f, 0.0f, 0.0f};
-    float range[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    const float init[8] = {INFINITY, 0.0f, 0.0f, 0.0f, 0.0f, INFINITY, 0.0f, 0.0f};
+    float range[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     if (e == hipSuccess) e = hipMemcpyAsync(ps->h_range, init, sizeof(init), hipMemcpyHostToDevice, s);
     if (e == hipSuccess && n_particles) e = launch_h_range(ps->buf[0], n_particles, ps->h_range, s);
     if (e == hipSuccess) e = hipMemcpyAsync(range, ps->h_range, sizeof(range), hipMemcpyDeviceToHost, s);
@@ -937,6 +952,9 @@ int32_t eppx_ps_upload(eppx_ps *ps, int32_t n_particles, const int32_t *is_alive
     ps->odd_tags = odd != 0;
     symmetry_bounds(!bad && has(L.H), range[0], range[1], &ps->rmin2, &ps->rmax2);
     ps->hinv_u = uniform_hinv(!bad && has(L.H), range[0], range[1]);
+    uint32_t mbad;
+    memcpy(&mbad, &range[7], sizeof(mbad));
+    ps->m_u = uniform_mass(!mbad && has(L.Mass), range[5], range[6]);
     ps->accum_live = accumulators_nonzero(L, float_props, cap);  // both buffers now hold these rows
     ps->dvel_nonzero_tag = columns_zero(L.dVelocityVec, 2, float_props, cap) ? -2 : -1;
     // keep the columns the kernels do not own
@@ -1815,8 +1833,8 @@ int32_t eppx_batch_add(eppx_batch *b, const eppx_domain2d *domain, const eppx_ph
     if ((rc = pull_steps(b))) return rc;
     // H range over alive particles and the tags present, on the host
     const size_t rows = (size_t)n_particles;
-    float hmin = INFINITY, hmax = 0.0f;
-    bool bad = false, odd = false;
+    float hmin = INFINITY, hmax = 0.0f, mmin = INFINITY, mmax = 0.0f;
+    bool bad = false, odd = false, mbad = !has(L.Mass);
     uint32_t tags = 0;
     for (size_t r = 0; r < rows; ++r) {
         const int32_t t = int_props[r + rows * (size_t)L.Tag];
@@ -1830,6 +1848,12 @@ int32_t eppx_batch_add(eppx_batch *b, const eppx_domain2d *domain, const eppx_ph
             hmin = std::min(hmin, h);
             hmax = std::max(hmax, h);
         }
+        const float m = float_props[r + rows * (size_t)L.Mass];
+        if (!(m > 0.0f) || !std::isfinite(m)) mbad = true;
+        else {
+            mmin = std::min(mmin, m);
+            mmax = std::max(mmax, m);
+        }
     }
     CaseDesc cd{};
     const eppx_domain2d &d = *domain;
@@ -1838,6 +1862,7 @@ int32_t eppx_batch_add(eppx_batch *b, const eppx_domain2d *domain, const eppx_ph
     cd.tags_present = tags;
     cd.check_all = (cd.rmin2 <= cd.rmax2) || odd;
     cd.hinv_u = uniform_hinv(!bad, hmin, hmax);
+    cd.m_u = uniform_mass(!mbad, mmin, mmax);
     cd.dt = ph.dt;
     cd.steps_left = steps;
     cd.fresh = 1;

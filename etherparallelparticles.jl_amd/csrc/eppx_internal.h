@@ -39,6 +39,8 @@ struct CaseDesc {
     uint32_t tags_present;  // tag bits of the case's particles (a superset)
     int32_t check_all;      // the band is not empty, or some tag lies outside 1..kMaxTag
     float hinv_u;           // > 0: every alive particle has one H, and this is (H + H) / (2 H H); 0: H varies
+    float m_u;              // > 0: every alive particle has this one Mass (with hinv_u > 0: the FAST terms
+                            // hoist it out of the pair sums); 0: Mass varies
     float dt;               // batch: the case's time step (a particle system uses the call's physics)
     int32_t steps_left;     // > 0: the case advances; a batch step decrements it
     int32_t fresh;          // batch: the case waits for its first search

@@ -556,14 +556,15 @@ __global__ __launch_bounds__(kBlock) void k_finite(Fields f, int32_t n, uint32_t
         atomicOr(bad, 1u);
 }
 
-// min / max of H over alive particles; h is positive so the float bit
-// patterns order like unsigned integers.  range[2] is set when some alive
-// h is not a positive finite number.  Grid-stride over the particles, one
-// set of atomics per block.  range[3] collects the tag bits of particles 0..n-1 (alive or not) and
-// range[4] is set when one of them has a tag outside 1..kMaxTag.
+// min / max of H and of Mass over alive particles; both are positive, so the
+// float bit patterns order like unsigned integers.  range[2] (range[7]) is
+// set when some alive h (mass) is not a positive finite number.  range[3]
+// collects the tag bits of particles 0..n-1 (alive or not) and range[4] is set
+// when one of them has a tag outside 1..kMaxTag; range[5], range[6]: the mass
+// min and max.  Grid-stride over the particles, one set of atomics per block.
 __global__ __launch_bounds__(kBlock) void k_h_range(Fields f, int32_t n, float *range) {
-    __shared__ unsigned int part[5][kBlock / 64];
-    unsigned int lo = 0xffffffffu, hi = 0u, bad = 0u, tags = 0u, odd = 0u;
+    __shared__ unsigned int part[8][kBlock / 64];
+    unsigned int lo = 0xffffffffu, hi = 0u, bad = 0u, tags = 0u, odd = 0u, mlo = 0xffffffffu, mhi = 0u, mbad = 0u;
     for (int s = blockIdx.x * kBlock + threadIdx.x; s < n; s += gridDim.x * kBlock) {
         const uint32_t tb = tag_bit(f.tag[s]);
         tags |= tb;
@@ -576,6 +577,13 @@ __global__ __launch_bounds__(kBlock) void k_h_range(Fields f, int32_t n, float *
                 lo = min(lo, __float_as_uint(h));
                 hi = max(hi, __float_as_uint(h));
             }
+            const float m = f.mass[s];
+            if (!(m > 0.0f) || !isfinite(m)) {
+                mbad = 1u;
+            } else {
+                mlo = min(mlo, __float_as_uint(m));
+                mhi = max(mhi, __float_as_uint(m));
+            }
         }
     }
     for (int off = 32; off > 0; off >>= 1) {
@@ -584,6 +592,9 @@ __global__ __launch_bounds__(kBlock) void k_h_range(Fields f, int32_t n, float *
         bad |= (unsigned int)__shfl_down((int)bad, off);
         tags |= (unsigned int)__shfl_down((int)tags, off);
         odd |= (unsigned int)__shfl_down((int)odd, off);
+        mlo = min(mlo, (unsigned int)__shfl_down((int)mlo, off));
+        mhi = max(mhi, (unsigned int)__shfl_down((int)mhi, off));
+        mbad |= (unsigned int)__shfl_down((int)mbad, off);
     }
     const int wave = threadIdx.x >> 6;
     if (lane_id() == 0) {
@@ -592,6 +603,9 @@ __global__ __launch_bounds__(kBlock) void k_h_range(Fields f, int32_t n, float *
         part[2][wave] = bad;
         part[3][wave] = tags;
         part[4][wave] = odd;
+        part[5][wave] = mlo;
+        part[6][wave] = mhi;
+        part[7][wave] = mbad;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -601,6 +615,9 @@ __global__ __launch_bounds__(kBlock) void k_h_range(Fields f, int32_t n, float *
             bad |= part[2][k];
             tags |= part[3][k];
             odd |= part[4][k];
+            mlo = min(mlo, part[5][k]);
+            mhi = max(mhi, part[6][k]);
+            mbad |= part[7][k];
         }
         unsigned int *r = reinterpret_cast<unsigned int *>(range);
         if (lo != 0xffffffffu) atomicMin(&r[0], lo);
@@ -608,6 +625,9 @@ __global__ __launch_bounds__(kBlock) void k_h_range(Fields f, int32_t n, float *
         if (bad) atomicMax(&r[2], 1u);
         if (tags) atomicOr(&r[3], tags);
         if (odd) atomicMax(&r[4], 1u);
+        if (mlo != 0xffffffffu) atomicMin(&r[5], mlo);
+        if (mhi) atomicMax(&r[6], mhi);
+        if (mbad) atomicMax(&r[7], 1u);
     }
 }
 

@@ -159,24 +159,30 @@ struct PairH {
 // av_a = 2 alpha c hinv.
 template <int K>
 struct UniformTerms {
-    float hinv, kfac, av_b, av_a;
+    float hinv, hinv2, kfac, av_b, av_a;
     __device__ __forceinline__ UniformTerms(float hinv_u, const Physics &ph) {
         hinv = hinv_u;
+        hinv2 = hinv * hinv;
         const float h3 = hinv * hinv * hinv;
         kfac = K == EPPX_KERNEL_CUBIC ? (kSigmaCubic * 0.75f) * h3 : ((-kSigmaWendland * 0.625f) * h3) * hinv;
         av_b = ph.beta * (2.0f * hinv * hinv);
         av_a = (ph.alpha * ph.av_c) * (2.0f * hinv);
     }
     __device__ __forceinline__ float shape(float r, float rinv) const {
-        const float q = r * hinv;
         if (K == EPPX_KERNEL_CUBIC) {
+            const float q = r * hinv;
             const float t = 2.0f - q;
             return (q < 1.0f ? q * fmaf(3.0f, q, -4.0f) : (q < 2.0f ? -(t * t) : 0.0f)) * rinv;
         }
-        const float t = fmaxf(2.0f - q, 0.0f);
+        const float t = fmaxf(fmaf(r, -hinv, 2.0f), 0.0f);  // 2 - q
         return t * (t * t);
     }
 };
+
+// p_j / rho_j^2 of the classic pressure term (FAST): one expression wherever
+// it is taken (per pair, or once per slot of a staged tile), so the pair sums
+// carry the same bits for every pair shape.
+__device__ __forceinline__ float pressure_ratio(float p, float rho) { return p * frcp(rho * rho); }
 
 // _value of CubicSpline.jl:17-31 / WendlandC2.jl:16-26 (2D), written order.
 __device__ __forceinline__ float value_exact(int kernel, float r, float hinv) {
@@ -357,6 +363,7 @@ __device__ __forceinline__ uint32_t accept_bits(const Walk &w, const CaseDesc &c
 // record, 2 its (u, v) half).
 struct GlobalRows {
     static constexpr int kStage = 0;
+    static constexpr bool kTile = false;
     const float4 *geo;
     const int32_t *tag;
     __device__ __forceinline__ int off(int) const { return 0; }
@@ -370,6 +377,7 @@ struct GlobalRows {
 template <int STAGE>
 struct TileRows {
     static constexpr int kStage = STAGE;
+    static constexpr bool kTile = true;
     const float2 *xy;     // LDS: (x, y) of row r, slot j at xy[j + off(r)]
     const float2 *hm;     // LDS: (h, m) of the same slot
     const uint8_t *tix;   // LDS: tag - 1 of the same slot (63 outside 1..kMaxTag)
@@ -485,10 +493,13 @@ __device__ __forceinline__ int walk_rows(const Walk &w, const CaseDesc &cd, cons
             cnt = __popcll(m0) + __popcll(m1) + __popcll(m2);
         }
     }
+    // dr2 of an accepted pair: the reference's two roundings in the ordered
+    // (EXACT) walk; one FMA in the FAST one, where it only feeds the pair terms
+    // (the accept decision above stays the reference's)
     auto visit = [&](int j, int roff, uint32_t tbj, const auto &p) {
         const float4 gj = src.get(j + roff);
         const float dx = xi.x - gj.x, dy = xi.y - gj.y;
-        apply(j, tbj, gj.z, gj.w, dx, dy, dx * dx + dy * dy, p);
+        apply(j, tbj, gj.z, gj.w, dx, dy, ORDERED ? dx * dx + dy * dy : fmaf(dx, dx, dy * dy), p);
     };
     if (ORDERED) {
         for (int s9 = 0; s9 < 9; ++s9) {
@@ -547,12 +558,15 @@ __device__ __forceinline__ int walk_rows(const Walk &w, const CaseDesc &cd, cons
 // rows or the ranges do not fit the tile; the block then reads global
 // memory.  only_tag: set when every staged slot (the block's own particles
 // among them) carries the tag `tag`.  info: kStageInfo ints of LDS.  Every
-// thread of the block calls it (two barriers).
+// thread of the block calls it (two barriers).  stage_pr (momentum, one H and
+// one Mass): the second tile array holds (x_j, rho_j) instead of (h_j, m_j),
+// with x_j = pressure_ratio(p_j, rho_j) for slots of tag `tag` (fluid) and p_j
+// for the others -- the pair terms then read them instead of h and m.
 constexpr int kStageInfo = 16;
 template <int BLOCK, int STAGE>
 __device__ __forceinline__ bool stage_rows(const Walk &w, const float4 *dyn, int i0, int n, float2 *txy,
                                            float2 *thm, uint8_t *tix, float4 *tdyn, float2 *tuv, int *info,
-                                           TileRows<STAGE> &rows, int tag, bool &only_tag) {
+                                           TileRows<STAGE> &rows, int tag, bool &only_tag, bool stage_pr = false) {
     constexpr int kCap = 3 * PairShape<BLOCK>::tile_row;
     const int t = threadIdx.x;
     if (t < 6) {
@@ -596,8 +610,13 @@ __device__ __forceinline__ bool stage_rows(const Walk &w, const float4 *dyn, int
             for (int m = t; m < len; m += BLOCK) {
                 const float4 g = w.geo[b + m];
                 txy[d + m] = make_float2(g.x, g.y);
-                thm[d + m] = make_float2(g.z, g.w);
                 const int tg = w.tag[b + m];
+                if (stage_pr) {
+                    const float2 rp = reinterpret_cast<const float2 *>(dyn)[2 * (size_t)(b + m) + 1];  // (rho, p)
+                    thm[d + m] = make_float2(tg == tag ? pressure_ratio(rp.y, rp.x) : rp.y, rp.x);
+                } else {
+                    thm[d + m] = make_float2(g.z, g.w);
+                }
                 other |= tg != tag;
                 const uint32_t t1 = (uint32_t)tg - 1u;
                 tix[d + m] = (uint8_t)(t1 < (uint32_t)kMaxTag ? t1 : 63u);
@@ -721,6 +740,15 @@ __device__ __forceinline__ float continuity_fast_u(const UniformTerms<K> &ut, fl
     return (mj_or_rhoVj * vdotx) * ut.shape(dr2 * rinv, rinv);
 }
 
+// The fluid-fluid term over kfac m_u (one H and one Mass in the case).
+template <int K>
+__device__ __forceinline__ float vdotx_shape_u(const UniformTerms<K> &ut, float2 ui, float2 uj, float dx, float dy,
+                                               float dr2) {
+    const float rinv = frsq(dr2);
+    const float vdotx = fmaf(dy, ui.y - uj.y, dx * (ui.x - uj.x));
+    return vdotx * ut.shape(dr2 * rinv, rinv);
+}
+
 // The reference's written order, given the pair's hinv = invharmonic(h_i, h_j).
 template <int K>
 __device__ __forceinline__ float continuity_exact(float hinv, float2 ui, float2 uj, float mj_or_rhoVj, float dx,
@@ -821,7 +849,41 @@ __global__ __launch_bounds__(BLOCK, BATCH ? 1 : 8) void k_search(Walk w, Fields 
                 // One H in the case: the pair's hinv is loop invariant.  Untiled
                 // blocks take the same form (through global memory), so every
                 // pair shape and a case alone or in a batch give the same bits.
-                if (cd.hinv_u > 0.0f) {
+                if (cd.hinv_u > 0.0f && cd.m_u > 0.0f) {
+                    // One Mass as well: the fluid-fluid terms (classic continuity) leave out m_j too,
+                    // the mixed ones (balanced) carry their rho_i V_j; both sums are taken in every
+                    // block shape, so a fluid-fluid pair adds the same bits everywhere.
+                    const UniformTerms<K> ut(cd.hinv_u, ph);
+                    float s_ff = 0.0f, s_mx = 0.0f;
+                    if (all_fluid) {
+                        auto fetch_f = [&](int j, int k, uint32_t, const auto &src) {
+                            ContinuityGather p;
+                            if constexpr (std::decay_t<decltype(src)>::kStage == 2) p.uv = src.uv[k];
+                            else p.uv = ld_slot(uv, 2 * j);
+                            p.vol = 0.0f;
+                            return p;
+                        };
+                        auto term_f = [&](int, uint32_t, float, float, float dx, float dy, float dr2,
+                                          const ContinuityGather &p) {
+                            s_ff += vdotx_shape_u<K>(ut, ui, p.uv, dx, dy, dr2);
+                        };
+                        cnt = walk_rows<false, true>(w, cd, rows, i, gi, keep, fetch_f, term_f);
+                    } else {
+                        auto term_m = [&](int, uint32_t tbj, float, float, float dx, float dy, float dr2,
+                                          const ContinuityGather &p) {
+                            const bool ff = i_fluid && tbj == fluid_bit;
+                            const float rinv = frsq(dr2);
+                            const float shape = ut.shape(dr2 * rinv, rinv);
+                            const float vdotx = fmaf(dy, ui.y - p.uv.y, dx * (ui.x - p.uv.x));
+                            s_ff += ff ? vdotx * shape : 0.0f;
+                            s_mx += ff ? 0.0f : ((rho_i * p.vol) * vdotx) * shape;
+                        };
+                        if (tiled) cnt = walk_rows<false, true>(w, cd, rows, i, gi, keep, fetch, term_m);
+                        else cnt = walk_rows<false, true>(w, cd, GlobalRows{w.geo, w.tag}, i, gi, keep, fetch, term_m);
+                    }
+                    acc = fmaf(ut.kfac, s_mx, fmaf(ut.kfac * cd.m_u, s_ff, acc));
+                    walked = true;
+                } else if (cd.hinv_u > 0.0f) {
                     const UniformTerms<K> ut(cd.hinv_u, ph);
                     float sum = 0.0f;  // the pair terms over kfac
                     if (all_fluid) {  // fluid-fluid pairs only (classic continuity): no tag per pair (tiled)
@@ -948,16 +1010,44 @@ __device__ __forceinline__ void momentum_fast_u(const Physics &ph, float2 &sum, 
     float prho2 = classic ? fmaf(p_j, frcp(rho_j * rho_j), pr_i) : (p_i + p_j) * frcp(rirj);
     prho2 = fmaf(fabsf(prho2), ph.coefficient, prho2);
     float radial = prho2 * mdwr;
-    const float q = r * ut.hinv;
-    const float qa = fmaf(q, q, 0.01f);
+    const float qa = fmaf(dr2, ut.hinv2, 0.01f);  // q^2 + 0.01
     if (VISC == EPPX_VISC_CLASSIC) {
-        const float force = (-2.0f * ph.mu) * (q * q) * mdwr * frcp(rirj * qa);
+        const float force = (-2.0f * ph.mu) * (dr2 * ut.hinv2) * mdwr * frcp(rirj * qa);
         sum.x = fmaf(force, ui.x - uj.x, sum.x);
         sum.y = fmaf(force, ui.y - uj.y, sum.y);
     } else {
         const float m = fminf(fmaf(dy, ui.y - uj.y, dx * (ui.x - uj.x)), 0.0f);
         const float den = (qa * qa) * ((rho_i + rho_j) * rirj);
         radial = fmaf(fmaf(ut.av_b, m, -ut.av_a * qa) * m, mdwr * frcp(den), radial);
+    }
+    sum.x = fmaf(radial, dx, sum.x);
+    sum.y = fmaf(radial, dy, sum.y);
+}
+
+// One H and one Mass in the case, and no pressure correction (coefficient 0,
+// as in the driver) (FAST): the sums leave out -m_u as well (every term
+// carries shape where the terms above carry mdwr = -m_j shape), and the
+// caller adds -m_u kfac x sum once.  x_j: p_j / rho_j^2 (pressure_ratio) for a
+// classic (fluid) neighbour, p_j for a balanced (wall) one.
+template <int K, int VISC>
+__device__ __forceinline__ void momentum_fast_um(const Physics &ph, float2 &sum, bool classic,
+                                                 const UniformTerms<K> &ut, float2 ui, float rho_i, float p_i,
+                                                 float pr_i, float2 uj, float rho_j, float x_j, float dx, float dy,
+                                                 float dr2) {
+    const float rinv = frsq(dr2);
+    const float shape = ut.shape(dr2 * rinv, rinv);
+    const float rirj = rho_i * rho_j;
+    const float prho2 = classic ? pr_i + x_j : (p_i + x_j) * frcp(rirj);  // (the caller has coefficient 0)
+    float radial = prho2 * shape;
+    const float qa = fmaf(dr2, ut.hinv2, 0.01f);  // q^2 + 0.01
+    if (VISC == EPPX_VISC_CLASSIC) {
+        const float force = (-2.0f * ph.mu) * (dr2 * ut.hinv2) * shape * frcp(rirj * qa);
+        sum.x = fmaf(force, ui.x - uj.x, sum.x);
+        sum.y = fmaf(force, ui.y - uj.y, sum.y);
+    } else {
+        const float m = fminf(fmaf(dy, ui.y - uj.y, dx * (ui.x - uj.x)), 0.0f);
+        const float den = (qa * qa) * ((rho_i + rho_j) * rirj);
+        radial = fmaf(fmaf(ut.av_b, m, -ut.av_a * qa) * m, shape * frcp(den), radial);
     }
     sum.x = fmaf(radial, dx, sum.x);
     sum.y = fmaf(radial, dy, sum.y);
@@ -1034,9 +1124,16 @@ __global__ __launch_bounds__(BLOCK, (BATCH && !FAST) ? 1 : 8) void k_momentum(Wa
     TileRows<STAGE> rows;
     bool all_fluid = false;  // every particle of the block's neighbourhood is fluid
     bool tiled = false;
-    if (i0 < n)  // the same for the whole block
+    bool stage_pr = false;   // the tile holds (x_j, rho_j) (the one-H, one-Mass FAST terms)
+    if (i0 < n) {  // the same for the whole block
+        if (FAST && STAGE == 0) {
+            // a tiled block lies in one case: the case of its first slot
+            const CaseDesc c0 = BATCH ? w.cases[w.cs[i0]] : w.one;
+            stage_pr = c0.hinv_u > 0.0f && c0.m_u > 0.0f && ph.coefficient == 0.0f;
+        }
         tiled = stage_rows<BLOCK, STAGE>(w, f.dyn, i0, n, txy, thm, tix, tdyn, nullptr, info, rows, ph.fluid_tag,
-                                         all_fluid);
+                                         all_fluid, stage_pr);
+    }
     const int i = i0 + threadIdx.x;
     if (!MOVE && i >= n) return;
     const CaseDesc cd = BATCH ? w.cases[w.cs[min(i, n - 1)]] : w.one;
@@ -1052,7 +1149,7 @@ __global__ __launch_bounds__(BLOCK, (BATCH && !FAST) ? 1 : 8) void k_momentum(Wa
     const float2 ui = make_float2(di.x, di.y);
     acc = (MOVE && mk.dvel_zero) ? make_float2(0.0f, 0.0f) : f.dvel[i];
     // p_i / rho_i^2 of the classic pressure, once (EXACT: the reference's expression, the same bits)
-    const float pr_i = FAST ? di.w * frcp(di.z * di.z) : di.w / (di.z * di.z);
+    const float pr_i = FAST ? pressure_ratio(di.w, di.z) : di.w / (di.z * di.z);
     const uint32_t fluid_bit = tag_bit(ph.fluid_tag);
     auto fetch = [&](int j, int k, uint32_t, const auto &src) -> float4 {
         if constexpr (std::decay_t<decltype(src)>::kStage == 1) return src.dyn[k];
@@ -1068,7 +1165,45 @@ __global__ __launch_bounds__(BLOCK, (BATCH && !FAST) ? 1 : 8) void k_momentum(Wa
         // One H in the case: the pair's hinv is loop invariant.  Untiled blocks
         // take the same form (through global memory), so every pair shape and
         // a case alone or in a batch give the same bits.
-        if (cd.hinv_u > 0.0f) {
+        if (cd.hinv_u > 0.0f && cd.m_u > 0.0f && ph.coefficient == 0.0f) {
+            // One Mass as well (and no pressure correction): the sums leave out -m_u, and every
+            // neighbour's x_j (p_j / rho_j^2 or p_j) comes from the tile or is taken per pair with the
+            // same expression.
+            const UniformTerms<K> ut(cd.hinv_u, ph);
+            float2 sum = make_float2(0.0f, 0.0f);  // the pair terms over -m_u kfac
+            auto fetch_um = [&](int j, int k, uint32_t tbj, const auto &src) -> float4 {
+                using Src = std::decay_t<decltype(src)>;
+                if constexpr (Src::kTile && Src::kStage == 0) {  // (u, v) gathered, (x_j, rho_j) from the tile
+                    const float2 uvj = ld_slot(reinterpret_cast<const float2 *>(f.dyn), 2 * j);
+                    const float2 xr = src.hm[k];
+                    return make_float4(uvj.x, uvj.y, xr.y, xr.x);
+                } else {
+                    float4 d;
+                    if constexpr (Src::kStage == 1) d = src.dyn[k];
+                    else d = ld_slot(f.dyn, j);
+                    d.w = tbj == fluid_bit ? pressure_ratio(d.w, d.z) : d.w;
+                    return d;
+                }
+            };
+            if (all_fluid) {  // fluid-fluid pairs only (classic pressure): no tag per pair (tiled blocks)
+                auto term_f = [&](int, uint32_t, float, float, float dx, float dy, float dr2, const float4 &dj) {
+                    momentum_fast_um<K, VISC>(ph, sum, true, ut, ui, di.z, di.w, pr_i, make_float2(dj.x, dj.y), dj.z,
+                                              dj.w, dx, dy, dr2);
+                };
+                walk_rows<false, false>(w, cd, rows, i, gi, keep, fetch_um, term_f);
+            } else {
+                auto term_m = [&](int, uint32_t tbj, float, float, float dx, float dy, float dr2, const float4 &dj) {
+                    momentum_fast_um<K, VISC>(ph, sum, tbj == fluid_bit, ut, ui, di.z, di.w, pr_i,
+                                              make_float2(dj.x, dj.y), dj.z, dj.w, dx, dy, dr2);
+                };
+                if (tiled) walk_rows<false, false>(w, cd, rows, i, gi, keep, fetch_um, term_m);
+                else walk_rows<false, false>(w, cd, GlobalRows{w.geo, w.tag}, i, gi, keep, fetch_um, term_m);
+            }
+            const float km = -(cd.m_u * ut.kfac);
+            acc.x = fmaf(km, sum.x, acc.x);
+            acc.y = fmaf(km, sum.y, acc.y);
+            walked = true;
+        } else if (cd.hinv_u > 0.0f) {
             const UniformTerms<K> ut(cd.hinv_u, ph);
             float2 sum = make_float2(0.0f, 0.0f);  // the pair terms over kfac
             if (all_fluid) {  // fluid-fluid pairs only (classic pressure): no tag per pair (tiled blocks)
