@@ -117,6 +117,8 @@ def capacity_expand(source: str, n: int) -> int:
     m = _EXPAND.match(source)
     if not m:
         raise ValueError(f"capacity_expand must read 'n -> <expression>', got {source!r}")
+    if "//" in m.group(1):
+        raise ValueError(f"capacity_expand: '//' is Julia's rational division, not an integer: {source!r}")
     expr = m.group(1).replace("÷", "//")
     expr = re.sub(r"(\d)\s*n\b", r"\1 * n", expr)  # Julia's juxtaposed 2n
     try:
@@ -144,7 +146,8 @@ def capacity_expand(source: str, n: int) -> int:
                 return a * b
             if b == 0:
                 raise ValueError("capacity_expand: division by zero")
-            return int(a / b) if (a < 0) != (b < 0) else a // b  # Julia's ÷ truncates toward zero
+            q = abs(a) // abs(b)  # Julia's ÷ truncates toward zero (integers only: exact at any size)
+            return -q if (a < 0) != (b < 0) else q
         if (isinstance(node, ast.Call) and isinstance(node.func, ast.Name) and node.func.id == "div"
                 and len(node.args) == 2 and not node.keywords):
             return ev(ast.BinOp(left=node.args[0], op=ast.FloorDiv(), right=node.args[1]))

@@ -51,13 +51,15 @@ def test_template_has_the_reference_sections():
     ("n -> n + div(n, 10)", 95, 104),
     ("n -> n ÷ 2 + n", 9, 13),
     ("n -> (n + 3) * 2 - n", 5, 11),
+    ("n -> 2n + (1 - 2n) ÷ 4", 4, 7),  # -7 ÷ 4 truncates toward zero: -1 (floor division gives -2)
+    ("n -> n + (n - 27021597764222980) ÷ 3 + 9007199254740993", 1, 1),  # -(2^53 + 1) exactly, past float64
 ])
 def test_capacity_expand(src, n, cap):
     assert DataIO.capacity_expand(src, n) == cap
 
 
 @pytest.mark.parametrize("src", ["n -> run(`ls`)", "n -> n + 1.5", "x -> x", "n -> n - 1", "n -> __import__('os')",
-                                 "n -> n ÷ 0", "n -> n ** 2"])
+                                 "n -> n ÷ 0", "n -> n ** 2", "n -> n // 2 + n"])
 def test_capacity_expand_refuses(src):
     with pytest.raises(ValueError):
         DataIO.capacity_expand(src, 4)
