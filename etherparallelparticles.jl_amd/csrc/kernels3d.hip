@@ -1,0 +1,699 @@
+// Device code of the 3D path of libeppx: the cell-list build, the pair
+// kernels (neighbour search + kernel + continuity, momentum), the self
+// kernels and the reference-layout conversion.  gfx950, wave64.
+//
+// Pair kernels keep one thread per particle.  A particle walks its 27
+// neighbour cells in the stencil order (x offset outer, y, z inner) and each
+// cell's slots in ascending particle index -- the reference's list order, so
+// EXACT sums are the restatement's bit for bit.  A cell range is taken in
+// chunks of 64 candidates: pass 1 tests every candidate of the chunk (one
+// 16-byte load of x, y, z, tag each) into a 64-bit accept mask, pass 2 runs
+// the pair physics over the set bits.  A cell holding no particle of a tag
+// adjacent to the particle's is skipped as a whole (wall particles away from
+// the fluid have no pairs and cost only the cell-tag reads).
+#include "d3_internal.h"
+#include "pair_device.h"
+
+namespace eppx {
+namespace {
+
+constexpr int kBlock3 = 256;
+
+// sigma of the 3D kernels: FT(1/pi) (CubicSpline.jl:15), FT(21/16/pi) (WendlandC2.jl:15)
+constexpr float kSigmaCubic3 = (float)(1.0 / 3.14159265358979323846);
+constexpr float kSigmaWendland3 = (float)(21.0 / 16.0 / 3.14159265358979323846);
+
+// _value with N = 3 (CubicSpline.jl:17-31, WendlandC2.jl:16-26) in the
+// written order; power(hinv, Val(3)) = hinv * hinv * hinv,
+// power(2 - q, Val(4)) = (t * t) * (t * t) (OptimizedFunction.jl:10-31).
+__device__ __forceinline__ float value3_exact(int kernel, float r, float hinv) {
+    const float q = r * hinv;
+    const float h3 = hinv * hinv * hinv;
+    if (kernel == EPPX_KERNEL_CUBIC) {
+        if (q < 1.0f) return kSigmaCubic3 * h3 * (3.0f * q * q * (q - 2.0f) + 4.0f) * 0.25f;
+        if (q < 2.0f) {
+            const float t = 2.0f - q;
+            return kSigmaCubic3 * h3 * (t * t * t) * 0.25f;
+        }
+        return 0.0f;
+    }
+    if (q < 2.0f) {
+        const float t = 2.0f - q;
+        const float t2 = t * t;
+        return kSigmaWendland3 * (t2 * t2) * h3 * (1.0f + 2.0f * q) * 0.0625f;
+    }
+    return 0.0f;
+}
+
+// _gradient with N = 3 (CubicSpline.jl:41-55, WendlandC2.jl:37-47), written
+// order; power(hinv, Val(4)) = (hinv * hinv) * (hinv * hinv).
+template <int K>
+__device__ __forceinline__ float grad3_exact(float r, float hinv) {
+    const float q = r * hinv;
+    const float hh = hinv * hinv;
+    const float h4 = hh * hh;
+    if (K == EPPX_KERNEL_CUBIC) {
+        if (q < 1.0f) return kSigmaCubic3 * h4 * q * (3.0f * q - 4.0f) * 0.75f;
+        if (q < 2.0f) {
+            const float t = 2.0f - q;
+            return -kSigmaCubic3 * h4 * t * t * 0.75f;
+        }
+        return 0.0f;
+    }
+    if (q < 2.0f) {
+        const float t = 2.0f - q;
+        return -kSigmaWendland3 * h4 * 0.625f * q * (t * t * t);
+    }
+    return 0.0f;
+}
+
+// dw / r of the same kernels, contracted (FAST): Wendland C2's dw carries q =
+// r hinv, so dw / r = (-sigma 5/8) hinv^5 (2 - q)^3 needs no reciprocal.
+template <int K>
+__device__ __forceinline__ float grad3_over_r_fast(float r, float rinv, float hinv) {
+    const float hh = hinv * hinv;
+    const float h4 = hh * hh;
+    const float q = r * hinv;
+    if (K == EPPX_KERNEL_CUBIC) {
+        const float t = 2.0f - q;
+        const float inner = (kSigmaCubic3 * 0.75f) * (h4 * hinv) * fmaf(3.0f, q, -4.0f);
+        const float outer = (-kSigmaCubic3 * 0.75f) * h4 * (t * t) * rinv;
+        return q < 1.0f ? inner : (q < 2.0f ? outer : 0.0f);
+    }
+    const float t = fmaxf(2.0f - q, 0.0f);
+    return (-kSigmaWendland3 * 0.625f) * (h4 * hinv) * (t * (t * t));
+}
+
+// The criterion of device_findNeighbourParticlesFromCells! for a pair in the
+// band the host bounds leave open (NeighbourSearch.jl:141-164).
+__device__ __forceinline__ bool criterion3(const Walk3 &w, float hi, float hj, float dr2) {
+    if (w.criterion == EPPX_CRIT_DEFAULT) return dr2 <= w.g.gap_square;
+    const float h2 = harmonic(hi, hj) * 2.0f;
+    return dr2 <= h2 * h2;
+}
+
+// dr2 = dx*dx + dy*dy + dz*dz, left to right, no FMA (the 2D sum with the z term added).
+__device__ __forceinline__ float dist2(float dx, float dy, float dz) { return dx * dx + dy * dy + dz * dz; }
+
+// Accept mask of candidates [base, base + len), len <= 64, of slot i: bit k
+// when base + k != i, (CHECK) the tags are adjacent and the criterion
+// accepts.  Candidates are taken four at a time; the slots read past the
+// range stay inside the padded arrays and are masked off.
+template <bool CHECK, bool BAND>
+__device__ __forceinline__ unsigned long long accept3(const Walk3 &w, int i, float4 gi, float hi, uint32_t adj_i,
+                                                      int base, int len) {
+    unsigned long long mask = 0ull;
+    for (int k = 0; k < len; k += 4) {
+        uint32_t nib = 0u;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const float4 gj = ld_slot(w.geo, base + k + u);
+            const float dr2 = dist2(gi.x - gj.x, gi.y - gj.y, gi.z - gj.z);
+            bool ok = dr2 < w.rmin2;
+            if (BAND && !ok && dr2 <= w.rmax2) ok = criterion3(w, hi, ld_slot(w.hmvp, base + k + u).x, dr2);
+            if (CHECK) ok = ok && (adj_i & tag_bit(__float_as_int(gj.w))) != 0u;
+            nib |= ok ? (1u << u) : 0u;
+        }
+        mask |= (unsigned long long)nib << k;
+    }
+    if (len < 64) mask &= (1ull << len) - 1ull;
+    const unsigned d = (unsigned)(i - base);
+    if (d < 64u) mask &= ~(1ull << d);
+    return mask;
+}
+
+// Visits the accepted neighbours of sorted slot i in the reference's list
+// order: visit(j, geo_j, dx, dy, dz, dr2).
+template <bool CHECK, bool BAND, class Visit>
+__device__ __forceinline__ void walk3(const Walk3 &w, int i, float4 gi, float hi, uint32_t adj_i, Visit &&visit) {
+    const Grid3 &g = w.g;
+    const int c = w.cell[i] - 1;
+    const int cx = c % g.n_x, t = c / g.n_x;
+    const int cy = t % g.n_y, cz = t / g.n_y;
+    for (int di = -1; di <= 1; ++di) {
+        const int x = cx + di;
+        if (x < 0 || x >= g.n_x) continue;
+        for (int dj = -1; dj <= 1; ++dj) {
+            const int y = cy + dj;
+            if (y < 0 || y >= g.n_y) continue;
+            for (int dk = -1; dk <= 1; ++dk) {
+                const int z = cz + dk;
+                if (z < 0 || z >= g.n_z) continue;
+                const int nb = x + g.n_x * (y + g.n_y * z);
+                if (!(w.cell_tags[nb] & adj_i)) continue;
+                const int b = w.start[nb], e = w.start[nb + 1];
+                for (int base = b; base < e; base += 64) {
+                    unsigned long long m = accept3<CHECK, BAND>(w, i, gi, hi, adj_i, base, min(64, e - base));
+                    while (m) {
+                        const int j = base + (__ffsll((long long)m) - 1);
+                        m &= m - 1ull;
+                        const float4 gj = ld_slot(w.geo, j);
+                        const float dx = gi.x - gj.x, dy = gi.y - gj.y, dz = gi.z - gj.z;
+                        visit(j, gj, dx, dy, dz, dist2(dx, dy, dz));
+                    }
+                }
+            }
+        }
+    }
+}
+
+// walk3 with the candidate tag test only in waves holding a particle that
+// is not adjacent to every present tag, and the exact criterion band only
+// when the host bounds leave one.
+template <class Visit>
+__device__ __forceinline__ void walk3_any(const Walk3 &w, int i, float4 gi, float hi, uint32_t adj_i, bool check,
+                                          Visit &&visit) {
+    const bool band = w.rmin2 <= w.rmax2;
+    if (__ballot(check)) {
+        if (band) walk3<true, true>(w, i, gi, hi, adj_i, visit);
+        else walk3<true, false>(w, i, gi, hi, adj_i, visit);
+    } else {
+        if (band) walk3<false, true>(w, i, gi, hi, adj_i, visit);
+        else walk3<false, false>(w, i, gi, hi, adj_i, visit);
+    }
+}
+
+__device__ __forceinline__ uint32_t adj3(const Walk3 &w, uint32_t tb) { return tb ? w.adj[__builtin_ctz(tb) + 1] : 0u; }
+
+// iValueGradient! + iClassicContinuity! / iBalancedContinuity!
+// (Library/Kernel.jl:114-128, Continuity.jl:23-47, vdotx Library.jl:21-34
+// with N = 3); coef is m_j (fluid-fluid) or rho_i V_j (fluid-wall, wall-fluid).
+template <bool FAST, int K>
+__device__ __forceinline__ float continuity3(float hinv, float4 vi, float4 vj, float coef, float dx, float dy,
+                                             float dz, float dr2) {
+    if (!FAST) {
+        const float r = sqrtf(dr2);
+        const float dw = grad3_exact<K>(r, hinv);
+        float vdotx = 0.0f;
+        vdotx += dx * (vi.x - vj.x);
+        vdotx += dy * (vi.y - vj.y);
+        vdotx += dz * (vi.z - vj.z);
+        return coef * vdotx * dw / r;
+    }
+    const float rinv = frsq(dr2);
+    const float dwr = grad3_over_r_fast<K>(dr2 * rinv, rinv, hinv);
+    const float vdotx = fmaf(dz, vi.z - vj.z, fmaf(dy, vi.y - vj.y, dx * (vi.x - vj.x)));
+    return (coef * vdotx) * dwr;
+}
+
+// iClassicPressure! / iBalancedPressure! (Pressure.jl:24-60) and
+// iClassicViscosity! / sArtificialViscosity! (Viscosity.jl:10-52) with N = 3,
+// in the order iMomentum! (main.jl:107-118) adds them to du.  pr_i = p_i / rho_i^2.
+template <bool FAST, int K, int VISC>
+__device__ __forceinline__ void momentum3(const Physics &ph, float4 &acc, bool classic, float hinv, float4 vi,
+                                          float p_i, float pr_i, float4 vj, float4 qj, float dx, float dy, float dz,
+                                          float dr2) {
+    const float rho_i = vi.w, rho_j = vj.w, p_j = qj.w, m_j = qj.y;
+    const float dux = vi.x - vj.x, duy = vi.y - vj.y, duz = vi.z - vj.z;
+    if (!FAST) {
+        const float r = sqrtf(dr2);
+        const float dw = grad3_exact<K>(r, hinv);
+        float prho2 = classic ? pr_i + p_j / (rho_j * rho_j) : (p_i + p_j) / (rho_i * rho_j);
+        prho2 += fabsf(prho2) * ph.coefficient;
+        prho2 *= -m_j * dw / r;
+        acc.x += prho2 * dx;
+        acc.y += prho2 * dy;
+        acc.z += prho2 * dz;
+        const float h = 1.0f / hinv;
+        const float az = r * r + 0.01f * h * h;  // avoidzero, Library.jl:17-19
+        if (VISC == EPPX_VISC_CLASSIC) {
+            const float force = 2.0f * ph.mu * m_j * r * dw / (rho_i * rho_j * az);
+            acc.x += force * dux;
+            acc.y += force * duy;
+            acc.z += force * duz;
+        } else {
+            float vdotx = 0.0f;
+            vdotx += dx * dux;
+            vdotx += dy * duy;
+            vdotx += dz * duz;
+            if (vdotx > 0.0f) return;
+            const float rho = (rho_i + rho_j) * 0.5f;
+            const float phi = vdotx / (hinv * az);
+            float force = (-ph.alpha * ph.av_c + ph.beta * phi) * phi / rho;
+            force *= -m_j / (rho_i * rho_j * r) * dw;
+            acc.x += force * dx;
+            acc.y += force * dy;
+            acc.z += force * dz;
+        }
+        return;
+    }
+    // FAST: dw / r from the contracted kernel; r^2 + 0.01 h^2 = h^2 (q^2 + 0.01)
+    const float rinv = frsq(dr2);
+    const float mdwr = -m_j * grad3_over_r_fast<K>(dr2 * rinv, rinv, hinv);
+    float prho2 = classic ? fmaf(p_j, frcp(rho_j * rho_j), pr_i) : (p_i + p_j) * frcp(rho_i * rho_j);
+    prho2 = fmaf(fabsf(prho2), ph.coefficient, prho2);
+    const float radial = prho2 * mdwr;
+    acc.x = fmaf(radial, dx, acc.x);
+    acc.y = fmaf(radial, dy, acc.y);
+    acc.z = fmaf(radial, dz, acc.z);
+    const float q2 = dr2 * (hinv * hinv);
+    const float qa = q2 + 0.01f;
+    if (VISC == EPPX_VISC_CLASSIC) {
+        const float force = (-2.0f * ph.mu) * mdwr * q2 * frcp(rho_i * rho_j * qa);
+        acc.x = fmaf(force, dux, acc.x);
+        acc.y = fmaf(force, duy, acc.y);
+        acc.z = fmaf(force, duz, acc.z);
+    } else {
+        // a separating pair (vdotx > 0) gets phi = 0 and with it no term
+        const float vdotx = fminf(fmaf(dz, duz, fmaf(dy, duy, dx * dux)), 0.0f);
+        const float phi = vdotx * hinv * frcp(qa);
+        const float force = fmaf(ph.beta, phi, -ph.alpha * ph.av_c) * phi * mdwr *
+                            frcp((rho_i + rho_j) * 0.5f * (rho_i * rho_j));
+        acc.x = fmaf(force, dx, acc.x);
+        acc.y = fmaf(force, dy, acc.y);
+        acc.z = fmaf(force, dz, acc.z);
+    }
+}
+
+// ------------------------------------------------------------ pair kernels
+// Search pass: the neighbours of every alive particle, counted into ncount,
+// with the fused action (EPPX_SEARCH_KERNEL_CONTINUITY:
+// sComputeKernelAndiContinuity!, main.jl:74-90, onto the existing dDensity).
+template <bool FAST, int ACTION, int K>
+__global__ __launch_bounds__(kBlock3) void k3_search(Walk3 w, Fields3 f, Physics ph, int32_t M, int32_t *ncount,
+                                                      unsigned long long *pairs, uint32_t *flags) {
+    __shared__ int lds[kBlock3 / 64];
+    const int i = xcd_block(blockIdx.x, gridDim.x) * kBlock3 + threadIdx.x;
+    const int n = w.n_alive[0];
+    int cnt = 0;
+    if (i < n) {
+        const float4 gi = w.geo[i];
+        const float4 vi = w.vel[i];
+        const float hi = w.hmvp[i].x;
+        const uint32_t tbi = tag_bit(__float_as_int(gi.w));
+        const uint32_t adj_i = adj3(w, tbi);
+        const uint32_t fluid_bit = tag_bit(ph.fluid_tag), wall_bit = tag_bit(ph.wall_tag);
+        const bool fluid_i = tbi == fluid_bit, wall_i = tbi == wall_bit;
+        const bool check = w.odd_tags || (adj_i & w.tags_present) != w.tags_present;
+        float drho = ACTION == EPPX_SEARCH_KERNEL_CONTINUITY ? f.drho[i] : 0.0f;
+        walk3_any(w, i, gi, hi, adj_i, check, [&](int j, float4 gj, float dx, float dy, float dz, float dr2) {
+            ++cnt;
+            if (ACTION == EPPX_SEARCH_KERNEL_CONTINUITY) {
+                const uint32_t tbj = tag_bit(__float_as_int(gj.w));
+                const bool classic = fluid_i && tbj == fluid_bit;
+                const bool balanced = (fluid_i && tbj == wall_bit) || (wall_i && tbj == fluid_bit);
+                if (classic || balanced) {
+                    const float4 vj = ld_slot(w.vel, j);
+                    const float4 qj = ld_slot(w.hmvp, j);
+                    const float hinv = w.hinv_u > 0.0f ? w.hinv_u : invharmonic(hi, qj.x);
+                    const float coef = classic ? qj.y : vi.w * qj.z;
+                    drho += continuity3<FAST, K>(hinv, vi, vj, coef, dx, dy, dz, dr2);
+                }
+            }
+        });
+        ncount[i] = cnt;
+        if (cnt > M) atomicOr(flags, kFlagNeighbourOverflow);
+        if (ACTION == EPPX_SEARCH_KERNEL_CONTINUITY) f.drho[i] = drho;
+    }
+    const int total = block_sum_i32(cnt, lds);
+    if (threadIdx.x == 0 && total) {
+        atomicAdd(&pairs[0], (unsigned long long)total);
+        atomicAdd(&pairs[1], (unsigned long long)total);
+    }
+}
+
+// Momentum pass (iMomentum!, main.jl:107-118): fluid particles add the
+// pressure and viscosity terms of their fluid and wall neighbours to du;
+// other particles take no terms.
+template <bool FAST, int K, int VISC>
+__global__ __launch_bounds__(kBlock3) void k3_momentum(Walk3 w, Fields3 f, Physics ph) {
+    const int i = xcd_block(blockIdx.x, gridDim.x) * kBlock3 + threadIdx.x;
+    if (i >= w.n_alive[0]) return;
+    const float4 gi = w.geo[i];
+    const uint32_t tbi = tag_bit(__float_as_int(gi.w));
+    const uint32_t fluid_bit = tag_bit(ph.fluid_tag), wall_bit = tag_bit(ph.wall_tag);
+    if (!tbi || tbi != fluid_bit) return;
+    const float4 vi = w.vel[i];
+    const float4 qi = w.hmvp[i];
+    const float p_i = qi.w;
+    const float pr_i = FAST ? p_i * frcp(vi.w * vi.w) : p_i / (vi.w * vi.w);
+    const uint32_t adj_i = adj3(w, tbi);
+    const bool check = w.odd_tags || (adj_i & w.tags_present) != w.tags_present;
+    float4 acc = f.dvel[i];
+    walk3_any(w, i, gi, qi.x, adj_i, check, [&](int j, float4 gj, float dx, float dy, float dz, float dr2) {
+        const uint32_t tbj = tag_bit(__float_as_int(gj.w));
+        const bool classic = tbj == fluid_bit;
+        if (!classic && tbj != wall_bit) return;
+        const float4 vj = ld_slot(w.vel, j);
+        const float4 qj = ld_slot(w.hmvp, j);
+        const float hinv = w.hinv_u > 0.0f ? w.hinv_u : invharmonic(qi.x, qj.x);
+        momentum3<FAST, K, VISC>(ph, acc, classic, hinv, vi, p_i, pr_i, vj, qj, dx, dy, dz, dr2);
+    });
+    f.dvel[i] = acc;
+}
+
+// Neighbour lists of the last search as a CSR by slot (row_start) and / or
+// in the reference's per-particle neighbour columns (nCount, nIndex, nRVec
+// with 3 floats per pair, nR, nW, nDW, nHInv; NeighbourSearch.jl:215-220,
+// Library/Kernel.jl:123-126).
+__global__ __launch_bounds__(kBlock3) void k3_export(Walk3 w, Fields3 f, const int64_t *row_start, int32_t *js,
+                                                      const int32_t *ncount, NeighbourExport ex) {
+    const int i = blockIdx.x * kBlock3 + threadIdx.x;
+    if (i >= w.n_alive[0]) return;
+    const int row = f.id[i];
+    const float4 gi = w.geo[i];
+    const float hi = w.hmvp[i].x;
+    const int ti = __float_as_int(gi.w);
+    const uint32_t tbi = tag_bit(ti), adj_i = adj3(w, tbi);
+    const uint32_t fluid_bit = tag_bit(ex.fluid_tag), wall_bit = tag_bit(ex.wall_tag);
+    const size_t cap = (size_t)ex.n_capacity;
+    const int64_t o = row_start ? row_start[i] : 0;
+    const int limit = js ? ncount[i] : 0;
+    int k = 0;
+    walk3<true, true>(w, i, gi, hi, adj_i, [&](int j, float4 gj, float dx, float dy, float dz, float dr2) {
+        if (js && k < limit) js[o + k] = f.id[j];
+        if (k < ex.M) {
+            if (ex.nIndex) ex.nIndex[row + cap * k] = f.id[j] + 1;
+            if (ex.nRVec) {
+                ex.nRVec[row + cap * (3 * k)] = dx;
+                ex.nRVec[row + cap * (3 * k + 1)] = dy;
+                ex.nRVec[row + cap * (3 * k + 2)] = dz;
+            }
+            const float r = sqrtf(dr2);
+            if (ex.nR) ex.nR[row + cap * k] = r;
+            const uint32_t tbj = tag_bit(__float_as_int(gj.w));
+            const bool kernel_pair = (tbi == fluid_bit && (tbj == fluid_bit || tbj == wall_bit)) ||
+                                     (tbi == wall_bit && tbj == fluid_bit);
+            if (ex.with_kernel && kernel_pair && tbi) {
+                const float hinv = invharmonic(hi, ld_slot(w.hmvp, j).x);
+                if (ex.nHInv) ex.nHInv[row + cap * k] = hinv;
+                if (ex.nW) ex.nW[row + cap * k] = value3_exact(ex.kernel, r, hinv);
+                if (ex.nDW)
+                    ex.nDW[row + cap * k] = ex.kernel == EPPX_KERNEL_CUBIC ? grad3_exact<EPPX_KERNEL_CUBIC>(r, hinv)
+                                                                         : grad3_exact<EPPX_KERNEL_WENDLAND_C2>(r, hinv);
+            }
+        }
+        ++k;
+    });
+    if (ex.nCount) ex.nCount[row] = k;
+}
+
+// ------------------------------------------------------------- cell list
+// device_insertParticlesIntoCells! (NeighbourSearch.jl:12-47) in 3D: the cell
+// of each alive particle (cached for immovable ones), death outside the
+// domain (inclusive bounds); key = 0-based cell, n_cells for dead particles.
+__global__ __launch_bounds__(kBlock3) void k3_key_count(Fields3 f, int32_t n, Grid3 g, int32_t *counts,
+                                                         int32_t *key, int32_t *tmpslot, uint32_t *flags,
+                                                         int32_t max_per_cell) {
+    const int32_t s = blockIdx.x * kBlock3 + threadIdx.x;
+    const bool valid = s < n;
+    int32_t k = g.n_cells;
+    if (valid && f.alive[s] == 1) {
+        int32_t c = f.cell[s];
+        if (c < 0 || c > g.n_cells) c = -1;  // a cached cell outside this grid is recomputed
+        if (!(f.mov[s] == 0 && c > 0)) {
+            const float4 x = f.pos[s];
+            if (g.first_x <= x.x && x.x <= g.last_x && g.first_y <= x.y && x.y <= g.last_y && g.first_z <= x.z &&
+                x.z <= g.last_z) {
+                int32_t ci = (int32_t)floorf((x.x - g.first_x) * g.gap_x_inv) + 1;
+                ci = max(1, min(g.n_x, ci));
+                int32_t cj = (int32_t)floorf((x.y - g.first_y) * g.gap_y_inv) + 1;
+                cj = max(1, min(g.n_y, cj));
+                int32_t ck = (int32_t)floorf((x.z - g.first_z) * g.gap_z_inv) + 1;
+                ck = max(1, min(g.n_z, ck));
+                c = ci + g.n_x * ((cj - 1) + g.n_y * (ck - 1));
+                f.cell[s] = c;
+            } else {
+                f.alive[s] = 0;
+                f.cell[s] = 0;
+                c = 0;
+            }
+        }
+        if (c > 0) k = c - 1;
+    }
+    if (valid) key[s] = k;
+    const int32_t slot = count_key(valid, k, g.n_cells, counts, flags, max_per_cell);
+    if (valid) tmpslot[s] = slot;
+}
+
+__global__ __launch_bounds__(kBlock3) void k3_fill_members(Fields3 f, int32_t n, int32_t n_cells, const int32_t *key,
+                                                            const int32_t *tmpslot, const int32_t *start,
+                                                            int32_t *members) {
+    const int32_t s = blockIdx.x * kBlock3 + threadIdx.x;
+    if (s >= n) return;
+    const int32_t k = key[s];
+    if (k < n_cells) members[start[k] + tmpslot[s]] = f.id[s];
+}
+
+// Final place of each slot: cell start + number of cell members with a
+// smaller particle index (the order of a serial insert); dead particles
+// follow the alive ones.  Moves every field and writes geo = (x, y, z, tag).
+__global__ __launch_bounds__(kBlock3) void k3_permute(Fields3 cur, Fields3 nxt, float4 *geo, int32_t n,
+                                                       int32_t n_cells, const int32_t *key, const int32_t *tmpslot,
+                                                       const int32_t *start, const int32_t *members,
+                                                       int32_t *n_alive, uint32_t *flags) {
+    const int32_t s = blockIdx.x * kBlock3 + threadIdx.x;
+    if (s == 0) {
+        n_alive[0] = start[n_cells];
+        n_alive[1] = start[n_cells + 1];
+    }
+    if (s >= n) return;
+    const int32_t k = key[s];
+    const int32_t me = cur.id[s];
+    const int32_t tmp = tmpslot[s];
+    const float4 x = cur.pos[s], v = cur.vel[s], q = cur.hmvp[s], d = cur.dvel[s];
+    const float drho = cur.drho[s];
+    const int32_t mov = cur.mov[s], cell_in = cur.cell[s], alive_in = cur.alive[s];
+    const int32_t b = start[k];
+    int32_t dest, alive, cell;
+    if (k < n_cells) {
+        const int32_t e = start[k + 1];
+        if (e - b <= kRankLimit) {
+            int32_t r0 = 0, r1 = 0, r2 = 0, r3 = 0, p = b;
+            for (; p + 4 <= e; p += 4) {
+                const int32_t a0 = members[p], a1 = members[p + 1], a2 = members[p + 2], a3 = members[p + 3];
+                r0 += a0 < me;
+                r1 += a1 < me;
+                r2 += a2 < me;
+                r3 += a3 < me;
+            }
+            for (; p < e; ++p) r0 += members[p] < me;
+            dest = b + r0 + r1 + r2 + r3;
+        } else {
+            dest = b + tmp;
+            atomicOr(flags, kFlagRankFallback);
+        }
+        alive = 1;
+        cell = k + 1;
+    } else {
+        dest = b + tmp;
+        alive = alive_in == 1 ? 0 : alive_in;
+        cell = cell_in;
+    }
+    nxt.pos[dest] = x;
+    geo[dest] = x;
+    nxt.vel[dest] = v;
+    nxt.hmvp[dest] = q;
+    nxt.dvel[dest] = d;
+    nxt.drho[dest] = drho;
+    nxt.mov[dest] = mov;
+    nxt.id[dest] = me;
+    nxt.cell[dest] = cell;
+    nxt.alive[dest] = alive;
+}
+
+// Tag bits of each cell's (alive, sorted) particles.
+__global__ __launch_bounds__(kBlock3) void k3_cell_tags(const float4 *geo, int32_t n_cells, const int32_t *start,
+                                                         uint32_t *cell_tags) {
+    const int32_t c = blockIdx.x * kBlock3 + threadIdx.x;
+    if (c >= n_cells) return;
+    uint32_t bits = 0u;
+    const int32_t e = start[c + 1];
+    for (int32_t s = start[c]; s < e; ++s) bits |= tag_bit(__float_as_int(geo[s].w));
+    cell_tags[c] = bits;
+}
+
+// ---------------------------------------------------------- self kernels
+// EPPX_SELF_CONTINUITY_EOS: sContinuity! (main.jl:92-105; Continuity.jl:10-21,
+// sVolume! State.jl:10-19) on fluid and wall particles; EPPX_SELF_GRAVITY_MOVE:
+// sMomentum! (main.jl:120-126; the 3D sGravity! State.jl:49-63,
+// sAccelerateMove! Motion.jl:61-75) on fluid particles.
+__global__ __launch_bounds__(kBlock3) void k3_self(Fields3 f, int32_t n, int32_t action, Physics3 P) {
+    const int32_t s = blockIdx.x * kBlock3 + threadIdx.x;
+    if (s >= n || f.alive[s] != 1) return;
+    const Physics &ph = P.p;
+    float4 x = f.pos[s];
+    const int32_t tag = __float_as_int(x.w);
+    if (action == EPPX_SELF_CONTINUITY_EOS) {
+        if (tag != ph.fluid_tag && tag != ph.wall_tag) return;
+        float4 v = f.vel[s], q = f.hmvp[s];
+        v.w += f.drho[s] * ph.dt;
+        f.drho[s] = 0.0f;
+        q.z = q.y / v.w;
+        q.w = eos(ph, v.w);
+        f.vel[s] = v;
+        f.hmvp[s] = q;
+    } else if (action == EPPX_SELF_GRAVITY_MOVE) {
+        if (tag != ph.fluid_tag) return;
+        float4 v = f.vel[s], d = f.dvel[s];
+        d.x += ph.gx;
+        d.y += ph.gy;
+        d.z += P.gz;
+        x.x += (v.x + d.x * ph.dt * 0.5f) * ph.dt;
+        v.x += d.x * ph.dt;
+        x.y += (v.y + d.y * ph.dt * 0.5f) * ph.dt;
+        v.y += d.y * ph.dt;
+        x.z += (v.z + d.z * ph.dt * 0.5f) * ph.dt;
+        v.z += d.z * ph.dt;
+        f.pos[s] = x;
+        f.vel[s] = v;
+        f.dvel[s] = make_float4(0.0f, 0.0f, 0.0f, d.w);
+    }
+}
+
+// ------------------------------------------------------- layout conversion
+__global__ __launch_bounds__(kBlock3) void k3_from_ref(Fields3 f, Columns c, const int32_t *ip, const float *fp,
+                                                        const int32_t *is_alive, const int32_t *cell_index) {
+    const int row = blockIdx.x * kBlock3 + threadIdx.x;
+    if (row >= c.n_capacity) return;
+    const size_t C = (size_t)c.n_capacity;
+    auto fcol = [&](int col, int k) { return col >= 0 ? fp[row + C * (col + k)] : 0.0f; };
+    auto icol = [&](int col) { return col >= 0 ? ip[row + C * col] : 0; };
+    f.pos[row] = make_float4(fcol(c.PositionVec, 0), fcol(c.PositionVec, 1), fcol(c.PositionVec, 2),
+                             __int_as_float(icol(c.Tag)));
+    f.vel[row] = make_float4(fcol(c.VelocityVec, 0), fcol(c.VelocityVec, 1), fcol(c.VelocityVec, 2),
+                             fcol(c.Density, 0));
+    f.hmvp[row] = make_float4(fcol(c.H, 0), fcol(c.Mass, 0), fcol(c.Volume, 0), fcol(c.Pressure, 0));
+    f.dvel[row] = make_float4(fcol(c.dVelocityVec, 0), fcol(c.dVelocityVec, 1), fcol(c.dVelocityVec, 2), 0.0f);
+    f.drho[row] = fcol(c.dDensity, 0);
+    f.mov[row] = icol(c.IsMovable);
+    f.id[row] = row;
+    f.alive[row] = is_alive[row];
+    f.cell[row] = cell_index[row];
+}
+
+__global__ __launch_bounds__(kBlock3) void k3_to_ref(Fields3 f, int32_t n, Columns c, int32_t *ip, float *fp,
+                                                      int32_t *is_alive, int32_t *cell_index) {
+    const int s = blockIdx.x * kBlock3 + threadIdx.x;
+    if (s >= n) return;
+    const size_t C = (size_t)c.n_capacity;
+    const int row = f.id[s];
+    auto put = [&](int col, int k, float v) {
+        if (col >= 0) fp[row + C * (col + k)] = v;
+    };
+    const float4 x = f.pos[s], v = f.vel[s], q = f.hmvp[s], d = f.dvel[s];
+    put(c.PositionVec, 0, x.x);
+    put(c.PositionVec, 1, x.y);
+    put(c.PositionVec, 2, x.z);
+    put(c.VelocityVec, 0, v.x);
+    put(c.VelocityVec, 1, v.y);
+    put(c.VelocityVec, 2, v.z);
+    put(c.dVelocityVec, 0, d.x);
+    put(c.dVelocityVec, 1, d.y);
+    put(c.dVelocityVec, 2, d.z);
+    put(c.Density, 0, v.w);
+    put(c.H, 0, q.x);
+    put(c.Mass, 0, q.y);
+    put(c.Volume, 0, q.z);
+    put(c.Pressure, 0, q.w);
+    put(c.dDensity, 0, f.drho[s]);
+    if (c.Tag >= 0) ip[row + C * c.Tag] = __float_as_int(x.w);
+    if (c.IsMovable >= 0) ip[row + C * c.IsMovable] = f.mov[s];
+    is_alive[row] = f.alive[s];
+    cell_index[row] = f.cell[s];
+}
+
+inline dim3 grid3_for(int64_t n) {
+    const int64_t g = (n + kBlock3 - 1) / kBlock3;
+    return dim3((unsigned)(g > 0 ? g : 1));
+}
+
+template <bool FAST, int K>
+hipError_t search_as(const Walk3 &w, Fields3 f, int32_t action, const Physics &ph, int32_t M, int32_t *ncount,
+                     unsigned long long *pairs, uint32_t *flags, int32_t n_slots, hipStream_t s) {
+    if (action == EPPX_SEARCH_KERNEL_CONTINUITY)
+        hipLaunchKernelGGL((k3_search<FAST, EPPX_SEARCH_KERNEL_CONTINUITY, K>), grid3_for(n_slots), dim3(kBlock3), 0,
+                           s, w, f, ph, M, ncount, pairs, flags);
+    else
+        hipLaunchKernelGGL((k3_search<false, EPPX_SEARCH_NONE, EPPX_KERNEL_CUBIC>), grid3_for(n_slots),
+                           dim3(kBlock3), 0, s, w, f, ph, M, ncount, pairs, flags);
+    return hipGetLastError();
+}
+
+template <bool FAST, int K>
+hipError_t momentum_as(const Walk3 &w, Fields3 f, const Physics &ph, int32_t n_slots, hipStream_t s) {
+    if (ph.visc == EPPX_VISC_CLASSIC)
+        hipLaunchKernelGGL((k3_momentum<FAST, K, EPPX_VISC_CLASSIC>), grid3_for(n_slots), dim3(kBlock3), 0, s, w, f,
+                           ph);
+    else
+        hipLaunchKernelGGL((k3_momentum<FAST, K, EPPX_VISC_ARTIFICIAL>), grid3_for(n_slots), dim3(kBlock3), 0, s, w,
+                           f, ph);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch3_from_ref(Fields3 f, Columns c, const int32_t *ip, const float *fp, const int32_t *is_alive,
+                            const int32_t *cell_index, hipStream_t s) {
+    hipLaunchKernelGGL(k3_from_ref, grid3_for(c.n_capacity), dim3(kBlock3), 0, s, f, c, ip, fp, is_alive,
+                       cell_index);
+    return hipGetLastError();
+}
+
+hipError_t launch3_to_ref(Fields3 f, int32_t n, Columns c, int32_t *ip, float *fp, int32_t *is_alive,
+                          int32_t *cell_index, hipStream_t s) {
+    hipLaunchKernelGGL(k3_to_ref, grid3_for(n), dim3(kBlock3), 0, s, f, n, c, ip, fp, is_alive, cell_index);
+    return hipGetLastError();
+}
+
+hipError_t launch3_key_count(Fields3 f, int32_t n, Grid3 g, int32_t *counts, int32_t *key, int32_t *tmpslot,
+                             uint32_t *flags, int32_t max_per_cell, hipStream_t s) {
+    hipLaunchKernelGGL(k3_key_count, grid3_for(n), dim3(kBlock3), 0, s, f, n, g, counts, key, tmpslot, flags,
+                       max_per_cell);
+    return hipGetLastError();
+}
+
+hipError_t launch3_fill_members(Fields3 f, int32_t n, int32_t n_cells, const int32_t *key, const int32_t *tmpslot,
+                                const int32_t *start, int32_t *members, hipStream_t s) {
+    hipLaunchKernelGGL(k3_fill_members, grid3_for(n), dim3(kBlock3), 0, s, f, n, n_cells, key, tmpslot, start,
+                       members);
+    return hipGetLastError();
+}
+
+hipError_t launch3_permute(Fields3 cur, Fields3 nxt, float4 *geo, int32_t n, int32_t n_cells, const int32_t *key,
+                           const int32_t *tmpslot, const int32_t *start, const int32_t *members, int32_t *n_alive,
+                           uint32_t *flags, hipStream_t s) {
+    hipLaunchKernelGGL(k3_permute, grid3_for(n), dim3(kBlock3), 0, s, cur, nxt, geo, n, n_cells, key, tmpslot, start,
+                       members, n_alive, flags);
+    return hipGetLastError();
+}
+
+hipError_t launch3_cell_tags(const float4 *geo, int32_t n_cells, const int32_t *start, uint32_t *cell_tags,
+                             hipStream_t s) {
+    hipLaunchKernelGGL(k3_cell_tags, grid3_for(n_cells), dim3(kBlock3), 0, s, geo, n_cells, start, cell_tags);
+    return hipGetLastError();
+}
+
+hipError_t launch3_search(const Walk3 &w, Fields3 f, int32_t action, const Physics &ph, int32_t fast, int32_t M,
+                          int32_t *ncount, unsigned long long *pairs, uint32_t *flags, int32_t n_slots,
+                          hipStream_t s) {
+    const bool cubic = ph.kernel == EPPX_KERNEL_CUBIC;
+    if (fast)
+        return cubic ? search_as<true, EPPX_KERNEL_CUBIC>(w, f, action, ph, M, ncount, pairs, flags, n_slots, s)
+                     : search_as<true, EPPX_KERNEL_WENDLAND_C2>(w, f, action, ph, M, ncount, pairs, flags, n_slots, s);
+    return cubic ? search_as<false, EPPX_KERNEL_CUBIC>(w, f, action, ph, M, ncount, pairs, flags, n_slots, s)
+                 : search_as<false, EPPX_KERNEL_WENDLAND_C2>(w, f, action, ph, M, ncount, pairs, flags, n_slots, s);
+}
+
+hipError_t launch3_momentum(const Walk3 &w, Fields3 f, const Physics &ph, int32_t fast, int32_t n_slots,
+                            hipStream_t s) {
+    const bool cubic = ph.kernel == EPPX_KERNEL_CUBIC;
+    if (fast)
+        return cubic ? momentum_as<true, EPPX_KERNEL_CUBIC>(w, f, ph, n_slots, s)
+                     : momentum_as<true, EPPX_KERNEL_WENDLAND_C2>(w, f, ph, n_slots, s);
+    return cubic ? momentum_as<false, EPPX_KERNEL_CUBIC>(w, f, ph, n_slots, s)
+                 : momentum_as<false, EPPX_KERNEL_WENDLAND_C2>(w, f, ph, n_slots, s);
+}
+
+hipError_t launch3_self(Fields3 f, int32_t n, int32_t action, const Physics3 &ph, hipStream_t s) {
+    hipLaunchKernelGGL(k3_self, grid3_for(n), dim3(kBlock3), 0, s, f, n, action, ph);
+    return hipGetLastError();
+}
+
+hipError_t launch3_export(const Walk3 &w, Fields3 f, int32_t n_slots, const int64_t *row_start, int32_t *js,
+                          const int32_t *ncount, NeighbourExport3 ex, hipStream_t s) {
+    hipLaunchKernelGGL(k3_export, grid3_for(n_slots), dim3(kBlock3), 0, s, w, f, row_start, js, ncount, ex);
+    return hipGetLastError();
+}
+
+}  // namespace eppx

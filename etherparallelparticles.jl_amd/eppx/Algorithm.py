@@ -55,8 +55,9 @@ def search(particle_system: ParticleSystem, domain: Domain2D, neighbour_system: 
     act = _action(action, "search")
     blk, size = act.params(particle_system)
     ctx = particle_system.parallel.ctx
-    _lib.check(_lib.lib().eppx_search(particle_system.handle, neighbour_system.handle, _criterion(criterion), act.id,
-                                      C.byref(blk), size), ctx, "search")
+    fn = _lib.lib().eppx_search3d if particle_system.dim == 3 else _lib.lib().eppx_search
+    _lib.check(fn(particle_system.handle, neighbour_system.handle, _criterion(criterion), act.id, C.byref(blk), size),
+               ctx, "search")
     particle_system.last_ns = neighbour_system
     if sync:
         particle_system.parallel.synchronize()
@@ -71,7 +72,8 @@ def selfaction(particle_system: ParticleSystem, action, *, n_threads: int = kDef
     act = _action(action, "self")
     blk, size = act.params(particle_system)
     ctx = particle_system.parallel.ctx
-    _lib.check(_lib.lib().eppx_selfaction(particle_system.handle, act.id, C.byref(blk), size), ctx, "selfaction")
+    fn = _lib.lib().eppx_selfaction3d if particle_system.dim == 3 else _lib.lib().eppx_selfaction
+    _lib.check(fn(particle_system.handle, act.id, C.byref(blk), size), ctx, "selfaction")
     if sync:
         particle_system.parallel.synchronize()
 
@@ -88,8 +90,9 @@ def interaction(particle_system: ParticleSystem, action, *, n_threads: int = kDe
         raise _lib.EppxError(_lib.EPPX_E_STATE, "interaction before any search")
     blk, size = act.params(particle_system)
     ctx = particle_system.parallel.ctx
-    _lib.check(_lib.lib().eppx_interaction(particle_system.handle, particle_system.last_ns.handle, act.id,
-                                           C.byref(blk), size), ctx, "interaction")
+    fn = _lib.lib().eppx_interaction3d if particle_system.dim == 3 else _lib.lib().eppx_interaction
+    _lib.check(fn(particle_system.handle, particle_system.last_ns.handle, act.id, C.byref(blk), size), ctx,
+               "interaction")
     if sync:
         particle_system.parallel.synchronize()
 
@@ -101,8 +104,17 @@ dynamic_interaction = interaction
 def step(particle_system: ParticleSystem, neighbour_system: NeighbourSystem, physics: SPH.Physics, n_steps: int = 1,
          filter_interval: int = 0, step_offset: int = 0, sync: bool = True):
     """n_steps of the dam-break loop body (main.jl:166-183) in one call."""
-    p = physics.c()
     ctx = particle_system.parallel.ctx
+    if particle_system.dim == 3:
+        if filter_interval:
+            raise NotImplementedError("the Shepard filter is not part of the 3D path")
+        p3 = physics.c3()
+        _lib.check(_lib.lib().eppx_step3d(particle_system.handle, neighbour_system.handle, C.byref(p3), int(n_steps)),
+                   ctx, "step")
+        if sync:
+            particle_system.parallel.synchronize()
+        return
+    p = physics.c()
     _lib.check(_lib.lib().eppx_step(particle_system.handle, neighbour_system.handle, C.byref(p), int(n_steps),
                                     int(filter_interval), int(step_offset)), ctx, "step")
     if sync:
@@ -124,6 +136,8 @@ def autotune(particle_system: ParticleSystem, neighbour_system: NeighbourSystem,
     """Picks the pair-kernel shapes of the search action and the momentum pass (256 or 512 threads per block;
     per-neighbour data gathered from HBM or staged in LDS) by timing each on the current state, which it leaves
     unchanged; FAST math only.  Returns the choice and the timings in microseconds (keys 256g, 512g, 256s, 512s)."""
+    if particle_system.dim == 3:
+        raise NotImplementedError("the 3D pair kernels have one shape")
     p = physics.c()
     choice = (C.c_int32 * 4)()
     us = (C.c_double * 8)()

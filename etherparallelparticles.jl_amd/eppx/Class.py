@@ -73,6 +73,54 @@ class Domain2D:
                 f"first=({self.c.first_x}, {self.c.first_y}), last=({self.c.last_x}, {self.c.last_y}))")
 
 
+class Domain3D:
+    """Domain3D{Int32, Float32}(gap, first_x, first_y, first_z, last_x, last_y, last_z): the reference's
+    Domain2D (Domain2D.jl:28-67) with a z axis (its 3D domain is a stub, AbstractDomain.jl:119).  Cell
+    (i, j, k) is linear i + n_x (j - 1) + n_x n_y (k - 1), 1-based.  Fields computed by eppx_domain3d_init."""
+
+    def __init__(self, gap, first_x, first_y, first_z, last_x, last_y, last_z):
+        self.args = tuple(float(v) for v in (gap, first_x, first_y, first_z, last_x, last_y, last_z))
+        self.c = _lib.Domain3D()
+        _lib.check(_lib.lib().eppx_domain3d_init(*self.args, C.byref(self.c)), None, "Domain3D")
+
+    def __getattr__(self, name):
+        if name.endswith("_") and name[:-1] in dict(_lib.Domain3D._fields_):
+            return getattr(self.c, name[:-1])
+        raise AttributeError(name)
+
+    def dimension(self):
+        return 3
+
+    def get_gap(self): return np.float32(self.c.gap)
+    def get_n_x(self): return self.c.n_x
+    def get_n_y(self): return self.c.n_y
+    def get_n_z(self): return self.c.n_z
+    def get_n(self): return self.c.n
+
+    def _locate(self, x, y, z):
+        ins, i, j, k, lin = (C.c_int32() for _ in range(5))
+        _lib.check(_lib.lib().eppx_domain3d_locate(C.byref(self.c), float(np.float32(x)), float(np.float32(y)),
+                                                  float(np.float32(z)), C.byref(ins), C.byref(i), C.byref(j),
+                                                  C.byref(k), C.byref(lin)))
+        return bool(ins.value), i.value, j.value, k.value, lin.value
+
+    def indexCartesianToLinear(self, i, j, k):
+        return int(i) + self.c.n_x * ((int(j) - 1) + self.c.n_y * (int(k) - 1))
+
+    def indexLinearToCartesian(self, index):
+        c = int(index) - 1
+        return c % self.c.n_x + 1, (c // self.c.n_x) % self.c.n_y + 1, c // (self.c.n_x * self.c.n_y) + 1
+
+    def inside(self, x, y, z): return self._locate(x, y, z)[0]
+    def indexCartesianFromPosition(self, x, y, z): return self._locate(x, y, z)[1:4]
+    def indexLinearFromPosition(self, x, y, z): return self._locate(x, y, z)[4]
+
+    def __repr__(self):
+        c = self.c
+        return (f"Domain3D{{Int32, Float32}}(gap={c.gap}, n=({c.n_x}, {c.n_y}, {c.n_z}), "
+                f"first=({c.first_x}, {c.first_y}, {c.first_z}), last=({c.last_x}, {c.last_y}, {c.last_z}))")
+
+
 # -------------------------------------------------------------- named index
 
 
@@ -262,10 +310,12 @@ class ParticleSystem(_ParticleSystemBase):
             self.layout.max_neighbour_number = int(max_neighbour_number)
         self.parallel = parallel
         self.domain = domain
+        self.dim = domain.dimension() if domain is not None else 2
         self.last_ns = None
         h = C.c_void_p()
-        _lib.check(_lib.lib().eppx_ps_create(parallel.ctx, self.n_capacity, C.byref(self.layout), C.byref(h)),
-                   parallel.ctx, "ParticleSystem")
+        create = _lib.lib().eppx_ps3d_create if self.dim == 3 else _lib.lib().eppx_ps_create
+        _lib.check(create(parallel.ctx, self.n_capacity, C.byref(self.layout), C.byref(h)), parallel.ctx,
+                   "ParticleSystem")
         self.handle = h
         _lib.track(self)
         # ParticleSystem.jl:74-77: the first n_particles rows start alive
@@ -275,7 +325,7 @@ class ParticleSystem(_ParticleSystemBase):
 
     def close(self):
         if getattr(self, "handle", None) is not None and not _lib.shutting_down():
-            _lib.lib().eppx_ps_destroy(self.handle)
+            (_lib.lib().eppx_ps3d_destroy if self.dim == 3 else _lib.lib().eppx_ps_destroy)(self.handle)
         self.handle = None
 
     def __del__(self):
@@ -286,7 +336,8 @@ class ParticleSystem(_ParticleSystemBase):
 
     def get_alive_n_particles(self):
         out = C.c_int32()
-        _lib.check(_lib.lib().eppx_ps_alive_count(self.handle, C.byref(out)), self.parallel.ctx, "alive count")
+        fn = _lib.lib().eppx_ps3d_alive_count if self.dim == 3 else _lib.lib().eppx_ps_alive_count
+        _lib.check(fn(self.handle, C.byref(out)), self.parallel.ctx, "alive count")
         return out.value
 
     def set_n_particles(self, n):
@@ -318,8 +369,9 @@ def syncto(destination, source) -> None:
                 fp.shape != (destination.n_capacity, destination.get_n_float_capacity()):
             raise ValueError("host and device particle systems differ in shape")
         L = _lib.lib()
-        _lib.check(L.eppx_ps_upload(destination.handle, destination.n_particles_, _lib.ptr(alive, C.c_int32),
-                                    _lib.ptr(cell, C.c_int32), _lib.ptr(ip, C.c_int32), _lib.ptr(fp, C.c_float)),
+        upload = L.eppx_ps3d_upload if destination.dim == 3 else L.eppx_ps_upload
+        _lib.check(upload(destination.handle, destination.n_particles_, _lib.ptr(alive, C.c_int32),
+                          _lib.ptr(cell, C.c_int32), _lib.ptr(ip, C.c_int32), _lib.ptr(fp, C.c_float)),
                    destination.parallel.ctx, "upload")
         destination._uploaded = True
         destination.last_ns = None
@@ -327,7 +379,8 @@ def syncto(destination, source) -> None:
         n = C.c_int32()
         ns = source.last_ns.handle if source.last_ns is not None else None
         L = _lib.lib()
-        st = L.eppx_ps_download(source.handle, ns, C.byref(n), _lib.ptr(destination.is_alive_, C.c_int32),
+        download = L.eppx_ps3d_download if source.dim == 3 else L.eppx_ps_download
+        st = download(source.handle, ns, C.byref(n), _lib.ptr(destination.is_alive_, C.c_int32),
                                 _lib.ptr(destination.cell_index_, C.c_int32),
                                 _lib.ptr(destination.int_properties_, C.c_int32),
                                 _lib.ptr(destination.float_properties_, C.c_float))
@@ -373,19 +426,20 @@ class NeighbourSystem:
         if periodic_boundary_policy is not NonePeriodicBoundaryPolicy:
             raise NotImplementedError("periodic boundaries are unfinished in the reference (PeriodicBoundary.jl:57)")
         self.parallel, self.domain = parallel, domain
+        self.dim = domain.dimension()
         self.active_pair_ = ActivePair(active_pair)
         self.max_neighbour_number = int(max_neighbour_number)
         adj = np.asfortranarray(self.active_pair_.adjacency_matrix_)
         h = C.c_void_p()
-        _lib.check(_lib.lib().eppx_ns_create(parallel.ctx, C.byref(domain.c), _lib.ptr(adj, C.c_int32),
-                                             adj.shape[0], self.max_neighbour_number, C.byref(h)),
-                   parallel.ctx, "NeighbourSystem")
+        create = _lib.lib().eppx_ns3d_create if self.dim == 3 else _lib.lib().eppx_ns_create
+        _lib.check(create(parallel.ctx, C.byref(domain.c), _lib.ptr(adj, C.c_int32), adj.shape[0],
+                          self.max_neighbour_number, C.byref(h)), parallel.ctx, "NeighbourSystem")
         self.handle = h
         _lib.track(self)
 
     def close(self):
         if getattr(self, "handle", None) is not None and not _lib.shutting_down():
-            _lib.lib().eppx_ns_destroy(self.handle)
+            (_lib.lib().eppx_ns3d_destroy if self.dim == 3 else _lib.lib().eppx_ns_destroy)(self.handle)
         self.handle = None
 
     def __del__(self):
@@ -401,9 +455,10 @@ class NeighbourSystem:
         """(count, list) of device_initializeNeighbourSystem! (NeighbourSystemBase.jl:28-48)."""
         n = self.domain.get_n()
         count = np.zeros(n, np.int32)
-        lst = np.zeros((n, 9), np.int32, order="F")
-        _lib.check(_lib.lib().eppx_ns_cell_stencil(self.handle, _lib.ptr(count, C.c_int32), _lib.ptr(lst, C.c_int32)),
-                   self.parallel.ctx, "cell stencil")
+        lst = np.zeros((n, 27 if self.dim == 3 else 9), np.int32, order="F")
+        fn = _lib.lib().eppx_ns3d_cell_stencil if self.dim == 3 else _lib.lib().eppx_ns_cell_stencil
+        _lib.check(fn(self.handle, _lib.ptr(count, C.c_int32), _lib.ptr(lst, C.c_int32)), self.parallel.ctx,
+                   "cell stencil")
         return count, lst
 
 

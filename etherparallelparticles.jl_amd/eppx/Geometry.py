@@ -54,3 +54,44 @@ def create(gap: float, rect: Rectangle):
     y = (np.float64(rect.first_y) + (i_y - 0.5) * np.float64(dy)).astype(np.float32)
     n = x.shape[0]
     return np.stack([x, y], axis=1), np.full(n, vol, np.float32), np.full(n, g, np.float32)
+
+
+class Cuboid:
+    """Cuboid{Float32}(x1, y1, z1, x2, y2, z2) (Geometry3D.jl:10-30): first corner and spans, as Rectangle."""
+
+    def __init__(self, x1, y1, z1, x2, y2, z2):
+        assert x1 < x2 and y1 < y2 and z1 < z2, "the first corner must lie below the last"
+        self.first_x, self.first_y, self.first_z = np.float32(x1), np.float32(y1), np.float32(z1)
+        self.span_x, self.span_y, self.span_z = np.float32(x2 - x1), np.float32(y2 - y1), np.float32(z2 - z1)
+
+
+def count3(gap: float, cub: Cuboid) -> int:
+    """Geometry.count for a Cuboid (Geometry3D.jl:74-79): the product of round(span / gap) per axis."""
+    return _n(cub.span_x, gap) * _n(cub.span_y, gap) * _n(cub.span_z, gap)
+
+
+def create3(gap: float, cub: Cuboid):
+    """A cuboid lattice: (positions (n, 3), volumes, gaps), Float32.
+
+    The reference's Cuboid create stops at the counts (Geometry3D.jl:89-94
+    "TODO"), so this lattice extends the rectangle's arithmetic per axis
+    (dx = span_x / n_x in Float32, x = FT(Float64(first_x) + (i_x - 0.5)
+    Float64(dx)), volume dx dy dz, gap the cube root of the volume) with
+    i_x = mod1(i, n_x), i_y = mod1(cld(i, n_x), n_y), i_z = cld(i, n_x n_y):
+    x fastest, and no wrapped last column.
+    """
+    n_x, n_y, n_z = _n(cub.span_x, gap), _n(cub.span_y, gap), _n(cub.span_z, gap)
+    dx = np.float32(cub.span_x / np.float32(n_x))
+    dy = np.float32(cub.span_y / np.float32(n_y))
+    dz = np.float32(cub.span_z / np.float32(n_z))
+    vol = np.float32(np.float32(dx * dy) * dz)
+    g = np.float32(np.float64(vol) ** (1.0 / 3.0))
+    i = np.arange(n_x * n_y * n_z, dtype=np.int64)
+    i_x = i % n_x + 1
+    i_y = (i // n_x) % n_y + 1
+    i_z = i // (n_x * n_y) + 1
+    x = (np.float64(cub.first_x) + (i_x - 0.5) * np.float64(dx)).astype(np.float32)
+    y = (np.float64(cub.first_y) + (i_y - 0.5) * np.float64(dy)).astype(np.float32)
+    z = (np.float64(cub.first_z) + (i_z - 0.5) * np.float64(dz)).astype(np.float32)
+    n = x.shape[0]
+    return np.stack([x, y, z], axis=1), np.full(n, vol, np.float32), np.full(n, g, np.float32)

@@ -50,6 +50,7 @@ class Physics:
     beta: float = 0.1
     av_c: float = 0.0
     coefficient: float = 0.0
+    gz: float = 0.0            # z component of gravity (3D systems; the 3D sGravity!, State.jl:49-63)
 
     def c(self) -> _lib.Physics:
         p = _lib.Physics()
@@ -59,8 +60,12 @@ class Physics:
             setattr(p, k, float(np.float32(getattr(self, k))))
         return p
 
+    def c3(self) -> _lib.Physics3D:
+        """The eppx_physics3d block of a 3D system."""
+        return _lib.Physics3D(self.c(), float(np.float32(self.gz)))
+
     def as_dict(self):
-        return {k: getattr(self, k) for k in _lib.PHYSICS_INT + _lib.PHYSICS_FLOAT}
+        return {k: getattr(self, k) for k in _lib.PHYSICS_INT + _lib.PHYSICS_FLOAT + ("gz",)}
 
 
 def dam_break_physics(dx: float, h_factor: float, kernel=WendlandC2, g: float = 9.81, column_height: float = 2.0,
@@ -73,6 +78,16 @@ def dam_break_physics(dx: float, h_factor: float, kernel=WendlandC2, g: float = 
     return Physics(kernel=kernel.id, eos=_lib.EOS_TAIT, visc=_lib.VISC_ARTIFICIAL, dt=float(np.float32(0.1 * h / c0)),
                    rho0=rho0, p0=0.0, c0=float(np.float32(c0)), tait_b=float(np.float32(rho0 * c0 * c0 / gamma)),
                    gx=0.0, gy=-g, alpha=alpha, beta=beta, av_c=float(np.float32(c0)))
+
+
+def dam_break_physics_3d(dx: float, h_factor: float = 2.0, kernel=WendlandC2, g: float = 9.81,
+                         column_height: float = 1.0, rho0: float = 1000.0, gamma: int = 7, alpha: float = 0.1,
+                         beta: float = 0.1) -> Physics:
+    """The 3D dam break: the constants of dam_break_physics for a column of height column_height, with
+    gravity along -z."""
+    p = dam_break_physics(dx, h_factor, kernel, g, column_height, rho0, gamma, alpha, beta)
+    p.gy, p.gz = 0.0, -g
+    return p
 
 
 def example_physics(dx: float, h_factor: float = 1.5, kernel=CubicSpline, c0: float = 120.0, g: float = 9.8,
@@ -103,8 +118,11 @@ class Action:
 
     def params(self, ps):
         if self.op is None:
-            blk = (self.physics or Physics()).c()
+            phys = self.physics or Physics()
+            blk = phys.c3() if getattr(ps, "dim", 2) == 3 else phys.c()
             return blk, C.sizeof(blk)
+        if getattr(ps, "dim", 2) == 3:
+            raise NotImplementedError("field actions are not part of the 3D path")
         col, comp, value = self.op
         blk = _lib.FieldOp(field_of(ps, col), int(comp), float(np.float32(value)))
         return blk, C.sizeof(blk)

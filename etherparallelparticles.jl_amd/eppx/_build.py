@@ -19,7 +19,7 @@ INCLUDE = os.path.join(REPO, "include")
 LIB = os.path.join(PKG, "libeppx.so")
 
 # (source, extra defines): one object each
-UNITS = [("api.hip", []), ("kernels.hip", [])] + [
+UNITS = [("api.hip", []), ("kernels.hip", []), ("api3d.hip", []), ("kernels3d.hip", [])] + [
     (src, [f"-DEPPX_TU_FAST={fast}", f"-DEPPX_TU_KERNEL={kernel}"])
     for src in ("search.hip", "momentum.hip") for fast in (1, 0) for kernel in (1, 0)
 ]

@@ -60,6 +60,20 @@ class Physics(C.Structure):
     _fields_ = [(n, C.c_int32) for n in PHYSICS_INT] + [(n, C.c_float) for n in PHYSICS_FLOAT]
 
 
+class Domain3D(C.Structure):
+    """eppx_domain3d: the Domain2D fields with a z axis."""
+    _fields_ = [(n, C.c_float) for n in ("gap", "gap_square")] + \
+               [(n, C.c_int32) for n in ("n_x", "n_y", "n_z", "n")] + \
+               [(n, C.c_float) for n in ("first_x", "last_x", "first_y", "last_y", "first_z", "last_z", "span_x",
+                                         "span_y", "span_z", "gap_x", "gap_y", "gap_z", "gap_x_inv", "gap_y_inv",
+                                         "gap_z_inv")]
+
+
+class Physics3D(C.Structure):
+    """eppx_physics3d: eppx_physics and the z component of gravity."""
+    _fields_ = [("base", Physics), ("gz", C.c_float)]
+
+
 class FieldOp(C.Structure):
     _fields_ = [("field", C.c_int32), ("component", C.c_int32), ("value", C.c_float)]
 
@@ -125,6 +139,24 @@ def lib() -> C.CDLL:
             "eppx_autotune": ([vp, vp, P(Physics), i32, P(i32), P(C.c_double)], i32),
             "eppx_set_pair_shape": ([vp, P(i32)], i32),
             "eppx_batch_autotune": ([vp, i32, P(i32), P(C.c_double)], i32),
+            "eppx_domain3d_init": ([C.c_double] * 7 + [P(Domain3D)], i32),
+            "eppx_domain3d_locate": ([P(Domain3D), C.c_float, C.c_float, C.c_float, P(i32), P(i32), P(i32), P(i32),
+                                      P(i32)], i32),
+            "eppx_ns3d_create": ([vp, P(Domain3D), P(i32), i32, i32, P(vp)], i32),
+            "eppx_ns3d_destroy": ([vp], i32),
+            "eppx_ns3d_cell_stencil": ([vp, P(i32), P(i32)], i32),
+            "eppx_ps3d_create": ([vp, i32, P(Layout), P(vp)], i32),
+            "eppx_ps3d_destroy": ([vp], i32),
+            "eppx_ps3d_upload": ([vp, i32, P(i32), P(i32), P(i32), P(C.c_float)], i32),
+            "eppx_ps3d_download": ([vp, vp, P(i32), P(i32), P(i32), P(i32), P(C.c_float)], i32),
+            "eppx_ps3d_alive_count": ([vp, P(i32)], i32),
+            "eppx_search3d": ([vp, vp, i32, i32, vp, C.c_size_t], i32),
+            "eppx_selfaction3d": ([vp, i32, vp, C.c_size_t], i32),
+            "eppx_interaction3d": ([vp, vp, i32, vp, C.c_size_t], i32),
+            "eppx_step3d": ([vp, vp, P(Physics3D), i32], i32),
+            "eppx_pair_count3d": ([vp, vp, P(i64)], i32),
+            "eppx_pair_total3d": ([vp, P(i64), i32], i32),
+            "eppx_export_pairs3d": ([vp, vp, P(i64), P(i32), i64], i32),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)

@@ -21,7 +21,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import Geometry, SPH
-from .Class import Domain2D, HostParticleSystem, NamedIndex
+from .Class import Domain2D, Domain3D, HostParticleSystem, NamedIndex
 
 FLUID_TAG, WALL_TAG = 1, 2
 ACTIVE_PAIR = [(FLUID_TAG, FLUID_TAG), (FLUID_TAG, WALL_TAG), (WALL_TAG, FLUID_TAG)]
@@ -106,6 +106,71 @@ def small(M: int = 40) -> DamBreakCase:
 def large(M: int = 64) -> DamBreakCase:
     """1 x 2 m column, dx = 0.001, Wendland C2 h = 2dx: 2,030,018 particles, 200 steps."""
     return dam_break(0.001, 2.0, SPH.WendlandC2, M=M, steps=200)
+
+
+def dam_break_3d_tables(M: int):
+    """The 3D named tuples: casegen.jl:48-70's columns with 3-wide vectors and 3 floats per nRVec pair.
+    M = 0 gives the neighbour columns width 0 (the device never stores neighbour lists; the neighbour
+    system's max_neighbour_number still bounds the counts), which keeps the host tables of a
+    million-particle case small."""
+    int_nt = {"Tag": 1, "IsMovable": 1, "nCount": 1, "nIndex": M}
+    float_nt = {"PositionVec": 3, "VelocityVec": 3, "dVelocityVec": 3, "Mass": 1, "Volume": 1, "Density": 1,
+                "dDensity": 1, "Pressure": 1, "Gap": 1, "H": 1, "nW": M, "nDW": M, "nHInv": M, "nRVec": 3 * M,
+                "nR": M}
+    return int_nt, float_nt
+
+
+def dam_break_3d(dx: float, h_factor: float = 2.0, kernel=SPH.WendlandC2, M: int = 0, block=(1.0, 0.5, 1.0),
+                 tank=(3.2, 1.0, 1.8), rho0: float = 1000.0, p0: float = 0.0, wall_layers: int = 3,
+                 steps: int = 0) -> DamBreakCase:
+    """The 3D dam break: a water block (block = (x, y, z) extents) in the corner x = y = z = 0 of a tank
+    (tank = extents) with wall_layers layers of fixed wall particles on the floor and the four sides,
+    Tait EOS, artificial viscosity, gravity along -z, H = Gap * h_factor, cell size and domain margin 2h.
+    The reference has no 3D case generator (Geometry3D.jl:89-94 stops at the counts); this one follows
+    casegen.jl's 2D construction: fluid first, then walls, each part a lattice of its own, merged."""
+    bx, by, bz = block
+    tx, ty, tz = tank
+    w = wall_layers * dx
+    margin = 2.0 * h_factor * dx
+    domain = Domain3D(margin, -margin, -margin, -margin, tx + margin, ty + margin, tz + margin)
+    int_nt, float_nt = dam_break_3d_tables(M)
+    basic_parameters = {"rho0": rho0, "p0": p0, "FLUID_TAG": FLUID_TAG, "WALL_TAG": WALL_TAG, "gap0": dx,
+                        "h0": h_factor * dx}
+    named_index = NamedIndex(int_nt, float_nt)
+    parts_spec = [
+        (Geometry.Cuboid(0.0, 0.0, 0.0, bx, by, bz), FLUID_TAG, 1),
+        (Geometry.Cuboid(-w, -w, -w, tx + w, ty + w, 0.0), WALL_TAG, 0),    # floor
+        (Geometry.Cuboid(-w, -w, 0.0, 0.0, ty + w, tz), WALL_TAG, 0),       # x = 0 side
+        (Geometry.Cuboid(tx, -w, 0.0, tx + w, ty + w, tz), WALL_TAG, 0),    # x = tank side
+        (Geometry.Cuboid(0.0, -w, 0.0, tx, 0.0, tz), WALL_TAG, 0),          # y = 0 side
+        (Geometry.Cuboid(0.0, ty, 0.0, tx, ty + w, tz), WALL_TAG, 0),       # y = tank side
+    ]
+    parts = []
+    for cub, tag, movable in parts_spec:
+        n = Geometry.count3(dx, cub)
+        ps = HostParticleSystem(n, named_index, basic_parameters, n_particles=n)
+        pos, vol, gap = Geometry.create3(dx, cub)
+        ps["PositionVec"] = pos
+        ps["Volume"] = vol
+        ps["Gap"] = gap
+        ps["Tag"] = tag
+        ps["IsMovable"] = movable
+        ps["Density"] = np.float32(rho0)
+        ps["Mass"] = ps["Density"] * ps["Volume"]
+        ps["H"] = (ps["Gap"].astype(np.float64) * h_factor).astype(np.float32)
+        ps["Pressure"] = np.float32(p0)
+        parts.append(ps)
+    host = parts[0].merge(*parts[1:])
+    host.set_is_alive()
+    physics = SPH.dam_break_physics_3d(dx, h_factor, kernel, column_height=bz, rho0=rho0)
+    return DamBreakCase(dx, h_factor, kernel, M, domain, host, physics, int_nt, float_nt, basic_parameters, steps,
+                        {"block": tuple(block), "tank": tuple(tank), "dimension": 3})
+
+
+def large_3d(M: int = 0) -> DamBreakCase:
+    """The 3D dam break at dx = 0.0075: a 1.0 x 0.5 x 1.0 m block (1,185,163 fluid particles) in a
+    3.2 x 1.0 x 1.8 m tank with 3 wall layers, Wendland C2 with h = 2dx, 100 steps."""
+    return dam_break_3d(0.0075, 2.0, SPH.WendlandC2, M=M, steps=100)
 
 
 def mix_specs(seed: int = 0, n_cases: int = 64, steps: int = 500):

@@ -304,6 +304,77 @@ int32_t eppx_batch_autotune(eppx_batch *b, int32_t trials, int32_t *choice, doub
 /* Running Σ nCount over the batch's searches since the last reset (waits). */
 int32_t eppx_batch_pair_total(eppx_batch *b, int64_t *out, int32_t reset);
 
+/* ------------------------------------------------------------------- 3D
+ * The reference leaves every 3D layer a stub: Domain3D (AbstractDomain.jl:119
+ * "TODO: add 3D support"), the 27-cell stencil (NeighbourSystemBase.jl:50-57),
+ * insert and find (NeighbourSearch.jl:49, :229), Cuboid lattices
+ * (Geometry3D.jl:89-94); its SPH library (Library.jl:21-34, Continuity.jl,
+ * Pressure.jl, Viscosity.jl, Motion.jl:61-75, the 3D sGravity! State.jl:49-63)
+ * and kernel functions (sigma for N = 3: CubicSpline.jl:15, WendlandC2.jl:15)
+ * are written for any dimension.  These entry points extend the 2D ones
+ * axis by axis and keep their arguments, results and errors; results are
+ * checked against the 3D restatement of the oracle (no reference output
+ * exists to pin them).  Vector columns (PositionVec, VelocityVec,
+ * dVelocityVec) are 3 wide and nRVec holds 3 floats per neighbour. */
+typedef struct eppx_ps3d eppx_ps3d; /* 3D particle system                        */
+typedef struct eppx_ns3d eppx_ns3d; /* 3D neighbour system                       */
+
+/* struct Domain2D's fields with a z axis. */
+typedef struct {
+    float gap, gap_square;
+    int32_t n_x, n_y, n_z, n;
+    float first_x, last_x, first_y, last_y, first_z, last_z;
+    float span_x, span_y, span_z, gap_x, gap_y, gap_z, gap_x_inv, gap_y_inv, gap_z_inv;
+} eppx_domain3d;
+
+/* eppx_physics with the z component of gravity (sGravity!, State.jl:49-63). */
+typedef struct {
+    eppx_physics base;
+    float gz;
+} eppx_physics3d;
+
+/* Domain2D.jl:28-67 per axis: n_x = floor(span_x / gap), gap_x = span_x / n_x,
+ * gap_x_inv = 1 / gap_x (host FP32 arithmetic). */
+int32_t eppx_domain3d_init(double gap, double first_x, double first_y, double first_z, double last_x, double last_y,
+                           double last_z, eppx_domain3d *out);
+/* Domain2D.jl:110-137 per axis; linear = i + n_x (j - 1) + n_x n_y (k - 1). */
+int32_t eppx_domain3d_locate(const eppx_domain3d *d, float x, float y, float z, int32_t *inside, int32_t *i,
+                             int32_t *j, int32_t *k, int32_t *linear);
+/* eppx_ns_create (NeighbourSystem.jl:68-86) over a Domain3D grid. */
+int32_t eppx_ns3d_create(eppx_ctx *ctx, const eppx_domain3d *domain, const int32_t *adjacency, int32_t max_tag,
+                         int32_t max_neighbour_number, eppx_ns3d **out);
+int32_t eppx_ns3d_destroy(eppx_ns3d *ns);
+/* The 27-cell stencil, device_initializeNeighbourSystem! (NeighbourSystemBase.jl:28-48)
+ * with a z offset innermost: count (n_cells) and list (n_cells, 27) column-major,
+ * 1-based, unused entries 0. */
+int32_t eppx_ns3d_cell_stencil(eppx_ns3d *ns, int32_t *count, int32_t *list);
+/* eppx_ps_create / _destroy / _upload / _download / _alive_count
+ * (ParticleSystemBase.jl:52-67, ParticleSystem.jl:154-158, :275-307) for 3D. */
+int32_t eppx_ps3d_create(eppx_ctx *ctx, int32_t n_capacity, const eppx_layout *layout, eppx_ps3d **out);
+int32_t eppx_ps3d_destroy(eppx_ps3d *ps);
+int32_t eppx_ps3d_upload(eppx_ps3d *ps, int32_t n_particles, const int32_t *is_alive, const int32_t *cell_index,
+                         const int32_t *int_props, const float *float_props);
+int32_t eppx_ps3d_download(eppx_ps3d *ps, eppx_ns3d *ns, int32_t *n_particles, int32_t *is_alive,
+                           int32_t *cell_index, int32_t *int_props, float *float_props);
+int32_t eppx_ps3d_alive_count(eppx_ps3d *ps, int32_t *out);
+/* Algorithm.search! (NeighbourSearch.jl:323-413) in 3D; params: an
+ * eppx_physics3d for EPPX_SEARCH_KERNEL_CONTINUITY, may be NULL for
+ * EPPX_SEARCH_NONE. */
+int32_t eppx_search3d(eppx_ps3d *ps, eppx_ns3d *ns, int32_t criterion, int32_t action, const void *params,
+                      size_t params_size);
+/* Algorithm.selfaction! (Action.jl:39-117): EPPX_SELF_CONTINUITY_EOS and
+ * EPPX_SELF_GRAVITY_MOVE, params an eppx_physics3d. */
+int32_t eppx_selfaction3d(eppx_ps3d *ps, int32_t action, const void *params, size_t params_size);
+/* Algorithm.interaction! (Action.jl:121-206): EPPX_INTER_MOMENTUM, params an
+ * eppx_physics3d, over the pairs of the last search on ns. */
+int32_t eppx_interaction3d(eppx_ps3d *ps, eppx_ns3d *ns, int32_t action, const void *params, size_t params_size);
+/* n_steps iterations of the loop body main.jl:166-183 (no filter) in 3D. */
+int32_t eppx_step3d(eppx_ps3d *ps, eppx_ns3d *ns, const eppx_physics3d *phys, int32_t n_steps);
+/* eppx_pair_count, eppx_pair_total, eppx_export_pairs for 3D. */
+int32_t eppx_pair_count3d(eppx_ps3d *ps, eppx_ns3d *ns, int64_t *out);
+int32_t eppx_pair_total3d(eppx_ps3d *ps, int64_t *out, int32_t reset);
+int32_t eppx_export_pairs3d(eppx_ps3d *ps, eppx_ns3d *ns, int64_t *offsets, int32_t *js, int64_t capacity);
+
 #ifdef __cplusplus
 }
 #endif

@@ -133,6 +133,66 @@ int32_t or_run_steps(const or_layout *L, int32_t n_particles, int32_t *is_alive,
                      float *FP, or_neighbour_system *ns, const or_physics *phys, int32_t n_steps,
                      int32_t filter_interval, int32_t serial_insert, int32_t n_threads, int64_t *pairs_per_step);
 
+/* ------------------------------------------------------------------ 3D
+ * eppx_oracle3d.c: the same algorithm with a z axis.  The reference's 3D
+ * layers are stubs (AbstractDomain.jl:119, NeighbourSystemBase.jl:50-57,
+ * NeighbourSearch.jl:49, :229), so these functions extend the 2D ones axis
+ * by axis and restate the dimension-generic SPH library with N = 3. */
+
+/* struct Domain2D's fields with a z axis. */
+typedef struct {
+    float gap, gap_square;
+    int32_t n_x, n_y, n_z, n;
+    float first_x, last_x, first_y, last_y, first_z, last_z;
+    float span_x, span_y, span_z, gap_x, gap_y, gap_z, gap_x_inv, gap_y_inv, gap_z_inv;
+} or_domain3d;
+
+typedef struct {
+    or_domain3d domain;
+    int32_t n_cells, max_contained;
+    int32_t *contained_count;      /* (n_cells)                          */
+    int32_t *contained_list;       /* (n_cells, max_contained) col-major */
+    int32_t *neighbour_cell_count; /* (n_cells)                          */
+    int32_t *neighbour_cell_list;  /* (n_cells, 27) col-major            */
+    int32_t max_tag;
+    int32_t *adjacency;
+} or_neighbour_system3d;
+
+typedef struct {
+    or_physics p;
+    float gz; /* sGravity! in 3D, State.jl:49-63 */
+} or_physics3d;
+
+void or3_domain_init(double gap, double first_x, double first_y, double first_z, double last_x, double last_y,
+                     double last_z, or_domain3d *d);
+void or3_index_cartesian_from_position(const or_domain3d *d, float x, float y, float z, int32_t *i, int32_t *j,
+                                       int32_t *k);
+int32_t or3_index_linear_from_position(const or_domain3d *d, float x, float y, float z);
+int32_t or3_inside(const or_domain3d *d, float x, float y, float z);
+int32_t or3_ns_create(const or_domain3d *d, int32_t max_contained, const int32_t *adjacency, int32_t max_tag,
+                      or_neighbour_system3d *ns);
+void or3_ns_destroy(or_neighbour_system3d *ns);
+float or3_kernel_value(int32_t kernel, float r, float hinv);
+float or3_kernel_gradient(int32_t kernel, float r, float hinv);
+float or3_kernel_value0(int32_t kernel, float h);
+/* As or_search; PositionVec, VelocityVec and dVelocityVec are 3 wide and
+ * nRVec holds 3 floats per neighbour. */
+int32_t or3_search(const or_layout *L, int32_t n_particles, int32_t *is_alive, int32_t *cell_index, int32_t *IP,
+                   float *FP, or_neighbour_system3d *ns, int32_t criterion, int32_t action, const or_physics3d *phys,
+                   int32_t serial_insert, int32_t n_threads);
+/* 1 continuity + volume + EOS (sContinuity!), 2 gravity + move (sMomentum!). */
+void or3_selfaction(const or_layout *L, int32_t n_particles, const int32_t *is_alive, int32_t *IP, float *FP,
+                    int32_t action, const or_physics3d *phys, int32_t n_threads);
+/* 1 momentum (iMomentum!). */
+void or3_interaction(const or_layout *L, int32_t n_particles, const int32_t *is_alive, int32_t *IP, float *FP,
+                     int32_t action, const or_physics3d *phys, int32_t n_threads);
+int32_t or3_run_steps(const or_layout *L, int32_t n_particles, int32_t *is_alive, int32_t *cell_index, int32_t *IP,
+                      float *FP, or_neighbour_system3d *ns, const or_physics3d *phys, int32_t n_steps,
+                      int32_t serial_insert, int32_t n_threads, int64_t *pairs_per_step);
+/* or_pair_term_magnitude in 3D: what = 0, out[I]; what = 1, out[3 I + k]. */
+void or3_pair_term_magnitude(const or_layout *L, int32_t n_particles, const int32_t *is_alive, int32_t *IP,
+                             float *FP, const or_physics3d *phys, int32_t what, double *out);
+
 #ifdef __cplusplus
 }
 #endif

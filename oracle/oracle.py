@@ -30,10 +30,16 @@ CRIT_DEFAULT, CRIT_SYMMETRY = 0, 1
 ERR_CELL_OVERFLOW, ERR_NEIGHBOUR_OVERFLOW = 1, 2
 
 
+SOURCES = ("eppx_oracle.c", "eppx_oracle3d.c", "eppx_oracle.h")
+
+
+def _newest_source() -> float:
+    return max(os.path.getmtime(os.path.join(HERE, f)) for f in SOURCES)
+
+
 def build(force: bool = False) -> str:
-    """Compile the restatement with its Makefile (gcc; seconds)."""
-    src = os.path.join(HERE, "eppx_oracle.c")
-    if force or not os.path.exists(LIB_PATH) or os.path.getmtime(LIB_PATH) < os.path.getmtime(src):
+    """Compile the restatement (2D and 3D) with its Makefile (gcc; seconds)."""
+    if force or not os.path.exists(LIB_PATH) or os.path.getmtime(LIB_PATH) < _newest_source():
         subprocess.run(["make", "-s", "-C", HERE], check=True)
     return LIB_PATH
 
@@ -119,8 +125,7 @@ def mutant(k: int):
     """Restatement with planted mistake OR_MUTATION=k (for tests that show a check catches it)."""
     if k not in _variants:
         path = os.path.join(HERE, "build", f"libeppx_oracle_m{k}.so")
-        src = os.path.join(HERE, "eppx_oracle.c")
-        if not os.path.exists(path) or os.path.getmtime(path) < os.path.getmtime(src):
+        if not os.path.exists(path) or os.path.getmtime(path) < _newest_source():
             subprocess.run(["make", "-s", "-C", HERE, f"build/libeppx_oracle_m{k}.so"], check=True)
         _variants[k] = _bind(C.CDLL(path))
     return _variants[k]

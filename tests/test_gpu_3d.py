@@ -1,0 +1,351 @@
+"""The 3D path on the device against the 3D restatement (oracle/eppx_oracle3d.c).
+
+The reference's 3D layers are stubs (NeighbourSystemBase.jl:50-57,
+NeighbourSearch.jl:229, AbstractDomain.jl:119), so parity here is with the
+restatement, itself pinned by closed forms, invariants and brute force
+(tests/test_oracle3d.py) -- unpinned by reference output.
+
+* EXACT math: pair sets, neighbour columns, dDensity and whole steps
+  bit-identical to the restatement (small and medium dam breaks, a jittered
+  lattice with varying H and wall tags, the default criterion).
+* FAST math, on a flow developed on the device: pair sets bit-exact;
+  dDensity and acceleration per particle within 1e-5 of the sum of |terms|
+  of the particle's sum (the float32 bound of a sum taken in another order
+  and with other rounding of its terms), density 1e-5 relative.
+* The benchmark's 3D configuration (2,180,764 particles): FAST and EXACT
+  device runs agree (same pairs, density 1e-5, positions within one float32
+  spacing) and the run conserves mass with every state finite.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+M = 300
+ADJ = np.array([[1, 1], [1, 0]], np.int32)
+STATE3 = ("PositionVec", "VelocityVec", "dVelocityVec", "Mass", "Volume", "Density", "dDensity", "Pressure", "H")
+PHYS_KEYS = ("fluid_tag", "wall_tag", "kernel", "eos", "visc", "dt", "rho0", "p0", "c0", "tait_b", "gx", "gy", "mu",
+             "alpha", "beta", "av_c", "coefficient")
+
+
+def width3(name):
+    return 3 if name.endswith("Vec") else 1
+
+
+@pytest.fixture(scope="module")
+def o3():
+    import oracle3d
+    oracle3d.build()
+    return oracle3d
+
+
+def small_case(M=M):
+    from eppx import cases
+    return cases.dam_break_3d(0.05, M=M, block=(0.5, 0.25, 0.5), tank=(1.2, 0.5, 0.8))
+
+
+def oracle_physics(o3, physics):
+    return o3.physics3d(gz=physics.gz, **{k: getattr(physics, k) for k in PHYS_KEYS})
+
+
+def device3(par, case, M=M):
+    from eppx import Class, cases
+    ps = Class.ParticleSystem(par, case.domain, case.n_particles, case.int_nt, case.float_nt, case.basic_parameters)
+    Class.syncto(ps, case.host)
+    ns = Class.NeighbourSystem(par, case.domain, cases.ACTIVE_PAIR, max_neighbour_number=M)
+    return ps, ns
+
+
+def reference3(o3, case, M=M):
+    import oracle as O
+    h = case.host
+    L = O.make_layout(case.int_nt, case.float_nt, h.n_capacity)
+    return o3.System3D(h.int_properties_.copy(order="F"), h.float_properties_.copy(order="F"), h.is_alive_.copy(),
+                       h.cell_index_.copy(), h.n_particles_, L, o3.domain3d(*case.domain.args), ADJ, M)
+
+
+def search_device(ps, case, ns, physics=None):
+    from eppx import Algorithm, SPH
+    Algorithm.search(ps, case.domain, ns, criterion=Algorithm.symmetrySearchCriterion,
+                     action=SPH.sComputeKernelAndiContinuity(physics or case.physics))
+
+
+def download(ps):
+    from eppx import Class
+    host = Class.mirror(ps)
+    Class.syncto(host, ps)
+    return host
+
+
+def export_pairs3(ps, ns):
+    from eppx import _lib
+    L = _lib.lib()
+    n = ps.n_particles_
+    off = np.zeros(n + 1, np.int64)
+    total = C.c_int64()
+    _lib.check(L.eppx_pair_count3d(ps.handle, ns.handle, C.byref(total)), ps.parallel.ctx, "pair_count3d")
+    js = np.zeros(max(1, total.value), np.int32)
+    _lib.check(L.eppx_export_pairs3d(ps.handle, ns.handle, _lib.ptr(off, C.c_int64), _lib.ptr(js, C.c_int32),
+                                     js.shape[0]), ps.parallel.ctx, "export_pairs3d")
+    return off, js[: total.value]
+
+
+def pair_total3(ps, reset=True):
+    from eppx import _lib
+    v = C.c_int64()
+    _lib.check(_lib.lib().eppx_pair_total3d(ps.handle, C.byref(v), int(reset)), ps.parallel.ctx, "pair_total3d")
+    return v.value
+
+
+def assert_state3_equal(host, ref, names=STATE3, alive_only=True):
+    assert np.array_equal(host.is_alive_[: ref.n], ref.is_alive[: ref.n])
+    rows = ref.is_alive[: ref.n] == 1 if alive_only else slice(0, ref.n)
+    for name in names:
+        c = getattr(ref.L, name)
+        got = host.float_properties_[: ref.n][rows, c: c + width3(name)]
+        exp = ref.FP[: ref.n][rows, c: c + width3(name)]
+        if not np.array_equal(got.view(np.uint32), exp.view(np.uint32)):
+            bad = np.argwhere(got != exp)
+            r, k = bad[0] if len(bad) else (0, 0)
+            raise AssertionError(f"{name}: {len(bad)} entries differ, first row {r}: {got[r, k]!r} vs {exp[r, k]!r}")
+
+
+def assert_neighbour_columns3_equal(host, ref):
+    L = ref.L
+    Mx = L.max_neighbour_number
+    alive = np.nonzero(ref.is_alive[: ref.n] == 1)[0]
+    cnt = ref.IP[alive, L.nCount]
+    assert np.array_equal(host.int_properties_[alive, L.nCount], cnt)
+    mask = np.arange(Mx)[None, :] < cnt[:, None]
+    for col, w, table in (("nIndex", 1, "int"), ("nR", 1, "f"), ("nW", 1, "f"), ("nDW", 1, "f"), ("nHInv", 1, "f"),
+                          ("nRVec", 3, "f")):
+        c = getattr(L, col)
+        got = (host.int_properties_ if table == "int" else host.float_properties_)[alive, c: c + w * Mx]
+        exp = (ref.IP if table == "int" else ref.FP)[alive, c: c + w * Mx]
+        m = np.repeat(mask, w, axis=1)
+        if not np.array_equal(got[m], exp[m]):
+            bad = np.argwhere((got != exp) & m)
+            raise AssertionError(f"{col}: {len(bad)} entries differ (first {bad[0]})")
+
+
+# ----------------------------------------------------------------- EXACT
+
+def test_3d_search_exact(parallel, o3):
+    """One search with the kernel + continuity action: pair lists in list order, every neighbour column and
+    dDensity identical to the restatement."""
+    import oracle as O
+    case = small_case()
+    print(f"3D small dam break: {case.n_particles} particles, cells {case.domain}")
+    ps, ns = device3(parallel, case)
+    search_device(ps, case, ns)
+    ref = reference3(o3, case)
+    assert ref.search(O.CRIT_SYMMETRY, 1, oracle_physics(o3, case.physics)) == 0
+    off, js = export_pairs3(ps, ns)
+    roff, rjs = ref.pair_csr()
+    assert np.array_equal(off, roff) and np.array_equal(js, rjs)
+    host = download(ps)
+    assert_neighbour_columns3_equal(host, ref)
+    assert_state3_equal(host, ref)
+    print(f"pairs {int(off[-1])}, max count {int(ref.count().max())}")
+    ps.close()
+    ns.close()
+
+
+def test_3d_steps_exact(parallel, o3):
+    """20 steps of the 3D loop body: every particle's state bit-identical, and the same pairs per search."""
+    case = small_case()
+    ps, ns = device3(parallel, case)
+    search_device(ps, case, ns)
+    ref = reference3(o3, case)
+    oph = oracle_physics(o3, case.physics)
+    import oracle as O
+    assert ref.search(O.CRIT_SYMMETRY, 1, oph) == 0
+    pair_total3(ps)
+    from eppx import Algorithm
+    Algorithm.step(ps, ns, case.physics, n_steps=20)
+    total = pair_total3(ps)
+    err, pairs = ref.run_steps(20, oph, threads=8)
+    assert err == 0
+    assert total == int(pairs.sum())
+    host = download(ps)
+    assert_state3_equal(host, ref, alive_only=False)
+    assert np.array_equal(host.cell_index_[: ref.n], ref.cell_index[: ref.n])
+    fluid = ref.IP[: ref.n, ref.L.Tag] == 1
+    w = ref.FP[: ref.n, ref.L.VelocityVec + 2][fluid]
+    print(f"20 steps: {total} pairs, fluid w in [{w.min():.3g}, {w.max():.3g}] m/s")
+    assert w.min() < 0.0  # the block has started to fall
+    ps.close()
+    ns.close()
+
+
+def jittered_case(o3, jitter=0.3, h_jitter=0.03, seed=2, nx=14, ny=12, nz=13, dx=0.01, gap=None):
+    """A jittered lattice of fluid and wall particles (30 % walls) with per-particle H; the cell size gap
+    (default: 2 max H) is the default criterion's radius."""
+    from eppx import Class, SPH, cases
+    import dataclasses
+    n = nx * ny * nz
+    int_nt, float_nt = cases.dam_break_3d_tables(M)
+    ni = Class.NamedIndex(int_nt, float_nt)
+    params = {"rho0": 1000.0, "p0": 0.0, "FLUID_TAG": 1, "WALL_TAG": 2, "gap0": dx, "h0": 2 * dx}
+    host = Class.HostParticleSystem(n, ni, params, n_particles=n)
+    rng = np.random.default_rng(seed)
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    pos = np.stack([(0.5 + i.ravel()) * dx, (0.5 + j.ravel()) * dx, (0.5 + k.ravel()) * dx], axis=1)
+    pos = (pos + rng.uniform(-jitter, jitter, pos.shape) * dx).astype(np.float32)
+    host["PositionVec"] = pos
+    host["VelocityVec"] = rng.uniform(-0.3, 0.3, (n, 3)).astype(np.float32)
+    host["Tag"] = np.where(rng.uniform(size=n) < 0.3, 2, 1).astype(np.int32)
+    host["IsMovable"] = 1
+    host["Density"] = (1000.0 + rng.uniform(-10, 10, n)).astype(np.float32)
+    host["Volume"] = np.float32(dx ** 3)
+    host["Mass"] = host["Density"] * host["Volume"]
+    host["H"] = (np.float32(2 * dx) * (1 + rng.uniform(-h_jitter, h_jitter, n))).astype(np.float32)
+    host["Pressure"] = rng.uniform(0, 2e4, n).astype(np.float32)
+    host.set_is_alive()
+    gap = gap or 4 * dx * (1 + h_jitter)
+    domain = Class.Domain3D(gap, -gap, -gap, -gap, nx * dx + gap, ny * dx + gap, nz * dx + gap)
+    physics = SPH.dam_break_physics_3d(dx, 2.0, SPH.WendlandC2, column_height=nz * dx)
+    case = cases.dam_break_3d(0.05, M=M, block=(0.1, 0.1, 0.1), tank=(0.2, 0.2, 0.2))
+    return dataclasses.replace(case, domain=domain, host=host, physics=physics, int_nt=int_nt, float_nt=float_nt,
+                               basic_parameters=params)
+
+
+@pytest.mark.parametrize("criterion", ["symmetry", "default"])
+def test_3d_jittered_varying_h_exact(parallel, o3, criterion):
+    """Jittered positions, per-particle H (the exact criterion band), fluid and wall tags (wall-wall pairs are not
+    adjacent), random velocities and pressures: pairs, dDensity and one momentum pass identical."""
+    import oracle as O
+    from eppx import Algorithm, SPH
+    # the default criterion accepts r <= the cell size: 3.5dx keeps its counts (about 180) below M
+    case = jittered_case(o3, gap=None if criterion == "symmetry" else 0.035)
+    crit = Algorithm.symmetrySearchCriterion if criterion == "symmetry" else Algorithm.defaultSearchCriterion
+    ps, ns = device3(parallel, case)
+    Algorithm.search(ps, case.domain, ns, criterion=crit, action=SPH.sComputeKernelAndiContinuity(case.physics))
+    Algorithm.interaction(ps, SPH.iMomentum(case.physics))
+    ref = reference3(o3, case)
+    oph = oracle_physics(o3, case.physics)
+    assert ref.search(O.CRIT_SYMMETRY if criterion == "symmetry" else O.CRIT_DEFAULT, 1, oph) == 0
+    ref.interaction(1, oph)
+    off, js = export_pairs3(ps, ns)
+    roff, rjs = ref.pair_csr()
+    assert np.array_equal(off, roff) and np.array_equal(js, rjs)
+    host = download(ps)
+    assert_state3_equal(host, ref)
+    print(f"jittered 3D lattice ({criterion}): {int(off[-1])} pairs")
+    ps.close()
+    ns.close()
+
+
+def test_3d_medium_exact_steps(parallel, o3):
+    """The benchmark geometry at dx = 0.02 (207k particles): the first search and 3 steps bit-identical."""
+    import oracle as O
+    from eppx import Algorithm, cases
+    case = cases.dam_break_3d(0.02, M=M)
+    print(f"3D dam break dx = 0.02: {case.n_particles} particles")
+    ps, ns = device3(parallel, case)
+    search_device(ps, case, ns)
+    ref = reference3(o3, case)
+    oph = oracle_physics(o3, case.physics)
+    assert ref.search(O.CRIT_SYMMETRY, 1, oph, threads=16) == 0
+    off, js = export_pairs3(ps, ns)
+    roff, rjs = ref.pair_csr()
+    assert np.array_equal(off, roff) and np.array_equal(js, rjs)
+    Algorithm.step(ps, ns, case.physics, n_steps=3)
+    err, pairs = ref.run_steps(3, oph, threads=16)
+    assert err == 0
+    host = download(ps)
+    assert_state3_equal(host, ref, alive_only=False)
+    ps.close()
+    ns.close()
+
+
+# ------------------------------------------------------------------ FAST
+
+def test_3d_fast_within_bound(parallel, o3):
+    """From the small dam break 400 EXACT steps in (both sides start from the same tables, dDensity zeroed): one
+    FAST search + continuity and one FAST momentum pass against the restatement."""
+    import dataclasses
+    import oracle as O
+    from eppx import Algorithm, SPH
+    case = small_case()
+    ps, ns = device3(parallel, case)
+    search_device(ps, case, ns)
+    Algorithm.step(ps, ns, case.physics, n_steps=400)
+    host = download(ps)
+    ps.close()
+    ns.close()
+    host.float_properties_[:, host.named_index_.float_table.index["dDensity"] - 1] = 0.0
+    dev = dataclasses.replace(case, host=host)
+    L = host.layout
+    w = host.float_properties_[: host.n_particles_, L.VelocityVec: L.VelocityVec + 3]
+    print(f"after 400 steps: |u| <= {np.abs(w).max():.3g} m/s")
+    parallel.set_math("fast")
+    try:
+        ps, ns = device3(parallel, dev)
+        search_device(ps, dev, ns)
+        Algorithm.interaction(ps, SPH.iMomentum(dev.physics))
+        off, js = export_pairs3(ps, ns)
+        got = download(ps)
+        ps.close()
+        ns.close()
+    finally:
+        parallel.set_math("exact")
+    ref = reference3(o3, dev)
+    oph = oracle_physics(o3, dev.physics)
+    assert ref.search(O.CRIT_SYMMETRY, 1, oph) == 0
+    roff, rjs = ref.pair_csr()
+    assert np.array_equal(off, roff) and np.array_equal(js, rjs)
+    bound_c = ref.term_magnitude(0, oph)
+    bound_m = ref.term_magnitude(1, oph)
+    ref.interaction(1, oph)
+    n = ref.n
+    RL = ref.L
+    alive = ref.is_alive[:n] == 1
+    g, r = got.float_properties_[:n][alive], ref.FP[:n][alive]
+    drho = np.abs(g[:, RL.dDensity].astype(np.float64) - r[:, RL.dDensity]) / (1e-5 * bound_c[alive] + 1e-30)
+    du = np.abs(g[:, RL.dVelocityVec: RL.dVelocityVec + 3].astype(np.float64)
+                - r[:, RL.dVelocityVec: RL.dVelocityVec + 3]) / (1e-5 * bound_m[alive] + 1e-30)
+    print(f"3D FAST: dDensity {drho.max():.3g} of 1e-5 x its bound, acceleration {du.max():.3g}")
+    assert drho.max() <= 1.0
+    assert du.max() <= 1.0
+    assert np.array_equal(g[:, RL.Density], r[:, RL.Density])
+
+
+# ------------------------------------------------------- benchmark config
+
+def test_3d_large_fast_matches_exact_and_conserves(parallel):
+    """2,180,764 particles (1.0 x 0.5 x 1.0 m block at dx = 0.0075 in a 3.2 x 1.0 x 1.8 m tank, Wendland C2,
+    h = 2dx): 10 steps in FAST and in EXACT math from the same start give the same pairs, density within 1e-5
+    relative and positions within one float32 spacing; mass is constant and every state finite."""
+    from eppx import Algorithm, cases
+    case = cases.large_3d()
+    out = {}
+    for math in ("exact", "fast"):
+        parallel.set_math(math)
+        try:
+            ps, ns = device3(parallel, case, M=400)
+            search_device(ps, case, ns)
+            pair_total3(ps)
+            Algorithm.step(ps, ns, case.physics, n_steps=10)
+            out[math] = (download(ps), pair_total3(ps))
+            ps.close()
+            ns.close()
+        finally:
+            parallel.set_math("exact")
+    (he, te), (hf, tf) = out["exact"], out["fast"]
+    print(f"3D 2.18M: {te} pairs over 10 steps")
+    assert te == tf and te > 0
+    L = he.layout
+    n = case.n_particles
+    assert np.array_equal(he.is_alive_[:n], hf.is_alive_[:n]) and he.is_alive_[:n].all()
+    m0 = case.host.float_properties_[:n, L.Mass]
+    assert np.array_equal(he.float_properties_[:n, L.Mass], m0)
+    for h in (he, hf):
+        assert np.all(np.isfinite(h.float_properties_[:n, : L.H + 1]))
+    rho_e, rho_f = he.float_properties_[:n, L.Density].astype(np.float64), hf.float_properties_[:n, L.Density]
+    assert np.max(np.abs(rho_f - rho_e) / rho_e) <= 1e-5
+    xe = he.float_properties_[:n, L.PositionVec: L.PositionVec + 3]
+    xf = hf.float_properties_[:n, L.PositionVec: L.PositionVec + 3]
+    assert np.all(np.abs(xf.astype(np.float64) - xe) <= np.spacing(np.abs(xe)))
