@@ -1,6 +1,6 @@
 """Benchmark: pair-interactions/s of the 2D dam break on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload large|small|mix|serve|serve-sweep]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload large|small|mix|serve|serve-sweep|3d]
                     [--math exact|fast]
 
 Workload "large" (default): 1.0 x 2.0 m water column in a 4 x 3 m tank,
@@ -56,6 +56,13 @@ ALG_BYTES = {
     # alive 4, tag 4, dvel 8, pos 8, vel 8 in; dvel 8, pos 8, vel 8 out
     "gravity_move": (56, 0),
 }
+
+
+# The 3D pair kernels (compulsory bytes per particle, per cell): the search with kernel + continuity reads
+# x, y, z, tag, u, v, w, rho, H, Mass, Volume, the cell and dDensity and writes dDensity and the count (64 B),
+# with the cell starts and cell tags (8 B/cell); momentum reads x, y, z, tag, u, v, w, rho, p, Mass, H, the cell
+# and du and writes du (72 B) with the same cell tables.
+ALG_BYTES_3D = {"search": (64, 8), "momentum": (72, 8), "continuity_eos": (36, 0), "gravity_move": (76, 0)}
 
 
 def log(msg):
@@ -158,6 +165,27 @@ def copy_calibration(n_bytes: int, repeats: int = 3) -> None:
     hip.hipFree(src)
     hip.hipFree(dst)
     print(json.dumps({"copy_calibration_bytes": n_bytes, "repeats": repeats}), flush=True)
+
+
+def valu_calibration(blocks: int = 8192, iters: int = 20000) -> dict:
+    """The VALU calibration kernel (8 x iters FP32 FMAs per lane, blocks x 4 waves): its measured duration
+    against the issue-rate model of one wave64 VALU instruction per 4 cycles per SIMD (1024 SIMDs at the
+    2.4 GHz peak clock).  Run under rocprofv3 --pmc, it gives the counter readings of a kernel whose VALU
+    instruction count is known."""
+    import eppx
+    from eppx import _lib
+    par = eppx.Environment.Parallel(device=0)
+    us = C.c_double()
+    _lib.check(_lib.lib().eppx_valu_calibration(par.ctx, blocks, iters, C.byref(us)), par.ctx, "valu_calibration")
+    waves = blocks * 4
+    valu_per_wave = 8 * iters
+    model_us = waves * valu_per_wave * 4 / 1024 / 2.4e9 * 1e6
+    out = {"valu_calibration": {"blocks": blocks, "waves": waves, "valu_per_wave": valu_per_wave,
+                                "launch_us": us.value, "model_us_at_2.4GHz": model_us,
+                                "issue_fraction_of_model": model_us / us.value}}
+    print(json.dumps(out), flush=True)
+    par.close()
+    return out
 
 
 def _warm_up(par):
@@ -296,12 +324,114 @@ def run_serve(args, par, rate):
     }
 
 
+def cpu_baseline_3d(seconds, threads, dx=0.015, max_steps=50):
+    """The 3D restatement (oracle/eppx_oracle3d.c) on the 3D dam break's geometry at dx (same h / dx, so the
+    same ~256 neighbours per particle as the benchmark case): whole steps after one search, until `seconds`."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle as O
+    import oracle3d as O3
+    from eppx import cases
+    c = cases.dam_break_3d(dx, M=300)
+    h = c.host
+    ref = O3.System3D(h.int_properties_, h.float_properties_, h.is_alive_, h.cell_index_, h.n_particles_,
+                      O.make_layout(c.int_nt, c.float_nt, h.n_capacity), O3.domain3d(*c.domain.args),
+                      np.array([[1, 1], [1, 0]]), 300)
+    p = c.physics
+    ph = O3.physics3d(gz=p.gz, **{k: v for k, v in p.as_dict().items() if k != "gz"})
+    ref.search(O.CRIT_SYMMETRY, 1, ph, serial=False, threads=threads)
+    total, steps, dt = 0.0, 0, 0.0
+    while steps < max_steps and (steps == 0 or dt < seconds):
+        t0 = time.perf_counter()
+        err, pairs = ref.run_steps(1, ph, serial=False, threads=threads)
+        dt += time.perf_counter() - t0
+        total += float(pairs.sum())
+        steps += 1
+    return total / dt, dt, steps, c.n_particles
+
+
+def run_3d(args, par):
+    """The 3D dam break (1.0 x 0.5 x 1.0 m block in a 3.2 x 1.0 x 1.8 m tank, dx = 0.0075: 2,180,764 particles,
+    Wendland C2 3D, h = 2dx, Tait EOS, artificial viscosity): args.steps steps (100 by default) of the loop body
+    on state resident in HBM, pair-interactions/s, the dominant kernel's roofline and the 3D restatement on the
+    host cores."""
+    from eppx import Algorithm, SPH, _lib, cases
+    L = _lib.lib()
+    steps = args.steps if args.steps_given else 100
+    t_setup = time.perf_counter()
+    case = cases.large_3d()
+    log(f"3D case: {case.n_particles} particles, {case.domain.get_n()} cells ({time.perf_counter() - t_setup:.1f}s)")
+    from eppx import Class
+    ps = Class.ParticleSystem(par, case.domain, case.n_particles, case.int_nt, case.float_nt, case.basic_parameters)
+    Class.syncto(ps, case.host)
+    ns = Class.NeighbourSystem(par, case.domain, cases.ACTIVE_PAIR, max_neighbour_number=400)
+    Algorithm.search(ps, case.domain, ns, criterion=Algorithm.symmetrySearchCriterion,
+                     action=SPH.sComputeKernelAndiContinuity(case.physics))
+    Algorithm.step(ps, ns, case.physics, n_steps=args.warmup)
+    log(f"warmup {args.warmup} steps done")
+    total = C.c_int64()
+    _lib.check(L.eppx_pair_total3d(ps.handle, C.byref(total), 1), par.ctx, "pair_total3d")
+    par.synchronize()
+    t0 = time.perf_counter()
+    Algorithm.step(ps, ns, case.physics, n_steps=steps, sync=False)
+    par.synchronize()
+    elapsed = time.perf_counter() - t0
+    _lib.check(L.eppx_pair_total3d(ps.handle, C.byref(total), 1), par.ctx, "pair_total3d")
+    pairs = total.value
+    alive = ps.get_alive_n_particles()
+    _lib.check(L.eppx_set_timing(par.ctx, 1), par.ctx, "timing")
+    L.eppx_kernel_time(par.ctx, b"search", None, None, 1)
+    Algorithm.step(ps, ns, case.physics, n_steps=args.profile_steps_3d)
+    _lib.check(L.eppx_set_timing(par.ctx, 0), par.ctx, "timing")
+    names = ["search", "momentum", "continuity_eos", "gravity_move", "cell_key", "cell_scan", "cell_permute"]
+    kt = kernel_times(par, names)
+    log("kernel averages (us): " + ", ".join(f"{k}={v[0]:.1f}x{v[1]}" for k, v in kt.items()))
+    dominant = max((k for k in kt if k in ALG_BYTES_3D), key=lambda k: kt[k][0] * kt[k][1])
+    per_p, per_cell = ALG_BYTES_3D[dominant]
+    alg_bytes = per_p * case.n_particles + per_cell * case.domain.get_n()
+    avg_us = kt[dominant][0]
+    achieved = alg_bytes / (avg_us * 1e-6) / 1e9
+    device_us_per_step = sum(v[0] * v[1] for v in kt.values()) / args.profile_steps_3d
+    cpu = None
+    if not args.no_cpu:
+        threads = len(os.sched_getaffinity(0))
+        threads = min(threads, int(os.environ.get("OMP_NUM_THREADS", threads)))
+        rate, secs, nsteps, n_cpu = cpu_baseline_3d(args.cpu_seconds, threads)
+        cpu = {"value": rate, "unit": "pair-interactions/s", "cores": threads, "kind": "port",
+               "sample": f"{nsteps} steps of the same 3D dam break geometry at dx = 0.015 ({n_cpu} particles, the "
+                         f"same h / dx and neighbours per particle) after one search, C/OpenMP 3D restatement on "
+                         f"the reference layout (per-cell atomic insert, materialised M = 300 lists), {secs:.1f} s",
+               "host": cpu_info(threads)}
+        log(f"cpu 3D baseline: {rate / 1e9:.3f} G/s on {threads} threads")
+    ps.close()
+    ns.close()
+    ms = 1000.0 * elapsed / steps
+    return {
+        "metric": "pair-interactions/s at 2.18M-particle 3D dam break",
+        "value": pairs / elapsed, "unit": "pair-interactions/s", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": ms, "steps_per_s": 1000.0 / ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"3D dam break: 1.0 x 0.5 x 1.0 m block in a 3.2 x 1.0 x 1.8 m tank, dx = 0.0075, "
+                               f"{case.n_particles} particles (3 wall layers), Wendland C2 3D, h = 2dx, Tait EOS, "
+                               f"artificial viscosity",
+                   "particles": case.n_particles, "alive_after": alive, "cells": case.domain.get_n(),
+                   "math": args.math, "pairs_per_step": pairs / steps,
+                   "parity": "3D restatement (the reference's 3D path is a stub: unpinned by reference)"},
+        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "alg_bytes_per_launch": alg_bytes,
+                     "avg_launch_us": avg_us},
+        "device_us_per_step": device_us_per_step,
+        "kernels_us": {k: round(v[0], 2) for k, v in kt.items()},
+        "cpu_baseline": cpu,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (200; the 3D workload: 100)")
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", choices=["large", "small", "mix", "serve", "serve-sweep"], default="large")
+    ap.add_argument("--workload", choices=["large", "small", "mix", "serve", "serve-sweep", "3d"], default="large")
+    ap.add_argument("--profile-steps-3d", type=int, default=5, help="3D: eager steps timed kernel by kernel")
     ap.add_argument("--mix-steps", type=int, default=500)
     ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--serve-rate", type=float, default=32.0, help="Poisson arrival rate (1/s)")
@@ -320,11 +450,18 @@ def main():
     ap.add_argument("--pair-shape", default=None, metavar="SB,SS,MB,MS",
                     help="pin the pair-kernel shapes instead of tuning: search block, staged, momentum block, staged")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--valu-calibration", action="store_true", help="only run the VALU calibration kernel")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run device copies of BYTES (profiler calibration)")
     args = ap.parse_args()
+    args.steps_given = args.steps is not None
+    if args.steps is None:
+        args.steps = 200
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.valu_calibration:
+        valu_calibration()
         return
     if args.gpus != 1:
         print(json.dumps({"error": "libeppx runs one GPU per process; multi-GPU runs are not part of this "
@@ -336,10 +473,12 @@ def main():
 
     if not os.path.exists(eppx.library_path()):
         eppx.build()
-    if args.workload in ("mix", "serve", "serve-sweep"):
+    if args.workload in ("mix", "serve", "serve-sweep", "3d"):
         par = eppx.Environment.Parallel(device=0, math=args.math)
         _warm_up(par)
-        if args.workload == "mix":
+        if args.workload == "3d":
+            out = run_3d(args, par)
+        elif args.workload == "mix":
             out = run_mix(args, par)
         elif args.workload == "serve":
             out = run_serve(args, par, args.serve_rate)

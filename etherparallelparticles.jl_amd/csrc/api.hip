@@ -1398,6 +1398,29 @@ int32_t eppx_pair_row_fallback(eppx_ps *ps, int64_t *out, int32_t reset) {
     return EPPX_OK;
 }
 
+int32_t eppx_valu_calibration(eppx_ctx *ctx, int32_t blocks, int32_t iters, double *us) {
+    if (!ctx || blocks < 1 || iters < 1) return set_error(ctx, EPPX_E_ARG, "eppx_valu_calibration: bad argument");
+    HIP_OK(ctx, hipSetDevice(ctx->device));
+    float *d_out = nullptr;
+    HIP_OK(ctx, dalloc(&d_out, (size_t)blocks));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = launch_valu_calibration(d_out, blocks, iters, ctx->stream);  // warm-up
+    if (e == hipSuccess) e = hipEventRecord(e0, ctx->stream);
+    if (e == hipSuccess) e = launch_valu_calibration(d_out, blocks, iters, ctx->stream);
+    if (e == hipSuccess) e = hipEventRecord(e1, ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    dfree(d_out);
+    if (e != hipSuccess) return hip_fail(ctx, e, "eppx_valu_calibration");
+    if (us) *us = 1000.0 * ms;
+    return EPPX_OK;
+}
+
 int32_t eppx_check_finite(eppx_ps *ps) {
     if (!ps) return EPPX_E_ARG;
     eppx_ctx *ctx = ps->ctx;

@@ -216,6 +216,7 @@ hipError_t launch_to_ref(Fields f, int32_t n, Columns c, int32_t *ip, float *fp,
                          int32_t *cell_index, int32_t cs, hipStream_t s);
 hipError_t launch_stencil(Grid g, int32_t *count, int32_t *list, hipStream_t s);
 hipError_t launch_finite(Fields f, int32_t n, uint32_t *bad, hipStream_t s);
+hipError_t launch_valu_calibration(float *out, int32_t blocks, int32_t iters, hipStream_t s);
 hipError_t launch_h_range(Fields f, int32_t n, float *hmin_hmax, hipStream_t s);
 
 }  // namespace eppx

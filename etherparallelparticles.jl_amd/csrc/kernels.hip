@@ -547,6 +547,25 @@ __global__ __launch_bounds__(kBlock) void k_stencil(Grid g, int32_t *count, int3
     count[I] = k;
 }
 
+// Profiler calibration: `iters` x 8 independent v_fma_f32 per lane (eight
+// chains, so every wave can issue one VALU instruction per issue slot), the
+// known VALU instruction count against which rocprofv3's VALU counters and
+// the kernel duration are scaled.
+__global__ __launch_bounds__(kBlock) void k_valu_calibration(float *out, int32_t iters) {
+    float a[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) a[k] = (float)(threadIdx.x + k) * 1e-3f;
+    const float b = 1.0000001f, c = 1e-7f;
+    for (int it = 0; it < iters; ++it) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) a[k] = fmaf(a[k], b, c);
+    }
+    float t = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) t += a[k];
+    if (t == 12345.0f) out[blockIdx.x] = t;  // keeps the chains live; never true for these inputs
+}
+
 __global__ __launch_bounds__(kBlock) void k_finite(Fields f, int32_t n, uint32_t *bad) {
     const int s = blockIdx.x * kBlock + threadIdx.x;
     if (s >= n || f.alive[s] != 1) return;
@@ -816,6 +835,11 @@ hipError_t launch_tick(CaseDesc *cases, int32_t n_cases, hipStream_t s) {
 
 hipError_t launch_stencil(Grid g, int32_t *count, int32_t *list, hipStream_t s) {
     hipLaunchKernelGGL(k_stencil, grid_for(g.n_cells), dim3(kBlock), 0, s, g, count, list);
+    return hipGetLastError();
+}
+
+hipError_t launch_valu_calibration(float *out, int32_t blocks, int32_t iters, hipStream_t s) {
+    hipLaunchKernelGGL(k_valu_calibration, dim3((unsigned)blocks), dim3(kBlock), 0, s, out, iters);
     return hipGetLastError();
 }
 

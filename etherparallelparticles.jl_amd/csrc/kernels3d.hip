@@ -266,47 +266,276 @@ __device__ __forceinline__ void momentum3(const Physics &ph, float4 &acc, bool c
 }
 
 // ------------------------------------------------------------ pair kernels
+// One wave per cell (four cells per 256-thread block, each wave on its own):
+// lane l holds the cell's l-th particle (cells of more than 64 particles take
+// their particles 64 at a time).  The particles of a cell share their 27
+// neighbour cells, so the wave stages each neighbour cell's records (geo,
+// vel, hmvp: 48 bytes a slot, read once and coalesced) in its own LDS area
+// and every lane tests and gathers from there: pass 1 reads one candidate for
+// all lanes at once (a broadcast), pass 2 each lane's own accepted
+// candidates.  Cells are walked in the stencil order and each cell's slots
+// in ascending particle index, the reference's list order.
+constexpr int kCellBlock = 64;     // lanes of a wave: particles of one cell at a time
+constexpr int kCellWaves = 4;      // waves (cells) per block
+
+// Orders a wave's LDS stores before its later LDS loads (the waves of a block
+// never share a staging area, so no block barrier is needed).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+struct Stage3 {
+    float4 g[kCellBlock];  // (x, y, z, tag bits) of the staged slots
+    float4 v[kCellBlock];  // (u, v, w, rho)
+    float4 q[kCellBlock];  // (H, Mass, Volume, Pressure)
+};
+
+// Adjacency of any tag present in cell c (a cell holding no particle of a tag
+// in it is skipped by every lane of the block).
+__device__ __forceinline__ uint32_t adj_of_cell(const Walk3 &w, uint32_t tags) {
+    uint32_t a = 0u;
+    while (tags) {
+        const int t = __builtin_ctz(tags);
+        tags &= tags - 1u;
+        a |= w.adj[t + 1];
+    }
+    return a;
+}
+
+// Pass 1 over the staged slots [0, len) against this lane's particle.
+template <bool CHECK, bool BAND>
+__device__ __forceinline__ unsigned long long accept_staged(const Walk3 &w, const Stage3 &st, float4 gi, float hi,
+                                                            uint32_t adj_i, int len) {
+    unsigned long long mask = 0ull;
+    for (int k = 0; k < len; k += 4) {
+        uint32_t nib = 0u;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const float4 gj = st.g[k + u];
+            const float dr2 = dist2(gi.x - gj.x, gi.y - gj.y, gi.z - gj.z);
+            bool ok = dr2 < w.rmin2;
+            if (BAND && !ok && dr2 <= w.rmax2) ok = criterion3(w, hi, st.q[k + u].x, dr2);
+            if (CHECK) ok = ok && (adj_i & tag_bit(__float_as_int(gj.w))) != 0u;
+            nib |= ok ? (1u << u) : 0u;
+        }
+        mask |= (unsigned long long)nib << k;
+    }
+    return len < 64 ? mask & ((1ull << len) - 1ull) : mask;
+}
+
+// The walk of one 64-particle chunk of cell c: visit(geo_j, vel_j, hmvp_j,
+// dx, dy, dz, dr2) for each accepted neighbour j.  `active`: this lane
+// holds a particle that takes pairs; every lane of the block calls it.
+template <bool CHECK, bool BAND, class Visit>
+__device__ __forceinline__ void cell_walk3(const Walk3 &w, Stage3 &st, int c, uint32_t adj_block, int i, bool active,
+                                           float4 gi, float hi, uint32_t adj_i, Visit &&visit) {
+    const Grid3 &g = w.g;
+    const int cx = c % g.n_x, t = c / g.n_x;
+    const int cy = t % g.n_y, cz = t / g.n_y;
+    const int lane = lane_id();
+    for (int di = -1; di <= 1; ++di) {
+        const int x = cx + di;
+        if (x < 0 || x >= g.n_x) continue;
+        for (int dj = -1; dj <= 1; ++dj) {
+            const int y = cy + dj;
+            if (y < 0 || y >= g.n_y) continue;
+            for (int dk = -1; dk <= 1; ++dk) {
+                const int z = cz + dk;
+                if (z < 0 || z >= g.n_z) continue;
+                const int nb = x + g.n_x * (y + g.n_y * z);
+                if (!(w.cell_tags[nb] & adj_block)) continue;
+                const int b = w.start[nb], e = w.start[nb + 1];
+                for (int base = b; base < e; base += kCellBlock) {
+                    const int len = min(kCellBlock, e - base);
+                    if (lane < len) {
+                        st.g[lane] = w.geo[base + lane];
+                        st.v[lane] = w.vel[base + lane];
+                        st.q[lane] = w.hmvp[base + lane];
+                    }
+                    wave_sync();
+                    if (active) {
+                        unsigned long long m = accept_staged<CHECK, BAND>(w, st, gi, hi, adj_i, len);
+                        const unsigned d = (unsigned)(i - base);
+                        if (d < 64u) m &= ~(1ull << d);
+                        while (m) {
+                            const int k = __ffsll((long long)m) - 1;
+                            m &= m - 1ull;
+                            // the three records at once: one LDS round trip per pair
+                            const float4 gj = st.g[k], vj = st.v[k], qj = st.q[k];
+                            const float dx = gi.x - gj.x, dy = gi.y - gj.y, dz = gi.z - gj.z;
+                            visit(gj, vj, qj, dx, dy, dz, dist2(dx, dy, dz));
+                        }
+                    }
+                    wave_sync();
+                }
+            }
+        }
+    }
+}
+
+// FAST walk of one 64-particle chunk of cell c: the neighbour cells in 14
+// groups -- the cell itself, then each offset together with its mirror (the
+// stencil cells s and 26 - s) -- with both cells of a group staged at once and
+// their accepted pairs walked in one loop.  A particle near one face of its
+// cell has many pairs in the cell beyond that face and few in the cell beyond
+// the opposite one, so a group's pair count varies much less across the wave
+// than a cell's: the wave runs about half the pass-2 iterations of the
+// cell-by-cell walk.  Sums then run in another order than the reference's
+// list order (FAST math only).
+template <bool CHECK, bool BAND, class Visit>
+__device__ __forceinline__ void cell_walk3_paired(const Walk3 &w, Stage3 *st2, int c, uint32_t adj_block, int i,
+                                                  bool active, float4 gi, float hi, uint32_t adj_i, Visit &&visit) {
+    const Grid3 &g = w.g;
+    const int cx = c % g.n_x, t = c / g.n_x;
+    const int cy = t % g.n_y, cz = t / g.n_y;
+    const int lane = lane_id();
+    // slot range of stencil cell s (x offset s / 9 - 1, y (s / 3) % 3 - 1, z s % 3 - 1), empty when the cell
+    // lies outside the grid or holds no particle of a tag adjacent to the wave's
+    auto range = [&](int sidx, int &b, int &e) {
+        const int x = cx + sidx / 9 - 1, y = cy + (sidx / 3) % 3 - 1, z = cz + sidx % 3 - 1;
+        b = e = 0;
+        if (x < 0 || x >= g.n_x || y < 0 || y >= g.n_y || z < 0 || z >= g.n_z) return;
+        const int nb = x + g.n_x * (y + g.n_y * z);
+        if (!(w.cell_tags[nb] & adj_block)) return;
+        b = w.start[nb];
+        e = w.start[nb + 1];
+    };
+    for (int grp = 0; grp < 14; ++grp) {
+        int ba, ea, bb, eb;
+        range(13 + grp, ba, ea);
+        if (grp) range(13 - grp, bb, eb);
+        else bb = eb = 0;
+        const int na = ea - ba, nbb = eb - bb;
+        const int chunks = (max(na, nbb) + kCellBlock - 1) / kCellBlock;
+        for (int ch = 0; ch < chunks; ++ch) {
+            const int base_a = ba + ch * kCellBlock, base_b = bb + ch * kCellBlock;
+            const int len_a = min(kCellBlock, max(0, ea - base_a)), len_b = min(kCellBlock, max(0, eb - base_b));
+            if (lane < len_a) {
+                st2[0].g[lane] = w.geo[base_a + lane];
+                st2[0].v[lane] = w.vel[base_a + lane];
+                st2[0].q[lane] = w.hmvp[base_a + lane];
+            }
+            if (lane < len_b) {
+                st2[1].g[lane] = w.geo[base_b + lane];
+                st2[1].v[lane] = w.vel[base_b + lane];
+                st2[1].q[lane] = w.hmvp[base_b + lane];
+            }
+            wave_sync();
+            if (active) {
+                unsigned long long ma = len_a ? accept_staged<CHECK, BAND>(w, st2[0], gi, hi, adj_i, len_a) : 0ull;
+                unsigned long long mb = len_b ? accept_staged<CHECK, BAND>(w, st2[1], gi, hi, adj_i, len_b) : 0ull;
+                const unsigned d = (unsigned)(i - base_a);
+                if (grp == 0 && d < 64u) ma &= ~(1ull << d);
+                while (ma | mb) {
+                    const bool in_a = ma != 0ull;
+                    const unsigned long long m = in_a ? ma : mb;
+                    const int k = __ffsll((long long)m) - 1;
+                    if (in_a) ma &= ma - 1ull;
+                    else mb &= mb - 1ull;
+                    const Stage3 &sx = st2[in_a ? 0 : 1];
+                    const float4 gj = sx.g[k], vj = sx.v[k], qj = sx.q[k];
+                    const float dx = gi.x - gj.x, dy = gi.y - gj.y, dz = gi.z - gj.z;
+                    visit(gj, vj, qj, dx, dy, dz, dist2(dx, dy, dz));
+                }
+            }
+            wave_sync();
+        }
+    }
+}
+
+template <class Visit>
+__device__ __forceinline__ void cell_walk3_any(const Walk3 &w, Stage3 &st, int c, uint32_t adj_block, int i,
+                                               bool active, float4 gi, float hi, uint32_t adj_i, bool check,
+                                               Visit &&visit) {
+    const bool band = w.rmin2 <= w.rmax2;
+    if (__ballot(check)) {
+        if (band) cell_walk3<true, true>(w, st, c, adj_block, i, active, gi, hi, adj_i, visit);
+        else cell_walk3<true, false>(w, st, c, adj_block, i, active, gi, hi, adj_i, visit);
+    } else {
+        if (band) cell_walk3<false, true>(w, st, c, adj_block, i, active, gi, hi, adj_i, visit);
+        else cell_walk3<false, false>(w, st, c, adj_block, i, active, gi, hi, adj_i, visit);
+    }
+}
+
+template <bool FAST, class Visit>
+__device__ __forceinline__ void cell_walk3_sel(const Walk3 &w, Stage3 *st2, int c, uint32_t adj_block, int i,
+                                               bool active, float4 gi, float hi, uint32_t adj_i, bool check,
+                                               Visit &&visit) {
+    if (!FAST) {
+        cell_walk3_any(w, st2[0], c, adj_block, i, active, gi, hi, adj_i, check, visit);
+        return;
+    }
+    const bool band = w.rmin2 <= w.rmax2;
+    if (__ballot(check)) {
+        if (band) cell_walk3_paired<true, true>(w, st2, c, adj_block, i, active, gi, hi, adj_i, visit);
+        else cell_walk3_paired<true, false>(w, st2, c, adj_block, i, active, gi, hi, adj_i, visit);
+    } else {
+        if (band) cell_walk3_paired<false, true>(w, st2, c, adj_block, i, active, gi, hi, adj_i, visit);
+        else cell_walk3_paired<false, false>(w, st2, c, adj_block, i, active, gi, hi, adj_i, visit);
+    }
+}
+
+
 // Search pass: the neighbours of every alive particle, counted into ncount,
 // with the fused action (EPPX_SEARCH_KERNEL_CONTINUITY:
 // sComputeKernelAndiContinuity!, main.jl:74-90, onto the existing dDensity).
 template <bool FAST, int ACTION, int K>
-__global__ __launch_bounds__(kBlock3) void k3_search(Walk3 w, Fields3 f, Physics ph, int32_t M, int32_t *ncount,
-                                                      unsigned long long *pairs, uint32_t *flags) {
-    __shared__ int lds[kBlock3 / 64];
-    const int i = xcd_block(blockIdx.x, gridDim.x) * kBlock3 + threadIdx.x;
-    const int n = w.n_alive[0];
-    int cnt = 0;
-    if (i < n) {
-        const float4 gi = w.geo[i];
-        const float4 vi = w.vel[i];
-        const float hi = w.hmvp[i].x;
-        const uint32_t tbi = tag_bit(__float_as_int(gi.w));
+__global__ __launch_bounds__(kCellBlock * kCellWaves) void k3_search(Walk3 w, Fields3 f, Physics ph, int32_t M,
+                                                                     int32_t *ncount, unsigned long long *pairs,
+                                                                     uint32_t *flags) {
+    __shared__ Stage3 stage[kCellWaves][2];
+    const int wave = threadIdx.x / kCellBlock;
+    Stage3 *st2 = stage[wave];
+    const int c = blockIdx.x * kCellWaves + wave;  // round robin over the XCDs: fluid fills only part of the grid
+    if (c >= w.g.n_cells) return;
+    const int c0 = w.start[c], c1 = w.start[c + 1];
+    if (c0 >= c1) return;
+    const uint32_t cell_tags = w.cell_tags[c];
+    const uint32_t adj_block = adj_of_cell(w, cell_tags);
+    const uint32_t fluid_bit = tag_bit(ph.fluid_tag), wall_bit = tag_bit(ph.wall_tag);
+    int total = 0;
+    for (int i0 = c0; i0 < c1; i0 += kCellBlock) {
+        const int i = i0 + lane_id();
+        const bool active = i < c1;
+        int cnt = 0;
+        float4 gi = make_float4(0.0f, 0.0f, 0.0f, 0.0f), vi = gi;
+        float hi = 0.0f, drho = 0.0f;
+        uint32_t tbi = 0u;
+        if (active) {
+            gi = w.geo[i];
+            vi = w.vel[i];
+            hi = w.hmvp[i].x;
+            tbi = tag_bit(__float_as_int(gi.w));
+            if (ACTION == EPPX_SEARCH_KERNEL_CONTINUITY) drho = f.drho[i];
+        }
         const uint32_t adj_i = adj3(w, tbi);
-        const uint32_t fluid_bit = tag_bit(ph.fluid_tag), wall_bit = tag_bit(ph.wall_tag);
-        const bool fluid_i = tbi == fluid_bit, wall_i = tbi == wall_bit;
-        const bool check = w.odd_tags || (adj_i & w.tags_present) != w.tags_present;
-        float drho = ACTION == EPPX_SEARCH_KERNEL_CONTINUITY ? f.drho[i] : 0.0f;
-        walk3_any(w, i, gi, hi, adj_i, check, [&](int j, float4 gj, float dx, float dy, float dz, float dr2) {
-            ++cnt;
-            if (ACTION == EPPX_SEARCH_KERNEL_CONTINUITY) {
-                const uint32_t tbj = tag_bit(__float_as_int(gj.w));
-                const bool classic = fluid_i && tbj == fluid_bit;
-                const bool balanced = (fluid_i && tbj == wall_bit) || (wall_i && tbj == fluid_bit);
-                if (classic || balanced) {
-                    const float4 vj = ld_slot(w.vel, j);
-                    const float4 qj = ld_slot(w.hmvp, j);
-                    const float hinv = w.hinv_u > 0.0f ? w.hinv_u : invharmonic(hi, qj.x);
-                    const float coef = classic ? qj.y : vi.w * qj.z;
-                    drho += continuity3<FAST, K>(hinv, vi, vj, coef, dx, dy, dz, dr2);
-                }
-            }
-        });
-        ncount[i] = cnt;
-        if (cnt > M) atomicOr(flags, kFlagNeighbourOverflow);
-        if (ACTION == EPPX_SEARCH_KERNEL_CONTINUITY) f.drho[i] = drho;
+        const bool fluid_i = tbi && tbi == fluid_bit, wall_i = tbi && tbi == wall_bit;
+        const bool check = active && (w.odd_tags || (adj_i & w.tags_present) != w.tags_present);
+        cell_walk3_sel<FAST>(w, st2, c, adj_block, i, active && adj_i != 0u, gi, hi, adj_i, check,
+                       [&](float4 gj, float4 vj, float4 qj, float dx, float dy, float dz, float dr2) {
+                           ++cnt;
+                           if (ACTION == EPPX_SEARCH_KERNEL_CONTINUITY) {
+                               const uint32_t tbj = tag_bit(__float_as_int(gj.w));
+                               const bool classic = fluid_i && tbj == fluid_bit;
+                               const bool balanced = (fluid_i && tbj == wall_bit) || (wall_i && tbj == fluid_bit);
+                               if (classic || balanced) {
+                                   const float hinv = w.hinv_u > 0.0f ? w.hinv_u : invharmonic(hi, qj.x);
+                                   const float coef = classic ? qj.y : vi.w * qj.z;
+                                   drho += continuity3<FAST, K>(hinv, vi, vj, coef, dx, dy, dz, dr2);
+                               }
+                           }
+                       });
+        if (active) {
+            ncount[i] = cnt;
+            if (cnt > M) atomicOr(flags, kFlagNeighbourOverflow);
+            if (ACTION == EPPX_SEARCH_KERNEL_CONTINUITY) f.drho[i] = drho;
+        }
+        total += cnt;
     }
-    const int total = block_sum_i32(cnt, lds);
-    if (threadIdx.x == 0 && total) {
+    for (int off = 32; off > 0; off >>= 1) total += __shfl_down(total, off);
+    if (lane_id() == 0 && total) {
         atomicAdd(&pairs[0], (unsigned long long)total);
         atomicAdd(&pairs[1], (unsigned long long)total);
     }
@@ -314,32 +543,44 @@ __global__ __launch_bounds__(kBlock3) void k3_search(Walk3 w, Fields3 f, Physics
 
 // Momentum pass (iMomentum!, main.jl:107-118): fluid particles add the
 // pressure and viscosity terms of their fluid and wall neighbours to du;
-// other particles take no terms.
+// other particles take no terms (cells without fluid have no work).
 template <bool FAST, int K, int VISC>
-__global__ __launch_bounds__(kBlock3) void k3_momentum(Walk3 w, Fields3 f, Physics ph) {
-    const int i = xcd_block(blockIdx.x, gridDim.x) * kBlock3 + threadIdx.x;
-    if (i >= w.n_alive[0]) return;
-    const float4 gi = w.geo[i];
-    const uint32_t tbi = tag_bit(__float_as_int(gi.w));
+__global__ __launch_bounds__(kCellBlock * kCellWaves) void k3_momentum(Walk3 w, Fields3 f, Physics ph) {
+    __shared__ Stage3 stage[kCellWaves][2];
+    const int wave = threadIdx.x / kCellBlock;
+    Stage3 *st2 = stage[wave];
+    const int c = blockIdx.x * kCellWaves + wave;  // round robin over the XCDs: fluid fills only part of the grid
+    if (c >= w.g.n_cells) return;
+    const int c0 = w.start[c], c1 = w.start[c + 1];
     const uint32_t fluid_bit = tag_bit(ph.fluid_tag), wall_bit = tag_bit(ph.wall_tag);
-    if (!tbi || tbi != fluid_bit) return;
-    const float4 vi = w.vel[i];
-    const float4 qi = w.hmvp[i];
-    const float p_i = qi.w;
-    const float pr_i = FAST ? p_i * frcp(vi.w * vi.w) : p_i / (vi.w * vi.w);
-    const uint32_t adj_i = adj3(w, tbi);
-    const bool check = w.odd_tags || (adj_i & w.tags_present) != w.tags_present;
-    float4 acc = f.dvel[i];
-    walk3_any(w, i, gi, qi.x, adj_i, check, [&](int j, float4 gj, float dx, float dy, float dz, float dr2) {
-        const uint32_t tbj = tag_bit(__float_as_int(gj.w));
-        const bool classic = tbj == fluid_bit;
-        if (!classic && tbj != wall_bit) return;
-        const float4 vj = ld_slot(w.vel, j);
-        const float4 qj = ld_slot(w.hmvp, j);
-        const float hinv = w.hinv_u > 0.0f ? w.hinv_u : invharmonic(qi.x, qj.x);
-        momentum3<FAST, K, VISC>(ph, acc, classic, hinv, vi, p_i, pr_i, vj, qj, dx, dy, dz, dr2);
-    });
-    f.dvel[i] = acc;
+    if (c0 >= c1 || !(w.cell_tags[c] & fluid_bit)) return;
+    const uint32_t adj_block = adj3(w, fluid_bit);
+    for (int i0 = c0; i0 < c1; i0 += kCellBlock) {
+        const int i = i0 + lane_id();
+        bool active = i < c1;
+        float4 gi = make_float4(0.0f, 0.0f, 0.0f, 0.0f), vi = gi, qi = gi, acc = gi;
+        if (active) {
+            gi = w.geo[i];
+            active = tag_bit(__float_as_int(gi.w)) == fluid_bit;
+        }
+        if (active) {
+            vi = w.vel[i];
+            qi = w.hmvp[i];
+            acc = f.dvel[i];
+        }
+        const float p_i = qi.w;
+        const float pr_i = FAST ? p_i * frcp(vi.w * vi.w) : p_i / (vi.w * vi.w);
+        const bool check = active && (w.odd_tags || (adj_block & w.tags_present) != w.tags_present);
+        cell_walk3_sel<FAST>(w, st2, c, adj_block, i, active, gi, qi.x, adj_block, check,
+                       [&](float4 gj, float4 vj, float4 qj, float dx, float dy, float dz, float dr2) {
+                           const uint32_t tbj = tag_bit(__float_as_int(gj.w));
+                           const bool classic = tbj == fluid_bit;
+                           if (!classic && tbj != wall_bit) return;
+                           const float hinv = w.hinv_u > 0.0f ? w.hinv_u : invharmonic(qi.x, qj.x);
+                           momentum3<FAST, K, VISC>(ph, acc, classic, hinv, vi, p_i, pr_i, vj, qj, dx, dy, dz, dr2);
+                       });
+        if (active) f.dvel[i] = acc;
+    }
 }
 
 // Neighbour lists of the last search as a CSR by slot (row_start) and / or
@@ -601,23 +842,25 @@ inline dim3 grid3_for(int64_t n) {
 template <bool FAST, int K>
 hipError_t search_as(const Walk3 &w, Fields3 f, int32_t action, const Physics &ph, int32_t M, int32_t *ncount,
                      unsigned long long *pairs, uint32_t *flags, int32_t n_slots, hipStream_t s) {
+    const dim3 grid((unsigned)std::max((w.g.n_cells + kCellWaves - 1) / kCellWaves, 1));
+    const dim3 block(kCellBlock * kCellWaves);
     if (action == EPPX_SEARCH_KERNEL_CONTINUITY)
-        hipLaunchKernelGGL((k3_search<FAST, EPPX_SEARCH_KERNEL_CONTINUITY, K>), grid3_for(n_slots), dim3(kBlock3), 0,
-                           s, w, f, ph, M, ncount, pairs, flags);
+        hipLaunchKernelGGL((k3_search<FAST, EPPX_SEARCH_KERNEL_CONTINUITY, K>), grid, block, 0, s, w, f, ph, M, ncount,
+                           pairs, flags);
     else
-        hipLaunchKernelGGL((k3_search<false, EPPX_SEARCH_NONE, EPPX_KERNEL_CUBIC>), grid3_for(n_slots),
-                           dim3(kBlock3), 0, s, w, f, ph, M, ncount, pairs, flags);
+        hipLaunchKernelGGL((k3_search<false, EPPX_SEARCH_NONE, EPPX_KERNEL_CUBIC>), grid, block, 0, s, w, f, ph, M,
+                           ncount, pairs, flags);
     return hipGetLastError();
 }
 
 template <bool FAST, int K>
 hipError_t momentum_as(const Walk3 &w, Fields3 f, const Physics &ph, int32_t n_slots, hipStream_t s) {
+    const dim3 grid((unsigned)std::max((w.g.n_cells + kCellWaves - 1) / kCellWaves, 1));
+    const dim3 block(kCellBlock * kCellWaves);
     if (ph.visc == EPPX_VISC_CLASSIC)
-        hipLaunchKernelGGL((k3_momentum<FAST, K, EPPX_VISC_CLASSIC>), grid3_for(n_slots), dim3(kBlock3), 0, s, w, f,
-                           ph);
+        hipLaunchKernelGGL((k3_momentum<FAST, K, EPPX_VISC_CLASSIC>), grid, block, 0, s, w, f, ph);
     else
-        hipLaunchKernelGGL((k3_momentum<FAST, K, EPPX_VISC_ARTIFICIAL>), grid3_for(n_slots), dim3(kBlock3), 0, s, w,
-                           f, ph);
+        hipLaunchKernelGGL((k3_momentum<FAST, K, EPPX_VISC_ARTIFICIAL>), grid, block, 0, s, w, f, ph);
     return hipGetLastError();
 }
 

@@ -121,6 +121,7 @@ def lib() -> C.CDLL:
             "eppx_pair_total": ([vp, P(i64), i32], i32),
             "eppx_export_pairs": ([vp, vp, P(i64), P(i32), i64], i32),
             "eppx_check_finite": ([vp], i32),
+            "eppx_valu_calibration": ([vp, i32, i32, P(C.c_double)], i32),
             "eppx_pair_fallback_blocks": ([vp, P(i64), i32], i32),
             "eppx_pair_row_fallback": ([vp, P(i64), i32], i32),
             "eppx_set_timing": ([vp, i32], i32),

@@ -255,6 +255,12 @@ int32_t eppx_pair_fallback_blocks(eppx_ps *ps, int64_t *out, int32_t reset);
  * (cells much larger than 2h, or crowded cells; a performance diagnostic,
  * results do not depend on it). */
 int32_t eppx_pair_row_fallback(eppx_ps *ps, int64_t *out, int32_t reset);
+/* Profiler calibration (no counterpart in the reference): runs `blocks`
+ * blocks of 256 threads, each lane issuing iters x 8 independent FP32 FMAs
+ * (8 x iters VALU instructions per wave plus loop control), once to warm up
+ * and once timed; us receives the timed launch's duration (waits).  The
+ * known instruction count scales rocprofv3's VALU counters. */
+int32_t eppx_valu_calibration(eppx_ctx *ctx, int32_t blocks, int32_t iters, double *us);
 /* Returns EPPX_E_NONFINITE when a live particle's density, pressure,
  * position or velocity is not finite (waits). */
 int32_t eppx_check_finite(eppx_ps *ps);
