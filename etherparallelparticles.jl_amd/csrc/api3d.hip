@@ -171,6 +171,9 @@ Walk3 make_walk3(const eppx_ps3d *ps, const eppx_ns3d *ns, int32_t criterion) {
         w.rmax2 = ps->rmax2;
     }
     w.hinv_u = ps->hinv_u;
+    const int32_t groups = (ns->grid.n_cells + 3) / 4;  // 4 cells per pair block
+    w.stripe = 8;                                       // 32 consecutive cells per stripe
+    w.stripes_per_xcd = std::max(1, (groups + 8 * w.stripe - 1) / (8 * w.stripe));
     return w;
 }
 

@@ -278,6 +278,19 @@ __device__ __forceinline__ void momentum3(const Physics &ph, float4 &acc, bool c
 constexpr int kCellBlock = 64;     // lanes of a wave: particles of one cell at a time
 constexpr int kCellWaves = 4;      // waves (cells) per block
 
+// Block -> group of kCellWaves consecutive cells.  The groups are cut into
+// stripes of `stripe` consecutive groups; the dispatcher hands block b to XCD
+// b % 8, and XCD x works through stripes x, x + 8, x + 16, ... in order.  The
+// eight XCDs thus work on neighbouring stripes at any time, which keeps them
+// equally loaded where the particles fill only part of the grid (one
+// contiguous range per XCD would leave XCDs idle), and each stripe's cells
+// share most of their neighbour cells in that XCD's L2.  The grid is
+// 8 x stripes_per_xcd x stripe blocks; groups past the last cell have no work.
+__device__ __forceinline__ int stripe_group(int b, int stripe) {
+    const int x = b % kXcds, l = b / kXcds;
+    return (x + kXcds * (l / stripe)) * stripe + l % stripe;
+}
+
 // Orders a wave's LDS stores before its later LDS loads (the waves of a block
 // never share a staging area, so no block barrier is needed).
 __device__ __forceinline__ void wave_sync() {
@@ -488,7 +501,7 @@ __global__ __launch_bounds__(kCellBlock * kCellWaves) void k3_search(Walk3 w, Fi
     __shared__ Stage3 stage[kCellWaves][2];
     const int wave = threadIdx.x / kCellBlock;
     Stage3 *st2 = stage[wave];
-    const int c = blockIdx.x * kCellWaves + wave;  // round robin over the XCDs: fluid fills only part of the grid
+    const int c = stripe_group(blockIdx.x, w.stripe) * kCellWaves + wave;
     if (c >= w.g.n_cells) return;
     const int c0 = w.start[c], c1 = w.start[c + 1];
     if (c0 >= c1) return;
@@ -549,7 +562,7 @@ __global__ __launch_bounds__(kCellBlock * kCellWaves) void k3_momentum(Walk3 w, 
     __shared__ Stage3 stage[kCellWaves][2];
     const int wave = threadIdx.x / kCellBlock;
     Stage3 *st2 = stage[wave];
-    const int c = blockIdx.x * kCellWaves + wave;  // round robin over the XCDs: fluid fills only part of the grid
+    const int c = stripe_group(blockIdx.x, w.stripe) * kCellWaves + wave;
     if (c >= w.g.n_cells) return;
     const int c0 = w.start[c], c1 = w.start[c + 1];
     const uint32_t fluid_bit = tag_bit(ph.fluid_tag), wall_bit = tag_bit(ph.wall_tag);
@@ -842,7 +855,7 @@ inline dim3 grid3_for(int64_t n) {
 template <bool FAST, int K>
 hipError_t search_as(const Walk3 &w, Fields3 f, int32_t action, const Physics &ph, int32_t M, int32_t *ncount,
                      unsigned long long *pairs, uint32_t *flags, int32_t n_slots, hipStream_t s) {
-    const dim3 grid((unsigned)std::max((w.g.n_cells + kCellWaves - 1) / kCellWaves, 1));
+    const dim3 grid((unsigned)(kXcds * w.stripes_per_xcd * w.stripe));
     const dim3 block(kCellBlock * kCellWaves);
     if (action == EPPX_SEARCH_KERNEL_CONTINUITY)
         hipLaunchKernelGGL((k3_search<FAST, EPPX_SEARCH_KERNEL_CONTINUITY, K>), grid, block, 0, s, w, f, ph, M, ncount,
@@ -855,7 +868,7 @@ hipError_t search_as(const Walk3 &w, Fields3 f, int32_t action, const Physics &p
 
 template <bool FAST, int K>
 hipError_t momentum_as(const Walk3 &w, Fields3 f, const Physics &ph, int32_t n_slots, hipStream_t s) {
-    const dim3 grid((unsigned)std::max((w.g.n_cells + kCellWaves - 1) / kCellWaves, 1));
+    const dim3 grid((unsigned)(kXcds * w.stripes_per_xcd * w.stripe));
     const dim3 block(kCellBlock * kCellWaves);
     if (ph.visc == EPPX_VISC_CLASSIC)
         hipLaunchKernelGGL((k3_momentum<FAST, K, EPPX_VISC_CLASSIC>), grid, block, 0, s, w, f, ph);
