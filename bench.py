@@ -188,6 +188,25 @@ def valu_calibration(blocks: int = 8192, iters: int = 20000) -> dict:
     return out
 
 
+def valu_issue_rate(par, blocks: int = 8192, iters: int = 2000) -> float:
+    """Wave-level VALU instructions per second the calibration kernel reaches on this GPU now (the 100 % mark of
+    roofline.valu_busy): 8 x iters FP32 FMAs per lane over blocks x 4 waves, timed with HIP events."""
+    from eppx import _lib
+    us = C.c_double()
+    _lib.check(_lib.lib().eppx_valu_calibration(par.ctx, blocks, iters, C.byref(us)), par.ctx, "valu_calibration")
+    return blocks * 4 * (8 * iters + 33) / (us.value * 1e-6)
+
+
+def profile_record(kernel: str, workload: str, math: str):
+    """HBM bytes and VALU instructions per launch of a kernel family, from the rocprofv3 counter runs committed
+    under profiles/ (counters cannot be read inside this run); the record names the commit it was taken at."""
+    path = os.path.join(REPO, "profiles", f"traffic_{kernel}_{workload}_{math}.json")
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        return json.load(f)
+
+
 def _warm_up(par):
     """Loads the kernels and the graph machinery once (a tiny batch and a tiny case)."""
     from eppx import Algorithm, Batch, Class, SPH, cases
@@ -391,16 +410,22 @@ def run_3d(args, par):
     avg_us = kt[dominant][0]
     achieved = alg_bytes / (avg_us * 1e-6) / 1e9
     device_us_per_step = sum(v[0] * v[1] for v in kt.values()) / args.profile_steps_3d
+    prof = profile_record(dominant, "3d", args.math)
+    rate = valu_issue_rate(par)
+    valu = prof["valu_insts_per_launch"] / (avg_us * 1e-6 * rate) if prof.get("valu_insts_per_launch") else None
+    frac = achieved / HBM_PEAK_GBS
+    limiter = None if valu is None else ("valu_issue" if valu >= 0.6 else ("hbm" if frac >= 0.6 else "latency"))
     cpu = None
     if not args.no_cpu:
         threads = len(os.sched_getaffinity(0))
         threads = min(threads, int(os.environ.get("OMP_NUM_THREADS", threads)))
+        host = cpu_info(threads)  # before OpenMP binds this thread to one core
         rate, secs, nsteps, n_cpu = cpu_baseline_3d(args.cpu_seconds, threads)
         cpu = {"value": rate, "unit": "pair-interactions/s", "cores": threads, "kind": "port",
                "sample": f"{nsteps} steps of the same 3D dam break geometry at dx = 0.015 ({n_cpu} particles, the "
                          f"same h / dx and neighbours per particle) after one search, C/OpenMP 3D restatement on "
                          f"the reference layout (per-cell atomic insert, materialised M = 300 lists), {secs:.1f} s",
-               "host": cpu_info(threads)}
+               "host": host}
         log(f"cpu 3D baseline: {rate / 1e9:.3f} G/s on {threads} threads")
     ps.close()
     ns.close()
@@ -417,8 +442,9 @@ def run_3d(args, par):
                    "math": args.math, "pairs_per_step": pairs / steps,
                    "parity": "3D restatement (the reference's 3D path is a stub: unpinned by reference)"},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "alg_bytes_per_launch": alg_bytes,
-                     "avg_launch_us": avg_us},
+                     "frac": frac, "traffic": prof.get("hbm_bytes_per_launch"), "alg_bytes_per_launch": alg_bytes,
+                     "avg_launch_us": avg_us, "valu_busy": valu, "valu_issue_rate": rate, "limiter": limiter,
+                     "counters_commit": prof.get("commit")},
         "device_us_per_step": device_us_per_step,
         "kernels_us": {k: round(v[0], 2) for k, v in kt.items()},
         "cpu_baseline": cpu,
@@ -558,16 +584,13 @@ def main():
     # (x, u, rho, p, m, tag, h in; x, u out), plus the 12 B/cell of the cell table
     step_bytes = 108 * case.n_particles + 12 * case.domain.get_n()
 
-    # HBM bytes per launch and VALU busy of the dominant kernel, from the rocprofv3 counter runs committed under
-    # profiles/ (counters cannot be read inside this run); the file names the commit it was taken at
-    traffic = valu = prof_commit = None
-    tpath = os.path.join(REPO, "profiles", f"traffic_{dominant}_{args.workload}_{args.math}.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            prof = json.load(f)
-        traffic = prof.get("hbm_bytes_per_launch")
-        valu = prof.get("valu_busy")
-        prof_commit = prof.get("commit")
+    # HBM bytes per launch of the dominant kernel and its VALU instructions per launch, from the rocprofv3
+    # counter runs committed under profiles/; VALU busy = those instructions over this run's launch duration
+    # times the issue rate the calibration kernel reaches now
+    prof = profile_record(dominant, args.workload, args.math)
+    traffic, prof_commit = prof.get("hbm_bytes_per_launch"), prof.get("commit")
+    rate = valu_issue_rate(par)
+    valu = prof["valu_insts_per_launch"] / (avg_us * 1e-6 * rate) if prof.get("valu_insts_per_launch") else None
     frac = achieved / HBM_PEAK_GBS
     limiter = None
     if valu is not None:
@@ -577,12 +600,13 @@ def main():
     if not args.no_cpu:
         threads = len(os.sched_getaffinity(0))
         threads = min(threads, int(os.environ.get("OMP_NUM_THREADS", threads)))
+        host = cpu_info(threads)  # before OpenMP binds this thread to one core
         rate, secs, nsteps = cpu_baseline(case, args.cpu_seconds, threads)
         cpu = {"value": rate, "unit": "pair-interactions/s", "cores": threads, "kind": "port",
                "sample": f"{nsteps} steps of the same {case.n_particles}-particle dam break after one "
                          f"warm-up step, C/OpenMP restatement of the reference algorithm on the reference layout "
                          f"(per-cell atomic insert, materialised M=64 neighbour lists), {secs:.1f} s"}
-        cpu["host"] = cpu_info(threads)
+        cpu["host"] = host
         if args.cpu_seconds_1t > 0 and threads > 1:
             rate1, secs1, n1 = cpu_baseline(case, args.cpu_seconds_1t, 1)
             cpu["single_thread"] = {"value": rate1, "cores": 1,
@@ -616,7 +640,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": frac, "traffic": traffic,
                      "alg_bytes_per_launch": alg_bytes, "avg_launch_us": avg_us, "valu_busy": valu,
-                     "limiter": limiter, "counters_commit": prof_commit},
+                     "valu_issue_rate": rate, "limiter": limiter, "counters_commit": prof_commit},
         "step_roofline": {"alg_bytes_per_step": step_bytes, "device_us_per_step": device_us_per_step,
                           "hbm_frac": step_bytes / (device_us_per_step * 1e-6) / 1e9 / HBM_PEAK_GBS},
         "kernels_us": {k: round(v[0], 2) for k, v in kt.items()},
