@@ -190,11 +190,18 @@ def valu_calibration(blocks: int = 8192, iters: int = 20000) -> dict:
 
 def valu_issue_rate(par, blocks: int = 8192, iters: int = 2000) -> float:
     """Wave-level VALU instructions per second the calibration kernel reaches on this GPU now (the 100 % mark of
-    roofline.valu_busy): 8 x iters FP32 FMAs per lane over blocks x 4 waves, timed with HIP events."""
+    roofline.valu_busy): 8 x iters FP32 FMAs per lane over blocks x 4 waves, timed with HIP events; the median
+    of five launches, since the clock the chip holds moves by some 10 % between launches."""
     from eppx import _lib
-    us = C.c_double()
-    _lib.check(_lib.lib().eppx_valu_calibration(par.ctx, blocks, iters, C.byref(us)), par.ctx, "valu_calibration")
-    return blocks * 4 * (8 * iters + 33) / (us.value * 1e-6)
+    times = []
+    for _ in range(5):
+        us = C.c_double()
+        _lib.check(_lib.lib().eppx_valu_calibration(par.ctx, blocks, iters, C.byref(us)), par.ctx,
+                   "valu_calibration")
+        times.append(us.value)
+    med = sorted(times)[len(times) // 2]
+    log("VALU calibration kernel (us): " + ", ".join(f"{t:.1f}" for t in times))
+    return blocks * 4 * (8 * iters + 33) / (med * 1e-6)
 
 
 def profile_record(kernel: str, workload: str, math: str):
@@ -411,8 +418,8 @@ def run_3d(args, par):
     achieved = alg_bytes / (avg_us * 1e-6) / 1e9
     device_us_per_step = sum(v[0] * v[1] for v in kt.values()) / args.profile_steps_3d
     prof = profile_record(dominant, "3d", args.math)
-    rate = valu_issue_rate(par)
-    valu = prof["valu_insts_per_launch"] / (avg_us * 1e-6 * rate) if prof.get("valu_insts_per_launch") else None
+    issue_rate = valu_issue_rate(par)
+    valu = prof["valu_insts_per_launch"] / (avg_us * 1e-6 * issue_rate) if prof.get("valu_insts_per_launch") else None
     frac = achieved / HBM_PEAK_GBS
     limiter = None if valu is None else ("valu_issue" if valu >= 0.6 else ("hbm" if frac >= 0.6 else "latency"))
     cpu = None
@@ -443,7 +450,7 @@ def run_3d(args, par):
                    "parity": "3D restatement (the reference's 3D path is a stub: unpinned by reference)"},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": frac, "traffic": prof.get("hbm_bytes_per_launch"), "alg_bytes_per_launch": alg_bytes,
-                     "avg_launch_us": avg_us, "valu_busy": valu, "valu_issue_rate": rate, "limiter": limiter,
+                     "avg_launch_us": avg_us, "valu_busy": valu, "valu_issue_rate": issue_rate, "limiter": limiter,
                      "counters_commit": prof.get("commit")},
         "device_us_per_step": device_us_per_step,
         "kernels_us": {k: round(v[0], 2) for k, v in kt.items()},
@@ -589,8 +596,8 @@ def main():
     # times the issue rate the calibration kernel reaches now
     prof = profile_record(dominant, args.workload, args.math)
     traffic, prof_commit = prof.get("hbm_bytes_per_launch"), prof.get("commit")
-    rate = valu_issue_rate(par)
-    valu = prof["valu_insts_per_launch"] / (avg_us * 1e-6 * rate) if prof.get("valu_insts_per_launch") else None
+    issue_rate = valu_issue_rate(par)
+    valu = prof["valu_insts_per_launch"] / (avg_us * 1e-6 * issue_rate) if prof.get("valu_insts_per_launch") else None
     frac = achieved / HBM_PEAK_GBS
     limiter = None
     if valu is not None:
@@ -640,7 +647,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": frac, "traffic": traffic,
                      "alg_bytes_per_launch": alg_bytes, "avg_launch_us": avg_us, "valu_busy": valu,
-                     "valu_issue_rate": rate, "limiter": limiter, "counters_commit": prof_commit},
+                     "valu_issue_rate": issue_rate, "limiter": limiter, "counters_commit": prof_commit},
         "step_roofline": {"alg_bytes_per_step": step_bytes, "device_us_per_step": device_us_per_step,
                           "hbm_frac": step_bytes / (device_us_per_step * 1e-6) / 1e9 / HBM_PEAK_GBS},
         "kernels_us": {k: round(v[0], 2) for k, v in kt.items()},
