@@ -70,7 +70,7 @@ class Physics:
 
 def dam_break_physics(dx: float, h_factor: float, kernel=WendlandC2, g: float = 9.81, column_height: float = 2.0,
                       rho0: float = 1000.0, gamma: int = 7, alpha: float = 0.1, beta: float = 0.1) -> Physics:
-    """BASELINE weakly compressible dam break: Tait EOS (rho0, gamma = 7),
+    """The benchmark dam break, weakly compressible: Tait EOS (rho0, gamma = 7),
     c0 = 10 sqrt(2 g H), B = rho0 c0^2 / gamma, artificial viscosity
     (alpha, beta, c = c0), dt = 0.1 h / c0; all in Float64, then Float32."""
     c0 = 10.0 * math.sqrt(2.0 * g * column_height)

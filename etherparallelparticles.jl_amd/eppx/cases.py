@@ -210,7 +210,7 @@ def serving_stream(seed: int = 1, n_requests: int = 32, rate_hz: float = 4.0, n_
 def serving_requests(seed: int = 1, n_requests: int = 32, rate_hz: float = 4.0, n_min: float = 1e4,
                      n_max: float = 1e6, steps_min: int = 200, steps_max: int = 2000, M: int = 1):
     """The serving stream as Serve.Request objects: each a 1 x 2 m column dam
-    break at its dx (Wendland C2, h = 2dx, the BASELINE physics)."""
+    break at its dx (Wendland C2, h = 2dx, dam_break_physics)."""
     from .Serve import Request
     out = []
     for t, _, dx, steps in serving_stream(seed, n_requests, rate_hz, n_min, n_max, steps_min, steps_max):

@@ -7,7 +7,7 @@ dam-break loop with the Shepard filter every 20 steps and an output every
     python etherparallelparticles.jl_amd/examples/collapse_dry.py main OUT [--steps N]
 
 The physics constants live in the driver, as in main.jl (they are not part of
-the configuration).  --physics baseline (the default): Tait EOS, artificial
+the configuration).  --physics benchmark (the default): Tait EOS, artificial
 viscosity, dt = 0.1 h / c0, g = 9.81.  --physics example: main.jl's own
 constants (main.jl:57-70): linear EOS with c0 = 120, classic viscosity with
 mu0 = 1e-3, g = 9.8, dt = 0.2 h0 / c0.  Both use the kernel and h of the case.
@@ -28,13 +28,13 @@ KERNELS = {"cubic": SPH.CubicSpline, "wendland": SPH.WendlandC2}
 
 def driver_physics(drv: dict) -> SPH.Physics:
     """The step's constants: main.jl's own (physics "example") or the benchmark's (the default)."""
-    if drv.get("physics", "baseline") == "example":
+    if drv.get("physics", "benchmark") == "example":
         return SPH.example_physics(drv["dx"], drv["h_factor"], KERNELS[drv["kernel"]])
     return SPH.dam_break_physics(drv["dx"], drv["h_factor"], KERNELS[drv["kernel"]])
 
 
 def casegen(out: str, dx: float = 0.02, h_factor: float = 1.5, kernel: str = "cubic", M: int = 40,
-            physics: str = "baseline") -> str:
+            physics: str = "benchmark") -> str:
     """casegen.jl: the dam break as a configuration (config/config.json) and
     the raw file of write step 0.  Returns the configuration's path."""
     case = cases.dam_break(dx, h_factor, KERNELS[kernel], M=M)
@@ -91,7 +91,7 @@ if __name__ == "__main__":
     ap.add_argument("--dx", type=float, default=0.02)
     ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--math", choices=["exact", "fast"], default="exact")
-    ap.add_argument("--physics", choices=["baseline", "example"], default="baseline",
+    ap.add_argument("--physics", choices=["benchmark", "example"], default="benchmark",
                     help="casegen: the step's constants (the benchmark's, or main.jl's own)")
     a = ap.parse_args()
     if a.what == "casegen":

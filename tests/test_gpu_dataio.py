@@ -47,7 +47,7 @@ def test_config_driven_case_matches_direct(parallel, small, tmp_path):
     U.assert_state_equal(b, ref)
 
 
-@pytest.mark.parametrize("physics", ["baseline", "example"])
+@pytest.mark.parametrize("physics", ["benchmark", "example"])
 def test_collapse_dry_driver_matches_direct_run(parallel, tmp_path, physics):
     """examples/collapse_dry.py (casegen, then main with the filter every 20
     steps and an output every 40), with the benchmark's constants or
