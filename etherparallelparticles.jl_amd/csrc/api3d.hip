@@ -49,6 +49,9 @@ struct eppx_ps3d {
     eppx_ns3d *last_ns = nullptr;
     int32_t last_crit = 0, last_action = 0, last_kernel = 0, last_fluid = 0, last_wall = 0;
     bool searched = false;
+    // the last search's accept masks, 27 per slot (allocated by the first search)
+    unsigned long long *masks = nullptr;
+    bool masks_valid = false;
     std::vector<int32_t> pass_int_cols, pass_float_cols;
     std::vector<int32_t> pass_int;
     std::vector<float> pass_float;
@@ -171,6 +174,9 @@ Walk3 make_walk3(const eppx_ps3d *ps, const eppx_ns3d *ns, int32_t criterion) {
         w.rmax2 = ps->rmax2;
     }
     w.hinv_u = ps->hinv_u;
+    w.masks = ps->masks;
+    w.mask_stride = ps->cap;
+    w.mask_mode = 0;
     const int32_t groups = (ns->grid.n_cells + 3) / 4;  // 4 cells per pair block
     w.stripe = 8;                                       // 32 consecutive cells per stripe
     w.stripes_per_xcd = std::max(1, (groups + 8 * w.stripe - 1) / (8 * w.stripe));
@@ -204,7 +210,10 @@ int32_t do_search3(eppx_ps3d *ps, eppx_ns3d *ns, int32_t criterion, int32_t acti
                return e;
            }));
     ps->cur ^= 1;
-    const Walk3 w = make_walk3(ps, ns, criterion);
+    if (!ps->masks) HIP_OK(ctx, dalloc(&ps->masks, (size_t)27 * (size_t)ps->cap));
+    Walk3 w = make_walk3(ps, ns, criterion);
+    w.mask_mode = 1;
+    ps->masks_valid = false;
     HIP_OK(ctx, timed(ctx, "search", [&] {
                return launch3_search(w, ps->f(), action, ph, ctx->math_mode == EPPX_MATH_FAST, ns->M, ps->ncount,
                                      ps->pairs, ctx->d_flags, ps->n, s);
@@ -216,12 +225,14 @@ int32_t do_search3(eppx_ps3d *ps, eppx_ns3d *ns, int32_t criterion, int32_t acti
     ps->last_fluid = ph.fluid_tag;
     ps->last_wall = ph.wall_tag;
     ps->searched = true;
+    ps->masks_valid = true;
     return EPPX_OK;
 }
 
 int32_t do_momentum3(eppx_ps3d *ps, eppx_ns3d *ns, const Physics &ph) {
     eppx_ctx *ctx = ps->ctx;
-    const Walk3 w = make_walk3(ps, ns, ps->last_crit);
+    Walk3 w = make_walk3(ps, ns, ps->last_crit);
+    if (ps->masks_valid) w.mask_mode = 2;
     HIP_OK(ctx, timed(ctx, "momentum", [&] {
                return launch3_momentum(w, ps->f(), ph, ctx->math_mode == EPPX_MATH_FAST, ps->n, ctx->stream);
            }));
@@ -417,6 +428,7 @@ int32_t eppx_ps3d_destroy(eppx_ps3d *ps) {
     dfree(ps->counts); dfree(ps->start); dfree(ps->tile_sums); dfree(ps->cell_tags);
     dfree(ps->key); dfree(ps->tmpslot); dfree(ps->members); dfree(ps->ncount); dfree(ps->n_alive);
     dfree(ps->pairs);
+    dfree(ps->masks);
     eppx_ctx *ctx = ps->ctx;
     delete ps;
     ctx->children -= 1;
@@ -486,6 +498,7 @@ int32_t eppx_ps3d_upload(eppx_ps3d *ps, int32_t n_particles, const int32_t *is_a
     for (size_t k = 0; k < ps->pass_float_cols.size(); ++k)
         memcpy(&ps->pass_float[k * cap], float_props + (size_t)ps->pass_float_cols[k] * cap, cap * sizeof(float));
     ps->searched = false;
+    ps->masks_valid = false;
     ps->last_ns = nullptr;
     return EPPX_OK;
 }

@@ -57,6 +57,13 @@ struct Walk3 {
     float hinv_u;                // > 0: every alive particle has one H, and this is its pair hinv
     int32_t stripe;              // groups of 4 cells per stripe of the pair-kernel grid
     int32_t stripes_per_xcd;     // stripes each of the 8 XCDs works through
+    // Accept masks of the last search, one 64-bit mask per (stencil cell s, slot i) at
+    // masks[s * mask_stride + i], for neighbour cells of at most 64 particles.  mask_mode:
+    // 0 the pair kernel tests every candidate; 1 it tests them and stores the masks (the
+    // search); 2 it reads the stored masks instead (momentum after that search).
+    unsigned long long *masks;
+    int64_t mask_stride;
+    int32_t mask_mode;
 };
 
 // Neighbour-column export of the 3D download (nRVec holds 3 floats per pair).

@@ -360,6 +360,13 @@ __device__ __forceinline__ void cell_walk3(const Walk3 &w, Stage3 &st, int c, ui
                 const int nb = x + g.n_x * (y + g.n_y * z);
                 if (!(w.cell_tags[nb] & adj_block)) continue;
                 const int b = w.start[nb], e = w.start[nb + 1];
+                const size_t mi = (size_t)((di + 1) * 9 + (dj + 1) * 3 + dk + 1) * (size_t)w.mask_stride + i;
+                const bool small = e - b <= kCellBlock;  // one chunk: its mask is stored
+                unsigned long long stored = 0ull;
+                if (w.mask_mode == 2 && small) {
+                    if (active) stored = w.masks[mi];
+                    if (!__ballot(stored != 0ull)) continue;  // no lane has a pair in this cell
+                }
                 for (int base = b; base < e; base += kCellBlock) {
                     const int len = min(kCellBlock, e - base);
                     if (lane < len) {
@@ -369,9 +376,15 @@ __device__ __forceinline__ void cell_walk3(const Walk3 &w, Stage3 &st, int c, ui
                     }
                     wave_sync();
                     if (active) {
-                        unsigned long long m = accept_staged<CHECK, BAND>(w, st, gi, hi, adj_i, len);
-                        const unsigned d = (unsigned)(i - base);
-                        if (d < 64u) m &= ~(1ull << d);
+                        unsigned long long m;
+                        if (w.mask_mode == 2 && small) {
+                            m = stored;
+                        } else {
+                            m = accept_staged<CHECK, BAND>(w, st, gi, hi, adj_i, len);
+                            const unsigned d = (unsigned)(i - base);
+                            if (d < 64u) m &= ~(1ull << d);
+                            if (w.mask_mode == 1 && small) w.masks[mi] = m;
+                        }
                         while (m) {
                             const int k = __ffsll((long long)m) - 1;
                             m &= m - 1ull;
@@ -415,16 +428,31 @@ __device__ __forceinline__ void cell_walk3_paired(const Walk3 &w, Stage3 *st2, i
         b = w.start[nb];
         e = w.start[nb + 1];
     };
+    const size_t ms = (size_t)w.mask_stride;
     for (int grp = 0; grp < 14; ++grp) {
         int ba, ea, bb, eb;
         range(13 + grp, ba, ea);
         if (grp) range(13 - grp, bb, eb);
         else bb = eb = 0;
         const int na = ea - ba, nbb = eb - bb;
-        const int chunks = (max(na, nbb) + kCellBlock - 1) / kCellBlock;
+        // a cell of at most 64 particles is one chunk, whose mask the search stores
+        const bool small_a = na > 0 && na <= kCellBlock, small_b = nbb > 0 && nbb <= kCellBlock;
+        const size_t mia = (size_t)(13 + grp) * ms + i, mib = (size_t)(13 - grp) * ms + i;
+        const bool load_a = w.mask_mode == 2 && small_a, load_b = w.mask_mode == 2 && small_b;
+        unsigned long long sa = 0ull, sb = 0ull;
+        if (active) {
+            if (load_a) sa = w.masks[mia];
+            if (load_b) sb = w.masks[mib];
+        }
+        // a stored mask that is empty in every lane: the cell needs no staging
+        const bool need_a = na > 0 && (!load_a || __ballot(sa != 0ull) != 0ull);
+        const bool need_b = nbb > 0 && (!load_b || __ballot(sb != 0ull) != 0ull);
+        if (!need_a && !need_b) continue;
+        const int chunks = (max(need_a ? na : 0, need_b ? nbb : 0) + kCellBlock - 1) / kCellBlock;
         for (int ch = 0; ch < chunks; ++ch) {
             const int base_a = ba + ch * kCellBlock, base_b = bb + ch * kCellBlock;
-            const int len_a = min(kCellBlock, max(0, ea - base_a)), len_b = min(kCellBlock, max(0, eb - base_b));
+            const int len_a = need_a ? min(kCellBlock, max(0, ea - base_a)) : 0;
+            const int len_b = need_b ? min(kCellBlock, max(0, eb - base_b)) : 0;
             if (lane < len_a) {
                 st2[0].g[lane] = w.geo[base_a + lane];
                 st2[0].v[lane] = w.vel[base_a + lane];
@@ -437,10 +465,21 @@ __device__ __forceinline__ void cell_walk3_paired(const Walk3 &w, Stage3 *st2, i
             }
             wave_sync();
             if (active) {
-                unsigned long long ma = len_a ? accept_staged<CHECK, BAND>(w, st2[0], gi, hi, adj_i, len_a) : 0ull;
-                unsigned long long mb = len_b ? accept_staged<CHECK, BAND>(w, st2[1], gi, hi, adj_i, len_b) : 0ull;
-                const unsigned d = (unsigned)(i - base_a);
-                if (grp == 0 && d < 64u) ma &= ~(1ull << d);
+                unsigned long long ma = 0ull, mb = 0ull;
+                if (load_a) {
+                    ma = len_a ? sa : 0ull;
+                } else if (len_a) {
+                    ma = accept_staged<CHECK, BAND>(w, st2[0], gi, hi, adj_i, len_a);
+                    const unsigned d = (unsigned)(i - base_a);
+                    if (grp == 0 && d < 64u) ma &= ~(1ull << d);
+                    if (w.mask_mode == 1 && small_a) w.masks[mia] = ma;
+                }
+                if (load_b) {
+                    mb = len_b ? sb : 0ull;
+                } else if (len_b) {
+                    mb = accept_staged<CHECK, BAND>(w, st2[1], gi, hi, adj_i, len_b);
+                    if (w.mask_mode == 1 && small_b) w.masks[mib] = mb;
+                }
                 while (ma | mb) {
                     const bool in_a = ma != 0ull;
                     const unsigned long long m = in_a ? ma : mb;
@@ -568,6 +607,7 @@ __global__ __launch_bounds__(kCellBlock * kCellWaves) void k3_momentum(Walk3 w, 
     const uint32_t fluid_bit = tag_bit(ph.fluid_tag), wall_bit = tag_bit(ph.wall_tag);
     if (c0 >= c1 || !(w.cell_tags[c] & fluid_bit)) return;
     const uint32_t adj_block = adj3(w, fluid_bit);
+    if (!adj_block) return;  // a fluid adjacent to nothing has no pairs (and no stored masks)
     for (int i0 = c0; i0 < c1; i0 += kCellBlock) {
         const int i = i0 + lane_id();
         bool active = i < c1;
