@@ -317,6 +317,63 @@ __device__ __forceinline__ uint32_t adj_of_cell(const Walk3 &w, uint32_t tags) {
     return a;
 }
 
+// The reach filter of a wave: the bounding box of its particles' positions
+// (the lanes holding one) and the largest dr2 the criterion can accept,
+// widened by 1e-4 relative -- far more than the float32 rounding of dr2 or of
+// the box distance.  A candidate farther than that from the box is no lane's
+// neighbour, so it is not staged.  Of the 27 cells around a full cell of a
+// lattice at h = 2dx (1,728 candidates) about 970 are within reach.
+struct Reach3 {
+    float lx, ly, lz, hx, hy, hz, lim2;
+};
+
+__device__ __forceinline__ Reach3 wave_reach(const Walk3 &w, float4 gi, bool valid) {
+    Reach3 r;
+    r.lx = valid ? gi.x : INFINITY;
+    r.ly = valid ? gi.y : INFINITY;
+    r.lz = valid ? gi.z : INFINITY;
+    r.hx = valid ? gi.x : -INFINITY;
+    r.hy = valid ? gi.y : -INFINITY;
+    r.hz = valid ? gi.z : -INFINITY;
+    for (int off = 32; off > 0; off >>= 1) {
+        r.lx = fminf(r.lx, __shfl_xor(r.lx, off));
+        r.ly = fminf(r.ly, __shfl_xor(r.ly, off));
+        r.lz = fminf(r.lz, __shfl_xor(r.lz, off));
+        r.hx = fmaxf(r.hx, __shfl_xor(r.hx, off));
+        r.hy = fmaxf(r.hy, __shfl_xor(r.hy, off));
+        r.hz = fmaxf(r.hz, __shfl_xor(r.hz, off));
+    }
+    r.lim2 = fmaxf(w.rmin2, w.rmax2) * 1.0001f;
+    return r;
+}
+
+// Stages the slots of [base, base + len) (len <= 64) within reach, in slot
+// order: slot base + L goes to position popcount(bits of the ballot below L).
+// Returns the ballot.  A staged g.w holds the slot's tag bit.  Every lane of
+// the wave calls it.
+__device__ __forceinline__ unsigned long long stage_reach(const Walk3 &w, Stage3 &st, const Reach3 &r, int base,
+                                                          int len) {
+    const int lane = lane_id();
+    float4 g = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    bool in = false;
+    if (lane < len) {
+        g = w.geo[base + lane];
+        const float dx = fmaxf(0.0f, fmaxf(r.lx - g.x, g.x - r.hx));
+        const float dy = fmaxf(0.0f, fmaxf(r.ly - g.y, g.y - r.hy));
+        const float dz = fmaxf(0.0f, fmaxf(r.lz - g.z, g.z - r.hz));
+        in = !(dx * dx + dy * dy + dz * dz > r.lim2);
+    }
+    const unsigned long long bal = __ballot(in);
+    if (in) {
+        const int pos = __popcll(bal & ((1ull << lane) - 1ull));
+        g.w = __uint_as_float(tag_bit(__float_as_int(g.w)));
+        st.g[pos] = g;
+        st.v[pos] = w.vel[base + lane];
+        st.q[pos] = w.hmvp[base + lane];
+    }
+    return bal;
+}
+
 // Pass 1 over the staged slots [0, len) against this lane's particle.
 template <bool CHECK, bool BAND>
 __device__ __forceinline__ unsigned long long accept_staged(const Walk3 &w, const Stage3 &st, float4 gi, float hi,
@@ -330,7 +387,7 @@ __device__ __forceinline__ unsigned long long accept_staged(const Walk3 &w, cons
             const float dr2 = dist2(gi.x - gj.x, gi.y - gj.y, gi.z - gj.z);
             bool ok = dr2 < w.rmin2;
             if (BAND && !ok && dr2 <= w.rmax2) ok = criterion3(w, hi, st.q[k + u].x, dr2);
-            if (CHECK) ok = ok && (adj_i & tag_bit(__float_as_int(gj.w))) != 0u;
+            if (CHECK) ok = ok && (adj_i & __float_as_uint(gj.w)) != 0u;
             nib |= ok ? (1u << u) : 0u;
         }
         mask |= (unsigned long long)nib << k;
@@ -338,16 +395,25 @@ __device__ __forceinline__ unsigned long long accept_staged(const Walk3 &w, cons
     return len < 64 ? mask & ((1ull << len) - 1ull) : mask;
 }
 
+// This lane's own slot among the staged ones (bal: the staging ballot of the
+// range starting at base): its bit, or 0 when it is not in the range.
+__device__ __forceinline__ unsigned long long self_bit(unsigned long long bal, int i, int base) {
+    const unsigned d = (unsigned)(i - base);
+    if (d >= 64u || !((bal >> d) & 1ull)) return 0ull;
+    return 1ull << __popcll(bal & ((1ull << d) - 1ull));
+}
+
 // The walk of one 64-particle chunk of cell c: visit(geo_j, vel_j, hmvp_j,
-// dx, dy, dz, dr2) for each accepted neighbour j.  `active`: this lane
-// holds a particle that takes pairs; every lane of the block calls it.
+// dx, dy, dz, dr2) for each accepted neighbour j (geo_j.w: its tag bit).
+// `active`: this lane holds a particle that takes pairs; every lane of the
+// wave calls it, with the wave's reach filter.  Stored masks (mask_mode 1 and
+// 2) index the staged order of a cell of at most 64 particles.
 template <bool CHECK, bool BAND, class Visit>
 __device__ __forceinline__ void cell_walk3(const Walk3 &w, Stage3 &st, int c, uint32_t adj_block, int i, bool active,
-                                           float4 gi, float hi, uint32_t adj_i, Visit &&visit) {
+                                           const Reach3 &reach, float4 gi, float hi, uint32_t adj_i, Visit &&visit) {
     const Grid3 &g = w.g;
     const int cx = c % g.n_x, t = c / g.n_x;
     const int cy = t % g.n_y, cz = t / g.n_y;
-    const int lane = lane_id();
     for (int di = -1; di <= 1; ++di) {
         const int x = cx + di;
         if (x < 0 || x >= g.n_x) continue;
@@ -362,27 +428,24 @@ __device__ __forceinline__ void cell_walk3(const Walk3 &w, Stage3 &st, int c, ui
                 const int b = w.start[nb], e = w.start[nb + 1];
                 const size_t mi = (size_t)((di + 1) * 9 + (dj + 1) * 3 + dk + 1) * (size_t)w.mask_stride + i;
                 const bool small = e - b <= kCellBlock;  // one chunk: its mask is stored
+                const bool load = w.mask_mode == 2 && small;
                 unsigned long long stored = 0ull;
-                if (w.mask_mode == 2 && small) {
+                if (load) {
                     if (active) stored = w.masks[mi];
                     if (!__ballot(stored != 0ull)) continue;  // no lane has a pair in this cell
                 }
                 for (int base = b; base < e; base += kCellBlock) {
-                    const int len = min(kCellBlock, e - base);
-                    if (lane < len) {
-                        st.g[lane] = w.geo[base + lane];
-                        st.v[lane] = w.vel[base + lane];
-                        st.q[lane] = w.hmvp[base + lane];
-                    }
+                    const int raw = min(kCellBlock, e - base);
+                    const unsigned long long bal = stage_reach(w, st, reach, base, raw);
+                    const int len = __popcll(bal);
                     wave_sync();
                     if (active) {
                         unsigned long long m;
-                        if (w.mask_mode == 2 && small) {
+                        if (load) {
                             m = stored;
                         } else {
-                            m = accept_staged<CHECK, BAND>(w, st, gi, hi, adj_i, len);
-                            const unsigned d = (unsigned)(i - base);
-                            if (d < 64u) m &= ~(1ull << d);
+                            m = len ? accept_staged<CHECK, BAND>(w, st, gi, hi, adj_i, len) & ~self_bit(bal, i, base)
+                                    : 0ull;
                             if (w.mask_mode == 1 && small) w.masks[mi] = m;
                         }
                         while (m) {
@@ -412,11 +475,11 @@ __device__ __forceinline__ void cell_walk3(const Walk3 &w, Stage3 &st, int c, ui
 // list order (FAST math only).
 template <bool CHECK, bool BAND, class Visit>
 __device__ __forceinline__ void cell_walk3_paired(const Walk3 &w, Stage3 *st2, int c, uint32_t adj_block, int i,
-                                                  bool active, float4 gi, float hi, uint32_t adj_i, Visit &&visit) {
+                                                  bool active, const Reach3 &reach, float4 gi, float hi,
+                                                  uint32_t adj_i, Visit &&visit) {
     const Grid3 &g = w.g;
     const int cx = c % g.n_x, t = c / g.n_x;
     const int cy = t % g.n_y, cz = t / g.n_y;
-    const int lane = lane_id();
     // slot range of stencil cell s (x offset s / 9 - 1, y (s / 3) % 3 - 1, z s % 3 - 1), empty when the cell
     // lies outside the grid or holds no particle of a tag adjacent to the wave's
     auto range = [&](int sidx, int &b, int &e) {
@@ -451,33 +514,25 @@ __device__ __forceinline__ void cell_walk3_paired(const Walk3 &w, Stage3 *st2, i
         const int chunks = (max(need_a ? na : 0, need_b ? nbb : 0) + kCellBlock - 1) / kCellBlock;
         for (int ch = 0; ch < chunks; ++ch) {
             const int base_a = ba + ch * kCellBlock, base_b = bb + ch * kCellBlock;
-            const int len_a = need_a ? min(kCellBlock, max(0, ea - base_a)) : 0;
-            const int len_b = need_b ? min(kCellBlock, max(0, eb - base_b)) : 0;
-            if (lane < len_a) {
-                st2[0].g[lane] = w.geo[base_a + lane];
-                st2[0].v[lane] = w.vel[base_a + lane];
-                st2[0].q[lane] = w.hmvp[base_a + lane];
-            }
-            if (lane < len_b) {
-                st2[1].g[lane] = w.geo[base_b + lane];
-                st2[1].v[lane] = w.vel[base_b + lane];
-                st2[1].q[lane] = w.hmvp[base_b + lane];
-            }
+            const int raw_a = need_a ? min(kCellBlock, max(0, ea - base_a)) : 0;
+            const int raw_b = need_b ? min(kCellBlock, max(0, eb - base_b)) : 0;
+            const unsigned long long bal_a = stage_reach(w, st2[0], reach, base_a, raw_a);
+            const unsigned long long bal_b = stage_reach(w, st2[1], reach, base_b, raw_b);
+            const int len_a = __popcll(bal_a), len_b = __popcll(bal_b);
             wave_sync();
             if (active) {
                 unsigned long long ma = 0ull, mb = 0ull;
                 if (load_a) {
-                    ma = len_a ? sa : 0ull;
-                } else if (len_a) {
-                    ma = accept_staged<CHECK, BAND>(w, st2[0], gi, hi, adj_i, len_a);
-                    const unsigned d = (unsigned)(i - base_a);
-                    if (grp == 0 && d < 64u) ma &= ~(1ull << d);
+                    ma = raw_a ? sa : 0ull;
+                } else if (raw_a) {
+                    ma = len_a ? accept_staged<CHECK, BAND>(w, st2[0], gi, hi, adj_i, len_a) : 0ull;
+                    if (grp == 0) ma &= ~self_bit(bal_a, i, base_a);
                     if (w.mask_mode == 1 && small_a) w.masks[mia] = ma;
                 }
                 if (load_b) {
-                    mb = len_b ? sb : 0ull;
-                } else if (len_b) {
-                    mb = accept_staged<CHECK, BAND>(w, st2[1], gi, hi, adj_i, len_b);
+                    mb = raw_b ? sb : 0ull;
+                } else if (raw_b) {
+                    mb = len_b ? accept_staged<CHECK, BAND>(w, st2[1], gi, hi, adj_i, len_b) : 0ull;
                     if (w.mask_mode == 1 && small_b) w.masks[mib] = mb;
                 }
                 while (ma | mb) {
@@ -499,33 +554,37 @@ __device__ __forceinline__ void cell_walk3_paired(const Walk3 &w, Stage3 *st2, i
 
 template <class Visit>
 __device__ __forceinline__ void cell_walk3_any(const Walk3 &w, Stage3 &st, int c, uint32_t adj_block, int i,
-                                               bool active, float4 gi, float hi, uint32_t adj_i, bool check,
-                                               Visit &&visit) {
+                                               bool active, const Reach3 &r, float4 gi, float hi, uint32_t adj_i,
+                                               bool check, Visit &&visit) {
     const bool band = w.rmin2 <= w.rmax2;
     if (__ballot(check)) {
-        if (band) cell_walk3<true, true>(w, st, c, adj_block, i, active, gi, hi, adj_i, visit);
-        else cell_walk3<true, false>(w, st, c, adj_block, i, active, gi, hi, adj_i, visit);
+        if (band) cell_walk3<true, true>(w, st, c, adj_block, i, active, r, gi, hi, adj_i, visit);
+        else cell_walk3<true, false>(w, st, c, adj_block, i, active, r, gi, hi, adj_i, visit);
     } else {
-        if (band) cell_walk3<false, true>(w, st, c, adj_block, i, active, gi, hi, adj_i, visit);
-        else cell_walk3<false, false>(w, st, c, adj_block, i, active, gi, hi, adj_i, visit);
+        if (band) cell_walk3<false, true>(w, st, c, adj_block, i, active, r, gi, hi, adj_i, visit);
+        else cell_walk3<false, false>(w, st, c, adj_block, i, active, r, gi, hi, adj_i, visit);
     }
 }
 
+// valid: this lane holds a particle of the chunk (its position joins the
+// wave's reach filter, which the search and momentum over the same cell list
+// then compute alike, so stored masks index the same staged order).
 template <bool FAST, class Visit>
 __device__ __forceinline__ void cell_walk3_sel(const Walk3 &w, Stage3 *st2, int c, uint32_t adj_block, int i,
-                                               bool active, float4 gi, float hi, uint32_t adj_i, bool check,
-                                               Visit &&visit) {
+                                               bool valid, bool active, float4 gi, float hi, uint32_t adj_i,
+                                               bool check, Visit &&visit) {
+    const Reach3 r = wave_reach(w, gi, valid);
     if (!FAST) {
-        cell_walk3_any(w, st2[0], c, adj_block, i, active, gi, hi, adj_i, check, visit);
+        cell_walk3_any(w, st2[0], c, adj_block, i, active, r, gi, hi, adj_i, check, visit);
         return;
     }
     const bool band = w.rmin2 <= w.rmax2;
     if (__ballot(check)) {
-        if (band) cell_walk3_paired<true, true>(w, st2, c, adj_block, i, active, gi, hi, adj_i, visit);
-        else cell_walk3_paired<true, false>(w, st2, c, adj_block, i, active, gi, hi, adj_i, visit);
+        if (band) cell_walk3_paired<true, true>(w, st2, c, adj_block, i, active, r, gi, hi, adj_i, visit);
+        else cell_walk3_paired<true, false>(w, st2, c, adj_block, i, active, r, gi, hi, adj_i, visit);
     } else {
-        if (band) cell_walk3_paired<false, true>(w, st2, c, adj_block, i, active, gi, hi, adj_i, visit);
-        else cell_walk3_paired<false, false>(w, st2, c, adj_block, i, active, gi, hi, adj_i, visit);
+        if (band) cell_walk3_paired<false, true>(w, st2, c, adj_block, i, active, r, gi, hi, adj_i, visit);
+        else cell_walk3_paired<false, false>(w, st2, c, adj_block, i, active, r, gi, hi, adj_i, visit);
     }
 }
 
@@ -565,11 +624,11 @@ __global__ __launch_bounds__(kCellBlock * kCellWaves) void k3_search(Walk3 w, Fi
         const uint32_t adj_i = adj3(w, tbi);
         const bool fluid_i = tbi && tbi == fluid_bit, wall_i = tbi && tbi == wall_bit;
         const bool check = active && (w.odd_tags || (adj_i & w.tags_present) != w.tags_present);
-        cell_walk3_sel<FAST>(w, st2, c, adj_block, i, active && adj_i != 0u, gi, hi, adj_i, check,
+        cell_walk3_sel<FAST>(w, st2, c, adj_block, i, active, active && adj_i != 0u, gi, hi, adj_i, check,
                        [&](float4 gj, float4 vj, float4 qj, float dx, float dy, float dz, float dr2) {
                            ++cnt;
                            if (ACTION == EPPX_SEARCH_KERNEL_CONTINUITY) {
-                               const uint32_t tbj = tag_bit(__float_as_int(gj.w));
+                               const uint32_t tbj = __float_as_uint(gj.w);
                                const bool classic = fluid_i && tbj == fluid_bit;
                                const bool balanced = (fluid_i && tbj == wall_bit) || (wall_i && tbj == fluid_bit);
                                if (classic || balanced) {
@@ -610,7 +669,8 @@ __global__ __launch_bounds__(kCellBlock * kCellWaves) void k3_momentum(Walk3 w, 
     if (!adj_block) return;  // a fluid adjacent to nothing has no pairs (and no stored masks)
     for (int i0 = c0; i0 < c1; i0 += kCellBlock) {
         const int i = i0 + lane_id();
-        bool active = i < c1;
+        const bool valid = i < c1;
+        bool active = valid;
         float4 gi = make_float4(0.0f, 0.0f, 0.0f, 0.0f), vi = gi, qi = gi, acc = gi;
         if (active) {
             gi = w.geo[i];
@@ -624,9 +684,9 @@ __global__ __launch_bounds__(kCellBlock * kCellWaves) void k3_momentum(Walk3 w, 
         const float p_i = qi.w;
         const float pr_i = FAST ? p_i * frcp(vi.w * vi.w) : p_i / (vi.w * vi.w);
         const bool check = active && (w.odd_tags || (adj_block & w.tags_present) != w.tags_present);
-        cell_walk3_sel<FAST>(w, st2, c, adj_block, i, active, gi, qi.x, adj_block, check,
+        cell_walk3_sel<FAST>(w, st2, c, adj_block, i, valid, active, gi, qi.x, adj_block, check,
                        [&](float4 gj, float4 vj, float4 qj, float dx, float dy, float dz, float dr2) {
-                           const uint32_t tbj = tag_bit(__float_as_int(gj.w));
+                           const uint32_t tbj = __float_as_uint(gj.w);
                            const bool classic = tbj == fluid_bit;
                            if (!classic && tbj != wall_bit) return;
                            const float hinv = w.hinv_u > 0.0f ? w.hinv_u : invharmonic(qi.x, qj.x);
