@@ -299,10 +299,12 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// A wave's staging area: the slots of up to two neighbour cells (64 each),
+// the second cell's right after the first's.
 struct Stage3 {
-    float4 g[kCellBlock];  // (x, y, z, tag bits) of the staged slots
-    float4 v[kCellBlock];  // (u, v, w, rho)
-    float4 q[kCellBlock];  // (H, Mass, Volume, Pressure)
+    float4 g[2 * kCellBlock];  // (x, y, z, tag bit) of the staged slots
+    float4 v[2 * kCellBlock];  // (u, v, w, rho)
+    float4 q[2 * kCellBlock];  // (H, Mass, Volume, Pressure); see stage_reach for pr_bit
 };
 
 // Adjacency of any tag present in cell c (a cell holding no particle of a tag
@@ -348,11 +350,12 @@ __device__ __forceinline__ Reach3 wave_reach(const Walk3 &w, float4 gi, bool val
 }
 
 // Stages the slots of [base, base + len) (len <= 64) within reach, in slot
-// order: slot base + L goes to position popcount(bits of the ballot below L).
-// Returns the ballot.  A staged g.w holds the slot's tag bit.  Every lane of
-// the wave calls it.
+// order, from position off: slot base + L goes to off + popcount(bits of the
+// ballot below L).  Returns the ballot.  A staged g.w holds the slot's tag
+// bit; with pr_bit, the staged pressure of a slot with that tag bit is
+// p / rho^2 (pressure_ratio) instead of p.  Every lane of the wave calls it.
 __device__ __forceinline__ unsigned long long stage_reach(const Walk3 &w, Stage3 &st, const Reach3 &r, int base,
-                                                          int len) {
+                                                          int len, int off, uint32_t pr_bit) {
     const int lane = lane_id();
     float4 g = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     bool in = false;
@@ -365,28 +368,33 @@ __device__ __forceinline__ unsigned long long stage_reach(const Walk3 &w, Stage3
     }
     const unsigned long long bal = __ballot(in);
     if (in) {
-        const int pos = __popcll(bal & ((1ull << lane) - 1ull));
-        g.w = __uint_as_float(tag_bit(__float_as_int(g.w)));
+        const int pos = off + __popcll(bal & ((1ull << lane) - 1ull));
+        const uint32_t tb = tag_bit(__float_as_int(g.w));
+        g.w = __uint_as_float(tb);
+        const float4 v = w.vel[base + lane];
+        float4 q = w.hmvp[base + lane];
+        if (pr_bit && tb == pr_bit) q.w = pressure_ratio(q.w, v.w);
         st.g[pos] = g;
-        st.v[pos] = w.vel[base + lane];
-        st.q[pos] = w.hmvp[base + lane];
+        st.v[pos] = v;
+        st.q[pos] = q;
     }
     return bal;
 }
 
-// Pass 1 over the staged slots [0, len) against this lane's particle.
+// Pass 1 over the staged slots [off, off + len) against this lane's particle
+// (bit k: slot off + k).
 template <bool CHECK, bool BAND>
-__device__ __forceinline__ unsigned long long accept_staged(const Walk3 &w, const Stage3 &st, float4 gi, float hi,
-                                                            uint32_t adj_i, int len) {
+__device__ __forceinline__ unsigned long long accept_staged(const Walk3 &w, const Stage3 &st, int off, float4 gi,
+                                                            float hi, uint32_t adj_i, int len) {
     unsigned long long mask = 0ull;
-    for (int k = 0; k < len; k += 4) {
+    for (int k = 0; k < len; k += 4) {  // reads up to 3 slots past the range, inside the area (off <= 64)
         uint32_t nib = 0u;
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const float4 gj = st.g[k + u];
+            const float4 gj = st.g[off + k + u];
             const float dr2 = dist2(gi.x - gj.x, gi.y - gj.y, gi.z - gj.z);
             bool ok = dr2 < w.rmin2;
-            if (BAND && !ok && dr2 <= w.rmax2) ok = criterion3(w, hi, st.q[k + u].x, dr2);
+            if (BAND && !ok && dr2 <= w.rmax2) ok = criterion3(w, hi, st.q[off + k + u].x, dr2);
             if (CHECK) ok = ok && (adj_i & __float_as_uint(gj.w)) != 0u;
             nib |= ok ? (1u << u) : 0u;
         }
@@ -410,7 +418,8 @@ __device__ __forceinline__ unsigned long long self_bit(unsigned long long bal, i
 // 2) index the staged order of a cell of at most 64 particles.
 template <bool CHECK, bool BAND, class Visit>
 __device__ __forceinline__ void cell_walk3(const Walk3 &w, Stage3 &st, int c, uint32_t adj_block, int i, bool active,
-                                           const Reach3 &reach, float4 gi, float hi, uint32_t adj_i, Visit &&visit) {
+                                           const Reach3 &reach, uint32_t pr_bit, float4 gi, float hi, uint32_t adj_i,
+                                           Visit &&visit) {
     const Grid3 &g = w.g;
     const int cx = c % g.n_x, t = c / g.n_x;
     const int cy = t % g.n_y, cz = t / g.n_y;
@@ -436,7 +445,7 @@ __device__ __forceinline__ void cell_walk3(const Walk3 &w, Stage3 &st, int c, ui
                 }
                 for (int base = b; base < e; base += kCellBlock) {
                     const int raw = min(kCellBlock, e - base);
-                    const unsigned long long bal = stage_reach(w, st, reach, base, raw);
+                    const unsigned long long bal = stage_reach(w, st, reach, base, raw, 0, pr_bit);
                     const int len = __popcll(bal);
                     wave_sync();
                     if (active) {
@@ -444,7 +453,7 @@ __device__ __forceinline__ void cell_walk3(const Walk3 &w, Stage3 &st, int c, ui
                         if (load) {
                             m = stored;
                         } else {
-                            m = len ? accept_staged<CHECK, BAND>(w, st, gi, hi, adj_i, len) & ~self_bit(bal, i, base)
+                            m = len ? accept_staged<CHECK, BAND>(w, st, 0, gi, hi, adj_i, len) & ~self_bit(bal, i, base)
                                     : 0ull;
                             if (w.mask_mode == 1 && small) w.masks[mi] = m;
                         }
@@ -466,17 +475,17 @@ __device__ __forceinline__ void cell_walk3(const Walk3 &w, Stage3 &st, int c, ui
 
 // FAST walk of one 64-particle chunk of cell c: the neighbour cells in 14
 // groups -- the cell itself, then each offset together with its mirror (the
-// stencil cells s and 26 - s) -- with both cells of a group staged at once and
-// their accepted pairs walked in one loop.  A particle near one face of its
-// cell has many pairs in the cell beyond that face and few in the cell beyond
-// the opposite one, so a group's pair count varies much less across the wave
-// than a cell's: the wave runs about half the pass-2 iterations of the
-// cell-by-cell walk.  Sums then run in another order than the reference's
-// list order (FAST math only).
+// stencil cells s and 26 - s) -- with both cells of a group staged at once,
+// one after the other, and their accepted pairs walked as one 128-bit mask.
+// A particle near one face of its cell has many pairs in the cell beyond that
+// face and few in the cell beyond the opposite one, so a group's pair count
+// varies much less across the wave than a cell's: the wave runs about half
+// the pass-2 iterations of the cell-by-cell walk.  Sums then run in another
+// order than the reference's list order (FAST math only).
 template <bool CHECK, bool BAND, class Visit>
-__device__ __forceinline__ void cell_walk3_paired(const Walk3 &w, Stage3 *st2, int c, uint32_t adj_block, int i,
-                                                  bool active, const Reach3 &reach, float4 gi, float hi,
-                                                  uint32_t adj_i, Visit &&visit) {
+__device__ __forceinline__ void cell_walk3_paired(const Walk3 &w, Stage3 &st, int c, uint32_t adj_block, int i,
+                                                  bool active, const Reach3 &reach, uint32_t pr_bit, float4 gi,
+                                                  float hi, uint32_t adj_i, Visit &&visit) {
     const Grid3 &g = w.g;
     const int cx = c % g.n_x, t = c / g.n_x;
     const int cy = t % g.n_y, cz = t / g.n_y;
@@ -516,33 +525,52 @@ __device__ __forceinline__ void cell_walk3_paired(const Walk3 &w, Stage3 *st2, i
             const int base_a = ba + ch * kCellBlock, base_b = bb + ch * kCellBlock;
             const int raw_a = need_a ? min(kCellBlock, max(0, ea - base_a)) : 0;
             const int raw_b = need_b ? min(kCellBlock, max(0, eb - base_b)) : 0;
-            const unsigned long long bal_a = stage_reach(w, st2[0], reach, base_a, raw_a);
-            const unsigned long long bal_b = stage_reach(w, st2[1], reach, base_b, raw_b);
-            const int len_a = __popcll(bal_a), len_b = __popcll(bal_b);
+            const unsigned long long bal_a = stage_reach(w, st, reach, base_a, raw_a, 0, pr_bit);
+            const int len_a = __popcll(bal_a);
+            const unsigned long long bal_b = stage_reach(w, st, reach, base_b, raw_b, len_a, pr_bit);
+            const int len_b = __popcll(bal_b);
             wave_sync();
             if (active) {
                 unsigned long long ma = 0ull, mb = 0ull;
                 if (load_a) {
                     ma = raw_a ? sa : 0ull;
                 } else if (raw_a) {
-                    ma = len_a ? accept_staged<CHECK, BAND>(w, st2[0], gi, hi, adj_i, len_a) : 0ull;
+                    ma = len_a ? accept_staged<CHECK, BAND>(w, st, 0, gi, hi, adj_i, len_a) : 0ull;
                     if (grp == 0) ma &= ~self_bit(bal_a, i, base_a);
                     if (w.mask_mode == 1 && small_a) w.masks[mia] = ma;
                 }
                 if (load_b) {
                     mb = raw_b ? sb : 0ull;
                 } else if (raw_b) {
-                    mb = len_b ? accept_staged<CHECK, BAND>(w, st2[1], gi, hi, adj_i, len_b) : 0ull;
+                    mb = len_b ? accept_staged<CHECK, BAND>(w, st, len_a, gi, hi, adj_i, len_b) : 0ull;
                     if (w.mask_mode == 1 && small_b) w.masks[mib] = mb;
                 }
-                while (ma | mb) {
-                    const bool in_a = ma != 0ull;
-                    const unsigned long long m = in_a ? ma : mb;
-                    const int k = __ffsll((long long)m) - 1;
-                    if (in_a) ma &= ma - 1ull;
-                    else mb &= mb - 1ull;
-                    const Stage3 &sx = st2[in_a ? 0 : 1];
-                    const float4 gj = sx.g[k], vj = sx.v[k], qj = sx.q[k];
+                // one 128-bit mask over the staged slots [0, len_a + len_b): bits of the
+                // current 64-slot word in cur, the rest in nxt
+                unsigned long long cur = ma, nxt = 0ull;
+                if (len_a == 0) {
+                    cur = mb;
+                } else if (len_a < 64) {
+                    cur |= mb << len_a;
+                    nxt = mb >> (64 - len_a);
+                } else {
+                    nxt = mb;
+                }
+                int off = 0;
+                if (!cur) {
+                    cur = nxt;
+                    nxt = 0ull;
+                    off = 64;
+                }
+                while (cur) {
+                    const int k = off + __builtin_ctzll(cur);
+                    cur &= cur - 1ull;
+                    if (!cur) {  // the second word, if any, after the last bit of the first
+                        cur = nxt;
+                        nxt = 0ull;
+                        off = 64;
+                    }
+                    const float4 gj = st.g[k], vj = st.v[k], qj = st.q[k];
                     const float dx = gi.x - gj.x, dy = gi.y - gj.y, dz = gi.z - gj.z;
                     visit(gj, vj, qj, dx, dy, dz, dist2(dx, dy, dz));
                 }
@@ -554,15 +582,15 @@ __device__ __forceinline__ void cell_walk3_paired(const Walk3 &w, Stage3 *st2, i
 
 template <class Visit>
 __device__ __forceinline__ void cell_walk3_any(const Walk3 &w, Stage3 &st, int c, uint32_t adj_block, int i,
-                                               bool active, const Reach3 &r, float4 gi, float hi, uint32_t adj_i,
-                                               bool check, Visit &&visit) {
+                                               bool active, const Reach3 &r, uint32_t pr_bit, float4 gi, float hi,
+                                               uint32_t adj_i, bool check, Visit &&visit) {
     const bool band = w.rmin2 <= w.rmax2;
     if (__ballot(check)) {
-        if (band) cell_walk3<true, true>(w, st, c, adj_block, i, active, r, gi, hi, adj_i, visit);
-        else cell_walk3<true, false>(w, st, c, adj_block, i, active, r, gi, hi, adj_i, visit);
+        if (band) cell_walk3<true, true>(w, st, c, adj_block, i, active, r, pr_bit, gi, hi, adj_i, visit);
+        else cell_walk3<true, false>(w, st, c, adj_block, i, active, r, pr_bit, gi, hi, adj_i, visit);
     } else {
-        if (band) cell_walk3<false, true>(w, st, c, adj_block, i, active, r, gi, hi, adj_i, visit);
-        else cell_walk3<false, false>(w, st, c, adj_block, i, active, r, gi, hi, adj_i, visit);
+        if (band) cell_walk3<false, true>(w, st, c, adj_block, i, active, r, pr_bit, gi, hi, adj_i, visit);
+        else cell_walk3<false, false>(w, st, c, adj_block, i, active, r, pr_bit, gi, hi, adj_i, visit);
     }
 }
 
@@ -570,21 +598,21 @@ __device__ __forceinline__ void cell_walk3_any(const Walk3 &w, Stage3 &st, int c
 // wave's reach filter, which the search and momentum over the same cell list
 // then compute alike, so stored masks index the same staged order).
 template <bool FAST, class Visit>
-__device__ __forceinline__ void cell_walk3_sel(const Walk3 &w, Stage3 *st2, int c, uint32_t adj_block, int i,
-                                               bool valid, bool active, float4 gi, float hi, uint32_t adj_i,
-                                               bool check, Visit &&visit) {
+__device__ __forceinline__ void cell_walk3_sel(const Walk3 &w, Stage3 &st, int c, uint32_t adj_block, int i,
+                                               bool valid, bool active, uint32_t pr_bit, float4 gi, float hi,
+                                               uint32_t adj_i, bool check, Visit &&visit) {
     const Reach3 r = wave_reach(w, gi, valid);
     if (!FAST) {
-        cell_walk3_any(w, st2[0], c, adj_block, i, active, r, gi, hi, adj_i, check, visit);
+        cell_walk3_any(w, st, c, adj_block, i, active, r, pr_bit, gi, hi, adj_i, check, visit);
         return;
     }
     const bool band = w.rmin2 <= w.rmax2;
     if (__ballot(check)) {
-        if (band) cell_walk3_paired<true, true>(w, st2, c, adj_block, i, active, r, gi, hi, adj_i, visit);
-        else cell_walk3_paired<true, false>(w, st2, c, adj_block, i, active, r, gi, hi, adj_i, visit);
+        if (band) cell_walk3_paired<true, true>(w, st, c, adj_block, i, active, r, pr_bit, gi, hi, adj_i, visit);
+        else cell_walk3_paired<true, false>(w, st, c, adj_block, i, active, r, pr_bit, gi, hi, adj_i, visit);
     } else {
-        if (band) cell_walk3_paired<false, true>(w, st2, c, adj_block, i, active, r, gi, hi, adj_i, visit);
-        else cell_walk3_paired<false, false>(w, st2, c, adj_block, i, active, r, gi, hi, adj_i, visit);
+        if (band) cell_walk3_paired<false, true>(w, st, c, adj_block, i, active, r, pr_bit, gi, hi, adj_i, visit);
+        else cell_walk3_paired<false, false>(w, st, c, adj_block, i, active, r, pr_bit, gi, hi, adj_i, visit);
     }
 }
 
@@ -596,9 +624,9 @@ template <bool FAST, int ACTION, int K>
 __global__ __launch_bounds__(kCellBlock * kCellWaves) void k3_search(Walk3 w, Fields3 f, Physics ph, int32_t M,
                                                                      int32_t *ncount, unsigned long long *pairs,
                                                                      uint32_t *flags) {
-    __shared__ Stage3 stage[kCellWaves][2];
+    __shared__ Stage3 stage[kCellWaves];
     const int wave = threadIdx.x / kCellBlock;
-    Stage3 *st2 = stage[wave];
+    Stage3 &st = stage[wave];
     const int c = stripe_group(blockIdx.x, w.stripe) * kCellWaves + wave;
     if (c >= w.g.n_cells) return;
     const int c0 = w.start[c], c1 = w.start[c + 1];
@@ -624,7 +652,7 @@ __global__ __launch_bounds__(kCellBlock * kCellWaves) void k3_search(Walk3 w, Fi
         const uint32_t adj_i = adj3(w, tbi);
         const bool fluid_i = tbi && tbi == fluid_bit, wall_i = tbi && tbi == wall_bit;
         const bool check = active && (w.odd_tags || (adj_i & w.tags_present) != w.tags_present);
-        cell_walk3_sel<FAST>(w, st2, c, adj_block, i, active, active && adj_i != 0u, gi, hi, adj_i, check,
+        cell_walk3_sel<FAST>(w, st, c, adj_block, i, active, active && adj_i != 0u, 0u, gi, hi, adj_i, check,
                        [&](float4 gj, float4 vj, float4 qj, float dx, float dy, float dz, float dr2) {
                            ++cnt;
                            if (ACTION == EPPX_SEARCH_KERNEL_CONTINUITY) {
@@ -657,9 +685,9 @@ __global__ __launch_bounds__(kCellBlock * kCellWaves) void k3_search(Walk3 w, Fi
 // other particles take no terms (cells without fluid have no work).
 template <bool FAST, int K, int VISC>
 __global__ __launch_bounds__(kCellBlock * kCellWaves) void k3_momentum(Walk3 w, Fields3 f, Physics ph) {
-    __shared__ Stage3 stage[kCellWaves][2];
+    __shared__ Stage3 stage[kCellWaves];
     const int wave = threadIdx.x / kCellBlock;
-    Stage3 *st2 = stage[wave];
+    Stage3 &st = stage[wave];
     const int c = stripe_group(blockIdx.x, w.stripe) * kCellWaves + wave;
     if (c >= w.g.n_cells) return;
     const int c0 = w.start[c], c1 = w.start[c + 1];
@@ -684,7 +712,7 @@ __global__ __launch_bounds__(kCellBlock * kCellWaves) void k3_momentum(Walk3 w, 
         const float p_i = qi.w;
         const float pr_i = FAST ? p_i * frcp(vi.w * vi.w) : p_i / (vi.w * vi.w);
         const bool check = active && (w.odd_tags || (adj_block & w.tags_present) != w.tags_present);
-        cell_walk3_sel<FAST>(w, st2, c, adj_block, i, valid, active, gi, qi.x, adj_block, check,
+        cell_walk3_sel<FAST>(w, st, c, adj_block, i, valid, active, 0u, gi, qi.x, adj_block, check,
                        [&](float4 gj, float4 vj, float4 qj, float dx, float dy, float dz, float dr2) {
                            const uint32_t tbj = __float_as_uint(gj.w);
                            const bool classic = tbj == fluid_bit;
