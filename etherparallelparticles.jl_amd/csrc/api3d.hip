@@ -44,6 +44,7 @@ struct eppx_ps3d {
     unsigned long long *pairs = nullptr;  // [0] last search, [1] running total
     float rmin2 = -1.0f, rmax2 = INFINITY;
     float hinv_u = 0.0f;
+    float m_u = 0.0f;  // one Mass for every alive particle: that mass; else 0 (the 3D actions never change Mass)
     uint32_t tags_present = 0xffffffffu;
     bool odd_tags = true;
     eppx_ns3d *last_ns = nullptr;
@@ -174,6 +175,7 @@ Walk3 make_walk3(const eppx_ps3d *ps, const eppx_ns3d *ns, int32_t criterion) {
         w.rmax2 = ps->rmax2;
     }
     w.hinv_u = ps->hinv_u;
+    w.m_u = ps->m_u;
     w.masks = ps->masks;
     w.mask_stride = ps->cap;
     w.mask_mode = 0;
@@ -471,8 +473,8 @@ int32_t eppx_ps3d_upload(eppx_ps3d *ps, int32_t n_particles, const int32_t *is_a
     ps->n = n_particles;
     // H range and tags of the alive particles: the criterion's exact-safe
     // bounds, the one-H pair hinv and the tag tests the walk needs
-    float hmin = INFINITY, hmax = 0.0f;
-    bool bad = !has(L.H);
+    float hmin = INFINITY, hmax = 0.0f, mmin = INFINITY, mmax = 0.0f;
+    bool bad = !has(L.H), mbad = !has(L.Mass);
     uint32_t tags = 0u;
     bool odd = false;
     for (int32_t r = 0; r < n_particles; ++r) {
@@ -483,6 +485,12 @@ int32_t eppx_ps3d_upload(eppx_ps3d *ps, int32_t n_particles, const int32_t *is_a
             hmin = std::min(hmin, h);
             hmax = std::max(hmax, h);
         }
+        if (has(L.Mass)) {
+            const float m = float_props[(size_t)r + cap * L.Mass];
+            if (!(m > 0.0f) || !std::isfinite(m)) mbad = true;
+            mmin = std::min(mmin, m);
+            mmax = std::max(mmax, m);
+        }
         const int32_t t = int_props[(size_t)r + cap * L.Tag];
         if (t >= 1 && t <= kMaxTag) tags |= 1u << (t - 1);
         else odd = true;
@@ -491,6 +499,7 @@ int32_t eppx_ps3d_upload(eppx_ps3d *ps, int32_t n_particles, const int32_t *is_a
     ps->odd_tags = odd;
     symmetry_bounds(!bad, hmin, hmax, &ps->rmin2, &ps->rmax2);
     ps->hinv_u = uniform_hinv(!bad, hmin, hmax);
+    ps->m_u = uniform_mass(!mbad, mmin, mmax);
     ps->pass_int.assign(ps->pass_int_cols.size() * cap, 0);
     ps->pass_float.assign(ps->pass_float_cols.size() * cap, 0.0f);
     for (size_t k = 0; k < ps->pass_int_cols.size(); ++k)

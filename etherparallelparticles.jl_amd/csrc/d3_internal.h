@@ -55,6 +55,7 @@ struct Walk3 {
     int32_t criterion;
     float rmin2, rmax2;          // the criterion accepts every dr2 < rmin2, rejects every dr2 > rmax2
     float hinv_u;                // > 0: every alive particle has one H, and this is its pair hinv
+    float m_u;                   // > 0: every alive particle has this one Mass
     int32_t stripe;              // groups of 4 cells per stripe of the pair-kernel grid
     int32_t stripes_per_xcd;     // stripes each of the 8 XCDs works through
     // Accept masks of the last search, one 64-bit mask per (stencil cell s, slot i) at

@@ -265,6 +265,49 @@ __device__ __forceinline__ void momentum3(const Physics &ph, float4 &acc, bool c
     }
 }
 
+// One H and one Mass in the case (FAST): every pair term is the 3D kernel's
+// constant factor kfac (Wendland C2: -sigma 0.625 hinv^5, cubic spline:
+// sigma 0.75 hinv^4) times the shape of UniformTerms, so the pair sums carry
+// the rest and the caller adds kfac x sum (and m_u) once.
+template <int K>
+struct Uniform3 : UniformTerms<K> {
+    __device__ __forceinline__ Uniform3(float hinv_u, const Physics &ph) : UniformTerms<K>(hinv_u, ph) {
+        const float h4 = (hinv_u * hinv_u) * (hinv_u * hinv_u);
+        this->kfac = K == EPPX_KERNEL_CUBIC ? (kSigmaCubic3 * 0.75f) * h4 : (-kSigmaWendland3 * 0.625f) * (h4 * hinv_u);
+    }
+};
+
+// momentum3 (FAST) over -m_u kfac, one H and one Mass, coefficient 0: x_j is
+// p_j / rho_j^2 (pressure_ratio) for a classic (fluid) neighbour, p_j for a
+// balanced (wall) one.
+template <int K, int VISC>
+__device__ __forceinline__ void momentum_fast_um3(const Physics &ph, float4 &sum, bool classic,
+                                                  const UniformTerms<K> &ut, float4 vi, float p_i, float pr_i,
+                                                  float4 vj, float x_j, float dx, float dy, float dz, float dr2) {
+    const float rho_i = vi.w, rho_j = vj.w;
+    const float rinv = frsq(dr2);
+    const float shape = ut.shape(dr2 * rinv, rinv);
+    const float rirj = rho_i * rho_j;
+    const float prho2 = classic ? pr_i + x_j : (p_i + x_j) * frcp(rirj);
+    float radial = prho2 * shape;
+    const float qa = fmaf(dr2, ut.hinv2, 0.01f);  // q^2 + 0.01
+    const float dux = vi.x - vj.x, duy = vi.y - vj.y, duz = vi.z - vj.z;
+    if (VISC == EPPX_VISC_CLASSIC) {
+        const float force = (-2.0f * ph.mu) * (dr2 * ut.hinv2) * shape * frcp(rirj * qa);
+        sum.x = fmaf(force, dux, sum.x);
+        sum.y = fmaf(force, duy, sum.y);
+        sum.z = fmaf(force, duz, sum.z);
+    } else {
+        // a separating pair (vdotx > 0) gets no term
+        const float m = fminf(fmaf(dz, duz, fmaf(dy, duy, dx * dux)), 0.0f);
+        const float den = (qa * qa) * ((rho_i + rho_j) * rirj);
+        radial = fmaf(fmaf(ut.av_b, m, -ut.av_a * qa) * m, shape * frcp(den), radial);
+    }
+    sum.x = fmaf(radial, dx, sum.x);
+    sum.y = fmaf(radial, dy, sum.y);
+    sum.z = fmaf(radial, dz, sum.z);
+}
+
 // ------------------------------------------------------------ pair kernels
 // One wave per cell (four cells per 256-thread block, each wave on its own):
 // lane l holds the cell's l-th particle (cells of more than 64 particles take
@@ -620,7 +663,10 @@ __device__ __forceinline__ void cell_walk3_sel(const Walk3 &w, Stage3 &st, int c
 // Search pass: the neighbours of every alive particle, counted into ncount,
 // with the fused action (EPPX_SEARCH_KERNEL_CONTINUITY:
 // sComputeKernelAndiContinuity!, main.jl:74-90, onto the existing dDensity).
-template <bool FAST, int ACTION, int K>
+// UM (FAST, continuity action): every alive particle has one H and one Mass
+// (w.hinv_u, w.m_u), and the terms leave them out (a kernel of its own, so
+// the general path's registers do not limit it, nor its the general path).
+template <bool FAST, int ACTION, int K, bool UM>
 __global__ __launch_bounds__(kCellBlock * kCellWaves) void k3_search(Walk3 w, Fields3 f, Physics ph, int32_t M,
                                                                      int32_t *ncount, unsigned long long *pairs,
                                                                      uint32_t *flags) {
@@ -652,20 +698,44 @@ __global__ __launch_bounds__(kCellBlock * kCellWaves) void k3_search(Walk3 w, Fi
         const uint32_t adj_i = adj3(w, tbi);
         const bool fluid_i = tbi && tbi == fluid_bit, wall_i = tbi && tbi == wall_bit;
         const bool check = active && (w.odd_tags || (adj_i & w.tags_present) != w.tags_present);
-        cell_walk3_sel<FAST>(w, st, c, adj_block, i, active, active && adj_i != 0u, 0u, gi, hi, adj_i, check,
-                       [&](float4 gj, float4 vj, float4 qj, float dx, float dy, float dz, float dr2) {
-                           ++cnt;
-                           if (ACTION == EPPX_SEARCH_KERNEL_CONTINUITY) {
+        if constexpr (UM) {
+            // The fluid-fluid terms (classic continuity) leave out kfac and m_j, the mixed ones
+            // (balanced) kfac only
+            const Uniform3<K> ut(w.hinv_u, ph);
+            float s_ff = 0.0f, s_mx = 0.0f;
+            cell_walk3_sel<FAST>(w, st, c, adj_block, i, active, active && adj_i != 0u, 0u, gi, hi, adj_i, check,
+                           [&](float4 gj, float4 vj, float4 qj, float dx, float dy, float dz, float dr2) {
+                               ++cnt;
                                const uint32_t tbj = __float_as_uint(gj.w);
                                const bool classic = fluid_i && tbj == fluid_bit;
                                const bool balanced = (fluid_i && tbj == wall_bit) || (wall_i && tbj == fluid_bit);
                                if (classic || balanced) {
-                                   const float hinv = w.hinv_u > 0.0f ? w.hinv_u : invharmonic(hi, qj.x);
-                                   const float coef = classic ? qj.y : vi.w * qj.z;
-                                   drho += continuity3<FAST, K>(hinv, vi, vj, coef, dx, dy, dz, dr2);
+                                   const float rinv = frsq(dr2);
+                                   const float shape = ut.shape(dr2 * rinv, rinv);
+                                   const float vdotx =
+                                       fmaf(dz, vi.z - vj.z, fmaf(dy, vi.y - vj.y, dx * (vi.x - vj.x)));
+                                   s_ff += classic ? vdotx * shape : 0.0f;
+                                   s_mx += classic ? 0.0f : ((vi.w * qj.z) * vdotx) * shape;
                                }
-                           }
-                       });
+                           });
+            drho = fmaf(ut.kfac, s_mx, fmaf(ut.kfac * w.m_u, s_ff, drho));
+        } else {
+            cell_walk3_sel<FAST>(w, st, c, adj_block, i, active, active && adj_i != 0u, 0u, gi, hi, adj_i, check,
+                           [&](float4 gj, float4 vj, float4 qj, float dx, float dy, float dz, float dr2) {
+                               ++cnt;
+                               if (ACTION == EPPX_SEARCH_KERNEL_CONTINUITY) {
+                                   const uint32_t tbj = __float_as_uint(gj.w);
+                                   const bool classic = fluid_i && tbj == fluid_bit;
+                                   const bool balanced =
+                                       (fluid_i && tbj == wall_bit) || (wall_i && tbj == fluid_bit);
+                                   if (classic || balanced) {
+                                       const float hinv = w.hinv_u > 0.0f ? w.hinv_u : invharmonic(hi, qj.x);
+                                       const float coef = classic ? qj.y : vi.w * qj.z;
+                                       drho += continuity3<FAST, K>(hinv, vi, vj, coef, dx, dy, dz, dr2);
+                                   }
+                               }
+                           });
+        }
         if (active) {
             ncount[i] = cnt;
             if (cnt > M) atomicOr(flags, kFlagNeighbourOverflow);
@@ -683,7 +753,9 @@ __global__ __launch_bounds__(kCellBlock * kCellWaves) void k3_search(Walk3 w, Fi
 // Momentum pass (iMomentum!, main.jl:107-118): fluid particles add the
 // pressure and viscosity terms of their fluid and wall neighbours to du;
 // other particles take no terms (cells without fluid have no work).
-template <bool FAST, int K, int VISC>
+// UM (FAST): one H and one Mass (w.hinv_u, w.m_u) and pressure coefficient 0;
+// the terms leave them out.
+template <bool FAST, int K, int VISC, bool UM>
 __global__ __launch_bounds__(kCellBlock * kCellWaves) void k3_momentum(Walk3 w, Fields3 f, Physics ph) {
     __shared__ Stage3 stage[kCellWaves];
     const int wave = threadIdx.x / kCellBlock;
@@ -712,14 +784,34 @@ __global__ __launch_bounds__(kCellBlock * kCellWaves) void k3_momentum(Walk3 w, 
         const float p_i = qi.w;
         const float pr_i = FAST ? p_i * frcp(vi.w * vi.w) : p_i / (vi.w * vi.w);
         const bool check = active && (w.odd_tags || (adj_block & w.tags_present) != w.tags_present);
-        cell_walk3_sel<FAST>(w, st, c, adj_block, i, valid, active, 0u, gi, qi.x, adj_block, check,
-                       [&](float4 gj, float4 vj, float4 qj, float dx, float dy, float dz, float dr2) {
-                           const uint32_t tbj = __float_as_uint(gj.w);
-                           const bool classic = tbj == fluid_bit;
-                           if (!classic && tbj != wall_bit) return;
-                           const float hinv = w.hinv_u > 0.0f ? w.hinv_u : invharmonic(qi.x, qj.x);
-                           momentum3<FAST, K, VISC>(ph, acc, classic, hinv, vi, p_i, pr_i, vj, qj, dx, dy, dz, dr2);
-                       });
+        if constexpr (UM) {
+            // The sums leave out -m_u kfac, and the staged pressure of a fluid neighbour is its
+            // p / rho^2
+            const Uniform3<K> ut(w.hinv_u, ph);
+            float4 sum = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            cell_walk3_sel<FAST>(w, st, c, adj_block, i, valid, active, fluid_bit, gi, qi.x, adj_block, check,
+                           [&](float4 gj, float4 vj, float4 qj, float dx, float dy, float dz, float dr2) {
+                               const uint32_t tbj = __float_as_uint(gj.w);
+                               const bool classic = tbj == fluid_bit;
+                               if (!classic && tbj != wall_bit) return;
+                               momentum_fast_um3<K, VISC>(ph, sum, classic, ut, vi, p_i, pr_i, vj, qj.w, dx, dy, dz,
+                                                          dr2);
+                           });
+            const float km = -(w.m_u * ut.kfac);
+            acc.x = fmaf(km, sum.x, acc.x);
+            acc.y = fmaf(km, sum.y, acc.y);
+            acc.z = fmaf(km, sum.z, acc.z);
+        } else {
+            cell_walk3_sel<FAST>(w, st, c, adj_block, i, valid, active, 0u, gi, qi.x, adj_block, check,
+                           [&](float4 gj, float4 vj, float4 qj, float dx, float dy, float dz, float dr2) {
+                               const uint32_t tbj = __float_as_uint(gj.w);
+                               const bool classic = tbj == fluid_bit;
+                               if (!classic && tbj != wall_bit) return;
+                               const float hinv = w.hinv_u > 0.0f ? w.hinv_u : invharmonic(qi.x, qj.x);
+                               momentum3<FAST, K, VISC>(ph, acc, classic, hinv, vi, p_i, pr_i, vj, qj, dx, dy, dz,
+                                                        dr2);
+                           });
+        }
         if (active) f.dvel[i] = acc;
     }
 }
@@ -985,12 +1077,17 @@ hipError_t search_as(const Walk3 &w, Fields3 f, int32_t action, const Physics &p
                      unsigned long long *pairs, uint32_t *flags, int32_t n_slots, hipStream_t s) {
     const dim3 grid((unsigned)(kXcds * w.stripes_per_xcd * w.stripe));
     const dim3 block(kCellBlock * kCellWaves);
-    if (action == EPPX_SEARCH_KERNEL_CONTINUITY)
-        hipLaunchKernelGGL((k3_search<FAST, EPPX_SEARCH_KERNEL_CONTINUITY, K>), grid, block, 0, s, w, f, ph, M, ncount,
-                           pairs, flags);
-    else
-        hipLaunchKernelGGL((k3_search<false, EPPX_SEARCH_NONE, EPPX_KERNEL_CUBIC>), grid, block, 0, s, w, f, ph, M,
-                           ncount, pairs, flags);
+    if (action == EPPX_SEARCH_KERNEL_CONTINUITY) {
+        if (FAST && w.hinv_u > 0.0f && w.m_u > 0.0f)
+            hipLaunchKernelGGL((k3_search<FAST, EPPX_SEARCH_KERNEL_CONTINUITY, K, FAST>), grid, block, 0, s, w, f, ph,
+                               M, ncount, pairs, flags);
+        else
+            hipLaunchKernelGGL((k3_search<FAST, EPPX_SEARCH_KERNEL_CONTINUITY, K, false>), grid, block, 0, s, w, f, ph,
+                               M, ncount, pairs, flags);
+    } else {
+        hipLaunchKernelGGL((k3_search<false, EPPX_SEARCH_NONE, EPPX_KERNEL_CUBIC, false>), grid, block, 0, s, w, f, ph,
+                           M, ncount, pairs, flags);
+    }
     return hipGetLastError();
 }
 
@@ -998,10 +1095,14 @@ template <bool FAST, int K>
 hipError_t momentum_as(const Walk3 &w, Fields3 f, const Physics &ph, int32_t n_slots, hipStream_t s) {
     const dim3 grid((unsigned)(kXcds * w.stripes_per_xcd * w.stripe));
     const dim3 block(kCellBlock * kCellWaves);
-    if (ph.visc == EPPX_VISC_CLASSIC)
-        hipLaunchKernelGGL((k3_momentum<FAST, K, EPPX_VISC_CLASSIC>), grid, block, 0, s, w, f, ph);
-    else
-        hipLaunchKernelGGL((k3_momentum<FAST, K, EPPX_VISC_ARTIFICIAL>), grid, block, 0, s, w, f, ph);
+    const bool um = FAST && w.hinv_u > 0.0f && w.m_u > 0.0f && ph.coefficient == 0.0f;
+    if (ph.visc == EPPX_VISC_CLASSIC) {
+        if (um) hipLaunchKernelGGL((k3_momentum<FAST, K, EPPX_VISC_CLASSIC, FAST>), grid, block, 0, s, w, f, ph);
+        else hipLaunchKernelGGL((k3_momentum<FAST, K, EPPX_VISC_CLASSIC, false>), grid, block, 0, s, w, f, ph);
+    } else {
+        if (um) hipLaunchKernelGGL((k3_momentum<FAST, K, EPPX_VISC_ARTIFICIAL, FAST>), grid, block, 0, s, w, f, ph);
+        else hipLaunchKernelGGL((k3_momentum<FAST, K, EPPX_VISC_ARTIFICIAL, false>), grid, block, 0, s, w, f, ph);
+    }
     return hipGetLastError();
 }
 

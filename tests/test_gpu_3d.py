@@ -8,7 +8,8 @@ restatement, itself pinned by closed forms, invariants and brute force
 * EXACT math: pair sets, neighbour columns, dDensity and whole steps
   bit-identical to the restatement (small and medium dam breaks, a jittered
   lattice with varying H and wall tags, the default criterion).
-* FAST math, on a flow developed on the device: pair sets bit-exact;
+* FAST math, on a flow developed on the device (one H and one Mass) and on
+  the jittered lattice (H and Mass per particle): pair sets bit-exact;
   dDensity and acceleration per particle within 1e-5 of the sum of |terms|
   of the particle's sum (the float32 bound of a sum taken in another order
   and with other rounding of its terms), density 1e-5 relative.
@@ -311,6 +312,42 @@ def test_3d_fast_within_bound(parallel, o3):
     assert drho.max() <= 1.0
     assert du.max() <= 1.0
     assert np.array_equal(g[:, RL.Density], r[:, RL.Density])
+
+
+def test_3d_fast_varying_h_within_bound(parallel, o3):
+    """FAST math where H and Mass vary per particle (the terms that take each pair's hinv and m_j), on the jittered
+    lattice of fluid and wall particles: pairs bit-exact, dDensity and acceleration within 1e-5 of their bounds."""
+    import oracle as O
+    from eppx import Algorithm, SPH
+    case = jittered_case(o3)  # Mass = Density x Volume with the density jittered: one Mass per particle
+    parallel.set_math("fast")
+    try:
+        ps, ns = device3(parallel, case)
+        search_device(ps, case, ns)
+        Algorithm.interaction(ps, SPH.iMomentum(case.physics))
+        off, js = export_pairs3(ps, ns)
+        got = download(ps)
+        ps.close()
+        ns.close()
+    finally:
+        parallel.set_math("exact")
+    ref = reference3(o3, case)
+    oph = oracle_physics(o3, case.physics)
+    assert ref.search(O.CRIT_SYMMETRY, 1, oph) == 0
+    roff, rjs = ref.pair_csr()
+    assert np.array_equal(off, roff) and np.array_equal(js, rjs)
+    bound_c = ref.term_magnitude(0, oph)
+    bound_m = ref.term_magnitude(1, oph)
+    ref.interaction(1, oph)
+    n = ref.n
+    RL = ref.L
+    g, r = got.float_properties_[:n], ref.FP[:n]
+    drho = np.abs(g[:, RL.dDensity].astype(np.float64) - r[:, RL.dDensity]) / (1e-5 * bound_c + 1e-30)
+    du = np.abs(g[:, RL.dVelocityVec: RL.dVelocityVec + 3].astype(np.float64)
+                - r[:, RL.dVelocityVec: RL.dVelocityVec + 3]) / (1e-5 * bound_m + 1e-30)
+    print(f"3D FAST, varying H and Mass: dDensity {drho.max():.3g} of 1e-5 x its bound, acceleration {du.max():.3g}")
+    assert drho.max() <= 1.0
+    assert du.max() <= 1.0
 
 
 # ------------------------------------------------------- benchmark config
