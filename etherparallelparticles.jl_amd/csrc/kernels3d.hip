@@ -277,23 +277,28 @@ struct Uniform3 : UniformTerms<K> {
     }
 };
 
-// momentum3 (FAST) over -m_u kfac, one H and one Mass, coefficient 0: x_j is
-// p_j / rho_j^2 (pressure_ratio) for a classic (fluid) neighbour, p_j for a
-// balanced (wall) one.
+// momentum3 (FAST) over -m_u kfac, one H and one Mass, coefficient 0, with the
+// staged record qj of StageX: (Mass, p_j / rho_j^2) for a classic (fluid)
+// neighbour, (1 / rho_j, p_j / rho_j) for a balanced (wall) one, whose
+// (p_i + p_j) / (rho_i rho_j) is then (p_i / rho_j + p_j / rho_j) / rho_i.
+// take = false: the pair adds nothing (its term is computed all the same and
+// dropped by a select, so not even a NaN of it gets through).
 template <int K, int VISC>
-__device__ __forceinline__ void momentum_fast_um3(const Physics &ph, float4 &sum, bool classic,
+__device__ __forceinline__ void momentum_fast_um3(const Physics &ph, float4 &sum, bool classic, bool take,
                                                   const UniformTerms<K> &ut, float4 vi, float p_i, float pr_i,
-                                                  float4 vj, float x_j, float dx, float dy, float dz, float dr2) {
+                                                  float inv_rho_i, float4 vj, float4 qj, float dx, float dy, float dz,
+                                                  float dr2) {
     const float rho_i = vi.w, rho_j = vj.w;
     const float rinv = frsq(dr2);
     const float shape = ut.shape(dr2 * rinv, rinv);
     const float rirj = rho_i * rho_j;
-    const float prho2 = classic ? pr_i + x_j : (p_i + x_j) * frcp(rirj);
+    const float prho2 = classic ? pr_i + qj.w : fmaf(p_i, qj.y, qj.w) * inv_rho_i;
     float radial = prho2 * shape;
     const float qa = fmaf(dr2, ut.hinv2, 0.01f);  // q^2 + 0.01
     const float dux = vi.x - vj.x, duy = vi.y - vj.y, duz = vi.z - vj.z;
     if (VISC == EPPX_VISC_CLASSIC) {
-        const float force = (-2.0f * ph.mu) * (dr2 * ut.hinv2) * shape * frcp(rirj * qa);
+        float force = (-2.0f * ph.mu) * (dr2 * ut.hinv2) * shape * frcp(rirj * qa);
+        force = take ? force : 0.0f;
         sum.x = fmaf(force, dux, sum.x);
         sum.y = fmaf(force, duy, sum.y);
         sum.z = fmaf(force, duz, sum.z);
@@ -303,6 +308,7 @@ __device__ __forceinline__ void momentum_fast_um3(const Physics &ph, float4 &sum
         const float den = (qa * qa) * ((rho_i + rho_j) * rirj);
         radial = fmaf(fmaf(ut.av_b, m, -ut.av_a * qa) * m, shape * frcp(den), radial);
     }
+    radial = take ? radial : 0.0f;
     sum.x = fmaf(radial, dx, sum.x);
     sum.y = fmaf(radial, dy, sum.y);
     sum.z = fmaf(radial, dz, sum.z);
@@ -347,7 +353,7 @@ __device__ __forceinline__ void wave_sync() {
 struct Stage3 {
     float4 g[2 * kCellBlock];  // (x, y, z, tag bit) of the staged slots
     float4 v[2 * kCellBlock];  // (u, v, w, rho)
-    float4 q[2 * kCellBlock];  // (H, Mass, Volume, Pressure); see stage_reach for pr_bit
+    float4 q[2 * kCellBlock];  // (H, Mass, Volume, Pressure); see StageX
 };
 
 // Adjacency of any tag present in cell c (a cell holding no particle of a tag
@@ -392,13 +398,19 @@ __device__ __forceinline__ Reach3 wave_reach(const Walk3 &w, float4 gi, bool val
     return r;
 }
 
+// What the one-H, one-Mass momentum stages in place of a slot's (Mass,
+// Pressure): for the tag bit `fluid`, (Mass, p / rho^2) (pressure_ratio); for
+// `wall`, (1 / rho, p / rho).  Zero bits: the records as they are.
+struct StageX {
+    uint32_t fluid = 0u, wall = 0u;
+};
+
 // Stages the slots of [base, base + len) (len <= 64) within reach, in slot
 // order, from position off: slot base + L goes to off + popcount(bits of the
 // ballot below L).  Returns the ballot.  A staged g.w holds the slot's tag
-// bit; with pr_bit, the staged pressure of a slot with that tag bit is
-// p / rho^2 (pressure_ratio) instead of p.  Every lane of the wave calls it.
+// bit; sx: see StageX.  Every lane of the wave calls it.
 __device__ __forceinline__ unsigned long long stage_reach(const Walk3 &w, Stage3 &st, const Reach3 &r, int base,
-                                                          int len, int off, uint32_t pr_bit) {
+                                                          int len, int off, StageX sx) {
     const int lane = lane_id();
     float4 g = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     bool in = false;
@@ -416,7 +428,13 @@ __device__ __forceinline__ unsigned long long stage_reach(const Walk3 &w, Stage3
         g.w = __uint_as_float(tb);
         const float4 v = w.vel[base + lane];
         float4 q = w.hmvp[base + lane];
-        if (pr_bit && tb == pr_bit) q.w = pressure_ratio(q.w, v.w);
+        if (sx.fluid && tb == sx.fluid) {
+            q.w = pressure_ratio(q.w, v.w);
+        } else if (sx.wall && tb == sx.wall) {
+            const float ir = frcp(v.w);
+            q.y = ir;
+            q.w *= ir;
+        }
         st.g[pos] = g;
         st.v[pos] = v;
         st.q[pos] = q;
@@ -461,7 +479,7 @@ __device__ __forceinline__ unsigned long long self_bit(unsigned long long bal, i
 // 2) index the staged order of a cell of at most 64 particles.
 template <bool CHECK, bool BAND, class Visit>
 __device__ __forceinline__ void cell_walk3(const Walk3 &w, Stage3 &st, int c, uint32_t adj_block, int i, bool active,
-                                           const Reach3 &reach, uint32_t pr_bit, float4 gi, float hi, uint32_t adj_i,
+                                           const Reach3 &reach, StageX sx, float4 gi, float hi, uint32_t adj_i,
                                            Visit &&visit) {
     const Grid3 &g = w.g;
     const int cx = c % g.n_x, t = c / g.n_x;
@@ -488,7 +506,7 @@ __device__ __forceinline__ void cell_walk3(const Walk3 &w, Stage3 &st, int c, ui
                 }
                 for (int base = b; base < e; base += kCellBlock) {
                     const int raw = min(kCellBlock, e - base);
-                    const unsigned long long bal = stage_reach(w, st, reach, base, raw, 0, pr_bit);
+                    const unsigned long long bal = stage_reach(w, st, reach, base, raw, 0, sx);
                     const int len = __popcll(bal);
                     wave_sync();
                     if (active) {
@@ -527,7 +545,7 @@ __device__ __forceinline__ void cell_walk3(const Walk3 &w, Stage3 &st, int c, ui
 // order than the reference's list order (FAST math only).
 template <bool CHECK, bool BAND, class Visit>
 __device__ __forceinline__ void cell_walk3_paired(const Walk3 &w, Stage3 &st, int c, uint32_t adj_block, int i,
-                                                  bool active, const Reach3 &reach, uint32_t pr_bit, float4 gi,
+                                                  bool active, const Reach3 &reach, StageX sx, float4 gi,
                                                   float hi, uint32_t adj_i, Visit &&visit) {
     const Grid3 &g = w.g;
     const int cx = c % g.n_x, t = c / g.n_x;
@@ -568,9 +586,9 @@ __device__ __forceinline__ void cell_walk3_paired(const Walk3 &w, Stage3 &st, in
             const int base_a = ba + ch * kCellBlock, base_b = bb + ch * kCellBlock;
             const int raw_a = need_a ? min(kCellBlock, max(0, ea - base_a)) : 0;
             const int raw_b = need_b ? min(kCellBlock, max(0, eb - base_b)) : 0;
-            const unsigned long long bal_a = stage_reach(w, st, reach, base_a, raw_a, 0, pr_bit);
+            const unsigned long long bal_a = stage_reach(w, st, reach, base_a, raw_a, 0, sx);
             const int len_a = __popcll(bal_a);
-            const unsigned long long bal_b = stage_reach(w, st, reach, base_b, raw_b, len_a, pr_bit);
+            const unsigned long long bal_b = stage_reach(w, st, reach, base_b, raw_b, len_a, sx);
             const int len_b = __popcll(bal_b);
             wave_sync();
             if (active) {
@@ -625,15 +643,15 @@ __device__ __forceinline__ void cell_walk3_paired(const Walk3 &w, Stage3 &st, in
 
 template <class Visit>
 __device__ __forceinline__ void cell_walk3_any(const Walk3 &w, Stage3 &st, int c, uint32_t adj_block, int i,
-                                               bool active, const Reach3 &r, uint32_t pr_bit, float4 gi, float hi,
+                                               bool active, const Reach3 &r, StageX sx, float4 gi, float hi,
                                                uint32_t adj_i, bool check, Visit &&visit) {
     const bool band = w.rmin2 <= w.rmax2;
     if (__ballot(check)) {
-        if (band) cell_walk3<true, true>(w, st, c, adj_block, i, active, r, pr_bit, gi, hi, adj_i, visit);
-        else cell_walk3<true, false>(w, st, c, adj_block, i, active, r, pr_bit, gi, hi, adj_i, visit);
+        if (band) cell_walk3<true, true>(w, st, c, adj_block, i, active, r, sx, gi, hi, adj_i, visit);
+        else cell_walk3<true, false>(w, st, c, adj_block, i, active, r, sx, gi, hi, adj_i, visit);
     } else {
-        if (band) cell_walk3<false, true>(w, st, c, adj_block, i, active, r, pr_bit, gi, hi, adj_i, visit);
-        else cell_walk3<false, false>(w, st, c, adj_block, i, active, r, pr_bit, gi, hi, adj_i, visit);
+        if (band) cell_walk3<false, true>(w, st, c, adj_block, i, active, r, sx, gi, hi, adj_i, visit);
+        else cell_walk3<false, false>(w, st, c, adj_block, i, active, r, sx, gi, hi, adj_i, visit);
     }
 }
 
@@ -642,20 +660,20 @@ __device__ __forceinline__ void cell_walk3_any(const Walk3 &w, Stage3 &st, int c
 // then compute alike, so stored masks index the same staged order).
 template <bool FAST, class Visit>
 __device__ __forceinline__ void cell_walk3_sel(const Walk3 &w, Stage3 &st, int c, uint32_t adj_block, int i,
-                                               bool valid, bool active, uint32_t pr_bit, float4 gi, float hi,
+                                               bool valid, bool active, StageX sx, float4 gi, float hi,
                                                uint32_t adj_i, bool check, Visit &&visit) {
     const Reach3 r = wave_reach(w, gi, valid);
     if (!FAST) {
-        cell_walk3_any(w, st, c, adj_block, i, active, r, pr_bit, gi, hi, adj_i, check, visit);
+        cell_walk3_any(w, st, c, adj_block, i, active, r, sx, gi, hi, adj_i, check, visit);
         return;
     }
     const bool band = w.rmin2 <= w.rmax2;
     if (__ballot(check)) {
-        if (band) cell_walk3_paired<true, true>(w, st, c, adj_block, i, active, r, pr_bit, gi, hi, adj_i, visit);
-        else cell_walk3_paired<true, false>(w, st, c, adj_block, i, active, r, pr_bit, gi, hi, adj_i, visit);
+        if (band) cell_walk3_paired<true, true>(w, st, c, adj_block, i, active, r, sx, gi, hi, adj_i, visit);
+        else cell_walk3_paired<true, false>(w, st, c, adj_block, i, active, r, sx, gi, hi, adj_i, visit);
     } else {
-        if (band) cell_walk3_paired<false, true>(w, st, c, adj_block, i, active, r, pr_bit, gi, hi, adj_i, visit);
-        else cell_walk3_paired<false, false>(w, st, c, adj_block, i, active, r, pr_bit, gi, hi, adj_i, visit);
+        if (band) cell_walk3_paired<false, true>(w, st, c, adj_block, i, active, r, sx, gi, hi, adj_i, visit);
+        else cell_walk3_paired<false, false>(w, st, c, adj_block, i, active, r, sx, gi, hi, adj_i, visit);
     }
 }
 
@@ -666,8 +684,10 @@ __device__ __forceinline__ void cell_walk3_sel(const Walk3 &w, Stage3 &st, int c
 // UM (FAST, continuity action): every alive particle has one H and one Mass
 // (w.hinv_u, w.m_u), and the terms leave them out (a kernel of its own, so
 // the general path's registers do not limit it, nor its the general path).
+// Six waves per SIMD (at most 80 VGPRs; a few spill on rare paths) run the
+// search faster than five; momentum spills in its pair loop at six.
 template <bool FAST, int ACTION, int K, bool UM>
-__global__ __launch_bounds__(kCellBlock * kCellWaves) void k3_search(Walk3 w, Fields3 f, Physics ph, int32_t M,
+__global__ __launch_bounds__(kCellBlock * kCellWaves, 6) void k3_search(Walk3 w, Fields3 f, Physics ph, int32_t M,
                                                                      int32_t *ncount, unsigned long long *pairs,
                                                                      uint32_t *flags) {
     __shared__ Stage3 stage[kCellWaves];
@@ -703,24 +723,22 @@ __global__ __launch_bounds__(kCellBlock * kCellWaves) void k3_search(Walk3 w, Fi
             // (balanced) kfac only
             const Uniform3<K> ut(w.hinv_u, ph);
             float s_ff = 0.0f, s_mx = 0.0f;
-            cell_walk3_sel<FAST>(w, st, c, adj_block, i, active, active && adj_i != 0u, 0u, gi, hi, adj_i, check,
+            cell_walk3_sel<FAST>(w, st, c, adj_block, i, active, active && adj_i != 0u, StageX{}, gi, hi, adj_i, check,
                            [&](float4 gj, float4 vj, float4 qj, float dx, float dy, float dz, float dr2) {
                                ++cnt;
                                const uint32_t tbj = __float_as_uint(gj.w);
                                const bool classic = fluid_i && tbj == fluid_bit;
                                const bool balanced = (fluid_i && tbj == wall_bit) || (wall_i && tbj == fluid_bit);
-                               if (classic || balanced) {
-                                   const float rinv = frsq(dr2);
-                                   const float shape = ut.shape(dr2 * rinv, rinv);
-                                   const float vdotx =
-                                       fmaf(dz, vi.z - vj.z, fmaf(dy, vi.y - vj.y, dx * (vi.x - vj.x)));
-                                   s_ff += classic ? vdotx * shape : 0.0f;
-                                   s_mx += classic ? 0.0f : ((vi.w * qj.z) * vdotx) * shape;
-                               }
+                               // (selects, not a branch: the pair's records are read at once)
+                               const float rinv = frsq(dr2);
+                               const float shape = ut.shape(dr2 * rinv, rinv);
+                               const float vdotx = fmaf(dz, vi.z - vj.z, fmaf(dy, vi.y - vj.y, dx * (vi.x - vj.x)));
+                               s_ff += classic ? vdotx * shape : 0.0f;
+                               s_mx += balanced ? ((vi.w * qj.z) * vdotx) * shape : 0.0f;
                            });
             drho = fmaf(ut.kfac, s_mx, fmaf(ut.kfac * w.m_u, s_ff, drho));
         } else {
-            cell_walk3_sel<FAST>(w, st, c, adj_block, i, active, active && adj_i != 0u, 0u, gi, hi, adj_i, check,
+            cell_walk3_sel<FAST>(w, st, c, adj_block, i, active, active && adj_i != 0u, StageX{}, gi, hi, adj_i, check,
                            [&](float4 gj, float4 vj, float4 qj, float dx, float dy, float dz, float dr2) {
                                ++cnt;
                                if (ACTION == EPPX_SEARCH_KERNEL_CONTINUITY) {
@@ -787,22 +805,26 @@ __global__ __launch_bounds__(kCellBlock * kCellWaves) void k3_momentum(Walk3 w, 
         if constexpr (UM) {
             // The sums leave out -m_u kfac, and the staged pressure of a fluid neighbour is its
             // p / rho^2
+            // every accepted neighbour is fluid or wall unless a third tag is adjacent to the fluid
+            const bool fw_only = (adj_block & ~(fluid_bit | wall_bit)) == 0u;
             const Uniform3<K> ut(w.hinv_u, ph);
+            const float inv_rho_i = frcp(vi.w);
             float4 sum = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            cell_walk3_sel<FAST>(w, st, c, adj_block, i, valid, active, fluid_bit, gi, qi.x, adj_block, check,
+            cell_walk3_sel<FAST>(w, st, c, adj_block, i, valid, active, StageX{fluid_bit, wall_bit}, gi, qi.x,
+                                 adj_block, check,
                            [&](float4 gj, float4 vj, float4 qj, float dx, float dy, float dz, float dr2) {
                                const uint32_t tbj = __float_as_uint(gj.w);
                                const bool classic = tbj == fluid_bit;
-                               if (!classic && tbj != wall_bit) return;
-                               momentum_fast_um3<K, VISC>(ph, sum, classic, ut, vi, p_i, pr_i, vj, qj.w, dx, dy, dz,
-                                                          dr2);
+                               const bool take = fw_only || classic || tbj == wall_bit;
+                               momentum_fast_um3<K, VISC>(ph, sum, classic, take, ut, vi, p_i, pr_i, inv_rho_i, vj,
+                                                          qj, dx, dy, dz, dr2);
                            });
             const float km = -(w.m_u * ut.kfac);
             acc.x = fmaf(km, sum.x, acc.x);
             acc.y = fmaf(km, sum.y, acc.y);
             acc.z = fmaf(km, sum.z, acc.z);
         } else {
-            cell_walk3_sel<FAST>(w, st, c, adj_block, i, valid, active, 0u, gi, qi.x, adj_block, check,
+            cell_walk3_sel<FAST>(w, st, c, adj_block, i, valid, active, StageX{}, gi, qi.x, adj_block, check,
                            [&](float4 gj, float4 vj, float4 qj, float dx, float dy, float dz, float dr2) {
                                const uint32_t tbj = __float_as_uint(gj.w);
                                const bool classic = tbj == fluid_bit;

@@ -8,8 +8,9 @@ restatement, itself pinned by closed forms, invariants and brute force
 * EXACT math: pair sets, neighbour columns, dDensity and whole steps
   bit-identical to the restatement (small and medium dam breaks, a jittered
   lattice with varying H and wall tags, the default criterion).
-* FAST math, on a flow developed on the device (one H and one Mass) and on
-  the jittered lattice (H and Mass per particle): pair sets bit-exact;
+* FAST math, on flows developed on the device (the dam break's parts with
+  their own H and Mass, and the same with one H and one Mass everywhere) and
+  on the jittered lattice (H and Mass per particle): pair sets bit-exact;
   dDensity and acceleration per particle within 1e-5 of the sum of |terms|
   of the particle's sum (the float32 bound of a sum taken in another order
   and with other rounding of its terms), density 1e-5 relative.
@@ -309,6 +310,59 @@ def test_3d_fast_within_bound(parallel, o3):
     du = np.abs(g[:, RL.dVelocityVec: RL.dVelocityVec + 3].astype(np.float64)
                 - r[:, RL.dVelocityVec: RL.dVelocityVec + 3]) / (1e-5 * bound_m[alive] + 1e-30)
     print(f"3D FAST: dDensity {drho.max():.3g} of 1e-5 x its bound, acceleration {du.max():.3g}")
+    assert drho.max() <= 1.0
+    assert du.max() <= 1.0
+    assert np.array_equal(g[:, RL.Density], r[:, RL.Density])
+
+
+def test_3d_fast_one_h_one_mass_within_bound(parallel, o3):
+    """FAST math with one H and one Mass for every particle (the kernels that factor them out of the sums): the
+    small dam break with H = 2 dx and Mass = rho0 dx^3 everywhere, 200 EXACT steps in, then one FAST search +
+    continuity and momentum pass against the restatement."""
+    import dataclasses
+    import oracle as O
+    from eppx import Algorithm, SPH
+    case = small_case()
+    n = case.n_particles
+    case.host["H"] = np.full(n, np.float32(2 * 0.05), np.float32)
+    case.host["Mass"] = np.full(n, np.float32(1000.0 * 0.05 ** 3), np.float32)
+    ps, ns = device3(parallel, case)
+    search_device(ps, case, ns)
+    Algorithm.step(ps, ns, case.physics, n_steps=200)
+    host = download(ps)
+    ps.close()
+    ns.close()
+    host.float_properties_[:, host.named_index_.float_table.index["dDensity"] - 1] = 0.0
+    dev = dataclasses.replace(case, host=host)
+    L = host.layout
+    assert len(np.unique(host.float_properties_[:n, L.H])) == 1
+    assert len(np.unique(host.float_properties_[:n, L.Mass])) == 1
+    parallel.set_math("fast")
+    try:
+        ps, ns = device3(parallel, dev)
+        search_device(ps, dev, ns)
+        Algorithm.interaction(ps, SPH.iMomentum(dev.physics))
+        off, js = export_pairs3(ps, ns)
+        got = download(ps)
+        ps.close()
+        ns.close()
+    finally:
+        parallel.set_math("exact")
+    ref = reference3(o3, dev)
+    oph = oracle_physics(o3, dev.physics)
+    assert ref.search(O.CRIT_SYMMETRY, 1, oph) == 0
+    roff, rjs = ref.pair_csr()
+    assert np.array_equal(off, roff) and np.array_equal(js, rjs)
+    bound_c = ref.term_magnitude(0, oph)
+    bound_m = ref.term_magnitude(1, oph)
+    ref.interaction(1, oph)
+    RL = ref.L
+    alive = ref.is_alive[: ref.n] == 1
+    g, r = got.float_properties_[: ref.n][alive], ref.FP[: ref.n][alive]
+    drho = np.abs(g[:, RL.dDensity].astype(np.float64) - r[:, RL.dDensity]) / (1e-5 * bound_c[alive] + 1e-30)
+    du = np.abs(g[:, RL.dVelocityVec: RL.dVelocityVec + 3].astype(np.float64)
+                - r[:, RL.dVelocityVec: RL.dVelocityVec + 3]) / (1e-5 * bound_m[alive] + 1e-30)
+    print(f"3D FAST, one H and one Mass: dDensity {drho.max():.3g} of 1e-5 x its bound, acceleration {du.max():.3g}")
     assert drho.max() <= 1.0
     assert du.max() <= 1.0
     assert np.array_equal(g[:, RL.Density], r[:, RL.Density])
