@@ -35,6 +35,10 @@ template <int BLOCK>
 struct PairShape {
     static constexpr int tile_row = BLOCK + BLOCK / 2;
 };
+// The tile arrays pass 1 reads keep their last kTilePad entries free (the
+// wave-uniform walk reads up to 63 positions past a lane's row range), so the
+// ranges a tile stages hold at most 3 tile_row - kTilePad slots.
+constexpr int kTilePad = 64;
 
 // sigma of the 2D kernels: FT(10/7/pi) (CubicSpline.jl:14), FT(7/4/pi) (WendlandC2.jl:13)
 constexpr float kSigmaCubic = (float)(10.0 / 7.0 / 3.14159265358979323846);
@@ -421,6 +425,66 @@ __device__ __forceinline__ unsigned long long accept_row(const Walk &w, const Ca
     return m;
 }
 
+// Largest v over the wave's active lanes, for 0 <= v < 128: a binary search
+// on ballots, so the result is wave-uniform (scalar registers).
+__device__ __forceinline__ int wave_max_u7(int v) {
+    int L = 0;
+#pragma unroll
+    for (int bit = 6; bit >= 0; --bit) {
+        const int c = L | (1 << bit);
+        if (__ballot(v >= c)) L = c;
+    }
+    return L;
+}
+
+// Pass 1 of one row for a lane whose rows are staged in LDS, when the
+// criterion is one threshold (no band) and every tag lies in 1..kMaxTag.
+// The wave steps through positions 0..L-1 of its lanes' row ranges together
+// (L: the longest range among its active lanes), four candidates a step, the
+// loop control in scalar registers.  Per candidate a lane shifts the sign bit
+// of dr2 - rmin2 into a 32-bit word (v_alignbit): the difference of two
+// finite floats is negative exactly when dr2 < rmin2 (it is zero only when
+// they are equal), and dr2 is the reference's two-rounding sum, so the
+// decision is the threshold test's bit for bit.  With TAGS the candidate's
+// adjacency enters a second word the same way (adj_rev has bit 31 - t set
+// when tag t + 1 is adjacent).  The first candidate shifted lies highest, so
+// a word is bit-reversed at the end; positions past the lane's n slots are
+// cleared.  s: tile index of the lane's first slot of the row; the tile
+// keeps kTilePad entries past the staged ranges, so reads up to s + 63 stay
+// inside it.
+template <bool TAGS, class Src>
+__device__ __forceinline__ unsigned long long accept_row_tiled(const Src &src, int s, int n, int L, float2 xi,
+                                                               float rmin2, uint32_t adj_rev) {
+    uint32_t word[2] = {0u, 0u};
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int k0 = 32 * h;
+        if (L > k0) {  // wave-uniform
+            const int cnt = min(L - k0, 32);
+            const int sk = s + k0;
+            uint32_t acc = 0u, tacc = 0u;
+            int k = 0;
+            do {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const float2 g = src.xy_of(sk + k + q);
+                    const float dx = xi.x - g.x, dy = xi.y - g.y;
+                    const float d = dx * dx + dy * dy;
+                    acc = __builtin_amdgcn_alignbit(acc, __float_as_uint(d - rmin2), 31);
+                    if (TAGS) tacc = __builtin_amdgcn_alignbit(tacc, adj_rev << (src.tix[sk + k + q] & 31u), 31);
+                }
+                k += 4;
+            } while (k < cnt);
+            // k candidates shifted (a multiple of 4, at most 32): candidate q sits at bit k - 1 - q
+            uint32_t wd = __builtin_bitreverse32(acc) >> (32 - k);
+            if (TAGS) wd &= __builtin_bitreverse32(tacc) >> (32 - k);
+            word[h] = wd;
+        }
+    }
+    const unsigned long long m = ((unsigned long long)word[1] << 32) | word[0];
+    return n > 0 ? m & (~0ull >> (64 - n)) : 0ull;
+}
+
 template <bool ORDERED, bool COUNT, class Src, class Fetch, class Apply>
 __device__ __forceinline__ int walk_rows(const Walk &w, const CaseDesc &cd, const Src &src, int i, float4 gi,
                                          uint32_t keep_tags, Fetch &&fetch, Apply &&apply) {
@@ -467,7 +531,29 @@ __device__ __forceinline__ int walk_rows(const Walk &w, const CaseDesc &cd, cons
         // The band and the tag test are skipped for the whole wave when no
         // lane needs them: the band is empty and every tag present is in kadj.
         const bool check = __ballot(cd.check_all || (cd.tags_present & ~kadj) != 0u) != 0ull;
-        if (check) {
+        bool done = false;
+        if constexpr (Src::kTile) {
+            if (__ballot(cd.check_all) == 0ull) {  // one threshold, tags in 1..kMaxTag: the wave-uniform pass 1
+                const int n0 = e[0] - b[0], n1 = e[1] - b[1], n2 = e[2] - b[2];
+                const int L = wave_max_u7(max(n0, max(n1, n2)));
+                const int s0 = n0 > 0 ? b[0] + src.off(0) : 0;
+                const int s1 = n1 > 0 ? b[1] + src.off(1) : 0;
+                const int s2 = n2 > 0 ? b[2] + src.off(2) : 0;
+                if (check) {
+                    const uint32_t adj_rev = __builtin_bitreverse32(kadj);
+                    m0 = accept_row_tiled<true>(src, s0, n0, L, xi, cd.rmin2, adj_rev);
+                    m1 = accept_row_tiled<true>(src, s1, n1, L, xi, cd.rmin2, adj_rev);
+                    m2 = accept_row_tiled<true>(src, s2, n2, L, xi, cd.rmin2, adj_rev);
+                } else {
+                    m0 = accept_row_tiled<false>(src, s0, n0, L, xi, cd.rmin2, 0u);
+                    m1 = accept_row_tiled<false>(src, s1, n1, L, xi, cd.rmin2, 0u);
+                    m2 = accept_row_tiled<false>(src, s2, n2, L, xi, cd.rmin2, 0u);
+                }
+                done = true;
+            }
+        }
+        if (done) {
+        } else if (check) {
             m0 = accept_row<true>(w, cd, src, 0, b[0], e[0], xi, gi.z, kadj);
             m1 = accept_row<true>(w, cd, src, 1, b[1], e[1], xi, gi.z, kadj);
             m2 = accept_row<true>(w, cd, src, 2, b[2], e[2], xi, gi.z, kadj);
@@ -496,8 +582,8 @@ __device__ __forceinline__ int walk_rows(const Walk &w, const CaseDesc &cd, cons
     // dr2 of an accepted pair: the reference's two roundings in the ordered
     // (EXACT) walk; one FMA in the FAST one, where it only feeds the pair terms
     // (the accept decision above stays the reference's)
-    auto visit = [&](int j, int roff, uint32_t tbj, const auto &p) {
-        const float4 gj = src.get(j + roff);
+    auto visit = [&](int j, int k, uint32_t tbj, const auto &p) {  // slot j, tile index k
+        const float4 gj = src.get(k);
         const float dx = xi.x - gj.x, dy = xi.y - gj.y;
         apply(j, tbj, gj.z, gj.w, dx, dy, ORDERED ? dx * dx + dy * dy : fmaf(dx, dx, dy * dy), p);
     };
@@ -516,7 +602,7 @@ __device__ __forceinline__ int walk_rows(const Walk &w, const CaseDesc &cd, cons
                 const int j = cb + __builtin_ctzll(m);
                 m &= m - 1ull;
                 const uint32_t tbj = src.tbit_adj(j + roff);
-                visit(j, roff, tbj, fetch(j, j + roff, tbj, src));
+                visit(j, j + roff, tbj, fetch(j, j + roff, tbj, src));
             }
         }
     } else {
@@ -531,7 +617,7 @@ __device__ __forceinline__ int walk_rows(const Walk &w, const CaseDesc &cd, cons
             const int j = ba + __builtin_ctzll(na), jo = oa;
             na &= na - 1ull;
             const uint32_t tbj = src.tbit_adj(j + jo);
-            visit(j, jo, tbj, fetch(j, j + jo, tbj, src));
+            visit(j, j + jo, tbj, fetch(j, j + jo, tbj, src));
             // the switch after the term: nothing of this pair is live across it
             if (!na) { na = nb; ba = bb; oa = ob; nb = 0ull; }
         }
@@ -541,7 +627,7 @@ __device__ __forceinline__ int walk_rows(const Walk &w, const CaseDesc &cd, cons
             const int j = bm + __builtin_ctzll(nm);
             nm &= nm - 1ull;
             const uint32_t tbj = src.tbit_adj(j + om);
-            visit(j, om, tbj, fetch(j, j + om, tbj, src));
+            visit(j, j + om, tbj, fetch(j, j + om, tbj, src));
         }
     }
     return cnt;
@@ -567,7 +653,7 @@ template <int BLOCK, int STAGE>
 __device__ __forceinline__ bool stage_rows(const Walk &w, const float4 *dyn, int i0, int n, float2 *txy,
                                            float2 *thm, uint8_t *tix, float4 *tdyn, float2 *tuv, int *info,
                                            TileRows<STAGE> &rows, int tag, bool &only_tag, bool stage_pr = false) {
-    constexpr int kCap = 3 * PairShape<BLOCK>::tile_row;
+    constexpr int kCap = 3 * PairShape<BLOCK>::tile_row - kTilePad;
     const int t = threadIdx.x;
     if (t < 6) {
         const int g = t / 3, r = t % 3;  // group (cell row) and neighbour row
@@ -603,25 +689,37 @@ __device__ __forceinline__ bool stage_rows(const Walk &w, const float4 *dyn, int
                     (u(6) | u(7) | u(8) | u(9) | u(10) | u(11)) >= 0;  // no range runs backwards
     int other = 0;  // this thread staged a slot of another tag
     if (ok) {
+        // One flat pass over the six ranges laid end to end: each thread loads
+        // all of its tile slots (at most kIter) before it writes any, so the
+        // block waits for one round trip to memory rather than one per range.
+        // Tile index q of range k holds slot q + e_k.
+        constexpr int kIter = (kCap + BLOCK - 1) / BLOCK;
+        const int e0 = u(0), e1 = u(1) - d1, e2 = u(2) - d2, e3 = u(3) - d3, e4 = u(4) - d4, e5 = u(5) - d5;
+        float4 g[kIter], dd[kIter];
+        float2 rp[kIter], uv[kIter];
+        int tg[kIter];
 #pragma unroll
-        for (int k = 0; k < 6; ++k) {
-            const int b = u(k), len = u(6 + k);
-            const int d = k == 0 ? 0 : k == 1 ? d1 : k == 2 ? d2 : k == 3 ? d3 : k == 4 ? d4 : d5;
-            for (int m = t; m < len; m += BLOCK) {
-                const float4 g = w.geo[b + m];
-                txy[d + m] = make_float2(g.x, g.y);
-                const int tg = w.tag[b + m];
-                if (stage_pr) {
-                    const float2 rp = reinterpret_cast<const float2 *>(dyn)[2 * (size_t)(b + m) + 1];  // (rho, p)
-                    thm[d + m] = make_float2(tg == tag ? pressure_ratio(rp.y, rp.x) : rp.y, rp.x);
-                } else {
-                    thm[d + m] = make_float2(g.z, g.w);
-                }
-                other |= tg != tag;
-                const uint32_t t1 = (uint32_t)tg - 1u;
-                tix[d + m] = (uint8_t)(t1 < (uint32_t)kMaxTag ? t1 : 63u);
-                if (STAGE == 1) tdyn[d + m] = dyn[b + m];
-                if (STAGE == 2) tuv[d + m] = reinterpret_cast<const float2 *>(dyn)[2 * (size_t)(b + m)];
+        for (int it = 0; it < kIter; ++it) {
+            const int q = min(t + it * BLOCK, total - 1);  // total > 0: the block's own slots are staged
+            const int sl = q + (q >= d5 ? e5 : q >= d4 ? e4 : q >= d3 ? e3 : q >= d2 ? e2 : q >= d1 ? e1 : e0);
+            g[it] = w.geo[sl];
+            tg[it] = w.tag[sl];
+            if (stage_pr) rp[it] = reinterpret_cast<const float2 *>(dyn)[2 * (size_t)sl + 1];  // (rho, p)
+            if (STAGE == 1) dd[it] = dyn[sl];
+            if (STAGE == 2) uv[it] = reinterpret_cast<const float2 *>(dyn)[2 * (size_t)sl];
+        }
+#pragma unroll
+        for (int it = 0; it < kIter; ++it) {
+            const int q = t + it * BLOCK;
+            if (q < total) {
+                txy[q] = make_float2(g[it].x, g[it].y);
+                thm[q] = stage_pr ? make_float2(tg[it] == tag ? pressure_ratio(rp[it].y, rp[it].x) : rp[it].y, rp[it].x)
+                                  : make_float2(g[it].z, g[it].w);
+                other |= tg[it] != tag;
+                const uint32_t t1 = (uint32_t)tg[it] - 1u;
+                tix[q] = (uint8_t)(t1 < (uint32_t)kMaxTag ? t1 : 63u);
+                if (STAGE == 1) tdyn[q] = dd[it];
+                if (STAGE == 2) tuv[q] = uv[it];
             }
         }
         const int i = min(i0 + t, n - 1);
@@ -809,23 +907,26 @@ __global__ __launch_bounds__(BLOCK, BATCH ? 1 : 8) void k_search(Walk w, Fields 
     if (i0 >= n) return;  // the whole block
     const int i = i0 + threadIdx.x;
     if (EOS && i >= n && i < eo.n_slots) eo.vol_new[i] = f.vol[i];
+    // the particle's own records, requested before the tile is staged (their loads overlap its loads)
+    const int ic = min(i, n - 1);
+    const float4 gi = w.geo[ic];
+    const int ti = f.tag[ic];
+    const float4 di = ACTION == EPPX_SEARCH_KERNEL_CONTINUITY ? f.dyn[ic] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float drho_i = w.drho_zero ? 0.0f : ld_slot(f.drho, ic);
     TileRows<STAGE> rows;
     bool all_fluid = false;  // every particle of the block's neighbourhood is fluid
     const bool tiled = ACTION == EPPX_SEARCH_KERNEL_CONTINUITY &&
                        stage_rows<BLOCK, STAGE>(w, f.dyn, i0, n, txy, thm, tix, nullptr, tuv, info, rows, ph.fluid_tag,
                                                 all_fluid);
     int cnt = 0;
-    const CaseDesc cd = BATCH ? w.cases[w.cs[min(i, n - 1)]] : w.one;
+    const CaseDesc cd = BATCH ? w.cases[w.cs[ic]] : w.one;
     // a batch runs the action for the cases that advance (which = 0) or wait for a first search (1)
     if (i < n && (which == 0 ? cd.steps_left > 0 : cd.fresh != 0)) {
-        const float4 gi = w.geo[i];
-        const int ti = f.tag[i];
         const bool i_fluid = ti == ph.fluid_tag, i_wall = ti == ph.wall_tag;
         if (ACTION == EPPX_SEARCH_KERNEL_CONTINUITY) {
-            const float4 di = f.dyn[i];
             const float2 ui = make_float2(di.x, di.y);
             const float rho_i = di.z;
-            float acc = w.drho_zero ? 0.0f : ld_slot(f.drho, i);
+            float acc = drho_i;
             // pair classes of the continuity action: fluid-fluid (classic), fluid-wall / wall-fluid (balanced)
             const uint32_t keep = i_fluid ? tag_bit(ph.fluid_tag) | tag_bit(ph.wall_tag)
                                           : (i_wall ? tag_bit(ph.fluid_tag) : 0u);
@@ -1121,6 +1222,12 @@ __global__ __launch_bounds__(BLOCK, (BATCH && !FAST) ? 1 : 8) void k_momentum(Wa
     const int i0 = xcd_block(blockIdx.x, gridDim.x) * BLOCK;
     if (!MOVE && i0 >= n) return;  // the whole block
     if (MOVE && blockIdx.x == 0 && threadIdx.x == 0) *mk.pairs_last = 0ull;
+    const int i = i0 + threadIdx.x;
+    // the particle's own records, requested before the tile is staged (their loads overlap its loads)
+    const int ic = min(i, max(n - 1, 0));
+    const int tag_i = f.tag[ic];
+    const float4 gi_own = w.geo[ic], di_own = f.dyn[ic];
+    const float2 dvel_own = (MOVE && mk.dvel_zero) ? make_float2(0.0f, 0.0f) : f.dvel[ic];
     TileRows<STAGE> rows;
     bool all_fluid = false;  // every particle of the block's neighbourhood is fluid
     bool tiled = false;
@@ -1134,20 +1241,19 @@ __global__ __launch_bounds__(BLOCK, (BATCH && !FAST) ? 1 : 8) void k_momentum(Wa
         tiled = stage_rows<BLOCK, STAGE>(w, f.dyn, i0, n, txy, thm, tix, tdyn, nullptr, info, rows, ph.fluid_tag,
                                          all_fluid, stage_pr);
     }
-    const int i = i0 + threadIdx.x;
     if (!MOVE && i >= n) return;
-    const CaseDesc cd = BATCH ? w.cases[w.cs[min(i, n - 1)]] : w.one;
+    const CaseDesc cd = BATCH ? w.cases[w.cs[ic]] : w.one;
     // iMomentum! acts on fluid particles only (of cases that advance)
-    const bool act = i < n && f.tag[i] == ph.fluid_tag && cd.steps_left > 0;
+    const bool act = i < n && tag_i == ph.fluid_tag && cd.steps_left > 0;
     if (!MOVE && !act) return;
     float4 di = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     float2 acc = make_float2(0.0f, 0.0f);
 
     if (act) {
-    const float4 gi = w.geo[i];
-    di = f.dyn[i];
+    const float4 gi = gi_own;
+    di = di_own;
     const float2 ui = make_float2(di.x, di.y);
-    acc = (MOVE && mk.dvel_zero) ? make_float2(0.0f, 0.0f) : f.dvel[i];
+    acc = dvel_own;
     // p_i / rho_i^2 of the classic pressure, once (EXACT: the reference's expression, the same bits)
     const float pr_i = FAST ? pressure_ratio(di.w, di.z) : di.w / (di.z * di.z);
     const uint32_t fluid_bit = tag_bit(ph.fluid_tag);
