@@ -56,7 +56,7 @@ struct eppx_ps {
     int32_t n_cells = -1;
     int32_t *counts = nullptr, *start = nullptr, *tile_sums = nullptr;
     int32_t *key = nullptr, *tmpslot = nullptr, *members = nullptr, *ncount = nullptr, *n_alive = nullptr;
-    unsigned long long *pairs = nullptr;  // [0] last search, [1] running total
+    unsigned long long *pairs = nullptr;  // pair counters: kPairWords words (kPairLast, kPairTotal slots)
     uint32_t *fallback = nullptr;         // pair blocks that walked global memory
     float *h_range = nullptr;
     float rmin2 = -1.0f, rmax2 = INFINITY;  // symmetry-criterion bounds from the H range
@@ -335,10 +335,11 @@ int32_t do_search(eppx_ps *ps, eppx_ns *ns, int32_t criterion, int32_t action, c
     const int fast = ctx->math_mode == EPPX_MATH_FAST;
     HIP_OK(ctx, timed(ctx, "search", [&] {
                if (eos)
-                   return launch_search_eos(w, ps->f(), ps->n, ph, *eos, fast, ns->M, ps->ncount, &ps->pairs[0],
-                                            &ps->pairs[1], ctx->d_flags, ps->search_block, s);
-               return launch_search(w, ps->f(), ps->n, action, ph, fast, ns->M, ps->ncount, &ps->pairs[0],
-                                    &ps->pairs[1], ctx->d_flags, 0, ps->search_block,
+                   return launch_search_eos(w, ps->f(), ps->n, ph, *eos, fast, ns->M, ps->ncount,
+                                            ps->pairs + kPairLast, ps->pairs + kPairTotal, ctx->d_flags,
+                                            ps->search_block, s);
+               return launch_search(w, ps->f(), ps->n, action, ph, fast, ns->M, ps->ncount,
+                                    ps->pairs + kPairLast, ps->pairs + kPairTotal, ctx->d_flags, 0, ps->search_block,
                                     ps->search_stage, s);
            }));
     ps->last_ns = ns;
@@ -473,12 +474,13 @@ int32_t tune_pair_blocks(eppx_ps *ps, const Walk &w, const Physics &ph, int32_t 
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hipError_t e = dalloc(&drho, n);
     if (e == hipSuccess) e = dalloc(&dvel, n);
-    if (e == hipSuccess) e = dalloc(&pairs, 2);
+    if (e == hipSuccess) e = dalloc(&pairs, kPairWords);
     if (e == hipSuccess) e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
     if (e == hipSuccess) e = hipMemcpyAsync(drho, f.drho, n * sizeof(float), hipMemcpyDeviceToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(dvel, f.dvel, n * sizeof(float2), hipMemcpyDeviceToDevice, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(pairs, ps->pairs, 2 * sizeof(unsigned long long), hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(pairs, ps->pairs, kPairWords * sizeof(unsigned long long), hipMemcpyDeviceToDevice, s);
     const eppx_field_op op{};
     Walk wm = w;  // the momentum pass reads the search's masks as it will in a step
     wm.mask_mode = masks_valid ? 2 : 0;
@@ -491,8 +493,8 @@ int32_t tune_pair_blocks(eppx_ps *ps, const Walk &w, const Physics &ph, int32_t 
                 if (e == hipSuccess) e = hipEventRecord(e0, s);
                 if (e == hipSuccess)
                     e = kind == 0 ? launch_search(w, f, ps->n, EPPX_SEARCH_KERNEL_CONTINUITY, ph, 1, M, ps->ncount,
-                                                  &ps->pairs[0], &ps->pairs[1], ctx->d_flags, which, kShapeBlock[v],
-                                                  kShapeStage[v], s)
+                                                  ps->pairs + kPairLast, ps->pairs + kPairTotal, ctx->d_flags, which,
+                                                  kShapeBlock[v], kShapeStage[v], s)
                                   : launch_interaction(wm, f, ps->n, EPPX_INTER_MOMENTUM, ph, op, 1, kShapeBlock[v],
                                                        kShapeStage[v], s);
                 if (e == hipSuccess) e = hipEventRecord(e1, s);
@@ -506,7 +508,8 @@ int32_t tune_pair_blocks(eppx_ps *ps, const Walk &w, const Physics &ph, int32_t 
     }
     if (e == hipSuccess) e = hipMemcpyAsync(f.drho, drho, n * sizeof(float), hipMemcpyDeviceToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(f.dvel, dvel, n * sizeof(float2), hipMemcpyDeviceToDevice, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(ps->pairs, pairs, 2 * sizeof(unsigned long long), hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(ps->pairs, pairs, kPairWords * sizeof(unsigned long long), hipMemcpyDeviceToDevice, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
@@ -691,9 +694,9 @@ int32_t eppx_ps_create(eppx_ctx *ctx, int32_t n_capacity, const eppx_layout *lay
     if (e == hipSuccess) e = dalloc(&ps->ncount, (size_t)n_capacity);
     if (e == hipSuccess) e = dalloc(&ps->n_alive, 2);
     if (e == hipSuccess) e = dalloc(&ps->masks, 3 * (size_t)n_capacity);
-    if (e == hipSuccess) e = dalloc(&ps->pairs, 2);
+    if (e == hipSuccess) e = dalloc(&ps->pairs, kPairWords);
     if (e == hipSuccess) e = dalloc(&ps->h_range, 8);
-    if (e == hipSuccess) e = hipMemsetAsync(ps->pairs, 0, 2 * sizeof(unsigned long long), ctx->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(ps->pairs, 0, kPairWords * sizeof(unsigned long long), ctx->stream);
     if (e == hipSuccess) e = hipMemsetAsync(ps->n_alive, 0, 2 * sizeof(int32_t), ctx->stream);
     // the filter and extrapolation sums start at zero in both buffers (the
     // permute does not move them while they stay zero)
@@ -1200,11 +1203,12 @@ int32_t eppx_step(eppx_ps *ps, eppx_ns *ns, const eppx_physics *phys, int32_t n_
         int32_t r;
         if (eos_first)
             HIP_OK(ctx, timed(ctx, "continuity_eos", [&] {
-                       return launch_eos_reset(ps->f(), ps->n, ph, nullptr, ps->counts, nc + 2, &ps->pairs[0], s);
+                       return launch_eos_reset(ps->f(), ps->n, ph, nullptr, ps->counts, nc + 2, ps->pairs + kPairLast,
+                                               s);
                    }));
         if (fuse_move) {
             const MoveKey mk{ps->n, nc, ns->M, ps->counts, ps->key, ps->tmpslot, ctx->d_flags, ps->dyn_new,
-                             &ps->pairs[0], dvel_zero ? 1 : 0};
+                             ps->pairs + kPairLast, dvel_zero ? 1 : 0};
             if ((r = do_momentum_move(ps, ns, ph, mk))) return r;
         } else {
             if ((r = do_interaction(ps, ns, EPPX_INTER_MOMENTUM, ph, op))) return r;
@@ -1309,8 +1313,7 @@ int32_t eppx_pair_count(eppx_ps *ps, eppx_ns *ns, int64_t *out) {
     if (!ps->searched || ps->last_ns != ns) return set_error(ctx, EPPX_E_STATE, "eppx_pair_count: no search yet");
     HIP_OK(ctx, hipSetDevice(ctx->device));
     unsigned long long v = 0;
-    HIP_OK(ctx, hipMemcpyAsync(&v, &ps->pairs[0], sizeof(v), hipMemcpyDeviceToHost, ctx->stream));
-    HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
+    HIP_OK(ctx, read_pair_slots(ctx, ps->pairs, kPairLast, &v));
     *out = (int64_t)v;
     return read_flags(ctx);
 }
@@ -1320,9 +1323,12 @@ int32_t eppx_pair_total(eppx_ps *ps, int64_t *out, int32_t reset) {
     eppx_ctx *ctx = ps->ctx;
     HIP_OK(ctx, hipSetDevice(ctx->device));
     unsigned long long v = 0;
-    HIP_OK(ctx, hipMemcpyAsync(&v, &ps->pairs[1], sizeof(v), hipMemcpyDeviceToHost, ctx->stream));
-    if (reset) HIP_OK(ctx, hipMemsetAsync(&ps->pairs[1], 0, sizeof(unsigned long long), ctx->stream));
-    HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
+    HIP_OK(ctx, read_pair_slots(ctx, ps->pairs, kPairTotal, &v));
+    if (reset) {
+        HIP_OK(ctx, hipMemsetAsync(ps->pairs + kPairTotal, 0, kPairSlots * kPairStride * sizeof(unsigned long long),
+                                   ctx->stream));
+        HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
+    }
     *out = (int64_t)v;
     return EPPX_OK;
 }
@@ -1582,8 +1588,8 @@ int32_t batch_cell_build(eppx_batch *b, bool search, int32_t which, bool keys_do
     const int fast = ctx->math_mode == EPPX_MATH_FAST;
     HIP_OK(ctx, timed(ctx, "search", [&] {
                return launch_search(w, ps->f(), ps->n, EPPX_SEARCH_KERNEL_CONTINUITY, b->ph, fast, b->ns.M,
-                                    ps->ncount, &ps->pairs[0], &ps->pairs[1], ctx->d_flags, which, ps->search_block,
-                                    ps->search_stage, s);
+                                    ps->ncount, ps->pairs + kPairLast, ps->pairs + kPairTotal, ctx->d_flags, which,
+                                    ps->search_block, ps->search_stage, s);
            }));
     ps->masks_valid = which == 0;  // a first search (1) leaves the other cases' moved slots without masks
     return EPPX_OK;
@@ -1597,8 +1603,8 @@ int32_t batch_enqueue_step(eppx_batch *b) {
     const eppx_field_op op{};
     const int fast = ctx->math_mode == EPPX_MATH_FAST;
     HIP_OK(ctx, timed(ctx, "continuity_eos", [&] {
-               return launch_eos_reset(ps->f(), ps->n, b->ph, b->d_cases, ps->counts, b->total_cells + 2, &ps->pairs[0],
-                                       s);
+               return launch_eos_reset(ps->f(), ps->n, b->ph, b->d_cases, ps->counts, b->total_cells + 2,
+                                       ps->pairs + kPairLast, s);
            }));
     Walk w = batch_walk(b);
     w.mask_mode = ps->masks_valid ? 2 : 0;

@@ -41,7 +41,7 @@ struct eppx_ps3d {
     int32_t *counts = nullptr, *start = nullptr, *tile_sums = nullptr;
     uint32_t *cell_tags = nullptr;
     int32_t *key = nullptr, *tmpslot = nullptr, *members = nullptr, *ncount = nullptr, *n_alive = nullptr;
-    unsigned long long *pairs = nullptr;  // [0] last search, [1] running total
+    unsigned long long *pairs = nullptr;  // pair counters: kPairWords words (kPairLast, kPairTotal slots)
     float rmin2 = -1.0f, rmax2 = INFINITY;
     float hinv_u = 0.0f;
     float m_u = 0.0f;  // one Mass for every alive particle: that mass; else 0 (the 3D actions never change Mass)
@@ -196,7 +196,7 @@ int32_t do_search3(eppx_ps3d *ps, eppx_ns3d *ns, int32_t criterion, int32_t acti
     const int32_t nb = nc + 2;
     Fields3 cur = ps->buf[ps->cur], nxt = ps->buf[ps->cur ^ 1];
     HIP_OK(ctx, timed(ctx, "cell_key", [&] {
-               hipError_t e = launch_reset_search(ps->counts, nb, &ps->pairs[0], s);
+               hipError_t e = launch_reset_search(ps->counts, nb, ps->pairs + kPairLast, s);
                if (e == hipSuccess)
                    e = launch3_key_count(cur, ps->n, ns->grid, ps->counts, ps->key, ps->tmpslot, ctx->d_flags,
                                          ns->M, s);
@@ -405,8 +405,8 @@ int32_t eppx_ps3d_create(eppx_ctx *ctx, int32_t n_capacity, const eppx_layout *l
     if (e == hipSuccess) e = dalloc(&ps->members, (size_t)n_capacity);
     if (e == hipSuccess) e = dalloc(&ps->ncount, (size_t)n_capacity);
     if (e == hipSuccess) e = dalloc(&ps->n_alive, 2);
-    if (e == hipSuccess) e = dalloc(&ps->pairs, 2);
-    if (e == hipSuccess) e = hipMemsetAsync(ps->pairs, 0, 2 * sizeof(unsigned long long), ctx->stream);
+    if (e == hipSuccess) e = dalloc(&ps->pairs, kPairWords);
+    if (e == hipSuccess) e = hipMemsetAsync(ps->pairs, 0, kPairWords * sizeof(unsigned long long), ctx->stream);
     if (e == hipSuccess) e = hipMemsetAsync(ps->n_alive, 0, 2 * sizeof(int32_t), ctx->stream);
     if (e != hipSuccess) {
         eppx_ps3d_destroy(ps);
@@ -682,8 +682,7 @@ int32_t eppx_pair_count3d(eppx_ps3d *ps, eppx_ns3d *ns, int64_t *out) {
     if (!ps->searched || ps->last_ns != ns) return set_error(ctx, EPPX_E_STATE, "eppx_pair_count3d: no search yet");
     HIP_OK(ctx, hipSetDevice(ctx->device));
     unsigned long long v = 0;
-    HIP_OK(ctx, hipMemcpyAsync(&v, &ps->pairs[0], sizeof(v), hipMemcpyDeviceToHost, ctx->stream));
-    HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
+    HIP_OK(ctx, read_pair_slots(ctx, ps->pairs, kPairLast, &v));
     *out = (int64_t)v;
     return read_flags(ctx);
 }
@@ -693,9 +692,12 @@ int32_t eppx_pair_total3d(eppx_ps3d *ps, int64_t *out, int32_t reset) {
     eppx_ctx *ctx = ps->ctx;
     HIP_OK(ctx, hipSetDevice(ctx->device));
     unsigned long long v = 0;
-    HIP_OK(ctx, hipMemcpyAsync(&v, &ps->pairs[1], sizeof(v), hipMemcpyDeviceToHost, ctx->stream));
-    if (reset) HIP_OK(ctx, hipMemsetAsync(&ps->pairs[1], 0, sizeof(unsigned long long), ctx->stream));
-    HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
+    HIP_OK(ctx, read_pair_slots(ctx, ps->pairs, kPairTotal, &v));
+    if (reset) {
+        HIP_OK(ctx, hipMemsetAsync(ps->pairs + kPairTotal, 0, kPairSlots * kPairStride * sizeof(unsigned long long),
+                                   ctx->stream));
+        HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
+    }
     *out = (int64_t)v;
     return EPPX_OK;
 }

@@ -89,7 +89,7 @@ hipError_t launch3_permute(Fields3 cur, Fields3 nxt, float4 *geo, int32_t n, int
 hipError_t launch3_cell_tags(const float4 *geo, int32_t n_cells, const int32_t *start, uint32_t *cell_tags,
                              hipStream_t s);
 // Neighbour search pass (counts per slot into ncount, pair totals into
-// pairs[0] (this search) and pairs[1] (running)) with the fused action.
+// the slots at kPairLast (this search) and kPairTotal (running)) with the fused action.
 hipError_t launch3_search(const Walk3 &w, Fields3 f, int32_t action, const Physics &ph, int32_t fast, int32_t M,
                           int32_t *ncount, unsigned long long *pairs, uint32_t *flags, int32_t n_slots,
                           hipStream_t s);

@@ -81,6 +81,19 @@ inline void dfree(T *&p) {
     p = nullptr;
 }
 
+// Sum of the pair-counter slots of region kPairLast or kPairTotal of the
+// counter buffer d (waits for the stream).
+inline hipError_t read_pair_slots(eppx_ctx *ctx, const unsigned long long *d, int region, unsigned long long *out) {
+    std::vector<unsigned long long> h((size_t)kPairSlots * kPairStride);
+    hipError_t e = hipMemcpyAsync(h.data(), d + region, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                  ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    unsigned long long sum = 0;
+    for (int k = 0; k < kPairSlots; ++k) sum += h[(size_t)k * kPairStride];
+    *out = sum;
+    return e;
+}
+
 // Event-bracketed launch when the context times kernels.
 template <class F>
 inline hipError_t timed(eppx_ctx *ctx, const char *name, F &&launch) {

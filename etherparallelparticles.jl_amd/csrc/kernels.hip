@@ -14,10 +14,11 @@ __global__ void k_fill_i32(int32_t *p, int32_t v, int64_t n) {
     if (i < n) p[i] = v;
 }
 
+// (a launch has at least kBlock >= kPairSlots threads)
 __global__ void k_reset_search(int32_t *counts, int32_t n, unsigned long long *pairs_last) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i < n) counts[i] = 0;
-    if (i == 0) *pairs_last = 0ull;
+    if (i < kPairSlots) pairs_last[i * kPairStride] = 0ull;
 }
 
 
@@ -365,7 +366,7 @@ __global__ __launch_bounds__(kBlock) void k_eos_reset(Fields f, int32_t n, Physi
                                                        unsigned long long *pairs_last) {
     const int s = blockIdx.x * kBlock + threadIdx.x;
     if (s < n_counts) counts[s] = 0;
-    if (s == 0) *pairs_last = 0ull;
+    if (s < kPairSlots) pairs_last[s * kPairStride] = 0ull;
     if (s >= n) return;
     // every load of the slot up front, so that they are in flight together
     const int32_t alive = f.alive[s], tag = f.tag[s];

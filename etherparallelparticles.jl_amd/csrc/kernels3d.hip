@@ -763,8 +763,9 @@ __global__ __launch_bounds__(kCellBlock * kCellWaves, 6) void k3_search(Walk3 w,
     }
     for (int off = 32; off > 0; off >>= 1) total += __shfl_down(total, off);
     if (lane_id() == 0 && total) {
-        atomicAdd(&pairs[0], (unsigned long long)total);
-        atomicAdd(&pairs[1], (unsigned long long)total);
+        const int slot = ((blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) % kPairSlots) * kPairStride;
+        atomicAdd(pairs + kPairLast + slot, (unsigned long long)total);
+        atomicAdd(pairs + kPairTotal + slot, (unsigned long long)total);
     }
 }
 

@@ -1051,8 +1051,9 @@ __global__ __launch_bounds__(BLOCK, BATCH ? 1 : 8) void k_search(Walk w, Fields 
     }
     const int total = block_sum_i32(cnt, lds);
     if (threadIdx.x == 0 && total) {
-        atomicAdd(pairs_last, (unsigned long long)total);
-        atomicAdd(pairs_total, (unsigned long long)total);
+        const int slot = (blockIdx.x % kPairSlots) * kPairStride;
+        atomicAdd(pairs_last + slot, (unsigned long long)total);
+        atomicAdd(pairs_total + slot, (unsigned long long)total);
     }
 }
 
@@ -1221,7 +1222,7 @@ __global__ __launch_bounds__(BLOCK, (BATCH && !FAST) ? 1 : 8) void k_momentum(Wa
     const int n = *w.n_alive;
     const int i0 = xcd_block(blockIdx.x, gridDim.x) * BLOCK;
     if (!MOVE && i0 >= n) return;  // the whole block
-    if (MOVE && blockIdx.x == 0 && threadIdx.x == 0) *mk.pairs_last = 0ull;
+    if (MOVE && blockIdx.x == 0 && threadIdx.x < kPairSlots) mk.pairs_last[threadIdx.x * kPairStride] = 0ull;
     const int i = i0 + threadIdx.x;
     // the particle's own records, requested before the tile is staged (their loads overlap its loads)
     const int ic = min(i, max(n - 1, 0));
