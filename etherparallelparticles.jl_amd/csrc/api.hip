@@ -237,6 +237,7 @@ Walk make_walk(const eppx_ps *ps, const eppx_ns *ns, int32_t criterion) {
     w.cs = f.cs;
     w.fallback = ps->fallback;
     w.n_alive = ps->n_alive;
+    w.cap = ps->cap;
     memcpy(w.adj, ns->adj, sizeof(w.adj));
     w.criterion = criterion;
     w.masks = ps->masks;
@@ -1545,6 +1546,7 @@ Walk batch_walk(const eppx_batch *b) {
     w.cs = f.cs;
     w.fallback = ps->fallback;
     w.n_alive = ps->n_alive;
+    w.cap = ps->cap;
     memcpy(w.adj, b->ns.adj, sizeof(w.adj));
     w.criterion = EPPX_CRIT_SYMMETRY;
     w.masks = ps->masks;

@@ -82,6 +82,7 @@ struct Walk {
     const int32_t *cell;      // cell within the particle's case (1-based), sorted order
     const int32_t *cs;        // case of each slot, sorted order
     const int32_t *n_alive;   // device scalars: [0] alive slots, [1] kept slots
+    int32_t cap;              // slots allocated in every per-slot array
     uint32_t adj[kMaxTag + 1];  // adj[t]: bit (u-1) set when tag t is adjacent to tag u
     int32_t criterion;
     uint32_t *fallback;       // counters: [0] blocks that read global memory instead of LDS,

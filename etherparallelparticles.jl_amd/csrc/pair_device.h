@@ -485,24 +485,39 @@ __device__ __forceinline__ unsigned long long accept_row_tiled(const Src &src, i
     return n > 0 ? m & (~0ull >> (64 - n)) : 0ull;
 }
 
-template <bool ORDERED, bool COUNT, class Src, class Fetch, class Apply>
-__device__ __forceinline__ int walk_rows(const Walk &w, const CaseDesc &cd, const Src &src, int i, float4 gi,
-                                         uint32_t keep_tags, Fetch &&fetch, Apply &&apply) {
-    const int c = w.cell[i] - 1;
-    const int nx = cd.g.n_x, base = cd.cell_base;
+// A particle's cell (cx, cy) within its case and, per neighbour row r (cell
+// row cy - 1 + r), the slot range [b[r], e[r]) of cells cx - 1 .. cx + 1
+// (clamped to the grid; empty outside it); tbit: its tag bit.
+struct PairRows {
     int cx, cy;
-    cell_xy(c, nx, cx, cy);
-    const int x0 = max(cx - 1, 0), x1 = min(cx + 1, nx - 1);
     int b[3], e[3];
+    uint32_t tbit;
+};
+// The same from the cell table in global memory (c: 0-based cell).
+__device__ __forceinline__ PairRows pair_rows_global(const Walk &w, const CaseDesc &cd, int c, int tag) {
+    PairRows pr;
+    const int nx = cd.g.n_x, base = cd.cell_base;
+    cell_xy(c, nx, pr.cx, pr.cy);
+    const int x0 = max(pr.cx - 1, 0), x1 = min(pr.cx + 1, nx - 1);
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
-        const int jj = cy + r - 1;
-        b[r] = e[r] = 0;
+        const int jj = pr.cy + r - 1;
+        pr.b[r] = pr.e[r] = 0;
         if (jj >= 0 && jj < cd.g.n_y) {
-            b[r] = w.start[base + x0 + nx * jj];
-            e[r] = w.start[base + x1 + 1 + nx * jj];
+            pr.b[r] = w.start[base + x0 + nx * jj];
+            pr.e[r] = w.start[base + x1 + 1 + nx * jj];
         }
     }
+    pr.tbit = tag_bit(tag);
+    return pr;
+}
+
+template <bool ORDERED, bool COUNT, class Src, class Fetch, class Apply>
+__device__ __forceinline__ int walk_rows(const Walk &w, const CaseDesc &cd, const Src &src, int i, float4 gi,
+                                         const PairRows &pr, uint32_t keep_tags, Fetch &&fetch, Apply &&apply) {
+    const int nx = cd.g.n_x, base = cd.cell_base;
+    const int cx = pr.cx, cy = pr.cy;
+    const int b[3] = {pr.b[0], pr.b[1], pr.b[2]}, e[3] = {pr.e[0], pr.e[1], pr.e[2]};
     if (e[0] - b[0] > 64 || e[1] - b[1] > 64 || e[2] - b[2] > 64) {
         // counted once per wave: the lanes that take this path
         const unsigned long long lanes = __ballot(1);
@@ -517,7 +532,7 @@ __device__ __forceinline__ int walk_rows(const Walk &w, const CaseDesc &cd, cons
         });
         return COUNT ? cnt : 0;
     }
-    const uint32_t adj = adj_of(w, tag_bit(w.tag[i]));
+    const uint32_t adj = adj_of(w, pr.tbit);
     const uint32_t kadj = adj & keep_tags;
     const float2 xi = make_float2(gi.x, gi.y);
     const unsigned long long self = 1ull << (i - b[1]);
@@ -648,37 +663,63 @@ __device__ __forceinline__ int walk_rows(const Walk &w, const CaseDesc &cd, cons
 // one Mass): the second tile array holds (x_j, rho_j) instead of (h_j, m_j),
 // with x_j = pressure_ratio(p_j, rho_j) for slots of tag `tag` (fluid) and p_j
 // for the others -- the pair terms then read them instead of h and m.
-constexpr int kStageInfo = 16;
+//
+// The block waits on three round trips to memory: the caller's loads of the
+// alive count, of the cells of slots i0 and iB = max(min(i0 + BLOCK, cap) - 1, i0)
+// (cell_a, cell_b, 1-based; the cell of the last alive slot is loaded again
+// in the one block that holds it) and of the thread's own records; the range
+// bounds; then the tile together with the cell starts of the groups' rows.
+// Those starts (tstart, relative to their range's first slot) give each
+// thread its three row ranges (pr) without another load; a group wider than
+// the table (a sparse block) takes them from global memory instead.
+// cell_i, tag_i: the thread's own cell (1-based) and tag (slot min(i, n - 1)).
+constexpr int kStageInfo = 24;
+template <int BLOCK>
+struct StartTable {  // 16-bit cell starts per group row: what the LDS left by the tile holds
+    static constexpr int row = BLOCK >= 512 ? 112 : 40;
+};
 template <int BLOCK, int STAGE>
-__device__ __forceinline__ bool stage_rows(const Walk &w, const float4 *dyn, int i0, int n, float2 *txy,
-                                           float2 *thm, uint8_t *tix, float4 *tdyn, float2 *tuv, int *info,
-                                           TileRows<STAGE> &rows, int tag, bool &only_tag, bool stage_pr = false) {
+__device__ __forceinline__ bool stage_rows(const Walk &w, const float4 *dyn, int i0, int n, int cell_a, int cell_b,
+                                           int cell_i, int tag_i, float2 *txy, float2 *thm, uint8_t *tix,
+                                           float4 *tdyn, float2 *tuv, int *info, uint16_t *tstart,
+                                           TileRows<STAGE> &rows, PairRows &pr, int tag, bool &only_tag,
+                                           bool stage_pr = false) {
     constexpr int kCap = 3 * PairShape<BLOCK>::tile_row - kTilePad;
+    constexpr int kTab = StartTable<BLOCK>::row;
     const int t = threadIdx.x;
     if (t < 6) {
         const int g = t / 3, r = t % 3;  // group (cell row) and neighbour row
         const int last = min(i0 + BLOCK, n) - 1;
         const CaseDesc cd = w.cases ? w.cases[w.cs[i0]] : w.one;
         const int nx = cd.g.n_x;
-        const int c0 = w.cell[i0] - 1, c1 = w.cell[last] - 1;
+        const int c0 = cell_a - 1;
+        const int c1 = (last == max(min(i0 + BLOCK, w.cap) - 1, i0) ? cell_b : w.cell[last]) - 1;
         const int cy0 = c0 / nx, cy1 = c1 / nx;
         const int ok = (cy1 == cy0 || cy1 == cy0 + 1) && (!w.cases || w.cs[last] == w.cs[i0]);
-        int b = 0, len = 0;
+        int b = 0, len = 0, xlo = 0, cnt = 0;
         if (ok && (g == 0 || cy1 != cy0)) {
             const int xa = g == 0 ? c0 % nx : 0;                          // the group's first x cell
             const int xb = (g == 1 || cy1 == cy0) ? c1 % nx : nx - 1;     // and its last
+            xlo = max(xa - 1, 0);
+            cnt = min(xb + 1, nx - 1) + 2 - xlo;  // cell starts xlo .. min(xb + 1, nx - 1) + 1
             const int jj = cy0 + g + r - 1;
             if (jj >= 0 && jj < cd.g.n_y) {
-                b = w.start[cd.cell_base + max(xa - 1, 0) + nx * jj];
-                len = w.start[cd.cell_base + min(xb + 1, nx - 1) + 1 + nx * jj] - b;
+                b = w.start[cd.cell_base + xlo + nx * jj];
+                len = w.start[cd.cell_base + xlo + cnt - 1 + nx * jj] - b;
             }
         }
         info[t] = b;
         info[6 + t] = len;
+        if (r == 0) {
+            info[15 + g] = xlo;
+            info[17 + g] = cnt;
+        }
         if (t == 0) {
             info[12] = ok;
             info[13] = cy0;
             info[14] = nx;
+            info[19] = cd.g.n_y;
+            info[20] = cd.cell_base;
         }
     }
     __syncthreads();
@@ -687,17 +728,20 @@ __device__ __forceinline__ bool stage_rows(const Walk &w, const float4 *dyn, int
     const int d1 = u(6), d2 = d1 + u(7), d3 = d2 + u(8), d4 = d3 + u(9), d5 = d4 + u(10), total = d5 + u(11);
     const bool ok = u(12) && total <= kCap &&
                     (u(6) | u(7) | u(8) | u(9) | u(10) | u(11)) >= 0;  // no range runs backwards
+    const int nx = u(14), cy0 = u(13), ny = u(19), cbase = u(20);
+    const bool table = u(17) <= kTab && u(18) <= kTab;
     int other = 0;  // this thread staged a slot of another tag
     if (ok) {
         // One flat pass over the six ranges laid end to end: each thread loads
-        // all of its tile slots (at most kIter) before it writes any, so the
-        // block waits for one round trip to memory rather than one per range.
-        // Tile index q of range k holds slot q + e_k.
+        // all of its tile slots (at most kIter) and table entries before it
+        // writes any, so the block waits for one round trip to memory rather
+        // than one per range.  Tile index q of range k holds slot q + e_k.
         constexpr int kIter = (kCap + BLOCK - 1) / BLOCK;
+        constexpr int kTabIter = (6 * kTab + BLOCK - 1) / BLOCK;
         const int e0 = u(0), e1 = u(1) - d1, e2 = u(2) - d2, e3 = u(3) - d3, e4 = u(4) - d4, e5 = u(5) - d5;
         float4 g[kIter], dd[kIter];
         float2 rp[kIter], uv[kIter];
-        int tg[kIter];
+        int tg[kIter], ts[kTabIter];
 #pragma unroll
         for (int it = 0; it < kIter; ++it) {
             const int q = min(t + it * BLOCK, total - 1);  // total > 0: the block's own slots are staged
@@ -707,6 +751,14 @@ __device__ __forceinline__ bool stage_rows(const Walk &w, const float4 *dyn, int
             if (stage_pr) rp[it] = reinterpret_cast<const float2 *>(dyn)[2 * (size_t)sl + 1];  // (rho, p)
             if (STAGE == 1) dd[it] = dyn[sl];
             if (STAGE == 2) uv[it] = reinterpret_cast<const float2 *>(dyn)[2 * (size_t)sl];
+        }
+        // table entry x of group row k = (g, r): the start of cell xlo_g + x of cell row cy0 + g + r - 1
+#pragma unroll
+        for (int it = 0; it < kTabIter; ++it) {
+            const int q = t + it * BLOCK, k = q / kTab, x = q - k * kTab, gk = k >= 3, jj = cy0 + k - 1 - 2 * gk;
+            ts[it] = 0;
+            if (table && q < 6 * kTab && x < info[17 + gk] && jj >= 0 && jj < ny)  // (k varies by thread)
+                ts[it] = w.start[cbase + info[15 + gk] + x + nx * jj] - info[k];
         }
 #pragma unroll
         for (int it = 0; it < kIter; ++it) {
@@ -722,22 +774,48 @@ __device__ __forceinline__ bool stage_rows(const Walk &w, const float4 *dyn, int
                 if (STAGE == 2) tuv[q] = uv[it];
             }
         }
-        const int i = min(i0 + t, n - 1);
+#pragma unroll
+        for (int it = 0; it < kTabIter; ++it) {
+            const int q = t + it * BLOCK;
+            if (q < 6 * kTab) tstart[q] = (uint16_t)ts[it];
+        }
+    } else if (t == 0) {
+        atomicAdd(w.fallback, 1u);
+    }
+    only_tag = ok && !__syncthreads_or(other);
+    if (ok) {
         int cxi, cyi;
-        cell_xy(w.cell[i] - 1, u(14), cxi, cyi);
-        const bool second = cyi != u(13);  // this thread's slot is in the second row
+        cell_xy(cell_i - 1, nx, cxi, cyi);
+        const int gi = cyi != cy0;  // this thread's slot is in the second row
         rows.xy = txy;
         rows.hm = thm;
         rows.tix = tix;
         rows.dyn = tdyn;
         rows.uv = tuv;
-        rows.off0 = second ? d3 - u(3) : -u(0);
-        rows.off1 = second ? d4 - u(4) : d1 - u(1);
-        rows.off2 = second ? d5 - u(5) : d2 - u(2);
-    } else if (t == 0) {
-        atomicAdd(w.fallback, 1u);
+        rows.off0 = gi ? d3 - u(3) : -u(0);
+        rows.off1 = gi ? d4 - u(4) : d1 - u(1);
+        rows.off2 = gi ? d5 - u(5) : d2 - u(2);
+        if (table) {
+            pr.cx = cxi;
+            pr.cy = cyi;
+            const int xlo = gi ? u(16) : u(15);
+            const int xl = max(cxi - 1, 0) - xlo, xr = min(cxi + 1, nx - 1) + 1 - xlo;
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                const int k = 3 * gi + r, jj = cyi + r - 1;
+                const int rb = gi ? u(3 + r) : u(r);  // the range's first slot
+                pr.b[r] = pr.e[r] = 0;
+                if (jj >= 0 && jj < ny) {
+                    pr.b[r] = rb + tstart[k * kTab + xl];
+                    pr.e[r] = rb + tstart[k * kTab + xr];
+                }
+            }
+            pr.tbit = tag_bit(tag_i);
+        } else {
+            const CaseDesc cd = w.cases ? w.cases[w.cs[i0]] : w.one;
+            pr = pair_rows_global(w, cd, cell_i - 1, tag_i);
+        }
     }
-    only_tag = ok && !__syncthreads_or(other);
     return ok;
 }
 
@@ -894,8 +972,12 @@ __global__ __launch_bounds__(BLOCK, BATCH ? 1 : 8) void k_search(Walk w, Fields 
     __shared__ uint8_t tix[3 * kTileRow];
     __shared__ float2 tuv[STAGE == 2 ? 3 * kTileRow : 1];
     __shared__ int info[kStageInfo];
-    const int n = *w.n_alive;
+    __shared__ uint16_t tstart[6 * StartTable<BLOCK>::row];
     const int i0 = xcd_block(blockIdx.x, gridDim.x) * BLOCK;
+    // the cells of the block's first and last slots, loaded with the alive count (i0 < cap: the grid
+    // covers the slots)
+    const int cell_a = w.cell[i0], cell_b = w.cell[max(min(i0 + BLOCK, w.cap) - 1, i0)];
+    const int n = *w.n_alive;
     if constexpr (EOS) {
         for (int c = blockIdx.x * BLOCK + threadIdx.x; c < eo.n_counts; c += gridDim.x * BLOCK) eo.counts[c] = 0;
         if (i0 >= n) {  // slots of dead particles only: their volumes as they are
@@ -913,13 +995,16 @@ __global__ __launch_bounds__(BLOCK, BATCH ? 1 : 8) void k_search(Walk w, Fields 
     const int ti = f.tag[ic];
     const float4 di = ACTION == EPPX_SEARCH_KERNEL_CONTINUITY ? f.dyn[ic] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     const float drho_i = w.drho_zero ? 0.0f : ld_slot(f.drho, ic);
+    const int cell_i = w.cell[ic];
     TileRows<STAGE> rows;
+    PairRows pr;
     bool all_fluid = false;  // every particle of the block's neighbourhood is fluid
     const bool tiled = ACTION == EPPX_SEARCH_KERNEL_CONTINUITY &&
-                       stage_rows<BLOCK, STAGE>(w, f.dyn, i0, n, txy, thm, tix, nullptr, tuv, info, rows, ph.fluid_tag,
-                                                all_fluid);
+                       stage_rows<BLOCK, STAGE>(w, f.dyn, i0, n, cell_a, cell_b, cell_i, ti, txy, thm, tix, nullptr,
+                                                tuv, info, tstart, rows, pr, ph.fluid_tag, all_fluid);
     int cnt = 0;
     const CaseDesc cd = BATCH ? w.cases[w.cs[ic]] : w.one;
+    if (!tiled) pr = pair_rows_global(w, cd, cell_i - 1, ti);
     // a batch runs the action for the cases that advance (which = 0) or wait for a first search (1)
     if (i < n && (which == 0 ? cd.steps_left > 0 : cd.fresh != 0)) {
         const bool i_fluid = ti == ph.fluid_tag, i_wall = ti == ph.wall_tag;
@@ -968,7 +1053,7 @@ __global__ __launch_bounds__(BLOCK, BATCH ? 1 : 8) void k_search(Walk w, Fields 
                                           const ContinuityGather &p) {
                             s_ff += vdotx_shape_u<K>(ut, ui, p.uv, dx, dy, dr2);
                         };
-                        cnt = walk_rows<false, true>(w, cd, rows, i, gi, keep, fetch_f, term_f);
+                        cnt = walk_rows<false, true>(w, cd, rows, i, gi, pr, keep, fetch_f, term_f);
                     } else {
                         auto term_m = [&](int, uint32_t tbj, float, float, float dx, float dy, float dr2,
                                           const ContinuityGather &p) {
@@ -979,8 +1064,8 @@ __global__ __launch_bounds__(BLOCK, BATCH ? 1 : 8) void k_search(Walk w, Fields 
                             s_ff += ff ? vdotx * shape : 0.0f;
                             s_mx += ff ? 0.0f : ((rho_i * p.vol) * vdotx) * shape;
                         };
-                        if (tiled) cnt = walk_rows<false, true>(w, cd, rows, i, gi, keep, fetch, term_m);
-                        else cnt = walk_rows<false, true>(w, cd, GlobalRows{w.geo, w.tag}, i, gi, keep, fetch, term_m);
+                        if (tiled) cnt = walk_rows<false, true>(w, cd, rows, i, gi, pr, keep, fetch, term_m);
+                        else cnt = walk_rows<false, true>(w, cd, GlobalRows{w.geo, w.tag}, i, gi, pr, keep, fetch, term_m);
                     }
                     acc = fmaf(ut.kfac, s_mx, fmaf(ut.kfac * cd.m_u, s_ff, acc));
                     walked = true;
@@ -999,15 +1084,15 @@ __global__ __launch_bounds__(BLOCK, BATCH ? 1 : 8) void k_search(Walk w, Fields 
                                           const ContinuityGather &p) {
                             sum += continuity_fast_u<K>(ut, ui, p.uv, mj, dx, dy, dr2);
                         };
-                        cnt = walk_rows<false, true>(w, cd, rows, i, gi, keep, fetch_f, term_f);
+                        cnt = walk_rows<false, true>(w, cd, rows, i, gi, pr, keep, fetch_f, term_f);
                     } else {
                         auto term_u = [&](int, uint32_t tbj, float, float mj, float dx, float dy, float dr2,
                                           const ContinuityGather &p) {
                             const float m = (i_fluid && tbj == fluid_bit) ? mj : rho_i * p.vol;
                             sum += continuity_fast_u<K>(ut, ui, p.uv, m, dx, dy, dr2);
                         };
-                        if (tiled) cnt = walk_rows<false, true>(w, cd, rows, i, gi, keep, fetch, term_u);
-                        else cnt = walk_rows<false, true>(w, cd, GlobalRows{w.geo, w.tag}, i, gi, keep, fetch, term_u);
+                        if (tiled) cnt = walk_rows<false, true>(w, cd, rows, i, gi, pr, keep, fetch, term_u);
+                        else cnt = walk_rows<false, true>(w, cd, GlobalRows{w.geo, w.tag}, i, gi, pr, keep, fetch, term_u);
                     }
                     acc = fmaf(ut.kfac, sum, acc);
                     walked = true;
@@ -1021,16 +1106,16 @@ __global__ __launch_bounds__(BLOCK, BATCH ? 1 : 8) void k_search(Walk w, Fields 
                         const float m = (i_fluid && tbj == fluid_bit) ? mj : rho_i * p.vol;
                         acc += continuity_exact<K>(hinv_e, ui, p.uv, m, dx, dy, dr2);
                     };
-                    if (tiled) cnt = walk_rows<true, true>(w, cd, rows, i, gi, keep, fetch, term_e);
-                    else cnt = walk_rows<true, true>(w, cd, GlobalRows{w.geo, w.tag}, i, gi, keep, fetch, term_e);
+                    if (tiled) cnt = walk_rows<true, true>(w, cd, rows, i, gi, pr, keep, fetch, term_e);
+                    else cnt = walk_rows<true, true>(w, cd, GlobalRows{w.geo, w.tag}, i, gi, pr, keep, fetch, term_e);
                     walked = true;
                 }
             }
             if (walked) {
             } else if (tiled) {
-                cnt = walk_rows<!FAST, true>(w, cd, rows, i, gi, keep, fetch, term);
+                cnt = walk_rows<!FAST, true>(w, cd, rows, i, gi, pr, keep, fetch, term);
             } else {
-                cnt = walk_rows<!FAST, true>(w, cd, GlobalRows{w.geo, w.tag}, i, gi, keep, fetch, term);
+                cnt = walk_rows<!FAST, true>(w, cd, GlobalRows{w.geo, w.tag}, i, gi, pr, keep, fetch, term);
             }
             if (EOS && (i_fluid || i_wall)) {
                 float4 d = di;
@@ -1219,8 +1304,12 @@ __global__ __launch_bounds__(BLOCK, (BATCH && !FAST) ? 1 : 8) void k_momentum(Wa
     __shared__ uint8_t tix[3 * kTileRow];
     __shared__ float4 tdyn[STAGE == 1 ? 3 * kTileRow : 1];
     __shared__ int info[kStageInfo];
-    const int n = *w.n_alive;
+    __shared__ uint16_t tstart[6 * StartTable<BLOCK>::row];
     const int i0 = xcd_block(blockIdx.x, gridDim.x) * BLOCK;
+    // the cells of the block's first and last slots, loaded with the alive count (i0 < cap: the grid
+    // covers the slots)
+    const int cell_a = w.cell[i0], cell_b = w.cell[max(min(i0 + BLOCK, w.cap) - 1, i0)];
+    const int n = *w.n_alive;
     if (!MOVE && i0 >= n) return;  // the whole block
     if (MOVE && blockIdx.x == 0 && threadIdx.x < kPairSlots) mk.pairs_last[threadIdx.x * kPairStride] = 0ull;
     const int i = i0 + threadIdx.x;
@@ -1229,7 +1318,9 @@ __global__ __launch_bounds__(BLOCK, (BATCH && !FAST) ? 1 : 8) void k_momentum(Wa
     const int tag_i = f.tag[ic];
     const float4 gi_own = w.geo[ic], di_own = f.dyn[ic];
     const float2 dvel_own = (MOVE && mk.dvel_zero) ? make_float2(0.0f, 0.0f) : f.dvel[ic];
+    const int cell_i = w.cell[ic];
     TileRows<STAGE> rows;
+    PairRows pr;
     bool all_fluid = false;  // every particle of the block's neighbourhood is fluid
     bool tiled = false;
     bool stage_pr = false;   // the tile holds (x_j, rho_j) (the one-H, one-Mass FAST terms)
@@ -1239,8 +1330,8 @@ __global__ __launch_bounds__(BLOCK, (BATCH && !FAST) ? 1 : 8) void k_momentum(Wa
             const CaseDesc c0 = BATCH ? w.cases[w.cs[i0]] : w.one;
             stage_pr = c0.hinv_u > 0.0f && c0.m_u > 0.0f && ph.coefficient == 0.0f;
         }
-        tiled = stage_rows<BLOCK, STAGE>(w, f.dyn, i0, n, txy, thm, tix, tdyn, nullptr, info, rows, ph.fluid_tag,
-                                         all_fluid, stage_pr);
+        tiled = stage_rows<BLOCK, STAGE>(w, f.dyn, i0, n, cell_a, cell_b, cell_i, tag_i, txy, thm, tix, tdyn,
+                                         nullptr, info, tstart, rows, pr, ph.fluid_tag, all_fluid, stage_pr);
     }
     if (!MOVE && i >= n) return;
     const CaseDesc cd = BATCH ? w.cases[w.cs[ic]] : w.one;
@@ -1251,6 +1342,7 @@ __global__ __launch_bounds__(BLOCK, (BATCH && !FAST) ? 1 : 8) void k_momentum(Wa
     float2 acc = make_float2(0.0f, 0.0f);
 
     if (act) {
+    if (!tiled) pr = pair_rows_global(w, cd, cell_i - 1, tag_i);
     const float4 gi = gi_own;
     di = di_own;
     const float2 ui = make_float2(di.x, di.y);
@@ -1297,14 +1389,14 @@ __global__ __launch_bounds__(BLOCK, (BATCH && !FAST) ? 1 : 8) void k_momentum(Wa
                     momentum_fast_um<K, VISC>(ph, sum, true, ut, ui, di.z, di.w, pr_i, make_float2(dj.x, dj.y), dj.z,
                                               dj.w, dx, dy, dr2);
                 };
-                walk_rows<false, false>(w, cd, rows, i, gi, keep, fetch_um, term_f);
+                walk_rows<false, false>(w, cd, rows, i, gi, pr, keep, fetch_um, term_f);
             } else {
                 auto term_m = [&](int, uint32_t tbj, float, float, float dx, float dy, float dr2, const float4 &dj) {
                     momentum_fast_um<K, VISC>(ph, sum, tbj == fluid_bit, ut, ui, di.z, di.w, pr_i,
                                               make_float2(dj.x, dj.y), dj.z, dj.w, dx, dy, dr2);
                 };
-                if (tiled) walk_rows<false, false>(w, cd, rows, i, gi, keep, fetch_um, term_m);
-                else walk_rows<false, false>(w, cd, GlobalRows{w.geo, w.tag}, i, gi, keep, fetch_um, term_m);
+                if (tiled) walk_rows<false, false>(w, cd, rows, i, gi, pr, keep, fetch_um, term_m);
+                else walk_rows<false, false>(w, cd, GlobalRows{w.geo, w.tag}, i, gi, pr, keep, fetch_um, term_m);
             }
             const float km = -(cd.m_u * ut.kfac);
             acc.x = fmaf(km, sum.x, acc.x);
@@ -1318,15 +1410,15 @@ __global__ __launch_bounds__(BLOCK, (BATCH && !FAST) ? 1 : 8) void k_momentum(Wa
                     momentum_fast_u<K, VISC>(ph, sum, true, ut, ui, di.z, di.w, pr_i, make_float2(dj.x, dj.y), dj.z,
                                              dj.w, mj, dx, dy, dr2);
                 };
-                walk_rows<false, false>(w, cd, rows, i, gi, keep, fetch, term_f);
+                walk_rows<false, false>(w, cd, rows, i, gi, pr, keep, fetch, term_f);
             } else {
                 auto term_u = [&](int, uint32_t tbj, float, float mj, float dx, float dy, float dr2,
                                   const float4 &dj) {
                     momentum_fast_u<K, VISC>(ph, sum, tbj == fluid_bit, ut, ui, di.z, di.w, pr_i,
                                              make_float2(dj.x, dj.y), dj.z, dj.w, mj, dx, dy, dr2);
                 };
-                if (tiled) walk_rows<false, false>(w, cd, rows, i, gi, keep, fetch, term_u);
-                else walk_rows<false, false>(w, cd, GlobalRows{w.geo, w.tag}, i, gi, keep, fetch, term_u);
+                if (tiled) walk_rows<false, false>(w, cd, rows, i, gi, pr, keep, fetch, term_u);
+                else walk_rows<false, false>(w, cd, GlobalRows{w.geo, w.tag}, i, gi, pr, keep, fetch, term_u);
             }
             acc.x = fmaf(ut.kfac, sum.x, acc.x);
             acc.y = fmaf(ut.kfac, sum.y, acc.y);
@@ -1341,16 +1433,16 @@ __global__ __launch_bounds__(BLOCK, (BATCH && !FAST) ? 1 : 8) void k_momentum(Wa
                 momentum_exact<K, VISC>(ph, acc, tbj == fluid_bit, hinv_e, h_e, ui, di.z, di.w, pr_i,
                                         make_float2(dj.x, dj.y), dj.z, dj.w, mj, dx, dy, dr2);
             };
-            if (tiled) walk_rows<true, false>(w, cd, rows, i, gi, keep, fetch, term_e);
-            else walk_rows<true, false>(w, cd, GlobalRows{w.geo, w.tag}, i, gi, keep, fetch, term_e);
+            if (tiled) walk_rows<true, false>(w, cd, rows, i, gi, pr, keep, fetch, term_e);
+            else walk_rows<true, false>(w, cd, GlobalRows{w.geo, w.tag}, i, gi, pr, keep, fetch, term_e);
             walked = true;
         }
     }
     if (walked) {
     } else if (tiled) {
-        walk_rows<!FAST, false>(w, cd, rows, i, gi, keep, fetch, term);
+        walk_rows<!FAST, false>(w, cd, rows, i, gi, pr, keep, fetch, term);
     } else {
-        walk_rows<!FAST, false>(w, cd, GlobalRows{w.geo, w.tag}, i, gi, keep, fetch, term);
+        walk_rows<!FAST, false>(w, cd, GlobalRows{w.geo, w.tag}, i, gi, pr, keep, fetch, term);
     }
     if (!MOVE) f.dvel[i] = acc;
     }
