@@ -78,6 +78,10 @@ struct eppx_ps {
     int32_t search_block = 256, momentum_block = 256, search_stage = 0, momentum_stage = 0;
     unsigned long long *masks = nullptr;  // the last search's accept masks (3 per slot)
     bool masks_valid = false;             // they describe every alive slot's pairs
+    // the pair kernels' block headers (256- and 512-thread blocks) and which
+    // of them the searches on the current layout wrote (bit 0: 256, bit 1: 512)
+    int32_t *headers256 = nullptr, *headers512 = nullptr;
+    int32_t header_blocks = 0;
     // SumWeight / SumWeightedDensity / SumWeightedPressure may be non-zero in
     // some buffer (an upload brought non-zero values, or a filter or
     // extrapolation interaction ran); while false the permute skips them
@@ -241,6 +245,8 @@ Walk make_walk(const eppx_ps *ps, const eppx_ns *ns, int32_t criterion) {
     memcpy(w.adj, ns->adj, sizeof(w.adj));
     w.criterion = criterion;
     w.masks = ps->masks;
+    w.headers256 = ps->headers256;
+    w.headers512 = ps->headers512;
     w.mask_stride = ps->cap;
     w.mask_mode = 0;
     return w;
@@ -295,6 +301,9 @@ int32_t field_arg(eppx_ctx *ctx, const eppx_ps *ps, const void *params, size_t s
     return EPPX_OK;
 }
 
+// Bit of a pair-kernel block size in eppx_ps::header_blocks.
+inline int32_t block_bit(int32_t block) { return block >= 512 ? 2 : 1; }
+
 // clean! + insert + cell-order move + find + fused action.
 // keys_done: the cell counts, keys and provisional slots of this search were
 // already taken (k_move_key in a step).
@@ -333,6 +342,7 @@ int32_t do_search(eppx_ps *ps, eppx_ns *ns, int32_t criterion, int32_t action, c
     Walk w = make_walk(ps, ns, criterion);
     w.drho_zero = drho_zero && action == EPPX_SEARCH_KERNEL_CONTINUITY;
     w.mask_mode = action == EPPX_SEARCH_KERNEL_CONTINUITY ? 1 : 0;  // keep the masks for the interactions
+    w.header_mode = w.mask_mode;  // and the block headers
     const int fast = ctx->math_mode == EPPX_MATH_FAST;
     HIP_OK(ctx, timed(ctx, "search", [&] {
                if (eos)
@@ -351,6 +361,7 @@ int32_t do_search(eppx_ps *ps, eppx_ns *ns, int32_t criterion, int32_t action, c
     ps->last_wall = ph.wall_tag;
     ps->searched = true;
     ps->masks_valid = action == EPPX_SEARCH_KERNEL_CONTINUITY;
+    ps->header_blocks = ps->masks_valid ? block_bit(fast ? ps->search_block : 256) : 0;
     return EPPX_OK;
 }
 
@@ -419,6 +430,7 @@ int32_t do_interaction(eppx_ps *ps, eppx_ns *ns, int32_t action, Physics ph, con
         ph.wall_tag == ps->last_wall)
         w.mask_mode = 2;
     const int fast = ctx->math_mode == EPPX_MATH_FAST;
+    if (w.mask_mode == 2 && (ps->header_blocks & block_bit(fast ? ps->momentum_block : 256))) w.header_mode = 2;
     const char *name = action == EPPX_INTER_MOMENTUM                ? "momentum"
                        : action == EPPX_INTER_FILTER                ? "filter_inter"
                        : action == EPPX_INTER_EXTRAPOLATE_PRESSURE ? "extrapolate_inter"
@@ -438,6 +450,7 @@ int32_t do_momentum_move(eppx_ps *ps, eppx_ns *ns, Physics ph, const MoveKey &mk
     Walk w = make_walk(ps, ns, ps->last_crit);
     if (ps->masks_valid && ph.fluid_tag == ps->last_fluid && ph.wall_tag == ps->last_wall) w.mask_mode = 2;
     const int fast = ctx->math_mode == EPPX_MATH_FAST;
+    if (w.mask_mode == 2 && (ps->header_blocks & block_bit(fast ? ps->momentum_block : 256))) w.header_mode = 2;
     HIP_OK(ctx, timed(ctx, "momentum", [&] {
                return launch_momentum_move(w, ps->f(), ps->n, ph, mk, fast, ps->momentum_block, ctx->stream);
            }));
@@ -483,8 +496,11 @@ int32_t tune_pair_blocks(eppx_ps *ps, const Walk &w, const Physics &ph, int32_t 
     if (e == hipSuccess)
         e = hipMemcpyAsync(pairs, ps->pairs, kPairWords * sizeof(unsigned long long), hipMemcpyDeviceToDevice, s);
     const eppx_field_op op{};
-    Walk wm = w;  // the momentum pass reads the search's masks as it will in a step
+    Walk ws = w;  // the search trials write the block headers of every shape on this layout
+    ws.header_mode = 1;
+    Walk wm = w;  // the momentum pass reads the search's masks and headers as it will in a step
     wm.mask_mode = masks_valid ? 2 : 0;
+    wm.header_mode = masks_valid ? 2 : 0;
     for (int kind = 0; kind < 2 && e == hipSuccess; ++kind) {
         for (int v = 0; v < kShapes && e == hipSuccess; ++v) {
             double total = 0.0;
@@ -493,7 +509,7 @@ int32_t tune_pair_blocks(eppx_ps *ps, const Walk &w, const Physics &ph, int32_t 
                 else e = hipMemcpyAsync(f.dvel, dvel, n * sizeof(float2), hipMemcpyDeviceToDevice, s);
                 if (e == hipSuccess) e = hipEventRecord(e0, s);
                 if (e == hipSuccess)
-                    e = kind == 0 ? launch_search(w, f, ps->n, EPPX_SEARCH_KERNEL_CONTINUITY, ph, 1, M, ps->ncount,
+                    e = kind == 0 ? launch_search(ws, f, ps->n, EPPX_SEARCH_KERNEL_CONTINUITY, ph, 1, M, ps->ncount,
                                                   ps->pairs + kPairLast, ps->pairs + kPairTotal, ctx->d_flags, which,
                                                   kShapeBlock[v], kShapeStage[v], s)
                                   : launch_interaction(wm, f, ps->n, EPPX_INTER_MOMENTUM, ph, op, 1, kShapeBlock[v],
@@ -518,6 +534,7 @@ int32_t tune_pair_blocks(eppx_ps *ps, const Walk &w, const Physics &ph, int32_t 
     dfree(dvel);
     dfree(pairs);
     if (e != hipSuccess) return hip_fail(ctx, e, "autotune");
+    if (masks_valid) ps->header_blocks = block_bit(256) | block_bit(512);
     if (trials > 0) {
         int best_s = 0, best_m = 0;
         for (int v = 1; v < kShapes; ++v) {
@@ -695,6 +712,8 @@ int32_t eppx_ps_create(eppx_ctx *ctx, int32_t n_capacity, const eppx_layout *lay
     if (e == hipSuccess) e = dalloc(&ps->ncount, (size_t)n_capacity);
     if (e == hipSuccess) e = dalloc(&ps->n_alive, 2);
     if (e == hipSuccess) e = dalloc(&ps->masks, 3 * (size_t)n_capacity);
+    if (e == hipSuccess) e = dalloc(&ps->headers256, ((size_t)n_capacity / 256 + 1) * kHeaderInts256);
+    if (e == hipSuccess) e = dalloc(&ps->headers512, ((size_t)n_capacity / 512 + 1) * kHeaderInts512);
     if (e == hipSuccess) e = dalloc(&ps->pairs, kPairWords);
     if (e == hipSuccess) e = dalloc(&ps->h_range, 8);
     if (e == hipSuccess) e = hipMemsetAsync(ps->pairs, 0, kPairWords * sizeof(unsigned long long), ctx->stream);
@@ -728,6 +747,7 @@ int32_t eppx_ps_destroy(eppx_ps *ps) {
     dfree(ps->fallback);
     dfree(ps->counts); dfree(ps->start); dfree(ps->tile_sums);
     dfree(ps->key); dfree(ps->tmpslot); dfree(ps->members); dfree(ps->ncount); dfree(ps->n_alive); dfree(ps->masks);
+    dfree(ps->headers256); dfree(ps->headers512);
     dfree(ps->pairs); dfree(ps->h_range);
     dfree(ps->dyn_new); dfree(ps->vol_new);
     eppx_ctx *ctx = ps->ctx;
@@ -1550,6 +1570,8 @@ Walk batch_walk(const eppx_batch *b) {
     memcpy(w.adj, b->ns.adj, sizeof(w.adj));
     w.criterion = EPPX_CRIT_SYMMETRY;
     w.masks = ps->masks;
+    w.headers256 = ps->headers256;
+    w.headers512 = ps->headers512;
     w.mask_stride = ps->cap;
     w.mask_mode = 0;
     return w;

@@ -26,6 +26,13 @@ constexpr int kPairLast = 0;
 constexpr int kPairTotal = kPairSlots * kPairStride;
 constexpr int kPairWords = 2 * kPairSlots * kPairStride;
 
+// Pair-kernel block headers: kStageInfo ints of range bounds, then a table of
+// 16-bit cell starts, 6 rows of kStartRow256 / kStartRow512 entries (as ints).
+constexpr int kStageInfo = 24;
+constexpr int kStartRow256 = 40, kStartRow512 = 112;
+constexpr int kHeaderInts256 = kStageInfo + 3 * kStartRow256;
+constexpr int kHeaderInts512 = kStageInfo + 3 * kStartRow512;
+
 // Sticky device flags (OR-ed by kernels, read at the next waiting call).
 enum : uint32_t {
     kFlagCellOverflow = 1u,       // a cell holds more than max_neighbour_number particles
@@ -92,6 +99,13 @@ struct Walk {
     // instead of testing the candidates again (same pair set), 0 neither.
     unsigned long long *masks;
     int32_t mask_stride, mask_mode;
+    // Block headers of the pair kernels (a block's range bounds and the cell
+    // starts of its rows, as its tile stager found them), one buffer per
+    // block size (256 and 512 threads): header_mode 1 writes them (a search),
+    // 2 reads them instead of finding them again (a momentum pass on the same
+    // sorted layout, with the search's block size), 0 neither.
+    int32_t *headers256, *headers512;
+    int32_t header_mode;
     // 1: dDensity is zero on every alive slot (a step's searches after its
     // reset): the continuity sum starts at zero and the fused EOS search
     // leaves the zero in place

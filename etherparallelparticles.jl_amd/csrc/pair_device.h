@@ -673,21 +673,37 @@ __device__ __forceinline__ int walk_rows(const Walk &w, const CaseDesc &cd, cons
 // thread its three row ranges (pr) without another load; a group wider than
 // the table (a sparse block) takes them from global memory instead.
 // cell_i, tag_i: the thread's own cell (1-based) and tag (slot min(i, n - 1)).
-constexpr int kStageInfo = 24;
+// hdr_out: this block's header, written (a search); hdr_in: read instead of
+// finding the ranges and the start table (a pass on the layout of the search
+// that wrote it, with the same block size): one round trip fewer.
 template <int BLOCK>
 struct StartTable {  // 16-bit cell starts per group row: what the LDS left by the tile holds
-    static constexpr int row = BLOCK >= 512 ? 112 : 40;
+    static constexpr int row = BLOCK >= 512 ? kStartRow512 : kStartRow256;
+    static constexpr int header_ints = BLOCK >= 512 ? kHeaderInts512 : kHeaderInts256;  // info, then the table
 };
+template <int BLOCK>
+__device__ __forceinline__ int32_t *block_header(const Walk &w, int i0) {
+    int32_t *base = BLOCK >= 512 ? w.headers512 : w.headers256;
+    return base + (size_t)(i0 / BLOCK) * StartTable<BLOCK>::header_ints;
+}
 template <int BLOCK, int STAGE>
 __device__ __forceinline__ bool stage_rows(const Walk &w, const float4 *dyn, int i0, int n, int cell_a, int cell_b,
                                            int cell_i, int tag_i, float2 *txy, float2 *thm, uint8_t *tix,
                                            float4 *tdyn, float2 *tuv, int *info, uint16_t *tstart,
                                            TileRows<STAGE> &rows, PairRows &pr, int tag, bool &only_tag,
-                                           bool stage_pr = false) {
+                                           bool stage_pr = false, const int32_t *hdr_in = nullptr,
+                                           int32_t *hdr_out = nullptr) {
     constexpr int kCap = 3 * PairShape<BLOCK>::tile_row - kTilePad;
     constexpr int kTab = StartTable<BLOCK>::row;
+    constexpr int kHdr = StartTable<BLOCK>::header_ints;
     const int t = threadIdx.x;
-    if (t < 6) {
+    if (hdr_in) {  // the block header a search wrote on this layout: info and the start table
+        for (int q = t; q < kHdr; q += BLOCK) {
+            const int v = hdr_in[q];
+            if (q < kStageInfo) info[q] = v;
+            else reinterpret_cast<int32_t *>(tstart)[q - kStageInfo] = v;
+        }
+    } else if (t < 6) {
         const int g = t / 3, r = t % 3;  // group (cell row) and neighbour row
         const int last = min(i0 + BLOCK, n) - 1;
         const CaseDesc cd = w.cases ? w.cases[w.cs[i0]] : w.one;
@@ -723,6 +739,7 @@ __device__ __forceinline__ bool stage_rows(const Walk &w, const float4 *dyn, int
         }
     }
     __syncthreads();
+    if (hdr_out && t < kStageInfo) hdr_out[t] = info[t];
     // block-uniform: range starts and lengths, and where each range goes in the tile
     auto u = [&](int k) { return __builtin_amdgcn_readfirstlane(info[k]); };
     const int d1 = u(6), d2 = d1 + u(7), d3 = d2 + u(8), d4 = d3 + u(9), d5 = d4 + u(10), total = d5 + u(11);
@@ -757,7 +774,7 @@ __device__ __forceinline__ bool stage_rows(const Walk &w, const float4 *dyn, int
         for (int it = 0; it < kTabIter; ++it) {
             const int q = t + it * BLOCK, k = q / kTab, x = q - k * kTab, gk = k >= 3, jj = cy0 + k - 1 - 2 * gk;
             ts[it] = 0;
-            if (table && q < 6 * kTab && x < info[17 + gk] && jj >= 0 && jj < ny)  // (k varies by thread)
+            if (!hdr_in && table && q < 6 * kTab && x < info[17 + gk] && jj >= 0 && jj < ny)  // (k varies by thread)
                 ts[it] = w.start[cbase + info[15 + gk] + x + nx * jj] - info[k];
         }
 #pragma unroll
@@ -774,15 +791,19 @@ __device__ __forceinline__ bool stage_rows(const Walk &w, const float4 *dyn, int
                 if (STAGE == 2) tuv[q] = uv[it];
             }
         }
+        if (!hdr_in) {
 #pragma unroll
-        for (int it = 0; it < kTabIter; ++it) {
-            const int q = t + it * BLOCK;
-            if (q < 6 * kTab) tstart[q] = (uint16_t)ts[it];
+            for (int it = 0; it < kTabIter; ++it) {
+                const int q = t + it * BLOCK;
+                if (q < 6 * kTab) tstart[q] = (uint16_t)ts[it];
+            }
         }
     } else if (t == 0) {
         atomicAdd(w.fallback, 1u);
     }
     only_tag = ok && !__syncthreads_or(other);
+    if (hdr_out && ok && table)
+        for (int q = t; q < 3 * kTab; q += BLOCK) hdr_out[kStageInfo + q] = reinterpret_cast<const int32_t *>(tstart)[q];
     if (ok) {
         int cxi, cyi;
         cell_xy(cell_i - 1, nx, cxi, cyi);
@@ -972,7 +993,7 @@ __global__ __launch_bounds__(BLOCK, BATCH ? 1 : 8) void k_search(Walk w, Fields 
     __shared__ uint8_t tix[3 * kTileRow];
     __shared__ float2 tuv[STAGE == 2 ? 3 * kTileRow : 1];
     __shared__ int info[kStageInfo];
-    __shared__ uint16_t tstart[6 * StartTable<BLOCK>::row];
+    __shared__ __align__(16) uint16_t tstart[6 * StartTable<BLOCK>::row];
     const int i0 = xcd_block(blockIdx.x, gridDim.x) * BLOCK;
     // the cells of the block's first and last slots, loaded with the alive count (i0 < cap: the grid
     // covers the slots)
@@ -1001,7 +1022,8 @@ __global__ __launch_bounds__(BLOCK, BATCH ? 1 : 8) void k_search(Walk w, Fields 
     bool all_fluid = false;  // every particle of the block's neighbourhood is fluid
     const bool tiled = ACTION == EPPX_SEARCH_KERNEL_CONTINUITY &&
                        stage_rows<BLOCK, STAGE>(w, f.dyn, i0, n, cell_a, cell_b, cell_i, ti, txy, thm, tix, nullptr,
-                                                tuv, info, tstart, rows, pr, ph.fluid_tag, all_fluid);
+                                                tuv, info, tstart, rows, pr, ph.fluid_tag, all_fluid, false, nullptr,
+                                                !BATCH && w.header_mode == 1 ? block_header<BLOCK>(w, i0) : nullptr);
     int cnt = 0;
     const CaseDesc cd = BATCH ? w.cases[w.cs[ic]] : w.one;
     if (!tiled) pr = pair_rows_global(w, cd, cell_i - 1, ti);
@@ -1304,7 +1326,7 @@ __global__ __launch_bounds__(BLOCK, (BATCH && !FAST) ? 1 : 8) void k_momentum(Wa
     __shared__ uint8_t tix[3 * kTileRow];
     __shared__ float4 tdyn[STAGE == 1 ? 3 * kTileRow : 1];
     __shared__ int info[kStageInfo];
-    __shared__ uint16_t tstart[6 * StartTable<BLOCK>::row];
+    __shared__ __align__(16) uint16_t tstart[6 * StartTable<BLOCK>::row];
     const int i0 = xcd_block(blockIdx.x, gridDim.x) * BLOCK;
     // the cells of the block's first and last slots, loaded with the alive count (i0 < cap: the grid
     // covers the slots)
@@ -1331,7 +1353,8 @@ __global__ __launch_bounds__(BLOCK, (BATCH && !FAST) ? 1 : 8) void k_momentum(Wa
             stage_pr = c0.hinv_u > 0.0f && c0.m_u > 0.0f && ph.coefficient == 0.0f;
         }
         tiled = stage_rows<BLOCK, STAGE>(w, f.dyn, i0, n, cell_a, cell_b, cell_i, tag_i, txy, thm, tix, tdyn,
-                                         nullptr, info, tstart, rows, pr, ph.fluid_tag, all_fluid, stage_pr);
+                                         nullptr, info, tstart, rows, pr, ph.fluid_tag, all_fluid, stage_pr,
+                                         !BATCH && w.header_mode == 2 ? block_header<BLOCK>(w, i0) : nullptr);
     }
     if (!MOVE && i >= n) return;
     const CaseDesc cd = BATCH ? w.cases[w.cs[ic]] : w.one;
