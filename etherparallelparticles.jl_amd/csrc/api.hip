@@ -541,6 +541,21 @@ int32_t tune_pair_blocks(eppx_ps *ps, const Walk &w, const Physics &ph, int32_t 
             if (us[v] < us[best_s]) best_s = v;
             if (us[kShapes + v] < us[kShapes + best_m]) best_m = v;
         }
+        // In a step the momentum pass reads the search's block headers only when both kernels have one block
+        // size (the trials gave every size its headers): the fastest pair of one size, when its total is
+        // within 2 % of the fastest pair of any sizes.
+        int pair_s = -1, pair_m = -1;
+        for (int v = 0; v < kShapes; ++v)
+            for (int m = 0; m < kShapes; ++m)
+                if (kShapeBlock[v] == kShapeBlock[m] &&
+                    (pair_s < 0 || us[v] + us[kShapes + m] < us[pair_s] + us[kShapes + pair_m])) {
+                    pair_s = v;
+                    pair_m = m;
+                }
+        if (pair_s >= 0 && us[pair_s] + us[kShapes + pair_m] <= 1.02 * (us[best_s] + us[kShapes + best_m])) {
+            best_s = pair_s;
+            best_m = pair_m;
+        }
         ps->search_block = kShapeBlock[best_s];
         ps->search_stage = kShapeStage[best_s];
         ps->momentum_block = kShapeBlock[best_m];
