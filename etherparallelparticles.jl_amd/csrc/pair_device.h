@@ -1325,6 +1325,7 @@ __global__ __launch_bounds__(BLOCK, (BATCH && !FAST) ? 1 : 8) void k_momentum(Wa
     __shared__ float2 thm[3 * kTileRow];
     __shared__ uint8_t tix[3 * kTileRow];
     __shared__ float4 tdyn[STAGE == 1 ? 3 * kTileRow : 1];
+    __shared__ float2 tuv[STAGE == 2 ? 3 * kTileRow : 1];
     __shared__ int info[kStageInfo];
     __shared__ __align__(16) uint16_t tstart[6 * StartTable<BLOCK>::row];
     const int i0 = xcd_block(blockIdx.x, gridDim.x) * BLOCK;
@@ -1347,13 +1348,13 @@ __global__ __launch_bounds__(BLOCK, (BATCH && !FAST) ? 1 : 8) void k_momentum(Wa
     bool tiled = false;
     bool stage_pr = false;   // the tile holds (x_j, rho_j) (the one-H, one-Mass FAST terms)
     if (i0 < n) {  // the same for the whole block
-        if (FAST && STAGE == 0) {
+        if (FAST && (STAGE == 0 || STAGE == 2)) {
             // a tiled block lies in one case: the case of its first slot
             const CaseDesc c0 = BATCH ? w.cases[w.cs[i0]] : w.one;
             stage_pr = c0.hinv_u > 0.0f && c0.m_u > 0.0f && ph.coefficient == 0.0f;
         }
         tiled = stage_rows<BLOCK, STAGE>(w, f.dyn, i0, n, cell_a, cell_b, cell_i, tag_i, txy, thm, tix, tdyn,
-                                         nullptr, info, tstart, rows, pr, ph.fluid_tag, all_fluid, stage_pr,
+                                         tuv, info, tstart, rows, pr, ph.fluid_tag, all_fluid, stage_pr,
                                          !BATCH && w.header_mode == 2 ? block_header<BLOCK>(w, i0) : nullptr);
     }
     if (!MOVE && i >= n) return;
@@ -1395,8 +1396,11 @@ __global__ __launch_bounds__(BLOCK, (BATCH && !FAST) ? 1 : 8) void k_momentum(Wa
             float2 sum = make_float2(0.0f, 0.0f);  // the pair terms over -m_u kfac
             auto fetch_um = [&](int j, int k, uint32_t tbj, const auto &src) -> float4 {
                 using Src = std::decay_t<decltype(src)>;
-                if constexpr (Src::kTile && Src::kStage == 0) {  // (u, v) gathered, (x_j, rho_j) from the tile
-                    const float2 uvj = ld_slot(reinterpret_cast<const float2 *>(f.dyn), 2 * j);
+                if constexpr (Src::kTile && (Src::kStage == 0 || Src::kStage == 2)) {
+                    // (x_j, rho_j) from the tile; (u, v) from the tile (STAGE 2) or gathered
+                    float2 uvj;
+                    if constexpr (Src::kStage == 2) uvj = src.uv[k];
+                    else uvj = ld_slot(reinterpret_cast<const float2 *>(f.dyn), 2 * j);
                     const float2 xr = src.hm[k];
                     return make_float4(uvj.x, uvj.y, xr.y, xr.x);
                 } else {
