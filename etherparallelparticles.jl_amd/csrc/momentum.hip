@@ -17,9 +17,9 @@ void launch_kb(const Walk &w, Fields f, const Physics &ph, int32_t n, const Move
     if (mk)  // a particle system's step: gravity + move and the coming search's cell keys in the epilogue
         hipLaunchKernelGGL((k_momentum<FAST, K, VISC, false, BLOCK, STAGE, true>), g, dim3(BLOCK), 0, s, w, f, ph,
                            *mk);
-    else if (w.cases)
-        hipLaunchKernelGGL((k_momentum<FAST, K, VISC, true, BLOCK, STAGE, false>), g, dim3(BLOCK), 0, s, w, f, ph,
-                           MoveKey{});
+    else if (w.cases)  // a batch gathers (u, v) (its kernels need more registers; staging them measured slower)
+        hipLaunchKernelGGL((k_momentum<FAST, K, VISC, true, BLOCK, STAGE == 2 ? 0 : STAGE, false>), g, dim3(BLOCK), 0,
+                           s, w, f, ph, MoveKey{});
     else
         hipLaunchKernelGGL((k_momentum<FAST, K, VISC, false, BLOCK, STAGE, false>), g, dim3(BLOCK), 0, s, w, f, ph,
                            MoveKey{});

@@ -181,6 +181,15 @@ struct UniformTerms {
         const float t = fmaxf(fmaf(r, -hinv, 2.0f), 0.0f);  // 2 - q
         return t * (t * t);
     }
+    // shape at dr2: Wendland C2 takes r from one hardware square root (one
+    // instruction fewer than r = dr2 rsq(dr2), and finite at dr2 = 0)
+    __device__ __forceinline__ float shape2(float dr2) const {
+        if (K == EPPX_KERNEL_CUBIC) {
+            const float rinv = frsq(dr2);
+            return shape(dr2 * rinv, rinv);
+        }
+        return shape(__builtin_amdgcn_sqrtf(dr2), 0.0f);
+    }
 };
 
 // p_j / rho_j^2 of the classic pressure term (FAST): one expression wherever
@@ -536,7 +545,7 @@ __device__ __forceinline__ int walk_rows(const Walk &w, const CaseDesc &cd, cons
     const uint32_t kadj = adj & keep_tags;
     const float2 xi = make_float2(gi.x, gi.y);
     const unsigned long long self = 1ull << (i - b[1]);
-    unsigned long long m0, m1, m2;
+    unsigned long long m0 = 0ull, m1 = 0ull, m2 = 0ull;
     const size_t ms = (size_t)w.mask_stride;
     if (!COUNT && w.mask_mode == 2) {  // the last search's masks: the same pair set, the same keep classes
         m0 = w.masks[i];
@@ -932,18 +941,16 @@ __device__ __forceinline__ float continuity_fast(const PairH &ph, float2 ui, flo
 template <int K>
 __device__ __forceinline__ float continuity_fast_u(const UniformTerms<K> &ut, float2 ui, float2 uj, float mj_or_rhoVj,
                                                    float dx, float dy, float dr2) {
-    const float rinv = frsq(dr2);
     const float vdotx = fmaf(dy, ui.y - uj.y, dx * (ui.x - uj.x));
-    return (mj_or_rhoVj * vdotx) * ut.shape(dr2 * rinv, rinv);
+    return (mj_or_rhoVj * vdotx) * ut.shape2(dr2);
 }
 
 // The fluid-fluid term over kfac m_u (one H and one Mass in the case).
 template <int K>
 __device__ __forceinline__ float vdotx_shape_u(const UniformTerms<K> &ut, float2 ui, float2 uj, float dx, float dy,
                                                float dr2) {
-    const float rinv = frsq(dr2);
     const float vdotx = fmaf(dy, ui.y - uj.y, dx * (ui.x - uj.x));
-    return vdotx * ut.shape(dr2 * rinv, rinv);
+    return vdotx * ut.shape2(dr2);
 }
 
 // The reference's written order, given the pair's hinv = invharmonic(h_i, h_j).
@@ -1080,8 +1087,7 @@ __global__ __launch_bounds__(BLOCK, BATCH ? 1 : 8) void k_search(Walk w, Fields 
                         auto term_m = [&](int, uint32_t tbj, float, float, float dx, float dy, float dr2,
                                           const ContinuityGather &p) {
                             const bool ff = i_fluid && tbj == fluid_bit;
-                            const float rinv = frsq(dr2);
-                            const float shape = ut.shape(dr2 * rinv, rinv);
+                            const float shape = ut.shape2(dr2);
                             const float vdotx = fmaf(dy, ui.y - p.uv.y, dx * (ui.x - p.uv.x));
                             s_ff += ff ? vdotx * shape : 0.0f;
                             s_mx += ff ? 0.0f : ((rho_i * p.vol) * vdotx) * shape;
@@ -1235,28 +1241,32 @@ __device__ __forceinline__ void momentum_fast_u(const Physics &ph, float2 &sum, 
 
 // One H and one Mass in the case, and no pressure correction (coefficient 0,
 // as in the driver) (FAST): the sums leave out -m_u as well (every term
-// carries shape where the terms above carry mdwr = -m_j shape), and the
-// caller adds -m_u kfac x sum once.  x_j: p_j / rho_j^2 (pressure_ratio) for a
-// classic (fluid) neighbour, p_j for a balanced (wall) one.
+// carries shape where the terms above carry mdwr = -m_j shape), and carry
+// rho_i times each term, so no term forms rho_i rho_j; the caller adds
+// -m_u kfac / rho_i x sum once.  x_j: p_j / rho_j^2 (pressure_ratio) for a
+// classic (fluid) neighbour, p_j for a balanced (wall) one; pir_i = p_i / rho_i.
+//   classic pressure  rho_i (p_i / rho_i^2 + x_j) = pir_i + rho_i x_j
+//   balanced pressure rho_i (p_i + x_j) / (rho_i rho_j) = (p_i + x_j) / rho_j
+//   artificial viscosity (av_b m - av_a qa) m / (qa^2 (rho_i + rho_j) rho_j)
+// and radial = shape (pressure + viscosity): one product by shape.
 template <int K, int VISC>
 __device__ __forceinline__ void momentum_fast_um(const Physics &ph, float2 &sum, bool classic,
                                                  const UniformTerms<K> &ut, float2 ui, float rho_i, float p_i,
-                                                 float pr_i, float2 uj, float rho_j, float x_j, float dx, float dy,
+                                                 float pir_i, float2 uj, float rho_j, float x_j, float dx, float dy,
                                                  float dr2) {
-    const float rinv = frsq(dr2);
-    const float shape = ut.shape(dr2 * rinv, rinv);
-    const float rirj = rho_i * rho_j;
-    const float prho2 = classic ? pr_i + x_j : (p_i + x_j) * frcp(rirj);  // (the caller has coefficient 0)
-    float radial = prho2 * shape;
+    const float shape = ut.shape2(dr2);
+    const float prho = classic ? fmaf(rho_i, x_j, pir_i) : (p_i + x_j) * frcp(rho_j);  // (coefficient 0)
     const float qa = fmaf(dr2, ut.hinv2, 0.01f);  // q^2 + 0.01
+    float radial;
     if (VISC == EPPX_VISC_CLASSIC) {
-        const float force = (-2.0f * ph.mu) * (dr2 * ut.hinv2) * shape * frcp(rirj * qa);
+        radial = prho * shape;
+        const float force = (-2.0f * ph.mu) * (dr2 * ut.hinv2) * shape * frcp(rho_j * qa);
         sum.x = fmaf(force, ui.x - uj.x, sum.x);
         sum.y = fmaf(force, ui.y - uj.y, sum.y);
     } else {
         const float m = fminf(fmaf(dy, ui.y - uj.y, dx * (ui.x - uj.x)), 0.0f);
-        const float den = (qa * qa) * ((rho_i + rho_j) * rirj);
-        radial = fmaf(fmaf(ut.av_b, m, -ut.av_a * qa) * m, shape * frcp(den), radial);
+        const float den = (qa * qa) * ((rho_i + rho_j) * rho_j);
+        radial = shape * fmaf(fmaf(ut.av_b, m, -ut.av_a * qa) * m, frcp(den), prho);
     }
     sum.x = fmaf(radial, dx, sum.x);
     sum.y = fmaf(radial, dy, sum.y);
@@ -1393,7 +1403,8 @@ __global__ __launch_bounds__(BLOCK, (BATCH && !FAST) ? 1 : 8) void k_momentum(Wa
             // neighbour's x_j (p_j / rho_j^2 or p_j) comes from the tile or is taken per pair with the
             // same expression.
             const UniformTerms<K> ut(cd.hinv_u, ph);
-            float2 sum = make_float2(0.0f, 0.0f);  // the pair terms over -m_u kfac
+            float2 sum = make_float2(0.0f, 0.0f);  // the pair terms over -m_u kfac / rho_i
+            const float pir_i = di.w / di.z;       // p_i / rho_i
             auto fetch_um = [&](int j, int k, uint32_t tbj, const auto &src) -> float4 {
                 using Src = std::decay_t<decltype(src)>;
                 if constexpr (Src::kTile && (Src::kStage == 0 || Src::kStage == 2)) {
@@ -1413,19 +1424,19 @@ __global__ __launch_bounds__(BLOCK, (BATCH && !FAST) ? 1 : 8) void k_momentum(Wa
             };
             if (all_fluid) {  // fluid-fluid pairs only (classic pressure): no tag per pair (tiled blocks)
                 auto term_f = [&](int, uint32_t, float, float, float dx, float dy, float dr2, const float4 &dj) {
-                    momentum_fast_um<K, VISC>(ph, sum, true, ut, ui, di.z, di.w, pr_i, make_float2(dj.x, dj.y), dj.z,
+                    momentum_fast_um<K, VISC>(ph, sum, true, ut, ui, di.z, di.w, pir_i, make_float2(dj.x, dj.y), dj.z,
                                               dj.w, dx, dy, dr2);
                 };
                 walk_rows<false, false>(w, cd, rows, i, gi, pr, keep, fetch_um, term_f);
             } else {
                 auto term_m = [&](int, uint32_t tbj, float, float, float dx, float dy, float dr2, const float4 &dj) {
-                    momentum_fast_um<K, VISC>(ph, sum, tbj == fluid_bit, ut, ui, di.z, di.w, pr_i,
+                    momentum_fast_um<K, VISC>(ph, sum, tbj == fluid_bit, ut, ui, di.z, di.w, pir_i,
                                               make_float2(dj.x, dj.y), dj.z, dj.w, dx, dy, dr2);
                 };
                 if (tiled) walk_rows<false, false>(w, cd, rows, i, gi, pr, keep, fetch_um, term_m);
                 else walk_rows<false, false>(w, cd, GlobalRows{w.geo, w.tag}, i, gi, pr, keep, fetch_um, term_m);
             }
-            const float km = -(cd.m_u * ut.kfac);
+            const float km = -(cd.m_u * ut.kfac) / di.z;  // the terms carry rho_i
             acc.x = fmaf(km, sum.x, acc.x);
             acc.y = fmaf(km, sum.y, acc.y);
             walked = true;
