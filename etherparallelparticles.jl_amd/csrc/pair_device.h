@@ -373,7 +373,8 @@ __device__ __forceinline__ uint32_t accept_bits(const Walk &w, const CaseDesc &c
 // get(k): the snapshot (x, y, h, m) of slot k; xy_of(k), h_of(k): its parts
 // (pass 1 reads (x, y) only); tbit(k): its tag bit.
 // kStage: what else a tile holds for pass 2 (0 nothing, 1 the (u, v, rho, p)
-// record, 2 its (u, v) half).
+// record, 2 its (u, v) half, 3 the (u, v) half in place of (h, m), which the
+// terms of a case with one H and one Mass do not read).
 struct GlobalRows {
     static constexpr int kStage = 0;
     static constexpr bool kTile = false;
@@ -409,6 +410,19 @@ struct TileRows {
     // tag bit of an accepted neighbour: its tag is adjacent, so in 1..kMaxTag
     __device__ __forceinline__ uint32_t tbit_adj(int k) const { return 1u << tix[k]; }
     __device__ __forceinline__ float4 at(int r, int j) const { return get(j + off(r)); }
+    // the same tile read with (u, v) in the second array (staged in place of (h, m))
+    __device__ __forceinline__ TileRows<3> uv_in_hm() const {
+        TileRows<3> t;
+        t.xy = xy;
+        t.hm = hm;
+        t.tix = tix;
+        t.dyn = dyn;
+        t.uv = hm;
+        t.off0 = off0;
+        t.off1 = off1;
+        t.off2 = off2;
+        return t;
+    }
 };
 
 template <bool CHECK, class Src>
@@ -685,6 +699,9 @@ __device__ __forceinline__ int walk_rows(const Walk &w, const CaseDesc &cd, cons
 // hdr_out: this block's header, written (a search); hdr_in: read instead of
 // finding the ranges and the start table (a pass on the layout of the search
 // that wrote it, with the same block size): one round trip fewer.
+// UV and stage_uv (search, one H and one Mass): the second tile array holds
+// (u_j, v_j) instead of (h_j, m_j), so pass 2 gathers nothing from global
+// memory for fluid-fluid pairs.
 template <int BLOCK>
 struct StartTable {  // 16-bit cell starts per group row: what the LDS left by the tile holds
     static constexpr int row = BLOCK >= 512 ? kStartRow512 : kStartRow256;
@@ -695,13 +712,13 @@ __device__ __forceinline__ int32_t *block_header(const Walk &w, int i0) {
     int32_t *base = BLOCK >= 512 ? w.headers512 : w.headers256;
     return base + (size_t)(i0 / BLOCK) * StartTable<BLOCK>::header_ints;
 }
-template <int BLOCK, int STAGE>
+template <int BLOCK, int STAGE, bool UV = false>
 __device__ __forceinline__ bool stage_rows(const Walk &w, const float4 *dyn, int i0, int n, int cell_a, int cell_b,
                                            int cell_i, int tag_i, float2 *txy, float2 *thm, uint8_t *tix,
                                            float4 *tdyn, float2 *tuv, int *info, uint16_t *tstart,
                                            TileRows<STAGE> &rows, PairRows &pr, int tag, bool &only_tag,
                                            bool stage_pr = false, const int32_t *hdr_in = nullptr,
-                                           int32_t *hdr_out = nullptr) {
+                                           int32_t *hdr_out = nullptr, bool stage_uv = false) {
     constexpr int kCap = 3 * PairShape<BLOCK>::tile_row - kTilePad;
     constexpr int kTab = StartTable<BLOCK>::row;
     constexpr int kHdr = StartTable<BLOCK>::header_ints;
@@ -774,9 +791,17 @@ __device__ __forceinline__ bool stage_rows(const Walk &w, const float4 *dyn, int
             const int sl = q + (q >= d5 ? e5 : q >= d4 ? e4 : q >= d3 ? e3 : q >= d2 ? e2 : q >= d1 ? e1 : e0);
             g[it] = w.geo[sl];
             tg[it] = w.tag[sl];
-            if (stage_pr) rp[it] = reinterpret_cast<const float2 *>(dyn)[2 * (size_t)sl + 1];  // (rho, p)
+            if constexpr (STAGE == 2) {  // the whole record in one load: (u, v), and (rho, p) for stage_pr
+                const float4 d = dyn[sl];
+                uv[it] = make_float2(d.x, d.y);
+                rp[it] = make_float2(d.z, d.w);
+            } else if (stage_pr) {
+                rp[it] = reinterpret_cast<const float2 *>(dyn)[2 * (size_t)sl + 1];  // (rho, p)
+            }
+            if constexpr (UV) {  // rp is read under the condition it is written on
+                if (stage_uv) rp[it] = reinterpret_cast<const float2 *>(dyn)[2 * (size_t)sl];  // (u, v)
+            }
             if (STAGE == 1) dd[it] = dyn[sl];
-            if (STAGE == 2) uv[it] = reinterpret_cast<const float2 *>(dyn)[2 * (size_t)sl];
         }
         // table entry x of group row k = (g, r): the start of cell xlo_g + x of cell row cy0 + g + r - 1
 #pragma unroll
@@ -791,8 +816,11 @@ __device__ __forceinline__ bool stage_rows(const Walk &w, const float4 *dyn, int
             const int q = t + it * BLOCK;
             if (q < total) {
                 txy[q] = make_float2(g[it].x, g[it].y);
-                thm[q] = stage_pr ? make_float2(tg[it] == tag ? pressure_ratio(rp[it].y, rp[it].x) : rp[it].y, rp[it].x)
-                                  : make_float2(g[it].z, g[it].w);
+                if (UV && stage_uv) thm[q] = rp[it];
+                else
+                    thm[q] = stage_pr ? make_float2(tg[it] == tag ? pressure_ratio(rp[it].y, rp[it].x) : rp[it].y,
+                                                    rp[it].x)
+                                      : make_float2(g[it].z, g[it].w);
                 other |= tg[it] != tag;
                 const uint32_t t1 = (uint32_t)tg[it] - 1u;
                 tix[q] = (uint8_t)(t1 < (uint32_t)kMaxTag ? t1 : 63u);
@@ -1027,10 +1055,16 @@ __global__ __launch_bounds__(BLOCK, BATCH ? 1 : 8) void k_search(Walk w, Fields 
     TileRows<STAGE> rows;
     PairRows pr;
     bool all_fluid = false;  // every particle of the block's neighbourhood is fluid
+    // the tile holds (u_j, v_j) in place of (h_j, m_j): FAST, gathered shapes, one H and one Mass
+    constexpr bool kUV = FAST && STAGE == 0 && !BATCH && ACTION == EPPX_SEARCH_KERNEL_CONTINUITY;
+    const bool stage_uv = kUV && w.one.hinv_u > 0.0f && w.one.m_u > 0.0f;
     const bool tiled = ACTION == EPPX_SEARCH_KERNEL_CONTINUITY &&
-                       stage_rows<BLOCK, STAGE>(w, f.dyn, i0, n, cell_a, cell_b, cell_i, ti, txy, thm, tix, nullptr,
-                                                tuv, info, tstart, rows, pr, ph.fluid_tag, all_fluid, false, nullptr,
-                                                !BATCH && w.header_mode == 1 ? block_header<BLOCK>(w, i0) : nullptr);
+                       stage_rows<BLOCK, STAGE, kUV>(w, f.dyn, i0, n, cell_a, cell_b, cell_i, ti, txy, thm, tix,
+                                                     nullptr, tuv, info, tstart, rows, pr, ph.fluid_tag, all_fluid,
+                                                     false, nullptr,
+                                                     !BATCH && w.header_mode == 1 ? block_header<BLOCK>(w, i0)
+                                                                                  : nullptr,
+                                                     stage_uv);
     int cnt = 0;
     const CaseDesc cd = BATCH ? w.cases[w.cs[ic]] : w.one;
     if (!tiled) pr = pair_rows_global(w, cd, cell_i - 1, ti);
@@ -1049,7 +1083,7 @@ __global__ __launch_bounds__(BLOCK, BATCH ? 1 : 8) void k_search(Walk w, Fields 
             // u_j, and V_j for mixed pairs only (fluid-fluid terms use m_j from the snapshot)
             auto fetch = [&](int j, int k, uint32_t tbj, const auto &src) {
                 ContinuityGather p;
-                if constexpr (std::decay_t<decltype(src)>::kStage == 2) p.uv = src.uv[k];
+                if constexpr (std::decay_t<decltype(src)>::kStage >= 2) p.uv = src.uv[k];
                 else p.uv = ld_slot(uv, 2 * j);
                 p.vol = (i_fluid && tbj == fluid_bit) ? 0.0f : ld_slot(f.vol, j);
                 return p;
@@ -1073,7 +1107,7 @@ __global__ __launch_bounds__(BLOCK, BATCH ? 1 : 8) void k_search(Walk w, Fields 
                     if (all_fluid) {
                         auto fetch_f = [&](int j, int k, uint32_t, const auto &src) {
                             ContinuityGather p;
-                            if constexpr (std::decay_t<decltype(src)>::kStage == 2) p.uv = src.uv[k];
+                            if constexpr (std::decay_t<decltype(src)>::kStage >= 2) p.uv = src.uv[k];
                             else p.uv = ld_slot(uv, 2 * j);
                             p.vol = 0.0f;
                             return p;
@@ -1082,7 +1116,9 @@ __global__ __launch_bounds__(BLOCK, BATCH ? 1 : 8) void k_search(Walk w, Fields 
                                           const ContinuityGather &p) {
                             s_ff += vdotx_shape_u<K>(ut, ui, p.uv, dx, dy, dr2);
                         };
-                        cnt = walk_rows<false, true>(w, cd, rows, i, gi, pr, keep, fetch_f, term_f);
+                        if (stage_uv)
+                            cnt = walk_rows<false, true>(w, cd, rows.uv_in_hm(), i, gi, pr, keep, fetch_f, term_f);
+                        else cnt = walk_rows<false, true>(w, cd, rows, i, gi, pr, keep, fetch_f, term_f);
                     } else {
                         auto term_m = [&](int, uint32_t tbj, float, float, float dx, float dy, float dr2,
                                           const ContinuityGather &p) {
@@ -1092,8 +1128,11 @@ __global__ __launch_bounds__(BLOCK, BATCH ? 1 : 8) void k_search(Walk w, Fields 
                             s_ff += ff ? vdotx * shape : 0.0f;
                             s_mx += ff ? 0.0f : ((rho_i * p.vol) * vdotx) * shape;
                         };
-                        if (tiled) cnt = walk_rows<false, true>(w, cd, rows, i, gi, pr, keep, fetch, term_m);
-                        else cnt = walk_rows<false, true>(w, cd, GlobalRows{w.geo, w.tag}, i, gi, pr, keep, fetch, term_m);
+                        if (tiled && stage_uv)
+                            cnt = walk_rows<false, true>(w, cd, rows.uv_in_hm(), i, gi, pr, keep, fetch, term_m);
+                        else if (tiled) cnt = walk_rows<false, true>(w, cd, rows, i, gi, pr, keep, fetch, term_m);
+                        else
+                            cnt = walk_rows<false, true>(w, cd, GlobalRows{w.geo, w.tag}, i, gi, pr, keep, fetch, term_m);
                     }
                     acc = fmaf(ut.kfac, s_mx, fmaf(ut.kfac * cd.m_u, s_ff, acc));
                     walked = true;
@@ -1103,7 +1142,7 @@ __global__ __launch_bounds__(BLOCK, BATCH ? 1 : 8) void k_search(Walk w, Fields 
                     if (all_fluid) {  // fluid-fluid pairs only (classic continuity): no tag per pair (tiled)
                         auto fetch_f = [&](int j, int k, uint32_t, const auto &src) {
                             ContinuityGather p;
-                            if constexpr (std::decay_t<decltype(src)>::kStage == 2) p.uv = src.uv[k];
+                            if constexpr (std::decay_t<decltype(src)>::kStage >= 2) p.uv = src.uv[k];
                             else p.uv = ld_slot(uv, 2 * j);
                             p.vol = 0.0f;
                             return p;
