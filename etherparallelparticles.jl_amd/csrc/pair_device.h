@@ -712,14 +712,14 @@ __device__ __forceinline__ int32_t *block_header(const Walk &w, int i0) {
     int32_t *base = BLOCK >= 512 ? w.headers512 : w.headers256;
     return base + (size_t)(i0 / BLOCK) * StartTable<BLOCK>::header_ints;
 }
-template <int BLOCK, int STAGE, bool UV = false>
+template <int BLOCK, int STAGE, bool UV = false, int TROW = PairShape<BLOCK>::tile_row>
 __device__ __forceinline__ bool stage_rows(const Walk &w, const float4 *dyn, int i0, int n, int cell_a, int cell_b,
                                            int cell_i, int tag_i, float2 *txy, float2 *thm, uint8_t *tix,
                                            float4 *tdyn, float2 *tuv, int *info, uint16_t *tstart,
                                            TileRows<STAGE> &rows, PairRows &pr, int tag, bool &only_tag,
                                            bool stage_pr = false, const int32_t *hdr_in = nullptr,
                                            int32_t *hdr_out = nullptr, bool stage_uv = false) {
-    constexpr int kCap = 3 * PairShape<BLOCK>::tile_row - kTilePad;
+    constexpr int kCap = 3 * TROW - kTilePad;
     constexpr int kTab = StartTable<BLOCK>::row;
     constexpr int kHdr = StartTable<BLOCK>::header_ints;
     const int t = threadIdx.x;
@@ -1369,7 +1369,10 @@ template <bool FAST, int K, int VISC, bool BATCH, int BLOCK, int STAGE, bool MOV
 // Eight waves per SIMD; the EXACT batch kernels would spill at that bound and keep the compiler's choice.
 __global__ __launch_bounds__(BLOCK, (BATCH && !FAST) ? 1 : 8) void k_momentum(Walk w, Fields f, Physics ph,
                                                                                MoveKey mk) {
-    constexpr int kTileRow = PairShape<BLOCK>::tile_row;
+    // 256 threads with (u, v) staged: a tile of 3 x 336 slots (25 B each) keeps the block at 26 KB of
+    // LDS, six blocks per CU instead of five (and four staging loads per thread instead of five); the
+    // blocks whose ranges exceed its 944 slots (0.6 % of them on the 2M dam break) read global memory
+    constexpr int kTileRow = (STAGE == 2 && BLOCK == 256) ? 336 : PairShape<BLOCK>::tile_row;
     __shared__ float2 txy[3 * kTileRow];
     __shared__ float2 thm[3 * kTileRow];
     __shared__ uint8_t tix[3 * kTileRow];
@@ -1402,7 +1405,7 @@ __global__ __launch_bounds__(BLOCK, (BATCH && !FAST) ? 1 : 8) void k_momentum(Wa
             const CaseDesc c0 = BATCH ? w.cases[w.cs[i0]] : w.one;
             stage_pr = c0.hinv_u > 0.0f && c0.m_u > 0.0f && ph.coefficient == 0.0f;
         }
-        tiled = stage_rows<BLOCK, STAGE>(w, f.dyn, i0, n, cell_a, cell_b, cell_i, tag_i, txy, thm, tix, tdyn,
+        tiled = stage_rows<BLOCK, STAGE, false, kTileRow>(w, f.dyn, i0, n, cell_a, cell_b, cell_i, tag_i, txy, thm, tix, tdyn,
                                          tuv, info, tstart, rows, pr, ph.fluid_tag, all_fluid, stage_pr,
                                          !BATCH && w.header_mode == 2 ? block_header<BLOCK>(w, i0) : nullptr);
     }
