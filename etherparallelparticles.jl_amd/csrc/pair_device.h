@@ -1373,6 +1373,9 @@ __global__ __launch_bounds__(BLOCK, (BATCH && !FAST) ? 1 : 8) void k_momentum(Wa
     // LDS, six blocks per CU instead of five (and four staging loads per thread instead of five); the
     // blocks whose ranges exceed its 944 slots (0.6 % of them on the 2M dam break) read global memory
     constexpr int kTileRow = (STAGE == 2 && BLOCK == 256) ? 336 : PairShape<BLOCK>::tile_row;
+    // a block header carries the start table only where the search's tile held the block; a tile no
+    // larger than the search's holds only such blocks
+    static_assert(kTileRow <= PairShape<BLOCK>::tile_row, "momentum tile larger than the search's");
     __shared__ float2 txy[3 * kTileRow];
     __shared__ float2 thm[3 * kTileRow];
     __shared__ uint8_t tix[3 * kTileRow];
