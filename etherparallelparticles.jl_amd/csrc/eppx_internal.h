@@ -28,7 +28,10 @@ constexpr int kPairWords = 2 * kPairSlots * kPairStride;
 
 // Pair-kernel block headers: kStageInfo ints of range bounds, then a table of
 // 16-bit cell starts, 6 rows of kStartRow256 / kStartRow512 entries (as ints).
+// Int kHeaderTable of the bounds: 1 when the search staged the block (the table is
+// then written wherever it fits).
 constexpr int kStageInfo = 24;
+constexpr int kHeaderTable = 21;
 constexpr int kStartRow256 = 40, kStartRow512 = 112;
 constexpr int kHeaderInts256 = kStageInfo + 3 * kStartRow256;
 constexpr int kHeaderInts512 = kStageInfo + 3 * kStartRow512;

@@ -762,15 +762,19 @@ __device__ __forceinline__ bool stage_rows(const Walk &w, const float4 *dyn, int
             info[14] = nx;
             info[19] = cd.g.n_y;
             info[20] = cd.cell_base;
+            info[kHeaderTable] = 0;
         }
     }
     __syncthreads();
-    if (hdr_out && t < kStageInfo) hdr_out[t] = info[t];
+    if (hdr_out && t < kStageInfo && t != kHeaderTable) hdr_out[t] = info[t];
     // block-uniform: range starts and lengths, and where each range goes in the tile
     auto u = [&](int k) { return __builtin_amdgcn_readfirstlane(info[k]); };
     const int d1 = u(6), d2 = d1 + u(7), d3 = d2 + u(8), d4 = d3 + u(9), d5 = d4 + u(10), total = d5 + u(11);
+    // a header holds the start table (where one fits) only for a block the search that wrote it
+    // staged (its tile may be smaller than this one)
     const bool ok = u(12) && total <= kCap &&
-                    (u(6) | u(7) | u(8) | u(9) | u(10) | u(11)) >= 0;  // no range runs backwards
+                    (u(6) | u(7) | u(8) | u(9) | u(10) | u(11)) >= 0 &&  // no range runs backwards
+                    (!hdr_in || u(kHeaderTable));
     const int nx = u(14), cy0 = u(13), ny = u(19), cbase = u(20);
     const bool table = u(17) <= kTab && u(18) <= kTab;
     int other = 0;  // this thread staged a slot of another tag
@@ -841,6 +845,7 @@ __device__ __forceinline__ bool stage_rows(const Walk &w, const float4 *dyn, int
     only_tag = ok && !__syncthreads_or(other);
     if (hdr_out && ok && table)
         for (int q = t; q < 3 * kTab; q += BLOCK) hdr_out[kStageInfo + q] = reinterpret_cast<const int32_t *>(tstart)[q];
+    if (hdr_out && t == 0) hdr_out[kHeaderTable] = ok;  // the table is there when one fits
     if (ok) {
         int cxi, cyi;
         cell_xy(cell_i - 1, nx, cxi, cyi);
@@ -1021,7 +1026,9 @@ __global__ __launch_bounds__(BLOCK, BATCH ? 1 : 8) void k_search(Walk w, Fields 
                                                                  int32_t *ncount, unsigned long long *pairs_last,
                                                                  unsigned long long *pairs_total, uint32_t *flags,
                                                                  int32_t which, EosOut eo) {
-    constexpr int kTileRow = PairShape<BLOCK>::tile_row;
+    // 256 threads, gathered shapes: a tile of 3 x 352 slots, four staging loads per thread instead of
+    // five (search 136.4 -> 130.5 us on the 2M dam break; 0.01 % more of its blocks read global memory)
+    constexpr int kTileRow = (STAGE == 0 && BLOCK == 256 && !BATCH) ? 352 : PairShape<BLOCK>::tile_row;
     __shared__ int lds[BLOCK / 64];
     __shared__ float2 txy[3 * kTileRow];
     __shared__ float2 thm[3 * kTileRow];
@@ -1059,7 +1066,7 @@ __global__ __launch_bounds__(BLOCK, BATCH ? 1 : 8) void k_search(Walk w, Fields 
     constexpr bool kUV = FAST && STAGE == 0 && !BATCH && ACTION == EPPX_SEARCH_KERNEL_CONTINUITY;
     const bool stage_uv = kUV && w.one.hinv_u > 0.0f && w.one.m_u > 0.0f;
     const bool tiled = ACTION == EPPX_SEARCH_KERNEL_CONTINUITY &&
-                       stage_rows<BLOCK, STAGE, kUV>(w, f.dyn, i0, n, cell_a, cell_b, cell_i, ti, txy, thm, tix,
+                       stage_rows<BLOCK, STAGE, kUV, kTileRow>(w, f.dyn, i0, n, cell_a, cell_b, cell_i, ti, txy, thm, tix,
                                                      nullptr, tuv, info, tstart, rows, pr, ph.fluid_tag, all_fluid,
                                                      false, nullptr,
                                                      !BATCH && w.header_mode == 1 ? block_header<BLOCK>(w, i0)
@@ -1373,9 +1380,6 @@ __global__ __launch_bounds__(BLOCK, (BATCH && !FAST) ? 1 : 8) void k_momentum(Wa
     // LDS, six blocks per CU instead of five (and four staging loads per thread instead of five); the
     // blocks whose ranges exceed its 944 slots (0.6 % of them on the 2M dam break) read global memory
     constexpr int kTileRow = (STAGE == 2 && BLOCK == 256) ? 336 : PairShape<BLOCK>::tile_row;
-    // a block header carries the start table only where the search's tile held the block; a tile no
-    // larger than the search's holds only such blocks
-    static_assert(kTileRow <= PairShape<BLOCK>::tile_row, "momentum tile larger than the search's");
     __shared__ float2 txy[3 * kTileRow];
     __shared__ float2 thm[3 * kTileRow];
     __shared__ uint8_t tix[3 * kTileRow];
