@@ -62,6 +62,9 @@ struct eppx_ps {
     float rmin2 = -1.0f, rmax2 = INFINITY;  // symmetry-criterion bounds from the H range
     float hinv_u = 0.0f;                    // one H for every alive particle: its pair hinv; else 0
     float m_u = 0.0f;                       // one Mass for every alive particle: that mass; else 0
+    // every slot of both buffers holds Mass mass_one and H h_one (the same bits): the permute moves neither
+    bool mh_one = false;
+    float mass_one = 0.0f, h_one = 0.0f;
     uint32_t tags_present = 0xffffffffu;    // tag bits of the uploaded particles
     bool odd_tags = true;                   // some tag outside 1..kMaxTag
     // search state
@@ -335,8 +338,8 @@ int32_t do_search(eppx_ps *ps, eppx_ns *ns, int32_t criterion, int32_t action, c
                return launch_permute(cur, nxt, ps->geo, ps->n, 0, nc, ps->key, ps->tmpslot, ps->start,
                                      ps->members, ps->n_alive, ctx->d_flags,
                                      (ps->accum_live ? kPermuteSums : 0) | (permute_no_drho ? kPermuteNoDrho : 0) |
-                                         (permute_no_dvel ? kPermuteNoDvel : 0),
-                                     dyn_src, vol_src, s);
+                                         (permute_no_dvel ? kPermuteNoDvel : 0) | (ps->mh_one ? kPermuteOneMassH : 0),
+                                     dyn_src, vol_src, ps->mass_one, ps->h_one, s);
            }));
     ps->cur ^= 1;
     Walk w = make_walk(ps, ns, criterion);
@@ -383,8 +386,9 @@ void drop_graphs(eppx_ps *ps) {
 // carry the old case table and are dropped).
 int32_t refresh_snapshot_mass(eppx_ps *ps, int32_t field) {
     if (field != EPPX_FIELD_MASS) return EPPX_OK;
-    if (ps->m_u != 0.0f) {
+    if (ps->m_u != 0.0f || ps->mh_one) {  // recorded permutes may skip the Mass column
         ps->m_u = 0.0f;
+        ps->mh_one = false;
         drop_graphs(ps);
     }
     if (!ps->searched) return EPPX_OK;
@@ -784,6 +788,22 @@ bool columns_zero(int32_t c0, int32_t width, const float *fp, size_t rows) {
     return true;
 }
 
+// Whether column c holds one bit pattern on every row (value: that pattern; an absent column is zero).
+bool column_one_value(int32_t c, const float *fp, size_t rows, float *value) {
+    *value = 0.0f;
+    if (c < 0 || rows == 0) return true;
+    const float *col = fp + rows * (size_t)c;
+    uint32_t first;
+    memcpy(&first, col, sizeof(first));
+    for (size_t r = 1; r < rows; ++r) {
+        uint32_t v;
+        memcpy(&v, col + r, sizeof(v));
+        if (v != first) return false;
+    }
+    memcpy(value, &first, sizeof(first));
+    return true;
+}
+
 bool accumulators_nonzero(const eppx_layout &L, const float *fp, size_t rows) {
     for (int32_t c : {L.SumWeight, L.SumWeightedDensity, L.SumWeightedPressure}) {
         if (c < 0) continue;
@@ -848,6 +868,8 @@ int32_t eppx_ps_upload(eppx_ps *ps, int32_t n_particles, const int32_t *is_alive
     memcpy(&mbad, &range[7], sizeof(mbad));
     ps->m_u = uniform_mass(!mbad && has(L.Mass), range[5], range[6]);
     ps->accum_live = accumulators_nonzero(L, float_props, cap);  // both buffers now hold these rows
+    ps->mh_one = column_one_value(L.Mass, float_props, cap, &ps->mass_one) &&
+                 column_one_value(L.H, float_props, cap, &ps->h_one);
     ps->dvel_nonzero_tag = columns_zero(L.dVelocityVec, 2, float_props, cap) ? -2 : -1;
     // keep the columns the kernels do not own
     ps->pass_int.assign(ps->pass_int_cols.size() * cap, 0);
@@ -1615,7 +1637,7 @@ int32_t batch_cell_build(eppx_batch *b, bool search, int32_t which, bool keys_do
     HIP_OK(ctx, timed(ctx, "cell_permute", [&] {
                return launch_permute(cur, nxt, ps->geo, ps->n, 1, nc, ps->key, ps->tmpslot, ps->start, ps->members,
                                      ps->n_alive, ctx->d_flags, ps->accum_live ? kPermuteSums : 0, nullptr, nullptr,
-                                     s);
+                                     0.0f, 0.0f, s);
            }));
     ps->cur ^= 1;
     if (!search) {

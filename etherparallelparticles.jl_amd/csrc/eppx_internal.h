@@ -160,13 +160,16 @@ hipError_t launch_fill_members(const Fields f, int32_t n, int32_t n_cells, const
 // opts: kPermuteSums moves SumWeight, SumWeightedDensity and
 // SumWeightedPressure (without it they are zero in both buffers and stay);
 // kPermuteNoDrho / kPermuteNoDvel leave dDensity / dVelocity of the alive
-// slots alone (zero there in both buffers).  dyn_src / vol_src (null: cur's) hold the slots'
+// slots alone (zero there in both buffers); kPermuteOneMassH: every slot of both buffers
+// holds Mass mass_one and H h_one (the same bits), so neither is read or moved and the
+// snapshot takes them.  dyn_src / vol_src (null: cur's) hold the slots'
 // (u, v, rho, p) records and volumes when a fused kernel wrote them aside.
-constexpr int32_t kPermuteSums = 1, kPermuteNoDrho = 2, kPermuteNoDvel = 4;
+constexpr int32_t kPermuteSums = 1, kPermuteNoDrho = 2, kPermuteNoDvel = 4, kPermuteOneMassH = 8;
 hipError_t launch_permute(Fields cur, Fields nxt, float4 *geo, int32_t n, int32_t batch, int32_t n_cells,
                           const int32_t *key, const int32_t *tmpslot, const int32_t *start,
                           const int32_t *members, int32_t *n_alive, uint32_t *flags, int32_t opts,
-                          const float4 *dyn_src, const float *vol_src, hipStream_t s);
+                          const float4 *dyn_src, const float *vol_src, float mass_one, float h_one,
+                          hipStream_t s);
 // which = 0: the action runs for particles of cases with steps left; 1: of fresh cases.
 // block: threads per pair block (256 or 512); stage: 1 when pass 2 reads
 // its per-neighbour data from the block's LDS tile.  EXACT math uses 256, 0.

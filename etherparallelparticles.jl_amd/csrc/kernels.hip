@@ -121,7 +121,8 @@ __global__ __launch_bounds__(kBlock) void k_permute(Fields cur, Fields nxt, floa
                                                      int32_t n_cells, const int32_t *key, const int32_t *tmpslot,
                                                      const int32_t *start, const int32_t *members,
                                                      int32_t *n_alive, uint32_t *flags, int32_t opts,
-                                                     const float4 *dyn_src, const float *vol_src) {
+                                                     const float4 *dyn_src, const float *vol_src, float mass_one,
+                                                     float h_one) {
     const int32_t s = blockIdx.x * kBlock + threadIdx.x;
     if (s == 0) {
         n_alive[0] = start[n_cells];      // alive slots
@@ -141,7 +142,9 @@ __global__ __launch_bounds__(kBlock) void k_permute(Fields cur, Fields nxt, floa
     const bool move_drho = !(opts & kPermuteNoDrho) || k >= n_cells;  // dead slots keep theirs
     const bool move_dvel = !(opts & kPermuteNoDvel) || k >= n_cells;
     const float2 dvel = move_dvel ? cur.dvel[s] : make_float2(0.0f, 0.0f);
-    const float mass = cur.mass[s], vol = (vol_src ? vol_src : cur.vol)[s], h = cur.h[s];
+    const bool one_mh = opts & kPermuteOneMassH;
+    const float mass = one_mh ? mass_one : cur.mass[s], vol = (vol_src ? vol_src : cur.vol)[s];
+    const float h = one_mh ? h_one : cur.h[s];
     const float drho = move_drho ? cur.drho[s] : 0.0f;
     // the filter and extrapolation sums, when they may be non-zero
     const float wv = move_accum ? cur.wv[s] : 0.0f, wvrho = move_accum ? cur.wvrho[s] : 0.0f;
@@ -182,10 +185,10 @@ __global__ __launch_bounds__(kBlock) void k_permute(Fields cur, Fields nxt, floa
     geo[dest] = make_float4(x.x, x.y, h, mass);
     nxt.dyn[dest] = dyn;
     if (move_dvel) nxt.dvel[dest] = dvel;
-    nxt.mass[dest] = mass;
+    if (!one_mh) nxt.mass[dest] = mass;
     nxt.vol[dest] = vol;
     if (move_drho) nxt.drho[dest] = drho;
-    nxt.h[dest] = h;
+    if (!one_mh) nxt.h[dest] = h;
     if (move_accum) {
         nxt.wv[dest] = wv;
         nxt.wvrho[dest] = wvrho;
@@ -689,9 +692,10 @@ hipError_t launch_fill_members(const Fields f, int32_t n, int32_t n_cells, const
 hipError_t launch_permute(Fields cur, Fields nxt, float4 *geo, int32_t n, int32_t batch, int32_t n_cells,
                           const int32_t *key, const int32_t *tmpslot, const int32_t *start,
                           const int32_t *members, int32_t *n_alive, uint32_t *flags, int32_t opts,
-                          const float4 *dyn_src, const float *vol_src, hipStream_t s) {
+                          const float4 *dyn_src, const float *vol_src, float mass_one, float h_one,
+                          hipStream_t s) {
     hipLaunchKernelGGL(k_permute, grid_for(n), dim3(kBlock), 0, s, cur, nxt, geo, n, batch, n_cells, key, tmpslot,
-                       start, members, n_alive, flags, opts, dyn_src, vol_src);
+                       start, members, n_alive, flags, opts, dyn_src, vol_src, mass_one, h_one);
     return hipGetLastError();
 }
 

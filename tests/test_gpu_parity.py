@@ -109,6 +109,12 @@ def test_mass_changed_after_search(parallel, small):
     ref.selfaction(10, column=ref.L.Mass, value=1.5)
     ref.interaction(1, U.oracle_physics(ph))
     U.assert_state_equal(U.download(ps), ref, names=("Mass", "dVelocityVec"))
+    # the case started with one Mass and one H on every slot, which lets the cell-order
+    # move skip both columns; the scaled masses (alive slots only) must move again
+    Algorithm.step(ps, ns, ph, n_steps=3)
+    err, _ = ref.run_steps(3, U.oracle_physics(ph))
+    assert err == 0
+    U.assert_state_equal(U.download(ps), ref, names=("Mass", "PositionVec", "VelocityVec", "Density"))
 
 
 def test_small_1000_steps(parallel, small, oracle):
