@@ -402,8 +402,100 @@ to = syncto
 # -------------------------------------------------------- neighbour system
 
 
-class NonePeriodicBoundaryPolicy:
-    """The only boundary policy the reference completes (PeriodicBoundary.jl:57 leaves the rest TODO)."""
+class AbstractPeriodicBoundaryPolicy:
+    """Periodic boundary policies, PeriodicBoundary.jl:10-26.  A policy is a
+    class; a periodic one carries its axes in `axis`, as the reference's type
+    parameters {X, Y} / {X, Y, Z} do."""
+
+    axis: tuple = ()
+
+
+class NonePeriodicBoundaryPolicy(AbstractPeriodicBoundaryPolicy):
+    """PeriodicBoundary.jl:12: no periodic axis (the policy of every reference example)."""
+
+
+class HavePeriodicBoundaryPolicy(AbstractPeriodicBoundaryPolicy):
+    """PeriodicBoundary.jl:13."""
+
+
+_PERIODIC_POLICIES: dict = {}
+
+
+def _periodic_policy(dim: int, axis) -> type:
+    axis = tuple(bool(a) for a in axis)
+    if len(axis) != dim:
+        raise ValueError(f"a {dim}D periodic policy takes {dim} axis flags, got {len(axis)}")
+    key = (dim, axis)
+    if key not in _PERIODIC_POLICIES:
+        base = PeriodicBoundaryPolicy2D if dim == 2 else PeriodicBoundaryPolicy3D
+        name = f"{base.__name__}{{{', '.join('true' if a else 'false' for a in axis)}}}"
+        _PERIODIC_POLICIES[key] = type(name, (base,), {"axis": axis, "__module__": __name__})
+    return _PERIODIC_POLICIES[key]
+
+
+class PeriodicBoundaryPolicy2D(HavePeriodicBoundaryPolicy):
+    """PeriodicBoundaryPolicy2D{X, Y} (PeriodicBoundary.jl:14): PeriodicBoundaryPolicy2D.of(x, y)."""
+
+    @staticmethod
+    def of(x: bool, y: bool) -> type:
+        return _periodic_policy(2, (x, y))
+
+
+class PeriodicBoundaryPolicy3D(HavePeriodicBoundaryPolicy):
+    """PeriodicBoundaryPolicy3D{X, Y, Z} (PeriodicBoundary.jl:15): PeriodicBoundaryPolicy3D.of(x, y, z)."""
+
+    @staticmethod
+    def of(x: bool, y: bool, z: bool) -> type:
+        return _periodic_policy(3, (x, y, z))
+
+
+# the named policies of PeriodicBoundary.jl:17-26
+PeriodicBoundaryPolicy2DAlongX = PeriodicBoundaryPolicy2D.of(True, False)
+PeriodicBoundaryPolicy2DAlongY = PeriodicBoundaryPolicy2D.of(False, True)
+PeriodicBoundaryPolicy2DAlongXY = PeriodicBoundaryPolicy2D.of(True, True)
+PeriodicBoundaryPolicy3DAlongX = PeriodicBoundaryPolicy3D.of(True, False, False)
+PeriodicBoundaryPolicy3DAlongY = PeriodicBoundaryPolicy3D.of(False, True, False)
+PeriodicBoundaryPolicy3DAlongZ = PeriodicBoundaryPolicy3D.of(False, False, True)
+PeriodicBoundaryPolicy3DAlongXY = PeriodicBoundaryPolicy3D.of(True, True, False)
+PeriodicBoundaryPolicy3DAlongYZ = PeriodicBoundaryPolicy3D.of(False, True, True)
+PeriodicBoundaryPolicy3DAlongZX = PeriodicBoundaryPolicy3D.of(True, False, True)
+PeriodicBoundaryPolicy3DAlongXYZ = PeriodicBoundaryPolicy3D.of(True, True, True)
+
+
+def _check_policy(policy) -> type:
+    if not (isinstance(policy, type) and issubclass(policy, AbstractPeriodicBoundaryPolicy)) \
+            or policy in (AbstractPeriodicBoundaryPolicy, HavePeriodicBoundaryPolicy, PeriodicBoundaryPolicy2D,
+                          PeriodicBoundaryPolicy3D):
+        raise TypeError(f"not a periodic boundary policy: {policy!r}")
+    return policy
+
+
+class PeriodicBoundary:
+    """PeriodicBoundary(parallel, domain, policy), PeriodicBoundary.jl:28-60.
+
+    `neighbour_cell_relative_position_list_` has the reference's shape: (1, 1, 1)
+    for NonePeriodicBoundaryPolicy (:36-46), (n_cells, 3^N, N) for a periodic
+    policy (:48-60).  The reference fills it with zeros and leaves filling it as a
+    TODO (:57); nothing else writes it, so the shift that
+    applyPeriodicBoundaryPolicy adds to (dx, dy) (NeighbourSearch.jl:129-139) is
+    always zero and a periodic policy finds the pairs and sums of the
+    non-periodic search.  The table is kept here, read-only, so that it stays
+    what the reference's is; the device search adds no shift."""
+
+    def __init__(self, parallel, domain, periodic_boundary_policy=NonePeriodicBoundaryPolicy):
+        self.policy = _check_policy(periodic_boundary_policy)
+        if self.policy is NonePeriodicBoundaryPolicy:
+            shape = (1, 1, 1)
+        else:
+            n = domain.dimension()
+            shape = (int(domain.get_n()), 3 ** n, n)
+        table = np.zeros(shape, np.float32, order="F")
+        table.flags.writeable = False
+        self.neighbour_cell_relative_position_list_ = table
+
+    def __repr__(self):
+        return (f"PeriodicBoundary{{Float32, {self.policy.__name__}}}(shape of "
+                f"`neighbour_cell_relative_position_list`: {self.neighbour_cell_relative_position_list_.shape})")
 
 
 class ActivePair:
@@ -423,10 +515,9 @@ class NeighbourSystem:
 
     def __init__(self, parallel: Parallel, domain: Domain2D, active_pair, periodic_boundary_policy=NonePeriodicBoundaryPolicy,
                  max_neighbour_number: int = 50, n_threads: int = 256):
-        if periodic_boundary_policy is not NonePeriodicBoundaryPolicy:
-            raise NotImplementedError("periodic boundaries are unfinished in the reference (PeriodicBoundary.jl:57)")
         self.parallel, self.domain = parallel, domain
         self.dim = domain.dimension()
+        self.periodic_boundary_ = PeriodicBoundary(parallel, domain, periodic_boundary_policy)
         self.active_pair_ = ActivePair(active_pair)
         self.max_neighbour_number = int(max_neighbour_number)
         adj = np.asfortranarray(self.active_pair_.adjacency_matrix_)

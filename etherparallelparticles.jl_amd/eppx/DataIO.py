@@ -202,11 +202,24 @@ def ParticleSystem(config: dict, parallel: Environment.Parallel, domain: Class.D
 def NeighbourSystem(config: dict, parallel: Environment.Parallel, domain: Class.Domain2D) -> Class.NeighbourSystem:
     """DataIO.NeighbourSystem (Input.jl:82-104)."""
     c = config["neighbour_system"]
-    if c["periodic_boundary_policy"]["type"] != "none":
-        raise NotImplementedError("periodic boundaries are unfinished in the reference (PeriodicBoundary.jl:57)")
     pairs = [(int(a), int(b)) for a, b in c["active_pair"]]
-    return Class.NeighbourSystem(parallel, domain, pairs, max_neighbour_number=int(c["max_neighbour_number"]),
+    return Class.NeighbourSystem(parallel, domain, pairs, periodic_policy(c["periodic_boundary_policy"], domain),
+                                 max_neighbour_number=int(c["max_neighbour_number"]),
                                  n_threads=int(c.get("n_threads", 256)))
+
+
+def periodic_policy(section: dict, domain) -> type:
+    """The policy of a configuration's `periodic_boundary_policy` section
+    (Input.jl:84-92): "none", else the policy of the domain's dimension with
+    the first N flags of `axis`."""
+    if section["type"] == "none":
+        return Class.NonePeriodicBoundaryPolicy
+    axis = section["axis"]
+    if not all(isinstance(a, bool) for a in axis):
+        raise ValueError(f"periodic_boundary_policy.axis must hold booleans, got {axis!r}")
+    if domain.dimension() == 2:
+        return Class.PeriodicBoundaryPolicy2D.of(axis[0], axis[1])
+    return Class.PeriodicBoundaryPolicy3D.of(axis[0], axis[1], axis[2])
 
 
 # ------------------------------------------------------------------ writers
@@ -332,7 +345,13 @@ def config(parallel: Environment.Parallel, domain: Class.Domain2D, particle_syst
     nc = c["neighbour_system"]
     nc["active_pair"] = [[int(a), int(b)] for a, b in ns.active_pair_.pair_vector_]
     nc["max_neighbour_number"] = int(ns.max_neighbour_number)
-    nc["periodic_boundary_policy"]["type"] = "none"
+    policy = ns.periodic_boundary_.policy  # Output.jl:86-97
+    if policy is Class.NonePeriodicBoundaryPolicy:
+        nc["periodic_boundary_policy"]["type"] = "none"
+    else:
+        nc["periodic_boundary_policy"]["type"] = "2D" if len(policy.axis) == 2 else "3D"
+        for k, a in enumerate(policy.axis):
+            nc["periodic_boundary_policy"]["axis"][k] = a
     w = c["writer"]
     w["path"], w["file_name"], w["connect"] = writer.path_, writer.file_name_, writer.connect_
     w["digits"], w["suffix"] = writer.digits_, writer.suffix_

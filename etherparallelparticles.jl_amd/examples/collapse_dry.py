@@ -41,7 +41,10 @@ def casegen(out: str, dx: float = 0.02, h_factor: float = 1.5, kernel: str = "cu
     par = DataIO.Parallel(DataIO.template())
     writer = DataIO.Writer(out)
     ns = type("NeighbourSystemRecord", (), {"active_pair_": Class.ActivePair(cases.ACTIVE_PAIR),
-                                             "max_neighbour_number": M})()
+                                             "max_neighbour_number": M,
+                                             # casegen.jl:151
+                                             "periodic_boundary_": Class.PeriodicBoundary(
+                                                 par, case.domain, Class.NonePeriodicBoundaryPolicy)})()
     config = DataIO.config(par, case.domain, case.host, ns, writer)
     # the physics main.jl hard-codes
     config["driver"] = {"dx": dx, "h_factor": h_factor, "kernel": kernel, "physics": physics}

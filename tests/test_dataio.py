@@ -130,7 +130,8 @@ def test_config_round_trip(tmp_path):
     case = cases.small()
     host = case.host
     par = DataIO.Parallel(DataIO.template())
-    ns = types.SimpleNamespace(active_pair_=Class.ActivePair(cases.ACTIVE_PAIR), max_neighbour_number=40)
+    ns = types.SimpleNamespace(active_pair_=Class.ActivePair(cases.ACTIVE_PAIR), max_neighbour_number=40,
+                               periodic_boundary_=Class.PeriodicBoundary(par, case.domain))
     w = DataIO.Writer(str(tmp_path))
     c = DataIO.config(par, case.domain, host, ns, w)
     name = DataIO.writeConfig(w, c)
