@@ -10,7 +10,7 @@
 namespace eppx {
 
 constexpr int kBlock = 256;
-constexpr int kScanTile = 2048;  // elements per block in the cell-count scan
+constexpr int kScanTile = 1024;  // elements per block in the cell-count scan (735 blocks for 752k cells)
 constexpr int kMaxTag = 32;
 
 // Pair counters.  A search adds the pairs of each block (each wave in 3D) to
